@@ -1,0 +1,204 @@
+/*
+ * swreax.h — C-ABI of the MI355X (gfx950) neighbour-list + bond-order + QEq
+ * path.  Library: sw_reaxff_amd/libswreax.so.
+ *
+ * The entry points below replace, one for one, the reference's Sunway
+ * offload functions on this path (the `_C` functions that a LAMMPS style
+ * calls with a POD "pack").  Each comment names the reference function and
+ * its caller.  Differences in form, all deliberate:
+ *
+ *  - State lives in an opaque swr_system (device buffers owned by the
+ *    library) instead of caller-owned host arrays inside packs.  Host views
+ *    are copied in with swr_set_atoms() and out with swr_get(); swr_get()
+ *    returns lists in the reference's record layouts (swreax_layout.h).
+ *  - Every function returns an int status (SWR_OK or an SWR_ERR_* code)
+ *    instead of void + MPI_Abort.  Capacity errors carry the required size
+ *    (swr_get_scalar "need_bond_cap", ...), so a caller can grow and retry.
+ *  - Calls enqueue on the system's HIP stream and return; swr_sync() joins
+ *    (the analogue of athread_join) and reports device-side errors.
+ *  - The CG loop of FixQEqReaxSunway::CG_v2 sits behind one call that
+ *    solves s and t together on the device.
+ *
+ * No CUDA, no torch types: plain pointers and sizes.
+ */
+#ifndef SWREAX_H
+#define SWREAX_H
+
+#include <stdint.h>
+
+#include "swreax_layout.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes */
+enum {
+  SWR_OK = 0,
+  SWR_ERR_ARG = 1,         /* bad argument / call order                      */
+  SWR_ERR_BOND_CAP = 2,    /* per-atom or total bond capacity exceeded        */
+  SWR_ERR_HBOND_CAP = 3,   /* per-atom or total hbond capacity exceeded       */
+  SWR_ERR_NOMEM = 4,       /* device allocation failed                        */
+  SWR_ERR_FFIELD = 5,      /* force-field file unreadable or type unmapped    */
+  SWR_ERR_STATE = 6,       /* stage called before its inputs exist            */
+  SWR_ERR_BIN = 7,         /* atom outside the neighbour bin grid             */
+  SWR_ERR_NUMERIC = 8,     /* non-finite CG scalar                            */
+  SWR_ERR_HIP = 10         /* HIP runtime error (see swr_last_error)          */
+};
+
+/* Settings of `pair_style reax/c NULL [keywords]` plus `fix qeq/reax nevery
+ * swa swb tol reax/c` (pair_reaxc_sunway.cpp:202-290,
+ * fix_qeq_reax_sunway.cpp:76-99).  swr_control_default() fills the NULL
+ * defaults: bond_cut 5, hbond_cut 7.5, safezone 1.2, saferzone 1.4, mincap
+ * 50, swa 0, swb 10, tol 1e-6, skin 2. */
+typedef struct swr_control_t {
+  double bond_cut, hbond_cut, thb_cut;
+  double safezone, saferzone;
+  int mincap, maxfar;
+  double swa, swb, tol;
+  double skin;          /* neighbor skin; cutneigh = max(nonb,hbond,2*bond) + skin */
+  int qeq_mixed;        /* 1: FP32 H + FP32 inner CG inside FP64 refinement    */
+  int qeq_imax;         /* CG iteration cap, 200 in the reference              */
+} swr_control_t;
+
+typedef struct swr_system swr_system;
+
+void        swr_control_default(swr_control_t *ctl);
+const char *swr_last_error(void);
+
+/* Create a system: reads the force field (Read_Force_Field,
+ * reaxc_ffield_sunway.cpp:35) and maps LAMMPS types 1..ntypes to the named
+ * elements (PairReaxCSunway::coeff, pair_reaxc_sunway.cpp:294-362). */
+int swr_create(const char *ffield_path, int ntypes, const char *const *elements,
+               const swr_control_t *ctl, swr_system **out);
+int swr_destroy(swr_system *sys);
+
+/* Atom state from host arrays (LAMMPS atom->tag/type/x/q, nlocal + nghost).
+ * owner[g] is the local atom that ghost g images (owner[i] = i locally) and
+ * img[3*g+d] in {-1,0,1} its periodic shift; owner/img replace LAMMPS's
+ * forward-comm maps (fix_qeq_reax_sunway.cpp:1759-1813). */
+int swr_set_atoms(swr_system *sys, int n, int N, const int *tag, const int *type,
+                  const double *x, const double *q, const int *owner,
+                  const signed char *img);
+/* Per-atom capacities (fix REAXC num_bonds/num_hbonds, fix_reaxc_sunway.cpp:39-161). */
+int swr_set_capacities(swr_system *sys, const int *num_bonds, const int *num_hbonds);
+
+/* a1: full neighbour list of local + ghost atoms on a GPU cell list.
+ * Replaces npair_full_bin_ghost_sunway_build_packed (npair_full_bin_ghost_sw5.c:64;
+ * CPE kernel :80-226) and its atomonly twin
+ * npair_full_bin_atomonly_sunway_build_packed (npair_full_bin_atomonly_sw5.c:6,
+ * called from npair_full_bin_atomonly_sunway.cpp:151).  cutneigh <= 0 uses
+ * the control's cutmax + skin. */
+int swr_npair_full_bin_ghost_build(swr_system *sys, double cutneigh);
+
+/* a2: pack tag, element type, x, q (write_reax_atoms_and_pack,
+ * pair_reaxc_sw64.c:13; caller pair_reaxc_sunway.cpp:818). */
+int swr_write_reax_atoms_and_pack(swr_system *sys);
+
+/* a4: Reset — Hindex/numH, bond and hbond start/end from the capacities
+ * (Reset, reaxc_reset_tools_sunway.cpp:200-211).  On the first call, if no
+ * capacities were set, runs the setup-time estimate (Estimate_Storages,
+ * reaxc_forces_sunway.cpp:1159; Init_Lists, reaxc_init_md_sunway.cpp:171). */
+int swr_reset(swr_system *sys);
+
+/* a3: far-neighbour list, d^2 <= nonb_cut^2 (write_reax_lists_c,
+ * pair_reaxc_sw64.c:42; caller pair_reaxc_sunway.cpp:906). */
+int swr_write_reax_lists(swr_system *sys);
+
+/* a5+a6: bonds with uncorrected BO and dBO terms, sorted by nbr, plus
+ * hbonds of local H, from the far list (Init_Forces_noQEq_Full_C,
+ * reaxc_forces_sw64.c:594; caller reaxc_forces_sunway.cpp:845). */
+int swr_init_forces_noqeq_full(swr_system *sys);
+
+/* a3+a5+a6 in one pass over the Verlet list: identical outputs to
+ * swr_write_reax_lists() followed by swr_init_forces_noqeq_full(), without
+ * re-reading the far list. */
+int swr_write_reax_lists_init_forces(swr_system *sys);
+
+/* a7: capacity update for the next step and bond_mark propagation
+ * (Validate_Lists_C, reaxc_forces_sw64.c:866; sort_bonds_C, :932; called from
+ * reaxc_nonbonded_sw64.c:436-437). */
+int swr_validate_lists(swr_system *sys);
+
+/* a8-a10: corrected bond orders, dBO coefficients, Delta and lone-pair terms
+ * (BO_C, reaxc_bond_orders_sw64.c:487; caller reaxc_bond_orders_sunway.cpp:454). */
+int swr_bo(swr_system *sys);
+
+/* a11-a13: H matrix (full rows of local atoms, taper x shielded Coulomb),
+ * preconditioner, right-hand sides and extrapolated warm starts
+ * (FixQEqReaxSunway::init_matvec, fix_qeq_reax_sunway.cpp:618-723, with
+ * compute_H_Full_C, fix_qeq_reax_sw64.c:75).  jlist holds the owning local
+ * atom of each column (ghost columns folded onto their owners). */
+int swr_qeq_init_matvec(swr_system *sys);
+
+/* a14: b[i] = eta_i x_i + sum_j H_ij x_j for local rows, device pointers of
+ * length n (sparse_matvec_C, fix_qeq_reax_sw64.c:42; _spawn/_join :67,:71;
+ * callers fix_qeq_reax_sunway.cpp:1068-1143).  Asynchronous like _spawn;
+ * swr_sync() is the _join. */
+int swr_sparse_matvec(swr_system *sys, const double *x_dev, double *b_dev);
+
+/* a15: both CG_v2 solves (s for b = -chi, t for b = -1) with one shared H
+ * read per iteration (FixQEqReaxSunway::CG_v2, fix_qeq_reax_sunway.cpp:983;
+ * called at :576-577).  Synchronises; writes the iteration counts. */
+int swr_qeq_cg(swr_system *sys, int *iters_s, int *iters_t);
+
+/* a16: q = s - (sum s / sum t) t, history shift, ghost q refresh
+ * (FixQEqReaxSunway::calculate_Q, fix_qeq_reax_sunway.cpp:1697-1755). */
+int swr_calculate_q(swr_system *sys);
+
+/* fix qeq/reax pre_force: init_matvec + CG(s,t) + calculate_Q
+ * (FixQEqReaxSunway::pre_force, fix_qeq_reax_sunway.cpp:539-600).
+ * iters may be NULL (then nothing synchronises). */
+int swr_qeq_pre_force(swr_system *sys, int *iters_s, int *iters_t);
+
+/* Pair compute stages in the reference order (PairReaxCSunway::compute,
+ * pair_reaxc_sunway.cpp:541-793, through BO): pack, Reset, lists + bonds,
+ * Validate_Lists, BO. */
+int swr_pair_compute(swr_system *sys);
+
+/* Join the stream; returns the first device-raised error. */
+int swr_sync(swr_system *sys);
+
+/* Mirrors: download a named device array into host memory in the
+ * reference's layout; returns bytes (or -1 for an unknown name).  Pass
+ * out = NULL to query the size.  Names: x q tag type owner packed_atoms
+ * verlet_offsets verlet_list far far_index far_end num_bonds num_hbonds
+ * Hindex bond_index bond_end bonds bo_data BO BOpi Cdbo Cdbopi Cdbopi2
+ * hbond_index hbond_end hbonds hbond_far bo_dboc dDeltap_self bond_mark
+ * Deltap Deltap_boc Delta Delta_e Delta_val vlpex nlp nlp_temp Clp Delta_lp
+ * dDelta_lp Delta_lp_temp dDelta_lp_temp H_firstnbr H_numnbrs H_jlist H_val
+ * Hdia_inv b_s b_t s t s_hist t_hist.  hbonds' ptr fields come back as
+ * device addresses; hbond_far gives each hbond's far-list slot. */
+int64_t swr_get(swr_system *sys, const char *name, void *out);
+int     swr_put(swr_system *sys, const char *name, const void *in, int64_t bytes);
+int64_t swr_get_scalar(swr_system *sys, const char *name);
+/* Device pointer of a named array (for callers that keep data resident). */
+void   *swr_device_ptr(swr_system *sys, const char *name);
+
+/* ---- mini-LAMMPS host harness (C++ classes behind a C handle) ----------
+ * Supplies what LAMMPS would around the path: box, ghost images (borders /
+ * forward comm on the device), the reneighbour trigger and NVE.  Commands
+ * follow LAMMPS syntax for the subset this path needs:
+ *   pair_style reax/c/amd NULL [checkqeq yes|no] [safezone v] [mincap n] [maxfar n]
+ *   pair_coeff * * <ffield> <element per type>
+ *   fix <id> all qeq/reax/amd <nevery> <swa> <swb> <tol> reax/c [mixed]
+ *   neighbor <skin> bin
+ *   neigh_modify every <k>
+ *   timestep <dt_fs> */
+typedef struct swr_lmp swr_lmp;
+int         swr_lmp_open(swr_lmp **out);
+int         swr_lmp_close(swr_lmp *lmp);
+int         swr_lmp_command(swr_lmp *lmp, const char *cmd);
+int         swr_lmp_create_atoms(swr_lmp *lmp, int n, const double *lo, const double *hi,
+                                 int periodic, const double *x, const double *v,
+                                 const int *type, const int *tag);
+int         swr_lmp_setup(swr_lmp *lmp);
+int         swr_lmp_run(swr_lmp *lmp, int nsteps);
+swr_system *swr_lmp_system(swr_lmp *lmp);
+double      swr_lmp_stat(swr_lmp *lmp, const char *name);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWREAX_H */

@@ -1,0 +1,168 @@
+// bo.hip — corrected bond orders (a9) and the Delta / lone-pair terms (a10).
+//
+// Restates BO_Section2_C / BO_Section3_C (host branches,
+// reaxc_bond_orders_sw64.c:163-376 and :451-481).  Section 1 (Deltap,
+// Deltap_boc) is produced by the list pass, which owns each atom's BO' sum.
+// One thread per atom walks its bonds in list order (nbr-sorted), so
+// bo_dboc[i][0] sums in the reference's order; sym_index is found by binary
+// search in j's nbr-sorted list (the reference scans it linearly and keeps
+// the last match; pairs are unique, so both give the same slot).
+#include "swr_internal.h"
+#include "swr_device.h"
+
+namespace {
+
+__global__ void __launch_bounds__(256)
+k_bo(int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__restrict__ atoms,
+     const int *__restrict__ bond_index, const int *__restrict__ bond_end, swr_bond_t *__restrict__ bonds,
+     swr_bond_order_t *__restrict__ bo_data, double *__restrict__ BO, double *__restrict__ BOpi,
+     double *__restrict__ bo_dboc, const double *__restrict__ Deltap, const double *__restrict__ Deltap_boc,
+     double *__restrict__ Delta, double *__restrict__ Delta_e, double *__restrict__ Delta_val,
+     double *__restrict__ vlpex, double *__restrict__ nlp, double *__restrict__ nlp_temp, double *__restrict__ Clp,
+     double *__restrict__ Delta_lp, double *__restrict__ dDelta_lp, double *__restrict__ Delta_lp_temp,
+     double *__restrict__ dDelta_lp_temp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int ei = atoms[i].type;
+  if (ei < 0) return;
+  const double p_boc1 = P->p_boc1, p_boc2 = P->p_boc2;
+  const double val_i = P->valency[ei];
+  const double Deltap_i = Deltap[i], Deltap_boc_i = Deltap_boc[i];
+  const int bs = bond_index[i], be = bond_end[i];
+  double total = 0.0;
+  for (int pj = bs; pj < be; pj++) {
+    const int j = bonds[pj].nbr;
+    const int ej = atoms[j].type;
+    if (ej < 0) continue;
+    // sym_index: slot of i in j's nbr-sorted list
+    int lo = bond_index[j], hi = bond_end[j] - 1, sym = bonds[pj].sym_index;
+    while (lo <= hi) {
+      const int mid = (lo + hi) >> 1;
+      const int v = bonds[mid].nbr;
+      if (v == i) { sym = mid; break; }
+      if (v < i) lo = mid + 1; else hi = mid - 1;
+    }
+    bonds[pj].sym_index = sym;
+    const DevTbp tw = P->tbp[ei * SWR_MAX_ELEM + ej];
+    swr_bond_order_t &bo = bo_data[pj];
+    double bo_ij = BO[pj];
+    double pi = BOpi[2 * (size_t)pj], pi2 = BOpi[2 * (size_t)pj + 1];
+    double BO_s = bo.BO_s;
+    if (tw.ovc < 0.001 && tw.v13cor < 0.001) {
+      bo.C1dbo = 1.0; bo.C2dbo = 0.0; bo.C3dbo = 0.0;
+      bo.C1dbopi = pi; bo.C2dbopi = 0.0; bo.C3dbopi = 0.0; bo.C4dbopi = 0.0;
+      bo.C1dbopi2 = pi2; bo.C2dbopi2 = 0.0; bo.C3dbopi2 = 0.0; bo.C4dbopi2 = 0.0;
+    } else {
+      const double val_j = P->valency[ej];
+      const double Deltap_j = Deltap[j], Deltap_boc_j = Deltap_boc[j];
+      double f1, Cf1_ij, Cf1_ji;
+      if (tw.ovc >= 0.001) {
+        const double exp_p1i = exp(-p_boc1 * Deltap_i);
+        const double exp_p2i = exp(-p_boc2 * Deltap_i);
+        const double exp_p1j = exp(-p_boc1 * Deltap_j);
+        const double exp_p2j = exp(-p_boc2 * Deltap_j);
+        const double f2 = exp_p1i + exp_p1j;
+        const double f3 = -1.0 / p_boc2 * log(0.5 * (exp_p2i + exp_p2j));
+        f1 = 0.5 * ((val_i + f2) / (val_i + f2 + f3) + (val_j + f2) / (val_j + f2 + f3));
+        const double temp = f2 + f3;
+        const double u1_ij = val_i + temp, u1_ji = val_j + temp;
+        const double Cf1A_ij = 0.5 * f3 * (1.0 / (u1_ij * u1_ij) + 1.0 / (u1_ji * u1_ji));
+        const double Cf1B_ij = -0.5 * ((u1_ij - f3) / (u1_ij * u1_ij) + (u1_ji - f3) / (u1_ji * u1_ji));
+        Cf1_ij = 0.50 * (-p_boc1 * exp_p1i / u1_ij -
+                         ((val_i + f2) / (u1_ij * u1_ij)) * (-p_boc1 * exp_p1i + exp_p2i / (exp_p2i + exp_p2j)) +
+                         -p_boc1 * exp_p1i / u1_ji -
+                         ((val_j + f2) / (u1_ji * u1_ji)) * (-p_boc1 * exp_p1i + exp_p2i / (exp_p2i + exp_p2j)));
+        Cf1_ji = -Cf1A_ij * p_boc1 * exp_p1j + Cf1B_ij * exp_p2j / (exp_p2i + exp_p2j);
+      } else {
+        f1 = 1.0;
+        Cf1_ij = Cf1_ji = 0.0;
+      }
+      double f4, f5, f4f5, Cf45_ij, Cf45_ji;
+      if (tw.v13cor >= 0.001) {
+        const double exp_f4 = exp(-(tw.p_boc4 * (bo_ij * bo_ij) - Deltap_boc_i) * tw.p_boc3 + tw.p_boc5);
+        const double exp_f5 = exp(-(tw.p_boc4 * (bo_ij * bo_ij) - Deltap_boc_j) * tw.p_boc3 + tw.p_boc5);
+        f4 = 1. / (1. + exp_f4);
+        f5 = 1. / (1. + exp_f5);
+        f4f5 = f4 * f5;
+        Cf45_ij = -f4 * exp_f4;
+        Cf45_ji = -f5 * exp_f5;
+      } else {
+        f4 = f5 = f4f5 = 1.0;
+        Cf45_ij = Cf45_ji = 0.0;
+      }
+      const double A0_ij = f1 * f4f5;
+      const double A1_ij = -2 * tw.p_boc3 * tw.p_boc4 * bo_ij * (Cf45_ij + Cf45_ji);
+      const double A2_ij = Cf1_ij / f1 + tw.p_boc3 * Cf45_ij;
+      const double A2_ji = Cf1_ji / f1 + tw.p_boc3 * Cf45_ji;
+      const double A3_ij = A2_ij + Cf1_ij / f1;
+      const double A3_ji = A2_ji + Cf1_ji / f1;
+      bo_ij = bo_ij * A0_ij;
+      pi = pi * A0_ij * f1;
+      pi2 = pi2 * A0_ij * f1;
+      BO_s = bo_ij - (pi + pi2);
+      bo.C1dbo = A0_ij + bo_ij * A1_ij;
+      bo.C2dbo = bo_ij * A2_ij;
+      bo.C3dbo = bo_ij * A2_ji;
+      bo.C1dbopi = f1 * f1 * f4 * f5;
+      bo.C2dbopi = pi * A1_ij;
+      bo.C3dbopi = pi * A3_ij;
+      bo.C4dbopi = pi * A3_ji;
+      bo.C1dbopi2 = f1 * f1 * f4 * f5;
+      bo.C2dbopi2 = pi2 * A1_ij;
+      bo.C3dbopi2 = pi2 * A3_ij;
+      bo.C4dbopi2 = pi2 * A3_ji;
+    }
+    if (bo_ij < 1e-10) bo_ij = 0.0;
+    if (BO_s < 1e-10) BO_s = 0.0;
+    if (pi < 1e-10) pi = 0.0;
+    if (pi2 < 1e-10) pi2 = 0.0;
+    BO[pj] = bo_ij;
+    BOpi[2 * (size_t)pj] = pi;
+    BOpi[2 * (size_t)pj + 1] = pi2;
+    bo.BO_s = BO_s;
+    total += bo_ij;
+  }
+  // Section 3 (reaxc_bond_orders_sw64.c:451-481)
+  const double p_lp1 = P->p_lp1;
+  bo_dboc[2 * (size_t)i] = total;
+  const double De = total - P->valency_e[ei];
+  Delta[i] = total - P->valency[ei];
+  Delta_e[i] = De;
+  bo_dboc[2 * (size_t)i + 1] = total - P->valency_boc[ei];
+  Delta_val[i] = total - P->valency_val[ei];
+  const double vx = De - 2.0 * (int)(De / 2.0);
+  vlpex[i] = vx;
+  const double explp1 = exp(-p_lp1 * ((2.0 + vx) * (2.0 + vx)));
+  const double nl = explp1 - (int)(De / 2.0);
+  nlp[i] = nl;
+  const double nlp_opt = P->nlp_opt[ei];
+  Delta_lp[i] = nlp_opt - nl;
+  const double clp = 2.0 * p_lp1 * explp1 * (2.0 + vx);
+  Clp[i] = clp;
+  dDelta_lp[i] = clp;
+  if (P->mass[ei] > 21.0) {
+    const double nt = 0.5 * (P->valency_e[ei] - P->valency[ei]);
+    nlp_temp[i] = nt;
+    Delta_lp_temp[i] = nlp_opt - nt;
+    dDelta_lp_temp[i] = 0.;
+  } else {
+    nlp_temp[i] = nl;
+    Delta_lp_temp[i] = nlp_opt - nl;
+    dDelta_lp_temp[i] = clp;
+  }
+}
+
+}  // namespace
+
+int swr_bo_run(swr_system *S) {
+  const int N = S->N;
+  const size_t c = S->Ncap;
+  double *w = S->ws;
+  hipLaunchKernelGGL(k_bo, dim3(div_up(N, 256)), dim3(256), 0, S->stream, N, S->dp, S->packed, S->bond_index,
+                     S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, S->bo_dboc, w + WS_DELTAP * c,
+                     w + WS_DELTAP_BOC * c, w + WS_DELTA * c, w + WS_DELTA_E * c, w + WS_DELTA_VAL * c,
+                     w + WS_VLPEX * c, w + WS_NLP * c, w + WS_NLP_TEMP * c, w + WS_CLP * c, w + WS_DELTA_LP * c,
+                     w + WS_DDELTA_LP * c, w + WS_DELTA_LP_TEMP * c, w + WS_DDELTA_LP_TEMP * c);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}

@@ -1,0 +1,603 @@
+// capi.cpp — the C-ABI of include/swreax.h: system lifecycle, device
+// allocation, the per-stage entry points, status and named mirrors.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+
+#include "swr_internal.h"
+
+// stage implementations (nbr.hip, lists.hip, bo.hip, qeq.hip)
+int swr_nbr_set_grid(swr_system *S, const double lo[3], const double hi[3], double cutghost, double cutneigh);
+int swr_nbr_build(swr_system *S);
+int swr_lists_pack(swr_system *S);
+int swr_lists_reset(swr_system *S);
+int swr_lists_run(swr_system *S, int mode);
+int swr_lists_estimate(swr_system *S);
+int swr_lists_validate(swr_system *S, int max_sweeps);
+int swr_bo_run(swr_system *S);
+int swr_qeq_build_H(swr_system *S);
+int swr_qeq_spmv(swr_system *S, const double *xv, double *b);
+int swr_qeq_cg_run(swr_system *S, int first_batch);
+int swr_qeq_calc_q(swr_system *S);
+int swr_qeq_blocks(int n);
+int swr_qeq_mixed_solve(swr_system *S);
+
+namespace {
+
+constexpr int MARK_SWEEPS = 64;
+
+int copy_grow(void **p, size_t old_bytes, size_t new_bytes, hipStream_t s) {
+  void *np = nullptr;
+  hipError_t e = hipMalloc(&np, new_bytes ? new_bytes : 1);
+  if (e != hipSuccess) {
+    swr_set_error_msg(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return SWR_ERR_NOMEM;
+  }
+  if (*p && old_bytes) {
+    e = hipMemcpyAsync(np, *p, std::min(old_bytes, new_bytes), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      swr_set_error_msg(std::string("hipMemcpy: ") + hipGetErrorString(e));
+      return SWR_ERR_HIP;
+    }
+    (void)hipFree(*p);
+  }
+  *p = np;
+  return SWR_OK;
+}
+
+void fill_params(swr_system *S, const std::vector<int> &map) {
+  DevParams &h = S->hp;
+  std::memset(&h, 0, sizeof(h));
+  const swr_ffield_t &ff = S->ff;
+  h.nelem = ff.ne();
+  h.ntypes = (int)map.size() - 1;
+  for (size_t t = 0; t < map.size(); t++) h.map[t] = map[t];
+  for (int e = 0; e < ff.ne() && e < SWR_MAX_ELEM; e++) {
+    const FfAtom &a = ff.atom[e];
+    h.r_s[e] = a.r_s; h.r_pi[e] = a.r_pi; h.r_pi_pi[e] = a.r_pi_pi;
+    h.valency[e] = a.valency; h.valency_e[e] = a.valency_e; h.valency_boc[e] = a.valency_boc;
+    h.valency_val[e] = a.valency_val; h.nlp_opt[e] = a.nlp_opt; h.mass[e] = a.mass;
+    h.p_hbond[e] = a.p_hbond;
+    for (int f = 0; f < ff.ne() && f < SWR_MAX_ELEM; f++) {
+      const FfPair &p = ff.tb(e, f);
+      DevTbp &d = h.tbp[e * SWR_MAX_ELEM + f];
+      d.p_bo1 = p.p_bo1; d.p_bo2 = p.p_bo2; d.p_bo3 = p.p_bo3; d.p_bo4 = p.p_bo4; d.p_bo5 = p.p_bo5;
+      d.p_bo6 = p.p_bo6; d.r_s = p.r_s; d.r_p = p.r_p; d.r_pp = p.r_pp; d.p_boc3 = p.p_boc3;
+      d.p_boc4 = p.p_boc4; d.p_boc5 = p.p_boc5; d.v13cor = p.v13cor; d.ovc = p.ovc;
+    }
+  }
+  h.p_boc1 = ff.gp[0];
+  h.p_boc2 = ff.gp[1];
+  h.p_lp1 = ff.gp[15];
+  h.bo_cut = ff.bo_cut;
+  h.bond_cut = S->ctl.bond_cut;
+  h.hbond_cut = S->ctl.hbond_cut;
+  h.nonb_cut = ff.nonb_cut;
+  h.nonb_cutsq = ff.nonb_cut * ff.nonb_cut;
+  // pair->extract("chi"/"eta"/"gamma") per LAMMPS type (pair_reaxc_sunway.cpp:1106-1128)
+  for (int t = 1; t <= h.ntypes; t++) {
+    const FfAtom &a = ff.atom[map[t]];
+    h.chi[t] = a.chi; h.eta[t] = a.eta; h.gamma[t] = a.gamma;
+  }
+  // init_shielding (fix_qeq_reax_sunway.cpp:440-454)
+  const int nt1 = SWR_MAX_TYPES + 1;
+  for (int i = 1; i <= h.ntypes; i++)
+    for (int j = 1; j <= h.ntypes; j++) h.shld[i * nt1 + j] = std::pow(h.gamma[i] * h.gamma[j], -1.5);
+  // init_taper (fix_qeq_reax_sunway.cpp:458-484)
+  const double swa = S->ctl.swa, swb = S->ctl.swb;
+  const double d7 = std::pow(swb - swa, 7);
+  const double swa2 = swa * swa, swa3 = swa * swa * swa, swb2 = swb * swb, swb3 = swb * swb * swb;
+  h.Tap[7] = 20.0 / d7;
+  h.Tap[6] = -70.0 * (swa + swb) / d7;
+  h.Tap[5] = 84.0 * (swa2 + 3.0 * swa * swb + swb2) / d7;
+  h.Tap[4] = -35.0 * (swa3 + 9.0 * swa2 * swb + 9.0 * swa * swb2 + swb3) / d7;
+  h.Tap[3] = 140.0 * (swa3 * swb + 3.0 * swa2 * swb2 + swa * swb3) / d7;
+  h.Tap[2] = -210.0 * (swa3 * swb2 + swa2 * swb3) / d7;
+  h.Tap[1] = 140.0 * swa3 * swb3 / d7;
+  h.Tap[0] = (-35.0 * swa3 * swb2 * swb2 + 21.0 * swa2 * swb3 * swb2 + 7.0 * swa * swb3 * swb3 +
+              swb3 * swb3 * swb) / d7;
+  h.swa = swa;
+  h.swb = swb;
+  h.swbsq = swb * swb;
+  h.tol = S->ctl.tol;
+}
+
+int status_to_rc(unsigned int f) {
+  if (f & (SWR_F_BOND_CAP | SWR_F_BOND_TOTAL | SWR_F_BOND_LDS)) return SWR_ERR_BOND_CAP;
+  if (f & (SWR_F_HBOND_CAP | SWR_F_HBOND_TOTAL)) return SWR_ERR_HBOND_CAP;
+  if (f & SWR_F_BIN) return SWR_ERR_BIN;
+  if (f & SWR_F_NAN) return SWR_ERR_NUMERIC;
+  if (f & SWR_F_MARK_DEPTH) return SWR_ERR_STATE;
+  return SWR_OK;
+}
+
+}  // namespace
+
+// Grow every per-atom array to hold Ncap atoms (contents preserved).
+int swr_nbr_ensure_atoms(swr_system *S, int Ncap) {
+  if (Ncap <= S->Ncap) return SWR_OK;
+  const size_t o = S->Ncap, nn = Ncap;
+  int rc;
+#define GROW(ptr, T, per)                                                                              \
+  if ((rc = copy_grow((void **)&S->ptr, o * (per) * sizeof(T), nn * (per) * sizeof(T), S->stream))) \
+    return rc;
+  GROW(x, double, 3) GROW(q, double, 1) GROW(type, int, 1) GROW(tag, int, 1) GROW(owner, int, 1)
+  GROW(img, signed char, 3) GROW(num_bonds, int, 1) GROW(num_hbonds, int, 1)
+#undef GROW
+  // scratch-like arrays: contents need not survive
+  if ((rc = swr_alloc(&S->packed, nn)) || (rc = swr_alloc(&S->Hindex, nn)) || (rc = swr_alloc(&S->bond_index, nn)) ||
+      (rc = swr_alloc(&S->bond_end, nn)) || (rc = swr_alloc(&S->hbond_index, nn)) ||
+      (rc = swr_alloc(&S->hbond_end, nn)) || (rc = swr_alloc(&S->bo_dboc, 2 * nn)) ||
+      (rc = swr_alloc(&S->dDeltap_self, 3 * nn)) || (rc = swr_alloc(&S->bond_mark, nn)) ||
+      (rc = swr_alloc(&S->bond_mark2, nn)) || (rc = swr_alloc(&S->counts, 2 * nn)) ||
+      (rc = swr_alloc(&S->ws, WS_COUNT * nn)) || (rc = swr_alloc(&S->far_index, nn)) ||
+      (rc = swr_alloc(&S->far_end, nn)))
+    return rc;
+  HIP_CHECK(hipMemsetAsync(S->counts, 0, 2 * nn * sizeof(int), S->stream));
+  S->Ncap = Ncap;
+  return SWR_OK;
+}
+
+static int ensure_local(swr_system *S, int ncap) {
+  if (ncap <= S->ncap) return SWR_OK;
+  const size_t o = S->ncap, nn = ncap;
+  int rc;
+  if ((rc = copy_grow((void **)&S->v, o * 3 * sizeof(double), nn * 3 * sizeof(double), S->stream))) return rc;
+  if ((rc = copy_grow((void **)&S->s_hist, o * 5 * sizeof(double), nn * 5 * sizeof(double), S->stream))) return rc;
+  if ((rc = copy_grow((void **)&S->t_hist, o * 5 * sizeof(double), nn * 5 * sizeof(double), S->stream))) return rc;
+  if ((rc = copy_grow((void **)&S->st, o * 2 * sizeof(double), nn * 2 * sizeof(double), S->stream))) return rc;
+  if (o < nn) {
+    HIP_CHECK(hipMemsetAsync(S->s_hist + o * 5, 0, (nn - o) * 5 * sizeof(double), S->stream));
+    HIP_CHECK(hipMemsetAsync(S->t_hist + o * 5, 0, (nn - o) * 5 * sizeof(double), S->stream));
+    HIP_CHECK(hipMemsetAsync(S->st + o * 2, 0, (nn - o) * 2 * sizeof(double), S->stream));
+  }
+  S->cg_blocks = swr_qeq_blocks(ncap);
+  if ((rc = swr_alloc(&S->H_firstnbr, nn)) || (rc = swr_alloc(&S->H_numnbrs, nn)) ||
+      (rc = swr_alloc(&S->Hdia_inv, nn)) || (rc = swr_alloc(&S->b_s, 2 * nn)) ||
+      (rc = swr_alloc(&S->cgv, (size_t)CGV_COUNT * 2 * nn)) ||
+      (rc = swr_alloc(&S->partials, (size_t)(S->cg_blocks + 2) * 8)))
+    return rc;
+  S->b_t = nullptr;
+  S->ncap = ncap;
+  return SWR_OK;
+}
+
+extern "C" {
+
+void swr_control_default(swr_control_t *c) {
+  std::memset(c, 0, sizeof(*c));
+  c->bond_cut = 5.0;    // pair_reaxc_sunway.cpp:216
+  c->hbond_cut = 7.5;   // :217
+  c->thb_cut = 0.001;   // :218
+  c->safezone = 1.2;    // SAFE_ZONE
+  c->saferzone = 1.4;   // SAFER_ZONE
+  c->mincap = 50;       // MIN_CAP
+  c->maxfar = 1024;     // :241
+  c->swa = 0.0;
+  c->swb = 10.0;
+  c->tol = 1e-6;
+  c->skin = 2.0;
+  c->qeq_mixed = 0;
+  c->qeq_imax = 200;    // fix_qeq_reax_sunway.cpp:998
+}
+
+int swr_create(const char *ffield_path, int ntypes, const char *const *elements, const swr_control_t *ctl,
+               swr_system **out) {
+  if (!ffield_path || !elements || !out || ntypes < 1 || ntypes > SWR_MAX_TYPES) {
+    swr_set_error_msg("swr_create: bad arguments");
+    return SWR_ERR_ARG;
+  }
+  swr_system *S = new swr_system();
+  if (ctl) S->ctl = *ctl; else swr_control_default(&S->ctl);
+  std::string err = swr_read_ffield(ffield_path, &S->ff);
+  if (!err.empty()) {
+    swr_set_error_msg(err);
+    delete S;
+    return SWR_ERR_FFIELD;
+  }
+  if (S->ff.ne() > SWR_MAX_ELEM) {
+    swr_set_error_msg("force field has more elements than supported");
+    delete S;
+    return SWR_ERR_FFIELD;
+  }
+  std::vector<int> map(ntypes + 1, -1);
+  for (int t = 1; t <= ntypes; t++) {
+    std::string e = elements[t - 1];
+    for (char &c : e) c = (char)std::toupper((unsigned char)c);
+    for (int k = 0; k < S->ff.ne(); k++)
+      if (S->ff.atom[k].name == e) map[t] = k;
+    if (map[t] < 0) {
+      swr_set_error_msg("Non-existent ReaxFF type " + e);
+      delete S;
+      return SWR_ERR_FFIELD;
+    }
+  }
+  S->mixed = S->ctl.qeq_mixed;
+  S->imax = S->ctl.qeq_imax > 0 ? S->ctl.qeq_imax : 200;
+  fill_params(S, map);
+  hipError_t e;
+  if ((e = hipGetDevice(&S->device)) != hipSuccess || (e = hipStreamCreate(&S->stream)) != hipSuccess) {
+    swr_set_error_msg(std::string("HIP init: ") + hipGetErrorString(e));
+    delete S;
+    return SWR_ERR_HIP;
+  }
+  int rc;
+  if ((rc = swr_alloc(&S->dp, 1)) || (rc = swr_alloc(&S->status, 1)) || (rc = swr_alloc(&S->cgs, 2)) ||
+      (rc = swr_alloc(&S->counters, 4))) {
+    swr_destroy(S);
+    return rc;
+  }
+  if (hipHostMalloc((void **)&S->status_host, sizeof(DevStatus), 0) != hipSuccess) {
+    swr_set_error_msg("hipHostMalloc failed");
+    swr_destroy(S);
+    return SWR_ERR_NOMEM;
+  }
+  std::memset(S->status_host, 0, sizeof(DevStatus));
+  (void)hipMemcpy(S->dp, &S->hp, sizeof(DevParams), hipMemcpyHostToDevice);
+  (void)hipMemset(S->status, 0, sizeof(DevStatus));
+  (void)hipMemset(S->counters, 0, 4 * sizeof(unsigned int));
+  (void)hipMemset(S->cgs, 0, 2 * sizeof(CgScalars));
+  *out = S;
+  return SWR_OK;
+}
+
+int swr_destroy(swr_system *S) {
+  if (!S) return SWR_OK;
+  if (S->stream) (void)hipStreamSynchronize(S->stream);
+  void *ptrs[] = {S->dp, S->x, S->v, S->q, S->type, S->tag, S->owner, S->img, S->mask, S->packed,
+                  S->bin_of, S->bin_count, S->bin_start, S->bin_fill, S->bp_pos, S->bp_id, S->stencil_rows,
+                  S->voff, S->vcount, S->vlist, S->far, S->far_index, S->far_end, S->num_bonds, S->num_hbonds,
+                  S->Hindex, S->bond_index, S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, S->Cdbo, S->Cdbopi,
+                  S->Cdbopi2, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,
+                  S->bond_mark2, S->counts, S->ws, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32,
+                  S->Hdia_inv, S->b_s, S->st, S->s_hist, S->t_hist, S->cgv, S->cgs, S->partials, S->counters,
+                  S->status, S->scan_tmp, S->tmp_i, S->mixw};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (S->status_host) (void)hipHostFree(S->status_host);
+  if (S->stream) (void)hipStreamDestroy(S->stream);
+  delete S;
+  return SWR_OK;
+}
+
+int swr_set_atoms(swr_system *S, int n, int N, const int *tag, const int *type, const double *x, const double *q,
+                  const int *owner, const signed char *img) {
+  if (!S || n < 0 || N < n || !tag || !type || !x) {
+    swr_set_error_msg("swr_set_atoms: bad arguments");
+    return SWR_ERR_ARG;
+  }
+  for (int i = 0; i < N; i++)
+    if (type[i] < 1 || type[i] > S->hp.ntypes) {
+      swr_set_error_msg("swr_set_atoms: atom type out of range");
+      return SWR_ERR_ARG;
+    }
+  int rc;
+  if ((rc = swr_nbr_ensure_atoms(S, std::max(N, 1))) || (rc = ensure_local(S, std::max(n, 1)))) return rc;
+  if (n != S->n) S->qeq_initialized = 0;
+  S->n = n;
+  S->N = N;
+  HIP_CHECK(hipMemcpyAsync(S->x, x, sizeof(double) * 3 * N, hipMemcpyHostToDevice, S->stream));
+  HIP_CHECK(hipMemcpyAsync(S->tag, tag, sizeof(int) * N, hipMemcpyHostToDevice, S->stream));
+  HIP_CHECK(hipMemcpyAsync(S->type, type, sizeof(int) * N, hipMemcpyHostToDevice, S->stream));
+  if (q) HIP_CHECK(hipMemcpyAsync(S->q, q, sizeof(double) * N, hipMemcpyHostToDevice, S->stream));
+  else HIP_CHECK(hipMemsetAsync(S->q, 0, sizeof(double) * N, S->stream));
+  std::vector<int> own(N);
+  std::vector<signed char> im(3 * (size_t)N, 0);
+  for (int i = 0; i < N; i++) own[i] = owner ? owner[i] : i;
+  if (img) std::memcpy(im.data(), img, 3 * (size_t)N);
+  for (int i = n; i < N; i++)
+    if (own[i] < 0 || own[i] >= n) {
+      swr_set_error_msg("swr_set_atoms: ghost owner out of range");
+      return SWR_ERR_ARG;
+    }
+  HIP_CHECK(hipMemcpyAsync(S->owner, own.data(), sizeof(int) * N, hipMemcpyHostToDevice, S->stream));
+  HIP_CHECK(hipMemcpyAsync(S->img, im.data(), 3 * (size_t)N, hipMemcpyHostToDevice, S->stream));
+  HIP_CHECK(hipStreamSynchronize(S->stream));
+  return SWR_OK;
+}
+
+int swr_set_capacities(swr_system *S, const int *num_bonds, const int *num_hbonds) {
+  if (!S || !num_bonds || !num_hbonds) return SWR_ERR_ARG;
+  HIP_CHECK(hipMemcpy(S->num_bonds, num_bonds, sizeof(int) * S->N, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(S->num_hbonds, num_hbonds, sizeof(int) * S->N, hipMemcpyHostToDevice));
+  S->capacities_set = true;
+  return SWR_OK;
+}
+
+int swr_npair_full_bin_ghost_build(swr_system *S, double cutneigh) {
+  if (!S || S->N <= 0) return SWR_ERR_STATE;
+  if (cutneigh <= 0) {
+    double cutmax = std::max(std::max(S->ff.nonb_cut, S->ctl.hbond_cut), 2 * S->ctl.bond_cut);
+    cutneigh = cutmax + S->ctl.skin;
+  }
+  // bin grid over the atoms' bounding box (host copy of x for the extent)
+  std::vector<double> xh(3 * (size_t)S->N);
+  HIP_CHECK(hipMemcpy(xh.data(), S->x, xh.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < S->N; i++)
+    for (int d = 0; d < 3; d++) { lo[d] = std::min(lo[d], xh[3 * i + d]); hi[d] = std::max(hi[d], xh[3 * i + d]); }
+  int rc = swr_nbr_set_grid(S, lo, hi, 0.0, cutneigh);
+  if (rc) return rc;
+  return swr_nbr_build(S);
+}
+
+int swr_write_reax_atoms_and_pack(swr_system *S) {
+  if (!S || S->N <= 0) return SWR_ERR_STATE;
+  return swr_lists_pack(S);
+}
+
+static int ensure_bond_lists(swr_system *S, int need_bonds, int need_hbonds) {
+  int rc;
+  if (need_bonds > S->bond_cap) {
+    int cap = std::max((int)(need_bonds * S->ctl.safezone), S->ctl.mincap * 25);
+    if ((rc = swr_alloc(&S->bonds, cap)) || (rc = swr_alloc(&S->bo_data, cap)) || (rc = swr_alloc(&S->BO, cap)) ||
+        (rc = swr_alloc(&S->BOpi, 2 * (size_t)cap)) || (rc = swr_alloc(&S->Cdbo, cap)) ||
+        (rc = swr_alloc(&S->Cdbopi, cap)) || (rc = swr_alloc(&S->Cdbopi2, cap)))
+      return rc;
+    S->bond_cap = cap;
+  }
+  if (need_hbonds > S->hbond_cap) {
+    int cap = std::max((int)(need_hbonds * S->ctl.saferzone), S->ctl.mincap * 25);
+    if ((rc = swr_alloc(&S->hbonds, cap))) return rc;
+    S->hbond_cap = cap;
+  }
+  return SWR_OK;
+}
+
+// Reads the device status block (synchronising) and grows the bond lists
+// when the capacity sum approaches the allocation (ReAllocate,
+// reaxc_allocate_sunway.cpp:433-549, DANGER_ZONE 0.9).
+static int refresh_status(swr_system *S) {
+  HIP_CHECK(hipMemcpyAsync(S->status_host, S->status, sizeof(DevStatus), hipMemcpyDeviceToHost, S->stream));
+  HIP_CHECK(hipStreamSynchronize(S->stream));
+  return SWR_OK;
+}
+
+int swr_reset(swr_system *S) {
+  if (!S || S->N <= 0 || !S->vlist) return SWR_ERR_STATE;
+  int rc;
+  if (!S->capacities_set) {
+    // setup-time estimate (Estimate_Storages) from a counting pass
+    if ((rc = swr_lists_pack(S)) || (rc = swr_lists_estimate(S))) return rc;
+    S->capacities_set = true;
+  }
+  // totals first, so the lists can be sized before they are written
+  if ((rc = swr_lists_reset(S)) || (rc = refresh_status(S))) return rc;
+  const DevStatus &h = *S->status_host;
+  if (h.total_bonds > S->bond_cap * 0.9 || h.total_hbonds > S->hbond_cap * 0.9 || !S->bonds || !S->hbonds) {
+    if ((rc = ensure_bond_lists(S, std::max(h.total_bonds, 1), std::max(h.total_hbonds, 1)))) return rc;
+    HIP_CHECK(hipMemsetAsync(&S->status->flags, 0, sizeof(unsigned int), S->stream));
+    if ((rc = swr_lists_reset(S))) return rc;
+  }
+  return SWR_OK;
+}
+
+int swr_write_reax_lists(swr_system *S) {
+  if (!S || !S->vlist) return SWR_ERR_STATE;
+  return swr_lists_run(S, 0);
+}
+
+int swr_init_forces_noqeq_full(swr_system *S) {
+  if (!S || !S->bonds) return SWR_ERR_STATE;
+  return swr_lists_run(S, 2);
+}
+
+int swr_write_reax_lists_init_forces(swr_system *S) {
+  if (!S || !S->vlist || !S->bonds) return SWR_ERR_STATE;
+  return swr_lists_run(S, 1);
+}
+
+int swr_validate_lists(swr_system *S) {
+  if (!S || !S->bonds) return SWR_ERR_STATE;
+  return swr_lists_validate(S, MARK_SWEEPS);
+}
+
+int swr_bo(swr_system *S) {
+  if (!S || !S->bonds) return SWR_ERR_STATE;
+  return swr_bo_run(S);
+}
+
+int swr_qeq_init_matvec(swr_system *S) {
+  if (!S || !S->vlist || S->n <= 0) return SWR_ERR_STATE;
+  long long need = 0;
+  HIP_CHECK(hipMemcpy(&need, S->voff + S->n, sizeof(long long), hipMemcpyDeviceToHost));
+  int rc;
+  if (need > S->Hcap) {
+    long long cap = (long long)(need * 1.15) + 1024;
+    if ((rc = swr_alloc(&S->H_jlist, cap)) || (rc = swr_alloc(&S->H_val, cap))) return rc;
+    if (S->mixed && (rc = swr_alloc(&S->H_val32, cap))) return rc;
+    S->Hcap = cap;
+  }
+  if (S->mixed && !S->H_val32 && (rc = swr_alloc(&S->H_val32, S->Hcap))) return rc;
+  return swr_qeq_build_H(S);
+}
+
+int swr_sparse_matvec(swr_system *S, const double *x_dev, double *b_dev) {
+  if (!S || !S->H_val || !x_dev || !b_dev) return SWR_ERR_STATE;
+  return swr_qeq_spmv(S, x_dev, b_dev);
+}
+
+int swr_qeq_cg(swr_system *S, int *iters_s, int *iters_t) {
+  if (!S || !S->H_val) return SWR_ERR_STATE;
+  int rc = S->mixed ? swr_qeq_mixed_solve(S) : swr_qeq_cg_run(S, S->status_host->cg_iters[0] + 2);
+  if (rc) return rc;
+  if ((rc = refresh_status(S))) return rc;
+  if (iters_s) *iters_s = S->status_host->cg_iters[0];
+  if (iters_t) *iters_t = S->status_host->cg_iters[1];
+  return status_to_rc(S->status_host->flags);
+}
+
+int swr_calculate_q(swr_system *S) {
+  if (!S || S->n <= 0) return SWR_ERR_STATE;
+  return swr_qeq_calc_q(S);
+}
+
+int swr_qeq_pre_force(swr_system *S, int *iters_s, int *iters_t) {
+  int rc;
+  if ((rc = swr_qeq_init_matvec(S))) return rc;
+  if ((rc = swr_qeq_cg(S, iters_s, iters_t))) return rc;
+  return swr_calculate_q(S);
+}
+
+int swr_pair_compute(swr_system *S) {
+  int rc;
+  if ((rc = swr_write_reax_atoms_and_pack(S)) || (rc = swr_reset(S)) || (rc = swr_write_reax_lists_init_forces(S)) ||
+      (rc = swr_validate_lists(S)) || (rc = swr_bo(S)))
+    return rc;
+  return SWR_OK;
+}
+
+int swr_sync(swr_system *S) {
+  if (!S) return SWR_ERR_ARG;
+  int rc = refresh_status(S);
+  if (rc) return rc;
+  unsigned int f = S->status_host->flags;
+  if (f) {
+    S->sticky_flags |= f;
+    HIP_CHECK(hipMemsetAsync(&S->status->flags, 0, sizeof(unsigned int), S->stream));
+    HIP_CHECK(hipStreamSynchronize(S->stream));
+  }
+  return status_to_rc(f);
+}
+
+// ---------------------------------------------------------------- mirrors
+namespace {
+
+struct View {
+  const void *ptr;
+  int64_t bytes;
+  int stride_kind;  // 0 plain, 1 de-interleave .x of double2, 2 de-interleave .y
+};
+
+bool lookup(swr_system *S, const std::string &k, View &v) {
+  const int64_t N = S->N, n = S->n;
+  const size_t c = S->Ncap;
+  auto set = [&](const void *p, int64_t b, int kind = 0) { v.ptr = p; v.bytes = b; v.stride_kind = kind; return true; };
+  if (k == "x") return set(S->x, 24 * N);
+  if (k == "q") return set(S->q, 8 * N);
+  if (k == "tag") return set(S->tag, 4 * N);
+  if (k == "type") return set(S->type, 4 * N);
+  if (k == "owner") return set(S->owner, 4 * N);
+  if (k == "img") return set(S->img, 3 * N);
+  if (k == "v") return set(S->v, 24 * n);
+  if (k == "packed_atoms") return set(S->packed, sizeof(swr_atom_pack_t) * N);
+  if (k == "verlet_offsets") return set(S->voff, 8 * (N + 1));
+  if (k == "verlet_list") return set(S->vlist, 4 * S->vtotal);
+  if (k == "far") return set(S->far, sizeof(swr_far_neighbor_t) * S->vtotal);
+  if (k == "far_index") return set(S->far_index, 4 * N);
+  if (k == "far_end") return set(S->far_end, 4 * N);
+  if (k == "num_bonds") return set(S->num_bonds, 4 * N);
+  if (k == "num_hbonds") return set(S->num_hbonds, 4 * N);
+  if (k == "Hindex") return set(S->Hindex, 4 * N);
+  if (k == "bond_index") return set(S->bond_index, 4 * N);
+  if (k == "bond_end") return set(S->bond_end, 4 * N);
+  if (k == "bonds") return set(S->bonds, sizeof(swr_bond_t) * (int64_t)S->bond_cap);
+  if (k == "bo_data") return set(S->bo_data, sizeof(swr_bond_order_t) * (int64_t)S->bond_cap);
+  if (k == "BO") return set(S->BO, 8 * (int64_t)S->bond_cap);
+  if (k == "BOpi") return set(S->BOpi, 16 * (int64_t)S->bond_cap);
+  if (k == "Cdbo") return set(S->Cdbo, 8 * (int64_t)S->bond_cap);
+  if (k == "Cdbopi") return set(S->Cdbopi, 8 * (int64_t)S->bond_cap);
+  if (k == "Cdbopi2") return set(S->Cdbopi2, 8 * (int64_t)S->bond_cap);
+  if (k == "hbond_index") return set(S->hbond_index, 4 * (int64_t)S->status_host->numH);
+  if (k == "hbond_end") return set(S->hbond_end, 4 * (int64_t)S->status_host->numH);
+  if (k == "hbonds") return set(S->hbonds, sizeof(swr_hbond_t) * (int64_t)S->hbond_cap);
+  if (k == "bo_dboc") return set(S->bo_dboc, 16 * N);
+  if (k == "dDeltap_self") return set(S->dDeltap_self, 24 * N);
+  if (k == "bond_mark") return set(S->bond_mark, 4 * N);
+  static const char *wsn[WS_COUNT] = {"Deltap", "Deltap_boc", "Delta", "Delta_e", "Delta_val", "vlpex", "nlp",
+                                      "nlp_temp", "Clp", "Delta_lp", "dDelta_lp", "Delta_lp_temp", "dDelta_lp_temp"};
+  for (int w = 0; w < WS_COUNT; w++)
+    if (k == wsn[w]) return set(S->ws + (size_t)w * c, 8 * N);
+  if (k == "H_firstnbr") return set(S->H_firstnbr, 4 * n);
+  if (k == "H_numnbrs") return set(S->H_numnbrs, 4 * n);
+  if (k == "H_jlist") return set(S->H_jlist, 4 * S->Hcap);
+  if (k == "H_val") return set(S->H_val, 8 * S->Hcap);
+  if (k == "Hdia_inv") return set(S->Hdia_inv, 8 * n);
+  if (k == "b_s") return set(S->b_s, 8 * n, 1);
+  if (k == "b_t") return set(S->b_s, 8 * n, 2);
+  if (k == "s") return set(S->st, 8 * n, 1);
+  if (k == "t") return set(S->st, 8 * n, 2);
+  if (k == "s_hist") return set(S->s_hist, 40 * n);
+  if (k == "t_hist") return set(S->t_hist, 40 * n);
+  return false;
+}
+
+}  // namespace
+
+int64_t swr_get(swr_system *S, const char *name, void *out) {
+  if (!S || !name) return -1;
+  std::string k = name;
+  if (k == "hbond_far") {
+    // far-list slot of each hbond (ptr rebased onto the far array)
+    int64_t cnt = S->hbond_cap;
+    if (!out) return 8 * cnt;
+    std::vector<swr_hbond_t> hb(cnt);
+    if (cnt) HIP_CHECK(hipMemcpy(hb.data(), S->hbonds, sizeof(swr_hbond_t) * cnt, hipMemcpyDeviceToHost));
+    int64_t *o = (int64_t *)out;
+    for (int64_t i = 0; i < cnt; i++) {
+      intptr_t off = (intptr_t)hb[i].ptr - (intptr_t)S->far;
+      o[i] = (hb[i].ptr && off >= 0) ? (int64_t)(off / (intptr_t)sizeof(swr_far_neighbor_t)) : -1;
+    }
+    return 8 * cnt;
+  }
+  View v;
+  if (!lookup(S, k, v)) return -1;
+  if (!out || v.bytes <= 0 || !v.ptr) return v.bytes < 0 ? 0 : v.bytes;
+  (void)hipStreamSynchronize(S->stream);
+  if (v.stride_kind == 0) {
+    HIP_CHECK(hipMemcpy(out, v.ptr, v.bytes, hipMemcpyDeviceToHost));
+  } else {
+    int64_t cnt = v.bytes / 8;
+    std::vector<double> tmp(2 * cnt);
+    HIP_CHECK(hipMemcpy(tmp.data(), v.ptr, 16 * cnt, hipMemcpyDeviceToHost));
+    double *o = (double *)out;
+    for (int64_t i = 0; i < cnt; i++) o[i] = tmp[2 * i + (v.stride_kind - 1)];
+  }
+  return v.bytes;
+}
+
+int swr_put(swr_system *S, const char *name, const void *in, int64_t bytes) {
+  if (!S || !name || !in) return SWR_ERR_ARG;
+  std::string k = name;
+  View v;
+  if (!lookup(S, k, v) || bytes > v.bytes || v.stride_kind != 0) return SWR_ERR_ARG;
+  HIP_CHECK(hipStreamSynchronize(S->stream));
+  HIP_CHECK(hipMemcpy((void *)v.ptr, in, bytes, hipMemcpyHostToDevice));
+  if (k == "num_bonds") S->capacities_set = true;
+  return SWR_OK;
+}
+
+void *swr_device_ptr(swr_system *S, const char *name) {
+  if (!S || !name) return nullptr;
+  View v;
+  if (!lookup(S, name, v)) return nullptr;
+  return (void *)v.ptr;
+}
+
+int64_t swr_get_scalar(swr_system *S, const char *name) {
+  if (!S || !name) return -1;
+  std::string k = name;
+  if (k == "n") return S->n;
+  if (k == "N") return S->N;
+  if (k == "verlet_total") return S->vtotal;
+  if (k == "bond_cap") return S->bond_cap;
+  if (k == "hbond_cap") return S->hbond_cap;
+  if (refresh_status(S)) return -1;
+  const DevStatus &h = *S->status_host;
+  if (k == "numH") return h.numH;
+  if (k == "total_bonds") return h.total_bonds;
+  if (k == "total_hbonds") return h.total_hbonds;
+  if (k == "need_bond_cap") return h.need_bond_cap;
+  if (k == "need_hbond_cap") return h.need_hbond_cap;
+  if (k == "flags") return h.flags | S->sticky_flags;
+  if (k == "matvecs_s") return h.cg_iters[0];
+  if (k == "matvecs_t") return h.cg_iters[1];
+  return -1;
+}
+
+}  // extern "C"

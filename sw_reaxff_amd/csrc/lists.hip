@@ -1,0 +1,477 @@
+// lists.hip — atom pack (a2), Reset (a4), far list (a3), bonds + hbonds
+// (a5, a6) and list validation (a7).
+//
+// The far/bond/hbond pass is one wave per atom i streaming i's Verlet
+// candidates 64 at a time:
+//   far   : d^2 <= nonb_cut^2, entry written with a wave-ballot compaction in
+//           candidate order (write_reax_lists_c_para, pair_reaxc_sw64.c:276-331);
+//   bond  : d <= bond_cut and uncorrected BO' >= bo_cut (BOp_single,
+//           reaxc_forces_sunway.cpp:677-825; kernel reaxc_forces_sw64.c:1126-1280),
+//           gathered in LDS, then sorted by nbr (my_bubble_sort, :31-75);
+//   hbond : local H, far entry with d <= hbond_cut and p_hbond(j) == 2
+//           (Init_Forces_noQEq_HB_Full_C, reaxc_forces_sw64.c:810-862).
+// Far slots of atom i are [voff[i], voff[i+1]) — bounded by its Verlet
+// count — so no separate counting pass is needed (the reference uses
+// i*maxfar, pair_reaxc_sw64.c:256).
+#include "swr_internal.h"
+#include "swr_device.h"
+
+namespace {
+
+constexpr int LT = 256;                 // threads per block
+constexpr int LW = LT / SWR_WAVE;       // waves per block
+constexpr int MB = SWR_MAX_ATOM_BONDS;  // LDS bond slots per wave
+
+// Orders this wave's LDS writes before its later LDS reads (each wave only
+// touches its own LDS slice, so no workgroup barrier is needed).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void k_pack(const int *__restrict__ tag, const int *__restrict__ type, const double *__restrict__ x,
+                       const double *__restrict__ q, const DevParams *__restrict__ P, int N,
+                       swr_atom_pack_t *__restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  swr_atom_pack_t a;
+  a.orig_id = tag[i];
+  a.type = P->map[type[i]];
+  a.x[0] = x[3 * i]; a.x[1] = x[3 * i + 1]; a.x[2] = x[3 * i + 2];
+  a.q = q[i];
+  out[i] = a;
+}
+
+// Reset_Atoms flags / Reset_Neighbor_Lists inputs.
+__global__ void k_reset_flags(const swr_atom_pack_t *__restrict__ P_atoms, const DevParams *__restrict__ P, int n,
+                              int N, double hbond_cut, const int *__restrict__ num_hbonds, int *__restrict__ hflag,
+                              int *__restrict__ hcap) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int t = P_atoms[i].type;
+  int h = (i < n && hbond_cut > 0 && t >= 0 && P->p_hbond[t] == 1) ? 1 : 0;
+  hflag[i] = h;
+  hcap[i] = h ? num_hbonds[i] : 0;
+}
+
+__global__ void k_reset_apply(int n, int N, const int *__restrict__ hflag, const int *__restrict__ hpos,
+                              const int *__restrict__ hstart, int *__restrict__ Hindex,
+                              const int *__restrict__ bond_index, int *__restrict__ bond_end,
+                              int *__restrict__ hbond_index, int *__restrict__ hbond_end, DevStatus *st,
+                              int bond_cap, int hbond_cap) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    if (st->total_bonds > bond_cap) atomicOr(&st->flags, SWR_F_BOND_TOTAL);
+    if (st->total_hbonds > hbond_cap) atomicOr(&st->flags, SWR_F_HBOND_TOTAL);
+  }
+  if (i >= N) return;
+  bond_end[i] = bond_index[i];
+  int H = hflag[i] ? hpos[i] : -1;
+  Hindex[i] = H;
+  if (H >= 0) hbond_index[H] = hbond_end[H] = hstart[i];
+}
+
+// One bond found by a wave, kept in LDS until the per-atom sort.
+struct BondRec {
+  double d, dvec0, dvec1, dvec2;
+  double BO, BO_s, pi, pi2, C12, C34, C56;
+  int nbr;
+  int ej;
+};
+
+__device__ __forceinline__ void store_far(swr_far_neighbor_t *dst, int j, int tj, int tagj, double d, double qj,
+                                          double dx, double dy, double dz) {
+  int4 *p = reinterpret_cast<int4 *>(dst);
+  p[0] = make_int4(j, tj, tagj, 0);
+  p[1] = make_int4(0, 0, __double2loint(d), __double2hiint(d));
+  double2 *pd = reinterpret_cast<double2 *>(dst);
+  pd[2] = make_double2(qj, dx);
+  pd[3] = make_double2(dy, dz);
+}
+
+// MODE 0: far list only; MODE 1: far + bonds + hbonds from the Verlet list;
+// MODE 2: bonds + hbonds from an existing far list (no far writes);
+// MODE 3: count bonds and hbonds only (setup-time capacity estimate).
+template <int MODE>
+__global__ void __launch_bounds__(LT)
+k_lists(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__restrict__ atoms,
+        const long long *__restrict__ voff, const int *__restrict__ vlist, swr_far_neighbor_t *__restrict__ far,
+        int *__restrict__ far_index, int *__restrict__ far_end,
+        const int *__restrict__ num_bonds, const int *__restrict__ bond_index, int *__restrict__ bond_end,
+        swr_bond_t *__restrict__ bonds, swr_bond_order_t *__restrict__ bo_data, double *__restrict__ BO,
+        double *__restrict__ BOpi, double *__restrict__ Cdbo, double *__restrict__ Cdbopi,
+        double *__restrict__ Cdbopi2, const int *__restrict__ Hindex, const int *__restrict__ num_hbonds,
+        const int *__restrict__ hbond_index, int *__restrict__ hbond_end, swr_hbond_t *__restrict__ hbonds,
+        double *__restrict__ bo_dboc, double *__restrict__ dDeltap_self, int *__restrict__ bond_mark,
+        double *__restrict__ Deltap, double *__restrict__ Deltap_boc, int *__restrict__ bond_count,
+        int *__restrict__ hb_count, int bond_cap, int hbond_cap, DevStatus *st) {
+  __shared__ BondRec srec[LW][MB];
+  __shared__ double sdbop[LW][MB][3];
+  const int w = wave_in_block();
+  const int blk = xcd_block(gridDim.x);
+  const int i = blk * LW + w;
+  if (i >= N) return;
+  const int lane = lane_id();
+  const swr_atom_pack_t ai = atoms[i];
+  const int ei = ai.type;
+  const double nonb_cutsq = P->nonb_cutsq, bond_cut = P->bond_cut, hbond_cut = P->hbond_cut, bo_cut = P->bo_cut;
+
+  long long cbeg, cend;
+  if (MODE == 2) { cbeg = far_index[i]; cend = far_end[i]; }
+  else { cbeg = voff[i]; cend = voff[i + 1]; }
+  const long long far0 = (MODE == 2) ? cbeg : voff[i];
+  int nfar = 0;
+  int nb = 0;
+  const bool doH = (MODE != 0) && (i < n) && ei >= 0 && P->p_hbond[ei] == 1 && hbond_cut > 0;
+  int H = (doH && MODE != 3) ? Hindex[i] : -1;
+  int htop = (doH && MODE != 3) ? hbond_end[H] : 0;
+  // capacity of this H's slots, never past the end of the list
+  const int hlim = (doH && MODE != 3) ? min(hbond_index[H] + num_hbonds[i], hbond_cap) : 0;
+
+  for (long long base = cbeg; base < cend; base += SWR_WAVE) {
+    const long long k = base + lane;
+    const bool valid = k < cend;
+    int j = 0, tj = -1, tagj = 0;
+    double dx = 0, dy = 0, dz = 0, dsq = 0, d = 0, qj = 0;
+    bool infar = false;
+    long long slot = 0;
+    if (MODE == 2) {
+      if (valid) {
+        const swr_far_neighbor_t e = far[k];
+        j = e.nbr; tj = e.type; d = e.d; dx = e.dvec[0]; dy = e.dvec[1]; dz = e.dvec[2];
+        infar = true;
+        slot = k;
+      }
+    } else if (valid) {
+      j = vlist[k];
+      const swr_atom_pack_t aj = atoms[j];
+      dx = aj.x[0] - ai.x[0]; dy = aj.x[1] - ai.x[1]; dz = aj.x[2] - ai.x[2];
+      dsq = dx * dx + dy * dy + dz * dz;
+      infar = dsq <= nonb_cutsq;
+      if (infar) {
+        d = sqrt(dsq);
+        tj = aj.type; tagj = aj.orig_id; qj = aj.q;
+      }
+    }
+    if (MODE == 0 || MODE == 1) {
+      const unsigned long long m = __ballot(infar);
+      slot = far0 + nfar + ballot_rank(m);
+      if (infar) store_far(far + slot, j, tj, tagj, d, qj, dx, dy, dz);
+      nfar += __popcll(m);
+    }
+    if (MODE == 0) continue;
+    // ---- bonds: uncorrected bond order (BOp_single)
+    bool isbond = false;
+    BondRec rec;
+    if (infar && ei >= 0 && tj >= 0 && d <= bond_cut) {
+      const DevTbp tw = P->tbp[ei * SWR_MAX_ELEM + tj];
+      double C12, C34, C56, BO_s, BO_pi, BO_pi2;
+      if (P->r_s[ei] > 0.0 && P->r_s[tj] > 0.0) {
+        C12 = tw.p_bo1 * pow(d / tw.r_s, tw.p_bo2);
+        BO_s = (1.0 + bo_cut) * exp(C12);
+      } else BO_s = C12 = 0.0;
+      if (P->r_pi[ei] > 0.0 && P->r_pi[tj] > 0.0) {
+        C34 = tw.p_bo3 * pow(d / tw.r_p, tw.p_bo4);
+        BO_pi = exp(C34);
+      } else BO_pi = C34 = 0.0;
+      if (P->r_pi_pi[ei] > 0.0 && P->r_pi_pi[tj] > 0.0) {
+        C56 = tw.p_bo5 * pow(d / tw.r_pp, tw.p_bo6);
+        BO_pi2 = exp(C56);
+      } else BO_pi2 = C56 = 0.0;
+      const double BOt = BO_s + BO_pi + BO_pi2;
+      if (BOt >= bo_cut) {
+        isbond = true;
+        rec.d = d; rec.dvec0 = dx; rec.dvec1 = dy; rec.dvec2 = dz;
+        rec.BO = BOt; rec.BO_s = BO_s; rec.pi = BO_pi; rec.pi2 = BO_pi2;
+        rec.C12 = C12; rec.C34 = C34; rec.C56 = C56;
+        rec.nbr = j; rec.ej = tj;
+      }
+    }
+    const unsigned long long bm = __ballot(isbond);
+    if (MODE != 3 && isbond) {
+      const int r = nb + ballot_rank(bm);
+      if (r < MB) srec[w][r] = rec;
+    }
+    nb += __popcll(bm);
+    // ---- hbonds of a local H: O/N partners within hbond_cut
+    if (doH) {
+      bool ish = false;
+      if (infar && d <= hbond_cut) {
+        const int tjj = (MODE == 2) ? atoms[j].type : tj;
+        ish = tjj >= 0 && P->p_hbond[tjj] == 2;
+      }
+      const unsigned long long hm = __ballot(ish);
+      if (ish) {
+        const int pos = htop + ballot_rank(hm);
+        if (MODE != 3 && pos < hlim) {
+          swr_hbond_t hb;
+          hb.nbr = j;
+          hb.scl = 1;
+          hb.ptr = far + slot;
+          hbonds[pos] = hb;
+        }
+      }
+      htop += __popcll(hm);
+    }
+  }
+  if ((MODE == 0 || MODE == 1) && lane == 0) {
+    far_index[i] = (int)far0;
+    far_end[i] = (int)(far0 + nfar);
+  }
+  if (MODE == 0) return;
+  if (MODE == 3) {
+    if (lane == 0) {
+      bond_count[i] = nb;
+      if (doH) hb_count[i] = htop;
+    }
+    return;
+  }
+  if (doH && lane == 0) {
+    if (htop > hlim) {
+      atomicOr(&st->flags, SWR_F_HBOND_CAP);
+      atomicMax(&st->need_hbond_cap, htop - hbond_index[H]);
+    }
+    hbond_end[H] = htop > hlim ? hlim : htop;
+    hb_count[i] = htop - hbond_index[H];
+  }
+  // ---- finish bonds: derivatives, per-atom sums in candidate order, sort
+  const int cap = num_bonds[i];
+  const int start = bond_index[i];
+  if (lane == 0) bond_count[i] = nb;
+  if (nb > MB || nb > cap || start + nb > bond_cap) {
+    if (lane == 0) {
+      atomicOr(&st->flags, nb > MB ? (SWR_F_BOND_LDS | SWR_F_BOND_CAP) : SWR_F_BOND_CAP);
+      atomicMax(&st->need_bond_cap, nb);
+      bond_end[i] = start;
+    }
+    return;
+  }
+  wave_lds_sync();  // LDS records of this wave visible to all its lanes
+  double dbop[3] = {0, 0, 0};
+  int rank = 0;
+  BondRec rc;
+  if (lane < nb) {
+    rc = srec[w][lane];
+    const DevTbp tw = P->tbp[ei * SWR_MAX_ELEM + rc.ej];
+    const double r2 = rc.d * rc.d;
+    const double Cs = tw.p_bo2 * rc.C12 / r2, Cp = tw.p_bo4 * rc.C34 / r2, Cpp = tw.p_bo6 * rc.C56 / r2;
+    const double dv[3] = {rc.dvec0, rc.dvec1, rc.dvec2};
+    const double cdb = -(rc.BO_s * Cs + rc.pi * Cp + rc.pi2 * Cpp);
+    swr_bond_order_t o;
+    o.BO_s = rc.BO_s - bo_cut;
+    for (int c = 0; c < 3; c++) {
+      o.dln_BOp_s[c] = -rc.BO_s * Cs * dv[c];
+      o.dln_BOp_pi[c] = -rc.pi * Cp * dv[c];
+      o.dln_BOp_pi2[c] = -rc.pi2 * Cpp * dv[c];
+      o.dBOp[c] = cdb * dv[c];
+      dbop[c] = o.dBOp[c];
+      sdbop[w][lane][c] = o.dBOp[c];
+    }
+    o.C1dbo = o.C2dbo = o.C3dbo = 0.0;
+    o.C1dbopi = o.C2dbopi = o.C3dbopi = o.C4dbopi = 0.0;
+    o.C1dbopi2 = o.C2dbopi2 = o.C3dbopi2 = o.C4dbopi2 = 0.0;
+    o.CdboReduction = nullptr;
+    for (int m = 0; m < nb; m++) rank += (srec[w][m].nbr < rc.nbr) ? 1 : 0;
+    const int p = start + rank;
+    swr_bond_t b;
+    b.d = rc.d; b.dvec[0] = dv[0]; b.dvec[1] = dv[1]; b.dvec[2] = dv[2];
+    b.nbr = rc.nbr;
+    b.dbond_index = start + lane;
+    b.rel_box[0] = b.rel_box[1] = b.rel_box[2] = 0;
+    b.sym_index = 0;
+    bonds[p] = b;
+    bo_data[p] = o;
+    BO[p] = rc.BO - bo_cut;
+    BOpi[2 * (size_t)p] = rc.pi;
+    BOpi[2 * (size_t)p + 1] = rc.pi2;
+    Cdbo[p] = 0.0; Cdbopi[p] = 0.0; Cdbopi2[p] = 0.0;
+  }
+  wave_lds_sync();
+  if (lane == 0) {
+    // bo_dboc[i][0] and dDeltap_self[i] accumulate in candidate order
+    // (reaxc_forces_sw64.c:1253,1262), starting from Reset's zeros.
+    double tot = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int m = 0; m < nb; m++) {
+      tot += srec[w][m].BO - bo_cut;
+      s0 += sdbop[w][m][0]; s1 += sdbop[w][m][1]; s2 += sdbop[w][m][2];
+    }
+    bond_end[i] = start + nb;
+    bo_dboc[2 * (size_t)i] = tot;
+    bo_dboc[2 * (size_t)i + 1] = 0.0;
+    dDeltap_self[3 * (size_t)i] = s0;
+    dDeltap_self[3 * (size_t)i + 1] = s1;
+    dDeltap_self[3 * (size_t)i + 2] = s2;
+    bond_mark[i] = (i < n) ? 0 : 1000;
+    // BO section 1 (reaxc_bond_orders_sw64.c:56-64), computed here since it
+    // only needs this atom's bond-order sum.
+    if (ei >= 0) {
+      Deltap[i] = tot - P->valency[ei];
+      Deltap_boc[i] = tot - P->valency_val[ei];
+    }
+  }
+  (void)dbop;
+}
+
+// Validate_Lists_C capacity update (reaxc_forces_sw64.c:880-929).
+// Counts come from the list pass (bond_count/hb_count hold the true
+// per-atom totals even when a capacity overflowed, so the next capacities
+// are large enough to retry the step).
+__global__ void k_validate(int n, int N, const int *__restrict__ bond_index, const int *__restrict__ bond_count,
+                           int *__restrict__ num_bonds, const int *__restrict__ Hindex,
+                           const int *__restrict__ hb_count, int *__restrict__ num_hbonds, double saferzone,
+                           int bond_total, DevStatus *st) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int c = bond_count[i];
+  num_bonds[i] = max(c * 2, 25);
+  const int comp = (i < N - 1) ? bond_index[i + 1] : bond_total;
+  if (bond_index[i] + c > comp) atomicOr(&st->flags, SWR_F_BOND_CAP);
+  if (i < n) {
+    const int H = Hindex[i];
+    if (H > -1) {
+      const double v = hb_count[i] * saferzone;
+      num_hbonds[i] = (int)(v > 25.0 ? v : 25.0);
+    }
+  }
+}
+
+// sort_bonds_C (reaxc_forces_sw64.c:932-971): one in-order pass
+// bond_mark[i] = min(bond_mark[i], bond_mark[j]+1).  Its result for ghost i
+// is min(1000, 1 + min over bonded j < i of the result at j) (locals stay 0,
+// later ghosts still hold 1000); Jacobi sweeps over that recursion reach the
+// same fixed point.  flags[3]: sweep k reads flags[(k+2)%3] (changed in
+// sweep k-1), sets flags[k%3], clears flags[(k+1)%3].
+__global__ void k_bond_mark(int n, int N, int sweep, const int *__restrict__ bond_index,
+                            const int *__restrict__ bond_end, const swr_bond_t *__restrict__ bonds,
+                            const int *__restrict__ src, int *__restrict__ dst, int *__restrict__ flags) {
+  if (flags[(sweep + 2) % 3] == 0) return;
+  int g = n + blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) flags[(sweep + 1) % 3] = 0;
+  if (g >= N) return;
+  int best = src[g];
+  for (int p = bond_index[g]; p < bond_end[g]; p++) {
+    int j = bonds[p].nbr;
+    if (j < g) {
+      int v = src[j] + 1;
+      if (v < best) best = v;
+    }
+  }
+  if (best != dst[g]) {
+    dst[g] = best;
+    atomicOr(&flags[sweep % 3], 1);
+  }
+}
+
+// Estimate_Storages (reaxc_forces_sunway.cpp:1189-1290): bond counts of
+// local atoms (ghosts start at MIN_BONDS) and hbond counts of local H.
+__global__ void k_estimate(int n, int N, const int *__restrict__ bond_count, const int *__restrict__ hb_count,
+                           const swr_atom_pack_t *__restrict__ atoms, const DevParams *__restrict__ P,
+                           double saferzone, int *__restrict__ num_bonds, int *__restrict__ num_hbonds) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (i < n) {
+    num_bonds[i] = max(bond_count[i] * 2, 25);
+    const int t = atoms[i].type;
+    const bool isH = t >= 0 && P->p_hbond[t] == 1 && P->hbond_cut > 0;
+    const double v = (isH ? hb_count[i] : 0) * saferzone;
+    num_hbonds[i] = (int)(v > 25.0 ? v : 25.0);
+  } else {
+    num_bonds[i] = 25;
+    num_hbonds[i] = 0;
+  }
+}
+
+__global__ void k_mark_init(int *__restrict__ flags, int *__restrict__ a, int *__restrict__ b, int N) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) { flags[0] = 0; flags[1] = 0; flags[2] = 1; }
+  if (i < N) b[i] = a[i];
+}
+
+__global__ void k_mark_finish(const int *__restrict__ flags, int last_sweep, DevStatus *st) {
+  if (flags[last_sweep % 3]) atomicOr(&st->flags, SWR_F_MARK_DEPTH);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+int swr_lists_pack(swr_system *S) {
+  hipLaunchKernelGGL(k_pack, dim3(div_up(S->N, 256)), dim3(256), 0, S->stream, S->tag, S->type, S->x, S->q, S->dp,
+                     S->N, S->packed);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_lists_alloc_bonds(swr_system *S, int bond_cap, int hbond_cap);
+
+int swr_lists_reset(swr_system *S) {
+  const int n = S->n, N = S->N;
+  int rc = swr_ensure_tmp(S, sizeof(int) * (4 * (size_t)N + 8));
+  if (rc) return rc;
+  int *hflag = S->tmp_i, *hcap = hflag + N + 1, *hpos = hcap + N + 1, *hstart = hpos + N + 1;
+  hipLaunchKernelGGL(k_reset_flags, dim3(div_up(N, 256)), dim3(256), 0, S->stream, S->packed, S->dp, n, N,
+                     S->ctl.hbond_cut, S->num_hbonds, hflag, hcap);
+  if ((rc = swr_scan_i32(S, hflag, hpos, N, &S->status->numH))) return rc;
+  if ((rc = swr_scan_i32(S, hcap, hstart, N, &S->status->total_hbonds))) return rc;
+  if ((rc = swr_scan_i32(S, S->num_bonds, S->bond_index, N, &S->status->total_bonds))) return rc;
+  hipLaunchKernelGGL(k_reset_apply, dim3(div_up(N, 256)), dim3(256), 0, S->stream, n, N, hflag, hpos, hstart,
+                     S->Hindex, S->bond_index, S->bond_end, S->hbond_index, S->hbond_end, S->status, S->bond_cap,
+                     S->hbond_cap);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_lists_run(swr_system *S, int mode) {
+  const int N = S->N;
+  const int nblk = div_up(N, LW);
+  double *ws = S->ws;
+#define LIST_ARGS                                                                                             \
+  S->n, N, S->dp, S->packed, S->voff, S->vlist, S->far, S->far_index, S->far_end, S->num_bonds, S->bond_index, \
+      S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, S->Cdbo, S->Cdbopi, S->Cdbopi2, S->Hindex,           \
+      S->num_hbonds, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,      \
+      ws + (size_t)WS_DELTAP * S->Ncap, ws + (size_t)WS_DELTAP_BOC * S->Ncap, S->counts,                 \
+      S->counts + S->Ncap, S->bond_cap, S->hbond_cap, S->status
+  if (mode == 0)
+    hipLaunchKernelGGL(k_lists<0>, dim3(nblk), dim3(LT), 0, S->stream, LIST_ARGS);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_lists<1>, dim3(nblk), dim3(LT), 0, S->stream, LIST_ARGS);
+  else if (mode == 2)
+    hipLaunchKernelGGL(k_lists<2>, dim3(nblk), dim3(LT), 0, S->stream, LIST_ARGS);
+  else
+    hipLaunchKernelGGL(k_lists<3>, dim3(nblk), dim3(LT), 0, S->stream, LIST_ARGS);
+#undef LIST_ARGS
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_lists_estimate(swr_system *S) {
+  int rc = swr_lists_run(S, 3);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_estimate, dim3(div_up(S->N, 256)), dim3(256), 0, S->stream, S->n, S->N, S->counts,
+                     S->counts + S->Ncap, S->packed, S->dp, S->ctl.saferzone, S->num_bonds, S->num_hbonds);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_lists_validate(swr_system *S, int max_sweeps) {
+  const int n = S->n, N = S->N;
+  hipLaunchKernelGGL(k_validate, dim3(div_up(N, 256)), dim3(256), 0, S->stream, n, N, S->bond_index, S->counts,
+                     S->num_bonds, S->Hindex, S->counts + S->Ncap, S->num_hbonds, S->ctl.saferzone, S->bond_cap,
+                     S->status);
+  if (N > n) {
+    int rc = swr_ensure_tmp(S, sizeof(int) * 4);
+    if (rc) return rc;
+    int *flags = S->tmp_i;
+    int *a = S->bond_mark, *b = S->bond_mark2;
+    hipLaunchKernelGGL(k_mark_init, dim3(div_up(N, 256)), dim3(256), 0, S->stream, flags, a, b, N);
+    for (int sw = 0; sw < max_sweeps; sw++) {
+      hipLaunchKernelGGL(k_bond_mark, dim3(div_up(N - n, 256)), dim3(256), 0, S->stream, n, N, sw, S->bond_index,
+                         S->bond_end, S->bonds, (const int *)a, b, flags);
+      std::swap(a, b);  // `a` now holds the latest sweep
+    }
+    hipLaunchKernelGGL(k_mark_finish, dim3(1), dim3(1), 0, S->stream, flags, max_sweeps - 1, S->status);
+    if (a != S->bond_mark) std::swap(S->bond_mark, S->bond_mark2);
+  }
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}

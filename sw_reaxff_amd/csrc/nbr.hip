@@ -1,0 +1,361 @@
+// nbr.hip — ghost images, cell binning and the full Verlet list (a1).
+//
+// The list replaces npair_full_bin_ghost_sunway_build_packed_para
+// (npair_full_bin_ghost_sw5.c:80-226): for every atom i, local or ghost, all
+// j != i with |x_i - x_j|^2 <= cutneigh^2, tested in FP64 with the
+// reference's expression (del = x_i - x_j; rsq = delx^2+dely^2+delz^2,
+// :183-193).  The set is the mathematically complete one: ghosts see every
+// atom that exists within the cutoff (no bin-0 exclusion, :151).
+//
+// Design (MI355X): atoms are counting-sorted into bins of about cutneigh/2,
+// each bin sorted by atom index so that the list order is deterministic.
+// One wave owns one atom i and walks the stencil as (dy,dz) rows; along x
+// the bins of a row are contiguous in the bin-sorted arrays, so each row is
+// one contiguous segment that the 64 lanes stream through.  Hits are
+// compacted with a wave ballot.  Pass 1 counts, a scan gives CSR offsets,
+// pass 2 writes.
+#include "swr_internal.h"
+#include "swr_device.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace {
+
+constexpr int NB_THREADS = 256;
+constexpr int WAVES_PER_BLOCK = NB_THREADS / SWR_WAVE;
+
+// ---------------------------------------------------------------- borders
+// Comm::borders with one process: for dimension d, atoms [0,nlast) with
+// x <= lo+cut are imaged at +L, then atoms with x >= hi-cut at -L.
+__global__ void k_border_flag(const double *__restrict__ x, int nlast, int d, double thr, int side,
+                              int *__restrict__ flag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nlast) return;
+  double v = x[3 * i + d];
+  flag[i] = side == 0 ? (v <= thr) : (v >= thr);
+}
+
+__global__ void k_border_scatter(double *__restrict__ x, int *__restrict__ type, int *__restrict__ tag,
+                                 int *__restrict__ owner, signed char *__restrict__ img,
+                                 int *__restrict__ num_bonds, int *__restrict__ num_hbonds,
+                                 const int *__restrict__ flag, const int *__restrict__ pos, int nlast, int ncur,
+                                 int d, double shift, int sgn) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nlast || !flag[i]) return;
+  int g = ncur + pos[i];
+  for (int k = 0; k < 3; k++) x[3 * g + k] = x[3 * i + k];
+  x[3 * g + d] = x[3 * i + d] + shift;
+  type[g] = type[i];
+  tag[g] = tag[i];
+  owner[g] = owner[i];
+  for (int k = 0; k < 3; k++) img[3 * g + k] = img[3 * i + k];
+  img[3 * g + d] = (signed char)(img[3 * i + d] + sgn);
+  num_bonds[g] = num_bonds[owner[i]];
+  num_hbonds[g] = num_hbonds[owner[i]];
+}
+
+// Forward comm of positions: ghost = owner + image shift (pbc add of +-L).
+__global__ void k_forward_x(double *__restrict__ x, const int *__restrict__ owner,
+                            const signed char *__restrict__ img, int n, int N, double Lx, double Ly, double Lz) {
+  int g = n + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= N) return;
+  int o = owner[g];
+  const double L[3] = {Lx, Ly, Lz};
+  for (int d = 0; d < 3; d++) {
+    double v = x[3 * o + d];
+    int s = img[3 * g + d];
+    // Images of images add one box length per crossing, as LAMMPS does.
+    for (int k = 0; k < s; k++) v = v + L[d];
+    for (int k = 0; k > s; k--) v = v - L[d];
+    x[3 * g + d] = v;
+  }
+}
+
+// Ghost refresh of capacities on re-neighbour steps (forward_comm_fix of
+// fix REAXC, pair_reaxc_sunway.cpp:551).
+__global__ void k_forward_caps(int *__restrict__ num_bonds, int *__restrict__ num_hbonds,
+                               const int *__restrict__ owner, int n, int N) {
+  int g = n + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= N) return;
+  num_bonds[g] = num_bonds[owner[g]];
+  num_hbonds[g] = num_hbonds[owner[g]];
+}
+
+// NVE with the forces this path does not compute left at zero: x += dt v.
+__global__ void k_nve(double *__restrict__ x, const double *__restrict__ v, int n, double dt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * n) return;
+  x[i] = x[i] + dt * v[i];
+}
+
+// Domain::pbc remap of local atoms into [lo, hi).
+__global__ void k_pbc(double *__restrict__ x, int n, double lox, double loy, double loz, double hix,
+                      double hiy, double hiz) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double lo[3] = {lox, loy, loz}, hi[3] = {hix, hiy, hiz};
+  for (int d = 0; d < 3; d++) {
+    double v = x[3 * i + d];
+    double L = hi[d] - lo[d];
+    if (v < lo[d]) v = v + L;
+    if (v >= hi[d]) { v = v - L; v = v > lo[d] ? v : lo[d]; }
+    x[3 * i + d] = v;
+  }
+}
+
+__global__ void k_init_local_maps(int *__restrict__ owner, signed char *__restrict__ img, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  owner[i] = i;
+  img[3 * i] = img[3 * i + 1] = img[3 * i + 2] = 0;
+}
+
+// ---------------------------------------------------------------- binning
+__device__ __forceinline__ int coord_bin(double v, double lo, double inv, int nb, int *out_of_range) {
+  double f = (v - lo) * inv;
+  int b = (int)floor(f);
+  if (b < 0) { if (b < -1 || f < -0.5) *out_of_range = 1; b = 0; }
+  if (b >= nb) { if (b > nb || f > nb + 0.5) *out_of_range = 1; b = nb - 1; }
+  return b;
+}
+
+__global__ void k_bin_assign(const double *__restrict__ x, int N, double lx, double ly, double lz, double ix,
+                             double iy, double iz, int nx, int ny, int nz, int *__restrict__ bin_of,
+                             int *__restrict__ bin_count, DevStatus *st) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int oor = 0;
+  int bx = coord_bin(x[3 * i], lx, ix, nx, &oor);
+  int by = coord_bin(x[3 * i + 1], ly, iy, ny, &oor);
+  int bz = coord_bin(x[3 * i + 2], lz, iz, nz, &oor);
+  if (oor) atomicOr(&st->flags, SWR_F_BIN);
+  int b = (bz * ny + by) * nx + bx;
+  bin_of[i] = b;
+  atomicAdd(&bin_count[b], 1);
+}
+
+__global__ void k_bin_scatter(const int *__restrict__ bin_of, int N, const int *__restrict__ bin_start,
+                              int *__restrict__ bin_fill, int *__restrict__ bp_id) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int b = bin_of[i];
+  int slot = bin_start[b] + atomicAdd(&bin_fill[b], 1);
+  bp_id[slot] = i;
+}
+
+// Sort each bin's atoms by index (bins hold ~20 atoms) and gather positions.
+__global__ void k_bin_sort(int nbins, const int *__restrict__ bin_start, int *__restrict__ bp_id,
+                           const double *__restrict__ x, double4 *__restrict__ bp_pos) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  int s = bin_start[b], e = bin_start[b + 1];
+  for (int k = s + 1; k < e; k++) {
+    int v = bp_id[k];
+    int m = k - 1;
+    while (m >= s && bp_id[m] > v) { bp_id[m + 1] = bp_id[m]; m--; }
+    bp_id[m + 1] = v;
+  }
+  for (int k = s; k < e; k++) {
+    int j = bp_id[k];
+    bp_pos[k] = make_double4(x[3 * j], x[3 * j + 1], x[3 * j + 2], 0.0);
+  }
+}
+
+// ---------------------------------------------------------------- Verlet
+template <bool WRITE>
+__global__ void __launch_bounds__(NB_THREADS)
+k_verlet(const double *__restrict__ x, int N, const int *__restrict__ bin_of, const int *__restrict__ bin_start,
+         const double4 *__restrict__ bp_pos, const int *__restrict__ bp_id, const int4 *__restrict__ rows, int nrows,
+         int nx, int ny, int nz, double cutsq, int *__restrict__ vcount, const long long *__restrict__ voff,
+         int *__restrict__ vlist) {
+  const int nblk = gridDim.x;
+  const int blk = xcd_block(nblk);
+  const int i = blk * WAVES_PER_BLOCK + wave_in_block();
+  if (i >= N) return;
+  const int lane = lane_id();
+  const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+  const int b = bin_of[i];
+  const int bx = b % nx, by = (b / nx) % ny, bz = b / (nx * ny);
+  long long out = WRITE ? voff[i] : 0;
+  int cnt = 0;
+  for (int r = 0; r < nrows; r++) {
+    const int4 rw = rows[r];
+    const int cy = by + rw.x, cz = bz + rw.y;
+    if (cy < 0 || cy >= ny || cz < 0 || cz >= nz) continue;
+    const int cx0 = max(bx - rw.z, 0), cx1 = min(bx + rw.z, nx - 1);
+    const int rowbase = (cz * ny + cy) * nx;
+    const int s = bin_start[rowbase + cx0], e = bin_start[rowbase + cx1 + 1];
+    for (int base = s; base < e; base += SWR_WAVE) {
+      const int k = base + lane;
+      bool hit = false;
+      int j = -1;
+      if (k < e) {
+        j = bp_id[k];
+        const double4 p = bp_pos[k];
+        const double dx = xi - p.x, dy = yi - p.y, dz = zi - p.z;
+        const double rsq = dx * dx + dy * dy + dz * dz;
+        hit = (j != i) && (rsq <= cutsq);
+      }
+      const unsigned long long m = __ballot(hit);
+      if (WRITE && hit) vlist[out + cnt + ballot_rank(m)] = j;
+      cnt += __popcll(m);
+    }
+  }
+  if (!WRITE && lane == 0) vcount[i] = cnt;
+}
+
+}  // namespace
+
+// Stencil rows for bins of edge bs[] and cutoff cut: every (dy,dz) whose
+// nearest bin distance is within cut, with the x half-width sx reaching it.
+static std::vector<int4> stencil_rows(const double bs[3], double cut) {
+  std::vector<int4> rows;
+  const int sy = (int)std::ceil(cut / bs[1]), sz = (int)std::ceil(cut / bs[2]);
+  const int sxm = (int)std::ceil(cut / bs[0]);
+  const double c2 = cut * cut;
+  for (int dz = -sz; dz <= sz; dz++)
+    for (int dy = -sy; dy <= sy; dy++) {
+      double ey = std::max(std::abs(dy) - 1, 0) * bs[1], ez = std::max(std::abs(dz) - 1, 0) * bs[2];
+      double rem = c2 - ey * ey - ez * ez;
+      if (rem < 0) continue;
+      int sx = std::min(sxm, (int)std::floor(std::sqrt(rem) / bs[0]) + 1);
+      rows.push_back(make_int4(dy, dz, sx, 0));
+    }
+  return rows;
+}
+
+// ---------------------------------------------------------------- host API
+int swr_nbr_set_grid(swr_system *S, const double lo[3], const double hi[3], double cutghost, double cutneigh) {
+  // Grid covering [lo - cutghost, hi + cutghost] with bins of ~cutneigh/2.
+  const double target = 0.5 * cutneigh;
+  for (int d = 0; d < 3; d++) {
+    double a = lo[d] - cutghost - 1e-6, b = hi[d] + cutghost + 1e-6;
+    int nb = std::max(1, (int)std::floor((b - a) / target));
+    S->binlo[d] = a;
+    S->binsize[d] = (b - a) / nb;
+    S->bininv[d] = 1.0 / S->binsize[d];
+    S->nbin[d] = nb;
+  }
+  S->cutneigh = cutneigh;
+  int nb = S->nbin[0] * S->nbin[1] * S->nbin[2];
+  if (nb + 1 > S->binscap) {
+    int rc;
+    if ((rc = swr_alloc(&S->bin_count, nb + 1)) || (rc = swr_alloc(&S->bin_start, nb + 2)) ||
+        (rc = swr_alloc(&S->bin_fill, nb + 1)))
+      return rc;
+    S->binscap = nb + 1;
+  }
+  S->nbins = nb;
+  std::vector<int4> rows = stencil_rows(S->binsize, cutneigh);
+  S->nrows = (int)rows.size();
+  int rc = swr_alloc(&S->stencil_rows, rows.size());
+  if (rc) return rc;
+  HIP_CHECK(hipMemcpy(S->stencil_rows, rows.data(), rows.size() * sizeof(int4), hipMemcpyHostToDevice));
+  return SWR_OK;
+}
+
+int swr_nbr_ensure_atoms(swr_system *S, int Ncap);
+
+// Builds ghost images of the n local atoms (already remapped into the box).
+int swr_nbr_borders(swr_system *S, const double lo[3], const double hi[3], double cutghost) {
+  const int n = S->n;
+  int ncur = n;
+  hipLaunchKernelGGL(k_init_local_maps, dim3(div_up(n, 256)), dim3(256), 0, S->stream, S->owner, S->img, n);
+  for (int d = 0; d < 3; d++) {
+    const int nlast = ncur;
+    for (int side = 0; side < 2; side++) {
+      int rc = swr_ensure_tmp(S, sizeof(int) * (2 * (size_t)nlast + 2));
+      if (rc) return rc;
+      int *flag = S->tmp_i, *pos = S->tmp_i + nlast + 1;
+      double thr = side == 0 ? lo[d] + cutghost : hi[d] - cutghost;
+      hipLaunchKernelGGL(k_border_flag, dim3(div_up(nlast, 256)), dim3(256), 0, S->stream, S->x, nlast, d, thr,
+                         side, flag);
+      rc = swr_scan_i32(S, flag, pos, nlast, pos + nlast);
+      if (rc) return rc;
+      int add = 0;
+      HIP_CHECK(hipMemcpyAsync(&add, pos + nlast, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+      HIP_CHECK(hipStreamSynchronize(S->stream));
+      if (ncur + add > S->Ncap) {
+        if ((rc = swr_nbr_ensure_atoms(S, (int)((ncur + add) * 1.3) + 64))) return rc;
+        flag = S->tmp_i;
+        pos = S->tmp_i + nlast + 1;
+      }
+      double L = hi[d] - lo[d];
+      hipLaunchKernelGGL(k_border_scatter, dim3(div_up(nlast, 256)), dim3(256), 0, S->stream, S->x, S->type,
+                         S->tag, S->owner, S->img, S->num_bonds, S->num_hbonds, flag, pos, nlast, ncur, d,
+                         side == 0 ? L : -L, side == 0 ? 1 : -1);
+      ncur += add;
+    }
+  }
+  S->N = ncur;
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_nbr_forward_x(swr_system *S, const double lo[3], const double hi[3]) {
+  if (S->N > S->n)
+    hipLaunchKernelGGL(k_forward_x, dim3(div_up(S->N - S->n, 256)), dim3(256), 0, S->stream, S->x, S->owner,
+                       S->img, S->n, S->N, hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_nbr_forward_caps(swr_system *S) {
+  if (S->N > S->n)
+    hipLaunchKernelGGL(k_forward_caps, dim3(div_up(S->N - S->n, 256)), dim3(256), 0, S->stream, S->num_bonds,
+                       S->num_hbonds, S->owner, S->n, S->N);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_nbr_nve(swr_system *S, double dt) {
+  hipLaunchKernelGGL(k_nve, dim3(div_up(3 * S->n, 256)), dim3(256), 0, S->stream, S->x, S->v, S->n, dt);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_nbr_pbc(swr_system *S, const double lo[3], const double hi[3]) {
+  hipLaunchKernelGGL(k_pbc, dim3(div_up(S->n, 256)), dim3(256), 0, S->stream, S->x, S->n, lo[0], lo[1], lo[2],
+                     hi[0], hi[1], hi[2]);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+// Bin all N atoms and build the full Verlet list (CSR: voff[N+1], vlist).
+int swr_nbr_build(swr_system *S) {
+  const int N = S->N;
+  int rc;
+  if ((rc = swr_alloc(&S->bin_of, N)) || (rc = swr_alloc(&S->bp_id, N)) || (rc = swr_alloc(&S->bp_pos, N)))
+    return rc;
+  HIP_CHECK(hipMemsetAsync(S->bin_count, 0, sizeof(int) * (S->nbins + 1), S->stream));
+  HIP_CHECK(hipMemsetAsync(S->bin_fill, 0, sizeof(int) * (S->nbins + 1), S->stream));
+  hipLaunchKernelGGL(k_bin_assign, dim3(div_up(N, 256)), dim3(256), 0, S->stream, S->x, N, S->binlo[0], S->binlo[1],
+                     S->binlo[2], S->bininv[0], S->bininv[1], S->bininv[2], S->nbin[0], S->nbin[1], S->nbin[2],
+                     S->bin_of, S->bin_count, S->status);
+  if ((rc = swr_scan_i32(S, S->bin_count, S->bin_start, S->nbins + 1, nullptr))) return rc;
+  hipLaunchKernelGGL(k_bin_scatter, dim3(div_up(N, 256)), dim3(256), 0, S->stream, S->bin_of, N, S->bin_start,
+                     S->bin_fill, S->bp_id);
+  hipLaunchKernelGGL(k_bin_sort, dim3(div_up(S->nbins, 128)), dim3(128), 0, S->stream, S->nbins, S->bin_start,
+                     S->bp_id, S->x, S->bp_pos);
+  if ((rc = swr_alloc(&S->vcount, N)) || (rc = swr_alloc(&S->voff, (size_t)N + 1))) return rc;
+  const double cutsq = S->cutneigh * S->cutneigh;
+  const int nblk = div_up(N, WAVES_PER_BLOCK);
+  hipLaunchKernelGGL(k_verlet<false>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->x, N, S->bin_of, S->bin_start,
+                     S->bp_pos, S->bp_id, S->stencil_rows, S->nrows, S->nbin[0], S->nbin[1], S->nbin[2], cutsq,
+                     S->vcount, (const long long *)nullptr, (int *)nullptr);
+  long long total = 0;
+  if ((rc = swr_scan_i32_to_i64(S, S->vcount, S->voff, N, &total))) return rc;
+  if (total > S->vcap) {
+    long long cap = (long long)(total * 1.15) + 1024;
+    if ((rc = swr_alloc(&S->vlist, (size_t)cap))) return rc;
+    if ((rc = swr_alloc(&S->far, (size_t)cap))) return rc;
+    S->vcap = cap;
+  }
+  S->vtotal = total;
+  hipLaunchKernelGGL(k_verlet<true>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->x, N, S->bin_of, S->bin_start,
+                     S->bp_pos, S->bp_id, S->stencil_rows, S->nrows, S->nbin[0], S->nbin[1], S->nbin[2], cutsq,
+                     S->vcount, (const long long *)S->voff, S->vlist);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}

@@ -1,0 +1,463 @@
+// qeq.hip — QEq: H matrix (a11-a13), SpMV (a14), the s/t pipelined PCG
+// (a15) and the charge update (a16).
+//
+// H (compute_H_Full_C_para, fix_qeq_reax_sw64.c:768-911): full rows for the
+// local atoms, every Verlet neighbour with r^2 <= swb^2,
+//   val = Tap(r) * 14.4 / (r^3 + shld[ti][tj])^0.3333333333333
+// with Tap by Horner (calculate_H, fix_qeq_reax_sunway.cpp:965-981).  Row i
+// holds jlist/val[firstnbr[i] .. firstnbr[i]+numnbrs[i]); firstnbr[i] is the
+// row's Verlet offset (the reference uses a 600*i stride,
+// fix_qeq_reax_sw64.c:828).  Ghost columns are folded onto their owning
+// local atom: on one domain the forward comm of x makes x[ghost] ==
+// x[owner], so the SpMV result is unchanged and x needs only n entries.
+//
+// CG: CG_v2 (fix_qeq_reax_sunway.cpp:983-1167) for s and t at once.  Vectors
+// are (s,t) pairs (double2), so each H entry is read once and feeds both
+// solves.  One kernel per iteration does the reference's B and C steps of
+// iteration i (SpMV of d, x/p/ss/v/z updates) and the A step of iteration
+// i+1 (r/u/w updates, the two dots, d = M^-1 w); the two dots are reduced in
+// a fixed order by the last workgroup to finish, which also computes the
+// next iteration's scalars.  Each solve stops by the reference's test
+// sqrt(u.r)/|b| <= tol (or i = imax) and is frozen while the other runs.
+#include "swr_internal.h"
+#include "swr_device.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace {
+
+constexpr int QT = 256;                    // threads per block
+constexpr int QW = QT / SWR_WAVE;          // waves per block
+constexpr int ROWS_PER_WAVE = 16;
+constexpr int RPB = QW * ROWS_PER_WAVE;    // rows per block (fixed: results do not depend on grid size)
+constexpr double EV_TO_KCAL_PER_MOL = 14.4;
+
+// ---------------------------------------------------------------- H build
+__global__ void __launch_bounds__(QT)
+k_build_H(int n, const DevParams *__restrict__ P, const double *__restrict__ x, const int *__restrict__ type,
+          const int *__restrict__ owner, const long long *__restrict__ voff, const int *__restrict__ vlist,
+          int *__restrict__ firstnbr, int *__restrict__ numnbrs, int *__restrict__ jlist, double *__restrict__ val,
+          float *__restrict__ val32, double *__restrict__ Hdia_inv, double2 *__restrict__ b2,
+          double2 *__restrict__ st, const double *__restrict__ s_hist, const double *__restrict__ t_hist) {
+  const int i = xcd_block(gridDim.x) * QW + wave_in_block();
+  if (i >= n) return;
+  const int lane = lane_id();
+  const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+  const int ti = type[i];
+  const int nt1 = SWR_MAX_TYPES + 1;
+  const double swbsq = P->swbsq;
+  double T[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) T[k] = P->Tap[k];
+  const long long beg = voff[i], end = voff[i + 1];
+  int cnt = 0;
+  for (long long base = beg; base < end; base += SWR_WAVE) {
+    const long long k = base + lane;
+    bool hit = false;
+    int jo = 0;
+    double hv = 0.0;
+    if (k < end) {
+      const int j = vlist[k];
+      const double dx = x[3 * j] - xi, dy = x[3 * j + 1] - yi, dz = x[3 * j + 2] - zi;
+      const double r_sqr = dx * dx + dy * dy + dz * dz;
+      if (r_sqr <= swbsq) {
+        hit = true;
+        const double r = sqrt(r_sqr);
+        const double gamma = P->shld[ti * nt1 + type[j]];
+        double Taper = T[7] * r + T[6];
+        Taper = Taper * r + T[5];
+        Taper = Taper * r + T[4];
+        Taper = Taper * r + T[3];
+        Taper = Taper * r + T[2];
+        Taper = Taper * r + T[1];
+        Taper = Taper * r + T[0];
+        double denom = r * r * r + gamma;
+        denom = pow(denom, 0.3333333333333);
+        hv = Taper * EV_TO_KCAL_PER_MOL / denom;
+        jo = owner[j];
+      }
+    }
+    const unsigned long long m = __ballot(hit);
+    if (hit) {
+      const long long slot = beg + cnt + ballot_rank(m);
+      jlist[slot] = jo;
+      val[slot] = hv;
+      if (val32) val32[slot] = (float)hv;
+    }
+    cnt += __popcll(m);
+  }
+  if (lane == 0) {
+    firstnbr[i] = (int)beg;
+    numnbrs[i] = cnt;
+    // init_matvec vectors (fix_qeq_reax_sunway.cpp:700-716)
+    Hdia_inv[i] = 1. / P->eta[ti];
+    b2[i] = make_double2(-P->chi[ti], -1.0);
+    const double *sh = s_hist + 5 * (size_t)i, *th = t_hist + 5 * (size_t)i;
+    const double tv = th[2] + 3 * (th[0] - th[1]);
+    const double sv = 4 * (sh[0] + sh[2]) - (6 * sh[1] + sh[3]);
+    st[i] = make_double2(sv, tv);
+  }
+}
+
+// ---------------------------------------------------------------- SpMV
+// One wave per row; the row's entries are streamed by the 64 lanes and the
+// products summed in a fixed tree order.
+__device__ __forceinline__ double2 row_spmv2(int i, const int *__restrict__ firstnbr, const int *__restrict__ numnbrs,
+                                             const int *__restrict__ jlist, const double *__restrict__ val,
+                                             const double2 *__restrict__ xv) {
+  const int lane = lane_id();
+  const int beg = firstnbr[i], end = beg + numnbrs[i];
+  double a = 0.0, b = 0.0;
+  for (int k = beg + lane; k < end; k += SWR_WAVE) {
+    const double v = val[k];
+    const double2 xx = xv[jlist[k]];
+    a += v * xx.x;
+    b += v * xx.y;
+  }
+  return make_double2(wave_sum(a), wave_sum(b));
+}
+
+__global__ void __launch_bounds__(QT)
+k_spmv1(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
+        const int *__restrict__ numnbrs, const int *__restrict__ jlist, const double *__restrict__ val,
+        const double *__restrict__ xv, double *__restrict__ b) {
+  const int i = xcd_block(gridDim.x) * QW + wave_in_block();
+  if (i >= n) return;
+  const int lane = lane_id();
+  const int beg = firstnbr[i], end = beg + numnbrs[i];
+  double a = 0.0;
+  for (int k = beg + lane; k < end; k += SWR_WAVE) a += val[k] * xv[jlist[k]];
+  a = wave_sum(a);
+  if (lane == 0) b[i] = P->eta[type[i]] * xv[i] + a;
+}
+
+// ---------------------------------------------------------------- CG
+enum { PH_S1 = 0, PH_S2, PH_S3, PH_A1, PH_K };
+
+struct CgVecs {
+  double2 *x, *r, *u, *w, *z, *ss, *v, *p, *m, *d[2];
+  const double2 *b;
+  const double *Hd;
+};
+
+// Scalars after the A step of an iteration reduced dots l0 = u.r, l1 = u.w
+// (fix_qeq_reax_sunway.cpp:1133-1141 and :1152).
+__device__ void cg_after_A(CgScalars &c, double l0, double l1, int imax, double tol) {
+  const double beta = l0 / c.sig_old;
+  c.heta = l1 - beta * beta * c.heta;
+  c.beta = beta;
+  c.alpha_x = c.alpha_a;
+  c.alpha_a = l0 / c.heta;
+  c.sig_old = l0;
+  c.dg0 = l0;
+  c.dg1 = l1;
+  c.iter += 1;
+  c.active_b = 1;
+  c.active_a = (c.iter < imax && sqrt(l0) / c.b_norm > tol) ? 1 : 0;
+}
+
+template <int PH>
+__global__ void __launch_bounds__(QT)
+k_cg(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
+     const int *__restrict__ numnbrs, const int *__restrict__ jlist, const double *__restrict__ val, CgVecs V,
+     int par_in, CgScalars *__restrict__ cgs, double *__restrict__ partials, unsigned int *__restrict__ counter,
+     int imax, double tol, DevStatus *st) {
+  __shared__ double sred[QW][6];
+  __shared__ int s_last;
+  CgScalars c[2];
+  bool act_b[2] = {true, true}, act_a[2] = {true, true};
+  if (PH == PH_A1 || PH == PH_K) {
+    c[0] = cgs[0];
+    c[1] = cgs[1];
+    for (int r = 0; r < 2; r++) {
+      act_b[r] = PH == PH_K ? c[r].active_b != 0 : false;
+      act_a[r] = c[r].active_a != 0;
+    }
+    if (!act_b[0] && !act_b[1] && !act_a[0] && !act_a[1]) return;  // both solves finished
+  }
+  const int w = wave_in_block(), lane = lane_id();
+  const int blk = blockIdx.x;
+  const int row0 = blk * RPB + w * ROWS_PER_WAVE;
+  // SpMV of this wave's rows; lane t keeps row t's result
+  double2 qrow = make_double2(0.0, 0.0);
+  if (PH != PH_A1) {
+    const double2 *src = PH == PH_S1 ? V.x : PH == PH_S2 ? V.d[0] : PH == PH_S3 ? V.d[1] : V.d[par_in];
+    for (int t = 0; t < ROWS_PER_WAVE; t++) {
+      const int i = row0 + t;
+      if (i >= n) break;
+      double2 s = row_spmv2(i, firstnbr, numnbrs, jlist, val, src);
+      const double eta = P->eta[type[i]];
+      const double2 xi = src[i];
+      // b_i = eta_i x_i + sum_j H_ij x_j (sparse_matvec_C_para, :424-466)
+      s.x = eta * xi.x + s.x;
+      s.y = eta * xi.y + s.y;
+      if (lane == t) qrow = s;
+    }
+  }
+  // element-wise part: lane t owns row row0+t
+  double dots[6] = {0, 0, 0, 0, 0, 0};
+  const int i = row0 + lane;
+  if (lane < ROWS_PER_WAVE && i < n) {
+    const double Hd = V.Hd[i];
+    double2 q = qrow;
+    if (PH == PH_S1) {
+      const double2 b = V.b[i];
+      double2 r = make_double2(b.x - 1 * q.x, b.y - 1 * q.y);
+      double2 u = make_double2(r.x * Hd, r.y * Hd);
+      V.r[i] = r; V.u[i] = u; V.d[0][i] = u;
+    } else if (PH == PH_S2) {
+      V.w[i] = q;
+      double2 m = make_double2(q.x * Hd, q.y * Hd);
+      V.m[i] = m; V.d[1][i] = m;
+    } else if (PH == PH_S3) {
+      const double2 b = V.b[i], u = V.u[i], r = V.r[i], ww = V.w[i], m = V.m[i];
+      V.z[i] = q; V.p[i] = u; V.ss[i] = ww; V.v[i] = m;
+      dots[0] = b.x * b.x; dots[1] = u.x * r.x; dots[2] = u.x * ww.x;
+      dots[3] = b.y * b.y; dots[4] = u.y * r.y; dots[5] = u.y * ww.y;
+    } else {
+      double *qq = (double *)&q;
+      double2 xx = V.x[i], p = V.p[i], u = V.u[i], ss = V.ss[i], w2 = V.w[i], v = V.v[i], z = V.z[i], r = V.r[i];
+      double *px = (double *)&xx, *pp = (double *)&p, *pu = (double *)&u, *pss = (double *)&ss,
+             *pw = (double *)&w2, *pv = (double *)&v, *pz = (double *)&z, *pr = (double *)&r;
+      double2 dcur = PH == PH_K ? V.d[par_in][i] : make_double2(0, 0);
+      double2 dnext = make_double2(0, 0);
+      double *pdc = (double *)&dcur, *pdn = (double *)&dnext;
+      for (int s = 0; s < 2; s++) {
+        if (PH == PH_K && act_b[s]) {
+          // B: x += alpha p; p = u + p beta; ss = w + ss beta; v = d + v beta   (:1135-1141)
+          const double al = c[s].alpha_x, be = c[s].beta;
+          px[s] += al * pp[s];
+          pp[s] = pu[s] + pp[s] * be;
+          pss[s] = pw[s] + pss[s] * be;
+          pv[s] = pdc[s] + pv[s] * be;
+          // C: z = q + z beta   (:1147-1150)
+          pz[s] = qq[s] + pz[s] * be;
+        }
+        if (act_a[s]) {
+          // A of the next iteration: r -= alpha ss; u -= alpha v; w -= alpha z (:1113-1123)
+          const double al = c[s].alpha_a;
+          pr[s] -= al * pss[s];
+          pu[s] -= al * pv[s];
+          pw[s] -= al * pz[s];
+          dots[2 * s] = pu[s] * pr[s];
+          dots[2 * s + 1] = pu[s] * pw[s];
+          pdn[s] = pw[s] * Hd;
+        } else {
+          pdn[s] = pdc[s];
+        }
+      }
+      if (PH == PH_K && (act_b[0] || act_b[1])) {
+        V.x[i] = xx; V.p[i] = p; V.ss[i] = ss; V.v[i] = v; V.z[i] = z;
+      }
+      if (act_a[0] || act_a[1]) {
+        V.r[i] = r; V.u[i] = u; V.w[i] = w2;
+        V.d[PH == PH_A1 ? 0 : (par_in ^ 1)][i] = dnext;
+      }
+    }
+  }
+  if (PH == PH_S1 || PH == PH_S2) return;
+  // ---- deterministic reduction: lanes -> wave -> block -> last block
+  const int nd = PH == PH_S3 ? 6 : 4;
+  for (int k = 0; k < nd; k++) {
+    double s = wave_sum(dots[k]);
+    if (lane == 0) sred[w][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < nd; k++) {
+      double s = 0.0;
+      for (int ww = 0; ww < QW; ww++) s += sred[ww][k];
+      partials[(size_t)blk * 8 + k] = s;
+    }
+    __threadfence();
+    const unsigned int prev = atomicAdd(counter, 1u);
+    s_last = (prev == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last || threadIdx.x != 0) return;
+  __threadfence();
+  double tot[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < (int)gridDim.x; b++)
+    for (int k = 0; k < nd; k++) tot[k] += ((volatile double *)partials)[(size_t)b * 8 + k];
+  *counter = 0u;
+  unsigned int bad = 0;
+  if (PH == PH_S3) {
+    for (int s = 0; s < 2; s++) {
+      CgScalars cc = cgs[s];
+      cc.bdot = tot[3 * s]; cc.udot_r = tot[3 * s + 1]; cc.udot_w = tot[3 * s + 2];
+      cc.b_norm = sqrt(tot[3 * s]);
+      cc.sig_old = tot[3 * s + 1];
+      cc.heta = tot[3 * s + 2];
+      cc.alpha_a = cc.sig_old / cc.heta;
+      cc.alpha_x = 0.0;
+      cc.beta = 0.0;
+      cc.dg0 = cc.sig_old;
+      cc.dg1 = 0.0;
+      cc.iter = 1;
+      cc.active_b = 0;
+      cc.active_a = (1 < imax && sqrt(cc.dg0) / cc.b_norm > tol) ? 1 : 0;
+      cc.done = cc.active_a ? 0 : 1;
+      if (cc.done) st->cg_iters[s] = cc.iter;
+      if (!isfinite(cc.alpha_a) && cc.active_a) bad = 1;
+      cgs[s] = cc;
+    }
+  } else {
+    for (int s = 0; s < 2; s++) {
+      CgScalars cc = c[s];
+      if (act_a[s]) {
+        cg_after_A(cc, tot[2 * s], tot[2 * s + 1], imax, tol);
+        if (!isfinite(cc.alpha_a) || !isfinite(cc.beta)) bad = 1;
+      } else if (act_b[s]) {
+        cc.active_b = 0;
+        cc.done = 1;
+        st->cg_iters[s] = cc.iter;
+      }
+      cgs[s] = cc;
+    }
+  }
+  if (bad) atomicOr(&st->flags, SWR_F_NAN);
+}
+
+// ---------------------------------------------------------------- calculate_Q
+__global__ void __launch_bounds__(QT)
+k_q_sums(int n, const double2 *__restrict__ st, double *__restrict__ partials, unsigned int *__restrict__ counter,
+         double *__restrict__ qsum) {
+  __shared__ double sred[QW][2];
+  __shared__ int s_last;
+  const int w = wave_in_block(), lane = lane_id();
+  const int row0 = blockIdx.x * RPB + w * ROWS_PER_WAVE;
+  double a = 0.0, b = 0.0;
+  const int i = row0 + lane;
+  if (lane < ROWS_PER_WAVE && i < n) { a = st[i].x; b = st[i].y; }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if (lane == 0) { sred[w][0] = a; sred[w][1] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0, s1 = 0;
+    for (int k = 0; k < QW; k++) { s0 += sred[k][0]; s1 += sred[k][1]; }
+    partials[(size_t)blockIdx.x * 8] = s0;
+    partials[(size_t)blockIdx.x * 8 + 1] = s1;
+    __threadfence();
+    s_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!s_last || threadIdx.x != 0) return;
+  __threadfence();
+  double s0 = 0, s1 = 0;
+  for (int k = 0; k < (int)gridDim.x; k++) {
+    s0 += ((volatile double *)partials)[(size_t)k * 8];
+    s1 += ((volatile double *)partials)[(size_t)k * 8 + 1];
+  }
+  *counter = 0u;
+  qsum[0] = s0;
+  qsum[1] = s1;
+  qsum[2] = s0 / s1;
+}
+
+// q = s - u t for local atoms, history shift, and the ghost q refresh
+// (calculate_Q :1736-1754).
+__global__ void k_q_apply(int n, int N, const double2 *__restrict__ st, const double *__restrict__ qsum,
+                          const int *__restrict__ owner, double *__restrict__ q, double *__restrict__ s_hist,
+                          double *__restrict__ t_hist) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double u = qsum[2];
+  const int o = i < n ? i : owner[i];
+  const double2 v = st[o];
+  q[i] = v.x - u * v.y;
+  if (i < n) {
+    double *sh = s_hist + 5 * (size_t)i, *th = t_hist + 5 * (size_t)i;
+    for (int k = 4; k > 0; --k) { sh[k] = sh[k - 1]; th[k] = th[k - 1]; }
+    sh[0] = v.x;
+    th[0] = v.y;
+  }
+}
+
+CgVecs make_vecs(swr_system *S) {
+  CgVecs V;
+  double2 *base = (double2 *)S->cgv;
+  const size_t c = S->ncap;
+  V.r = base + CGV_R * c; V.u = base + CGV_U * c; V.w = base + CGV_W * c; V.z = base + CGV_Z * c;
+  V.ss = base + CGV_SS * c; V.v = base + CGV_V * c; V.p = base + CGV_P * c; V.m = base + CGV_M * c;
+  V.d[0] = base + CGV_D0 * c; V.d[1] = base + CGV_D1 * c;
+  V.x = (double2 *)S->st;
+  V.b = (const double2 *)S->b_s;
+  V.Hd = S->Hdia_inv;
+  return V;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+int swr_qeq_build_H(swr_system *S) {
+  const int n = S->n;
+  hipLaunchKernelGGL(k_build_H, dim3(div_up(n, QW)), dim3(QT), 0, S->stream, n, S->dp, S->x, S->type, S->owner,
+                     S->voff, S->vlist, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val,
+                     S->mixed ? S->H_val32 : (float *)nullptr, S->Hdia_inv, (double2 *)S->b_s, (double2 *)S->st,
+                     S->s_hist, S->t_hist);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_qeq_spmv(swr_system *S, const double *xv, double *b) {
+  const int n = S->n;
+  hipLaunchKernelGGL(k_spmv1, dim3(div_up(n, QW)), dim3(QT), 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
+                     S->H_numnbrs, S->H_jlist, S->H_val, xv, b);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+static int launch_cg(swr_system *S, int ph, int par_in) {
+  const int n = S->n;
+  const int nblk = div_up(n, RPB);
+  CgVecs V = make_vecs(S);
+  const double tol = S->ctl.tol;
+#define CG_ARGS n, S->dp, S->type, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, V, par_in, S->cgs, \
+                S->partials, S->counters, S->imax, tol, S->status
+  switch (ph) {
+    case PH_S1: hipLaunchKernelGGL(k_cg<PH_S1>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
+    case PH_S2: hipLaunchKernelGGL(k_cg<PH_S2>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
+    case PH_S3: hipLaunchKernelGGL(k_cg<PH_S3>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
+    case PH_A1: hipLaunchKernelGGL(k_cg<PH_A1>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
+    default: hipLaunchKernelGGL(k_cg<PH_K>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
+  }
+#undef CG_ARGS
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+// Runs both solves to convergence.  Iteration kernels are enqueued in
+// batches; between batches the host reads the two done flags.
+int swr_qeq_cg_run(swr_system *S, int first_batch) {
+  int rc;
+  if ((rc = launch_cg(S, PH_S1, 0)) || (rc = launch_cg(S, PH_S2, 0)) || (rc = launch_cg(S, PH_S3, 0)) ||
+      (rc = launch_cg(S, PH_A1, 0)))
+    return rc;
+  int k = 1;
+  int batch = std::max(2, first_batch);
+  while (k < S->imax + 1) {
+    for (int b = 0; b < batch && k < S->imax + 1; b++, k++)
+      if ((rc = launch_cg(S, PH_K, (k - 1) & 1))) return rc;
+    CgScalars h[2];
+    HIP_CHECK(hipMemcpyAsync(h, S->cgs, sizeof(h), hipMemcpyDeviceToHost, S->stream));
+    HIP_CHECK(hipStreamSynchronize(S->stream));
+    if (h[0].done && h[1].done) break;
+    batch = 4;
+  }
+  return SWR_OK;
+}
+
+int swr_qeq_calc_q(swr_system *S) {
+  const int n = S->n;
+  const int nblk = div_up(n, RPB);
+  hipLaunchKernelGGL(k_q_sums, dim3(nblk), dim3(QT), 0, S->stream, n, (const double2 *)S->st, S->partials,
+                     S->counters + 1, S->partials + (size_t)S->cg_blocks * 8);
+  hipLaunchKernelGGL(k_q_apply, dim3(div_up(S->N, 256)), dim3(256), 0, S->stream, n, S->N, (const double2 *)S->st,
+                     S->partials + (size_t)S->cg_blocks * 8, S->owner, S->q, S->s_hist, S->t_hist);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_qeq_blocks(int n) { return div_up(n, RPB); }

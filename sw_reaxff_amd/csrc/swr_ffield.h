@@ -1,0 +1,37 @@
+// swr_ffield.h — force-field parameters as read from a ReaxFF `ffield` file.
+#pragma once
+
+#include <string>
+#include <vector>
+
+struct FfAtom {
+  std::string name;
+  double r_s = 0, valency = 0, mass = 0, r_vdw = 0, epsilon = 0, gamma = 0, r_pi = 0, valency_e = 0;
+  double nlp_opt = 0;
+  double alpha = 0, gamma_w = 0, valency_boc = 0, p_ovun5 = 0, chi = 0, eta = 0;
+  int p_hbond = 0;
+  double r_pi_pi = 0, p_lp2 = 0, b_o_131 = 0, b_o_132 = 0, b_o_133 = 0;
+  double p_ovun2 = 0, p_val3 = 0, valency_val = 0, p_val5 = 0, rcore2 = 0, ecore2 = 0, acore2 = 0;
+};
+
+struct FfPair {
+  double p_bo1 = 0, p_bo2 = 0, p_bo3 = 0, p_bo4 = 0, p_bo5 = 0, p_bo6 = 0;
+  double r_s = 0, r_p = 0, r_pp = 0;
+  double p_boc3 = 0, p_boc4 = 0, p_boc5 = 0;
+  double De_s = 0, De_p = 0, De_pp = 0, p_be1 = 0, p_be2 = 0, p_ovun1 = 0;
+  double D = 0, alpha = 0, r_vdW = 0, gamma_w = 0, gamma = 0;
+  double v13cor = 0, ovc = 0;
+};
+
+struct swr_ffield_t {
+  std::vector<double> gp;           // general parameters l[0..]
+  std::vector<FfAtom> atom;
+  std::vector<FfPair> pair;         // [ne*ne]
+  double bo_cut = 0, nonb_low = 0, nonb_cut = 0;
+  int ne() const { return (int)atom.size(); }
+  const FfPair &tb(int i, int j) const { return pair[(size_t)i * atom.size() + j]; }
+  FfPair &tb(int i, int j) { return pair[(size_t)i * atom.size() + j]; }
+};
+
+// Returns an empty string on success, else the reason.
+std::string swr_read_ffield(const char *path, swr_ffield_t *ff);

@@ -1,0 +1,227 @@
+// swr_internal.h — device-side state and helpers of the MI355X path.
+//
+// One swr_system owns every device buffer of one simulated system: atoms
+// (local + ghost images), the Verlet list, the far/bond/hbond lists in the
+// reference's record layouts, the BO workspace and the QEq matrix and solver
+// vectors.  All kernels run on the system's stream; entry points enqueue and
+// return, and swr_sync() reports errors raised on the device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/swreax.h"
+#include "swr_ffield.h"
+
+#define SWR_MAX_ELEM 8    // force-field elements usable per system
+#define SWR_MAX_TYPES 8   // LAMMPS atom types per system
+#define SWR_WAVE 64
+
+// ---- error flags raised by kernels (bit mask in DevStatus::flags)
+#define SWR_F_BOND_CAP   (1u << 0)   // an atom has more bonds than num_bonds[i]
+#define SWR_F_HBOND_CAP  (1u << 1)   // an H has more hbonds than num_hbonds[i]
+#define SWR_F_BOND_TOTAL (1u << 2)   // sum of bond capacities exceeds the list
+#define SWR_F_HBOND_TOTAL (1u << 3)  // sum of hbond capacities exceeds the list
+#define SWR_F_BOND_LDS   (1u << 4)   // more than SWR_MAX_ATOM_BONDS bonds for one atom
+#define SWR_F_MARK_DEPTH (1u << 5)   // bond_mark relaxation did not converge
+#define SWR_F_BIN        (1u << 6)   // an atom fell outside the bin grid
+#define SWR_F_NAN        (1u << 7)   // non-finite value in CG scalars
+
+#define SWR_MAX_ATOM_BONDS 64        // bonds one atom may hold while building its list
+
+#define HIP_CHECK(expr)                                                    \
+  do {                                                                     \
+    hipError_t _e = (expr);                                                \
+    if (_e != hipSuccess) {                                                \
+      swr_set_error_msg(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+      return SWR_ERR_HIP;                                                  \
+    }                                                                      \
+  } while (0)
+
+void swr_set_error_msg(const std::string &m);
+
+// Force-field parameters the path reads, element-indexed (sbp/tbp subset,
+// reaxc_ctypes_sunway.h:100-170) plus QEq parameters per LAMMPS type.
+struct DevTbp {
+  double p_bo1, p_bo2, p_bo3, p_bo4, p_bo5, p_bo6;
+  double r_s, r_p, r_pp;
+  double p_boc3, p_boc4, p_boc5;
+  double v13cor, ovc;
+};
+
+struct DevParams {
+  int nelem, ntypes;
+  int map[SWR_MAX_TYPES + 1];
+  double r_s[SWR_MAX_ELEM], r_pi[SWR_MAX_ELEM], r_pi_pi[SWR_MAX_ELEM];
+  double valency[SWR_MAX_ELEM], valency_e[SWR_MAX_ELEM], valency_boc[SWR_MAX_ELEM];
+  double valency_val[SWR_MAX_ELEM], nlp_opt[SWR_MAX_ELEM], mass[SWR_MAX_ELEM];
+  int p_hbond[SWR_MAX_ELEM];
+  DevTbp tbp[SWR_MAX_ELEM * SWR_MAX_ELEM];
+  double p_boc1, p_boc2, p_lp1;
+  double bo_cut, bond_cut, hbond_cut, nonb_cut, nonb_cutsq;
+  // QEq, LAMMPS-type indexed
+  double chi[SWR_MAX_TYPES + 1], eta[SWR_MAX_TYPES + 1], gamma[SWR_MAX_TYPES + 1];
+  double shld[(SWR_MAX_TYPES + 1) * (SWR_MAX_TYPES + 1)];
+  double Tap[8];
+  double swa, swb, swbsq;
+  double tol;
+};
+
+// Small block of device-written results the host reads back asynchronously.
+struct DevStatus {
+  unsigned int flags;
+  int need_bond_cap;     // largest per-atom bond count that overflowed
+  int need_hbond_cap;
+  int total_bonds;       // sum of num_bonds (Reset)
+  int total_hbonds;
+  int numH;
+  int mark_changed;
+  int pad0;
+  long long num_far;     // far entries written (sum over atoms)
+  long long H_nnz;
+  int cg_iters[2];       // s, t
+  int pad1[2];
+};
+
+// Pipelined-CG scalars of one right-hand side (CG_v2,
+// fix_qeq_reax_sunway.cpp:1080-1152).  Written by the last workgroup of each
+// iteration kernel, read by the next.
+struct CgScalars {
+  double alpha_x;     // alpha used in x += alpha p (value before the update)
+  double alpha_a;     // alpha used in r -= alpha ss, ... (value after the update)
+  double beta;
+  double sig_old, heta;
+  double dg0, dg1;    // latest reduced u.r, u.w
+  double b_norm;
+  double bdot, udot_r, udot_w;   // setup dots
+  int active_b;       // B/C of the current iteration run
+  int active_a;       // A of the next iteration runs
+  int iter;           // reference loop counter i
+  int done;
+};
+
+struct swr_system {
+  // ---- host-side configuration
+  int device = 0;
+  hipStream_t stream = nullptr;
+  swr_ffield_t ff;               // parsed force field (host)
+  swr_control_t ctl;
+  DevParams hp;                  // host copy of device params
+  DevParams *dp = nullptr;       // device params
+
+  // ---- atoms (device)
+  int n = 0, N = 0, ncap = 0, Ncap = 0;
+  double *x = nullptr;           // [N][3]
+  double *v = nullptr;           // [n][3] (harness)
+  double *q = nullptr;           // [N]
+  int *type = nullptr, *tag = nullptr, *owner = nullptr;  // [N]
+  signed char *img = nullptr;    // [N][3] image shift of ghosts, in box lengths
+  int *mask = nullptr;           // unused on this path (group all)
+  swr_atom_pack_t *packed = nullptr;
+
+  // ---- bins + Verlet list
+  int nbin[3] = {0, 0, 0};
+  double binlo[3] = {0, 0, 0}, binsize[3] = {0, 0, 0}, bininv[3] = {0, 0, 0};
+  int nbins = 0, binscap = 0;
+  int *bin_of = nullptr, *bin_count = nullptr, *bin_start = nullptr, *bin_fill = nullptr;
+  double4 *bp_pos = nullptr;     // atoms sorted by bin: x, y, z, unused
+  int *bp_id = nullptr;
+  int nrows = 0;
+  int4 *stencil_rows = nullptr;  // (dy, dz, sx, 0)
+  double cutneigh = 12.0;
+  long long *voff = nullptr;     // [N+1]
+  int *vcount = nullptr;         // [N]
+  int *vlist = nullptr;
+  long long vcap = 0, vtotal = 0;
+
+  // ---- far list (reax_list FAR_NBRS_FULL)
+  swr_far_neighbor_t *far = nullptr;   // capacity vcap, atom i slots [voff[i], voff[i+1])
+  int *far_index = nullptr, *far_end = nullptr;
+
+  // ---- capacities, bonds, hbonds
+  int *num_bonds = nullptr, *num_hbonds = nullptr, *Hindex = nullptr;
+  int *bond_index = nullptr, *bond_end = nullptr;
+  int bond_cap = 0;
+  swr_bond_t *bonds = nullptr;
+  swr_bond_order_t *bo_data = nullptr;
+  double *BO = nullptr, *BOpi = nullptr, *Cdbo = nullptr, *Cdbopi = nullptr, *Cdbopi2 = nullptr;
+  int *hbond_index = nullptr, *hbond_end = nullptr;   // [ncap] by Hindex
+  int hbond_cap = 0;
+  swr_hbond_t *hbonds = nullptr;
+  bool capacities_set = false;
+
+  // ---- BO workspace
+  double *bo_dboc = nullptr;       // [N][2]
+  double *dDeltap_self = nullptr;  // [N][3]
+  int *bond_mark = nullptr, *bond_mark2 = nullptr;
+  int *counts = nullptr;           // [2][Ncap]: bonds found per atom, hbonds found per H
+  double *ws = nullptr;            // 13 arrays of N doubles, see WS_* below
+  // ---- QEq
+  int *H_firstnbr = nullptr, *H_numnbrs = nullptr, *H_jlist = nullptr;
+  double *H_val = nullptr;
+  float *H_val32 = nullptr;
+  long long Hcap = 0;
+  double *Hdia_inv = nullptr, *b_s = nullptr, *b_t = nullptr;
+  double *st = nullptr;            // [2][ncap]: s, t
+  double *s_hist = nullptr, *t_hist = nullptr;  // [n][5]
+  double *cgv = nullptr;           // CG vectors, see CGV_* below
+  CgScalars *cgs = nullptr;        // [2]
+  double *partials = nullptr;      // per-workgroup partial dots
+  unsigned int *counters = nullptr;
+  int cg_blocks = 0;
+  int qeq_initialized = 0;
+  int mixed = 0;                   // mixed-precision QEq
+  void *mixw = nullptr;            // mixed-precision workspace
+  int imax = 200;
+
+  // ---- status
+  DevStatus *status = nullptr;     // device
+  DevStatus *status_host = nullptr;  // pinned
+  unsigned int sticky_flags = 0;
+
+  // ---- scratch for scans
+  void *scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+  int *tmp_i = nullptr;
+  size_t tmp_i_cap = 0;
+
+  // ---- timers
+  hipEvent_t ev[32];
+  bool events = false;
+};
+
+// workspace array slots (each N doubles) for BO sections 1 and 3
+enum {
+  WS_DELTAP = 0, WS_DELTAP_BOC, WS_DELTA, WS_DELTA_E, WS_DELTA_VAL, WS_VLPEX, WS_NLP, WS_NLP_TEMP,
+  WS_CLP, WS_DELTA_LP, WS_DDELTA_LP, WS_DELTA_LP_TEMP, WS_DDELTA_LP_TEMP, WS_COUNT
+};
+// CG vector slots (each ncap doubles, two right-hand sides interleaved per slot)
+enum { CGV_R = 0, CGV_U, CGV_W, CGV_Z, CGV_SS, CGV_V, CGV_P, CGV_D0, CGV_D1, CGV_M, CGV_COUNT };
+
+// ---- helpers implemented in the .hip files
+int swr_scan_i32_to_i64(swr_system *S, const int *in, long long *out, int n, long long *total_host);
+int swr_scan_i32(swr_system *S, const int *in, int *out, int n, int *total_dev);
+int swr_ensure_tmp(swr_system *S, size_t bytes);
+
+template <class T>
+inline int swr_alloc(T **p, size_t count) {
+  if (*p) { (void)hipFree(*p); *p = nullptr; }
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    swr_set_error_msg(std::string("hipMalloc: ") + hipGetErrorString(e));
+    *p = nullptr;
+    return SWR_ERR_NOMEM;
+  }
+  return SWR_OK;
+}
+
+template <class T>
+inline void swr_free(T *&p) {
+  if (p) { (void)hipFree(p); p = nullptr; }
+}
+
+inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }

@@ -1,0 +1,148 @@
+// util.hip — exclusive prefix sums (3-phase, deterministic) and small helpers.
+#include "swr_internal.h"
+#include "swr_device.h"
+
+#include <mutex>
+
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+std::mutex g_err_mu;
+std::string g_err;
+
+// Block-wide exclusive scan of one value per thread; returns the exclusive
+// prefix and writes the block total.
+template <class T>
+__device__ T block_excl_scan(T v, T *lds, T *total) {
+  const int w = wave_in_block(), l = lane_id();
+  T inc = wave_incl_scan(v);
+  if (l == SWR_WAVE - 1) lds[w] = inc;
+  __syncthreads();
+  T wave_off = 0, tot = 0;
+  for (int k = 0; k < SCAN_THREADS / SWR_WAVE; k++) {
+    if (k < w) wave_off += lds[k];
+    tot += lds[k];
+  }
+  __syncthreads();
+  *total = tot;
+  return wave_off + inc - v;
+}
+
+template <class TI, class TO>
+__global__ void __launch_bounds__(SCAN_THREADS) k_tile_sums(const TI *in, int n, TO *tile_sums) {
+  __shared__ TO lds[SCAN_THREADS / SWR_WAVE];
+  const long long base = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  TO s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++)
+    if (base + k < n) s += (TO)in[base + k];
+  TO tot;
+  block_excl_scan<TO>(s, lds, &tot);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+// One block scans all tile sums in order (carry across chunks).
+template <class TO>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(TO *tile_sums, int ntiles, TO *total) {
+  __shared__ TO lds[SCAN_THREADS / SWR_WAVE];
+  TO carry = 0;
+  for (int c = 0; c < ntiles; c += SCAN_THREADS) {
+    int i = c + threadIdx.x;
+    TO v = i < ntiles ? tile_sums[i] : 0;
+    TO tot;
+    TO ex = block_excl_scan<TO>(v, lds, &tot);
+    if (i < ntiles) tile_sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <class TI, class TO>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const TI *in, int n, const TO *tile_off, TO *out) {
+  __shared__ TO lds[SCAN_THREADS / SWR_WAVE];
+  const long long base = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  TO vals[SCAN_ITEMS];
+  TO s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    vals[k] = (base + k < n) ? (TO)in[base + k] : 0;
+    s += vals[k];
+  }
+  TO tot;
+  TO ex = block_excl_scan<TO>(s, lds, &tot) + tile_off[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    if (base + k < n) out[base + k] = ex;
+    ex += vals[k];
+  }
+}
+
+template <class TI, class TO>
+int scan_impl(swr_system *S, const TI *in, TO *out, int n, TO *total_dev) {
+  int ntiles = div_up(n > 0 ? n : 1, SCAN_TILE);
+  size_t need = sizeof(TO) * (size_t)(ntiles + 1);
+  if (need > S->scan_tmp_bytes) {
+    if (S->scan_tmp) (void)hipFree(S->scan_tmp);
+    S->scan_tmp = nullptr;
+    S->scan_tmp_bytes = 0;
+    HIP_CHECK(hipMalloc(&S->scan_tmp, need * 2));
+    S->scan_tmp_bytes = need * 2;
+  }
+  TO *tiles = (TO *)S->scan_tmp;
+  TO *tot = total_dev ? total_dev : tiles + ntiles;
+  if (n > 0) {
+    hipLaunchKernelGGL((k_tile_sums<TI, TO>), dim3(ntiles), dim3(SCAN_THREADS), 0, S->stream, in, n, tiles);
+    hipLaunchKernelGGL((k_scan_tiles<TO>), dim3(1), dim3(SCAN_THREADS), 0, S->stream, tiles, ntiles, tot);
+    hipLaunchKernelGGL((k_scan_apply<TI, TO>), dim3(ntiles), dim3(SCAN_THREADS), 0, S->stream, in, n,
+                       (const TO *)tiles, out);
+  } else {
+    HIP_CHECK(hipMemsetAsync(tot, 0, sizeof(TO), S->stream));
+  }
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+__global__ void k_copy_total_i64(const long long *src, long long *dst) { *dst = *src; }
+
+}  // namespace
+
+void swr_set_error_msg(const std::string &m) {
+  std::lock_guard<std::mutex> g(g_err_mu);
+  g_err = m;
+}
+
+extern "C" const char *swr_last_error(void) {
+  static thread_local std::string copy;
+  std::lock_guard<std::mutex> g(g_err_mu);
+  copy = g_err;
+  return copy.c_str();
+}
+
+// Exclusive scan of n ints into int64 offsets; out has n+1 entries
+// (out[n] = total).  If total_host is given, synchronises and returns it.
+int swr_scan_i32_to_i64(swr_system *S, const int *in, long long *out, int n, long long *total_host) {
+  int rc = scan_impl<int, long long>(S, in, out, n, out + n);
+  if (rc) return rc;
+  if (total_host) {
+    HIP_CHECK(hipMemcpyAsync(total_host, out + n, sizeof(long long), hipMemcpyDeviceToHost, S->stream));
+    HIP_CHECK(hipStreamSynchronize(S->stream));
+  }
+  return SWR_OK;
+}
+
+// Exclusive scan of n ints into ints; total written to *total_dev (device).
+int swr_scan_i32(swr_system *S, const int *in, int *out, int n, int *total_dev) {
+  return scan_impl<int, int>(S, in, out, n, total_dev);
+}
+
+int swr_ensure_tmp(swr_system *S, size_t bytes) {
+  size_t count = (bytes + sizeof(int) - 1) / sizeof(int);
+  if (count <= S->tmp_i_cap) return SWR_OK;
+  int rc = swr_alloc(&S->tmp_i, count);
+  if (rc) return rc;
+  S->tmp_i_cap = count;
+  return SWR_OK;
+}

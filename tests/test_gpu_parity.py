@@ -1,0 +1,202 @@
+"""GPU (HIP, gfx950) vs CPU FP64 oracle parity on the 648-water box.
+
+Lists must agree bit for bit as sets (and here also in order, since the
+oracle consumes the GPU's Verlet order); bond orders and their derivative
+terms within 1e-10 relative; charges within 1e-6 e.
+"""
+import numpy as np
+import pytest
+
+from sw_reaxff_amd import systems
+
+from .parity import (BO_RTOL, Q_ATOL, atoms_of, harness, make_oracle, per_atom, rel_err, verlet_of,
+                     water_c0)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c0_setup(gpu_lib, oracle_lib, ffield):
+    s = water_c0()
+    lmp = harness(s, ffield)
+    sysh = lmp.system
+    A = atoms_of(sysh)
+    V = verlet_of(sysh)
+    o = make_oracle(oracle_lib, ffield, s.elements, dict(A, q=np.zeros(A["N"])), verlet=V)
+    its = o.step()
+    return dict(s=s, lmp=lmp, sys=sysh, A=A, V=V, o=o, its=its)
+
+
+def test_ghost_images_match_borders(c0_setup):
+    s, A = c0_setup["s"], c0_setup["A"]
+    xa, owner = systems.borders(s.x, s.lo, s.hi, 12.0)
+    assert A["N"] == xa.shape[0]
+    assert np.array_equal(A["owner"], owner)
+    assert np.array_equal(A["x"], xa)
+    assert np.array_equal(A["tag"], s.tag[owner])
+
+
+def test_verlet_list_equals_oracle_set(c0_setup, oracle_lib, ffield):
+    A, (off, lst) = c0_setup["A"], c0_setup["V"]
+    o = make_oracle(oracle_lib, ffield, c0_setup["s"].elements, A)
+    o.build_verlet(12.0)
+    ooff, olst = o.get("verlet_offsets"), o.get("verlet_list")
+    assert np.array_equal(np.diff(off), np.diff(ooff)), "per-atom neighbour counts differ"
+    N = A["N"]
+    gi = np.repeat(np.arange(N), np.diff(off))
+    oi = np.repeat(np.arange(N), np.diff(ooff))
+    g = np.lexsort((lst, gi))
+    h = np.lexsort((olst, oi))
+    assert np.array_equal(lst[g], olst[h])
+    # the cell list is complete: no j == i, every pair within 12 A
+    assert not np.any(lst == gi)
+    d = A["x"][lst] - A["x"][gi]
+    assert np.all((d * d).sum(1) <= 144.0)
+
+
+def test_charges_and_solver(c0_setup):
+    sysh, o, its = c0_setup["sys"], c0_setup["o"], c0_setup["its"]
+    n = sysh.scalar("n")
+    gq, oq = sysh.get("q"), o.get("q")
+    assert np.max(np.abs(gq - oq)) <= Q_ATOL
+    assert abs(gq[:n].sum()) < 1e-9
+    gs, gt = sysh.get("s"), sysh.get("t")
+    assert np.max(np.abs(gs - o.get("s")[:n])) <= Q_ATOL
+    assert np.max(np.abs(gt - o.get("t")[:n])) <= Q_ATOL
+    gits = (sysh.scalar("matvecs_s"), sysh.scalar("matvecs_t"))
+    assert abs(gits[0] - its[0]) <= 1 and abs(gits[1] - its[1]) <= 1, (gits, its)
+
+
+def test_H_matrix(c0_setup):
+    sysh, o, A = c0_setup["sys"], c0_setup["o"], c0_setup["A"]
+    n = A["n"]
+    gf, gn, gj, gv = sysh.get("H_firstnbr"), sysh.get("H_numnbrs"), sysh.get("H_jlist"), sysh.get("H_val")
+    of, on, oj, ov = o.get("H_firstnbr"), o.get("H_numnbrs"), o.get("H_jlist"), o.get("H_val")
+    assert np.array_equal(gn, on)
+    owner = A["owner"]
+    worst = 0.0
+    for i in range(0, n, 7):
+        a = slice(gf[i], gf[i] + gn[i])
+        b = slice(of[i], of[i] + on[i])
+        assert np.array_equal(gj[a], owner[oj[b]])
+        worst = max(worst, rel_err(gv[a], ov[b]))
+    assert worst < 1e-13, worst
+    assert np.array_equal(sysh.get("Hdia_inv"), o.get("Hdia_inv")[:n])
+
+
+def test_far_list_exact(c0_setup):
+    sysh, o, A = c0_setup["sys"], c0_setup["o"], c0_setup["A"]
+    gi, ge, gfar = sysh.get("far_index"), sysh.get("far_end"), sysh.get("far")
+    oi, oe, ofar = o.get("far_index"), o.get("far_end"), o.get("far")
+    assert np.array_equal(ge - gi, oe - oi)
+    gq = sysh.get("q")
+    for i in range(A["N"]):
+        a = gfar[gi[i]:ge[i]]
+        b = ofar[oi[i]:oe[i]]
+        assert np.array_equal(a["nbr"], b["nbr"])
+        assert np.array_equal(a["type"], b["type"])
+        assert np.array_equal(a["orig_id"], b["orig_id"])
+        assert np.array_equal(a["d"], b["d"]), i
+        assert np.array_equal(a["dvec"], b["dvec"])
+        assert np.all(a["rel_box"] == 0)
+        assert np.array_equal(a["q"], gq[a["nbr"]])
+        assert np.max(np.abs(a["q"] - b["q"]), initial=0) <= Q_ATOL
+
+
+def _bond_views(h, idx_name, end_name):
+    return h.get(idx_name), h.get(end_name), h.get("bonds"), h.get("bo_data"), h.get("BO"), h.get("BOpi")
+
+
+def test_bonds_and_bond_orders(c0_setup):
+    sysh, o, A = c0_setup["sys"], c0_setup["o"], c0_setup["A"]
+    gb, ge, gbonds, gbo, gBO, gpi = _bond_views(sysh, "bond_index", "bond_end")
+    ob, oe, obonds, obo, oBO, opi = _bond_views(o, "bond_index", "bond_end")
+    assert np.array_equal(ge - gb, oe - ob), "bond counts differ (membership flip at bo_cut?)"
+    gpi = gpi.reshape(-1, 2)
+    opi = opi.reshape(-1, 2)
+    worst = {}
+
+    def upd(k, v):
+        worst[k] = max(worst.get(k, 0.0), v)
+
+    for i in range(A["N"]):
+        a = slice(gb[i], ge[i])
+        b = slice(ob[i], oe[i])
+        assert np.array_equal(gbonds["nbr"][a], obonds["nbr"][b])
+        assert np.array_equal(gbonds["d"][a], obonds["d"][b])
+        assert np.array_equal(gbonds["dvec"][a], obonds["dvec"][b])
+        assert np.array_equal(gbonds["dbond_index"][a] - gb[i], obonds["dbond_index"][b] - ob[i])
+        js = gbonds["nbr"][a]
+        assert np.array_equal(gbonds["sym_index"][a] - gb[js], obonds["sym_index"][b] - ob[js])
+        upd("BO", rel_err(gBO[a], oBO[b], 1e-12))
+        upd("BOpi", rel_err(gpi[a], opi[b], 1e-12))
+        for f in ("BO_s", "dBOp", "dln_BOp_s", "dln_BOp_pi", "dln_BOp_pi2", "C1dbo", "C2dbo", "C3dbo",
+                  "C1dbopi", "C2dbopi", "C3dbopi", "C4dbopi", "C1dbopi2", "C2dbopi2", "C3dbopi2", "C4dbopi2"):
+            upd(f, rel_err(gbo[f][a], obo[f][b], 1e-12))
+    bad = {k: v for k, v in worst.items() if v > BO_RTOL}
+    assert not bad, bad
+
+
+def test_workspace_terms(c0_setup):
+    sysh, o, A = c0_setup["sys"], c0_setup["o"], c0_setup["A"]
+    for nm in ("bo_dboc", "dDeltap_self", "Deltap", "Deltap_boc", "Delta", "Delta_e", "Delta_val", "vlpex", "nlp",
+               "nlp_temp", "Clp", "Delta_lp", "dDelta_lp", "Delta_lp_temp", "dDelta_lp_temp"):
+        g, r = sysh.get(nm), o.get(nm)
+        assert g.shape == r.shape, nm
+        assert rel_err(g, r, 1e-12) <= BO_RTOL, nm
+    assert np.array_equal(sysh.get("bond_mark"), o.get("bond_mark"))
+    n = A["n"]
+    assert np.array_equal(sysh.get("num_bonds"), o.get("num_bonds")[: A["N"]])
+    assert np.array_equal(sysh.get("num_hbonds")[:n], o.get("num_hbonds")[:n])
+
+
+def test_hbonds(c0_setup):
+    sysh, o, A = c0_setup["sys"], c0_setup["o"], c0_setup["A"]
+    gH, oH = sysh.get("Hindex"), o.get("Hindex")
+    n = A["n"]
+    assert np.array_equal(gH[:n], oH[:n])
+    numH = int((gH[:n] >= 0).sum())
+    gi, ge, ghb, gfs = sysh.get("hbond_index"), sysh.get("hbond_end"), sysh.get("hbonds"), sysh.get("hbond_far")
+    oi, oe, ohb, ofs = o.get("hbond_index"), o.get("hbond_end"), o.get("hbonds"), o.get("hbond_far")
+    gfar, gfi = sysh.get("far"), sysh.get("far_index")
+    ofi = o.get("far_index")
+    assert len(gi) == numH
+    for h in range(numH):
+        a = slice(gi[h], ge[h])
+        b = slice(oi[h], oe[h])
+        assert np.array_equal(ghb["nbr"][a], ohb["nbr"][b])
+        assert np.all(ghb["scl"][a] == 1)
+        # ptr targets the far entry of the same pair, at the same position
+        assert np.array_equal(gfar["nbr"][gfs[a]], ghb["nbr"][a])
+    # position of each hbond's far entry within its atom's far range agrees
+    Hatoms = np.nonzero(gH[:n] >= 0)[0]
+    for i in Hatoms[::5]:
+        h = gH[i]
+        assert np.array_equal(gfs[gi[h]:ge[h]] - gfi[i], ofs[oi[h]:oe[h]] - ofi[i])
+
+
+def test_split_stages_equal_fused(c0_setup, gpu_lib, ffield):
+    """write_reax_lists + init_forces (re-reading the far list) == the fused pass."""
+    h = c0_setup["sys"]
+    keys = ("far", "far_index", "far_end", "bonds", "bo_data", "BO", "BOpi", "hbonds", "hbond_end", "bo_dboc",
+            "dDeltap_self", "Deltap", "Deltap_boc", "bond_end")
+
+    def snap():
+        h.sync()
+        return {k: h.get(k) for k in keys}
+
+    h.call("write_reax_atoms_and_pack")
+    h.call("reset")
+    h.call("write_reax_lists_init_forces")
+    fused = snap()
+    h.call("write_reax_atoms_and_pack")
+    h.call("reset")
+    h.call("write_reax_lists")
+    h.call("init_forces_noqeq_full")
+    split = snap()
+    for k in keys:
+        assert np.array_equal(split[k], fused[k]), k
+    # restore the post-BO state for any later test
+    h.call("validate_lists")
+    h.call("bo")
+    h.sync()
