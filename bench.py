@@ -1,0 +1,236 @@
+#!/usr/bin/env python3
+"""Benchmark of the nbr + BO + QEq path on one MI355X.
+
+Workload (default): 32,000 rigid waters (96,000 atoms) in a periodic box at
+1.0 g/cm^3, seeded lattice placement and orientations, Maxwell velocities at
+300 K; NVE at dt = 0.25 fs with the Verlet list rebuilt every 10 steps
+(2 A skin); per step the full path runs on the GPU: ghost refresh, QEq
+(H build, s/t CG to 1e-6, charges), atom pack, Reset, far list + bonds +
+hbonds, list validation and the corrected bond orders.
+
+Prints ONE JSON line: ms/step (lower is better), atom-steps/s, the roofline
+of the dominant kernel (HIP events on the library's stream) and the CPU
+oracle timed on this machine's host cores on a bounded sample of the same
+workload.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--config c1|c0|c2big]
+                    [--mixed] [--cpu-budget-s 20] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from sw_reaxff_amd import _lib, systems  # noqa: E402
+from sw_reaxff_amd.lammps import DEFAULT_FFIELD, Lammps, standard_script  # noqa: E402
+
+METRIC = "ms/step (nbr+BO+QEq) at 96k atoms; batched systems/s + p99 at 1e-6 e tol"
+HBM_PEAK_GBS = 8000.0
+CONFIGS = {
+    "c0": dict(nmol=648, label="648-water box (1,944 atoms)"),
+    "c1": dict(nmol=32000, label="32,000-water box (96,000 atoms)"),
+    "c2big": dict(nmol=256000, label="256,000-water box (768,000 atoms)"),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def device_sync():
+    # The path runs on the library's own HIP stream; this is the device-wide
+    # barrier (torch is not imported: it carries a second HIP runtime).
+    _lib.check(_lib.lib().swr_device_synchronize(), "swr_device_synchronize")
+
+
+def counts(sysh):
+    n, N = sysh.scalar("n"), sysh.scalar("N")
+    fi, fe = sysh.get("far_index"), sysh.get("far_end")
+    bi, be = sysh.get("bond_index"), sysh.get("bond_end")
+    hi, he = sysh.get("hbond_index"), sysh.get("hbond_end")
+    return dict(n=n, N=N, verlet=int(sysh.scalar("verlet_total")), far=int((fe - fi).sum()),
+                bonds=int((be - bi).sum()), hbonds=int((he - hi).sum()), nnz=int(sysh.get("H_numnbrs").sum()))
+
+
+def algorithmic_bytes(slot, c, mixed=False):
+    """Compulsory HBM bytes of one launch of a kernel slot, from its inputs and outputs."""
+    n, N = c["n"], c["N"]
+    if slot == "cg_iter":
+        # H (jlist 4 B + val 8 B per entry), the (s,t) d gather once per row,
+        # 9 (s,t) vectors read and 9 written, Hdia_inv, eta type, row pointers
+        return 12 * c["nnz"] + n * (16 + 18 * 16 + 8 + 4 + 8)
+    if slot == "lists":
+        # Verlet ids + offsets, packed atoms once, far entries, bond records
+        # (56 bond_data + 200 bond_order_data + 8 BO + 16 BOpi + 24 Cdbo*),
+        # hbonds, per-atom outputs and capacities
+        return (4 * c["verlet"] + 8 * (N + 1) + 40 * N + 64 * c["far"] + 304 * c["bonds"] + 16 * c["hbonds"]
+                + N * (16 + 24 + 4 + 16 + 8 + 4 + 4 + 12))
+    if slot == "build_H":
+        return 4 * (c["verlet"] * n // max(N, 1)) + 24 * N + 4 * N + 12 * c["nnz"] + n * (8 + 16 + 16 + 80 + 8)
+    if slot == "bo":
+        return 8 * N * 20 + c["bonds"] * (56 + 200 + 8 + 16)
+    if slot == "verlet":
+        return 32 * N + 4 * c["verlet"] + 8 * N
+    return None
+
+
+def cpu_baseline(sysh, elements, ffield, lmp_dt, every, budget_s):
+    """Times the FP64 oracle on this host on the GPU's state, moving atoms like the GPU loop."""
+    from oracle import pyoracle  # test-infrastructure checker, only for this baseline leg
+    if not os.path.exists(pyoracle.LIB_PATH):
+        import subprocess
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, stdout=subprocess.DEVNULL)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n, N = sysh.scalar("n"), sysh.scalar("N")
+    x = sysh.get("x").reshape(-1, 3).copy()
+    typ, tag, owner = sysh.get("type"), sysh.get("tag"), sysh.get("owner")
+    img = sysh.get("img").reshape(-1, 3).astype(np.float64)
+    v = sysh.get("v").reshape(-1, 3)
+    lo_hi = None
+    o = pyoracle.Oracle(ffield, elements, threads=threads)
+    o.set_atoms(n, x, typ, tag, owner, q=sysh.get("q"))
+    o.set("s_hist", sysh.get("s_hist"))
+    o.set("t_hist", sysh.get("t_hist"))
+    o.set("num_bonds", sysh.get("num_bonds"))
+    o.set("num_hbonds", sysh.get("num_hbonds"))
+    # box lengths from the ghost images
+    gh = np.nonzero(np.abs(img).sum(1) > 0)[0]
+    L = np.zeros(3)
+    for d in range(3):
+        k = gh[img[gh, d] != 0][0]
+        L[d] = (x[k, d] - x[owner[k], d]) / img[k, d]
+    t0 = time.perf_counter()
+    o.build_verlet(12.0)
+    t_verlet = time.perf_counter() - t0
+    steps, t_steps, iters = 0, 0.0, []
+    while True:
+        x[:n] += lmp_dt * v
+        x[n:] = x[owner[n:]] + img[n:] * L
+        o.set_atoms(n, x, typ, tag, owner, q=o.get("q"))
+        t0 = time.perf_counter()
+        its = o.step()
+        t_steps += time.perf_counter() - t0
+        iters.append(its)
+        steps += 1
+        if t_steps + t_verlet > budget_s or steps >= 10:
+            break
+    ms = (t_steps / steps + t_verlet / every) * 1e3
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(value=round(ms, 3), unit="ms/step", cores=threads, kind="port",
+                sample=f"{steps} oracle steps of the same {n}-atom box on the GPU run's state "
+                       f"(+ one Verlet build amortised over {every} steps); CG iterations {iters}",
+                nproc=os.cpu_count(), cpu=cpu_model, verlet_build_s=round(t_verlet, 3))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--mixed", action="store_true", help="mixed-precision QEq")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--every", type=int, default=10)
+    ap.add_argument("--dt", type=float, default=0.25)
+    ap.add_argument("--cpu-budget-s", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this path has no multi-GPU form (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    cfg = CONFIGS[args.config]
+    log(f"building {cfg['label']} seed {args.seed}")
+    s = systems.water_box(cfg["nmol"], seed=args.seed)
+    lmp = Lammps()
+    lmp.commands(standard_script(s.elements, ffield=DEFAULT_FFIELD, every=args.every, dt=args.dt,
+                                 mixed=args.mixed))
+    lmp.create_atoms(s)
+    log("setup (first neighbour build, capacity estimate, QEq from zero history)")
+    lmp.setup()
+    sysh = lmp.system
+    log(f"warmup {args.warmup} steps")
+    lmp.run(args.warmup)
+    sysh.ktimer_enable(True)
+    for slot in ("verlet", "bins", "lists", "reset", "validate", "bo", "build_H", "cg_setup", "cg_iter", "qeq_q",
+                 "mixed_inner", "mixed_outer"):
+        sysh.ktimer_read(slot, reset=True)
+    its0 = (lmp.stat("total_iters_s"), lmp.stat("total_iters_t"))
+    device_sync()
+    log(f"timed {args.steps} steps")
+    t0 = time.perf_counter()
+    lmp.run(args.steps)
+    device_sync()
+    elapsed = time.perf_counter() - t0
+    ms_step = elapsed * 1e3 / args.steps
+    its = ((lmp.stat("total_iters_s") - its0[0]) / args.steps, (lmp.stat("total_iters_t") - its0[1]) / args.steps)
+    kt = {}
+    for slot in ("verlet", "bins", "lists", "reset", "validate", "bo", "build_H", "cg_setup", "cg_iter", "qeq_q",
+                 "mixed_inner", "mixed_outer"):
+        ms, cnt = sysh.ktimer_read(slot)
+        if cnt:
+            kt[slot] = (ms, cnt)
+    sysh.ktimer_enable(False)
+    c = counts(sysh)
+    n = c["n"]
+    log(f"ms/step {ms_step:.3f}; CG iterations s {its[0]:.1f} t {its[1]:.1f}; kernels {kt}")
+    # dominant kernel and its roofline
+    dom = max(kt, key=lambda k: kt[k][0])
+    dom_ms, dom_cnt = kt[dom]
+    launches = dom_cnt
+    if dom == "cg_iter":
+        # speculative launches after both solves converged exit at once;
+        # count only the launches that did an iteration's work
+        launches = max(1, int(round((max(its) - 1) * args.steps)))
+    per_launch_bytes = algorithmic_bytes(dom, c, args.mixed)
+    avg_ms = dom_ms / launches
+    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9 if per_launch_bytes else None
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1) if achieved else None,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
+                "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_ms * 1e3, 2),
+                "launches": launches}
+    kernels = {k: {"ms_per_step": round(v[0] / args.steps, 4), "launches": v[1],
+                   "gbs": (round(algorithmic_bytes(k, c) * (v[1] if k != "cg_iter" else launches)
+                                 / (v[0] * 1e-3) / 1e9, 1) if algorithmic_bytes(k, c) else None)}
+               for k, v in kt.items()}
+    cpu = None
+    if not args.no_cpu:
+        log(f"CPU baseline (oracle, budget {args.cpu_budget_s:.0f} s)")
+        cpu = cpu_baseline(sysh, s.elements, DEFAULT_FFIELD, args.dt, args.every, args.cpu_budget_s)
+        log(f"CPU {cpu['value']} ms/step on {cpu['cores']} threads")
+    out = {
+        "metric": METRIC, "value": round(ms_step, 4), "unit": "ms/step", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if not args.mixed else "f64+f32", "data": "synthetic",
+        "config": {"workload": cfg["label"] + f", NVE 300 K dt {args.dt} fs, rebuild every {args.every} steps, "
+                               f"skin 2 A, QEq tol 1e-6" + (", mixed-precision QEq" if args.mixed else ""),
+                   "atoms": n, "atoms_with_ghosts": c["N"], "seed": args.seed, "parallelism": "single GPU"},
+        "atom_steps_per_s": round(n / (ms_step * 1e-3), 1),
+        "cg_iters_per_step": [round(its[0], 2), round(its[1], 2)],
+        "counts": c, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(cpu["value"] / ms_step, 2) if cpu else None,
+        "host": platform.node(),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

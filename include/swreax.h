@@ -136,6 +136,9 @@ int swr_qeq_init_matvec(swr_system *sys);
  * callers fix_qeq_reax_sunway.cpp:1068-1143).  Asynchronous like _spawn;
  * swr_sync() is the _join. */
 int swr_sparse_matvec(swr_system *sys, const double *x_dev, double *b_dev);
+/* The same with host arrays of length n (the reference's sparse_matvec takes
+ * the fix's host vectors, fix_qeq_reax_sunway.cpp:1563). Synchronises. */
+int swr_sparse_matvec_h(swr_system *sys, const double *x_host, double *b_host);
 
 /* a15: both CG_v2 solves (s for b = -chi, t for b = -1) with one shared H
  * read per iteration (FixQEqReaxSunway::CG_v2, fix_qeq_reax_sunway.cpp:983;
@@ -158,6 +161,9 @@ int swr_pair_compute(swr_system *sys);
 
 /* Join the stream; returns the first device-raised error. */
 int swr_sync(swr_system *sys);
+/* hipDeviceSynchronize on the current device (a barrier for callers that
+ * time several systems or streams). */
+int swr_device_synchronize(void);
 
 /* Mirrors: download a named device array into host memory in the
  * reference's layout; returns bytes (or -1 for an unknown name).  Pass
@@ -174,6 +180,13 @@ int     swr_put(swr_system *sys, const char *name, const void *in, int64_t bytes
 int64_t swr_get_scalar(swr_system *sys, const char *name);
 /* Device pointer of a named array (for callers that keep data resident). */
 void   *swr_device_ptr(swr_system *sys, const char *name);
+
+/* Per-kernel timing with HIP events on the system's stream.  Slots: verlet
+ * bins lists reset validate bo build_H cg_setup cg_iter spmv qeq_q
+ * mixed_inner mixed_outer.  read() synchronises, returns the accumulated
+ * milliseconds and launch count, and clears them when reset != 0. */
+int swr_ktimer_enable(swr_system *sys, int on);
+int swr_ktimer_read(swr_system *sys, const char *slot, double *ms, long long *count, int reset);
 
 /* ---- mini-LAMMPS host harness (C++ classes behind a C handle) ----------
  * Supplies what LAMMPS would around the path: box, ghost images (borders /

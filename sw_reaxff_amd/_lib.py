@@ -26,7 +26,8 @@ EXPORTS = [
     "swr_qeq_init_matvec", "swr_sparse_matvec", "swr_qeq_cg", "swr_calculate_q", "swr_qeq_pre_force",
     "swr_pair_compute", "swr_sync", "swr_get", "swr_put", "swr_get_scalar", "swr_device_ptr",
     "swr_lmp_open", "swr_lmp_close", "swr_lmp_command", "swr_lmp_create_atoms", "swr_lmp_setup", "swr_lmp_run",
-    "swr_lmp_system", "swr_lmp_stat",
+    "swr_lmp_system", "swr_lmp_stat", "swr_ktimer_enable", "swr_ktimer_read", "swr_device_synchronize",
+    "swr_sparse_matvec_h",
 ]
 
 
@@ -67,6 +68,7 @@ def lib():
                    "swr_calculate_q", "swr_pair_compute", "swr_sync"]:
             getattr(L, nm).argtypes = [vp]
         L.swr_sparse_matvec.argtypes = [vp, vp, vp]
+        L.swr_sparse_matvec_h.argtypes = [vp, dp, dp]
         L.swr_qeq_cg.argtypes = [vp, ip, ip]
         L.swr_qeq_pre_force.argtypes = [vp, ip, ip]
         L.swr_get.restype = i64
@@ -86,6 +88,9 @@ def lib():
         L.swr_lmp_system.argtypes = [vp]
         L.swr_lmp_stat.restype = C.c_double
         L.swr_lmp_stat.argtypes = [vp, C.c_char_p]
+        L.swr_ktimer_enable.argtypes = [vp, C.c_int]
+        L.swr_device_synchronize.argtypes = []
+        L.swr_ktimer_read.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
         _lib = L
     return _lib
 
@@ -178,6 +183,12 @@ class System:
     def build_neighbors(self, cutneigh=0.0):
         self.call("npair_full_bin_ghost_build", C.c_double(cutneigh))
 
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        b = np.zeros(self.scalar("n"))
+        check(self.L.swr_sparse_matvec_h(self.h, ptr(x, C.c_double), ptr(b, C.c_double)), "swr_sparse_matvec_h")
+        return b
+
     def qeq_cg(self):
         a, b = C.c_int(0), C.c_int(0)
         check(self.L.swr_qeq_cg(self.h, C.byref(a), C.byref(b)), "swr_qeq_cg")
@@ -204,3 +215,12 @@ class System:
 
     def scalar(self, name):
         return self.L.swr_get_scalar(self.h, name.encode())
+
+    def ktimer_enable(self, on=True):
+        check(self.L.swr_ktimer_enable(self.h, 1 if on else 0), "swr_ktimer_enable")
+
+    def ktimer_read(self, slot, reset=False):
+        ms, cnt = C.c_double(0), C.c_longlong(0)
+        check(self.L.swr_ktimer_read(self.h, slot.encode(), C.byref(ms), C.byref(cnt), 1 if reset else 0),
+              "swr_ktimer_read " + slot)
+        return ms.value, cnt.value

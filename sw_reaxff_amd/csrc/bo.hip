@@ -158,11 +158,13 @@ int swr_bo_run(swr_system *S) {
   const int N = S->N;
   const size_t c = S->Ncap;
   double *w = S->ws;
+  swr_kt_begin(S, KT_BO);
   hipLaunchKernelGGL(k_bo, dim3(div_up(N, 256)), dim3(256), 0, S->stream, N, S->dp, S->packed, S->bond_index,
                      S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, S->bo_dboc, w + WS_DELTAP * c,
                      w + WS_DELTAP_BOC * c, w + WS_DELTA * c, w + WS_DELTA_E * c, w + WS_DELTA_VAL * c,
                      w + WS_VLPEX * c, w + WS_NLP * c, w + WS_NLP_TEMP * c, w + WS_CLP * c, w + WS_DELTA_LP * c,
                      w + WS_DDELTA_LP * c, w + WS_DELTA_LP_TEMP * c, w + WS_DDELTA_LP_TEMP * c);
+  swr_kt_end(S, KT_BO);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }

@@ -28,7 +28,7 @@ int swr_qeq_mixed_solve(swr_system *S);
 
 namespace {
 
-constexpr int MARK_SWEEPS = 64;
+constexpr int MARK_SWEEPS = 1024;
 
 int copy_grow(void **p, size_t old_bytes, size_t new_bytes, hipStream_t s) {
   void *np = nullptr;
@@ -71,6 +71,36 @@ void fill_params(swr_system *S, const std::vector<int> &map) {
       d.p_boc4 = p.p_boc4; d.p_boc5 = p.p_boc5; d.v13cor = p.v13cor; d.ovc = p.ovc;
     }
   }
+  // bond-order radius per pair (host bisection, widened by 1e-9 relative)
+  for (int e = 0; e < ff.ne() && e < SWR_MAX_ELEM; e++)
+    for (int f = 0; f < ff.ne() && f < SWR_MAX_ELEM; f++) {
+      const FfAtom &a = ff.atom[e], &b = ff.atom[f];
+      const FfPair &p = ff.tb(e, f);
+      const bool es = a.r_s > 0.0 && b.r_s > 0.0, ep = a.r_pi > 0.0 && b.r_pi > 0.0,
+                 epp = a.r_pi_pi > 0.0 && b.r_pi_pi > 0.0;
+      const bool mono = (!es || (p.p_bo1 < 0 && p.p_bo2 > 0 && p.r_s > 0)) &&
+                        (!ep || (p.p_bo3 < 0 && p.p_bo4 > 0 && p.r_p > 0)) &&
+                        (!epp || (p.p_bo5 < 0 && p.p_bo6 > 0 && p.r_pp > 0));
+      auto bop = [&](double d) {
+        double v = 0.0;
+        if (es) v += (1.0 + ff.bo_cut) * std::exp(p.p_bo1 * std::pow(d / p.r_s, p.p_bo2));
+        if (ep) v += std::exp(p.p_bo3 * std::pow(d / p.r_p, p.p_bo4));
+        if (epp) v += std::exp(p.p_bo5 * std::pow(d / p.r_pp, p.p_bo6));
+        return v;
+      };
+      double r;
+      if (!es && !ep && !epp) r = -1.0;
+      else if (!mono || bop(S->ctl.bond_cut) >= ff.bo_cut) r = S->ctl.bond_cut;
+      else {
+        double lo = 0.0, hi = S->ctl.bond_cut;
+        for (int it = 0; it < 200; it++) {
+          const double mid = 0.5 * (lo + hi);
+          if (bop(mid) >= ff.bo_cut) lo = mid; else hi = mid;
+        }
+        r = std::min(S->ctl.bond_cut, hi * (1.0 + 1e-9) + 1e-12);
+      }
+      h.rbo[e * SWR_MAX_ELEM + f] = r;
+    }
   h.p_boc1 = ff.gp[0];
   h.p_boc2 = ff.gp[1];
   h.p_lp1 = ff.gp[15];
@@ -256,7 +286,7 @@ int swr_destroy(swr_system *S) {
                   S->Cdbopi2, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,
                   S->bond_mark2, S->counts, S->ws, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32,
                   S->Hdia_inv, S->b_s, S->st, S->s_hist, S->t_hist, S->cgv, S->cgs, S->partials, S->counters,
-                  S->status, S->scan_tmp, S->tmp_i, S->mixw};
+                  S->status, S->scan_tmp, S->tmp_i, S->mixw, S->cand, S->cand_count};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (S->status_host) (void)hipHostFree(S->status_host);
@@ -422,6 +452,22 @@ int swr_sparse_matvec(swr_system *S, const double *x_dev, double *b_dev) {
   return swr_qeq_spmv(S, x_dev, b_dev);
 }
 
+int swr_sparse_matvec_h(swr_system *S, const double *x_host, double *b_host) {
+  if (!S || !S->H_val || !x_host || !b_host) return SWR_ERR_STATE;
+  double *xd = nullptr, *bd = nullptr;
+  int rc;
+  if ((rc = swr_alloc(&xd, S->n)) || (rc = swr_alloc(&bd, S->n))) { swr_free(xd); return rc; }
+  hipError_t e = hipMemcpyAsync(xd, x_host, sizeof(double) * S->n, hipMemcpyHostToDevice, S->stream);
+  if (e == hipSuccess) rc = swr_qeq_spmv(S, xd, bd);
+  if (e == hipSuccess && rc == SWR_OK)
+    e = hipMemcpyAsync(b_host, bd, sizeof(double) * S->n, hipMemcpyDeviceToHost, S->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(S->stream);
+  swr_free(xd);
+  swr_free(bd);
+  if (e != hipSuccess) { swr_set_error_msg(hipGetErrorString(e)); return SWR_ERR_HIP; }
+  return rc;
+}
+
 int swr_qeq_cg(swr_system *S, int *iters_s, int *iters_t) {
   if (!S || !S->H_val) return SWR_ERR_STATE;
   int rc = S->mixed ? swr_qeq_mixed_solve(S) : swr_qeq_cg_run(S, S->status_host->cg_iters[0] + 2);
@@ -463,6 +509,11 @@ int swr_sync(swr_system *S) {
     HIP_CHECK(hipStreamSynchronize(S->stream));
   }
   return status_to_rc(f);
+}
+
+int swr_device_synchronize(void) {
+  HIP_CHECK(hipDeviceSynchronize());
+  return SWR_OK;
 }
 
 // ---------------------------------------------------------------- mirrors
@@ -523,6 +574,7 @@ bool lookup(swr_system *S, const std::string &k, View &v) {
   if (k == "s") return set(S->st, 8 * n, 1);
   if (k == "t") return set(S->st, 8 * n, 2);
   if (k == "s_hist") return set(S->s_hist, 40 * n);
+  if (k == "bo_radius") return set(S->hp.rbo, sizeof(S->hp.rbo), 3);
   if (k == "t_hist") return set(S->t_hist, 40 * n);
   return false;
 }
@@ -549,7 +601,9 @@ int64_t swr_get(swr_system *S, const char *name, void *out) {
   if (!lookup(S, k, v)) return -1;
   if (!out || v.bytes <= 0 || !v.ptr) return v.bytes < 0 ? 0 : v.bytes;
   (void)hipStreamSynchronize(S->stream);
-  if (v.stride_kind == 0) {
+  if (v.stride_kind == 3) {  // host-side table
+    std::memcpy(out, v.ptr, v.bytes);
+  } else if (v.stride_kind == 0) {
     HIP_CHECK(hipMemcpy(out, v.ptr, v.bytes, hipMemcpyDeviceToHost));
   } else {
     int64_t cnt = v.bytes / 8;

@@ -72,12 +72,13 @@ __global__ void k_reset_apply(int n, int N, const int *__restrict__ hflag, const
   if (H >= 0) hbond_index[H] = hbond_end[H] = hstart[i];
 }
 
-// One bond found by a wave, kept in LDS until the per-atom sort.
+// Per atom, the far-list slots of its bond candidates (d <= bond_cut and
+// inside the pair's BO' radius); the bond pass evaluates them.
+constexpr int MCAND = 64;
+
 struct BondRec {
-  double d, dvec0, dvec1, dvec2;
-  double BO, BO_s, pi, pi2, C12, C34, C56;
-  int nbr;
-  int ej;
+  double d, dx, dy, dz, BO;
+  int nbr, ej;
 };
 
 __device__ __forceinline__ void store_far(swr_far_neighbor_t *dst, int j, int tj, int tagj, double d, double qj,
@@ -90,24 +91,41 @@ __device__ __forceinline__ void store_far(swr_far_neighbor_t *dst, int j, int tj
   pd[3] = make_double2(dy, dz);
 }
 
-// MODE 0: far list only; MODE 1: far + bonds + hbonds from the Verlet list;
-// MODE 2: bonds + hbonds from an existing far list (no far writes);
-// MODE 3: count bonds and hbonds only (setup-time capacity estimate).
-template <int MODE>
+// Uncorrected bond order of one pair (BOp_single, reaxc_forces_sunway.cpp:695-717).
+struct BOpTerms { double BO, BO_s, pi, pi2, C12, C34, C56; };
+__device__ __forceinline__ BOpTerms bop_terms(const DevParams *__restrict__ P, int ei, int ej, double d) {
+  const DevTbp &tw = P->tbp[ei * SWR_MAX_ELEM + ej];
+  BOpTerms t;
+  if (P->r_s[ei] > 0.0 && P->r_s[ej] > 0.0) {
+    t.C12 = tw.p_bo1 * pow(d / tw.r_s, tw.p_bo2);
+    t.BO_s = (1.0 + P->bo_cut) * exp(t.C12);
+  } else t.BO_s = t.C12 = 0.0;
+  if (P->r_pi[ei] > 0.0 && P->r_pi[ej] > 0.0) {
+    t.C34 = tw.p_bo3 * pow(d / tw.r_p, tw.p_bo4);
+    t.pi = exp(t.C34);
+  } else t.pi = t.C34 = 0.0;
+  if (P->r_pi_pi[ei] > 0.0 && P->r_pi_pi[ej] > 0.0) {
+    t.C56 = tw.p_bo5 * pow(d / tw.r_pp, tw.p_bo6);
+    t.pi2 = exp(t.C56);
+  } else t.pi2 = t.C56 = 0.0;
+  t.BO = t.BO_s + t.pi + t.pi2;
+  return t;
+}
+
+// Pass 1: one wave per atom streams its candidates (the Verlet list, or an
+// existing far list when SRC_FAR) 64 at a time.
+//   FAR  : write the far entries (write_reax_lists_c_para, pair_reaxc_sw64.c:276-331)
+//   CAND : record bond candidates' far slots
+//   HB   : 1 count, 2 write hbonds of local H (Init_Forces_noQEq_HB_Full_C,
+//          reaxc_forces_sw64.c:810-862)
+template <bool SRC_FAR, bool FAR, bool CAND, int HB>
 __global__ void __launch_bounds__(LT)
-k_lists(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__restrict__ atoms,
-        const long long *__restrict__ voff, const int *__restrict__ vlist, swr_far_neighbor_t *__restrict__ far,
-        int *__restrict__ far_index, int *__restrict__ far_end,
-        const int *__restrict__ num_bonds, const int *__restrict__ bond_index, int *__restrict__ bond_end,
-        swr_bond_t *__restrict__ bonds, swr_bond_order_t *__restrict__ bo_data, double *__restrict__ BO,
-        double *__restrict__ BOpi, double *__restrict__ Cdbo, double *__restrict__ Cdbopi,
-        double *__restrict__ Cdbopi2, const int *__restrict__ Hindex, const int *__restrict__ num_hbonds,
-        const int *__restrict__ hbond_index, int *__restrict__ hbond_end, swr_hbond_t *__restrict__ hbonds,
-        double *__restrict__ bo_dboc, double *__restrict__ dDeltap_self, int *__restrict__ bond_mark,
-        double *__restrict__ Deltap, double *__restrict__ Deltap_boc, int *__restrict__ bond_count,
-        int *__restrict__ hb_count, int bond_cap, int hbond_cap, DevStatus *st) {
-  __shared__ BondRec srec[LW][MB];
-  __shared__ double sdbop[LW][MB][3];
+k_far(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__restrict__ atoms,
+      const long long *__restrict__ voff, const int *__restrict__ vlist, swr_far_neighbor_t *__restrict__ far,
+      int *__restrict__ far_index, int *__restrict__ far_end, int *__restrict__ cand, int *__restrict__ cand_count,
+      const int *__restrict__ Hindex, const int *__restrict__ num_hbonds, const int *__restrict__ hbond_index,
+      int *__restrict__ hbond_end, swr_hbond_t *__restrict__ hbonds, int *__restrict__ hb_count, int hbond_cap,
+      DevStatus *st) {
   const int w = wave_in_block();
   const int blk = xcd_block(gridDim.x);
   const int i = blk * LW + w;
@@ -115,96 +133,76 @@ k_lists(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__
   const int lane = lane_id();
   const swr_atom_pack_t ai = atoms[i];
   const int ei = ai.type;
-  const double nonb_cutsq = P->nonb_cutsq, bond_cut = P->bond_cut, hbond_cut = P->hbond_cut, bo_cut = P->bo_cut;
-
+  const double nonb_cutsq = P->nonb_cutsq, bond_cut = P->bond_cut, hbond_cut = P->hbond_cut;
   long long cbeg, cend;
-  if (MODE == 2) { cbeg = far_index[i]; cend = far_end[i]; }
+  if (SRC_FAR) { cbeg = far_index[i]; cend = far_end[i]; }
   else { cbeg = voff[i]; cend = voff[i + 1]; }
-  const long long far0 = (MODE == 2) ? cbeg : voff[i];
-  int nfar = 0;
-  int nb = 0;
-  const bool doH = (MODE != 0) && (i < n) && ei >= 0 && P->p_hbond[ei] == 1 && hbond_cut > 0;
-  int H = (doH && MODE != 3) ? Hindex[i] : -1;
-  int htop = (doH && MODE != 3) ? hbond_end[H] : 0;
+  const long long far0 = SRC_FAR ? cbeg : voff[i];
+  int nfar = 0, ncand = 0;
+  const bool doH = HB != 0 && (i < n) && ei >= 0 && P->p_hbond[ei] == 1 && hbond_cut > 0;
+  const int H = (doH && HB == 2) ? Hindex[i] : -1;
+  int htop = (doH && HB == 2) ? hbond_end[H] : 0;
   // capacity of this H's slots, never past the end of the list
-  const int hlim = (doH && MODE != 3) ? min(hbond_index[H] + num_hbonds[i], hbond_cap) : 0;
+  const int hlim = (doH && HB == 2) ? min(hbond_index[H] + num_hbonds[i], hbond_cap) : 0;
+  const double *rbo = P->rbo + (ei >= 0 ? ei : 0) * SWR_MAX_ELEM;
+  int *mycand = cand + (size_t)i * MCAND;
 
+  int jnext = 0;
+  if (!SRC_FAR && cbeg + lane < cend) jnext = vlist[cbeg + lane];
   for (long long base = cbeg; base < cend; base += SWR_WAVE) {
     const long long k = base + lane;
     const bool valid = k < cend;
     int j = 0, tj = -1, tagj = 0;
-    double dx = 0, dy = 0, dz = 0, dsq = 0, d = 0, qj = 0;
+    double dx = 0, dy = 0, dz = 0, d = 0, qj = 0;
     bool infar = false;
     long long slot = 0;
-    if (MODE == 2) {
+    if (SRC_FAR) {
       if (valid) {
-        const swr_far_neighbor_t e = far[k];
-        j = e.nbr; tj = e.type; d = e.d; dx = e.dvec[0]; dy = e.dvec[1]; dz = e.dvec[2];
+        const double2 *e = reinterpret_cast<const double2 *>(far + k);
+        const int4 h0 = *reinterpret_cast<const int4 *>(far + k);
+        const double2 e1 = e[1], e3 = e[3];
+        const double2 e2 = e[2];
+        j = h0.x; tj = h0.y; d = e1.y; dx = e2.y; dy = e3.x; dz = e3.y;
         infar = true;
         slot = k;
       }
-    } else if (valid) {
-      j = vlist[k];
-      const swr_atom_pack_t aj = atoms[j];
-      dx = aj.x[0] - ai.x[0]; dy = aj.x[1] - ai.x[1]; dz = aj.x[2] - ai.x[2];
-      dsq = dx * dx + dy * dy + dz * dz;
-      infar = dsq <= nonb_cutsq;
-      if (infar) {
-        d = sqrt(dsq);
-        tj = aj.type; tagj = aj.orig_id; qj = aj.q;
+    } else {
+      j = jnext;
+      if (k + SWR_WAVE < cend) jnext = vlist[k + SWR_WAVE];
+      if (valid) {
+        const swr_atom_pack_t aj = atoms[j];
+        dx = aj.x[0] - ai.x[0]; dy = aj.x[1] - ai.x[1]; dz = aj.x[2] - ai.x[2];
+        const double dsq = dx * dx + dy * dy + dz * dz;
+        infar = dsq <= nonb_cutsq;
+        if (infar) {
+          d = sqrt(dsq);
+          tj = aj.type; tagj = aj.orig_id; qj = aj.q;
+        }
       }
-    }
-    if (MODE == 0 || MODE == 1) {
       const unsigned long long m = __ballot(infar);
       slot = far0 + nfar + ballot_rank(m);
-      if (infar) store_far(far + slot, j, tj, tagj, d, qj, dx, dy, dz);
+      if (FAR && infar) store_far(far + slot, j, tj, tagj, d, qj, dx, dy, dz);
       nfar += __popcll(m);
     }
-    if (MODE == 0) continue;
-    // ---- bonds: uncorrected bond order (BOp_single)
-    bool isbond = false;
-    BondRec rec;
-    if (infar && ei >= 0 && tj >= 0 && d <= bond_cut) {
-      const DevTbp tw = P->tbp[ei * SWR_MAX_ELEM + tj];
-      double C12, C34, C56, BO_s, BO_pi, BO_pi2;
-      if (P->r_s[ei] > 0.0 && P->r_s[tj] > 0.0) {
-        C12 = tw.p_bo1 * pow(d / tw.r_s, tw.p_bo2);
-        BO_s = (1.0 + bo_cut) * exp(C12);
-      } else BO_s = C12 = 0.0;
-      if (P->r_pi[ei] > 0.0 && P->r_pi[tj] > 0.0) {
-        C34 = tw.p_bo3 * pow(d / tw.r_p, tw.p_bo4);
-        BO_pi = exp(C34);
-      } else BO_pi = C34 = 0.0;
-      if (P->r_pi_pi[ei] > 0.0 && P->r_pi_pi[tj] > 0.0) {
-        C56 = tw.p_bo5 * pow(d / tw.r_pp, tw.p_bo6);
-        BO_pi2 = exp(C56);
-      } else BO_pi2 = C56 = 0.0;
-      const double BOt = BO_s + BO_pi + BO_pi2;
-      if (BOt >= bo_cut) {
-        isbond = true;
-        rec.d = d; rec.dvec0 = dx; rec.dvec1 = dy; rec.dvec2 = dz;
-        rec.BO = BOt; rec.BO_s = BO_s; rec.pi = BO_pi; rec.pi2 = BO_pi2;
-        rec.C12 = C12; rec.C34 = C34; rec.C56 = C56;
-        rec.nbr = j; rec.ej = tj;
+    if (CAND) {
+      const bool iscand = infar && ei >= 0 && tj >= 0 && d <= bond_cut && d <= rbo[tj];
+      const unsigned long long cm = __ballot(iscand);
+      if (iscand) {
+        const int c = ncand + ballot_rank(cm);
+        if (c < MCAND) mycand[c] = (int)slot;
       }
+      ncand += __popcll(cm);
     }
-    const unsigned long long bm = __ballot(isbond);
-    if (MODE != 3 && isbond) {
-      const int r = nb + ballot_rank(bm);
-      if (r < MB) srec[w][r] = rec;
-    }
-    nb += __popcll(bm);
-    // ---- hbonds of a local H: O/N partners within hbond_cut
     if (doH) {
       bool ish = false;
       if (infar && d <= hbond_cut) {
-        const int tjj = (MODE == 2) ? atoms[j].type : tj;
+        const int tjj = SRC_FAR ? atoms[j].type : tj;
         ish = tjj >= 0 && P->p_hbond[tjj] == 2;
       }
       const unsigned long long hm = __ballot(ish);
-      if (ish) {
+      if (HB == 2 && ish) {
         const int pos = htop + ballot_rank(hm);
-        if (MODE != 3 && pos < hlim) {
+        if (pos < hlim) {
           swr_hbond_t hb;
           hb.nbr = j;
           hb.scl = 1;
@@ -215,30 +213,79 @@ k_lists(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__
       htop += __popcll(hm);
     }
   }
-  if ((MODE == 0 || MODE == 1) && lane == 0) {
-    far_index[i] = (int)far0;
-    far_end[i] = (int)(far0 + nfar);
-  }
-  if (MODE == 0) return;
-  if (MODE == 3) {
-    if (lane == 0) {
-      bond_count[i] = nb;
-      if (doH) hb_count[i] = htop;
+  if (lane == 0) {
+    if (!SRC_FAR) {
+      far_index[i] = (int)far0;
+      far_end[i] = (int)(far0 + nfar);
     }
-    return;
-  }
-  if (doH && lane == 0) {
-    if (htop > hlim) {
-      atomicOr(&st->flags, SWR_F_HBOND_CAP);
-      atomicMax(&st->need_hbond_cap, htop - hbond_index[H]);
+    if (CAND) {
+      cand_count[i] = ncand;
+      if (ncand > MCAND) atomicOr(&st->flags, SWR_F_BOND_LDS);
     }
-    hbond_end[H] = htop > hlim ? hlim : htop;
-    hb_count[i] = htop - hbond_index[H];
+    if (doH) {
+      if (HB == 1) {
+        hb_count[i] = htop;
+      } else {
+        if (htop > hlim) {
+          atomicOr(&st->flags, SWR_F_HBOND_CAP);
+          atomicMax(&st->need_hbond_cap, htop - hbond_index[H]);
+        }
+        hbond_end[H] = htop > hlim ? hlim : htop;
+        hb_count[i] = htop - hbond_index[H];
+      }
+    }
   }
-  // ---- finish bonds: derivatives, per-atom sums in candidate order, sort
+}
+
+// Pass 2: one wave per atom evaluates its bond candidates (one per lane),
+// keeps BO' >= bo_cut in candidate (= far-list) order, sorts the bonds by
+// nbr (my_bubble_sort, reaxc_forces_sw64.c:31-75) and writes the bond
+// records, the per-atom sums and BO section 1.
+template <bool COUNT_ONLY>
+__global__ void __launch_bounds__(LT)
+k_bonds(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__restrict__ atoms,
+        const swr_far_neighbor_t *__restrict__ far, const int *__restrict__ cand,
+        const int *__restrict__ cand_count, const int *__restrict__ num_bonds, const int *__restrict__ bond_index,
+        int *__restrict__ bond_end, swr_bond_t *__restrict__ bonds, swr_bond_order_t *__restrict__ bo_data,
+        double *__restrict__ BO, double *__restrict__ BOpi, double *__restrict__ Cdbo, double *__restrict__ Cdbopi,
+        double *__restrict__ Cdbopi2, double *__restrict__ bo_dboc, double *__restrict__ dDeltap_self,
+        int *__restrict__ bond_mark, double *__restrict__ Deltap, double *__restrict__ Deltap_boc,
+        int *__restrict__ bond_count, int bond_cap, DevStatus *st) {
+  __shared__ BondRec srec[LW][MB];
+  __shared__ double sdbop[LW][MB][3];
+  const int w = wave_in_block();
+  const int blk = xcd_block(gridDim.x);
+  const int i = blk * LW + w;
+  if (i >= N) return;
+  const int lane = lane_id();
+  const int ei = atoms[i].type;
+  const double bo_cut = P->bo_cut;
+  const int ncand = min(cand_count[i], MCAND);
+  const int *mycand = cand + (size_t)i * MCAND;
+  int nb = 0;
+  for (int c0 = 0; c0 < ncand; c0 += SWR_WAVE) {
+    const int c = c0 + lane;
+    bool isb = false;
+    BondRec r;
+    if (c < ncand && ei >= 0) {
+      const swr_far_neighbor_t e = far[mycand[c]];
+      const BOpTerms t = bop_terms(P, ei, e.type, e.d);
+      if (t.BO >= bo_cut) {
+        isb = true;
+        r.d = e.d; r.dx = e.dvec[0]; r.dy = e.dvec[1]; r.dz = e.dvec[2]; r.BO = t.BO; r.nbr = e.nbr; r.ej = e.type;
+      }
+    }
+    const unsigned long long bm = __ballot(isb);
+    if (!COUNT_ONLY && isb) {
+      const int slot = nb + ballot_rank(bm);
+      if (slot < MB) srec[w][slot] = r;
+    }
+    nb += __popcll(bm);
+  }
+  if (lane == 0) bond_count[i] = nb;
+  if (COUNT_ONLY) return;
   const int cap = num_bonds[i];
   const int start = bond_index[i];
-  if (lane == 0) bond_count[i] = nb;
   if (nb > MB || nb > cap || start + nb > bond_cap) {
     if (lane == 0) {
       atomicOr(&st->flags, nb > MB ? (SWR_F_BOND_LDS | SWR_F_BOND_CAP) : SWR_F_BOND_CAP);
@@ -247,31 +294,29 @@ k_lists(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__
     }
     return;
   }
-  wave_lds_sync();  // LDS records of this wave visible to all its lanes
-  double dbop[3] = {0, 0, 0};
-  int rank = 0;
-  BondRec rc;
+  wave_lds_sync();
   if (lane < nb) {
-    rc = srec[w][lane];
-    const DevTbp tw = P->tbp[ei * SWR_MAX_ELEM + rc.ej];
+    const BondRec rc = srec[w][lane];
+    const BOpTerms t = bop_terms(P, ei, rc.ej, rc.d);
+    const DevTbp &tw = P->tbp[ei * SWR_MAX_ELEM + rc.ej];
     const double r2 = rc.d * rc.d;
-    const double Cs = tw.p_bo2 * rc.C12 / r2, Cp = tw.p_bo4 * rc.C34 / r2, Cpp = tw.p_bo6 * rc.C56 / r2;
-    const double dv[3] = {rc.dvec0, rc.dvec1, rc.dvec2};
-    const double cdb = -(rc.BO_s * Cs + rc.pi * Cp + rc.pi2 * Cpp);
+    const double Cs = tw.p_bo2 * t.C12 / r2, Cp = tw.p_bo4 * t.C34 / r2, Cpp = tw.p_bo6 * t.C56 / r2;
+    const double dv[3] = {rc.dx, rc.dy, rc.dz};
+    const double cdb = -(t.BO_s * Cs + t.pi * Cp + t.pi2 * Cpp);
     swr_bond_order_t o;
-    o.BO_s = rc.BO_s - bo_cut;
+    o.BO_s = t.BO_s - bo_cut;
     for (int c = 0; c < 3; c++) {
-      o.dln_BOp_s[c] = -rc.BO_s * Cs * dv[c];
-      o.dln_BOp_pi[c] = -rc.pi * Cp * dv[c];
-      o.dln_BOp_pi2[c] = -rc.pi2 * Cpp * dv[c];
+      o.dln_BOp_s[c] = -t.BO_s * Cs * dv[c];
+      o.dln_BOp_pi[c] = -t.pi * Cp * dv[c];
+      o.dln_BOp_pi2[c] = -t.pi2 * Cpp * dv[c];
       o.dBOp[c] = cdb * dv[c];
-      dbop[c] = o.dBOp[c];
       sdbop[w][lane][c] = o.dBOp[c];
     }
     o.C1dbo = o.C2dbo = o.C3dbo = 0.0;
     o.C1dbopi = o.C2dbopi = o.C3dbopi = o.C4dbopi = 0.0;
     o.C1dbopi2 = o.C2dbopi2 = o.C3dbopi2 = o.C4dbopi2 = 0.0;
     o.CdboReduction = nullptr;
+    int rank = 0;
     for (int m = 0; m < nb; m++) rank += (srec[w][m].nbr < rc.nbr) ? 1 : 0;
     const int p = start + rank;
     swr_bond_t b;
@@ -282,9 +327,9 @@ k_lists(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__
     b.sym_index = 0;
     bonds[p] = b;
     bo_data[p] = o;
-    BO[p] = rc.BO - bo_cut;
-    BOpi[2 * (size_t)p] = rc.pi;
-    BOpi[2 * (size_t)p + 1] = rc.pi2;
+    BO[p] = t.BO - bo_cut;
+    BOpi[2 * (size_t)p] = t.pi;
+    BOpi[2 * (size_t)p + 1] = t.pi2;
     Cdbo[p] = 0.0; Cdbopi[p] = 0.0; Cdbopi2[p] = 0.0;
   }
   wave_lds_sync();
@@ -310,10 +355,8 @@ k_lists(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__
       Deltap_boc[i] = tot - P->valency_val[ei];
     }
   }
-  (void)dbop;
 }
 
-// Validate_Lists_C capacity update (reaxc_forces_sw64.c:880-929).
 // Counts come from the list pass (bond_count/hb_count hold the true
 // per-atom totals even when a capacity overflowed, so the next capacities
 // are large enough to retry the step).
@@ -340,12 +383,17 @@ __global__ void k_validate(int n, int N, const int *__restrict__ bond_index, con
 // bond_mark[i] = min(bond_mark[i], bond_mark[j]+1).  Its result for ghost i
 // is min(1000, 1 + min over bonded j < i of the result at j) (locals stay 0,
 // later ghosts still hold 1000); Jacobi sweeps over that recursion reach the
-// same fixed point.  flags[3]: sweep k reads flags[(k+2)%3] (changed in
-// sweep k-1), sets flags[k%3], clears flags[(k+1)%3].
+// same fixed point.  flags[0..2]: sweep k reads flags[(k+2)%3] (changed in
+// sweep k-1), sets flags[k%3], clears flags[(k+1)%3].  flags[3] turns 1 once
+// a sweep finds that the previous one changed nothing; later sweeps skip.
 __global__ void k_bond_mark(int n, int N, int sweep, const int *__restrict__ bond_index,
                             const int *__restrict__ bond_end, const swr_bond_t *__restrict__ bonds,
                             const int *__restrict__ src, int *__restrict__ dst, int *__restrict__ flags) {
-  if (flags[(sweep + 2) % 3] == 0) return;
+  if (flags[3]) return;
+  if (flags[(sweep + 2) % 3] == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) { flags[3] = 1; flags[4] = sweep; }
+    return;
+  }
   int g = n + blockIdx.x * blockDim.x + threadIdx.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) flags[(sweep + 1) % 3] = 0;
   if (g >= N) return;
@@ -384,12 +432,15 @@ __global__ void k_estimate(int n, int N, const int *__restrict__ bond_count, con
 
 __global__ void k_mark_init(int *__restrict__ flags, int *__restrict__ a, int *__restrict__ b, int N) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) { flags[0] = 0; flags[1] = 0; flags[2] = 1; }
+  if (i == 0) { flags[0] = 0; flags[1] = 0; flags[2] = 1; flags[3] = 0; flags[4] = 0; }
   if (i < N) b[i] = a[i];
 }
 
-__global__ void k_mark_finish(const int *__restrict__ flags, int last_sweep, DevStatus *st) {
-  if (flags[last_sweep % 3]) atomicOr(&st->flags, SWR_F_MARK_DEPTH);
+__global__ void k_set_flag(DevStatus *st, unsigned int f) { atomicOr(&st->flags, f); }
+
+// Reports whether the sweeps converged and at which sweep (host reads it).
+__global__ void k_mark_finish(const int *__restrict__ flags, DevStatus *st) {
+  st->mark_changed = flags[3] ? flags[4] : -1;
 }
 
 }  // namespace
@@ -409,6 +460,7 @@ int swr_lists_reset(swr_system *S) {
   int rc = swr_ensure_tmp(S, sizeof(int) * (4 * (size_t)N + 8));
   if (rc) return rc;
   int *hflag = S->tmp_i, *hcap = hflag + N + 1, *hpos = hcap + N + 1, *hstart = hpos + N + 1;
+  swr_kt_begin(S, KT_RESET);
   hipLaunchKernelGGL(k_reset_flags, dim3(div_up(N, 256)), dim3(256), 0, S->stream, S->packed, S->dp, n, N,
                      S->ctl.hbond_cut, S->num_hbonds, hflag, hcap);
   if ((rc = swr_scan_i32(S, hflag, hpos, N, &S->status->numH))) return rc;
@@ -417,6 +469,7 @@ int swr_lists_reset(swr_system *S) {
   hipLaunchKernelGGL(k_reset_apply, dim3(div_up(N, 256)), dim3(256), 0, S->stream, n, N, hflag, hpos, hstart,
                      S->Hindex, S->bond_index, S->bond_end, S->hbond_index, S->hbond_end, S->status, S->bond_cap,
                      S->hbond_cap);
+  swr_kt_end(S, KT_RESET);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }
@@ -425,21 +478,38 @@ int swr_lists_run(swr_system *S, int mode) {
   const int N = S->N;
   const int nblk = div_up(N, LW);
   double *ws = S->ws;
-#define LIST_ARGS                                                                                             \
-  S->n, N, S->dp, S->packed, S->voff, S->vlist, S->far, S->far_index, S->far_end, S->num_bonds, S->bond_index, \
-      S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, S->Cdbo, S->Cdbopi, S->Cdbopi2, S->Hindex,           \
-      S->num_hbonds, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,      \
-      ws + (size_t)WS_DELTAP * S->Ncap, ws + (size_t)WS_DELTAP_BOC * S->Ncap, S->counts,                 \
-      S->counts + S->Ncap, S->bond_cap, S->hbond_cap, S->status
-  if (mode == 0)
-    hipLaunchKernelGGL(k_lists<0>, dim3(nblk), dim3(LT), 0, S->stream, LIST_ARGS);
-  else if (mode == 1)
-    hipLaunchKernelGGL(k_lists<1>, dim3(nblk), dim3(LT), 0, S->stream, LIST_ARGS);
-  else if (mode == 2)
-    hipLaunchKernelGGL(k_lists<2>, dim3(nblk), dim3(LT), 0, S->stream, LIST_ARGS);
-  else
-    hipLaunchKernelGGL(k_lists<3>, dim3(nblk), dim3(LT), 0, S->stream, LIST_ARGS);
-#undef LIST_ARGS
+  int rc = SWR_OK;
+  if (mode != 0 && (size_t)N * MCAND > S->cand_cap) {
+    if ((rc = swr_alloc(&S->cand, (size_t)S->Ncap * MCAND)) || (rc = swr_alloc(&S->cand_count, S->Ncap))) return rc;
+    S->cand_cap = (size_t)S->Ncap * MCAND;
+  }
+  int *hb_count = S->counts + S->Ncap;
+#define FAR_ARGS(k)                                                                                              \
+  hipLaunchKernelGGL(k, dim3(nblk), dim3(LT), 0, S->stream, S->n, N, S->dp, S->packed, S->voff, S->vlist, S->far, \
+                     S->far_index, S->far_end, S->cand, S->cand_count, S->Hindex, S->num_hbonds, S->hbond_index,  \
+                     S->hbond_end, S->hbonds, hb_count, S->hbond_cap, S->status)
+#define BOND_ARGS(k)                                                                                              \
+  hipLaunchKernelGGL(k, dim3(nblk), dim3(LT), 0, S->stream, S->n, N, S->dp, S->packed, S->far, S->cand,          \
+                     S->cand_count, S->num_bonds, S->bond_index, S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, \
+                     S->Cdbo, S->Cdbopi, S->Cdbopi2, S->bo_dboc, S->dDeltap_self, S->bond_mark,                    \
+                     ws + (size_t)WS_DELTAP * S->Ncap, ws + (size_t)WS_DELTAP_BOC * S->Ncap, S->counts,            \
+                     S->bond_cap, S->status)
+  swr_kt_begin(S, KT_LISTS);
+  if (mode == 0) {
+    FAR_ARGS((k_far<false, true, false, 0>));
+  } else if (mode == 1) {
+    FAR_ARGS((k_far<false, true, true, 2>));
+    BOND_ARGS((k_bonds<false>));
+  } else if (mode == 2) {
+    FAR_ARGS((k_far<true, false, true, 2>));
+    BOND_ARGS((k_bonds<false>));
+  } else {
+    FAR_ARGS((k_far<false, true, true, 1>));
+    BOND_ARGS((k_bonds<true>));
+  }
+#undef FAR_ARGS
+#undef BOND_ARGS
+  swr_kt_end(S, KT_LISTS);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }
@@ -453,25 +523,47 @@ int swr_lists_estimate(swr_system *S) {
   return SWR_OK;
 }
 
+// Validate_Lists + sort_bonds.  bond_mark sweeps are launched in batches
+// sized from the previous call's depth; the host checks convergence after
+// each batch (one small read-back per call) and continues if needed.
 int swr_lists_validate(swr_system *S, int max_sweeps) {
   const int n = S->n, N = S->N;
+  swr_kt_begin(S, KT_VALIDATE);
   hipLaunchKernelGGL(k_validate, dim3(div_up(N, 256)), dim3(256), 0, S->stream, n, N, S->bond_index, S->counts,
                      S->num_bonds, S->Hindex, S->counts + S->Ncap, S->num_hbonds, S->ctl.saferzone, S->bond_cap,
                      S->status);
   if (N > n) {
-    int rc = swr_ensure_tmp(S, sizeof(int) * 4);
+    int rc = swr_ensure_tmp(S, sizeof(int) * 8);
     if (rc) return rc;
     int *flags = S->tmp_i;
     int *a = S->bond_mark, *b = S->bond_mark2;
     hipLaunchKernelGGL(k_mark_init, dim3(div_up(N, 256)), dim3(256), 0, S->stream, flags, a, b, N);
-    for (int sw = 0; sw < max_sweeps; sw++) {
-      hipLaunchKernelGGL(k_bond_mark, dim3(div_up(N - n, 256)), dim3(256), 0, S->stream, n, N, sw, S->bond_index,
-                         S->bond_end, S->bonds, (const int *)a, b, flags);
-      std::swap(a, b);  // `a` now holds the latest sweep
+    int sw = 0;
+    int batch = std::max(16, S->mark_depth_hint + 8);
+    bool converged = false;
+    while (sw < max_sweeps) {
+      for (int k = 0; k < batch && sw < max_sweeps; k++, sw++) {
+        hipLaunchKernelGGL(k_bond_mark, dim3(div_up(N - n, 256)), dim3(256), 0, S->stream, n, N, sw,
+                           S->bond_index, S->bond_end, S->bonds, (const int *)a, b, flags);
+        std::swap(a, b);  // `a` now holds the latest sweep
+      }
+      hipLaunchKernelGGL(k_mark_finish, dim3(1), dim3(1), 0, S->stream, flags, S->status);
+      int conv = -1;
+      HIP_CHECK(hipMemcpyAsync(&conv, &S->status->mark_changed, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+      HIP_CHECK(hipStreamSynchronize(S->stream));
+      if (conv >= 0) {
+        S->mark_depth_hint = conv;
+        converged = true;
+        break;
+      }
+      batch = 32;
     }
-    hipLaunchKernelGGL(k_mark_finish, dim3(1), dim3(1), 0, S->stream, flags, max_sweeps - 1, S->status);
+    if (!converged) {
+      hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, S->stream, S->status, (unsigned)SWR_F_MARK_DEPTH);
+    }
     if (a != S->bond_mark) std::swap(S->bond_mark, S->bond_mark2);
   }
+  swr_kt_end(S, KT_VALIDATE);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }

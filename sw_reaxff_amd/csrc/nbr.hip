@@ -328,6 +328,7 @@ int swr_nbr_build(swr_system *S) {
   int rc;
   if ((rc = swr_alloc(&S->bin_of, N)) || (rc = swr_alloc(&S->bp_id, N)) || (rc = swr_alloc(&S->bp_pos, N)))
     return rc;
+  swr_kt_begin(S, KT_BINS);
   HIP_CHECK(hipMemsetAsync(S->bin_count, 0, sizeof(int) * (S->nbins + 1), S->stream));
   HIP_CHECK(hipMemsetAsync(S->bin_fill, 0, sizeof(int) * (S->nbins + 1), S->stream));
   hipLaunchKernelGGL(k_bin_assign, dim3(div_up(N, 256)), dim3(256), 0, S->stream, S->x, N, S->binlo[0], S->binlo[1],
@@ -338,12 +339,15 @@ int swr_nbr_build(swr_system *S) {
                      S->bin_fill, S->bp_id);
   hipLaunchKernelGGL(k_bin_sort, dim3(div_up(S->nbins, 128)), dim3(128), 0, S->stream, S->nbins, S->bin_start,
                      S->bp_id, S->x, S->bp_pos);
+  swr_kt_end(S, KT_BINS);
   if ((rc = swr_alloc(&S->vcount, N)) || (rc = swr_alloc(&S->voff, (size_t)N + 1))) return rc;
   const double cutsq = S->cutneigh * S->cutneigh;
   const int nblk = div_up(N, WAVES_PER_BLOCK);
+  swr_kt_begin(S, KT_VERLET);
   hipLaunchKernelGGL(k_verlet<false>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->x, N, S->bin_of, S->bin_start,
                      S->bp_pos, S->bp_id, S->stencil_rows, S->nrows, S->nbin[0], S->nbin[1], S->nbin[2], cutsq,
                      S->vcount, (const long long *)nullptr, (int *)nullptr);
+  swr_kt_end(S, KT_VERLET);
   long long total = 0;
   if ((rc = swr_scan_i32_to_i64(S, S->vcount, S->voff, N, &total))) return rc;
   if (total > S->vcap) {
@@ -353,9 +357,11 @@ int swr_nbr_build(swr_system *S) {
     S->vcap = cap;
   }
   S->vtotal = total;
+  swr_kt_begin(S, KT_VERLET);
   hipLaunchKernelGGL(k_verlet<true>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->x, N, S->bin_of, S->bin_start,
                      S->bp_pos, S->bp_id, S->stencil_rows, S->nrows, S->nbin[0], S->nbin[1], S->nbin[2], cutsq,
                      S->vcount, (const long long *)S->voff, S->vlist);
+  swr_kt_end(S, KT_VERLET);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }

@@ -29,7 +29,7 @@ namespace {
 
 constexpr int QT = 256;                    // threads per block
 constexpr int QW = QT / SWR_WAVE;          // waves per block
-constexpr int ROWS_PER_WAVE = 16;
+constexpr int ROWS_PER_WAVE = 2;         // one row per half-wave in the CG kernels
 constexpr int RPB = QW * ROWS_PER_WAVE;    // rows per block (fixed: results do not depend on grid size)
 constexpr double EV_TO_KCAL_PER_MOL = 14.4;
 
@@ -101,23 +101,6 @@ k_build_H(int n, const DevParams *__restrict__ P, const double *__restrict__ x, 
 }
 
 // ---------------------------------------------------------------- SpMV
-// One wave per row; the row's entries are streamed by the 64 lanes and the
-// products summed in a fixed tree order.
-__device__ __forceinline__ double2 row_spmv2(int i, const int *__restrict__ firstnbr, const int *__restrict__ numnbrs,
-                                             const int *__restrict__ jlist, const double *__restrict__ val,
-                                             const double2 *__restrict__ xv) {
-  const int lane = lane_id();
-  const int beg = firstnbr[i], end = beg + numnbrs[i];
-  double a = 0.0, b = 0.0;
-  for (int k = beg + lane; k < end; k += SWR_WAVE) {
-    const double v = val[k];
-    const double2 xx = xv[jlist[k]];
-    a += v * xx.x;
-    b += v * xx.y;
-  }
-  return make_double2(wave_sum(a), wave_sum(b));
-}
-
 __global__ void __launch_bounds__(QT)
 k_spmv1(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
         const int *__restrict__ numnbrs, const int *__restrict__ jlist, const double *__restrict__ val,
@@ -126,10 +109,43 @@ k_spmv1(int n, const DevParams *__restrict__ P, const int *__restrict__ type, co
   if (i >= n) return;
   const int lane = lane_id();
   const int beg = firstnbr[i], end = beg + numnbrs[i];
-  double a = 0.0;
-  for (int k = beg + lane; k < end; k += SWR_WAVE) a += val[k] * xv[jlist[k]];
-  a = wave_sum(a);
+  double a0 = 0.0, a1 = 0.0;
+  int k = beg + lane;
+  for (; k + SWR_WAVE < end; k += 2 * SWR_WAVE) {
+    const int j0 = jlist[k], j1 = jlist[k + SWR_WAVE];
+    const double v0 = val[k], v1 = val[k + SWR_WAVE];
+    a0 += v0 * xv[j0];
+    a1 += v1 * xv[j1];
+  }
+  if (k < end) a0 += val[k] * xv[jlist[k]];
+  const double a = wave_sum(a0 + a1);
   if (lane == 0) b[i] = P->eta[type[i]] * xv[i] + a;
+}
+
+// Sums partials[b*8 + k] over all blocks b for k < ND with every thread of
+// the (last) block: thread t adds blocks t, t+QT, ... in order, then a fixed
+// LDS tree combines the QT sums.  The order depends only on the grid size,
+// so every run of the same system gives the same bits.
+template <int ND>
+__device__ void block_reduce_partials(const double *partials, int nblocks, double *out) {
+  __shared__ double sr[ND][QT];
+  double acc[ND];
+#pragma unroll
+  for (int k = 0; k < ND; k++) acc[k] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += QT)
+#pragma unroll
+    for (int k = 0; k < ND; k++) acc[k] += ((volatile const double *)partials)[(size_t)b * 8 + k];
+#pragma unroll
+  for (int k = 0; k < ND; k++) sr[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int s = QT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int k = 0; k < ND; k++) sr[k][threadIdx.x] += sr[k][threadIdx.x + s];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < ND; k++) out[k] = sr[k][0];
 }
 
 // ---------------------------------------------------------------- CG
@@ -157,6 +173,39 @@ __device__ void cg_after_A(CgScalars &c, double l0, double l1, int imax, double 
   c.active_a = (c.iter < imax && sqrt(l0) / c.b_norm > tol) ? 1 : 0;
 }
 
+// Half-wave (32 lanes) per row, 2 rows per wave: four independent
+// index/value/gather loads in flight per lane; fixed summation order
+// (per lane a0..a3, then a fixed 32-lane tree).
+__device__ __forceinline__ double2 row_spmv2_h(int beg, int cnt, const int *__restrict__ jlist,
+                                               const double *__restrict__ val, const double2 *__restrict__ xv) {
+  const int sl = lane_id() & 31;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+  const int end = beg + cnt;
+  int k = beg + sl;
+  for (; k + 96 < end; k += 128) {
+    const int j0 = jlist[k], j1 = jlist[k + 32], j2 = jlist[k + 64], j3 = jlist[k + 96];
+    const double v0 = val[k], v1 = val[k + 32], v2 = val[k + 64], v3 = val[k + 96];
+    const double2 x0 = xv[j0], x1 = xv[j1], x2 = xv[j2], x3 = xv[j3];
+    a0 += v0 * x0.x; b0 += v0 * x0.y;
+    a1 += v1 * x1.x; b1 += v1 * x1.y;
+    a2 += v2 * x2.x; b2 += v2 * x2.y;
+    a3 += v3 * x3.x; b3 += v3 * x3.y;
+  }
+  for (; k < end; k += 32) {
+    const double2 xx = xv[jlist[k]];
+    const double v = val[k];
+    a0 += v * xx.x;
+    b0 += v * xx.y;
+  }
+  double a = (a0 + a1) + (a2 + a3), b = (b0 + b1) + (b2 + b3);
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, SWR_WAVE);
+    b += __shfl_xor(b, o, SWR_WAVE);
+  }
+  return make_double2(a, b);
+}
+
 template <int PH>
 __global__ void __launch_bounds__(QT)
 k_cg(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
@@ -165,50 +214,45 @@ k_cg(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const
      int imax, double tol, DevStatus *st) {
   __shared__ double sred[QW][6];
   __shared__ int s_last;
-  CgScalars c[2];
-  bool act_b[2] = {true, true}, act_a[2] = {true, true};
+  CgScalars c0, c1;
+  bool b0 = false, b1 = false, a0 = true, a1 = true;
   if (PH == PH_A1 || PH == PH_K) {
-    c[0] = cgs[0];
-    c[1] = cgs[1];
-    for (int r = 0; r < 2; r++) {
-      act_b[r] = PH == PH_K ? c[r].active_b != 0 : false;
-      act_a[r] = c[r].active_a != 0;
-    }
-    if (!act_b[0] && !act_b[1] && !act_a[0] && !act_a[1]) return;  // both solves finished
+    c0 = cgs[0];
+    c1 = cgs[1];
+    b0 = PH == PH_K && c0.active_b != 0;
+    b1 = PH == PH_K && c1.active_b != 0;
+    a0 = c0.active_a != 0;
+    a1 = c1.active_a != 0;
+    if (!b0 && !b1 && !a0 && !a1) return;  // both solves finished
   }
   const int w = wave_in_block(), lane = lane_id();
   const int blk = blockIdx.x;
-  const int row0 = blk * RPB + w * ROWS_PER_WAVE;
-  // SpMV of this wave's rows; lane t keeps row t's result
-  double2 qrow = make_double2(0.0, 0.0);
+  const int i = blk * RPB + w * 2 + (lane >> 5);
+  const bool owner = (lane & 31) == 0 && i < n;
+  double2 q = make_double2(0.0, 0.0);
   if (PH != PH_A1) {
-    const double2 *src = PH == PH_S1 ? V.x : PH == PH_S2 ? V.d[0] : PH == PH_S3 ? V.d[1] : V.d[par_in];
-    for (int t = 0; t < ROWS_PER_WAVE; t++) {
-      const int i = row0 + t;
-      if (i >= n) break;
-      double2 s = row_spmv2(i, firstnbr, numnbrs, jlist, val, src);
+    const double2 *src = PH == PH_S1 ? V.x : PH == PH_S2 ? V.d[0] : PH == PH_S3 ? V.d[1] : (par_in ? V.d[1] : V.d[0]);
+    const int beg = i < n ? firstnbr[i] : 0, cnt = i < n ? numnbrs[i] : 0;
+    q = row_spmv2_h(beg, cnt, jlist, val, src);
+    if (owner) {
+      // b_i = eta_i x_i + sum_j H_ij x_j (sparse_matvec_C_para, :424-466)
       const double eta = P->eta[type[i]];
       const double2 xi = src[i];
-      // b_i = eta_i x_i + sum_j H_ij x_j (sparse_matvec_C_para, :424-466)
-      s.x = eta * xi.x + s.x;
-      s.y = eta * xi.y + s.y;
-      if (lane == t) qrow = s;
+      q.x = eta * xi.x + q.x;
+      q.y = eta * xi.y + q.y;
     }
   }
-  // element-wise part: lane t owns row row0+t
   double dots[6] = {0, 0, 0, 0, 0, 0};
-  const int i = row0 + lane;
-  if (lane < ROWS_PER_WAVE && i < n) {
+  if (owner) {
     const double Hd = V.Hd[i];
-    double2 q = qrow;
     if (PH == PH_S1) {
       const double2 b = V.b[i];
-      double2 r = make_double2(b.x - 1 * q.x, b.y - 1 * q.y);
-      double2 u = make_double2(r.x * Hd, r.y * Hd);
+      const double2 r = make_double2(b.x - 1 * q.x, b.y - 1 * q.y);
+      const double2 u = make_double2(r.x * Hd, r.y * Hd);
       V.r[i] = r; V.u[i] = u; V.d[0][i] = u;
     } else if (PH == PH_S2) {
       V.w[i] = q;
-      double2 m = make_double2(q.x * Hd, q.y * Hd);
+      const double2 m = make_double2(q.x * Hd, q.y * Hd);
       V.m[i] = m; V.d[1][i] = m;
     } else if (PH == PH_S3) {
       const double2 b = V.b[i], u = V.u[i], r = V.r[i], ww = V.w[i], m = V.m[i];
@@ -216,55 +260,54 @@ k_cg(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const
       dots[0] = b.x * b.x; dots[1] = u.x * r.x; dots[2] = u.x * ww.x;
       dots[3] = b.y * b.y; dots[4] = u.y * r.y; dots[5] = u.y * ww.y;
     } else {
-      double *qq = (double *)&q;
       double2 xx = V.x[i], p = V.p[i], u = V.u[i], ss = V.ss[i], w2 = V.w[i], v = V.v[i], z = V.z[i], r = V.r[i];
-      double *px = (double *)&xx, *pp = (double *)&p, *pu = (double *)&u, *pss = (double *)&ss,
-             *pw = (double *)&w2, *pv = (double *)&v, *pz = (double *)&z, *pr = (double *)&r;
-      double2 dcur = PH == PH_K ? V.d[par_in][i] : make_double2(0, 0);
-      double2 dnext = make_double2(0, 0);
-      double *pdc = (double *)&dcur, *pdn = (double *)&dnext;
-      for (int s = 0; s < 2; s++) {
-        if (PH == PH_K && act_b[s]) {
-          // B: x += alpha p; p = u + p beta; ss = w + ss beta; v = d + v beta   (:1135-1141)
-          const double al = c[s].alpha_x, be = c[s].beta;
-          px[s] += al * pp[s];
-          pp[s] = pu[s] + pp[s] * be;
-          pss[s] = pw[s] + pss[s] * be;
-          pv[s] = pdc[s] + pv[s] * be;
-          // C: z = q + z beta   (:1147-1150)
-          pz[s] = qq[s] + pz[s] * be;
-        }
-        if (act_a[s]) {
-          // A of the next iteration: r -= alpha ss; u -= alpha v; w -= alpha z (:1113-1123)
-          const double al = c[s].alpha_a;
-          pr[s] -= al * pss[s];
-          pu[s] -= al * pv[s];
-          pw[s] -= al * pz[s];
-          dots[2 * s] = pu[s] * pr[s];
-          dots[2 * s + 1] = pu[s] * pw[s];
-          pdn[s] = pw[s] * Hd;
-        } else {
-          pdn[s] = pdc[s];
-        }
+      const double2 dc = PH == PH_K ? (par_in ? V.d[1] : V.d[0])[i] : make_double2(0.0, 0.0);
+      double2 dn = dc;
+      // B: x += alpha p; p = u + p beta; ss = w + ss beta; v = d + v beta   (:1135-1141)
+      // C: z = q + z beta                                                   (:1147-1150)
+      if (PH == PH_K && b0) {
+        const double al = c0.alpha_x, be = c0.beta;
+        xx.x += al * p.x; p.x = u.x + p.x * be; ss.x = w2.x + ss.x * be; v.x = dc.x + v.x * be;
+        z.x = q.x + z.x * be;
       }
-      if (PH == PH_K && (act_b[0] || act_b[1])) {
+      if (PH == PH_K && b1) {
+        const double al = c1.alpha_x, be = c1.beta;
+        xx.y += al * p.y; p.y = u.y + p.y * be; ss.y = w2.y + ss.y * be; v.y = dc.y + v.y * be;
+        z.y = q.y + z.y * be;
+      }
+      // A of the next iteration: r -= alpha ss; u -= alpha v; w -= alpha z; d = M^-1 w (:1113-1123)
+      if (a0) {
+        const double al = c0.alpha_a;
+        r.x -= al * ss.x; u.x -= al * v.x; w2.x -= al * z.x;
+        dots[0] = u.x * r.x; dots[1] = u.x * w2.x;
+        dn.x = w2.x * Hd;
+      }
+      if (a1) {
+        const double al = c1.alpha_a;
+        r.y -= al * ss.y; u.y -= al * v.y; w2.y -= al * z.y;
+        dots[2] = u.y * r.y; dots[3] = u.y * w2.y;
+        dn.y = w2.y * Hd;
+      }
+      if (PH == PH_K && (b0 || b1)) {
         V.x[i] = xx; V.p[i] = p; V.ss[i] = ss; V.v[i] = v; V.z[i] = z;
       }
-      if (act_a[0] || act_a[1]) {
+      if (a0 || a1) {
         V.r[i] = r; V.u[i] = u; V.w[i] = w2;
-        V.d[PH == PH_A1 ? 0 : (par_in ^ 1)][i] = dnext;
+        (PH == PH_A1 ? V.d[0] : (par_in ? V.d[0] : V.d[1]))[i] = dn;
       }
     }
   }
   if (PH == PH_S1 || PH == PH_S2) return;
   // ---- deterministic reduction: lanes -> wave -> block -> last block
-  const int nd = PH == PH_S3 ? 6 : 4;
+  constexpr int nd = PH == PH_S3 ? 6 : 4;
+#pragma unroll
   for (int k = 0; k < nd; k++) {
-    double s = wave_sum(dots[k]);
+    const double s = wave_sum(dots[k]);
     if (lane == 0) sred[w][k] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+#pragma unroll
     for (int k = 0; k < nd; k++) {
       double s = 0.0;
       for (int ww = 0; ww < QW; ww++) s += sred[ww][k];
@@ -275,20 +318,19 @@ k_cg(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const
     s_last = (prev == gridDim.x - 1) ? 1 : 0;
   }
   __syncthreads();
-  if (!s_last || threadIdx.x != 0) return;
+  if (!s_last) return;
   __threadfence();
   double tot[6] = {0, 0, 0, 0, 0, 0};
-  for (int b = 0; b < (int)gridDim.x; b++)
-    for (int k = 0; k < nd; k++) tot[k] += ((volatile double *)partials)[(size_t)b * 8 + k];
+  block_reduce_partials<nd>(partials, gridDim.x, tot);
+  if (threadIdx.x != 0) return;
   *counter = 0u;
   unsigned int bad = 0;
   if (PH == PH_S3) {
-    for (int s = 0; s < 2; s++) {
-      CgScalars cc = cgs[s];
-      cc.bdot = tot[3 * s]; cc.udot_r = tot[3 * s + 1]; cc.udot_w = tot[3 * s + 2];
-      cc.b_norm = sqrt(tot[3 * s]);
-      cc.sig_old = tot[3 * s + 1];
-      cc.heta = tot[3 * s + 2];
+    auto setup = [&](CgScalars &cc, double bb, double ur, double uw, int s) {
+      cc.bdot = bb; cc.udot_r = ur; cc.udot_w = uw;
+      cc.b_norm = sqrt(bb);
+      cc.sig_old = ur;
+      cc.heta = uw;
       cc.alpha_a = cc.sig_old / cc.heta;
       cc.alpha_x = 0.0;
       cc.beta = 0.0;
@@ -300,21 +342,27 @@ k_cg(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const
       cc.done = cc.active_a ? 0 : 1;
       if (cc.done) st->cg_iters[s] = cc.iter;
       if (!isfinite(cc.alpha_a) && cc.active_a) bad = 1;
-      cgs[s] = cc;
-    }
+    };
+    CgScalars s0 = cgs[0], s1 = cgs[1];
+    setup(s0, tot[0], tot[1], tot[2], 0);
+    setup(s1, tot[3], tot[4], tot[5], 1);
+    cgs[0] = s0;
+    cgs[1] = s1;
   } else {
-    for (int s = 0; s < 2; s++) {
-      CgScalars cc = c[s];
-      if (act_a[s]) {
-        cg_after_A(cc, tot[2 * s], tot[2 * s + 1], imax, tol);
+    auto fin = [&](CgScalars &cc, bool act_a, bool act_b, double l0, double l1, int s) {
+      if (act_a) {
+        cg_after_A(cc, l0, l1, imax, tol);
         if (!isfinite(cc.alpha_a) || !isfinite(cc.beta)) bad = 1;
-      } else if (act_b[s]) {
+      } else if (act_b) {
         cc.active_b = 0;
         cc.done = 1;
         st->cg_iters[s] = cc.iter;
       }
-      cgs[s] = cc;
-    }
+    };
+    fin(c0, a0, b0, tot[0], tot[1], 0);
+    fin(c1, a1, b1, tot[2], tot[3], 1);
+    cgs[0] = c0;
+    cgs[1] = c1;
   }
   if (bad) atomicOr(&st->flags, SWR_F_NAN);
 }
@@ -343,17 +391,15 @@ k_q_sums(int n, const double2 *__restrict__ st, double *__restrict__ partials, u
     s_last = atomicAdd(counter, 1u) == gridDim.x - 1;
   }
   __syncthreads();
-  if (!s_last || threadIdx.x != 0) return;
+  if (!s_last) return;
   __threadfence();
-  double s0 = 0, s1 = 0;
-  for (int k = 0; k < (int)gridDim.x; k++) {
-    s0 += ((volatile double *)partials)[(size_t)k * 8];
-    s1 += ((volatile double *)partials)[(size_t)k * 8 + 1];
-  }
+  double tot[2];
+  block_reduce_partials<2>(partials, gridDim.x, tot);
+  if (threadIdx.x != 0) return;
   *counter = 0u;
-  qsum[0] = s0;
-  qsum[1] = s1;
-  qsum[2] = s0 / s1;
+  qsum[0] = tot[0];
+  qsum[1] = tot[1];
+  qsum[2] = tot[0] / tot[1];
 }
 
 // q = s - u t for local atoms, history shift, and the ghost q refresh
@@ -393,18 +439,22 @@ CgVecs make_vecs(swr_system *S) {
 // ---------------------------------------------------------------- host side
 int swr_qeq_build_H(swr_system *S) {
   const int n = S->n;
+  swr_kt_begin(S, KT_BUILD_H);
   hipLaunchKernelGGL(k_build_H, dim3(div_up(n, QW)), dim3(QT), 0, S->stream, n, S->dp, S->x, S->type, S->owner,
                      S->voff, S->vlist, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val,
                      S->mixed ? S->H_val32 : (float *)nullptr, S->Hdia_inv, (double2 *)S->b_s, (double2 *)S->st,
                      S->s_hist, S->t_hist);
+  swr_kt_end(S, KT_BUILD_H);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }
 
 int swr_qeq_spmv(swr_system *S, const double *xv, double *b) {
   const int n = S->n;
+  swr_kt_begin(S, KT_SPMV);
   hipLaunchKernelGGL(k_spmv1, dim3(div_up(n, QW)), dim3(QT), 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
                      S->H_numnbrs, S->H_jlist, S->H_val, xv, b);
+  swr_kt_end(S, KT_SPMV);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }
@@ -414,6 +464,8 @@ static int launch_cg(swr_system *S, int ph, int par_in) {
   const int nblk = div_up(n, RPB);
   CgVecs V = make_vecs(S);
   const double tol = S->ctl.tol;
+  const int kt = ph == PH_K ? KT_CG_ITER : KT_CG_SETUP;
+  swr_kt_begin(S, kt);
 #define CG_ARGS n, S->dp, S->type, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, V, par_in, S->cgs, \
                 S->partials, S->counters, S->imax, tol, S->status
   switch (ph) {
@@ -424,6 +476,7 @@ static int launch_cg(swr_system *S, int ph, int par_in) {
     default: hipLaunchKernelGGL(k_cg<PH_K>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
   }
 #undef CG_ARGS
+  swr_kt_end(S, kt);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }
@@ -452,10 +505,12 @@ int swr_qeq_cg_run(swr_system *S, int first_batch) {
 int swr_qeq_calc_q(swr_system *S) {
   const int n = S->n;
   const int nblk = div_up(n, RPB);
+  swr_kt_begin(S, KT_QEQ_Q);
   hipLaunchKernelGGL(k_q_sums, dim3(nblk), dim3(QT), 0, S->stream, n, (const double2 *)S->st, S->partials,
                      S->counters + 1, S->partials + (size_t)S->cg_blocks * 8);
   hipLaunchKernelGGL(k_q_apply, dim3(div_up(S->N, 256)), dim3(256), 0, S->stream, n, S->N, (const double2 *)S->st,
                      S->partials + (size_t)S->cg_blocks * 8, S->owner, S->q, S->s_hist, S->t_hist);
+  swr_kt_end(S, KT_QEQ_Q);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }

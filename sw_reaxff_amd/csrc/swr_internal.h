@@ -60,6 +60,10 @@ struct DevParams {
   double valency_val[SWR_MAX_ELEM], nlp_opt[SWR_MAX_ELEM], mass[SWR_MAX_ELEM];
   int p_hbond[SWR_MAX_ELEM];
   DevTbp tbp[SWR_MAX_ELEM * SWR_MAX_ELEM];
+  // Per element pair, a distance beyond which BO' < bo_cut for certain (BO'
+  // falls monotonically with d for these parameters); bond candidates are
+  // pre-filtered by it and then tested exactly.  bond_cut when no bound holds.
+  double rbo[SWR_MAX_ELEM * SWR_MAX_ELEM];
   double p_boc1, p_boc2, p_lp1;
   double bo_cut, bond_cut, hbond_cut, nonb_cut, nonb_cutsq;
   // QEq, LAMMPS-type indexed
@@ -158,6 +162,10 @@ struct swr_system {
   double *dDeltap_self = nullptr;  // [N][3]
   int *bond_mark = nullptr, *bond_mark2 = nullptr;
   int *counts = nullptr;           // [2][Ncap]: bonds found per atom, hbonds found per H
+  int mark_depth_hint = 0;         // bond_mark sweeps needed last time
+  int *cand = nullptr;             // [Ncap][64]: far slots of bond candidates
+  int *cand_count = nullptr;
+  size_t cand_cap = 0;
   double *ws = nullptr;            // 13 arrays of N doubles, see WS_* below
   // ---- QEq
   int *H_firstnbr = nullptr, *H_numnbrs = nullptr, *H_jlist = nullptr;
@@ -188,10 +196,23 @@ struct swr_system {
   int *tmp_i = nullptr;
   size_t tmp_i_cap = 0;
 
-  // ---- timers
-  hipEvent_t ev[32];
-  bool events = false;
+  // ---- per-kernel HIP-event timers (enabled by swr_ktimer_enable)
+  bool ktime = false;
+  struct KTimer {
+    std::vector<hipEvent_t> ev;   // start/stop pairs
+    int used = 0;                 // pairs recorded since the last read
+    double ms = 0.0;
+    long long count = 0;
+  };
+  KTimer kt[16];
 };
+
+// Kernel timer slots.
+enum { KT_VERLET = 0, KT_LISTS, KT_BUILD_H, KT_CG_ITER, KT_CG_SETUP, KT_BO, KT_VALIDATE, KT_SPMV, KT_RESET,
+       KT_QEQ_Q, KT_BINS, KT_MIXED_INNER, KT_MIXED_OUTER, KT_COUNT };
+extern const char *const swr_kt_names[KT_COUNT];
+void swr_kt_begin(swr_system *S, int k);
+void swr_kt_end(swr_system *S, int k);
 
 // workspace array slots (each N doubles) for BO sections 1 and 3
 enum {

@@ -105,8 +105,6 @@ int scan_impl(swr_system *S, const TI *in, TO *out, int n, TO *total_dev) {
   return SWR_OK;
 }
 
-__global__ void k_copy_total_i64(const long long *src, long long *dst) { *dst = *src; }
-
 }  // namespace
 
 void swr_set_error_msg(const std::string &m) {
@@ -136,6 +134,61 @@ int swr_scan_i32_to_i64(swr_system *S, const int *in, long long *out, int n, lon
 // Exclusive scan of n ints into ints; total written to *total_dev (device).
 int swr_scan_i32(swr_system *S, const int *in, int *out, int n, int *total_dev) {
   return scan_impl<int, int>(S, in, out, n, total_dev);
+}
+
+const char *const swr_kt_names[KT_COUNT] = {"verlet", "lists", "build_H", "cg_iter", "cg_setup", "bo",
+                                            "validate", "spmv", "reset", "qeq_q", "bins", "mixed_inner",
+                                            "mixed_outer"};
+
+// Records a start event for kernel slot k on the system stream.
+void swr_kt_begin(swr_system *S, int k) {
+  if (!S->ktime) return;
+  swr_system::KTimer &t = S->kt[k];
+  if ((size_t)(2 * t.used + 2) > t.ev.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    t.ev.push_back(a);
+    t.ev.push_back(b);
+  }
+  (void)hipEventRecord(t.ev[2 * t.used], S->stream);
+}
+
+void swr_kt_end(swr_system *S, int k) {
+  if (!S->ktime) return;
+  swr_system::KTimer &t = S->kt[k];
+  if ((size_t)(2 * t.used + 2) > t.ev.size()) return;
+  (void)hipEventRecord(t.ev[2 * t.used + 1], S->stream);
+  t.used++;
+}
+
+extern "C" int swr_ktimer_enable(swr_system *S, int on) {
+  if (!S) return SWR_ERR_ARG;
+  S->ktime = on != 0;
+  return SWR_OK;
+}
+
+// Accumulates the recorded launches of slot `name` (synchronises) and
+// returns the total milliseconds and launch count since the last reset.
+extern "C" int swr_ktimer_read(swr_system *S, const char *name, double *ms, long long *count, int reset) {
+  if (!S || !name) return SWR_ERR_ARG;
+  int k = -1;
+  for (int i = 0; i < KT_COUNT; i++)
+    if (std::string(name) == swr_kt_names[i]) k = i;
+  if (k < 0) return SWR_ERR_ARG;
+  swr_system::KTimer &t = S->kt[k];
+  if (t.used) {
+    HIP_CHECK(hipEventSynchronize(t.ev[2 * t.used - 1]));
+    for (int u = 0; u < t.used; u++) {
+      float e = 0.f;
+      if (hipEventElapsedTime(&e, t.ev[2 * u], t.ev[2 * u + 1]) == hipSuccess) t.ms += e;
+    }
+    t.count += t.used;
+    t.used = 0;
+  }
+  if (ms) *ms = t.ms;
+  if (count) *count = t.count;
+  if (reset) { t.ms = 0; t.count = 0; }
+  return SWR_OK;
 }
 
 int swr_ensure_tmp(swr_system *S, size_t bytes) {

@@ -19,10 +19,10 @@ BO_RTOL = 1e-10
 Q_ATOL = 1e-6
 
 
-def harness(system, ffield, steps=0, every=10, dt=0.25, mixed=False, profile=False):
+def harness(system, ffield, steps=0, every=10, dt=0.25, mixed=False, profile=False, tol=1e-6):
     lmp = Lammps()
     lmp.commands(standard_script(system.elements, ffield=ffield, every=every, dt=dt, mixed=mixed,
-                                 profile=profile))
+                                 profile=profile, tol=tol))
     lmp.create_atoms(system)
     lmp.setup()
     if steps:
@@ -46,8 +46,8 @@ def per_atom(index, end, arr, i):
     return arr[index[i]:end[i]]
 
 
-def make_oracle(oracle_lib, ffield, elements, A, verlet=None, threads=8):
-    o = oracle_lib.Oracle(ffield, elements, threads=threads)
+def make_oracle(oracle_lib, ffield, elements, A, verlet=None, threads=8, tol=1e-6):
+    o = oracle_lib.Oracle(ffield, elements, threads=threads, tol=tol)
     o.set_atoms(A["n"], A["x"], A["type"], A["tag"], A["owner"], q=A["q"])
     if verlet is not None:
         o.set_verlet(*verlet)

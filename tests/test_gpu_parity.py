@@ -65,6 +65,7 @@ def test_charges_and_solver(c0_setup):
     assert np.max(np.abs(gt - o.get("t")[:n])) <= Q_ATOL
     gits = (sysh.scalar("matvecs_s"), sysh.scalar("matvecs_t"))
     assert abs(gits[0] - its[0]) <= 1 and abs(gits[1] - its[1]) <= 1, (gits, its)
+    print(f"iterations gpu {gits} oracle {its}; max|dq| {np.max(np.abs(gq - oq)):.3e}")
 
 
 def test_H_matrix(c0_setup):
