@@ -81,14 +81,22 @@ struct BondRec {
   int nbr, ej;
 };
 
-__device__ __forceinline__ void store_far(swr_far_neighbor_t *dst, int j, int tj, int tagj, double d, double qj,
-                                          double dx, double dy, double dz) {
+// Streaming 16-byte store: the far list is written once per step and read
+// back only sparsely, so it should not displace the gathered atoms in L2.
+__device__ __forceinline__ void store_nt(double2 *dst, double2 v) {
+  typedef double nd2 __attribute__((ext_vector_type(2)));
+  nd2 t = {v.x, v.y};
+  __builtin_nontemporal_store(t, reinterpret_cast<nd2 *>(dst));
+}
+
+// One far entry as four 16-byte pieces at dst (rel_box stays 0).
+__device__ __forceinline__ void stage_far(double2 *dst, int j, int tj, int tagj, double d, double qj, double dx,
+                                          double dy, double dz) {
   int4 *p = reinterpret_cast<int4 *>(dst);
   p[0] = make_int4(j, tj, tagj, 0);
   p[1] = make_int4(0, 0, __double2loint(d), __double2hiint(d));
-  double2 *pd = reinterpret_cast<double2 *>(dst);
-  pd[2] = make_double2(qj, dx);
-  pd[3] = make_double2(dy, dz);
+  dst[2] = make_double2(qj, dx);
+  dst[3] = make_double2(dy, dz);
 }
 
 // Uncorrected bond order of one pair (BOp_single, reaxc_forces_sunway.cpp:695-717).
@@ -126,8 +134,9 @@ k_far(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__re
       const int *__restrict__ Hindex, const int *__restrict__ num_hbonds, const int *__restrict__ hbond_index,
       int *__restrict__ hbond_end, swr_hbond_t *__restrict__ hbonds, int *__restrict__ hb_count, int hbond_cap,
       DevStatus *st) {
+  __shared__ double2 sfar[(FAR && !SRC_FAR) ? LW : 1][4 * SWR_WAVE];
   const int w = wave_in_block();
-  const int blk = xcd_block(gridDim.x);
+  const int blk = blockIdx.x;
   const int i = blk * LW + w;
   if (i >= N) return;
   const int lane = lane_id();
@@ -180,8 +189,20 @@ k_far(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__re
         }
       }
       const unsigned long long m = __ballot(infar);
-      slot = far0 + nfar + ballot_rank(m);
-      if (FAR && infar) store_far(far + slot, j, tj, tagj, d, qj, dx, dy, dz);
+      const int r = ballot_rank(m);
+      slot = far0 + nfar + r;
+      if (FAR) {
+        // Compact the wave's entries in LDS, then write them as contiguous
+        // 16-byte pieces (lane t writes piece t, t+64, ...): full-line
+        // stores instead of one 64-byte entry per lane.
+        double2 *sw = sfar[w];
+        if (infar) stage_far(sw + 4 * r, j, tj, tagj, d, qj, dx, dy, dz);
+        wave_lds_sync();
+        double2 *dst = reinterpret_cast<double2 *>(far + far0 + nfar);
+        const int pieces = 4 * __popcll(m);
+        for (int t = lane; t < pieces; t += SWR_WAVE) store_nt(dst + t, sw[t]);
+        wave_lds_sync();
+      }
       nfar += __popcll(m);
     }
     if (CAND) {

@@ -15,9 +15,10 @@
 // are (s,t) pairs (double2), so each H entry is read once and feeds both
 // solves.  One kernel per iteration does the reference's B and C steps of
 // iteration i (SpMV of d, x/p/ss/v/z updates) and the A step of iteration
-// i+1 (r/u/w updates, the two dots, d = M^-1 w); the two dots are reduced in
-// a fixed order by the last workgroup to finish, which also computes the
-// next iteration's scalars.  Each solve stops by the reference's test
+// i+1 (r/u/w updates, the two dots, d = M^-1 w).  Each workgroup writes its
+// partial dots; a one-workgroup kernel sums them in a fixed order and
+// computes the next iteration's scalars (no atomics, so results are the same
+// bits on every run).  Each solve stops by the reference's test
 // sqrt(u.r)/|b| <= tol (or i = imax) and is frozen while the other runs.
 #include "swr_internal.h"
 #include "swr_device.h"
@@ -122,23 +123,25 @@ k_spmv1(int n, const DevParams *__restrict__ P, const int *__restrict__ type, co
   if (lane == 0) b[i] = P->eta[type[i]] * xv[i] + a;
 }
 
-// Sums partials[b*8 + k] over all blocks b for k < ND with every thread of
-// the (last) block: thread t adds blocks t, t+QT, ... in order, then a fixed
-// LDS tree combines the QT sums.  The order depends only on the grid size,
-// so every run of the same system gives the same bits.
+constexpr int RT = 1024;  // threads of the one-workgroup reduction kernels
+
+// Sums partials[k*pstride + b] over b < nblocks for k < ND with one
+// workgroup: thread t adds b = t, t+RT, ... in order, then a fixed LDS tree
+// combines the RT sums.  The order depends only on the block count.
 template <int ND>
-__device__ void block_reduce_partials(const double *partials, int nblocks, double *out) {
-  __shared__ double sr[ND][QT];
+__device__ void reduce_partials(const double *__restrict__ partials, int pstride, int nblocks, double *out) {
+  __shared__ double sr[ND][RT];
   double acc[ND];
 #pragma unroll
   for (int k = 0; k < ND; k++) acc[k] = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += QT)
+#pragma unroll 4
+  for (int b = threadIdx.x; b < nblocks; b += RT)
 #pragma unroll
-    for (int k = 0; k < ND; k++) acc[k] += ((volatile const double *)partials)[(size_t)b * 8 + k];
+    for (int k = 0; k < ND; k++) acc[k] += partials[(size_t)k * pstride + b];
 #pragma unroll
   for (int k = 0; k < ND; k++) sr[k][threadIdx.x] = acc[k];
   __syncthreads();
-  for (int s = QT / 2; s > 0; s >>= 1) {
+  for (int s = RT / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s)
 #pragma unroll
       for (int k = 0; k < ND; k++) sr[k][threadIdx.x] += sr[k][threadIdx.x + s];
@@ -146,6 +149,28 @@ __device__ void block_reduce_partials(const double *partials, int nblocks, doubl
   }
 #pragma unroll
   for (int k = 0; k < ND; k++) out[k] = sr[k][0];
+}
+
+// Block sum of per-thread values d[k] in a fixed order (wave tree, then
+// waves in order); thread 0 writes partials[k*pstride + blk].
+template <int ND, int NT>
+__device__ __forceinline__ void block_partials(const double *d, double *__restrict__ partials, int pstride,
+                                               int blk) {
+  constexpr int NW = NT / SWR_WAVE;
+  __shared__ double sred[NW][ND];
+  const int w = wave_in_block();
+#pragma unroll
+  for (int k = 0; k < ND; k++) {
+    const double s = wave_sum(d[k]);
+    if (lane_id() == 0) sred[w][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < ND) {
+    double s = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < NW; ww++) s += sred[ww][threadIdx.x];
+    partials[(size_t)threadIdx.x * pstride + blk] = s;
+  }
 }
 
 // ---------------------------------------------------------------- CG
@@ -210,10 +235,7 @@ template <int PH>
 __global__ void __launch_bounds__(QT)
 k_cg(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
      const int *__restrict__ numnbrs, const int *__restrict__ jlist, const double *__restrict__ val, CgVecs V,
-     int par_in, CgScalars *__restrict__ cgs, double *__restrict__ partials, unsigned int *__restrict__ counter,
-     int imax, double tol, DevStatus *st) {
-  __shared__ double sred[QW][6];
-  __shared__ int s_last;
+     int par_in, const CgScalars *__restrict__ cgs, double *__restrict__ partials, int pstride) {
   CgScalars c0, c1;
   bool b0 = false, b1 = false, a0 = true, a1 = true;
   if (PH == PH_A1 || PH == PH_K) {
@@ -298,34 +320,24 @@ k_cg(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const
     }
   }
   if (PH == PH_S1 || PH == PH_S2) return;
-  // ---- deterministic reduction: lanes -> wave -> block -> last block
+  block_partials<PH == PH_S3 ? 6 : 4, QT>(dots, partials, pstride, blk);
+}
+
+// Scalars of the iteration whose dots k_cg<PH> just wrote (one workgroup).
+template <int PH>
+__global__ void __launch_bounds__(RT)
+k_cg_scalars(int nblk, const double *__restrict__ partials, int pstride, CgScalars *__restrict__ cgs, int imax,
+             double tol, DevStatus *st) {
+  CgScalars c0 = cgs[0], c1 = cgs[1];
+  const bool b0 = PH == PH_K && c0.active_b != 0, b1 = PH == PH_K && c1.active_b != 0;
+  const bool a0 = PH == PH_S3 || c0.active_a != 0, a1 = PH == PH_S3 || c1.active_a != 0;
+  if (!b0 && !b1 && !a0 && !a1) return;
   constexpr int nd = PH == PH_S3 ? 6 : 4;
-#pragma unroll
-  for (int k = 0; k < nd; k++) {
-    const double s = wave_sum(dots[k]);
-    if (lane == 0) sred[w][k] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < nd; k++) {
-      double s = 0.0;
-      for (int ww = 0; ww < QW; ww++) s += sred[ww][k];
-      partials[(size_t)blk * 8 + k] = s;
-    }
-    __threadfence();
-    const unsigned int prev = atomicAdd(counter, 1u);
-    s_last = (prev == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  double tot[6] = {0, 0, 0, 0, 0, 0};
-  block_reduce_partials<nd>(partials, gridDim.x, tot);
+  double tot[nd];
+  reduce_partials<nd>(partials, pstride, nblk, tot);
   if (threadIdx.x != 0) return;
-  *counter = 0u;
   unsigned int bad = 0;
-  if (PH == PH_S3) {
+  if constexpr (PH == PH_S3) {
     auto setup = [&](CgScalars &cc, double bb, double ur, double uw, int s) {
       cc.bdot = bb; cc.udot_r = ur; cc.udot_w = uw;
       cc.b_norm = sqrt(bb);
@@ -343,11 +355,8 @@ k_cg(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const
       if (cc.done) st->cg_iters[s] = cc.iter;
       if (!isfinite(cc.alpha_a) && cc.active_a) bad = 1;
     };
-    CgScalars s0 = cgs[0], s1 = cgs[1];
-    setup(s0, tot[0], tot[1], tot[2], 0);
-    setup(s1, tot[3], tot[4], tot[5], 1);
-    cgs[0] = s0;
-    cgs[1] = s1;
+    setup(c0, tot[0], tot[1], tot[2], 0);
+    setup(c1, tot[3], tot[4], tot[5], 1);
   } else {
     auto fin = [&](CgScalars &cc, bool act_a, bool act_b, double l0, double l1, int s) {
       if (act_a) {
@@ -361,42 +370,28 @@ k_cg(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const
     };
     fin(c0, a0, b0, tot[0], tot[1], 0);
     fin(c1, a1, b1, tot[2], tot[3], 1);
-    cgs[0] = c0;
-    cgs[1] = c1;
   }
+  cgs[0] = c0;
+  cgs[1] = c1;
   if (bad) atomicOr(&st->flags, SWR_F_NAN);
 }
 
 // ---------------------------------------------------------------- calculate_Q
+// Block partial sums of s and t over the local atoms (calculate_Q,
+// fix_qeq_reax_sunway.cpp:1733-1734: the two parallel_vector_acc calls).
 __global__ void __launch_bounds__(QT)
-k_q_sums(int n, const double2 *__restrict__ st, double *__restrict__ partials, unsigned int *__restrict__ counter,
-         double *__restrict__ qsum) {
-  __shared__ double sred[QW][2];
-  __shared__ int s_last;
-  const int w = wave_in_block(), lane = lane_id();
-  const int row0 = blockIdx.x * RPB + w * ROWS_PER_WAVE;
-  double a = 0.0, b = 0.0;
-  const int i = row0 + lane;
-  if (lane < ROWS_PER_WAVE && i < n) { a = st[i].x; b = st[i].y; }
-  a = wave_sum(a);
-  b = wave_sum(b);
-  if (lane == 0) { sred[w][0] = a; sred[w][1] = b; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s0 = 0, s1 = 0;
-    for (int k = 0; k < QW; k++) { s0 += sred[k][0]; s1 += sred[k][1]; }
-    partials[(size_t)blockIdx.x * 8] = s0;
-    partials[(size_t)blockIdx.x * 8 + 1] = s1;
-    __threadfence();
-    s_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
+k_q_sums(int n, const double2 *__restrict__ st, double *__restrict__ partials, int pstride) {
+  const int i = blockIdx.x * QT + threadIdx.x;
+  double d[2] = {0.0, 0.0};
+  if (i < n) { const double2 v = st[i]; d[0] = v.x; d[1] = v.y; }
+  block_partials<2, QT>(d, partials, pstride, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(RT)
+k_q_total(int nblk, const double *__restrict__ partials, int pstride, double *__restrict__ qsum) {
   double tot[2];
-  block_reduce_partials<2>(partials, gridDim.x, tot);
+  reduce_partials<2>(partials, pstride, nblk, tot);
   if (threadIdx.x != 0) return;
-  *counter = 0u;
   qsum[0] = tot[0];
   qsum[1] = tot[1];
   qsum[2] = tot[0] / tot[1];
@@ -462,20 +457,31 @@ int swr_qeq_spmv(swr_system *S, const double *xv, double *b) {
 static int launch_cg(swr_system *S, int ph, int par_in) {
   const int n = S->n;
   const int nblk = div_up(n, RPB);
+  const int ps = S->cg_blocks;
   CgVecs V = make_vecs(S);
   const double tol = S->ctl.tol;
   const int kt = ph == PH_K ? KT_CG_ITER : KT_CG_SETUP;
   swr_kt_begin(S, kt);
-#define CG_ARGS n, S->dp, S->type, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, V, par_in, S->cgs, \
-                S->partials, S->counters, S->imax, tol, S->status
+#define CG_ARGS n, S->dp, S->type, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, V, par_in, S->cgs, S->partials, ps
+#define SC_ARGS nblk, S->partials, ps, S->cgs, S->imax, tol, S->status
   switch (ph) {
     case PH_S1: hipLaunchKernelGGL(k_cg<PH_S1>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
     case PH_S2: hipLaunchKernelGGL(k_cg<PH_S2>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
-    case PH_S3: hipLaunchKernelGGL(k_cg<PH_S3>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
-    case PH_A1: hipLaunchKernelGGL(k_cg<PH_A1>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
-    default: hipLaunchKernelGGL(k_cg<PH_K>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
+    case PH_S3:
+      hipLaunchKernelGGL(k_cg<PH_S3>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS);
+      hipLaunchKernelGGL(k_cg_scalars<PH_S3>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS);
+      break;
+    case PH_A1:
+      hipLaunchKernelGGL(k_cg<PH_A1>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS);
+      hipLaunchKernelGGL(k_cg_scalars<PH_A1>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS);
+      break;
+    default:
+      hipLaunchKernelGGL(k_cg<PH_K>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS);
+      hipLaunchKernelGGL(k_cg_scalars<PH_K>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS);
+      break;
   }
 #undef CG_ARGS
+#undef SC_ARGS
   swr_kt_end(S, kt);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
@@ -504,15 +510,19 @@ int swr_qeq_cg_run(swr_system *S, int first_batch) {
 
 int swr_qeq_calc_q(swr_system *S) {
   const int n = S->n;
-  const int nblk = div_up(n, RPB);
+  const int nblk = div_up(n, QT);
+  const int ps = S->cg_blocks;
+  double *qsum = S->partials + (size_t)ps * 8;
   swr_kt_begin(S, KT_QEQ_Q);
-  hipLaunchKernelGGL(k_q_sums, dim3(nblk), dim3(QT), 0, S->stream, n, (const double2 *)S->st, S->partials,
-                     S->counters + 1, S->partials + (size_t)S->cg_blocks * 8);
+  hipLaunchKernelGGL(k_q_sums, dim3(nblk), dim3(QT), 0, S->stream, n, (const double2 *)S->st, S->partials, ps);
+  hipLaunchKernelGGL(k_q_total, dim3(1), dim3(RT), 0, S->stream, nblk, S->partials, ps, qsum);
   hipLaunchKernelGGL(k_q_apply, dim3(div_up(S->N, 256)), dim3(256), 0, S->stream, n, S->N, (const double2 *)S->st,
-                     S->partials + (size_t)S->cg_blocks * 8, S->owner, S->q, S->s_hist, S->t_hist);
+                     qsum, S->owner, S->q, S->s_hist, S->t_hist);
   swr_kt_end(S, KT_QEQ_Q);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }
 
+// Partial-sum slots per dot (the stride of the partials array): enough for
+// every grid the QEq kernels launch on n rows.
 int swr_qeq_blocks(int n) { return div_up(n, RPB); }
