@@ -1,0 +1,134 @@
+"""An independent reader of the ReaxFF force-field text format, for known-answer tests.
+
+It reads the raw numbers of `ffield.reax` by position, following the published
+format (van Duin's file layout, as LAMMPS' reaxc_ffield reads it).  It shares
+no code with the oracle's or the library's readers, so a test that derives
+its expected values from here also pins those readers:
+
+  * atoms: 4 lines of 8 numbers after the element name;
+  * bonds: 2 lines of 8 numbers after the two 1-based atom indices;
+  * off-diagonal: 6 numbers after the two indices (values > 0 override the
+    mixed radii r_s, r_p, r_pp).
+
+Cited reference reader: reaxc_ffield_sunway.cpp:35-518 (eta is stored as
+2x the file value at :189; the bond-order cutoff is 0.01 x gp[29] per
+pair_reaxc_sunway.cpp:202-362 and reaxc_ffield_sunway.cpp).
+"""
+from __future__ import annotations
+
+import math
+
+ATOM_KEYS = [
+    # line 1
+    "r_s", "valency", "mass", "r_vdw", "epsilon", "gamma", "r_pi", "valency_e",
+    # line 2
+    "alpha", "gamma_w", "valency_boc", "p_ovun5", "_u0", "chi", "eta_file", "p_hbond",
+    # line 3
+    "r_pi_pi", "p_lp2", "_heat", "b_o_131", "b_o_132", "b_o_133", "_u1", "_u2",
+    # line 4
+    "p_ovun2", "p_val3", "_u3", "valency_val", "p_val5", "rcore2", "ecore2", "acore2",
+]
+BOND_KEYS = [
+    "De_s", "De_p", "De_pp", "p_be1", "p_bo5", "v13cor", "p_bo6", "p_ovun1",
+    "p_be2", "p_bo3", "p_bo4", "_u0", "p_bo1", "p_bo2", "ovc", "_u1",
+]
+
+
+class FField:
+    def __init__(self, path: str):
+        with open(path) as f:
+            lines = f.read().splitlines()
+        k = 1
+        ngp = int(lines[k].split()[0])
+        self.gp = [float(lines[k + 1 + i].split()[0]) for i in range(ngp)]
+        k += 1 + ngp
+        nat = int(lines[k].split()[0])
+        k += 4
+        self.atoms = {}
+        self.order = []
+        for _ in range(nat):
+            t = lines[k].split()
+            name, vals = t[0], [float(v) for v in t[1:9]]
+            for m in range(1, 4):
+                vals += [float(v) for v in lines[k + m].split()[:8]]
+            self.atoms[name] = dict(zip(ATOM_KEYS, vals))
+            self.order.append(name)
+            k += 4
+        nb = int(lines[k].split()[0])
+        k += 2
+        self.bonds = {}
+        for _ in range(nb):
+            t = lines[k].split()
+            i, j = int(t[0]) - 1, int(t[1]) - 1
+            vals = [float(v) for v in t[2:10]] + [float(v) for v in lines[k + 1].split()[:8]]
+            d = dict(zip(BOND_KEYS, vals))
+            self.bonds[(i, j)] = d
+            self.bonds[(j, i)] = d
+            k += 2
+        no = int(lines[k].split()[0])
+        k += 1
+        self.offdiag = {}
+        for _ in range(no):
+            t = lines[k].split()
+            i, j = int(t[0]) - 1, int(t[1]) - 1
+            vals = [float(v) for v in t[2:8]]
+            self.offdiag[(i, j)] = self.offdiag[(j, i)] = dict(zip(["D", "r_vdw", "alpha", "r_s", "r_p", "r_pp"], vals))
+            k += 1
+
+    # --- derived quantities (standard ReaxFF definitions)
+    @property
+    def bo_cut(self) -> float:
+        return 0.01 * self.gp[29]
+
+    def idx(self, el: str) -> int:
+        return self.order.index(el)
+
+    def atom(self, el: str) -> dict:
+        return self.atoms[el]
+
+    def eta(self, el: str) -> float:
+        return 2.0 * self.atoms[el]["eta_file"]
+
+    def pair(self, a: str, b: str) -> dict:
+        """Mixed bond radii and the bond-order parameters of the pair."""
+        A, B = self.atoms[a], self.atoms[b]
+        ia, ib = self.idx(a), self.idx(b)
+        r_s = 0.5 * (A["r_s"] + B["r_s"])
+        r_p = 0.5 * (A["r_pi"] + B["r_pi"])
+        r_pp = 0.5 * (A["r_pi_pi"] + B["r_pi_pi"])
+        od = self.offdiag.get((ia, ib))
+        if od is not None:
+            if od["r_s"] > 0:
+                r_s = od["r_s"]
+            if od["r_p"] > 0:
+                r_p = od["r_p"]
+            if od["r_pp"] > 0:
+                r_pp = od["r_pp"]
+        bd = self.bonds[(ia, ib)]
+        return dict(bd, r_s=r_s, r_p=r_p, r_pp=r_pp,
+                    p_boc3=math.sqrt(A["b_o_132"] * B["b_o_132"]),
+                    p_boc4=math.sqrt(A["b_o_131"] * B["b_o_131"]),
+                    p_boc5=math.sqrt(A["b_o_133"] * B["b_o_133"]))
+
+    def bop(self, a: str, b: str, d: float):
+        """Uncorrected bond-order terms (sigma, pi, pi-pi) at distance d,
+        each gated on both atoms having the radius (ReaxFF BO' definition)."""
+        A, B, t = self.atoms[a], self.atoms[b], self.pair(a, b)
+        s = pi = pp = 0.0
+        if A["r_s"] > 0 and B["r_s"] > 0:
+            s = (1.0 + self.bo_cut) * math.exp(t["p_bo1"] * (d / t["r_s"]) ** t["p_bo2"])
+        if A["r_pi"] > 0 and B["r_pi"] > 0:
+            pi = math.exp(t["p_bo3"] * (d / t["r_p"]) ** t["p_bo4"])
+        if A["r_pi_pi"] > 0 and B["r_pi_pi"] > 0:
+            pp = math.exp(t["p_bo5"] * (d / t["r_pp"]) ** t["p_bo6"])
+        return s, pi, pp
+
+    def taper_closed_form(self, r: float) -> float:
+        """7th-order taper for swa = 0: 20x^7 - 70x^6 + 84x^5 - 35x^4 + 1, x = r/swb."""
+        x = r / self.gp[12]
+        return 20 * x ** 7 - 70 * x ** 6 + 84 * x ** 5 - 35 * x ** 4 + 1
+
+    def h_offdiag(self, a: str, b: str, r: float) -> float:
+        """Shielded Coulomb H entry in kcal/mol-per-e^2 units of the QEq matrix."""
+        g = (self.atoms[a]["gamma"] * self.atoms[b]["gamma"]) ** -1.5
+        return self.taper_closed_form(r) * 14.4 / (r ** 3 + g) ** (1.0 / 3.0)
