@@ -61,25 +61,49 @@ def counts(sysh):
                 bonds=int((be - bi).sum()), hbonds=int((he - hi).sum()), nnz=int(sysh.get("H_numnbrs").sum()))
 
 
+SLOTS = ("verlet", "bins", "far", "bonds", "reset", "validate", "bo", "build_H", "cg_setup", "cg_iter", "cg_scalars",
+         "qeq_q", "mixed_inner", "mixed_outer")
+
+
 def algorithmic_bytes(slot, c, mixed=False):
     """Compulsory HBM bytes of one launch of a kernel slot, from its inputs and outputs."""
     n, N = c["n"], c["N"]
     if slot == "cg_iter":
-        # H (jlist 4 B + val 8 B per entry), the (s,t) d gather once per row,
-        # 9 (s,t) vectors read and 9 written, Hdia_inv, eta type, row pointers
-        return 12 * c["nnz"] + n * (16 + 18 * 16 + 8 + 4 + 8)
-    if slot == "lists":
-        # Verlet ids + offsets, packed atoms once, far entries, bond records
+        # H (jlist 4 B + val 8 B per entry; 4 B val in the FP32 inner solve),
+        # the (s,t) d gather once per row, 9 (s,t) vectors read and 9 written,
+        # Hdia_inv, eta type, row pointers
+        return (8 if mixed else 12) * c["nnz"] + n * (16 + 18 * 16 + 8 + 4 + 8)
+    if slot == "far":
+        # Verlet ids + offsets, packed atoms once, far entries, far_index/end,
+        # hbond records, bond-candidate slots (at least one per bond) and counts
+        return (4 * c["verlet"] + 8 * (N + 1) + 40 * N + 64 * c["far"] + 8 * N + 16 * c["hbonds"]
+                + 4 * c["bonds"] + 8 * N)
+    if slot == "bonds":
+        # candidate far entries (at least the bonded ones), bond records
         # (56 bond_data + 200 bond_order_data + 8 BO + 16 BOpi + 24 Cdbo*),
-        # hbonds, per-atom outputs and capacities
-        return (4 * c["verlet"] + 8 * (N + 1) + 40 * N + 64 * c["far"] + 304 * c["bonds"] + 16 * c["hbonds"]
-                + N * (16 + 24 + 4 + 16 + 8 + 4 + 4 + 12))
+        # per-atom sums and marks
+        return 64 * c["bonds"] + 304 * c["bonds"] + N * (40 + 8 + 16 + 24 + 4 + 16 + 8)
     if slot == "build_H":
         return 4 * (c["verlet"] * n // max(N, 1)) + 24 * N + 4 * N + 12 * c["nnz"] + n * (8 + 16 + 16 + 80 + 8)
     if slot == "bo":
         return 8 * N * 20 + c["bonds"] * (56 + 200 + 8 + 16)
     if slot == "verlet":
         return 32 * N + 4 * c["verlet"] + 8 * N
+    return None
+
+
+def pmc_traffic(kernel_slot, config):
+    """Measured HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None when no summary covers this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel_slot}_{config}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        return None
     return None
 
 
@@ -139,6 +163,20 @@ def cpu_baseline(sysh, elements, ffield, lmp_dt, every, budget_s):
                 nproc=os.cpu_count(), cpu=cpu_model, verlet_build_s=round(t_verlet, 3))
 
 
+def copy_calibration(mib):
+    # A separate mode and process: torch brings its own HIP runtime, so the
+    # library is never loaded here.
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
+                      "launches": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,7 +189,13 @@ def main():
     ap.add_argument("--dt", type=float, default=0.25)
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (read MIB, write MIB) five times: the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this path has no multi-GPU form (asked for --gpus {args.gpus}).",
               file=sys.stderr)
@@ -169,10 +213,10 @@ def main():
     log(f"warmup {args.warmup} steps")
     lmp.run(args.warmup)
     sysh.ktimer_enable(True)
-    for slot in ("verlet", "bins", "lists", "reset", "validate", "bo", "build_H", "cg_setup", "cg_iter", "qeq_q",
-                 "mixed_inner", "mixed_outer"):
+    for slot in SLOTS:
         sysh.ktimer_read(slot, reset=True)
     its0 = (lmp.stat("total_iters_s"), lmp.stat("total_iters_t"))
+    work0 = lmp.stat("cg_work_launches")
     device_sync()
     log(f"timed {args.steps} steps")
     t0 = time.perf_counter()
@@ -181,9 +225,9 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_step = elapsed * 1e3 / args.steps
     its = ((lmp.stat("total_iters_s") - its0[0]) / args.steps, (lmp.stat("total_iters_t") - its0[1]) / args.steps)
+    cg_work = int(lmp.stat("cg_work_launches") - work0)
     kt = {}
-    for slot in ("verlet", "bins", "lists", "reset", "validate", "bo", "build_H", "cg_setup", "cg_iter", "qeq_q",
-                 "mixed_inner", "mixed_outer"):
+    for slot in SLOTS:
         ms, cnt = sysh.ktimer_read(slot)
         if cnt:
             kt[slot] = (ms, cnt)
@@ -191,26 +235,31 @@ def main():
     c = counts(sysh)
     n = c["n"]
     log(f"ms/step {ms_step:.3f}; CG iterations s {its[0]:.1f} t {its[1]:.1f}; kernels {kt}")
-    # dominant kernel and its roofline
+    # dominant kernel and its roofline.  Iteration launches enqueued after
+    # both solves converged exit at once; only the launches that did an
+    # iteration's work count for the per-launch average.
+    def work_launches(slot, cnt):
+        return max(1, cg_work) if slot == "cg_iter" else cnt
+
     dom = max(kt, key=lambda k: kt[k][0])
     dom_ms, dom_cnt = kt[dom]
-    launches = dom_cnt
-    if dom == "cg_iter":
-        # speculative launches after both solves converged exit at once;
-        # count only the launches that did an iteration's work
-        launches = max(1, int(round((max(its) - 1) * args.steps)))
+    launches = work_launches(dom, dom_cnt)
     per_launch_bytes = algorithmic_bytes(dom, c, args.mixed)
     avg_ms = dom_ms / launches
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9 if per_launch_bytes else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1) if achieved else None,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                "traffic": pmc_traffic(dom, args.config),
                 "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_ms * 1e3, 2),
                 "launches": launches}
-    kernels = {k: {"ms_per_step": round(v[0] / args.steps, 4), "launches": v[1],
-                   "gbs": (round(algorithmic_bytes(k, c) * (v[1] if k != "cg_iter" else launches)
-                                 / (v[0] * 1e-3) / 1e9, 1) if algorithmic_bytes(k, c) else None)}
-               for k, v in kt.items()}
+    kernels = {}
+    for k, (ms, cnt) in kt.items():
+        b = algorithmic_bytes(k, c, args.mixed)
+        wl = work_launches(k, cnt)
+        kernels[k] = {"ms_per_step": round(ms / args.steps, 4), "launches": cnt,
+                      "avg_us": round(ms * 1e3 / wl, 2),
+                      "gbs": round(b * wl / (ms * 1e-3) / 1e9, 1) if b else None}
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline (oracle, budget {args.cpu_budget_s:.0f} s)")

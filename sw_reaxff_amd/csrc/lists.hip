@@ -515,22 +515,20 @@ int swr_lists_run(swr_system *S, int mode) {
                      S->Cdbo, S->Cdbopi, S->Cdbopi2, S->bo_dboc, S->dDeltap_self, S->bond_mark,                    \
                      ws + (size_t)WS_DELTAP * S->Ncap, ws + (size_t)WS_DELTAP_BOC * S->Ncap, S->counts,            \
                      S->bond_cap, S->status)
-  swr_kt_begin(S, KT_LISTS);
-  if (mode == 0) {
-    FAR_ARGS((k_far<false, true, false, 0>));
-  } else if (mode == 1) {
-    FAR_ARGS((k_far<false, true, true, 2>));
-    BOND_ARGS((k_bonds<false>));
-  } else if (mode == 2) {
-    FAR_ARGS((k_far<true, false, true, 2>));
-    BOND_ARGS((k_bonds<false>));
-  } else {
-    FAR_ARGS((k_far<false, true, true, 1>));
-    BOND_ARGS((k_bonds<true>));
+  swr_kt_begin(S, KT_FAR);
+  if (mode == 0) FAR_ARGS((k_far<false, true, false, 0>));
+  else if (mode == 1) FAR_ARGS((k_far<false, true, true, 2>));
+  else if (mode == 2) FAR_ARGS((k_far<true, false, true, 2>));
+  else FAR_ARGS((k_far<false, true, true, 1>));
+  swr_kt_end(S, KT_FAR);
+  if (mode != 0) {
+    swr_kt_begin(S, KT_BONDS);
+    if (mode == 3) BOND_ARGS((k_bonds<true>));
+    else BOND_ARGS((k_bonds<false>));
+    swr_kt_end(S, KT_BONDS);
   }
 #undef FAR_ARGS
 #undef BOND_ARGS
-  swr_kt_end(S, KT_LISTS);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }

@@ -147,6 +147,7 @@ struct swr_lmp {
   long long last_rebuild = 0;
   int iters_s = 0, iters_t = 0;
   long long total_iters_s = 0, total_iters_t = 0;
+  long long cg_work_launches = 0;  // iteration launches that did work: max(iters_s, iters_t) - 1 per solve
   int profile = 0;
   double stage_ms[ST_COUNT] = {0};
   double wall_ms = 0;
@@ -189,6 +190,7 @@ struct swr_lmp {
       if ((rc = swr_qeq_cg(sys, &iters_s, &iters_t))) return rc;
       total_iters_s += iters_s;
       total_iters_t += iters_t;
+      cg_work_launches += std::max(0, std::max(iters_s, iters_t) - 1);
       mark(5);
       if ((rc = swr_calculate_q(sys))) return rc;
     } else {
@@ -356,6 +358,7 @@ double swr_lmp_stat(swr_lmp *L, const char *name) {
   if (k == "iters_t") return L->iters_t;
   if (k == "total_iters_s") return (double)L->total_iters_s;
   if (k == "total_iters_t") return (double)L->total_iters_t;
+  if (k == "cg_work_launches") return (double)L->cg_work_launches;
   if (k == "n") return L->n;
   if (k == "N") return L->sys ? L->sys->N : 0;
   for (int s = 0; s < ST_COUNT; s++)

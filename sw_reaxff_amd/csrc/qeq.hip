@@ -461,28 +461,28 @@ static int launch_cg(swr_system *S, int ph, int par_in) {
   CgVecs V = make_vecs(S);
   const double tol = S->ctl.tol;
   const int kt = ph == PH_K ? KT_CG_ITER : KT_CG_SETUP;
-  swr_kt_begin(S, kt);
+  const int kts = ph == PH_K ? KT_CG_SCALARS : KT_CG_SETUP;
 #define CG_ARGS n, S->dp, S->type, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, V, par_in, S->cgs, S->partials, ps
 #define SC_ARGS nblk, S->partials, ps, S->cgs, S->imax, tol, S->status
+  swr_kt_begin(S, kt);
   switch (ph) {
     case PH_S1: hipLaunchKernelGGL(k_cg<PH_S1>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
     case PH_S2: hipLaunchKernelGGL(k_cg<PH_S2>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
-    case PH_S3:
-      hipLaunchKernelGGL(k_cg<PH_S3>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS);
-      hipLaunchKernelGGL(k_cg_scalars<PH_S3>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS);
-      break;
-    case PH_A1:
-      hipLaunchKernelGGL(k_cg<PH_A1>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS);
-      hipLaunchKernelGGL(k_cg_scalars<PH_A1>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS);
-      break;
-    default:
-      hipLaunchKernelGGL(k_cg<PH_K>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS);
-      hipLaunchKernelGGL(k_cg_scalars<PH_K>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS);
-      break;
+    case PH_S3: hipLaunchKernelGGL(k_cg<PH_S3>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
+    case PH_A1: hipLaunchKernelGGL(k_cg<PH_A1>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
+    default: hipLaunchKernelGGL(k_cg<PH_K>, dim3(nblk), dim3(QT), 0, S->stream, CG_ARGS); break;
   }
+  swr_kt_end(S, kt);
+  swr_kt_begin(S, kts);
+  switch (ph) {
+    case PH_S1: case PH_S2: break;
+    case PH_S3: hipLaunchKernelGGL(k_cg_scalars<PH_S3>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS); break;
+    case PH_A1: hipLaunchKernelGGL(k_cg_scalars<PH_A1>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS); break;
+    default: hipLaunchKernelGGL(k_cg_scalars<PH_K>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS); break;
+  }
+  swr_kt_end(S, kts);
 #undef CG_ARGS
 #undef SC_ARGS
-  swr_kt_end(S, kt);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }

@@ -208,8 +208,8 @@ struct swr_system {
 };
 
 // Kernel timer slots.
-enum { KT_VERLET = 0, KT_LISTS, KT_BUILD_H, KT_CG_ITER, KT_CG_SETUP, KT_BO, KT_VALIDATE, KT_SPMV, KT_RESET,
-       KT_QEQ_Q, KT_BINS, KT_MIXED_INNER, KT_MIXED_OUTER, KT_COUNT };
+enum { KT_VERLET = 0, KT_FAR, KT_BUILD_H, KT_CG_ITER, KT_CG_SETUP, KT_BO, KT_VALIDATE, KT_SPMV, KT_RESET,
+       KT_QEQ_Q, KT_BINS, KT_MIXED_INNER, KT_MIXED_OUTER, KT_BONDS, KT_CG_SCALARS, KT_COUNT };
 extern const char *const swr_kt_names[KT_COUNT];
 void swr_kt_begin(swr_system *S, int k);
 void swr_kt_end(swr_system *S, int k);

@@ -2,8 +2,9 @@
 over-coordination branch and compressed bonds, non-periodic systems.
 
 Both sides run QEq at tol 1e-10, so charges agree to 1e-9 e; lists are
-compared exactly (per-atom counts and sorted bond partners, hbond partners
-in far-list order) and bond orders to 1e-10 relative.
+compared exactly (per-atom counts, sorted bond partners, hbond partners of
+each H as sets) and bond orders to 1e-10 relative.  Every check runs and the
+failures are reported together.
 """
 import numpy as np
 import pytest
@@ -21,9 +22,22 @@ ATOM_KEYS = ("Deltap", "Deltap_boc", "Delta", "Delta_e", "Delta_val", "vlpex", "
              "Delta_lp", "dDelta_lp", "Delta_lp_temp", "dDelta_lp_temp")
 
 
-def close(a, b, rtol=1e-10, floor=1e-12):
-    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
-    return np.all(np.abs(a - b) <= rtol * np.maximum(np.abs(b), floor))
+def close(a, b, rtol=1e-10, floor=1e-12, arel=1e-13):
+    """|a-b| <= rtol max(|b|, floor) + arel max|b|.
+
+    Relative to 1e-10 for every entry above the array's rounding level.  The
+    correction coefficients C2dbo/C3dbo (and C3/C4dbopi) are differences of
+    O(1..10) terms that cancel exactly for some bonds (e.g. C-N in HCN, where
+    Cf1 carries exp(-50 Deltap)); there the values are rounding noise of
+    ~1e-14 whose sign depends on the last bit of exp/log, so only the
+    absolute term is meaningful."""
+    a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+    scale = np.max(np.abs(b)) if b.size else 0.0
+    bad = np.abs(a - b) > rtol * np.maximum(np.abs(b), floor) + arel * scale
+    if bad.any():
+        k = np.nonzero(bad)[0][:6]
+        print("mismatch at", k.tolist(), "got", a[k].tolist(), "want", b[k].tolist(), flush=True)
+    return not bad.any()
 
 
 @pytest.mark.parametrize("name", sorted(G.CASES))
@@ -36,26 +50,30 @@ def test_gpu_matches_golden(gpu_lib, ffield, name):
     lmp = harness(s, ffield, tol=float(z["tol"]))
     h = lmp.system
     assert h.scalar("N") == n
-    # charges
-    assert np.max(np.abs(h.get("q")[:n] - z["q"])) <= 1e-9
-    # far list and H rows: exact counts
-    assert np.array_equal(h.get("far_end") - h.get("far_index"), z["far_count"])
-    assert np.array_equal(h.get("H_numnbrs")[:n], z["H_numnbrs"])
-    # bonds: per-atom partners in nbr order
+    fails = []
+
+    def check(ok, what):
+        if not ok:
+            fails.append(what)
+
+    dq = np.max(np.abs(h.get("q")[:n] - z["q"]))
+    check(dq <= 1e-9, f"q {dq:.2e}")
+    check(np.array_equal(h.get("far_end") - h.get("far_index"), z["far_count"]), "far counts")
+    check(np.array_equal(h.get("H_numnbrs")[:n], z["H_numnbrs"]), "H row counts")
     bi, be = h.get("bond_index"), h.get("bond_end")
-    assert np.array_equal(be - bi, z["bond_count"])
+    assert np.array_equal(be - bi, z["bond_count"]), "bond counts"
     sel = np.concatenate([np.arange(bi[i], be[i]) for i in range(n)])
     bonds, bo = h.get("bonds"), h.get("bo_data")
-    assert np.array_equal(bonds["nbr"][sel], z["bond_nbr"])
-    assert close(h.get("BO")[sel], z["BO"])
-    assert close(h.get("BOpi").reshape(-1, 2)[sel], z["BOpi"])
-    assert close(bo["dBOp"][sel], z["dBOp"], floor=1e-10)
+    assert np.array_equal(bonds["nbr"][sel], z["bond_nbr"]), "bond partners"
+    check(close(h.get("BO")[sel], z["BO"]), "BO")
+    check(close(h.get("BOpi").reshape(-1, 2)[sel], z["BOpi"]), "BOpi")
+    check(close(bo["dBOp"][sel], z["dBOp"], floor=1e-10), "dBOp")
     for k in FLOAT_KEYS:
-        assert close(bo[k][sel], z[k]), k
+        check(close(bo[k][sel], z[k]), k)
     for k in ATOM_KEYS:
-        assert close(h.get(k)[:n], z[k]), k
-    assert close(h.get("bo_dboc").reshape(-1, 2)[:n, 1], z["Delta_boc"])
-    # hbonds: partners of each local H in far-list order
+        check(close(h.get(k)[:n], z[k]), k)
+    check(close(h.get("bo_dboc").reshape(-1, 2)[:n, 1], z["Delta_boc"]), "Delta_boc")
+    # hbonds: partners of each local H
     Hindex, hi_, he_ = h.get("Hindex"), h.get("hbond_index"), h.get("hbond_end")
     hb = h.get("hbonds")
     cnt = np.zeros(n, np.int32)
@@ -65,5 +83,13 @@ def test_gpu_matches_golden(gpu_lib, ffield, name):
             seg = hb["nbr"][hi_[Hindex[i]]:he_[Hindex[i]]]
             cnt[i] = seg.size
             nbrs.append(seg)
-    assert np.array_equal(cnt, z["hbond_count"])
-    assert np.array_equal(np.concatenate(nbrs) if nbrs else np.zeros(0, np.int32), z["hbond_nbr"])
+    check(np.array_equal(cnt, z["hbond_count"]), "hbond counts")
+    if np.array_equal(cnt, z["hbond_count"]):
+        # sets per H: the fixture's far-list order comes from the oracle's own
+        # Verlet list, the GPU's from its cell list (order parity is checked
+        # where both sides share one Verlet list, in test_gpu_parity.py)
+        got = np.concatenate(nbrs) if nbrs else np.zeros(0, np.int32)
+        off = np.concatenate([[0], np.cumsum(cnt)])
+        check(all(sorted(got[off[i]:off[i + 1]].tolist()) == sorted(z["hbond_nbr"][off[i]:off[i + 1]].tolist())
+                  for i in range(n)), "hbond partners")
+    assert not fails, fails
