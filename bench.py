@@ -13,7 +13,7 @@ of the dominant kernel (HIP events on the library's stream) and the CPU
 oracle timed on this machine's host cores on a bounded sample of the same
 workload.
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--config c1|c0|c2big]
+    python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--config water96k|water1944|water768k]
                     [--mixed] [--cpu-budget-s 20] [--no-cpu]
 """
 from __future__ import annotations
@@ -36,9 +36,9 @@ from sw_reaxff_amd.lammps import DEFAULT_FFIELD, Lammps, standard_script  # noqa
 METRIC = "ms/step (nbr+BO+QEq) at 96k atoms; batched systems/s + p99 at 1e-6 e tol"
 HBM_PEAK_GBS = 8000.0
 CONFIGS = {
-    "c0": dict(nmol=648, label="648-water box (1,944 atoms)"),
-    "c1": dict(nmol=32000, label="32,000-water box (96,000 atoms)"),
-    "c2big": dict(nmol=256000, label="256,000-water box (768,000 atoms)"),
+    "water1944": dict(nmol=648, label="648-water box (1,944 atoms)"),
+    "water96k": dict(nmol=32000, label="32,000-water box (96,000 atoms)"),
+    "water768k": dict(nmol=256000, label="256,000-water box (768,000 atoms)"),
 }
 
 
@@ -182,7 +182,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="water96k", choices=sorted(CONFIGS))
     ap.add_argument("--mixed", action="store_true", help="mixed-precision QEq")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--every", type=int, default=10)
