@@ -61,18 +61,22 @@ def counts(sysh):
                 bonds=int((be - bi).sum()), hbonds=int((he - hi).sum()), nnz=int(sysh.get("H_numnbrs").sum()))
 
 
-SLOTS = ("verlet", "bins", "far", "bonds", "reset", "validate", "bo", "build_H", "cg_setup", "cg_iter", "cg_scalars",
-         "qeq_q", "mixed_inner", "mixed_outer")
+SLOTS = ("verlet", "bins", "far", "bonds", "reset", "validate", "mark", "bo", "build_H", "cg_setup", "cg_iter",
+         "cg_update", "cg_scalars", "qeq_q", "mixed_inner", "mixed_outer")
 
 
 def algorithmic_bytes(slot, c, mixed=False):
     """Compulsory HBM bytes of one launch of a kernel slot, from its inputs and outputs."""
     n, N = c["n"], c["N"]
     if slot == "cg_iter":
-        # H (jlist 4 B + val 8 B per entry; 4 B val in the FP32 inner solve),
-        # the (s,t) d gather once per row, 9 (s,t) vectors read and 9 written,
-        # Hdia_inv, eta type, row pointers
-        return (8 if mixed else 12) * c["nnz"] + n * (16 + 18 * 16 + 8 + 4 + 8)
+        # the iteration's SpMV: H (jlist 4 B + val 8 B per entry; 4 B val in
+        # the FP32 inner solve), the (s,t) d vector gathered at least once,
+        # q written, type and the two row-pointer arrays
+        return (8 if mixed else 12) * c["nnz"] + n * (16 + 16 + 4 + 8)
+    if slot == "cg_update":
+        # the iteration's vector step: 10 (s,t) vectors read (x p u ss w v z r
+        # d q), Hdia_inv, 9 written (x p ss v z r u w d)
+        return n * (10 * 16 + 8 + 9 * 16)
     if slot == "far":
         # Verlet ids + offsets, packed atoms once, far entries, far_index/end,
         # hbond records, bond-candidate slots (at least one per bond) and counts
@@ -239,7 +243,7 @@ def main():
     # both solves converged exit at once; only the launches that did an
     # iteration's work count for the per-launch average.
     def work_launches(slot, cnt):
-        return max(1, cg_work) if slot == "cg_iter" else cnt
+        return max(1, cg_work) if slot in ("cg_iter", "cg_update") else cnt
 
     dom = max(kt, key=lambda k: kt[k][0])
     dom_ms, dom_cnt = kt[dom]

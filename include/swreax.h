@@ -85,11 +85,14 @@ int swr_set_capacities(swr_system *sys, const int *num_bonds, const int *num_hbo
 
 /* a1: full neighbour list of local + ghost atoms on a GPU cell list.
  * Replaces npair_full_bin_ghost_sunway_build_packed (npair_full_bin_ghost_sw5.c:64;
- * CPE kernel :80-226) and its atomonly twin
- * npair_full_bin_atomonly_sunway_build_packed (npair_full_bin_atomonly_sw5.c:6,
- * called from npair_full_bin_atomonly_sunway.cpp:151).  cutneigh <= 0 uses
- * the control's cutmax + skin. */
+ * CPE kernel :80-226).  cutneigh <= 0 uses the control's cutmax + skin. */
 int swr_npair_full_bin_ghost_build(swr_system *sys, double cutneigh);
+
+/* a1': the same list for local atoms only; ghost rows are empty (gnum = 0).
+ * Replaces npair_full_bin_atomonly_sunway_build_packed
+ * (npair_full_bin_atomonly_sw5.c:6, CPE kernel :57-180, called from
+ * NPairFullBinAtomonlySunway::build, npair_full_bin_atomonly_sunway.cpp:151). */
+int swr_npair_full_bin_atomonly_build(swr_system *sys, double cutneigh);
 
 /* a2: pack tag, element type, x, q (write_reax_atoms_and_pack,
  * pair_reaxc_sw64.c:13; caller pair_reaxc_sunway.cpp:818). */
@@ -115,10 +118,16 @@ int swr_init_forces_noqeq_full(swr_system *sys);
  * re-reading the far list. */
 int swr_write_reax_lists_init_forces(swr_system *sys);
 
-/* a7: capacity update for the next step and bond_mark propagation
- * (Validate_Lists_C, reaxc_forces_sw64.c:866; sort_bonds_C, :932; called from
- * reaxc_nonbonded_sw64.c:436-437). */
+/* a7: per-atom capacities for the next step from this step's counts, and
+ * the overflow check (Validate_Lists_C, reaxc_forces_sw64.c:866-930): an
+ * end index past the next atom's start returns SWR_ERR_BOND_CAP /
+ * SWR_ERR_HBOND_CAP (the reference aborts). */
 int swr_validate_lists(swr_system *sys);
+
+/* a7: bond_mark[i] = min(bond_mark[i], bond_mark[j] + 1) over i's bonds in
+ * one pass in atom order (sort_bonds_C, reaxc_forces_sw64.c:932-971); the
+ * result equals the reference's sequential pass exactly. */
+int swr_sort_bonds(swr_system *sys);
 
 /* a8-a10: corrected bond orders, dBO coefficients, Delta and lone-pair terms
  * (BO_C, reaxc_bond_orders_sw64.c:487; caller reaxc_bond_orders_sunway.cpp:454). */
@@ -156,7 +165,7 @@ int swr_qeq_pre_force(swr_system *sys, int *iters_s, int *iters_t);
 
 /* Pair compute stages in the reference order (PairReaxCSunway::compute,
  * pair_reaxc_sunway.cpp:541-793, through BO): pack, Reset, lists + bonds,
- * Validate_Lists, BO. */
+ * Validate_Lists, sort_bonds, BO. */
 int swr_pair_compute(swr_system *sys);
 
 /* Join the stream; returns the first device-raised error. */
@@ -182,8 +191,8 @@ int64_t swr_get_scalar(swr_system *sys, const char *name);
 void   *swr_device_ptr(swr_system *sys, const char *name);
 
 /* Per-kernel timing with HIP events on the system's stream.  Slots: verlet
- * bins lists reset validate bo build_H cg_setup cg_iter spmv qeq_q
- * mixed_inner mixed_outer.  read() synchronises, returns the accumulated
+ * bins far bonds reset validate mark bo build_H cg_setup cg_iter cg_update
+ * cg_scalars spmv qeq_q mixed_inner mixed_outer.  read() synchronises, returns the accumulated
  * milliseconds and launch count, and clears them when reset != 0. */
 int swr_ktimer_enable(swr_system *sys, int on);
 int swr_ktimer_read(swr_system *sys, const char *slot, double *ms, long long *count, int reset);

@@ -10,22 +10,6 @@
 
 #include "swr_internal.h"
 
-// stage implementations (nbr.hip, lists.hip, bo.hip, qeq.hip)
-int swr_nbr_set_grid(swr_system *S, const double lo[3], const double hi[3], double cutghost, double cutneigh);
-int swr_nbr_build(swr_system *S);
-int swr_lists_pack(swr_system *S);
-int swr_lists_reset(swr_system *S);
-int swr_lists_run(swr_system *S, int mode);
-int swr_lists_estimate(swr_system *S);
-int swr_lists_validate(swr_system *S, int max_sweeps);
-int swr_bo_run(swr_system *S);
-int swr_qeq_build_H(swr_system *S);
-int swr_qeq_spmv(swr_system *S, const double *xv, double *b);
-int swr_qeq_cg_run(swr_system *S, int first_batch);
-int swr_qeq_calc_q(swr_system *S);
-int swr_qeq_blocks(int n);
-int swr_qeq_mixed_solve(swr_system *S);
-
 namespace {
 
 constexpr int MARK_SWEEPS = 1024;
@@ -257,7 +241,7 @@ int swr_create(const char *ffield_path, int ntypes, const char *const *elements,
     return SWR_ERR_HIP;
   }
   int rc;
-  if ((rc = swr_alloc(&S->dp, 1)) || (rc = swr_alloc(&S->status, 1)) || (rc = swr_alloc(&S->cgs, 2)) ||
+  if ((rc = swr_alloc(&S->dp, 1)) || (rc = swr_alloc(&S->status, 1)) || (rc = swr_alloc(&S->cgs, 4)) ||
       (rc = swr_alloc(&S->counters, 4))) {
     swr_destroy(S);
     return rc;
@@ -271,7 +255,7 @@ int swr_create(const char *ffield_path, int ntypes, const char *const *elements,
   (void)hipMemcpy(S->dp, &S->hp, sizeof(DevParams), hipMemcpyHostToDevice);
   (void)hipMemset(S->status, 0, sizeof(DevStatus));
   (void)hipMemset(S->counters, 0, 4 * sizeof(unsigned int));
-  (void)hipMemset(S->cgs, 0, 2 * sizeof(CgScalars));
+  (void)hipMemset(S->cgs, 0, 4 * sizeof(CgScalars));
   *out = S;
   return SWR_OK;
 }
@@ -339,7 +323,7 @@ int swr_set_capacities(swr_system *S, const int *num_bonds, const int *num_hbond
   return SWR_OK;
 }
 
-int swr_npair_full_bin_ghost_build(swr_system *S, double cutneigh) {
+static int nbr_build_common(swr_system *S, double cutneigh, int local_only) {
   if (!S || S->N <= 0) return SWR_ERR_STATE;
   if (cutneigh <= 0) {
     double cutmax = std::max(std::max(S->ff.nonb_cut, S->ctl.hbond_cut), 2 * S->ctl.bond_cut);
@@ -353,8 +337,12 @@ int swr_npair_full_bin_ghost_build(swr_system *S, double cutneigh) {
     for (int d = 0; d < 3; d++) { lo[d] = std::min(lo[d], xh[3 * i + d]); hi[d] = std::max(hi[d], xh[3 * i + d]); }
   int rc = swr_nbr_set_grid(S, lo, hi, 0.0, cutneigh);
   if (rc) return rc;
-  return swr_nbr_build(S);
+  return swr_nbr_build(S, local_only);
 }
+
+int swr_npair_full_bin_ghost_build(swr_system *S, double cutneigh) { return nbr_build_common(S, cutneigh, 0); }
+
+int swr_npair_full_bin_atomonly_build(swr_system *S, double cutneigh) { return nbr_build_common(S, cutneigh, 1); }
 
 int swr_write_reax_atoms_and_pack(swr_system *S) {
   if (!S || S->N <= 0) return SWR_ERR_STATE;
@@ -424,7 +412,12 @@ int swr_write_reax_lists_init_forces(swr_system *S) {
 
 int swr_validate_lists(swr_system *S) {
   if (!S || !S->bonds) return SWR_ERR_STATE;
-  return swr_lists_validate(S, MARK_SWEEPS);
+  return swr_lists_validate(S);
+}
+
+int swr_sort_bonds(swr_system *S) {
+  if (!S || !S->bonds) return SWR_ERR_STATE;
+  return swr_lists_mark(S, MARK_SWEEPS);
 }
 
 int swr_bo(swr_system *S) {
@@ -493,7 +486,7 @@ int swr_qeq_pre_force(swr_system *S, int *iters_s, int *iters_t) {
 int swr_pair_compute(swr_system *S) {
   int rc;
   if ((rc = swr_write_reax_atoms_and_pack(S)) || (rc = swr_reset(S)) || (rc = swr_write_reax_lists_init_forces(S)) ||
-      (rc = swr_validate_lists(S)) || (rc = swr_bo(S)))
+      (rc = swr_validate_lists(S)) || (rc = swr_sort_bonds(S)) || (rc = swr_bo(S)))
     return rc;
   return SWR_OK;
 }
@@ -568,6 +561,7 @@ bool lookup(swr_system *S, const std::string &k, View &v) {
   if (k == "H_numnbrs") return set(S->H_numnbrs, 4 * n);
   if (k == "H_jlist") return set(S->H_jlist, 4 * S->Hcap);
   if (k == "H_val") return set(S->H_val, 8 * S->Hcap);
+  if (k == "H_val32") return set(S->H_val32, S->H_val32 ? 4 * S->Hcap : 0);
   if (k == "Hdia_inv") return set(S->Hdia_inv, 8 * n);
   if (k == "b_s") return set(S->b_s, 8 * n, 1);
   if (k == "b_t") return set(S->b_s, 8 * n, 2);

@@ -545,12 +545,28 @@ int swr_lists_estimate(swr_system *S) {
 // Validate_Lists + sort_bonds.  bond_mark sweeps are launched in batches
 // sized from the previous call's depth; the host checks convergence after
 // each batch (one small read-back per call) and continues if needed.
-int swr_lists_validate(swr_system *S, int max_sweeps) {
+int swr_lists_validate(swr_system *S) {
   const int n = S->n, N = S->N;
   swr_kt_begin(S, KT_VALIDATE);
   hipLaunchKernelGGL(k_validate, dim3(div_up(N, 256)), dim3(256), 0, S->stream, n, N, S->bond_index, S->counts,
                      S->num_bonds, S->Hindex, S->counts + S->Ncap, S->num_hbonds, S->ctl.saferzone, S->bond_cap,
                      S->status);
+  swr_kt_end(S, KT_VALIDATE);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+// bond_mark of sort_bonds_C (reaxc_forces_sw64.c:932-971): one pass over
+// the atoms in index order, each taking min(mark[i], mark[j] + 1) over its
+// bonds with the values current at that point.  Local atoms start at 0 and
+// ghosts at 1000, so only ghosts change, and ghost i ends at
+//   min(1000, min over bonded j < i of final[j] + 1, min over j > i of init[j] + 1).
+// The sweeps below evaluate that recursion over the j < i graph (Jacobi
+// sweeps: sweep s fixes every ghost whose dependency chain has depth <= s)
+// until no mark changes; the host reads the convergence flag between batches.
+int swr_lists_mark(swr_system *S, int max_sweeps) {
+  const int n = S->n, N = S->N;
+  swr_kt_begin(S, KT_MARK);
   if (N > n) {
     int rc = swr_ensure_tmp(S, sizeof(int) * 8);
     if (rc) return rc;
@@ -582,7 +598,7 @@ int swr_lists_validate(swr_system *S, int max_sweeps) {
     }
     if (a != S->bond_mark) std::swap(S->bond_mark, S->bond_mark2);
   }
-  swr_kt_end(S, KT_VALIDATE);
+  swr_kt_end(S, KT_MARK);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }

@@ -25,14 +25,6 @@
 
 #include "swr_internal.h"
 
-int swr_nbr_set_grid(swr_system *S, const double lo[3], const double hi[3], double cutghost, double cutneigh);
-int swr_nbr_build(swr_system *S);
-int swr_nbr_borders(swr_system *S, const double lo[3], const double hi[3], double cutghost);
-int swr_nbr_forward_x(swr_system *S, const double lo[3], const double hi[3]);
-int swr_nbr_nve(swr_system *S, double dt);
-int swr_nbr_pbc(swr_system *S, const double lo[3], const double hi[3]);
-int swr_nbr_ensure_atoms(swr_system *S, int Ncap);
-
 namespace {
 
 std::vector<std::string> split(const std::string &s) {
@@ -175,7 +167,7 @@ struct swr_lmp {
       sys->N = sys->n;
       if ((rc = swr_nbr_set_grid(sys, lo, hi, 0.0, cutghost))) return rc;
     }
-    if ((rc = swr_nbr_build(sys))) return rc;
+    if ((rc = swr_nbr_build(sys, 0))) return rc;
     last_rebuild = step;
     return SWR_OK;
   }
@@ -202,7 +194,7 @@ struct swr_lmp {
     mark(7);
     if ((rc = swr_write_reax_lists_init_forces(sys))) return rc;
     mark(8);
-    if ((rc = swr_validate_lists(sys))) return rc;
+    if ((rc = swr_validate_lists(sys)) || (rc = swr_sort_bonds(sys))) return rc;
     mark(9);
     if ((rc = swr_bo(sys))) return rc;
     mark(10);

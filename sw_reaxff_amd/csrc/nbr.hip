@@ -168,12 +168,16 @@ __global__ void __launch_bounds__(NB_THREADS)
 k_verlet(const double *__restrict__ x, int N, const int *__restrict__ bin_of, const int *__restrict__ bin_start,
          const double4 *__restrict__ bp_pos, const int *__restrict__ bp_id, const int4 *__restrict__ rows, int nrows,
          int nx, int ny, int nz, double cutsq, int *__restrict__ vcount, const long long *__restrict__ voff,
-         int *__restrict__ vlist) {
+         int *__restrict__ vlist, int nrow) {
   const int nblk = gridDim.x;
   const int blk = xcd_block(nblk);
   const int i = blk * WAVES_PER_BLOCK + wave_in_block();
   if (i >= N) return;
   const int lane = lane_id();
+  if (i >= nrow) {  // rows past nrow stay empty (the local-atom-only list)
+    if (!WRITE && lane == 0) vcount[i] = 0;
+    return;
+  }
   const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
   const int b = bin_of[i];
   const int bx = b % nx, by = (b / nx) % ny, bz = b / (nx * ny);
@@ -323,8 +327,11 @@ int swr_nbr_pbc(swr_system *S, const double lo[3], const double hi[3]) {
 }
 
 // Bin all N atoms and build the full Verlet list (CSR: voff[N+1], vlist).
-int swr_nbr_build(swr_system *S) {
+// Full neighbour list of all N atoms (nrow = N: the ghost list) or of the
+// local atoms only (nrow = n: the atom-only list, ghost rows empty).
+int swr_nbr_build(swr_system *S, int local_only) {
   const int N = S->N;
+  const int nrow = local_only ? S->n : N;
   int rc;
   if ((rc = swr_alloc(&S->bin_of, N)) || (rc = swr_alloc(&S->bp_id, N)) || (rc = swr_alloc(&S->bp_pos, N)))
     return rc;
@@ -346,7 +353,7 @@ int swr_nbr_build(swr_system *S) {
   swr_kt_begin(S, KT_VERLET);
   hipLaunchKernelGGL(k_verlet<false>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->x, N, S->bin_of, S->bin_start,
                      S->bp_pos, S->bp_id, S->stencil_rows, S->nrows, S->nbin[0], S->nbin[1], S->nbin[2], cutsq,
-                     S->vcount, (const long long *)nullptr, (int *)nullptr);
+                     S->vcount, (const long long *)nullptr, (int *)nullptr, nrow);
   swr_kt_end(S, KT_VERLET);
   long long total = 0;
   if ((rc = swr_scan_i32_to_i64(S, S->vcount, S->voff, N, &total))) return rc;
@@ -360,7 +367,7 @@ int swr_nbr_build(swr_system *S) {
   swr_kt_begin(S, KT_VERLET);
   hipLaunchKernelGGL(k_verlet<true>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->x, N, S->bin_of, S->bin_start,
                      S->bp_pos, S->bp_id, S->stencil_rows, S->nrows, S->nbin[0], S->nbin[1], S->nbin[2], cutsq,
-                     S->vcount, (const long long *)S->voff, S->vlist);
+                     S->vcount, (const long long *)S->voff, S->vlist, nrow);
   swr_kt_end(S, KT_VERLET);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;

@@ -1,0 +1,409 @@
+// qeq_cg.h — the pipelined Jacobi-PCG of the QEq solve (CG_v2,
+// fix_qeq_reax_sunway.cpp:983-1167) as templates over the vector type, shared
+// by the FP64 solve (qeq.hip) and the FP32 inner solve of the mixed-precision
+// refinement (qeq_mixed.hip).
+//
+// Vectors hold the s and t systems side by side (T2 = double2 or float2), so
+// each H entry is read once and feeds both solves.  An iteration is three
+// launches: k_cg_spmv (q = H d, half a wave per row, nothing else so the
+// gathers run at full occupancy), k_cg_upd (one thread per row, coalesced:
+// the reference's B and C steps of iteration i -- x/p/ss/v/z updates -- and
+// the A step of iteration i+1 -- r/u/w updates, the two dots, d = M^-1 w) and
+// k_cg_scalars.  Each k_cg_upd workgroup writes its partial dots; the
+// one-workgroup k_cg_scalars sums them in a fixed order and computes the next
+// iteration's scalars (no atomics, so results are the same bits on every
+// run).  Dot products and all
+// scalars are FP64 whatever T is.  Each solve stops by the reference's test
+// sqrt(u.r)/|b| <= tol (or i = imax) and is frozen while the other runs.
+#pragma once
+
+#include "swr_internal.h"
+#include "swr_device.h"
+
+namespace swr_cg {
+
+constexpr int QT = 256;                    // threads per block
+constexpr int QW = QT / SWR_WAVE;          // waves per block
+constexpr int ROWS_PER_WAVE = 2;           // one row per half-wave
+constexpr int RPB = QW * ROWS_PER_WAVE;    // rows per block (fixed: results do not depend on grid size)
+constexpr int RT = 1024;                   // threads of the one-workgroup reduction kernels
+constexpr int UT = 256;                    // rows (threads) per block of the update kernels
+
+enum { PH_S1 = 0, PH_S2, PH_S3, PH_A1, PH_K };
+
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { typedef double2 type; };
+template <> struct Vec2<float> { typedef float2 type; };
+
+template <typename T>
+struct CgVecsT {
+  typedef typename Vec2<T>::type T2;
+  T2 *x, *r, *u, *w, *z, *ss, *v, *p, *m, *d[2], *q;
+  const T2 *b;
+  const double *Hd;
+};
+
+// Sums partials[k*pstride + b] over b < nblocks for k < ND with one
+// workgroup: thread t adds b = t, t+RT, ... in order, then a fixed LDS tree
+// combines the RT sums.  The order depends only on the block count.
+template <int ND>
+__device__ void reduce_partials(const double *__restrict__ partials, int pstride, int nblocks, double *out) {
+  __shared__ double sr[ND][RT];
+  double acc[ND];
+#pragma unroll
+  for (int k = 0; k < ND; k++) acc[k] = 0.0;
+#pragma unroll 4
+  for (int b = threadIdx.x; b < nblocks; b += RT)
+#pragma unroll
+    for (int k = 0; k < ND; k++) acc[k] += partials[(size_t)k * pstride + b];
+#pragma unroll
+  for (int k = 0; k < ND; k++) sr[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int s = RT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int k = 0; k < ND; k++) sr[k][threadIdx.x] += sr[k][threadIdx.x + s];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < ND; k++) out[k] = sr[k][0];
+}
+
+// Block sum of per-thread values d[k] in a fixed order (wave tree, then
+// waves in order); thread k < ND writes partials[k*pstride + blk].
+template <int ND, int NT>
+__device__ __forceinline__ void block_partials(const double *d, double *__restrict__ partials, int pstride,
+                                               int blk) {
+  constexpr int NW = NT / SWR_WAVE;
+  __shared__ double sred[NW][ND];
+  const int w = wave_in_block();
+#pragma unroll
+  for (int k = 0; k < ND; k++) {
+    const double s = wave_sum(d[k]);
+    if (lane_id() == 0) sred[w][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < ND) {
+    double s = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < NW; ww++) s += sred[ww][threadIdx.x];
+    partials[(size_t)threadIdx.x * pstride + blk] = s;
+  }
+}
+
+// Scalars after the A step of an iteration reduced dots l0 = u.r, l1 = u.w
+// (fix_qeq_reax_sunway.cpp:1133-1141 and :1152).
+__device__ inline void cg_after_A(CgScalars &c, double l0, double l1, int imax) {
+  const double beta = l0 / c.sig_old;
+  c.heta = l1 - beta * beta * c.heta;
+  c.beta = beta;
+  c.alpha_x = c.alpha_a;
+  c.alpha_a = l0 / c.heta;
+  c.sig_old = l0;
+  c.dg0 = l0;
+  c.dg1 = l1;
+  c.iter += 1;
+  c.active_b = 1;
+  c.active_a = (c.iter < imax && sqrt(l0) / c.b_norm > c.tol) ? 1 : 0;
+}
+
+// Half-wave (32 lanes) per row, 2 rows per wave: four independent
+// index/value/gather loads in flight per lane; fixed summation order (per
+// lane a0..a3, then a fixed 32-lane tree).  Products accumulate in T.
+template <typename T, typename VT>
+__device__ __forceinline__ typename Vec2<T>::type row_spmv2_h(int beg, int cnt, const int *__restrict__ jlist,
+                                                              const VT *__restrict__ val,
+                                                              const typename Vec2<T>::type *__restrict__ xv) {
+  typedef typename Vec2<T>::type T2;
+  const int sl = lane_id() & 31;
+  T a0 = 0, a1 = 0, a2 = 0, a3 = 0, b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+  const int end = beg + cnt;
+  int k = beg + sl;
+  for (; k + 96 < end; k += 128) {
+    const int j0 = jlist[k], j1 = jlist[k + 32], j2 = jlist[k + 64], j3 = jlist[k + 96];
+    const T v0 = (T)val[k], v1 = (T)val[k + 32], v2 = (T)val[k + 64], v3 = (T)val[k + 96];
+    const T2 x0 = xv[j0], x1 = xv[j1], x2 = xv[j2], x3 = xv[j3];
+    a0 += v0 * x0.x; b0 += v0 * x0.y;
+    a1 += v1 * x1.x; b1 += v1 * x1.y;
+    a2 += v2 * x2.x; b2 += v2 * x2.y;
+    a3 += v3 * x3.x; b3 += v3 * x3.y;
+  }
+  for (; k < end; k += 32) {
+    const T2 xx = xv[jlist[k]];
+    const T v = (T)val[k];
+    a0 += v * xx.x;
+    b0 += v * xx.y;
+  }
+  T a = (a0 + a1) + (a2 + a3), b = (b0 + b1) + (b2 + b3);
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, SWR_WAVE);
+    b += __shfl_xor(b, o, SWR_WAVE);
+  }
+  T2 r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+
+template <typename T2, typename T>
+__device__ __forceinline__ T2 mk2(T a, T b) {
+  T2 r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+
+// Whether any solve still runs in phase PH (PH_A1, PH_K read the scalars).
+__device__ __forceinline__ bool cg_any_active(int PH, const CgScalars *__restrict__ cgs) {
+  if (PH != PH_A1 && PH != PH_K) return true;
+  const CgScalars &c0 = cgs[0], &c1 = cgs[1];
+  return (PH == PH_K && (c0.active_b || c1.active_b)) || c0.active_a || c1.active_a;
+}
+
+// q = eta d + H d (sparse_matvec_C_para, fix_qeq_reax_sw64.c:424-466) of the
+// vector phase PH multiplies: x (PH_S1), d0 (PH_S2), d1 (PH_S3) or the
+// current d (PH_K).  Half a wave per row, two rows per wave.
+template <int PH, typename T, typename VT>
+__global__ void __launch_bounds__(QT)
+k_cg_spmv(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
+          const int *__restrict__ numnbrs, const int *__restrict__ jlist, const VT *__restrict__ val, CgVecsT<T> V,
+          int par_in, const CgScalars *__restrict__ cgs) {
+  typedef typename Vec2<T>::type T2;
+  if (!cg_any_active(PH, cgs)) return;  // both solves finished
+  const int w = wave_in_block(), lane = lane_id();
+  const int i = blockIdx.x * RPB + w * 2 + (lane >> 5);
+  const T2 *src = PH == PH_S1 ? V.x : PH == PH_S2 ? V.d[0] : PH == PH_S3 ? V.d[1] : (par_in ? V.d[1] : V.d[0]);
+  const int beg = i < n ? firstnbr[i] : 0, cnt = i < n ? numnbrs[i] : 0;
+  T2 q = row_spmv2_h<T, VT>(beg, cnt, jlist, val, src);
+  if ((lane & 31) == 0 && i < n) {
+    const T eta = (T)P->eta[type[i]];
+    const T2 xi = src[i];
+    q.x = eta * xi.x + q.x;
+    q.y = eta * xi.y + q.y;
+    V.q[i] = q;
+  }
+}
+
+// The vector part of phase PH, one thread per row.  PH_S1..S3 are the setup
+// steps of CG_v2 (:1020-1087), PH_A1 the first A step, PH_K the fused
+// iteration.  xzero (PH_S1 only): x is the zero vector (written here), so
+// r = b and no SpMV ran.
+template <int PH, typename T>
+__global__ void __launch_bounds__(UT)
+k_cg_upd(int n, CgVecsT<T> V, int par_in, const CgScalars *__restrict__ cgs, double *__restrict__ partials,
+         int pstride, int xzero) {
+  typedef typename Vec2<T>::type T2;
+  CgScalars c0, c1;
+  bool b0 = false, b1 = false, a0 = true, a1 = true;
+  if (PH == PH_A1 || PH == PH_K) {
+    c0 = cgs[0];
+    c1 = cgs[1];
+    b0 = PH == PH_K && c0.active_b != 0;
+    b1 = PH == PH_K && c1.active_b != 0;
+    a0 = c0.active_a != 0;
+    a1 = c1.active_a != 0;
+    if (!b0 && !b1 && !a0 && !a1) return;  // both solves finished
+  }
+  const int i = blockIdx.x * UT + threadIdx.x;
+  double dots[6] = {0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    const T Hd = (T)V.Hd[i];
+    const T2 q = (PH == PH_A1 || (PH == PH_S1 && xzero)) ? mk2<T2>((T)0, (T)0) : V.q[i];
+    if (PH == PH_S1) {
+      if (xzero) V.x[i] = mk2<T2>((T)0, (T)0);
+      const T2 b = V.b[i];
+      const T2 r = mk2<T2>(b.x - q.x, b.y - q.y);
+      const T2 u = mk2<T2>(r.x * Hd, r.y * Hd);
+      V.r[i] = r; V.u[i] = u; V.d[0][i] = u;
+    } else if (PH == PH_S2) {
+      V.w[i] = q;
+      const T2 m = mk2<T2>(q.x * Hd, q.y * Hd);
+      V.m[i] = m; V.d[1][i] = m;
+    } else if (PH == PH_S3) {
+      const T2 b = V.b[i], u = V.u[i], r = V.r[i], ww = V.w[i], m = V.m[i];
+      V.z[i] = q; V.p[i] = u; V.ss[i] = ww; V.v[i] = m;
+      dots[0] = (double)b.x * b.x; dots[1] = (double)u.x * r.x; dots[2] = (double)u.x * ww.x;
+      dots[3] = (double)b.y * b.y; dots[4] = (double)u.y * r.y; dots[5] = (double)u.y * ww.y;
+    } else {
+      T2 xx = V.x[i], p = V.p[i], u = V.u[i], ss = V.ss[i], w2 = V.w[i], v = V.v[i], z = V.z[i], r = V.r[i];
+      const T2 dc = PH == PH_K ? (par_in ? V.d[1] : V.d[0])[i] : mk2<T2>((T)0, (T)0);
+      T2 dn = dc;
+      // B: x += alpha p; p = u + p beta; ss = w + ss beta; v = d + v beta   (:1135-1141)
+      // C: z = q + z beta                                                   (:1147-1150)
+      if (PH == PH_K && b0) {
+        const T al = (T)c0.alpha_x, be = (T)c0.beta;
+        xx.x += al * p.x; p.x = u.x + p.x * be; ss.x = w2.x + ss.x * be; v.x = dc.x + v.x * be;
+        z.x = q.x + z.x * be;
+      }
+      if (PH == PH_K && b1) {
+        const T al = (T)c1.alpha_x, be = (T)c1.beta;
+        xx.y += al * p.y; p.y = u.y + p.y * be; ss.y = w2.y + ss.y * be; v.y = dc.y + v.y * be;
+        z.y = q.y + z.y * be;
+      }
+      // A of the next iteration: r -= alpha ss; u -= alpha v; w -= alpha z; d = M^-1 w (:1113-1123)
+      if (a0) {
+        const T al = (T)c0.alpha_a;
+        r.x -= al * ss.x; u.x -= al * v.x; w2.x -= al * z.x;
+        dots[0] = (double)u.x * r.x; dots[1] = (double)u.x * w2.x;
+        dn.x = w2.x * Hd;
+      }
+      if (a1) {
+        const T al = (T)c1.alpha_a;
+        r.y -= al * ss.y; u.y -= al * v.y; w2.y -= al * z.y;
+        dots[2] = (double)u.y * r.y; dots[3] = (double)u.y * w2.y;
+        dn.y = w2.y * Hd;
+      }
+      if (PH == PH_K && (b0 || b1)) {
+        V.x[i] = xx; V.p[i] = p; V.ss[i] = ss; V.v[i] = v; V.z[i] = z;
+      }
+      if (a0 || a1) {
+        V.r[i] = r; V.u[i] = u; V.w[i] = w2;
+        (PH == PH_A1 ? V.d[0] : (par_in ? V.d[0] : V.d[1]))[i] = dn;
+      }
+    }
+  }
+  if (PH == PH_S1 || PH == PH_S2) return;
+  block_partials<PH == PH_S3 ? 6 : 4, UT>(dots, partials, pstride, blockIdx.x);
+}
+
+// Scalars of the iteration whose dots k_cg_upd<PH> just wrote (one workgroup).
+// tol > 0 sets both solves' tolerance at setup; tol <= 0 keeps the values
+// already in cgs[].tol (a negative one marks a solve with nothing to do).
+// The iteration count of a finished solve goes to iters_out[s].
+template <int PH>
+__global__ void __launch_bounds__(RT)
+k_cg_scalars(int nblk, const double *__restrict__ partials, int pstride, CgScalars *__restrict__ cgs, int imax,
+             double tol, DevStatus *st, int *__restrict__ iters_out) {
+  CgScalars c0 = cgs[0], c1 = cgs[1];
+  const bool b0 = PH == PH_K && c0.active_b != 0, b1 = PH == PH_K && c1.active_b != 0;
+  const bool a0 = PH == PH_S3 || c0.active_a != 0, a1 = PH == PH_S3 || c1.active_a != 0;
+  if (!b0 && !b1 && !a0 && !a1) return;
+  constexpr int nd = PH == PH_S3 ? 6 : 4;
+  double tot[nd];
+  reduce_partials<nd>(partials, pstride, nblk, tot);
+  if (threadIdx.x != 0) return;
+  unsigned int bad = 0;
+  if constexpr (PH == PH_S3) {
+    auto setup = [&](CgScalars &cc, double bb, double ur, double uw, int s) {
+      if (tol > 0) cc.tol = tol;
+      cc.bdot = bb; cc.udot_r = ur; cc.udot_w = uw;
+      cc.b_norm = sqrt(bb);
+      cc.sig_old = ur;
+      cc.heta = uw;
+      cc.alpha_a = cc.sig_old / cc.heta;
+      cc.alpha_x = 0.0;
+      cc.beta = 0.0;
+      cc.dg0 = cc.sig_old;
+      cc.dg1 = 0.0;
+      cc.iter = 1;
+      cc.active_b = 0;
+      cc.active_a = (1 < imax && cc.tol > 0 && cc.b_norm > 0 && sqrt(cc.dg0) / cc.b_norm > cc.tol) ? 1 : 0;
+      cc.done = cc.active_a ? 0 : 1;
+      if (cc.done) iters_out[s] = cc.iter;
+      if (!isfinite(cc.alpha_a) && cc.active_a) bad = 1;
+    };
+    setup(c0, tot[0], tot[1], tot[2], 0);
+    setup(c1, tot[3], tot[4], tot[5], 1);
+  } else {
+    auto fin = [&](CgScalars &cc, bool act_a, bool act_b, double l0, double l1, int s) {
+      if (act_a) {
+        cg_after_A(cc, l0, l1, imax);
+        if (!isfinite(cc.alpha_a) || !isfinite(cc.beta)) bad = 1;
+      } else if (act_b) {
+        cc.active_b = 0;
+        cc.done = 1;
+        iters_out[s] = cc.iter;
+      }
+    };
+    fin(c0, a0, b0, tot[0], tot[1], 0);
+    fin(c1, a1, b1, tot[2], tot[3], 1);
+  }
+  cgs[0] = c0;
+  cgs[1] = c1;
+  if (bad) atomicOr(&st->flags, SWR_F_NAN);
+}
+
+// Everything one CG solve needs from the system.
+template <typename T, typename VT>
+struct CgRun {
+  CgVecsT<T> V;
+  const VT *val;
+  CgScalars *cgs;     // two consecutive entries (s, t)
+  int *iters_out;     // two ints on the device
+  double tol;         // > 0: set at setup; <= 0: preset in cgs[].tol
+  int imax;           // iteration limit (the reference's imax = 200 for the FP64 solve)
+  int kt_iter, kt_update, kt_setup, kt_scalars;
+};
+
+template <typename T, typename VT>
+int launch_cg(swr_system *S, const CgRun<T, VT> &R, int ph, int par_in, int xzero) {
+  const int n = S->n;
+  const int nsp = div_up(n, RPB), nup = div_up(n, UT);
+  const int ps = S->cg_blocks;
+  const int kt = ph == PH_K ? R.kt_iter : R.kt_setup;
+  const int ktu = ph == PH_K ? R.kt_update : R.kt_setup;
+  const int kts = ph == PH_K ? R.kt_scalars : R.kt_setup;
+#define SP_ARGS n, S->dp, S->type, S->H_firstnbr, S->H_numnbrs, S->H_jlist, R.val, R.V, par_in, R.cgs
+#define UP_ARGS n, R.V, par_in, R.cgs, S->partials, ps, xzero
+#define SC_ARGS nup, S->partials, ps, R.cgs, R.imax, R.tol, S->status, R.iters_out
+  swr_kt_begin(S, kt);
+  switch (ph) {
+    case PH_S1:
+      if (!xzero) hipLaunchKernelGGL((k_cg_spmv<PH_S1, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS);
+      break;
+    case PH_S2: hipLaunchKernelGGL((k_cg_spmv<PH_S2, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
+    case PH_S3: hipLaunchKernelGGL((k_cg_spmv<PH_S3, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
+    case PH_A1: break;
+    default: hipLaunchKernelGGL((k_cg_spmv<PH_K, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
+  }
+  swr_kt_end(S, kt);
+  swr_kt_begin(S, ktu);
+  switch (ph) {
+    case PH_S1: hipLaunchKernelGGL((k_cg_upd<PH_S1, T>), dim3(nup), dim3(UT), 0, S->stream, UP_ARGS); break;
+    case PH_S2: hipLaunchKernelGGL((k_cg_upd<PH_S2, T>), dim3(nup), dim3(UT), 0, S->stream, UP_ARGS); break;
+    case PH_S3: hipLaunchKernelGGL((k_cg_upd<PH_S3, T>), dim3(nup), dim3(UT), 0, S->stream, UP_ARGS); break;
+    case PH_A1: hipLaunchKernelGGL((k_cg_upd<PH_A1, T>), dim3(nup), dim3(UT), 0, S->stream, UP_ARGS); break;
+    default: hipLaunchKernelGGL((k_cg_upd<PH_K, T>), dim3(nup), dim3(UT), 0, S->stream, UP_ARGS); break;
+  }
+  swr_kt_end(S, ktu);
+  swr_kt_begin(S, kts);
+  switch (ph) {
+    case PH_S1: case PH_S2: break;
+    case PH_S3: hipLaunchKernelGGL(k_cg_scalars<PH_S3>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS); break;
+    case PH_A1: hipLaunchKernelGGL(k_cg_scalars<PH_A1>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS); break;
+    default: hipLaunchKernelGGL(k_cg_scalars<PH_K>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS); break;
+  }
+  swr_kt_end(S, kts);
+#undef SP_ARGS
+#undef UP_ARGS
+#undef SC_ARGS
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+// Runs both solves of R to convergence.  Iteration kernels are enqueued in
+// batches (the first of first_batch launches); between batches the host
+// reads the two done flags.  Launches enqueued after both solves finished
+// exit at once.
+template <typename T, typename VT>
+int cg_solve(swr_system *S, const CgRun<T, VT> &R, int first_batch, int xzero) {
+  int rc;
+  if ((rc = launch_cg(S, R, PH_S1, 0, xzero)) || (rc = launch_cg(S, R, PH_S2, 0, 0)) ||
+      (rc = launch_cg(S, R, PH_S3, 0, 0)) || (rc = launch_cg(S, R, PH_A1, 0, 0)))
+    return rc;
+  int k = 1;
+  int batch = first_batch > 2 ? first_batch : 2;
+  while (k < R.imax + 1) {
+    for (int b = 0; b < batch && k < R.imax + 1; b++, k++)
+      if ((rc = launch_cg(S, R, PH_K, (k - 1) & 1, 0))) return rc;
+    CgScalars h[2];
+    HIP_CHECK(hipMemcpyAsync(h, R.cgs, sizeof(h), hipMemcpyDeviceToHost, S->stream));
+    HIP_CHECK(hipStreamSynchronize(S->stream));
+    if (h[0].done && h[1].done) break;
+    batch = 4;
+  }
+  return SWR_OK;
+}
+
+}  // namespace swr_cg

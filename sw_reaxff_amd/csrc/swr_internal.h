@@ -29,6 +29,7 @@
 #define SWR_F_MARK_DEPTH (1u << 5)   // bond_mark relaxation did not converge
 #define SWR_F_BIN        (1u << 6)   // an atom fell outside the bin grid
 #define SWR_F_NAN        (1u << 7)   // non-finite value in CG scalars
+#define SWR_F_CG_MAXIT   (1u << 8)   // a QEq solve stopped at its iteration limit (a warning, as in the reference)
 
 #define SWR_MAX_ATOM_BONDS 64        // bonds one atom may hold while building its list
 
@@ -87,7 +88,7 @@ struct DevStatus {
   long long num_far;     // far entries written (sum over atoms)
   long long H_nnz;
   int cg_iters[2];       // s, t
-  int pad1[2];
+  int inner_iters[2];    // last FP32 inner solve of the mixed-precision refinement
 };
 
 // Pipelined-CG scalars of one right-hand side (CG_v2,
@@ -101,11 +102,16 @@ struct CgScalars {
   double dg0, dg1;    // latest reduced u.r, u.w
   double b_norm;
   double bdot, udot_r, udot_w;   // setup dots
+  double tol;         // stopping tolerance (negative: nothing to solve)
   int active_b;       // B/C of the current iteration run
   int active_a;       // A of the next iteration runs
   int iter;           // reference loop counter i
   int done;
 };
+
+// Kernel timer slots.
+enum { KT_VERLET = 0, KT_FAR, KT_BUILD_H, KT_CG_ITER, KT_CG_SETUP, KT_BO, KT_VALIDATE, KT_SPMV, KT_RESET,
+       KT_QEQ_Q, KT_BINS, KT_MIXED_INNER, KT_MIXED_OUTER, KT_BONDS, KT_CG_SCALARS, KT_MARK, KT_CG_UPDATE, KT_COUNT };
 
 struct swr_system {
   // ---- host-side configuration
@@ -176,13 +182,14 @@ struct swr_system {
   double *st = nullptr;            // [2][ncap]: s, t
   double *s_hist = nullptr, *t_hist = nullptr;  // [n][5]
   double *cgv = nullptr;           // CG vectors, see CGV_* below
-  CgScalars *cgs = nullptr;        // [2]
+  CgScalars *cgs = nullptr;        // [4]: s, t of the FP64 solve; s, t of the mixed inner solve
   double *partials = nullptr;      // per-workgroup partial dots
   unsigned int *counters = nullptr;
   int cg_blocks = 0;
   int qeq_initialized = 0;
   int mixed = 0;                   // mixed-precision QEq
   void *mixw = nullptr;            // mixed-precision workspace
+  size_t mixw_bytes = 0;
   int imax = 200;
 
   // ---- status
@@ -204,12 +211,9 @@ struct swr_system {
     double ms = 0.0;
     long long count = 0;
   };
-  KTimer kt[16];
+  KTimer kt[KT_COUNT];
 };
 
-// Kernel timer slots.
-enum { KT_VERLET = 0, KT_FAR, KT_BUILD_H, KT_CG_ITER, KT_CG_SETUP, KT_BO, KT_VALIDATE, KT_SPMV, KT_RESET,
-       KT_QEQ_Q, KT_BINS, KT_MIXED_INNER, KT_MIXED_OUTER, KT_BONDS, KT_CG_SCALARS, KT_COUNT };
 extern const char *const swr_kt_names[KT_COUNT];
 void swr_kt_begin(swr_system *S, int k);
 void swr_kt_end(swr_system *S, int k);
@@ -220,12 +224,34 @@ enum {
   WS_CLP, WS_DELTA_LP, WS_DDELTA_LP, WS_DELTA_LP_TEMP, WS_DDELTA_LP_TEMP, WS_COUNT
 };
 // CG vector slots (each ncap doubles, two right-hand sides interleaved per slot)
-enum { CGV_R = 0, CGV_U, CGV_W, CGV_Z, CGV_SS, CGV_V, CGV_P, CGV_D0, CGV_D1, CGV_M, CGV_COUNT };
+enum { CGV_R = 0, CGV_U, CGV_W, CGV_Z, CGV_SS, CGV_V, CGV_P, CGV_D0, CGV_D1, CGV_M, CGV_Q, CGV_COUNT };
 
 // ---- helpers implemented in the .hip files
 int swr_scan_i32_to_i64(swr_system *S, const int *in, long long *out, int n, long long *total_host);
 int swr_scan_i32(swr_system *S, const int *in, int *out, int n, int *total_dev);
 int swr_ensure_tmp(swr_system *S, size_t bytes);
+
+// Stage implementations (nbr.hip, lists.hip, bo.hip, qeq.hip, qeq_mixed.hip).
+int swr_nbr_set_grid(swr_system *S, const double lo[3], const double hi[3], double cutghost, double cutneigh);
+int swr_nbr_build(swr_system *S, int local_only);
+int swr_nbr_borders(swr_system *S, const double lo[3], const double hi[3], double cutghost);
+int swr_nbr_forward_x(swr_system *S, const double lo[3], const double hi[3]);
+int swr_nbr_nve(swr_system *S, double dt);
+int swr_nbr_pbc(swr_system *S, const double lo[3], const double hi[3]);
+int swr_nbr_ensure_atoms(swr_system *S, int Ncap);
+int swr_lists_pack(swr_system *S);
+int swr_lists_reset(swr_system *S);
+int swr_lists_run(swr_system *S, int mode);
+int swr_lists_estimate(swr_system *S);
+int swr_lists_validate(swr_system *S);
+int swr_lists_mark(swr_system *S, int max_sweeps);
+int swr_bo_run(swr_system *S);
+int swr_qeq_build_H(swr_system *S);
+int swr_qeq_spmv(swr_system *S, const double *xv, double *b);
+int swr_qeq_cg_run(swr_system *S, int first_batch);
+int swr_qeq_calc_q(swr_system *S);
+int swr_qeq_blocks(int n);
+int swr_qeq_mixed_solve(swr_system *S);
 
 template <class T>
 inline int swr_alloc(T **p, size_t count) {

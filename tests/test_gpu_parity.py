@@ -199,5 +199,6 @@ def test_split_stages_equal_fused(c0_setup, gpu_lib, ffield):
         assert np.array_equal(split[k], fused[k]), k
     # restore the post-BO state for any later test
     h.call("validate_lists")
+    h.call("sort_bonds")
     h.call("bo")
     h.sync()

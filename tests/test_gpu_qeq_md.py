@@ -145,3 +145,26 @@ def test_bond_order_radius_table_is_conservative(gpu_lib, oracle_lib, ffield):
             assert bop(r * (1 - 1e-7)) >= bo_cut
             for d in np.linspace(r, 5.0, 50):
                 assert bop(d) < bo_cut
+
+
+def test_atomonly_list_is_the_local_rows_of_the_ghost_list(gpu_lib, ffield):
+    """npair full/bin/atomonly: local rows equal the ghost list's local rows
+    (as sets), ghost rows are empty (gnum = 0)."""
+    import ctypes as C
+    s = water_c0(seed=4)
+    lmp = harness(s, ffield)
+    h = lmp.system
+    n, N = h.scalar("n"), h.scalar("N")
+    off, lst = verlet_of(h)
+    gpu_lib.check(h.L.swr_npair_full_bin_atomonly_build(h.h, C.c_double(12.0)), "atomonly")
+    h.sync()
+    aoff, alst = verlet_of(h)
+    assert aoff.size == N + 1
+    assert np.all(np.diff(aoff)[n:] == 0)
+    for i in range(0, n, 3):
+        assert sorted(alst[aoff[i]:aoff[i + 1]].tolist()) == sorted(lst[off[i]:off[i + 1]].tolist()), i
+    # the ghost list again, through the entry point
+    gpu_lib.check(h.L.swr_npair_full_bin_ghost_build(h.h, C.c_double(12.0)), "ghost")
+    h.sync()
+    goff, glst = verlet_of(h)
+    assert np.array_equal(np.diff(goff), np.diff(off))
