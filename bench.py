@@ -181,12 +181,72 @@ def copy_calibration(mib):
                       "launches": 5}), flush=True)
 
 
+def bench_clusters(args):
+    """512 non-periodic water/methanol clusters (sizes log-uniform in [50, 3000]):
+    one batched evaluation (neighbour list, QEq, lists, BO; one launch per
+    stage for all clusters) against the plain per-system loop of the same
+    library, both from host arrays to results on the device."""
+    from sw_reaxff_amd import batch as B
+    count, lo, hi = 512, 50, 3000
+    sizes = systems.cluster_sizes(count, lo, hi, seed=args.seed)
+    log(f"building {count} clusters, {int(sizes.sum()):,} atoms (seed {args.seed})")
+    parts = [systems.cluster(int(k), seed=args.seed + 1 + i) for i, k in enumerate(sizes)]
+    natoms = sum(p.n for p in parts)
+    bt = B.Batch(tol=1e-6)
+    log("batched: warm-up evaluation")
+    bt.load(parts)
+    bt.evaluate()
+    bt.sync()
+    lat = []
+    for _ in range(max(1, args.steps)):
+        device_sync()
+        t0 = time.perf_counter()
+        bt.load(parts)
+        its = bt.evaluate()
+        bt.sync()
+        lat.append(time.perf_counter() - t0)
+    t_batch = float(np.median(lat))
+    log(f"batched: {t_batch * 1e3:.1f} ms per batch of {count} (CG iterations max {its})")
+    # plain loop: one system at a time through the same library (sampled)
+    nplain = min(count, args.plain_sample)
+    one = B.Batch(tol=1e-6)
+    one.load([parts[0]])
+    one.evaluate()
+    one.sync()
+    device_sync()
+    t0 = time.perf_counter()
+    for k in range(nplain):
+        one.load([parts[k]])
+        one.evaluate()
+        one.sync()
+    t_plain = (time.perf_counter() - t0) / nplain
+    log(f"plain loop: {t_plain * 1e3:.2f} ms per system ({nplain} systems)")
+    sps_batch, sps_plain = count / t_batch, 1.0 / t_plain
+    out = {
+        "metric": "batched systems/s (nbr+BO+QEq, 512 clusters)", "value": round(sps_batch, 1),
+        "unit": "systems/s", "n_gpus": 1, "steps": len(lat), "warmup": 1,
+        "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{count} water/methanol clusters, sizes log-uniform [{lo}, {hi}], "
+                               f"{natoms:,} atoms, 0.9 g/cm^3, 12 A vacuum padding, QEq tol 1e-6",
+                   "systems": count, "atoms": natoms, "seed": args.seed, "parallelism": "single GPU"},
+        "plain_loop_systems_per_s": round(sps_plain, 1), "speedup_vs_plain_loop": round(sps_batch / sps_plain, 2),
+        "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                             "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+        "atom_evals_per_s": round(natoms / t_batch, 1), "cg_iters_max": list(its),
+        "plain_sample": nplain, "host": platform.node(),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="water96k", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="water96k", choices=sorted(CONFIGS) + ["clusters512"])
+    ap.add_argument("--plain-sample", type=int, default=128,
+                    help="clusters timed one at a time for the plain-loop comparison")
     ap.add_argument("--mixed", action="store_true", help="mixed-precision QEq")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--every", type=int, default=10)
@@ -199,6 +259,13 @@ def main():
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this path has no multi-GPU form (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.config == "clusters512":
+        bench_clusters(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this path has no multi-GPU form (asked for --gpus {args.gpus}).",

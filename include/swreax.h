@@ -80,6 +80,14 @@ int swr_destroy(swr_system *sys);
 int swr_set_atoms(swr_system *sys, int n, int N, const int *tag, const int *type,
                   const double *x, const double *q, const int *owner,
                   const signed char *img);
+/* The atoms of the last swr_set_atoms are a new configuration, not the next
+ * MD step of the previous one: clears the QEq warm-start history (s, t and
+ * their 5-step history, fix_qeq_reax_sunway.cpp:518-535 at first setup) and
+ * re-estimates the list capacities at the next Reset (Estimate_Storages,
+ * reaxc_forces_sunway.cpp:1190-1292).  A caller stepping one system keeps
+ * both by not calling it. */
+int swr_new_atoms(swr_system *sys);
+
 /* Per-atom capacities (fix REAXC num_bonds/num_hbonds, fix_reaxc_sunway.cpp:39-161). */
 int swr_set_capacities(swr_system *sys, const int *num_bonds, const int *num_hbonds);
 
@@ -167,6 +175,20 @@ int swr_qeq_pre_force(swr_system *sys, int *iters_s, int *iters_t);
  * pair_reaxc_sunway.cpp:541-793, through BO): pack, Reset, lists + bonds,
  * Validate_Lists, sort_bonds, BO. */
 int swr_pair_compute(swr_system *sys);
+
+/* Batch of independent systems in one swr_system (no reference counterpart:
+ * the batched and serving entry points of this build).  The local atoms of
+ * system k are rows [row_offsets[k], row_offsets[k+1]) (row_offsets[0] = 0,
+ * row_offsets[nsys] = n); their ghosts may come in any order after n, with
+ * owner[] pointing at the system's own local atoms.  Systems must be placed
+ * so that no atom of one lies within the neighbour cutoff of another.  Every
+ * stage then runs once over all systems (one launch per stage); QEq keeps
+ * per-system CG scalars, stopping tests and charge neutrality, so each
+ * system's charges are those it gets alone.  Per-system iteration counts:
+ * swr_get(sys, "sys_iters", ...) (2 ints per system: s, t).  Reset to one
+ * system whenever swr_set_atoms changes the local atom count.  The
+ * mixed-precision solve takes one system at a time. */
+int swr_set_batch(swr_system *sys, int nsys, const int *row_offsets);
 
 /* Join the stream; returns the first device-raised error. */
 int swr_sync(swr_system *sys);

@@ -27,7 +27,8 @@ EXPORTS = [
     "swr_pair_compute", "swr_sync", "swr_get", "swr_put", "swr_get_scalar", "swr_device_ptr",
     "swr_lmp_open", "swr_lmp_close", "swr_lmp_command", "swr_lmp_create_atoms", "swr_lmp_setup", "swr_lmp_run",
     "swr_lmp_system", "swr_lmp_stat", "swr_ktimer_enable", "swr_ktimer_read", "swr_device_synchronize",
-    "swr_sparse_matvec_h", "swr_npair_full_bin_atomonly_build", "swr_sort_bonds",
+    "swr_sparse_matvec_h", "swr_npair_full_bin_atomonly_build", "swr_sort_bonds", "swr_set_batch",
+    "swr_new_atoms",
 ]
 
 
@@ -64,6 +65,8 @@ def lib():
         L.swr_set_capacities.argtypes = [vp, ip, ip]
         L.swr_npair_full_bin_ghost_build.argtypes = [vp, C.c_double]
         L.swr_npair_full_bin_atomonly_build.argtypes = [vp, C.c_double]
+        L.swr_set_batch.argtypes = [vp, C.c_int, ip]
+        L.swr_new_atoms.argtypes = [vp]
         for nm in ["swr_write_reax_atoms_and_pack", "swr_reset", "swr_write_reax_lists", "swr_init_forces_noqeq_full",
                    "swr_write_reax_lists_init_forces", "swr_validate_lists", "swr_sort_bonds", "swr_bo",
                    "swr_qeq_init_matvec",
@@ -124,7 +127,7 @@ DTYPES = {
     "far_index": np.int32, "far_end": np.int32, "num_bonds": np.int32, "num_hbonds": np.int32,
     "Hindex": np.int32, "bond_index": np.int32, "bond_end": np.int32, "bonds": BOND_DT, "bo_data": BO_DT,
     "hbond_index": np.int32, "hbond_end": np.int32, "hbonds": HBOND_DT, "hbond_far": np.int64,
-    "H_val32": np.float32,
+    "H_val32": np.float32, "sys_iters": np.int32,
     "bond_mark": np.int32, "H_firstnbr": np.int32, "H_numnbrs": np.int32, "H_jlist": np.int32,
 }
 

@@ -170,12 +170,11 @@ static int ensure_local(swr_system *S, int ncap) {
     HIP_CHECK(hipMemsetAsync(S->t_hist + o * 5, 0, (nn - o) * 5 * sizeof(double), S->stream));
     HIP_CHECK(hipMemsetAsync(S->st + o * 2, 0, (nn - o) * 2 * sizeof(double), S->stream));
   }
-  S->cg_blocks = swr_qeq_blocks(ncap);
   if ((rc = swr_alloc(&S->H_firstnbr, nn)) || (rc = swr_alloc(&S->H_numnbrs, nn)) ||
       (rc = swr_alloc(&S->Hdia_inv, nn)) || (rc = swr_alloc(&S->b_s, 2 * nn)) ||
-      (rc = swr_alloc(&S->cgv, (size_t)CGV_COUNT * 2 * nn)) ||
-      (rc = swr_alloc(&S->partials, (size_t)(S->cg_blocks + 2) * 8)))
+      (rc = swr_alloc(&S->cgv, (size_t)CGV_COUNT * 2 * nn)))
     return rc;
+  S->layout_dirty = true;
   S->b_t = nullptr;
   S->ncap = ncap;
   return SWR_OK;
@@ -241,8 +240,7 @@ int swr_create(const char *ffield_path, int ntypes, const char *const *elements,
     return SWR_ERR_HIP;
   }
   int rc;
-  if ((rc = swr_alloc(&S->dp, 1)) || (rc = swr_alloc(&S->status, 1)) || (rc = swr_alloc(&S->cgs, 4)) ||
-      (rc = swr_alloc(&S->counters, 4))) {
+  if ((rc = swr_alloc(&S->dp, 1)) || (rc = swr_alloc(&S->status, 1)) || (rc = swr_alloc(&S->counters, 4))) {
     swr_destroy(S);
     return rc;
   }
@@ -255,7 +253,6 @@ int swr_create(const char *ffield_path, int ntypes, const char *const *elements,
   (void)hipMemcpy(S->dp, &S->hp, sizeof(DevParams), hipMemcpyHostToDevice);
   (void)hipMemset(S->status, 0, sizeof(DevStatus));
   (void)hipMemset(S->counters, 0, 4 * sizeof(unsigned int));
-  (void)hipMemset(S->cgs, 0, 4 * sizeof(CgScalars));
   *out = S;
   return SWR_OK;
 }
@@ -270,7 +267,8 @@ int swr_destroy(swr_system *S) {
                   S->Cdbopi2, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,
                   S->bond_mark2, S->counts, S->ws, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32,
                   S->Hdia_inv, S->b_s, S->st, S->s_hist, S->t_hist, S->cgv, S->cgs, S->partials, S->counters,
-                  S->status, S->scan_tmp, S->tmp_i, S->mixw, S->cand, S->cand_count};
+                  S->status, S->scan_tmp, S->tmp_i, S->mixw, S->cand, S->cand_count, S->row_sys, S->tile_beg,
+                  S->tile_sys, S->sys_tile, S->sys_iters, S->done_count, S->qsum};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (S->status_host) (void)hipHostFree(S->status_host);
@@ -292,7 +290,11 @@ int swr_set_atoms(swr_system *S, int n, int N, const int *tag, const int *type, 
     }
   int rc;
   if ((rc = swr_nbr_ensure_atoms(S, std::max(N, 1))) || (rc = ensure_local(S, std::max(n, 1)))) return rc;
-  if (n != S->n) S->qeq_initialized = 0;
+  if (n != S->n) {
+    S->qeq_initialized = 0;
+    S->sys_row.clear();  // a new atom set is one system until swr_set_batch
+  }
+  S->layout_dirty = true;
   S->n = n;
   S->N = N;
   HIP_CHECK(hipMemcpyAsync(S->x, x, sizeof(double) * 3 * N, hipMemcpyHostToDevice, S->stream));
@@ -312,6 +314,18 @@ int swr_set_atoms(swr_system *S, int n, int N, const int *tag, const int *type, 
   HIP_CHECK(hipMemcpyAsync(S->owner, own.data(), sizeof(int) * N, hipMemcpyHostToDevice, S->stream));
   HIP_CHECK(hipMemcpyAsync(S->img, im.data(), 3 * (size_t)N, hipMemcpyHostToDevice, S->stream));
   HIP_CHECK(hipStreamSynchronize(S->stream));
+  return SWR_OK;
+}
+
+int swr_new_atoms(swr_system *S) {
+  if (!S) return SWR_ERR_ARG;
+  if (S->ncap > 0) {
+    HIP_CHECK(hipMemsetAsync(S->s_hist, 0, sizeof(double) * 5 * S->ncap, S->stream));
+    HIP_CHECK(hipMemsetAsync(S->t_hist, 0, sizeof(double) * 5 * S->ncap, S->stream));
+    HIP_CHECK(hipMemsetAsync(S->st, 0, sizeof(double) * 2 * S->ncap, S->stream));
+  }
+  S->capacities_set = false;
+  S->h_sys_iters.assign(S->h_sys_iters.size(), 0);
   return SWR_OK;
 }
 
@@ -463,12 +477,41 @@ int swr_sparse_matvec_h(swr_system *S, const double *x_host, double *b_host) {
 
 int swr_qeq_cg(swr_system *S, int *iters_s, int *iters_t) {
   if (!S || !S->H_val) return SWR_ERR_STATE;
-  int rc = S->mixed ? swr_qeq_mixed_solve(S) : swr_qeq_cg_run(S, S->status_host->cg_iters[0] + 2);
+  int its0 = 0;
+  for (size_t k = 0; k < S->h_sys_iters.size(); k += 2) its0 = std::max(its0, S->h_sys_iters[k]);
+  int rc = S->mixed ? swr_qeq_mixed_solve(S) : swr_qeq_cg_run(S, its0 + 2);
   if (rc) return rc;
-  if ((rc = refresh_status(S))) return rc;
-  if (iters_s) *iters_s = S->status_host->cg_iters[0];
-  if (iters_t) *iters_t = S->status_host->cg_iters[1];
+  S->h_sys_iters.resize(2 * (size_t)S->nsys);
+  HIP_CHECK(hipMemcpyAsync(S->h_sys_iters.data(), S->sys_iters, sizeof(int) * 2 * S->nsys, hipMemcpyDeviceToHost,
+                           S->stream));
+  if ((rc = refresh_status(S))) return rc;  // synchronises the stream
+  int ms = 0, mt = 0;
+  for (int k = 0; k < S->nsys; k++) {
+    ms = std::max(ms, S->h_sys_iters[2 * k]);
+    mt = std::max(mt, S->h_sys_iters[2 * k + 1]);
+  }
+  if (iters_s) *iters_s = ms;
+  if (iters_t) *iters_t = mt;
   return status_to_rc(S->status_host->flags);
+}
+
+int swr_set_batch(swr_system *S, int nsys, const int *row_offsets) {
+  if (!S || nsys < 1 || !row_offsets || row_offsets[0] != 0 || row_offsets[nsys] != S->n) {
+    swr_set_error_msg("swr_set_batch: offsets must run from 0 to the local atom count");
+    return SWR_ERR_ARG;
+  }
+  for (int k = 0; k < nsys; k++)
+    if (row_offsets[k + 1] < row_offsets[k]) {
+      swr_set_error_msg("swr_set_batch: offsets must not decrease");
+      return SWR_ERR_ARG;
+    }
+  if (S->mixed && nsys > 1) {
+    swr_set_error_msg("swr_set_batch: the mixed-precision solve runs one system at a time");
+    return SWR_ERR_STATE;
+  }
+  S->sys_row.assign(row_offsets, row_offsets + nsys + 1);
+  S->layout_dirty = true;
+  return swr_qeq_layout(S);
 }
 
 int swr_calculate_q(swr_system *S) {
@@ -569,6 +612,7 @@ bool lookup(swr_system *S, const std::string &k, View &v) {
   if (k == "t") return set(S->st, 8 * n, 2);
   if (k == "s_hist") return set(S->s_hist, 40 * n);
   if (k == "bo_radius") return set(S->hp.rbo, sizeof(S->hp.rbo), 3);
+  if (k == "sys_iters") return set(S->h_sys_iters.data(), 4 * (int64_t)S->h_sys_iters.size(), 3);
   if (k == "t_hist") return set(S->t_hist, 40 * n);
   return false;
 }
@@ -643,8 +687,12 @@ int64_t swr_get_scalar(swr_system *S, const char *name) {
   if (k == "need_bond_cap") return h.need_bond_cap;
   if (k == "need_hbond_cap") return h.need_hbond_cap;
   if (k == "flags") return h.flags | S->sticky_flags;
-  if (k == "matvecs_s") return h.cg_iters[0];
-  if (k == "matvecs_t") return h.cg_iters[1];
+  if (k == "matvecs_s" || k == "matvecs_t") {
+    int m = 0;
+    for (size_t j = (k == "matvecs_s" ? 0 : 1); j < S->h_sys_iters.size(); j += 2) m = std::max(m, S->h_sys_iters[j]);
+    return m;
+  }
+  if (k == "nsys") return S->nsys;
   return -1;
 }
 

@@ -116,35 +116,38 @@ k_spmv1(int n, const DevParams *__restrict__ P, const int *__restrict__ type, co
 }
 
 // ---------------------------------------------------------------- calculate_Q
-// Block partial sums of s and t over the local atoms (calculate_Q,
+// Tile partial sums of s and t over the local atoms (calculate_Q,
 // fix_qeq_reax_sunway.cpp:1733-1734: the two parallel_vector_acc calls).
-__global__ void __launch_bounds__(QT)
-k_q_sums(int n, const double2 *__restrict__ st, double *__restrict__ partials, int pstride) {
-  const int i = blockIdx.x * QT + threadIdx.x;
+__global__ void __launch_bounds__(UT)
+k_q_sums(const double2 *__restrict__ st, CgLayout L, double *__restrict__ partials, int pstride) {
+  const int tile = blockIdx.x;
+  const int i = L.tile_beg[tile] + threadIdx.x;
   double d[2] = {0.0, 0.0};
-  if (i < n) { const double2 v = st[i]; d[0] = v.x; d[1] = v.y; }
-  block_partials<2, QT>(d, partials, pstride, blockIdx.x);
+  if (i < L.tile_beg[tile + 1]) { const double2 v = st[i]; d[0] = v.x; d[1] = v.y; }
+  block_partials<2, UT>(d, partials, pstride, tile);
 }
 
+// Per system: sum s, sum t and u = sum s / sum t (one workgroup each).
 __global__ void __launch_bounds__(RT)
-k_q_total(int nblk, const double *__restrict__ partials, int pstride, double *__restrict__ qsum) {
+k_q_total(const double *__restrict__ partials, int pstride, CgLayout L, double *__restrict__ qsum) {
+  const int sys = blockIdx.x;
   double tot[2];
-  reduce_partials<2>(partials, pstride, nblk, tot);
+  reduce_partials<2>(partials, pstride, L.sys_tile[sys], L.sys_tile[sys + 1], tot);
   if (threadIdx.x != 0) return;
-  qsum[0] = tot[0];
-  qsum[1] = tot[1];
-  qsum[2] = tot[0] / tot[1];
+  qsum[3 * sys] = tot[0];
+  qsum[3 * sys + 1] = tot[1];
+  qsum[3 * sys + 2] = tot[0] / tot[1];
 }
 
-// q = s - u t for local atoms, history shift, and the ghost q refresh
-// (calculate_Q :1736-1754).
+// q = s - u t for local atoms (u of the atom's system), history shift, and
+// the ghost q refresh (calculate_Q :1736-1754).
 __global__ void k_q_apply(int n, int N, const double2 *__restrict__ st, const double *__restrict__ qsum,
-                          const int *__restrict__ owner, double *__restrict__ q, double *__restrict__ s_hist,
-                          double *__restrict__ t_hist) {
+                          const int *__restrict__ owner, const int *__restrict__ row_sys, double *__restrict__ q,
+                          double *__restrict__ s_hist, double *__restrict__ t_hist) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const double u = qsum[2];
   const int o = i < n ? i : owner[i];
+  const double u = qsum[3 * row_sys[o] + 2];
   const double2 v = st[o];
   q[i] = v.x - u * v.y;
   if (i < n) {
@@ -197,14 +200,83 @@ int swr_qeq_spmv(swr_system *S, const double *xv, double *b) {
   return SWR_OK;
 }
 
-// Runs both FP64 solves to convergence (first batch of first_batch
-// iteration launches, then batches of 4 between host checks).
+// Row/tile layout of the (batch of) systems and the buffers sized by it.
+// Rebuilt when the atoms or the batch partition changed.
+int swr_qeq_layout(swr_system *S) {
+  if (!S->layout_dirty) return SWR_OK;
+  const int n = S->n;
+  if (S->sys_row.size() < 2 || S->sys_row.back() != n) S->sys_row = {0, n};
+  const int nsys = (int)S->sys_row.size() - 1;
+  std::vector<int> row_sys(std::max(n, 1)), tile_beg, tile_sys, sys_tile(nsys + 1);
+  for (int s = 0; s < nsys; s++) {
+    sys_tile[s] = (int)tile_sys.size();
+    for (int r = S->sys_row[s]; r < S->sys_row[s + 1]; r += UT) {
+      tile_beg.push_back(r);
+      tile_sys.push_back(s);
+    }
+    for (int r = S->sys_row[s]; r < S->sys_row[s + 1]; r++) row_sys[r] = s;
+  }
+  sys_tile[nsys] = (int)tile_sys.size();
+  tile_beg.push_back(n);
+  if (tile_sys.empty()) {  // no rows: one empty tile keeps every grid non-empty
+    tile_beg = {0, 0};
+    tile_sys = {0};
+    sys_tile.assign(nsys + 1, 1);
+    sys_tile[0] = 0;
+  }
+  const int ntiles = (int)tile_sys.size();
+  int rc;
+  if ((rc = swr_alloc(&S->row_sys, row_sys.size())) || (rc = swr_alloc(&S->tile_beg, tile_beg.size())) ||
+      (rc = swr_alloc(&S->tile_sys, tile_sys.size())) || (rc = swr_alloc(&S->sys_tile, sys_tile.size())) ||
+      (rc = swr_alloc(&S->sys_iters, 2 * (size_t)nsys)) || (rc = swr_alloc(&S->cgs, 4 * (size_t)nsys)) ||
+      (rc = swr_alloc(&S->qsum, 3 * (size_t)nsys)))
+    return rc;
+  HIP_CHECK(hipMemcpyAsync(S->row_sys, row_sys.data(), sizeof(int) * row_sys.size(), hipMemcpyHostToDevice, S->stream));
+  HIP_CHECK(hipMemcpyAsync(S->tile_beg, tile_beg.data(), sizeof(int) * tile_beg.size(), hipMemcpyHostToDevice,
+                           S->stream));
+  HIP_CHECK(hipMemcpyAsync(S->tile_sys, tile_sys.data(), sizeof(int) * tile_sys.size(), hipMemcpyHostToDevice,
+                           S->stream));
+  HIP_CHECK(hipMemcpyAsync(S->sys_tile, sys_tile.data(), sizeof(int) * sys_tile.size(), hipMemcpyHostToDevice,
+                           S->stream));
+  HIP_CHECK(hipMemsetAsync(S->sys_iters, 0, sizeof(int) * 2 * nsys, S->stream));
+  HIP_CHECK(hipMemsetAsync(S->cgs, 0, sizeof(CgScalars) * 4 * nsys, S->stream));
+  const int ps = std::max(div_up(std::max(n, 1), RPB), ntiles);
+  if (ps > S->cg_pstride || !S->partials) {
+    if ((rc = swr_alloc(&S->partials, (size_t)ps * 8))) return rc;
+    S->cg_pstride = ps;
+  }
+  if (!S->done_count && (rc = swr_alloc(&S->done_count, 1))) return rc;
+  HIP_CHECK(hipStreamSynchronize(S->stream));
+  S->nsys = nsys;
+  S->ntiles = ntiles;
+  S->h_sys_iters.assign(2 * nsys, 0);
+  S->layout_dirty = false;
+  return SWR_OK;
+}
+
+CgLayout swr_qeq_cg_layout(swr_system *S) {
+  CgLayout L;
+  L.row_sys = S->row_sys;
+  L.tile_beg = S->tile_beg;
+  L.tile_sys = S->tile_sys;
+  L.sys_tile = S->sys_tile;
+  L.nsys = S->nsys;
+  L.ntiles = S->ntiles;
+  L.done_count = S->done_count;
+  return L;
+}
+
+// Runs the FP64 solves of every system to convergence (first batch of
+// first_batch iteration launches, then batches of 4 between host checks).
 int swr_qeq_cg_run(swr_system *S, int first_batch) {
+  int rc = swr_qeq_layout(S);
+  if (rc) return rc;
   CgRun<double, double> R;
   R.V = make_vecs(S);
   R.val = S->H_val;
+  R.L = swr_qeq_cg_layout(S);
   R.cgs = S->cgs;
-  R.iters_out = S->status->cg_iters;
+  R.iters_out = S->sys_iters;
   R.tol = S->ctl.tol;
   R.imax = S->imax;
   R.kt_iter = KT_CG_ITER;
@@ -215,20 +287,17 @@ int swr_qeq_cg_run(swr_system *S, int first_batch) {
 }
 
 int swr_qeq_calc_q(swr_system *S) {
+  int rc = swr_qeq_layout(S);
+  if (rc) return rc;
   const int n = S->n;
-  const int nblk = div_up(n, QT);
-  const int ps = S->cg_blocks;
-  double *qsum = S->partials + (size_t)ps * 8;
+  const int ps = S->cg_pstride;
+  const CgLayout L = swr_qeq_cg_layout(S);
   swr_kt_begin(S, KT_QEQ_Q);
-  hipLaunchKernelGGL(k_q_sums, dim3(nblk), dim3(QT), 0, S->stream, n, (const double2 *)S->st, S->partials, ps);
-  hipLaunchKernelGGL(k_q_total, dim3(1), dim3(RT), 0, S->stream, nblk, S->partials, ps, qsum);
+  hipLaunchKernelGGL(k_q_sums, dim3(L.ntiles), dim3(UT), 0, S->stream, (const double2 *)S->st, L, S->partials, ps);
+  hipLaunchKernelGGL(k_q_total, dim3(L.nsys), dim3(RT), 0, S->stream, S->partials, ps, L, S->qsum);
   hipLaunchKernelGGL(k_q_apply, dim3(div_up(S->N, 256)), dim3(256), 0, S->stream, n, S->N, (const double2 *)S->st,
-                     qsum, S->owner, S->q, S->s_hist, S->t_hist);
+                     S->qsum, S->owner, S->row_sys, S->q, S->s_hist, S->t_hist);
   swr_kt_end(S, KT_QEQ_Q);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }
-
-// Partial-sum slots per dot (the stride of the partials array): enough for
-// every grid the QEq kernels launch on n rows.
-int swr_qeq_blocks(int n) { return div_up(n, RPB); }

@@ -31,6 +31,19 @@ constexpr int UT = 256;                    // rows (threads) per block of the up
 
 enum { PH_S1 = 0, PH_S2, PH_S3, PH_A1, PH_K };
 
+// Rows of a batch of independent systems (one system when unbatched).
+// System s owns local rows [row_off[s], row_off[s+1]); the vector kernels
+// run over tiles of at most UT rows of one system each (tile t: rows
+// [tile_beg[t], tile_beg[t+1]) of system tile_sys[t]; system s owns tiles
+// [sys_tile[s], sys_tile[s+1])), so every partial sum belongs to one system
+// and each system's scalars come from its own tiles in a fixed order: a
+// system's iterations are the same alone and in a batch.
+struct CgLayout {
+  const int *row_sys, *tile_beg, *tile_sys, *sys_tile;
+  int nsys, ntiles;
+  int *done_count;    // systems whose s and t solves have both finished
+};
+
 template <typename T> struct Vec2;
 template <> struct Vec2<double> { typedef double2 type; };
 template <> struct Vec2<float> { typedef float2 type; };
@@ -43,17 +56,17 @@ struct CgVecsT {
   const double *Hd;
 };
 
-// Sums partials[k*pstride + b] over b < nblocks for k < ND with one
-// workgroup: thread t adds b = t, t+RT, ... in order, then a fixed LDS tree
-// combines the RT sums.  The order depends only on the block count.
+// Sums partials[k*pstride + b] over b in [b0, b1) for k < ND with one
+// workgroup: thread t adds b = b0+t, b0+t+RT, ... in order, then a fixed LDS
+// tree combines the RT sums.  The order depends only on the range.
 template <int ND>
-__device__ void reduce_partials(const double *__restrict__ partials, int pstride, int nblocks, double *out) {
+__device__ void reduce_partials(const double *__restrict__ partials, int pstride, int b0, int b1, double *out) {
   __shared__ double sr[ND][RT];
   double acc[ND];
 #pragma unroll
   for (int k = 0; k < ND; k++) acc[k] = 0.0;
 #pragma unroll 4
-  for (int b = threadIdx.x; b < nblocks; b += RT)
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += RT)
 #pragma unroll
     for (int k = 0; k < ND; k++) acc[k] += partials[(size_t)k * pstride + b];
 #pragma unroll
@@ -154,11 +167,11 @@ __device__ __forceinline__ T2 mk2(T a, T b) {
   return r;
 }
 
-// Whether any solve still runs in phase PH (PH_A1, PH_K read the scalars).
-__device__ __forceinline__ bool cg_any_active(int PH, const CgScalars *__restrict__ cgs) {
+// Whether a system's solves still run in phase PH (PH_A1, PH_K read the
+// scalars; c points at the system's s and t scalars).
+__device__ __forceinline__ bool cg_any_active(int PH, const CgScalars *__restrict__ c) {
   if (PH != PH_A1 && PH != PH_K) return true;
-  const CgScalars &c0 = cgs[0], &c1 = cgs[1];
-  return (PH == PH_K && (c0.active_b || c1.active_b)) || c0.active_a || c1.active_a;
+  return (PH == PH_K && (c[0].active_b || c[1].active_b)) || c[0].active_a || c[1].active_a;
 }
 
 // q = eta d + H d (sparse_matvec_C_para, fix_qeq_reax_sw64.c:424-466) of the
@@ -168,15 +181,17 @@ template <int PH, typename T, typename VT>
 __global__ void __launch_bounds__(QT)
 k_cg_spmv(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
           const int *__restrict__ numnbrs, const int *__restrict__ jlist, const VT *__restrict__ val, CgVecsT<T> V,
-          int par_in, const CgScalars *__restrict__ cgs) {
+          int par_in, const CgScalars *__restrict__ cgs, const int *__restrict__ row_sys) {
   typedef typename Vec2<T>::type T2;
-  if (!cg_any_active(PH, cgs)) return;  // both solves finished
   const int w = wave_in_block(), lane = lane_id();
   const int i = blockIdx.x * RPB + w * 2 + (lane >> 5);
+  // rows of systems whose solves have finished are skipped (a half-wave
+  // decides for its own row)
+  const bool live = i < n && cg_any_active(PH, cgs + 2 * row_sys[i]);
   const T2 *src = PH == PH_S1 ? V.x : PH == PH_S2 ? V.d[0] : PH == PH_S3 ? V.d[1] : (par_in ? V.d[1] : V.d[0]);
-  const int beg = i < n ? firstnbr[i] : 0, cnt = i < n ? numnbrs[i] : 0;
+  const int beg = live ? firstnbr[i] : 0, cnt = live ? numnbrs[i] : 0;
   T2 q = row_spmv2_h<T, VT>(beg, cnt, jlist, val, src);
-  if ((lane & 31) == 0 && i < n) {
+  if ((lane & 31) == 0 && live) {
     const T eta = (T)P->eta[type[i]];
     const T2 xi = src[i];
     q.x = eta * xi.x + q.x;
@@ -191,23 +206,25 @@ k_cg_spmv(int n, const DevParams *__restrict__ P, const int *__restrict__ type, 
 // r = b and no SpMV ran.
 template <int PH, typename T>
 __global__ void __launch_bounds__(UT)
-k_cg_upd(int n, CgVecsT<T> V, int par_in, const CgScalars *__restrict__ cgs, double *__restrict__ partials,
+k_cg_upd(CgVecsT<T> V, int par_in, const CgScalars *__restrict__ cgs, CgLayout L, double *__restrict__ partials,
          int pstride, int xzero) {
   typedef typename Vec2<T>::type T2;
+  const int tile = blockIdx.x;
+  const int sys = L.tile_sys[tile];
   CgScalars c0, c1;
   bool b0 = false, b1 = false, a0 = true, a1 = true;
   if (PH == PH_A1 || PH == PH_K) {
-    c0 = cgs[0];
-    c1 = cgs[1];
+    c0 = cgs[2 * sys];
+    c1 = cgs[2 * sys + 1];
     b0 = PH == PH_K && c0.active_b != 0;
     b1 = PH == PH_K && c1.active_b != 0;
     a0 = c0.active_a != 0;
     a1 = c1.active_a != 0;
     if (!b0 && !b1 && !a0 && !a1) return;  // both solves finished
   }
-  const int i = blockIdx.x * UT + threadIdx.x;
+  const int i = L.tile_beg[tile] + threadIdx.x;
   double dots[6] = {0, 0, 0, 0, 0, 0};
-  if (i < n) {
+  if (i < L.tile_beg[tile + 1]) {
     const T Hd = (T)V.Hd[i];
     const T2 q = (PH == PH_A1 || (PH == PH_S1 && xzero)) ? mk2<T2>((T)0, (T)0) : V.q[i];
     if (PH == PH_S1) {
@@ -264,26 +281,31 @@ k_cg_upd(int n, CgVecsT<T> V, int par_in, const CgScalars *__restrict__ cgs, dou
     }
   }
   if (PH == PH_S1 || PH == PH_S2) return;
-  block_partials<PH == PH_S3 ? 6 : 4, UT>(dots, partials, pstride, blockIdx.x);
+  block_partials<PH == PH_S3 ? 6 : 4, UT>(dots, partials, pstride, tile);
 }
 
-// Scalars of the iteration whose dots k_cg_upd<PH> just wrote (one workgroup).
-// tol > 0 sets both solves' tolerance at setup; tol <= 0 keeps the values
-// already in cgs[].tol (a negative one marks a solve with nothing to do).
-// The iteration count of a finished solve goes to iters_out[s].
+// Scalars of the iteration whose dots k_cg_upd<PH> just wrote: one
+// workgroup per system, over the system's tiles.  tol > 0 sets the solves'
+// tolerance at setup; tol <= 0 keeps the values already in cgs[].tol (a
+// negative one marks a solve with nothing to do).  A finished solve's
+// iteration count goes to iters_out[2 sys + rhs]; a system whose two solves
+// have both finished adds one to L.done_count.
 template <int PH>
 __global__ void __launch_bounds__(RT)
-k_cg_scalars(int nblk, const double *__restrict__ partials, int pstride, CgScalars *__restrict__ cgs, int imax,
+k_cg_scalars(const double *__restrict__ partials, int pstride, CgScalars *__restrict__ cgs, CgLayout L, int imax,
              double tol, DevStatus *st, int *__restrict__ iters_out) {
-  CgScalars c0 = cgs[0], c1 = cgs[1];
+  const int sys = blockIdx.x;
+  CgScalars c0 = cgs[2 * sys], c1 = cgs[2 * sys + 1];
   const bool b0 = PH == PH_K && c0.active_b != 0, b1 = PH == PH_K && c1.active_b != 0;
   const bool a0 = PH == PH_S3 || c0.active_a != 0, a1 = PH == PH_S3 || c1.active_a != 0;
   if (!b0 && !b1 && !a0 && !a1) return;
+  const bool was_done = PH != PH_S3 && c0.done && c1.done;
   constexpr int nd = PH == PH_S3 ? 6 : 4;
   double tot[nd];
-  reduce_partials<nd>(partials, pstride, nblk, tot);
+  reduce_partials<nd>(partials, pstride, L.sys_tile[sys], L.sys_tile[sys + 1], tot);
   if (threadIdx.x != 0) return;
   unsigned int bad = 0;
+  int *its = iters_out + 2 * sys;
   if constexpr (PH == PH_S3) {
     auto setup = [&](CgScalars &cc, double bb, double ur, double uw, int s) {
       if (tol > 0) cc.tol = tol;
@@ -300,7 +322,7 @@ k_cg_scalars(int nblk, const double *__restrict__ partials, int pstride, CgScala
       cc.active_b = 0;
       cc.active_a = (1 < imax && cc.tol > 0 && cc.b_norm > 0 && sqrt(cc.dg0) / cc.b_norm > cc.tol) ? 1 : 0;
       cc.done = cc.active_a ? 0 : 1;
-      if (cc.done) iters_out[s] = cc.iter;
+      if (cc.done) its[s] = cc.iter;
       if (!isfinite(cc.alpha_a) && cc.active_a) bad = 1;
     };
     setup(c0, tot[0], tot[1], tot[2], 0);
@@ -313,14 +335,15 @@ k_cg_scalars(int nblk, const double *__restrict__ partials, int pstride, CgScala
       } else if (act_b) {
         cc.active_b = 0;
         cc.done = 1;
-        iters_out[s] = cc.iter;
+        its[s] = cc.iter;
       }
     };
     fin(c0, a0, b0, tot[0], tot[1], 0);
     fin(c1, a1, b1, tot[2], tot[3], 1);
   }
-  cgs[0] = c0;
-  cgs[1] = c1;
+  cgs[2 * sys] = c0;
+  cgs[2 * sys + 1] = c1;
+  if (!was_done && c0.done && c1.done) atomicAdd(L.done_count, 1);
   if (bad) atomicOr(&st->flags, SWR_F_NAN);
 }
 
@@ -329,8 +352,9 @@ template <typename T, typename VT>
 struct CgRun {
   CgVecsT<T> V;
   const VT *val;
-  CgScalars *cgs;     // two consecutive entries (s, t)
-  int *iters_out;     // two ints on the device
+  CgLayout L;
+  CgScalars *cgs;     // 2 per system (s, t)
+  int *iters_out;     // 2 ints per system on the device
   double tol;         // > 0: set at setup; <= 0: preset in cgs[].tol
   int imax;           // iteration limit (the reference's imax = 200 for the FP64 solve)
   int kt_iter, kt_update, kt_setup, kt_scalars;
@@ -339,14 +363,14 @@ struct CgRun {
 template <typename T, typename VT>
 int launch_cg(swr_system *S, const CgRun<T, VT> &R, int ph, int par_in, int xzero) {
   const int n = S->n;
-  const int nsp = div_up(n, RPB), nup = div_up(n, UT);
-  const int ps = S->cg_blocks;
+  const int nsp = div_up(n, RPB);
+  const int ps = S->cg_pstride;
   const int kt = ph == PH_K ? R.kt_iter : R.kt_setup;
   const int ktu = ph == PH_K ? R.kt_update : R.kt_setup;
   const int kts = ph == PH_K ? R.kt_scalars : R.kt_setup;
-#define SP_ARGS n, S->dp, S->type, S->H_firstnbr, S->H_numnbrs, S->H_jlist, R.val, R.V, par_in, R.cgs
-#define UP_ARGS n, R.V, par_in, R.cgs, S->partials, ps, xzero
-#define SC_ARGS nup, S->partials, ps, R.cgs, R.imax, R.tol, S->status, R.iters_out
+#define SP_ARGS n, S->dp, S->type, S->H_firstnbr, S->H_numnbrs, S->H_jlist, R.val, R.V, par_in, R.cgs, R.L.row_sys
+#define UP_ARGS R.V, par_in, R.cgs, R.L, S->partials, ps, xzero
+#define SC_ARGS S->partials, ps, R.cgs, R.L, R.imax, R.tol, S->status, R.iters_out
   swr_kt_begin(S, kt);
   switch (ph) {
     case PH_S1:
@@ -359,20 +383,25 @@ int launch_cg(swr_system *S, const CgRun<T, VT> &R, int ph, int par_in, int xzer
   }
   swr_kt_end(S, kt);
   swr_kt_begin(S, ktu);
+  const dim3 gt(R.L.ntiles);
   switch (ph) {
-    case PH_S1: hipLaunchKernelGGL((k_cg_upd<PH_S1, T>), dim3(nup), dim3(UT), 0, S->stream, UP_ARGS); break;
-    case PH_S2: hipLaunchKernelGGL((k_cg_upd<PH_S2, T>), dim3(nup), dim3(UT), 0, S->stream, UP_ARGS); break;
-    case PH_S3: hipLaunchKernelGGL((k_cg_upd<PH_S3, T>), dim3(nup), dim3(UT), 0, S->stream, UP_ARGS); break;
-    case PH_A1: hipLaunchKernelGGL((k_cg_upd<PH_A1, T>), dim3(nup), dim3(UT), 0, S->stream, UP_ARGS); break;
-    default: hipLaunchKernelGGL((k_cg_upd<PH_K, T>), dim3(nup), dim3(UT), 0, S->stream, UP_ARGS); break;
+    case PH_S1: hipLaunchKernelGGL((k_cg_upd<PH_S1, T>), gt, dim3(UT), 0, S->stream, UP_ARGS); break;
+    case PH_S2: hipLaunchKernelGGL((k_cg_upd<PH_S2, T>), gt, dim3(UT), 0, S->stream, UP_ARGS); break;
+    case PH_S3: hipLaunchKernelGGL((k_cg_upd<PH_S3, T>), gt, dim3(UT), 0, S->stream, UP_ARGS); break;
+    case PH_A1: hipLaunchKernelGGL((k_cg_upd<PH_A1, T>), gt, dim3(UT), 0, S->stream, UP_ARGS); break;
+    default: hipLaunchKernelGGL((k_cg_upd<PH_K, T>), gt, dim3(UT), 0, S->stream, UP_ARGS); break;
   }
   swr_kt_end(S, ktu);
   swr_kt_begin(S, kts);
+  const dim3 gs(R.L.nsys);
   switch (ph) {
     case PH_S1: case PH_S2: break;
-    case PH_S3: hipLaunchKernelGGL(k_cg_scalars<PH_S3>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS); break;
-    case PH_A1: hipLaunchKernelGGL(k_cg_scalars<PH_A1>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS); break;
-    default: hipLaunchKernelGGL(k_cg_scalars<PH_K>, dim3(1), dim3(RT), 0, S->stream, SC_ARGS); break;
+    case PH_S3:
+      HIP_CHECK(hipMemsetAsync(R.L.done_count, 0, sizeof(int), S->stream));
+      hipLaunchKernelGGL(k_cg_scalars<PH_S3>, gs, dim3(RT), 0, S->stream, SC_ARGS);
+      break;
+    case PH_A1: hipLaunchKernelGGL(k_cg_scalars<PH_A1>, gs, dim3(RT), 0, S->stream, SC_ARGS); break;
+    default: hipLaunchKernelGGL(k_cg_scalars<PH_K>, gs, dim3(RT), 0, S->stream, SC_ARGS); break;
   }
   swr_kt_end(S, kts);
 #undef SP_ARGS
@@ -382,10 +411,10 @@ int launch_cg(swr_system *S, const CgRun<T, VT> &R, int ph, int par_in, int xzer
   return SWR_OK;
 }
 
-// Runs both solves of R to convergence.  Iteration kernels are enqueued in
-// batches (the first of first_batch launches); between batches the host
-// reads the two done flags.  Launches enqueued after both solves finished
-// exit at once.
+// Runs the solves of every system of R to convergence.  Iteration kernels
+// are enqueued in batches (the first of first_batch launches); between
+// batches the host reads how many systems have finished.  Launches enqueued
+// after a system finished skip its rows.
 template <typename T, typename VT>
 int cg_solve(swr_system *S, const CgRun<T, VT> &R, int first_batch, int xzero) {
   int rc;
@@ -397,13 +426,16 @@ int cg_solve(swr_system *S, const CgRun<T, VT> &R, int first_batch, int xzero) {
   while (k < R.imax + 1) {
     for (int b = 0; b < batch && k < R.imax + 1; b++, k++)
       if ((rc = launch_cg(S, R, PH_K, (k - 1) & 1, 0))) return rc;
-    CgScalars h[2];
-    HIP_CHECK(hipMemcpyAsync(h, R.cgs, sizeof(h), hipMemcpyDeviceToHost, S->stream));
+    int done = 0;
+    HIP_CHECK(hipMemcpyAsync(&done, R.L.done_count, sizeof(int), hipMemcpyDeviceToHost, S->stream));
     HIP_CHECK(hipStreamSynchronize(S->stream));
-    if (h[0].done && h[1].done) break;
+    if (done >= R.L.nsys) break;
     batch = 4;
   }
   return SWR_OK;
 }
 
 }  // namespace swr_cg
+
+// The row/tile layout of S's systems for the CG kernels (qeq.hip).
+swr_cg::CgLayout swr_qeq_cg_layout(swr_system *S);

@@ -82,9 +82,9 @@ k_resid(int n, const DevParams *__restrict__ P, const int *__restrict__ type, co
 // does nothing and its d stays 0.
 __global__ void __launch_bounds__(RT)
 k_resid_scalars(int nblk, const double *__restrict__ partials, int pstride, MixState *__restrict__ ms,
-                CgScalars *__restrict__ inner, double tol, int first, DevStatus *st) {
+                CgScalars *__restrict__ inner, double tol, int first, DevStatus *st, int *__restrict__ iters) {
   double tot[6];
-  reduce_partials<6>(partials, pstride, nblk, tot);
+  reduce_partials<6>(partials, pstride, 0, nblk, tot);
   if (threadIdx.x != 0) return;
   MixState m = *ms;
   if (first) {
@@ -122,8 +122,8 @@ k_resid_scalars(int nblk, const double *__restrict__ partials, int pstride, MixS
     atomicOr(&st->flags, SWR_F_CG_MAXIT);
   }
   if (m.done) {
-    st->cg_iters[0] = m.matvecs[0];
-    st->cg_iters[1] = m.matvecs[1];
+    iters[0] = m.matvecs[0];
+    iters[1] = m.matvecs[1];
   }
   *ms = m;
 }
@@ -165,10 +165,14 @@ int swr_qeq_mixed_solve(swr_system *S) {
     swr_set_error_msg("mixed-precision QEq needs the FP32 H (fix qeq/reax ... mixed at setup)");
     return SWR_ERR_STATE;
   }
-  if ((rc = ensure_mixw(S))) return rc;
+  if ((rc = swr_qeq_layout(S)) || (rc = ensure_mixw(S))) return rc;
+  if (S->nsys != 1) {
+    swr_set_error_msg("the mixed-precision solve runs one system at a time");
+    return SWR_ERR_STATE;
+  }
   const int n = S->n;
   const int nblk = div_up(n, RPB);
-  const int ps = S->cg_blocks;
+  const int ps = S->cg_pstride;
   MixState *ms = (MixState *)S->mixw;
   float2 *base = (float2 *)((char *)S->mixw + 256);
   const size_t c = S->ncap;
@@ -180,7 +184,8 @@ int swr_qeq_mixed_solve(swr_system *S) {
   R.V.b = rhs;
   R.V.Hd = S->Hdia_inv;
   R.val = S->H_val32;
-  R.cgs = S->cgs + 2;
+  R.L = swr_qeq_cg_layout(S);
+  R.cgs = S->cgs + 2;   // after the FP64 solve's s, t
   R.iters_out = S->status->inner_iters;
   R.tol = -1.0;  // set per rhs by k_resid_scalars
   R.imax = INNER_IMAX;
@@ -195,7 +200,7 @@ int swr_qeq_mixed_solve(swr_system *S) {
     hipLaunchKernelGGL(k_resid, dim3(nblk), dim3(QT), 0, S->stream, n, S->dp, S->type, S->H_firstnbr, S->H_numnbrs,
                        S->H_jlist, S->H_val32, (const double2 *)x, b, S->Hdia_inv, rhs, S->partials, ps);
     hipLaunchKernelGGL(k_resid_scalars, dim3(1), dim3(RT), 0, S->stream, nblk, S->partials, ps, ms, R.cgs,
-                       S->ctl.tol * OUTER_TOL_FACTOR, outer == 0 ? 1 : 0, S->status);
+                       S->ctl.tol * OUTER_TOL_FACTOR, outer == 0 ? 1 : 0, S->status, S->sys_iters);
     swr_kt_end(S, KT_MIXED_OUTER);
     HIP_CHECK(hipGetLastError());
     MixState h;

@@ -182,10 +182,23 @@ struct swr_system {
   double *st = nullptr;            // [2][ncap]: s, t
   double *s_hist = nullptr, *t_hist = nullptr;  // [n][5]
   double *cgv = nullptr;           // CG vectors, see CGV_* below
-  CgScalars *cgs = nullptr;        // [4]: s, t of the FP64 solve; s, t of the mixed inner solve
-  double *partials = nullptr;      // per-workgroup partial dots
+  CgScalars *cgs = nullptr;        // [4 nsys]: s, t per system of the FP64 solve, then of the mixed inner solve
+  double *partials = nullptr;      // per-workgroup partial dots, [8][cg_pstride]
+  int cg_pstride = 0;
   unsigned int *counters = nullptr;
-  int cg_blocks = 0;
+
+  // ---- batch of independent systems (one unless swr_set_batch)
+  std::vector<int> sys_row;        // nsys + 1 local-row offsets (host)
+  int nsys = 1, ntiles = 0;
+  bool layout_dirty = true;
+  int *row_sys = nullptr;          // [n] system of each local row
+  int *tile_beg = nullptr;         // [ntiles + 1]
+  int *tile_sys = nullptr;         // [ntiles]
+  int *sys_tile = nullptr;         // [nsys + 1]
+  int *sys_iters = nullptr;        // [2 nsys] iterations of s, t per system (device)
+  std::vector<int> h_sys_iters;    // host copy after each solve
+  int *done_count = nullptr;       // systems finished in the running solve
+  double *qsum = nullptr;          // [3 nsys] sum s, sum t, u per system
   int qeq_initialized = 0;
   int mixed = 0;                   // mixed-precision QEq
   void *mixw = nullptr;            // mixed-precision workspace
@@ -250,7 +263,7 @@ int swr_qeq_build_H(swr_system *S);
 int swr_qeq_spmv(swr_system *S, const double *xv, double *b);
 int swr_qeq_cg_run(swr_system *S, int first_batch);
 int swr_qeq_calc_q(swr_system *S);
-int swr_qeq_blocks(int n);
+int swr_qeq_layout(swr_system *S);
 int swr_qeq_mixed_solve(swr_system *S);
 
 template <class T>

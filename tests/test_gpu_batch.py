@@ -1,0 +1,108 @@
+"""Batched evaluation: every system gives the same results alone and in a mix.
+
+A batch of clusters and small periodic water boxes runs as one device system
+(one launch per stage, per-system QEq scalars).  Each system is also run
+alone.  In the batch a system is translated into its own region and binned
+on another grid, so its coordinates differ in the last bit and its sums run
+in another order; per system the checks are therefore: CG iteration counts
+equal (within one at tol 1e-10, where the stopping test sits at rounding
+level), charges within 1e-9 e at tol 1e-10 and within 1e-6 e at the
+reference tolerance, identical bond partners, bond orders within 1e-9
+relative (a weak bond's BO = BO' - 1e-4 cancels, which turns last-bit
+coordinate changes into ~1e-10 relative).  One cluster is also checked
+against the FP64 oracle at 1e-10.
+"""
+import numpy as np
+import pytest
+
+from sw_reaxff_amd import batch as B
+from sw_reaxff_amd import systems
+
+from .parity import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def mix():
+    sizes = [50, 120, 333, 700, 1500]
+    out = [systems.cluster(k, seed=40 + i) for i, k in enumerate(sizes)]
+    out.insert(2, systems.water_box(216, seed=3))     # periodic, 648 atoms
+    out.append(systems.water_box(343, seed=4))         # periodic, 1029 atoms
+    return out
+
+
+def bonds_of(bt, k):
+    h = bt.sys
+    P = bt.packed
+    bi, be, bonds, BO = h.get("bond_index"), h.get("bond_end"), h.get("bonds"), h.get("BO")
+    rows = range(P.row_off[k], P.row_off[k + 1])
+    cnt = np.array([be[i] - bi[i] for i in rows])
+    sel = np.concatenate([np.arange(bi[i], be[i]) for i in rows])
+    nbr = bonds["nbr"][sel]
+    # partners as (owner row - system start): ghosts map to their owner
+    owner = h.get("owner")
+    return cnt, owner[nbr] - P.row_off[k], BO[sel]
+
+
+@pytest.mark.parametrize("tol,qtol", [(1e-10, 1e-9), (1e-6, 1e-6)])
+def test_alone_equals_in_the_mix(gpu_lib, tol, qtol):
+    parts = mix()
+    bt = B.Batch(tol=tol)
+    bt.load(parts)
+    bt.evaluate()
+    bt.sync()
+    q_mix = bt.charges()
+    it_mix = bt.iterations()
+    for k, s in enumerate(parts):
+        alone = B.Batch(tol=tol)
+        alone.load([s])
+        alone.evaluate()
+        alone.sync()
+        q1 = alone.charges()[0]
+        it1 = alone.iterations()[0]
+        dq = float(np.max(np.abs(q1 - q_mix[k])))
+        print(f"[batch] system {k} n={s.n} periodic={s.periodic} iters alone {tuple(it1)} mix {tuple(it_mix[k])} "
+              f"max|dq| {dq:.2e}", flush=True)
+        slack = 1 if tol < 1e-8 else 0
+        assert np.all(np.abs(np.asarray(it1) - np.asarray(it_mix[k])) <= slack), k
+        assert dq <= qtol, k
+        assert abs(q_mix[k].sum()) < 1e-8, k
+        c1, n1, bo1 = bonds_of(alone, 0)
+        c2, n2, bo2 = bonds_of(bt, k)
+        assert np.array_equal(c1, c2), k
+        assert np.array_equal(n1, n2), k
+        assert rel_err(bo2, bo1, 1e-12) <= 1e-9, k
+
+
+def test_batch_cluster_against_oracle(gpu_lib, oracle_lib, ffield):
+    parts = mix()
+    bt = B.Batch(tol=1e-10)
+    P = bt.load(parts)
+    bt.evaluate()
+    bt.sync()
+    k = 3                                   # the 333-atom cluster
+    s = parts[k]
+    o = oracle_lib.Oracle(ffield, B.ELEMENTS, tol=1e-10, threads=4)
+    tmap = np.array([B.ELEMENTS.index(e) + 1 for e in s.elements], dtype=np.int32)
+    o.set_atoms(s.n, s.x, tmap[s.type - 1], s.tag, np.arange(s.n, dtype=np.int32))
+    o.build_verlet(12.0)
+    o.step()
+    dq = np.max(np.abs(bt.charges()[k] - o.get("q")[:s.n]))
+    assert dq <= 1e-9
+    bi, be = o.get("bond_index"), o.get("bond_end")
+    c2, _, bo2 = bonds_of(bt, k)
+    assert np.array_equal(be - bi, c2)
+    obo = np.concatenate([o.get("BO")[bi[i]:be[i]] for i in range(s.n)])
+    assert rel_err(bo2, obo, 1e-12) <= 1e-10
+
+
+def test_batch_rejects_bad_offsets(gpu_lib):
+    import ctypes as C
+    bt = B.Batch()
+    bt.load(mix()[:2])
+    h = bt.sys
+    bad = np.array([0, 10, 5, bt.packed.n], dtype=np.int32)
+    rc = h.L.swr_set_batch(h.h, 3, bad.ctypes.data_as(C.POINTER(C.c_int)))
+    assert rc == 1
+    short = np.array([0, 7], dtype=np.int32)
+    assert h.L.swr_set_batch(h.h, 1, short.ctypes.data_as(C.POINTER(C.c_int))) == 1
