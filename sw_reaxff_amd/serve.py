@@ -1,0 +1,271 @@
+"""Serving loop: evaluation requests of independent systems, continuously batched.
+
+Requests arrive over time; each is either a new system to evaluate once
+(neighbour list, QEq charges, far/bond/hbond lists, bond orders) or a
+follow-on MD step of a resident system.  The server:
+
+  * runs `lanes` workers, each owning one device system (its own HIP stream).
+    A free worker takes every new-system request queued at that moment that
+    fits the HBM budget and evaluates them as ONE batch (Batch: one launch
+    per stage for all of them); requests that arrive while a batch is on the
+    GPU wait for the next free worker, and the workers' batches overlap on
+    the device (ctypes releases the GIL during library calls);
+  * keeps resident systems on the device between their steps (positions,
+    velocities, Verlet list, s/t warm-start history in their own device
+    system); a follow-on request advances one MD step there;
+  * admits work by estimated HBM footprint against a budget (bytes per atom
+    dominated by the Verlet and far lists).
+
+Latency is measured from a request's arrival to its charges being on the
+host.  Nothing here changes a result: a system's charges are the ones it
+gets evaluated alone (tests/test_gpu_serve.py).
+"""
+from __future__ import annotations
+
+import math
+import threading
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import systems
+from .batch import Batch, CUTGHOST, ELEMENTS
+from .lammps import DEFAULT_FFIELD, Lammps, standard_script
+
+# Per-atom device bytes (local + ghost): ~700 Verlet ids (4 B) + ~420 far
+# entries (64 B) + bonds, H, CG vectors.
+BYTES_PER_ATOM = 32 * 1024
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float                  # seconds from the start of the run
+    kind: str                       # "new" | "step"
+    system: systems.System | None = None
+    resident: int = -1              # resident system id (kind "step", or a new system kept resident)
+    # filled in by the server
+    done: float = 0.0
+    q: np.ndarray | None = None
+    iters: tuple = ()
+    lane: int = -1
+    batch_size: int = 0
+    step_index: int = 0             # kind "step": which step of its resident system this was
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+def footprint(s: systems.System) -> int:
+    """Estimated device bytes of a system (ghost shell included when periodic)."""
+    n = s.n
+    if s.periodic:
+        L = float(np.min(s.hi - s.lo))
+        n = int(n * ((L + 2 * CUTGHOST) / L) ** 3)
+    return n * BYTES_PER_ATOM
+
+
+def make_system(kind: str, natoms: int, seed: int) -> systems.System:
+    if kind == "water":
+        # the periodic box must be at least one ghost cutoff wide (L >= 12 A: >= 58 waters)
+        return systems.water_box(max(64, natoms // 3), seed=seed)
+    return systems.cluster(max(3, natoms), seed=seed)
+
+
+def make_requests(count: int, rate: float, seed: int, lo: int = 20, hi: int = 20000, frac_step: float = 0.3,
+                  frac_resident: float = 0.2, dt_sizes=None):
+    """Poisson arrivals at `rate` per second; sizes log-uniform in [lo, hi];
+    half periodic water boxes, half water/methanol clusters; a fraction of
+    requests are follow-on MD steps of a resident system (new systems become
+    resident with probability frac_resident)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    reqs, residents = [], []
+    for k in range(count):
+        if residents and rng.random() < frac_step:
+            reqs.append(Request(rid=k, arrival=float(t[k]), kind="step", resident=int(rng.choice(residents))))
+            continue
+        size = int(math.exp(rng.uniform(math.log(lo), math.log(hi))))
+        kind = "water" if rng.random() < 0.5 else "cluster"
+        s = make_system(kind, size, seed=int(rng.integers(1 << 30)))
+        r = Request(rid=k, arrival=float(t[k]), kind="new", system=s)
+        if rng.random() < frac_resident:
+            r.resident = len(residents)
+            residents.append(r.resident)
+        reqs.append(r)
+    return reqs
+
+
+class _Resident:
+    """A system kept on the device between its MD steps."""
+
+    def __init__(self, s: systems.System, ffield, tol, mixed):
+        self.lock = threading.Lock()
+        self.lmp = Lammps()
+        self.lmp.commands(standard_script(s.elements, ffield=ffield, tol=tol, mixed=mixed))
+        self.lmp.create_atoms(s)
+        self.n = s.n
+        self.bytes = footprint(s)
+        self.steps = 0
+
+    def first(self):
+        self.lmp.setup()
+        return self.q()
+
+    def step(self):
+        self.lmp.run(1)
+        self.steps += 1
+        return self.q()
+
+    def q(self):
+        return self.lmp.system.get("q")[: self.n].copy()
+
+
+class Server:
+    def __init__(self, lanes: int = 2, hbm_budget: float = 200e9, max_batch_atoms: int = 400_000,
+                 ffield: str = DEFAULT_FFIELD, tol: float = 1e-6, mixed: bool = False, batches=None):
+        self.lanes = lanes
+        self.batches = batches or [None] * lanes   # reusable per-lane Batch objects (device buffers kept)
+        self.budget = hbm_budget
+        self.max_batch_atoms = max_batch_atoms
+        self.ffield, self.tol, self.mixed = ffield, tol, mixed
+        self.pending: list[Request] = []
+        self.cv = threading.Condition()
+        self.in_use = 0.0                         # admitted bytes (batches in flight + residents)
+        self.residents: dict[int, _Resident] = {}
+        self.finished: list[Request] = []
+        self.closed = False
+        self.t0 = 0.0
+        self.errors: list[BaseException] = []
+
+    # ---- admission
+    def _take(self, lane):
+        """Requests for a lane: all queued follow-on steps, or a batch of new
+        systems that fits the budget and the batch size cap."""
+        with self.cv:
+            while True:
+                if self.pending:
+                    steps = [r for r in self.pending if r.kind == "step" and r.resident in self.residents]
+                    if steps:
+                        for r in steps:
+                            self.pending.remove(r)
+                        return "step", steps, 0.0
+                    news = [r for r in self.pending if r.kind == "new"]
+                    take, atoms, nbytes = [], 0, 0.0
+                    for r in news:
+                        b = footprint(r.system)
+                        if take and (atoms + r.system.n > self.max_batch_atoms or
+                                     self.in_use + nbytes + b > self.budget):
+                            break
+                        if not take and self.in_use + b > self.budget:
+                            break
+                        take.append(r)
+                        atoms += r.system.n
+                        nbytes += b
+                    if take:
+                        for r in take:
+                            self.pending.remove(r)
+                        self.in_use += nbytes
+                        return "new", take, nbytes
+                if self.closed and not self.pending:
+                    return None, [], 0.0
+                self.cv.wait(0.002)
+
+    def _release(self, nbytes):
+        with self.cv:
+            self.in_use -= nbytes
+            self.cv.notify_all()
+
+    # ---- workers
+    def _worker(self, lane):
+        bt = self.batches[lane] or Batch(ffield=self.ffield, elements=ELEMENTS, tol=self.tol)
+        self.batches[lane] = bt
+        try:
+            while True:
+                kind, reqs, nbytes = self._take(lane)
+                if kind is None:
+                    return
+                if kind == "step":
+                    for r in reqs:
+                        res = self.residents[r.resident]
+                        with res.lock:
+                            r.q = res.step()
+                            r.step_index = res.steps
+                        self._finish(r, lane, 1)
+                    continue
+                # new systems: residents are set up in their own device system,
+                # the rest are evaluated together as one batch
+                batch = []
+                for r in reqs:
+                    if r.resident >= 0:
+                        res = _Resident(r.system, self.ffield, self.tol, self.mixed)
+                        with res.lock:
+                            r.q = res.first()
+                        with self.cv:
+                            self.residents[r.resident] = res
+                            self.in_use += res.bytes
+                        self._finish(r, lane, 1)
+                    else:
+                        batch.append(r)
+                if batch:
+                    bt.load([r.system for r in batch])
+                    bt.evaluate()
+                    qs = bt.charges()
+                    its = bt.iterations()
+                    for r, q, it in zip(batch, qs, its):
+                        r.q = q.copy()
+                        r.iters = tuple(int(v) for v in it)
+                    for r in batch:
+                        self._finish(r, lane, len(batch))
+                self._release(nbytes)
+        except BaseException as e:  # surface worker failures to run()
+            self.errors.append(e)
+            with self.cv:
+                self.closed = True
+                self.cv.notify_all()
+
+    def _finish(self, r, lane, bsize):
+        r.done = time.perf_counter() - self.t0
+        r.lane = lane
+        r.batch_size = bsize
+        with self.cv:
+            self.finished.append(r)
+
+    # ---- driver
+    def run(self, requests: list[Request]):
+        """Feeds the requests at their arrival times; returns them when all are done."""
+        self.t0 = time.perf_counter()
+        workers = [threading.Thread(target=self._worker, args=(k,), daemon=True) for k in range(self.lanes)]
+        for w in workers:
+            w.start()
+        for r in sorted(requests, key=lambda r: r.arrival):
+            wait = r.arrival - (time.perf_counter() - self.t0)
+            if wait > 0:
+                time.sleep(wait)
+            with self.cv:
+                self.pending.append(r)
+                self.cv.notify_all()
+        with self.cv:
+            self.closed = True
+            self.cv.notify_all()
+        for w in workers:
+            w.join()
+        if self.errors:
+            raise self.errors[0]
+        return requests
+
+
+def summary(reqs: list[Request], wall: float):
+    lat = np.array([r.latency for r in reqs if r.q is not None])
+    return {
+        "requests": len(reqs), "completed": int(lat.size),
+        "systems_per_s": round(lat.size / wall, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                       "max": round(float(lat.max()) * 1e3, 2)},
+        "mean_batch": round(float(np.mean([r.batch_size for r in reqs if r.kind == "new"] or [0])), 2),
+        "steps": sum(r.kind == "step" for r in reqs),
+        "atoms": int(sum(r.system.n for r in reqs if r.system is not None)),
+    }

@@ -1,0 +1,57 @@
+"""Serving loop: every served result equals the same evaluation done alone.
+
+A short stream of requests (new clusters and periodic water boxes, some kept
+resident and stepped by follow-on requests) runs through the server with two
+overlapping workers.  Afterwards each new system is evaluated alone and each
+resident system is replayed alone step by step.
+
+Tolerances: a batched system is translated into its own region and binned on
+another grid, so its sums run in another order; at the reference tolerance
+(1e-6) its CG can stop one iteration apart from the lone run, which moves
+charges by up to the solver's own truncation (~5e-6 e).  Served charges must
+therefore agree with the lone run within 1e-5 e at tol 1e-6 and within 1e-9 e
+at tol 1e-10 (where the truncation is negligible).  A resident system is the
+same device system stepped the same way, so its charges agree to 1e-12 e.
+"""
+import numpy as np
+import pytest
+
+from sw_reaxff_amd import serve
+from sw_reaxff_amd.batch import Batch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("tol,qtol", [(1e-6, 1e-5), (1e-10, 1e-9)])
+def test_served_results_equal_alone(gpu_lib, ffield, tol, qtol):
+    # arrivals faster than one evaluation, so batches of several systems form
+    reqs = serve.make_requests(30, rate=5000.0, seed=11, lo=20, hi=1500, frac_step=0.3, frac_resident=0.25)
+    srv = serve.Server(lanes=2, hbm_budget=200e9, ffield=ffield, tol=tol)
+    srv.run(reqs)
+    assert all(r.q is not None for r in reqs)
+    assert any(r.kind == "step" for r in reqs) and any(r.batch_size > 1 for r in reqs)
+    print(f"[serve] {serve.summary(reqs, max(r.done for r in reqs))}", flush=True)
+    alone = Batch(ffield=ffield, tol=tol)
+    worst = 0.0
+    for r in reqs:
+        if r.kind == "new" and r.resident < 0:
+            alone.load([r.system])
+            alone.evaluate()
+            alone.sync()
+            dq = float(np.max(np.abs(alone.charges()[0] - r.q)))
+            worst = max(worst, dq)
+            its = tuple(int(v) for v in alone.iterations()[0])
+            if dq > 1e-7:
+                print(f"[serve] request {r.rid} n={r.system.n} batch {r.batch_size}: max|dq| {dq:.2e}, "
+                      f"iterations served {r.iters} alone {its}", flush=True)
+            assert dq <= qtol, (r.rid, r.system.n, r.batch_size, dq, r.iters, its)
+    print(f"[serve] tol {tol}: worst served-vs-alone charge difference {worst:.2e}", flush=True)
+    # residents: replay alone in the order the server stepped them
+    for rid in sorted({r.resident for r in reqs if r.resident >= 0}):
+        first = next(r for r in reqs if r.kind == "new" and r.resident == rid)
+        steps = sorted((r for r in reqs if r.kind == "step" and r.resident == rid), key=lambda r: r.step_index)
+        res = serve._Resident(first.system, ffield, tol, False)
+        assert np.max(np.abs(res.first() - first.q)) <= 1e-12
+        for k, r in enumerate(steps, 1):
+            assert r.step_index == k
+            assert np.max(np.abs(res.step() - r.q)) <= 1e-12, (rid, k)
