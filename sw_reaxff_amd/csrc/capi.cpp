@@ -612,6 +612,7 @@ bool lookup(swr_system *S, const std::string &k, View &v) {
   if (k == "t") return set(S->st, 8 * n, 2);
   if (k == "s_hist") return set(S->s_hist, 40 * n);
   if (k == "bo_radius") return set(S->hp.rbo, sizeof(S->hp.rbo), 3);
+  if (k == "dev_params") return set(S->dp, sizeof(DevParams));
   if (k == "sys_iters") return set(S->h_sys_iters.data(), 4 * (int64_t)S->h_sys_iters.size(), 3);
   if (k == "t_hist") return set(S->t_hist, 40 * n);
   return false;

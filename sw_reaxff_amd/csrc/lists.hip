@@ -407,33 +407,41 @@ __global__ void k_validate(int n, int N, const int *__restrict__ bond_index, con
 // same fixed point.  flags[0..2]: sweep k reads flags[(k+2)%3] (changed in
 // sweep k-1), sets flags[k%3], clears flags[(k+1)%3].  flags[3] turns 1 once
 // a sweep finds that the previous one changed nothing; later sweeps skip.
-__global__ void k_bond_mark(int n, int N, int sweep, const int *__restrict__ bond_index,
+// In-place relaxation of the marks of ghost atoms (local atoms are 0 and
+// never change).  A ghost g takes min(mark[g], mark[j] + 1) over its bonds to
+// lower-index atoms j, `inner` times per launch, reading neighbours' current
+// values (relaxed atomic loads, so other workgroups' updates are seen as they
+// land).  Every value written is >= the value of the reference's sequential
+// pass and the updates only follow j < g, so the marks settle on exactly that
+// pass's result whatever the update order; a launch in which no mark changed
+// has reached it.  chg[launch] is set when the launch changed a mark; a launch
+// after an unchanged one does nothing.
+__global__ void k_bond_mark(int n, int N, int launch, int inner, const int *__restrict__ bond_index,
                             const int *__restrict__ bond_end, const swr_bond_t *__restrict__ bonds,
-                            const int *__restrict__ src, int *__restrict__ dst, int *__restrict__ flags) {
-  if (flags[3]) return;
-  if (flags[(sweep + 2) % 3] == 0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) { flags[3] = 1; flags[4] = sweep; }
-    return;
-  }
-  int g = n + blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0) flags[(sweep + 1) % 3] = 0;
+                            int *__restrict__ mark, int *__restrict__ chg) {
+  if (launch > 0 && __hip_atomic_load(&chg[launch - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+  const int g = n + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= N) return;
-  int best = src[g];
-  for (int p = bond_index[g]; p < bond_end[g]; p++) {
-    int j = bonds[p].nbr;
-    if (j < g) {
-      int v = src[j] + 1;
-      if (v < best) best = v;
+  const int p0 = bond_index[g], p1 = bond_end[g];
+  int cur = __hip_atomic_load(&mark[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  bool changed = false;
+  for (int it = 0; it < inner; it++) {
+    int best = cur;
+    for (int p = p0; p < p1; p++) {
+      const int j = bonds[p].nbr;
+      if (j < g) {
+        const int v = __hip_atomic_load(&mark[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+        if (v < best) best = v;
+      }
+    }
+    if (best < cur) {
+      cur = best;
+      changed = true;
+      __hip_atomic_store(&mark[g], best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
-  if (best != dst[g]) {
-    dst[g] = best;
-    atomicOr(&flags[sweep % 3], 1);
-  }
+  if (changed) atomicOr(&chg[launch], 1);
 }
-
-// Estimate_Storages (reaxc_forces_sunway.cpp:1189-1290): bond counts of
-// local atoms (ghosts start at MIN_BONDS) and hbond counts of local H.
 __global__ void k_estimate(int n, int N, const int *__restrict__ bond_count, const int *__restrict__ hb_count,
                            const swr_atom_pack_t *__restrict__ atoms, const DevParams *__restrict__ P,
                            double saferzone, int *__restrict__ num_bonds, int *__restrict__ num_hbonds) {
@@ -451,19 +459,13 @@ __global__ void k_estimate(int n, int N, const int *__restrict__ bond_count, con
   }
 }
 
-__global__ void k_mark_init(int *__restrict__ flags, int *__restrict__ a, int *__restrict__ b, int N) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) { flags[0] = 0; flags[1] = 0; flags[2] = 1; flags[3] = 0; flags[4] = 0; }
-  if (i < N) b[i] = a[i];
+__global__ void k_mark_init(int *__restrict__ chg, int nchg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nchg) chg[i] = 0;
 }
-
 __global__ void k_set_flag(DevStatus *st, unsigned int f) { atomicOr(&st->flags, f); }
 
 // Reports whether the sweeps converged and at which sweep (host reads it).
-__global__ void k_mark_finish(const int *__restrict__ flags, DevStatus *st) {
-  st->mark_changed = flags[3] ? flags[4] : -1;
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------- host side
@@ -566,37 +568,36 @@ int swr_lists_validate(swr_system *S) {
 // until no mark changes; the host reads the convergence flag between batches.
 int swr_lists_mark(swr_system *S, int max_sweeps) {
   const int n = S->n, N = S->N;
+  constexpr int INNER = 8;  // relaxation sweeps per launch
   swr_kt_begin(S, KT_MARK);
   if (N > n) {
-    int rc = swr_ensure_tmp(S, sizeof(int) * 8);
+    const int max_launch = div_up(max_sweeps, INNER);
+    int rc = swr_ensure_tmp(S, sizeof(int) * (size_t)max_launch);
     if (rc) return rc;
-    int *flags = S->tmp_i;
-    int *a = S->bond_mark, *b = S->bond_mark2;
-    hipLaunchKernelGGL(k_mark_init, dim3(div_up(N, 256)), dim3(256), 0, S->stream, flags, a, b, N);
-    int sw = 0;
-    int batch = std::max(16, S->mark_depth_hint + 8);
+    int *chg = S->tmp_i;
+    hipLaunchKernelGGL(k_mark_init, dim3(div_up(max_launch, 256)), dim3(256), 0, S->stream, chg, max_launch);
+    int launch = 0;
+    int batch = std::max(2, div_up(S->mark_depth_hint, INNER) + 1);
     bool converged = false;
-    while (sw < max_sweeps) {
-      for (int k = 0; k < batch && sw < max_sweeps; k++, sw++) {
-        hipLaunchKernelGGL(k_bond_mark, dim3(div_up(N - n, 256)), dim3(256), 0, S->stream, n, N, sw,
-                           S->bond_index, S->bond_end, S->bonds, (const int *)a, b, flags);
-        std::swap(a, b);  // `a` now holds the latest sweep
-      }
-      hipLaunchKernelGGL(k_mark_finish, dim3(1), dim3(1), 0, S->stream, flags, S->status);
-      int conv = -1;
-      HIP_CHECK(hipMemcpyAsync(&conv, &S->status->mark_changed, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    std::vector<int> h(max_launch);
+    while (launch < max_launch && !converged) {
+      const int l0 = launch;
+      for (int k = 0; k < batch && launch < max_launch; k++, launch++)
+        hipLaunchKernelGGL(k_bond_mark, dim3(div_up(N - n, 256)), dim3(256), 0, S->stream, n, N, launch, INNER,
+                           S->bond_index, S->bond_end, S->bonds, S->bond_mark, chg);
+      HIP_CHECK(hipMemcpyAsync(h.data() + l0, chg + l0, sizeof(int) * (launch - l0), hipMemcpyDeviceToHost,
+                               S->stream));
       HIP_CHECK(hipStreamSynchronize(S->stream));
-      if (conv >= 0) {
-        S->mark_depth_hint = conv;
-        converged = true;
-        break;
-      }
-      batch = 32;
+      for (int k = l0; k < launch; k++)
+        if (h[k] == 0) {
+          S->mark_depth_hint = k * INNER;
+          converged = true;
+          break;
+        }
+      batch = 4;
     }
-    if (!converged) {
+    if (!converged)
       hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, S->stream, S->status, (unsigned)SWR_F_MARK_DEPTH);
-    }
-    if (a != S->bond_mark) std::swap(S->bond_mark, S->bond_mark2);
   }
   swr_kt_end(S, KT_MARK);
   HIP_CHECK(hipGetLastError());

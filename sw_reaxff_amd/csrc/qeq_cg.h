@@ -177,17 +177,20 @@ __device__ __forceinline__ bool cg_any_active(int PH, const CgScalars *__restric
 // q = eta d + H d (sparse_matvec_C_para, fix_qeq_reax_sw64.c:424-466) of the
 // vector phase PH multiplies: x (PH_S1), d0 (PH_S2), d1 (PH_S3) or the
 // current d (PH_K).  Half a wave per row, two rows per wave.
-template <int PH, typename T, typename VT>
+template <int PH, bool BATCH, typename T, typename VT>
 __global__ void __launch_bounds__(QT)
 k_cg_spmv(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
           const int *__restrict__ numnbrs, const int *__restrict__ jlist, const VT *__restrict__ val, CgVecsT<T> V,
           int par_in, const CgScalars *__restrict__ cgs, const int *__restrict__ row_sys) {
   typedef typename Vec2<T>::type T2;
+  // one system: the whole launch is skipped once both solves have finished
+  // (uniform scalars, no per-row lookup before the gathers start)
+  if (!BATCH && !cg_any_active(PH, cgs)) return;
   const int w = wave_in_block(), lane = lane_id();
   const int i = blockIdx.x * RPB + w * 2 + (lane >> 5);
-  // rows of systems whose solves have finished are skipped (a half-wave
-  // decides for its own row)
-  const bool live = i < n && cg_any_active(PH, cgs + 2 * row_sys[i]);
+  // a batch: rows of systems whose solves have finished are skipped (a
+  // half-wave decides for its own row)
+  const bool live = i < n && (!BATCH || cg_any_active(PH, cgs + 2 * row_sys[i]));
   const T2 *src = PH == PH_S1 ? V.x : PH == PH_S2 ? V.d[0] : PH == PH_S3 ? V.d[1] : (par_in ? V.d[1] : V.d[0]);
   const int beg = live ? firstnbr[i] : 0, cnt = live ? numnbrs[i] : 0;
   T2 q = row_spmv2_h<T, VT>(beg, cnt, jlist, val, src);
@@ -372,14 +375,26 @@ int launch_cg(swr_system *S, const CgRun<T, VT> &R, int ph, int par_in, int xzer
 #define UP_ARGS R.V, par_in, R.cgs, R.L, S->partials, ps, xzero
 #define SC_ARGS S->partials, ps, R.cgs, R.L, R.imax, R.tol, S->status, R.iters_out
   swr_kt_begin(S, kt);
-  switch (ph) {
-    case PH_S1:
-      if (!xzero) hipLaunchKernelGGL((k_cg_spmv<PH_S1, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS);
-      break;
-    case PH_S2: hipLaunchKernelGGL((k_cg_spmv<PH_S2, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
-    case PH_S3: hipLaunchKernelGGL((k_cg_spmv<PH_S3, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
-    case PH_A1: break;
-    default: hipLaunchKernelGGL((k_cg_spmv<PH_K, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
+  if (R.L.nsys > 1) {
+    switch (ph) {
+      case PH_S1:
+        if (!xzero) hipLaunchKernelGGL((k_cg_spmv<PH_S1, true, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS);
+        break;
+      case PH_S2: hipLaunchKernelGGL((k_cg_spmv<PH_S2, true, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
+      case PH_S3: hipLaunchKernelGGL((k_cg_spmv<PH_S3, true, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
+      case PH_A1: break;
+      default: hipLaunchKernelGGL((k_cg_spmv<PH_K, true, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
+    }
+  } else {
+    switch (ph) {
+      case PH_S1:
+        if (!xzero) hipLaunchKernelGGL((k_cg_spmv<PH_S1, false, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS);
+        break;
+      case PH_S2: hipLaunchKernelGGL((k_cg_spmv<PH_S2, false, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
+      case PH_S3: hipLaunchKernelGGL((k_cg_spmv<PH_S3, false, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
+      case PH_A1: break;
+      default: hipLaunchKernelGGL((k_cg_spmv<PH_K, false, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
+    }
   }
   swr_kt_end(S, kt);
   swr_kt_begin(S, ktu);

