@@ -295,6 +295,8 @@ int swr_set_atoms(swr_system *S, int n, int N, const int *tag, const int *type, 
     S->sys_row.clear();  // a new atom set is one system until swr_set_batch
   }
   S->layout_dirty = true;
+  S->next_totals_known = false;
+  S->vtotal_local = -1;
   S->n = n;
   S->N = N;
   HIP_CHECK(hipMemcpyAsync(S->x, x, sizeof(double) * 3 * N, hipMemcpyHostToDevice, S->stream));
@@ -325,6 +327,7 @@ int swr_new_atoms(swr_system *S) {
     HIP_CHECK(hipMemsetAsync(S->st, 0, sizeof(double) * 2 * S->ncap, S->stream));
   }
   S->capacities_set = false;
+  S->next_totals_known = false;
   S->h_sys_iters.assign(S->h_sys_iters.size(), 0);
   return SWR_OK;
 }
@@ -334,6 +337,7 @@ int swr_set_capacities(swr_system *S, const int *num_bonds, const int *num_hbond
   HIP_CHECK(hipMemcpy(S->num_bonds, num_bonds, sizeof(int) * S->N, hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(S->num_hbonds, num_hbonds, sizeof(int) * S->N, hipMemcpyHostToDevice));
   S->capacities_set = true;
+  S->next_totals_known = false;
   return SWR_OK;
 }
 
@@ -397,10 +401,20 @@ int swr_reset(swr_system *S) {
     // setup-time estimate (Estimate_Storages) from a counting pass
     if ((rc = swr_lists_pack(S)) || (rc = swr_lists_estimate(S))) return rc;
     S->capacities_set = true;
+    S->next_totals_known = false;
   }
+  // The previous step's Validate_Lists wrote the capacities and their totals
+  // (read back at the bond_mark round trip); when they fit, Reset runs
+  // without waiting for its own totals.
+  const DevStatus &h = *S->status_host;
+  if (S->next_totals_known && S->bonds && S->hbonds && h.next_total_bonds <= S->bond_cap * 0.9 &&
+      h.next_total_hbonds <= S->hbond_cap * 0.9) {
+    S->next_totals_known = false;
+    return swr_lists_reset(S);
+  }
+  S->next_totals_known = false;
   // totals first, so the lists can be sized before they are written
   if ((rc = swr_lists_reset(S)) || (rc = refresh_status(S))) return rc;
-  const DevStatus &h = *S->status_host;
   if (h.total_bonds > S->bond_cap * 0.9 || h.total_hbonds > S->hbond_cap * 0.9 || !S->bonds || !S->hbonds) {
     if ((rc = ensure_bond_lists(S, std::max(h.total_bonds, 1), std::max(h.total_hbonds, 1)))) return rc;
     HIP_CHECK(hipMemsetAsync(&S->status->flags, 0, sizeof(unsigned int), S->stream));
@@ -441,8 +455,8 @@ int swr_bo(swr_system *S) {
 
 int swr_qeq_init_matvec(swr_system *S) {
   if (!S || !S->vlist || S->n <= 0) return SWR_ERR_STATE;
-  long long need = 0;
-  HIP_CHECK(hipMemcpy(&need, S->voff + S->n, sizeof(long long), hipMemcpyDeviceToHost));
+  long long need = S->vtotal_local;
+  if (need < 0) HIP_CHECK(hipMemcpy(&need, S->voff + S->n, sizeof(long long), hipMemcpyDeviceToHost));
   int rc;
   if (need > S->Hcap) {
     long long cap = (long long)(need * 1.15) + 1024;
@@ -662,6 +676,7 @@ int swr_put(swr_system *S, const char *name, const void *in, int64_t bytes) {
   HIP_CHECK(hipStreamSynchronize(S->stream));
   HIP_CHECK(hipMemcpy((void *)v.ptr, in, bytes, hipMemcpyHostToDevice));
   if (k == "num_bonds") S->capacities_set = true;
+  S->next_totals_known = false;
   return SWR_OK;
 }
 

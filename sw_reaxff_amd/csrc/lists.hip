@@ -385,18 +385,35 @@ __global__ void k_validate(int n, int N, const int *__restrict__ bond_index, con
                            int *__restrict__ num_bonds, const int *__restrict__ Hindex,
                            const int *__restrict__ hb_count, int *__restrict__ num_hbonds, double saferzone,
                            int bond_total, DevStatus *st) {
+  __shared__ int sb[8], sh[8];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const int c = bond_count[i];
-  num_bonds[i] = max(c * 2, 25);
-  const int comp = (i < N - 1) ? bond_index[i + 1] : bond_total;
-  if (bond_index[i] + c > comp) atomicOr(&st->flags, SWR_F_BOND_CAP);
-  if (i < n) {
-    const int H = Hindex[i];
-    if (H > -1) {
-      const double v = hb_count[i] * saferzone;
-      num_hbonds[i] = (int)(v > 25.0 ? v : 25.0);
+  int tb = 0, th = 0;
+  if (i < N) {
+    const int c = bond_count[i];
+    tb = max(c * 2, 25);
+    num_bonds[i] = tb;
+    const int comp = (i < N - 1) ? bond_index[i + 1] : bond_total;
+    if (bond_index[i] + c > comp) atomicOr(&st->flags, SWR_F_BOND_CAP);
+    if (i < n) {
+      const int H = Hindex[i];
+      if (H > -1) {
+        const double v = hb_count[i] * saferzone;
+        th = (int)(v > 25.0 ? v : 25.0);
+        num_hbonds[i] = th;
+      }
     }
+  }
+  // totals of the new capacities: the next step's Reset sizes its lists from
+  // them without a host round trip (integer sums: order-independent)
+  tb = wave_sum(tb);
+  th = wave_sum(th);
+  if (lane_id() == 0) { sb[wave_in_block()] = tb; sh[wave_in_block()] = th; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int a = 0, b = 0;
+    for (int w = 0; w < (int)blockDim.x / SWR_WAVE; w++) { a += sb[w]; b += sh[w]; }
+    atomicAdd(&st->next_total_bonds, a);
+    atomicAdd(&st->next_total_hbonds, b);
   }
 }
 
@@ -550,6 +567,7 @@ int swr_lists_estimate(swr_system *S) {
 int swr_lists_validate(swr_system *S) {
   const int n = S->n, N = S->N;
   swr_kt_begin(S, KT_VALIDATE);
+  HIP_CHECK(hipMemsetAsync(&S->status->next_total_bonds, 0, 2 * sizeof(int), S->stream));
   hipLaunchKernelGGL(k_validate, dim3(div_up(N, 256)), dim3(256), 0, S->stream, n, N, S->bond_index, S->counts,
                      S->num_bonds, S->Hindex, S->counts + S->Ncap, S->num_hbonds, S->ctl.saferzone, S->bond_cap,
                      S->status);
@@ -587,7 +605,10 @@ int swr_lists_mark(swr_system *S, int max_sweeps) {
                            S->bond_index, S->bond_end, S->bonds, S->bond_mark, chg);
       HIP_CHECK(hipMemcpyAsync(h.data() + l0, chg + l0, sizeof(int) * (launch - l0), hipMemcpyDeviceToHost,
                                S->stream));
+      // the same round trip brings back Validate_Lists' capacity totals
+      HIP_CHECK(hipMemcpyAsync(S->status_host, S->status, sizeof(DevStatus), hipMemcpyDeviceToHost, S->stream));
       HIP_CHECK(hipStreamSynchronize(S->stream));
+      S->next_totals_known = true;
       for (int k = l0; k < launch; k++)
         if (h[k] == 0) {
           S->mark_depth_hint = k * INNER;

@@ -263,6 +263,7 @@ int swr_nbr_ensure_atoms(swr_system *S, int Ncap);
 
 // Builds ghost images of the n local atoms (already remapped into the box).
 int swr_nbr_borders(swr_system *S, const double lo[3], const double hi[3], double cutghost) {
+  S->next_totals_known = false;  // new ghosts take their owners' capacities
   const int n = S->n;
   int ncur = n;
   hipLaunchKernelGGL(k_init_local_maps, dim3(div_up(n, 256)), dim3(256), 0, S->stream, S->owner, S->img, n);
@@ -364,6 +365,8 @@ int swr_nbr_build(swr_system *S, int local_only) {
     S->vcap = cap;
   }
   S->vtotal = total;
+  // local rows' share (the H matrix capacity), read once per build
+  HIP_CHECK(hipMemcpy(&S->vtotal_local, S->voff + S->n, sizeof(long long), hipMemcpyDeviceToHost));
   swr_kt_begin(S, KT_VERLET);
   hipLaunchKernelGGL(k_verlet<true>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->x, N, S->bin_of, S->bin_start,
                      S->bp_pos, S->bp_id, S->stencil_rows, S->nrows, S->nbin[0], S->nbin[1], S->nbin[2], cutsq,

@@ -40,6 +40,11 @@ __device__ __forceinline__ float wave_sum(float v) {
   for (int o = SWR_WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, SWR_WAVE);
   return v;
 }
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = SWR_WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, SWR_WAVE);
+  return v;
+}
 
 // XCD-aware block index: MI355X dispatches workgroups round-robin over 8
 // XCDs, so consecutive blockIdx.x land on different L2s.  Remapping makes

@@ -89,6 +89,8 @@ struct DevStatus {
   long long H_nnz;
   int cg_iters[2];       // s, t
   int inner_iters[2];    // last FP32 inner solve of the mixed-precision refinement
+  int next_total_bonds;  // sums of the capacities Validate_Lists wrote (next step's Reset totals)
+  int next_total_hbonds;
 };
 
 // Pipelined-CG scalars of one right-hand side (CG_v2,
@@ -162,6 +164,8 @@ struct swr_system {
   int hbond_cap = 0;
   swr_hbond_t *hbonds = nullptr;
   bool capacities_set = false;
+  bool next_totals_known = false;  // status_host holds the totals of the current capacities
+  long long vtotal_local = -1;     // Verlet entries of the local rows (voff[n]), known at each build
 
   // ---- BO workspace
   double *bo_dboc = nullptr;       // [N][2]
