@@ -20,7 +20,8 @@ namespace {
 
 constexpr int LT = 256;                 // threads per block
 constexpr int LW = LT / SWR_WAVE;       // waves per block
-constexpr int MB = SWR_MAX_ATOM_BONDS;  // LDS bond slots per wave
+constexpr int MB = SWR_MAX_ATOM_BONDS;  // LDS bond slots per atom
+constexpr int HW = SWR_WAVE / 2;         // lanes per atom in k_bonds (two atoms per wave)
 
 // Orders this wave's LDS writes before its later LDS reads (each wave only
 // touches its own LDS slice, so no workgroup barrier is needed).
@@ -258,10 +259,12 @@ k_far(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__re
   }
 }
 
-// Pass 2: one wave per atom evaluates its bond candidates (one per lane),
-// keeps BO' >= bo_cut in candidate (= far-list) order, sorts the bonds by
-// nbr (my_bubble_sort, reaxc_forces_sw64.c:31-75) and writes the bond
-// records, the per-atom sums and BO section 1.
+// Pass 2: half a wave (32 lanes) per atom, two atoms per wave, evaluates the
+// atom's bond candidates (one per lane), keeps BO' >= bo_cut in candidate
+// (= far-list) order, sorts the bonds by nbr (my_bubble_sort,
+// reaxc_forces_sw64.c:31-75) and writes the bond records, the per-atom sums
+// and BO section 1.  An atom has a handful of candidates, so a full wave per
+// atom left most lanes idle.
 template <bool COUNT_ONLY>
 __global__ void __launch_bounds__(LT)
 k_bonds(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__restrict__ atoms,
@@ -272,20 +275,25 @@ k_bonds(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__
         double *__restrict__ Cdbopi2, double *__restrict__ bo_dboc, double *__restrict__ dDeltap_self,
         int *__restrict__ bond_mark, double *__restrict__ Deltap, double *__restrict__ Deltap_boc,
         int *__restrict__ bond_count, int bond_cap, DevStatus *st) {
-  __shared__ BondRec srec[LW][MB];
-  __shared__ double sdbop[LW][MB][3];
+  __shared__ BondRec srec[2 * LW][MB];
+  __shared__ double sdbop[2 * LW][MB][3];
   const int w = wave_in_block();
-  const int blk = xcd_block(gridDim.x);
-  const int i = blk * LW + w;
-  if (i >= N) return;
   const int lane = lane_id();
-  const int ei = atoms[i].type;
+  const int half = lane / HW, sl = lane % HW;
+  const int slot_a = 2 * w + half;  // this atom's LDS row
+  const int i0 = (xcd_block(gridDim.x) * LW + w) * 2;
+  if (i0 >= N) return;              // both atoms of the wave past the end
+  const int i = i0 + half;
+  const bool live = i < N;
+  const int ei = live ? atoms[i].type : -1;
   const double bo_cut = P->bo_cut;
-  const int ncand = min(cand_count[i], MCAND);
-  const int *mycand = cand + (size_t)i * MCAND;
+  const int ncand = live ? min(cand_count[i], MCAND) : 0;
+  const int ncmax = max(ncand, __shfl_xor(ncand, HW, SWR_WAVE));  // uniform loop bound over the two atoms
+  const int *mycand = cand + (size_t)(live ? i : 0) * MCAND;
+  const unsigned long long below = (1ull << sl) - 1ull;
   int nb = 0;
-  for (int c0 = 0; c0 < ncand; c0 += SWR_WAVE) {
-    const int c = c0 + lane;
+  for (int c0 = 0; c0 < ncmax; c0 += HW) {
+    const int c = c0 + sl;
     bool isb = false;
     BondRec r;
     if (c < ncand && ei >= 0) {
@@ -296,28 +304,27 @@ k_bonds(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__
         r.d = e.d; r.dx = e.dvec[0]; r.dy = e.dvec[1]; r.dz = e.dvec[2]; r.BO = t.BO; r.nbr = e.nbr; r.ej = e.type;
       }
     }
-    const unsigned long long bm = __ballot(isb);
+    const unsigned long long bm = (__ballot(isb) >> (HW * half)) & 0xffffffffull;
     if (!COUNT_ONLY && isb) {
-      const int slot = nb + ballot_rank(bm);
-      if (slot < MB) srec[w][slot] = r;
+      const int slot = nb + __popcll(bm & below);
+      if (slot < MB) srec[slot_a][slot] = r;
     }
     nb += __popcll(bm);
   }
-  if (lane == 0) bond_count[i] = nb;
+  if (live && sl == 0) bond_count[i] = nb;
   if (COUNT_ONLY) return;
-  const int cap = num_bonds[i];
-  const int start = bond_index[i];
-  if (nb > MB || nb > cap || start + nb > bond_cap) {
-    if (lane == 0) {
-      atomicOr(&st->flags, nb > MB ? (SWR_F_BOND_LDS | SWR_F_BOND_CAP) : SWR_F_BOND_CAP);
-      atomicMax(&st->need_bond_cap, nb);
-      bond_end[i] = start;
-    }
-    return;
+  const int cap = live ? num_bonds[i] : 0;
+  const int start = live ? bond_index[i] : 0;
+  const bool over = live && (nb > MB || nb > cap || start + nb > bond_cap);
+  if (over && sl == 0) {
+    atomicOr(&st->flags, nb > MB ? (SWR_F_BOND_LDS | SWR_F_BOND_CAP) : SWR_F_BOND_CAP);
+    atomicMax(&st->need_bond_cap, nb);
+    bond_end[i] = start;
   }
+  const bool write = live && !over;
   wave_lds_sync();
-  if (lane < nb) {
-    const BondRec rc = srec[w][lane];
+  if (write && sl < nb) {
+    const BondRec rc = srec[slot_a][sl];
     const BOpTerms t = bop_terms(P, ei, rc.ej, rc.d);
     const DevTbp &tw = P->tbp[ei * SWR_MAX_ELEM + rc.ej];
     const double r2 = rc.d * rc.d;
@@ -331,19 +338,19 @@ k_bonds(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__
       o.dln_BOp_pi[c] = -t.pi * Cp * dv[c];
       o.dln_BOp_pi2[c] = -t.pi2 * Cpp * dv[c];
       o.dBOp[c] = cdb * dv[c];
-      sdbop[w][lane][c] = o.dBOp[c];
+      sdbop[slot_a][sl][c] = o.dBOp[c];
     }
     o.C1dbo = o.C2dbo = o.C3dbo = 0.0;
     o.C1dbopi = o.C2dbopi = o.C3dbopi = o.C4dbopi = 0.0;
     o.C1dbopi2 = o.C2dbopi2 = o.C3dbopi2 = o.C4dbopi2 = 0.0;
     o.CdboReduction = nullptr;
     int rank = 0;
-    for (int m = 0; m < nb; m++) rank += (srec[w][m].nbr < rc.nbr) ? 1 : 0;
+    for (int m = 0; m < nb; m++) rank += (srec[slot_a][m].nbr < rc.nbr) ? 1 : 0;
     const int p = start + rank;
     swr_bond_t b;
     b.d = rc.d; b.dvec[0] = dv[0]; b.dvec[1] = dv[1]; b.dvec[2] = dv[2];
     b.nbr = rc.nbr;
-    b.dbond_index = start + lane;
+    b.dbond_index = start + sl;
     b.rel_box[0] = b.rel_box[1] = b.rel_box[2] = 0;
     b.sym_index = 0;
     bonds[p] = b;
@@ -354,13 +361,13 @@ k_bonds(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__
     Cdbo[p] = 0.0; Cdbopi[p] = 0.0; Cdbopi2[p] = 0.0;
   }
   wave_lds_sync();
-  if (lane == 0) {
+  if (write && sl == 0) {
     // bo_dboc[i][0] and dDeltap_self[i] accumulate in candidate order
     // (reaxc_forces_sw64.c:1253,1262), starting from Reset's zeros.
     double tot = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int m = 0; m < nb; m++) {
-      tot += srec[w][m].BO - bo_cut;
-      s0 += sdbop[w][m][0]; s1 += sdbop[w][m][1]; s2 += sdbop[w][m][2];
+      tot += srec[slot_a][m].BO - bo_cut;
+      s0 += sdbop[slot_a][m][0]; s1 += sdbop[slot_a][m][1]; s2 += sdbop[slot_a][m][2];
     }
     bond_end[i] = start + nb;
     bo_dboc[2 * (size_t)i] = tot;
@@ -529,7 +536,7 @@ int swr_lists_run(swr_system *S, int mode) {
                      S->far_index, S->far_end, S->cand, S->cand_count, S->Hindex, S->num_hbonds, S->hbond_index,  \
                      S->hbond_end, S->hbonds, hb_count, S->hbond_cap, S->status)
 #define BOND_ARGS(k)                                                                                              \
-  hipLaunchKernelGGL(k, dim3(nblk), dim3(LT), 0, S->stream, S->n, N, S->dp, S->packed, S->far, S->cand,          \
+  hipLaunchKernelGGL(k, dim3(div_up(N, 2 * LW)), dim3(LT), 0, S->stream, S->n, N, S->dp, S->packed, S->far, S->cand,          \
                      S->cand_count, S->num_bonds, S->bond_index, S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, \
                      S->Cdbo, S->Cdbopi, S->Cdbopi2, S->bo_dboc, S->dDeltap_self, S->bond_mark,                    \
                      ws + (size_t)WS_DELTAP * S->Ncap, ws + (size_t)WS_DELTAP_BOC * S->Ncap, S->counts,            \

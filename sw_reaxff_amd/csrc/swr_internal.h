@@ -31,7 +31,7 @@
 #define SWR_F_NAN        (1u << 7)   // non-finite value in CG scalars
 #define SWR_F_CG_MAXIT   (1u << 8)   // a QEq solve stopped at its iteration limit (a warning, as in the reference)
 
-#define SWR_MAX_ATOM_BONDS 64        // bonds one atom may hold while building its list
+#define SWR_MAX_ATOM_BONDS 32        // bonds one atom may hold while building its list (the reference: MAX_LEN 20, MAXBOND 35)
 
 #define HIP_CHECK(expr)                                                    \
   do {                                                                     \
