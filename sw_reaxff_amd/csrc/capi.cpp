@@ -268,10 +268,11 @@ int swr_destroy(swr_system *S) {
                   S->bond_mark2, S->counts, S->ws, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32,
                   S->Hdia_inv, S->b_s, S->st, S->s_hist, S->t_hist, S->cgv, S->cgs, S->partials, S->counters,
                   S->status, S->scan_tmp, S->tmp_i, S->mixw, S->cand, S->cand_count, S->row_sys, S->tile_beg,
-                  S->tile_sys, S->sys_tile, S->sys_iters, S->done_count, S->qsum};
+                  S->tile_sys, S->sys_tile, S->sys_iters, S->done_count, S->qsum, S->mark_chg};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (S->status_host) (void)hipHostFree(S->status_host);
+  if (S->mark_chg_host) (void)hipHostFree(S->mark_chg_host);
   if (S->stream) (void)hipStreamDestroy(S->stream);
   delete S;
   return SWR_OK;
@@ -283,6 +284,7 @@ int swr_set_atoms(swr_system *S, int n, int N, const int *tag, const int *type, 
     swr_set_error_msg("swr_set_atoms: bad arguments");
     return SWR_ERR_ARG;
   }
+  if (int rc = swr_quiesce(S)) return rc;
   for (int i = 0; i < N; i++)
     if (type[i] < 1 || type[i] > S->hp.ntypes) {
       swr_set_error_msg("swr_set_atoms: atom type out of range");
@@ -321,6 +323,7 @@ int swr_set_atoms(swr_system *S, int n, int N, const int *tag, const int *type, 
 
 int swr_new_atoms(swr_system *S) {
   if (!S) return SWR_ERR_ARG;
+  if (int rc = swr_quiesce(S)) return rc;
   if (S->ncap > 0) {
     HIP_CHECK(hipMemsetAsync(S->s_hist, 0, sizeof(double) * 5 * S->ncap, S->stream));
     HIP_CHECK(hipMemsetAsync(S->t_hist, 0, sizeof(double) * 5 * S->ncap, S->stream));
@@ -334,6 +337,7 @@ int swr_new_atoms(swr_system *S) {
 
 int swr_set_capacities(swr_system *S, const int *num_bonds, const int *num_hbonds) {
   if (!S || !num_bonds || !num_hbonds) return SWR_ERR_ARG;
+  if (int rc = swr_quiesce(S)) return rc;
   HIP_CHECK(hipMemcpy(S->num_bonds, num_bonds, sizeof(int) * S->N, hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(S->num_hbonds, num_hbonds, sizeof(int) * S->N, hipMemcpyHostToDevice));
   S->capacities_set = true;
@@ -391,12 +395,17 @@ static int ensure_bond_lists(swr_system *S, int need_bonds, int need_hbonds) {
 static int refresh_status(swr_system *S) {
   HIP_CHECK(hipMemcpyAsync(S->status_host, S->status, sizeof(DevStatus), hipMemcpyDeviceToHost, S->stream));
   HIP_CHECK(hipStreamSynchronize(S->stream));
+  if (swr_after_sync(S)) {  // a finished relaxation raised a flag after the copy
+    HIP_CHECK(hipMemcpyAsync(S->status_host, S->status, sizeof(DevStatus), hipMemcpyDeviceToHost, S->stream));
+    HIP_CHECK(hipStreamSynchronize(S->stream));
+  }
   return SWR_OK;
 }
 
 int swr_reset(swr_system *S) {
   if (!S || S->N <= 0 || !S->vlist) return SWR_ERR_STATE;
   int rc;
+  if ((rc = swr_quiesce(S))) return rc;  // Reset rewrites the bond ranges a pending relaxation reads
   if (!S->capacities_set) {
     // setup-time estimate (Estimate_Storages) from a counting pass
     if ((rc = swr_lists_pack(S)) || (rc = swr_lists_estimate(S))) return rc;
@@ -404,7 +413,7 @@ int swr_reset(swr_system *S) {
     S->next_totals_known = false;
   }
   // The previous step's Validate_Lists wrote the capacities and their totals
-  // (read back at the bond_mark round trip); when they fit, Reset runs
+  // (read back at the host's next sync, normally QEq's); when they fit, Reset runs
   // without waiting for its own totals.
   const DevStatus &h = *S->status_host;
   if (S->next_totals_known && S->bonds && S->hbonds && h.next_total_bonds <= S->bond_cap * 0.9 &&
@@ -495,10 +504,12 @@ int swr_qeq_cg(swr_system *S, int *iters_s, int *iters_t) {
   for (size_t k = 0; k < S->h_sys_iters.size(); k += 2) its0 = std::max(its0, S->h_sys_iters[k]);
   int rc = S->mixed ? swr_qeq_mixed_solve(S) : swr_qeq_cg_run(S, its0 + 2);
   if (rc) return rc;
-  S->h_sys_iters.resize(2 * (size_t)S->nsys);
-  HIP_CHECK(hipMemcpyAsync(S->h_sys_iters.data(), S->sys_iters, sizeof(int) * 2 * S->nsys, hipMemcpyDeviceToHost,
-                           S->stream));
-  if ((rc = refresh_status(S))) return rc;  // synchronises the stream
+  if (S->mixed) {  // the FP64 solve's last done check already brought status and counts back
+    S->h_sys_iters.resize(2 * (size_t)S->nsys);
+    HIP_CHECK(hipMemcpyAsync(S->h_sys_iters.data(), S->sys_iters, sizeof(int) * 2 * S->nsys,
+                             hipMemcpyDeviceToHost, S->stream));
+    if ((rc = refresh_status(S))) return rc;
+  }
   int ms = 0, mt = 0;
   for (int k = 0; k < S->nsys; k++) {
     ms = std::max(ms, S->h_sys_iters[2 * k]);
@@ -654,6 +665,7 @@ int64_t swr_get(swr_system *S, const char *name, void *out) {
   if (!lookup(S, k, v)) return -1;
   if (!out || v.bytes <= 0 || !v.ptr) return v.bytes < 0 ? 0 : v.bytes;
   (void)hipStreamSynchronize(S->stream);
+  (void)swr_after_sync(S);  // a pending bond_mark relaxation finishes before anything is read
   if (v.stride_kind == 3) {  // host-side table
     std::memcpy(out, v.ptr, v.bytes);
   } else if (v.stride_kind == 0) {
@@ -673,6 +685,7 @@ int swr_put(swr_system *S, const char *name, const void *in, int64_t bytes) {
   std::string k = name;
   View v;
   if (!lookup(S, k, v) || bytes > v.bytes || v.stride_kind != 0) return SWR_ERR_ARG;
+  if (int rc = swr_quiesce(S)) return rc;
   HIP_CHECK(hipStreamSynchronize(S->stream));
   HIP_CHECK(hipMemcpy((void *)v.ptr, in, bytes, hipMemcpyHostToDevice));
   if (k == "num_bonds") S->capacities_set = true;

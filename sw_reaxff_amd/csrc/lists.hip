@@ -522,6 +522,10 @@ int swr_lists_reset(swr_system *S) {
 }
 
 int swr_lists_run(swr_system *S, int mode) {
+  {  // k_bonds rewrites bond_mark: finish the last relaxation first
+    int rc = swr_quiesce(S);
+    if (rc) return rc;
+  }
   const int N = S->N;
   const int nblk = div_up(N, LW);
   double *ws = S->ws;
@@ -578,6 +582,10 @@ int swr_lists_validate(swr_system *S) {
   hipLaunchKernelGGL(k_validate, dim3(div_up(N, 256)), dim3(256), 0, S->stream, n, N, S->bond_index, S->counts,
                      S->num_bonds, S->Hindex, S->counts + S->Ncap, S->num_hbonds, S->ctl.saferzone, S->bond_cap,
                      S->status);
+  // the capacity totals travel back with the next sync (next step's Reset)
+  HIP_CHECK(hipMemcpyAsync(S->status_host, S->status, sizeof(DevStatus), hipMemcpyDeviceToHost, S->stream));
+  S->status_copy_queued = true;
+  S->next_totals_known = false;
   swr_kt_end(S, KT_VALIDATE);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
@@ -591,43 +599,84 @@ int swr_lists_validate(swr_system *S) {
 // The sweeps below evaluate that recursion over the j < i graph (Jacobi
 // sweeps: sweep s fixes every ghost whose dependency chain has depth <= s)
 // until no mark changes; the host reads the convergence flag between batches.
+// Enqueues the relaxation (last step's launch count + 2, no host wait) and a
+// copy of the per-launch change flags; swr_mark_settle checks them at the next
+// sync and runs further launches only if the last one still changed a mark.
 int swr_lists_mark(swr_system *S, int max_sweeps) {
   const int n = S->n, N = S->N;
-  constexpr int INNER = 8;  // relaxation sweeps per launch
-  swr_kt_begin(S, KT_MARK);
-  if (N > n) {
-    const int max_launch = div_up(max_sweeps, INNER);
-    int rc = swr_ensure_tmp(S, sizeof(int) * (size_t)max_launch);
+  S->mark_launched = 0;
+  if (N <= n) return SWR_OK;
+  if (!S->mark_chg) {
+    int rc = swr_alloc(&S->mark_chg, MARK_MAX_LAUNCH);
     if (rc) return rc;
-    int *chg = S->tmp_i;
-    hipLaunchKernelGGL(k_mark_init, dim3(div_up(max_launch, 256)), dim3(256), 0, S->stream, chg, max_launch);
-    int launch = 0;
-    int batch = std::max(2, div_up(S->mark_depth_hint, INNER) + 1);
-    bool converged = false;
-    std::vector<int> h(max_launch);
-    while (launch < max_launch && !converged) {
-      const int l0 = launch;
-      for (int k = 0; k < batch && launch < max_launch; k++, launch++)
-        hipLaunchKernelGGL(k_bond_mark, dim3(div_up(N - n, 256)), dim3(256), 0, S->stream, n, N, launch, INNER,
-                           S->bond_index, S->bond_end, S->bonds, S->bond_mark, chg);
-      HIP_CHECK(hipMemcpyAsync(h.data() + l0, chg + l0, sizeof(int) * (launch - l0), hipMemcpyDeviceToHost,
-                               S->stream));
-      // the same round trip brings back Validate_Lists' capacity totals
-      HIP_CHECK(hipMemcpyAsync(S->status_host, S->status, sizeof(DevStatus), hipMemcpyDeviceToHost, S->stream));
-      HIP_CHECK(hipStreamSynchronize(S->stream));
-      S->next_totals_known = true;
-      for (int k = l0; k < launch; k++)
-        if (h[k] == 0) {
-          S->mark_depth_hint = k * INNER;
-          converged = true;
-          break;
-        }
-      batch = 4;
+    if (hipHostMalloc((void **)&S->mark_chg_host, sizeof(int) * MARK_MAX_LAUNCH, 0) != hipSuccess) {
+      swr_set_error_msg("hipHostMalloc failed");
+      return SWR_ERR_NOMEM;
     }
-    if (!converged)
-      hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, S->stream, S->status, (unsigned)SWR_F_MARK_DEPTH);
   }
+  const int max_launch = std::min(MARK_MAX_LAUNCH, div_up(max_sweeps, MARK_INNER));
+  const int L = std::min(max_launch, std::max(2, div_up(S->mark_depth_hint, MARK_INNER) + 2));
+  swr_kt_begin(S, KT_MARK);
+  hipLaunchKernelGGL(k_mark_init, dim3(1), dim3(MARK_MAX_LAUNCH), 0, S->stream, S->mark_chg, MARK_MAX_LAUNCH);
+  for (int k = 0; k < L; k++)
+    hipLaunchKernelGGL(k_bond_mark, dim3(div_up(N - n, 256)), dim3(256), 0, S->stream, n, N, k, MARK_INNER,
+                       S->bond_index, S->bond_end, S->bonds, S->bond_mark, S->mark_chg);
+  HIP_CHECK(hipMemcpyAsync(S->mark_chg_host, S->mark_chg, sizeof(int) * L, hipMemcpyDeviceToHost, S->stream));
   swr_kt_end(S, KT_MARK);
   HIP_CHECK(hipGetLastError());
+  S->mark_launched = L;
+  return SWR_OK;
+}
+
+// Finishes a pending relaxation: the flags copied at enqueue time are valid
+// once the stream has passed them (call after a sync).  Converged when some
+// launch changed nothing; otherwise relaxes further, synchronously.  Returns
+// 1 when it gave up and raised SWR_F_MARK_DEPTH on the device.
+int swr_mark_settle(swr_system *S) {
+  if (S->mark_launched <= 0) return SWR_OK;
+  const int n = S->n, N = S->N;
+  int launch = S->mark_launched;
+  S->mark_launched = 0;
+  for (int k = 0; k < launch; k++)
+    if (S->mark_chg_host[k] == 0) {
+      S->mark_depth_hint = k * MARK_INNER;
+      return SWR_OK;
+    }
+  while (launch < MARK_MAX_LAUNCH) {
+    const int l0 = launch;
+    for (int k = 0; k < 4 && launch < MARK_MAX_LAUNCH; k++, launch++)
+      hipLaunchKernelGGL(k_bond_mark, dim3(div_up(N - n, 256)), dim3(256), 0, S->stream, n, N, launch, MARK_INNER,
+                         S->bond_index, S->bond_end, S->bonds, S->bond_mark, S->mark_chg);
+    HIP_CHECK(hipMemcpyAsync(S->mark_chg_host + l0, S->mark_chg + l0, sizeof(int) * (launch - l0),
+                             hipMemcpyDeviceToHost, S->stream));
+    HIP_CHECK(hipStreamSynchronize(S->stream));
+    for (int k = l0; k < launch; k++)
+      if (S->mark_chg_host[k] == 0) {
+        S->mark_depth_hint = k * MARK_INNER;
+        return SWR_OK;
+      }
+  }
+  hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, S->stream, S->status, (unsigned)SWR_F_MARK_DEPTH);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(S->stream));
+  return 1;
+}
+
+// Bookkeeping after the host has synchronised with the system's stream.
+// Returns 1 when it changed the device status block after that sync.
+int swr_after_sync(swr_system *S) {
+  if (S->status_copy_queued) {
+    S->status_copy_queued = false;
+    S->next_totals_known = true;
+  }
+  return swr_mark_settle(S) == 1;
+}
+
+// Waits for and finishes a pending relaxation before list arrays are
+// reallocated or the atoms change.
+int swr_quiesce(swr_system *S) {
+  if (S->mark_launched <= 0) return SWR_OK;
+  HIP_CHECK(hipStreamSynchronize(S->stream));
+  (void)swr_after_sync(S);
   return SWR_OK;
 }

@@ -282,6 +282,8 @@ int swr_qeq_cg_run(swr_system *S, int first_batch) {
   R.L = swr_qeq_cg_layout(S);
   R.cgs = S->cgs;
   R.iters_out = S->sys_iters;
+  S->h_sys_iters.resize(2 * (size_t)S->nsys);
+  R.iters_host = S->h_sys_iters.data();
   R.tol = S->ctl.tol;
   R.imax = S->imax;
   R.kt_iter = KT_CG_ITER;

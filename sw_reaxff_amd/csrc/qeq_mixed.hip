@@ -187,6 +187,7 @@ int swr_qeq_mixed_solve(swr_system *S) {
   R.L = swr_qeq_cg_layout(S);
   R.cgs = S->cgs + 2;   // after the FP64 solve's s, t
   R.iters_out = S->status->inner_iters;
+  R.iters_host = nullptr;
   R.tol = -1.0;  // set per rhs by k_resid_scalars
   R.imax = INNER_IMAX;
   R.kt_iter = KT_MIXED_INNER;

@@ -165,6 +165,11 @@ struct swr_system {
   swr_hbond_t *hbonds = nullptr;
   bool capacities_set = false;
   bool next_totals_known = false;  // status_host holds the totals of the current capacities
+  bool status_copy_queued = false; // a status copy was queued after Validate_Lists (done at the next sync)
+  // bond_mark relaxation whose convergence is checked at the next sync
+  int *mark_chg = nullptr;         // [MARK_MAX_LAUNCH] device: launch k changed a mark
+  int *mark_chg_host = nullptr;    // pinned copy
+  int mark_launched = 0;           // launches of the pending relaxation (0: none pending)
   long long vtotal_local = -1;     // Verlet entries of the local rows (voff[n]), known at each build
 
   // ---- BO workspace
@@ -262,6 +267,11 @@ int swr_lists_run(swr_system *S, int mode);
 int swr_lists_estimate(swr_system *S);
 int swr_lists_validate(swr_system *S);
 int swr_lists_mark(swr_system *S, int max_sweeps);
+int swr_mark_settle(swr_system *S);
+int swr_after_sync(swr_system *S);
+int swr_quiesce(swr_system *S);
+constexpr int MARK_INNER = 8;         // bond_mark relaxation sweeps per launch
+constexpr int MARK_MAX_LAUNCH = 128;  // launches of one relaxation at most
 int swr_bo_run(swr_system *S);
 int swr_qeq_build_H(swr_system *S);
 int swr_qeq_spmv(swr_system *S, const double *xv, double *b);
