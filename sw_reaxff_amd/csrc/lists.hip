@@ -259,6 +259,243 @@ k_far(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__re
   }
 }
 
+// Orders LDS accesses around a workgroup barrier without waiting for the
+// wave's outstanding global stores (a full workgroup fence would add
+// s_waitcnt vmcnt(0), and on this architecture vmcnt counts stores too).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Pass 1 for a Verlet-list source, split between two kinds of waves.
+//
+// A wave's vector-memory counter (vmcnt) counts loads and stores together
+// and retires them in issue order, so a wave that writes its far entries
+// and then gathers the next neighbours waits for its own stores to reach
+// memory before every gather.  Here FG "gatherer" waves (one atom each)
+// only load: per phase each takes one 64-candidate chunk of its atom's
+// Verlet row, keeps d^2 <= nonb_cut^2 and stages the compacted 64-byte
+// entries (plus hbond and bond-candidate records) in LDS, while FWR
+// "writer" waves, which never load from global memory, stream the previous
+// phase's staged entries out as contiguous 16-byte nontemporal stores.  One
+// LDS barrier per phase; double-buffered staging.  Entry order and content
+// are those of k_far (a far list is its Verlet row filtered in order).
+constexpr int FG = 4;                        // gatherer waves per block
+constexpr int FWR = 2;                       // writer waves per block
+constexpr int FT = (FG + FWR) * SWR_WAVE;    // threads per block
+
+struct FarStage {
+  double2 far[2][FG][4 * SWR_WAVE];  // staged entries, 4 pieces each
+  int2 hb[2][FG][SWR_WAVE];          // staged hbonds: (nbr, far slot)
+  int cand[FG][MCAND];               // bond-candidate far slots of the atom
+  double rbo[FG][SWR_MAX_ELEM];      // BO' radius row of the atom's element
+  long long dst[2][FG];              // far slot of the phase's first entry
+  int cnt[2][FG], hdst[2][FG], hcnt[2][FG];
+  int nphase[FG];
+};
+
+// A gatherer's neighbour record for one chunk: the id and the packed atom.
+struct FarNbr {
+  int j;
+  swr_atom_pack_t a;
+};
+
+template <bool CAND, int HB>
+__global__ void __launch_bounds__(FT)
+k_far_ws(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__restrict__ atoms,
+         const long long *__restrict__ voff, const int *__restrict__ vlist, swr_far_neighbor_t *__restrict__ far,
+         int *__restrict__ far_index, int *__restrict__ far_end, int *__restrict__ cand,
+         int *__restrict__ cand_count, const int *__restrict__ Hindex, const int *__restrict__ num_hbonds,
+         const int *__restrict__ hbond_index, int *__restrict__ hbond_end, swr_hbond_t *__restrict__ hbonds,
+         int *__restrict__ hb_count, int hbond_cap, DevStatus *st) {
+  __shared__ FarStage S;
+  const int w = __builtin_amdgcn_readfirstlane(wave_in_block());
+  const int lane = lane_id();
+  const bool gatherer = w < FG;
+
+  // ---- gatherer set-up: the atom, its Verlet row, its element's tables
+  const int g = gatherer ? w : 0;
+  const int i = blockIdx.x * FG + g;
+  const bool live = gatherer && i < N;
+  swr_atom_pack_t ai = {};
+  long long cbeg = 0, cend = 0;
+  int ei = -1;
+  if (live) {
+    ai = atoms[i];
+    cbeg = voff[i];
+    cend = voff[i + 1];
+    ei = __builtin_amdgcn_readfirstlane(ai.type);
+  }
+  const int nch = live ? (int)((cend - cbeg + SWR_WAVE - 1) / SWR_WAVE) : 0;
+  const double nonb_cutsq = P->nonb_cutsq, bond_cut = P->bond_cut, hbond_cut = P->hbond_cut;
+  unsigned hmask2 = 0;  // elements with p_hbond == 2 (acceptors)
+  for (int e = 0; e < SWR_MAX_ELEM; e++) hmask2 |= (P->p_hbond[e] == 2 ? 1u : 0u) << e;
+  const bool doH = HB != 0 && live && i < n && ei >= 0 && P->p_hbond[ei] == 1 && hbond_cut > 0;
+  const int H = (doH && HB == 2) ? Hindex[i] : -1;
+  int htop = (doH && HB == 2) ? hbond_end[H] : 0;
+  const int hlim = (doH && HB == 2) ? min(hbond_index[H] + num_hbonds[i], hbond_cap) : 0;
+  if (gatherer) {
+    if (CAND && lane < SWR_MAX_ELEM) S.rbo[g][lane] = P->rbo[(ei >= 0 ? ei : 0) * SWR_MAX_ELEM + lane];
+    if (lane == 0) {
+      S.nphase[g] = nch;
+      S.cnt[0][g] = S.cnt[1][g] = 0;
+      S.hcnt[0][g] = S.hcnt[1][g] = 0;
+    }
+  }
+  // Software pipeline without register copies (a copy of a loaded value
+  // waits for the load): while chunk p is evaluated, chunk p+2's atoms and
+  // chunk p+3's ids are requested, into three neighbour-record sets and
+  // three id registers that rotate with period 3.  Every phase of an active
+  // gatherer issues the same loads (indices clamped to the row), and the
+  // gatherer, idle-gatherer and writer loops are separate, so the
+  // compiler's wait counts let two phases of loads stay in flight.
+  const long long clast = cend - 1;
+  FarNbr X = {}, Y = {}, Z = {};
+  int R0 = 0, R1 = 0, R2 = 0;
+  if (nch > 0) {
+    const int j0 = vlist[min(cbeg + lane, clast)];
+    const int j1 = vlist[min(cbeg + SWR_WAVE + lane, clast)];
+    R0 = vlist[min(cbeg + 2 * SWR_WAVE + lane, clast)];
+    X.a = atoms[j0];
+    X.j = j0;
+    Y.a = atoms[j1];
+    Y.j = j1;
+  }
+  lds_barrier();
+  int nph = 0;
+  for (int k = 0; k < FG; k++) nph = max(nph, S.nphase[k]);
+  // phases 0..nph (the writers drain phase p-1 in phase p), rounded up to
+  // whole rotations; identical in every wave of the block
+  const int nphr = __builtin_amdgcn_readfirstlane((nph + 1 + 2) / 3 * 3);
+
+  const long long far0 = cbeg;
+  int nfar = 0, ncand = 0;
+
+  // gatherer phase: evaluate chunk p (cur), request chunk p+3's ids into
+  // jfree and chunk p+2's atoms (ids jaddr) into far2
+  auto gather = [&](int p, const FarNbr &cur, FarNbr &far2, const int &jaddr, int &jfree) {
+    const int buf = p & 1;
+    const long long k = cbeg + (long long)p * SWR_WAVE + lane;
+    jfree = vlist[min(k + 3 * SWR_WAVE, clast)];
+    far2.a = atoms[jaddr];
+    far2.j = jaddr;
+    if (p < nch) {
+      const bool valid = k < cend;
+      double dx = 0, dy = 0, dz = 0, d = 0;
+      bool infar = false;
+      if (valid) {
+        dx = cur.a.x[0] - ai.x[0]; dy = cur.a.x[1] - ai.x[1]; dz = cur.a.x[2] - ai.x[2];
+        const double dsq = dx * dx + dy * dy + dz * dz;
+        infar = dsq <= nonb_cutsq;
+        if (infar) d = sqrt(dsq);
+      }
+      const int tj = infar ? cur.a.type : -1;
+      const unsigned long long m = __ballot(infar);
+      const int r = ballot_rank(m);
+      const long long slot = far0 + nfar + r;
+      if (infar) stage_far(S.far[buf][g] + 4 * r, cur.j, tj, cur.a.orig_id, d, cur.a.q, dx, dy, dz);
+      if (lane == 0) {
+        S.dst[buf][g] = far0 + nfar;
+        S.cnt[buf][g] = __popcll(m);
+      }
+      if (CAND) {
+        const bool iscand = infar && ei >= 0 && tj >= 0 && d <= bond_cut && d <= S.rbo[g][tj];
+        const unsigned long long cm = __ballot(iscand);
+        if (iscand) {
+          const int c = ncand + ballot_rank(cm);
+          if (c < MCAND) S.cand[g][c] = (int)slot;
+        }
+        ncand += __popcll(cm);
+      }
+      if (doH) {
+        const bool ish = infar && d <= hbond_cut && tj >= 0 && ((hmask2 >> tj) & 1u);
+        const unsigned long long hm = __ballot(ish);
+        if (HB == 2) {
+          const int hr = ballot_rank(hm);
+          if (ish && htop + hr < hlim) S.hb[buf][g][hr] = make_int2(cur.j, (int)slot);
+          if (lane == 0) {
+            S.hdst[buf][g] = htop;
+            S.hcnt[buf][g] = max(0, min(__popcll(hm), hlim - htop));
+          }
+        }
+        htop += __popcll(hm);
+      } else if (HB == 2 && lane == 0) {
+        S.hcnt[buf][g] = 0;
+      }
+      nfar += __popcll(m);
+    } else if (lane == 0) {
+      S.cnt[buf][g] = 0;
+      S.hcnt[buf][g] = 0;
+    }
+    lds_barrier();
+  };
+
+  if (gatherer && nch > 0) {
+    for (int p = 0; p < nphr; p += 3) {
+      gather(p, X, Z, R0, R1);
+      gather(p + 1, Y, X, R1, R2);
+      gather(p + 2, Z, Y, R2, R0);
+    }
+  } else if (gatherer) {
+    for (int p = 0; p < nphr; p++) lds_barrier();
+  } else {
+    // writer: in phase p, drain phase p-1's staging of its gatherers
+    for (int p = 0; p < nphr; p++) {
+      if (p > 0) {
+        const int pb = (p - 1) & 1;
+        for (int gg = w - FG; gg < FG; gg += FWR) {
+          const int cnt = S.cnt[pb][gg];
+          if (cnt > 0) {
+            double2 *dstp = reinterpret_cast<double2 *>(far + S.dst[pb][gg]);
+            const double2 *src = S.far[pb][gg];
+            for (int t = lane; t < 4 * cnt; t += SWR_WAVE) store_nt(dstp + t, src[t]);
+          }
+          if (HB == 2) {
+            const int hc = S.hcnt[pb][gg];
+            if (lane < hc) {
+              const int2 e = S.hb[pb][gg][lane];
+              swr_hbond_t hb;
+              hb.nbr = e.x;
+              hb.scl = 1;
+              hb.ptr = far + e.y;
+              hbonds[S.hdst[pb][gg] + lane] = hb;
+            }
+          }
+        }
+      }
+      lds_barrier();
+    }
+  }
+  if (!live) return;
+  // per-atom results (the gatherer's only global stores)
+  if (CAND) {
+    wave_lds_sync();
+    const int nc = min(ncand, MCAND);
+    for (int t = lane; t < nc; t += SWR_WAVE) cand[(size_t)i * MCAND + t] = S.cand[g][t];
+  }
+  if (lane == 0) {
+    far_index[i] = (int)far0;
+    far_end[i] = (int)(far0 + nfar);
+    if (CAND) {
+      cand_count[i] = ncand;
+      if (ncand > MCAND) atomicOr(&st->flags, SWR_F_BOND_LDS);
+    }
+    if (doH) {
+      if (HB == 1) {
+        hb_count[i] = htop;
+      } else {
+        if (htop > hlim) {
+          atomicOr(&st->flags, SWR_F_HBOND_CAP);
+          atomicMax(&st->need_hbond_cap, htop - hbond_index[H]);
+        }
+        hbond_end[H] = htop > hlim ? hlim : htop;
+        hb_count[i] = htop - hbond_index[H];
+      }
+    }
+  }
+}
+
 // Pass 2: half a wave (32 lanes) per atom, two atoms per wave, evaluates the
 // atom's bond candidates (one per lane), keeps BO' >= bo_cut in candidate
 // (= far-list) order, sorts the bonds by nbr (my_bubble_sort,
@@ -545,11 +782,15 @@ int swr_lists_run(swr_system *S, int mode) {
                      S->Cdbo, S->Cdbopi, S->Cdbopi2, S->bo_dboc, S->dDeltap_self, S->bond_mark,                    \
                      ws + (size_t)WS_DELTAP * S->Ncap, ws + (size_t)WS_DELTAP_BOC * S->Ncap, S->counts,            \
                      S->bond_cap, S->status)
+#define FARWS_ARGS(k)                                                                                            \
+  hipLaunchKernelGGL(k, dim3(div_up(N, FG)), dim3(FT), 0, S->stream, S->n, N, S->dp, S->packed, S->voff, S->vlist,  \
+                     S->far, S->far_index, S->far_end, S->cand, S->cand_count, S->Hindex, S->num_hbonds,            \
+                     S->hbond_index, S->hbond_end, S->hbonds, hb_count, S->hbond_cap, S->status)
   swr_kt_begin(S, KT_FAR);
-  if (mode == 0) FAR_ARGS((k_far<false, true, false, 0>));
-  else if (mode == 1) FAR_ARGS((k_far<false, true, true, 2>));
+  if (mode == 0) FARWS_ARGS((k_far_ws<false, 0>));
+  else if (mode == 1) FARWS_ARGS((k_far_ws<true, 2>));
   else if (mode == 2) FAR_ARGS((k_far<true, false, true, 2>));
-  else FAR_ARGS((k_far<false, true, true, 1>));
+  else FARWS_ARGS((k_far_ws<true, 1>));
   swr_kt_end(S, KT_FAR);
   if (mode != 0) {
     swr_kt_begin(S, KT_BONDS);
@@ -558,6 +799,7 @@ int swr_lists_run(swr_system *S, int mode) {
     swr_kt_end(S, KT_BONDS);
   }
 #undef FAR_ARGS
+#undef FARWS_ARGS
 #undef BOND_ARGS
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
@@ -572,9 +814,9 @@ int swr_lists_estimate(swr_system *S) {
   return SWR_OK;
 }
 
-// Validate_Lists + sort_bonds.  bond_mark sweeps are launched in batches
-// sized from the previous call's depth; the host checks convergence after
-// each batch (one small read-back per call) and continues if needed.
+// Validate_Lists: next capacities and overflow flags; the status block
+// (with the capacity totals Reset needs next step) is copied back
+// asynchronously and read at the host's next sync.
 int swr_lists_validate(swr_system *S) {
   const int n = S->n, N = S->N;
   swr_kt_begin(S, KT_VALIDATE);
