@@ -252,17 +252,21 @@ def bench_serve(args):
     warm = serve.make_requests(24, rate=1e4, seed=args.seed + 1, hi=3000)
     serve.Server(lanes=args.lanes, max_batch_atoms=args.max_batch_atoms, batches=lanes).run(warm)
     runs = []
+    base = serve.make_requests(args.requests, rate=1.0, seed=args.seed)   # systems generated once
     for rate in args.rates:
-        reqs = serve.make_requests(args.requests, rate=rate, seed=args.seed)
+        reqs = serve.at_rate(base, rate)
         t0 = time.perf_counter()
-        serve.Server(lanes=args.lanes, max_batch_atoms=args.max_batch_atoms, batches=lanes).run(reqs)
+        srv = serve.Server(lanes=args.lanes, max_batch_atoms=args.max_batch_atoms, batches=lanes)
+        srv.run(reqs)
         wall = time.perf_counter() - t0
         summ = serve.summary(reqs, wall)
         summ["rate"] = rate
+        summ["lane_busy_s"] = {k: round(v, 3) for k, v in srv.busy.items()}
+        summ["wall_s"] = round(wall, 3)
         runs.append(summ)
         log(f"rate {rate:.0f}/s: {summ}")
     # plain loop: one worker, batches of one, every request queued at t = 0
-    reqs = serve.make_requests(args.plain_sample, rate=1e9, seed=args.seed)
+    reqs = serve.at_rate(base[:args.plain_sample], 1e9)
     t0 = time.perf_counter()
     serve.Server(lanes=1, max_batch_atoms=1, batches=[lanes[0]]).run(reqs)
     wall = time.perf_counter() - t0
@@ -301,6 +305,7 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=128,
                     help="clusters timed one at a time for the plain-loop comparison")
     ap.add_argument("--mixed", action="store_true", help="mixed-precision QEq")
+    ap.add_argument("--tol", type=float, default=1e-6, help="QEq tolerance (the reference's: 1e-6)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--every", type=int, default=10)
     ap.add_argument("--dt", type=float, default=0.25)
@@ -332,7 +337,7 @@ def main():
     s = systems.water_box(cfg["nmol"], seed=args.seed)
     lmp = Lammps()
     lmp.commands(standard_script(s.elements, ffield=DEFAULT_FFIELD, every=args.every, dt=args.dt,
-                                 mixed=args.mixed))
+                                 mixed=args.mixed, tol=args.tol))
     lmp.create_atoms(s)
     log("setup (first neighbour build, capacity estimate, QEq from zero history)")
     lmp.setup()
@@ -405,7 +410,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if not args.mixed else "f64+f32", "data": "synthetic",
         "config": {"workload": cfg["label"] + f", NVE 300 K dt {args.dt} fs, rebuild every {args.every} steps, "
-                               f"skin 2 A, QEq tol 1e-6" + (", mixed-precision QEq" if args.mixed else ""),
+                               f"skin 2 A, QEq tol {args.tol:g}" + (", mixed-precision QEq" if args.mixed else ""),
                    "atoms": n, "atoms_with_ghosts": c["N"], "seed": args.seed, "parallelism": "single GPU"},
         "atom_steps_per_s": round(n / (ms_step * 1e-3), 1),
         "cg_iters_per_step": [round(its[0], 2), round(its[1], 2)],

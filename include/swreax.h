@@ -190,6 +190,26 @@ int swr_pair_compute(swr_system *sys);
  * mixed-precision solve takes one system at a time. */
 int swr_set_batch(swr_system *sys, int nsys, const int *row_offsets);
 
+/* Host-side packing of nsys independent systems into one batch (no
+ * reference counterpart; host code, no device calls).  System k has nat[k]
+ * local atoms, listed in order in x (3 per atom), type (1-based, already in
+ * the batch's element numbering) and tag; its box is lo/hi[3k..3k+2].  A
+ * periodic system gets ghost images within cutghost in LAMMPS
+ * Comm::borders order (per dimension, atoms with x <= lo+cut imaged at +L,
+ * then x >= hi-cut at -L, over the atoms present when the dimension starts).
+ * Each system is then translated into its own cell of an m x m x m grid
+ * (m = ceil(nsys^(1/3)), cell edge = the largest extent + cutghost + 1), so
+ * no atom lies within cutghost of another system's.  Outputs: local atoms of
+ * all systems first (rows row_off[k]..row_off[k+1]), then their ghosts
+ * (n + ghost_off[k]...); tag of system k offset by row_off[k]; owner = the
+ * local row a ghost copies; img = its periodic image; shift[3k..] = the
+ * translation applied.  *N_out = n + ghosts.  With x_out == NULL only
+ * *N_out is computed; otherwise cap must be >= that N. */
+int swr_batch_pack(int nsys, const int *nat, const double *x, const int *type, const int *tag,
+                   const double *lo, const double *hi, const int *periodic, double cutghost, long long cap,
+                   double *x_out, int *type_out, int *tag_out, int *owner_out, signed char *img_out,
+                   int *row_off, int *ghost_off, double *shift, long long *N_out);
+
 /* Join the stream; returns the first device-raised error. */
 int swr_sync(swr_system *sys);
 /* hipDeviceSynchronize on the current device (a barrier for callers that
@@ -236,6 +256,10 @@ int         swr_lmp_command(swr_lmp *lmp, const char *cmd);
 int         swr_lmp_create_atoms(swr_lmp *lmp, int n, const double *lo, const double *hi,
                                  int periodic, const double *x, const double *v,
                                  const int *type, const int *tag);
+/* Optional, before swr_lmp_setup: QEq s/t histories (n x 5 each, atom-major,
+ * the layout of fix qeq/reax's s_hist/t_hist) to warm-start the setup solve,
+ * as a restart file's fix state would. */
+int         swr_lmp_set_history(swr_lmp *lmp, int n, const double *s_hist, const double *t_hist);
 int         swr_lmp_setup(swr_lmp *lmp);
 int         swr_lmp_run(swr_lmp *lmp, int nsteps);
 swr_system *swr_lmp_system(swr_lmp *lmp);

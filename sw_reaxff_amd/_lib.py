@@ -25,10 +25,11 @@ EXPORTS = [
     "swr_init_forces_noqeq_full", "swr_write_reax_lists_init_forces", "swr_validate_lists", "swr_bo",
     "swr_qeq_init_matvec", "swr_sparse_matvec", "swr_qeq_cg", "swr_calculate_q", "swr_qeq_pre_force",
     "swr_pair_compute", "swr_sync", "swr_get", "swr_put", "swr_get_scalar", "swr_device_ptr",
-    "swr_lmp_open", "swr_lmp_close", "swr_lmp_command", "swr_lmp_create_atoms", "swr_lmp_setup", "swr_lmp_run",
+    "swr_lmp_open", "swr_lmp_close", "swr_lmp_command", "swr_lmp_create_atoms", "swr_lmp_set_history",
+    "swr_lmp_setup", "swr_lmp_run",
     "swr_lmp_system", "swr_lmp_stat", "swr_ktimer_enable", "swr_ktimer_read", "swr_device_synchronize",
     "swr_sparse_matvec_h", "swr_npair_full_bin_atomonly_build", "swr_sort_bonds", "swr_set_batch",
-    "swr_new_atoms",
+    "swr_new_atoms", "swr_batch_pack",
 ]
 
 
@@ -87,6 +88,8 @@ def lib():
         L.swr_lmp_close.argtypes = [vp]
         L.swr_lmp_command.argtypes = [vp, C.c_char_p]
         L.swr_lmp_create_atoms.argtypes = [vp, C.c_int, dp, dp, C.c_int, dp, dp, ip, ip]
+        L.swr_lmp_set_history.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p]
+        L.swr_batch_pack.argtypes = [C.c_int] + [C.c_void_p] * 7 + [C.c_double, C.c_longlong] + [C.c_void_p] * 9
         L.swr_lmp_setup.argtypes = [vp]
         L.swr_lmp_run.argtypes = [vp, C.c_int]
         L.swr_lmp_system.restype = vp
@@ -174,9 +177,10 @@ class System:
         gg = np.ascontiguousarray(tag, dtype=np.int32)
         oo = np.ascontiguousarray(owner if owner is not None else np.arange(N), dtype=np.int32)
         im = np.ascontiguousarray(img if img is not None else np.zeros((N, 3)), dtype=np.int8)
-        qq = np.ascontiguousarray(q if q is not None else np.zeros(N), dtype=np.float64)
+        qq = np.ascontiguousarray(q, dtype=np.float64) if q is not None else None   # None: zero charges
         check(self.L.swr_set_atoms(self.h, int(n), N, ptr(gg, C.c_int), ptr(tt, C.c_int), ptr(xx, C.c_double),
-                                   ptr(qq, C.c_double), ptr(oo, C.c_int), ptr(im, C.c_int8)), "swr_set_atoms")
+                                   ptr(qq, C.c_double) if qq is not None else None, ptr(oo, C.c_int),
+                                   ptr(im, C.c_int8)), "swr_set_atoms")
 
     def set_capacities(self, num_bonds, num_hbonds):
         a = np.ascontiguousarray(num_bonds, dtype=np.int32)

@@ -27,7 +27,12 @@ import numpy as np
 from . import _lib, systems
 from .lammps import DEFAULT_FFIELD
 
-CUTGHOST = 12.0          # max(nonb 10, hbond 7.5, 2 x bond 5) + 2 A skin
+# Ghost shell and Verlet cutoff of a one-shot evaluation: max(nonb 10,
+# hbond 7.5, 2 x bond 5).  The lists are built fresh for every evaluation,
+# so no skin is needed (an MD run adds one so lists survive several steps);
+# every result of a local atom (its far/bond/hbond lists, H row, corrected
+# bond orders, which reach atoms within 2 x bond cutoff) lies inside it.
+CUTGHOST = 10.0
 ELEMENTS = ["H", "O", "C", "N"]
 
 
@@ -44,60 +49,56 @@ class Packed:
     shift: np.ndarray        # [K,3] translation applied to each system
 
 
-def ghosts_of(s: systems.System):
-    """Local + ghost coordinates, owner and periodic image of one system."""
-    if not s.periodic:
-        n = s.n
-        return s.x.copy(), np.arange(n, dtype=np.int32), np.zeros((n, 3), np.int8)
-    xa, owner = systems.borders(s.x, s.lo, s.hi, CUTGHOST)
-    L = s.hi - s.lo
-    img = np.rint((xa - s.x[owner]) / L).astype(np.int8)
-    return xa, owner.astype(np.int32), img
+def _buffer(bufs, name, shape, dtype):
+    """A reusable output array of at least `shape` (first axis grows)."""
+    if bufs is None:
+        return np.empty(shape, dtype)
+    b = bufs.get(name)
+    if b is None or b.shape[0] < shape[0]:
+        b = np.empty((max(shape[0], int(shape[0] * 1.25) + 16),) + tuple(shape[1:]), dtype)
+        bufs[name] = b
+    return b[:shape[0]]
 
 
-def pack(batch: list[systems.System], elements=ELEMENTS) -> Packed:
-    """Places the systems in disjoint regions of one box (see module doc)."""
+def pack(batch: list[systems.System], elements=ELEMENTS, cutghost: float = CUTGHOST, bufs=None) -> Packed:
+    """Places the systems in disjoint regions of one box (see module doc);
+    ghosts, translation and numbering by swr_batch_pack (host C++).  `bufs`
+    (a dict kept by the caller) lets successive packs reuse their output
+    arrays; the returned arrays are then views into them."""
+    L = _lib.lib()
     K = len(batch)
-    parts = [ghosts_of(s) for s in batch]
-    ext = []
-    for s, (xa, _, _) in zip(batch, parts):
-        lo = np.minimum(xa.min(0), s.lo) if s.n else s.lo
-        hi = np.maximum(xa.max(0), s.hi) if s.n else s.hi
-        ext.append((lo, hi))
-    # uniform cubic cells on an m x m x m grid, each a system's extent plus the cutoff gap
-    size = max(float(np.max(hi - lo)) for lo, hi in ext) + CUTGHOST + 1.0
-    m = max(1, int(np.ceil(K ** (1.0 / 3.0) - 1e-9)))
-    shifts = np.zeros((K, 3))
-    for k, (lo, hi) in enumerate(ext):
-        cell = np.array([k % m, (k // m) % m, k // (m * m)], dtype=np.float64)
-        shifts[k] = cell * size - lo
-    n = sum(s.n for s in batch)
-    row_off = np.zeros(K + 1, np.int32)
-    row_off[1:] = np.cumsum([s.n for s in batch])
-    ng = [xa.shape[0] - s.n for s, (xa, _, _) in zip(batch, parts)]
-    ghost_off = np.zeros(K + 1, np.int32)
-    ghost_off[1:] = np.cumsum(ng)
-    N = n + int(ghost_off[-1])
-    x = np.empty((N, 3))
-    typ = np.empty(N, np.int32)
-    tag = np.empty(N, np.int32)
-    owner = np.empty(N, np.int32)
-    img = np.zeros((N, 3), np.int8)
-    for k, (s, (xa, own, im)) in enumerate(zip(batch, parts)):
+    nat = np.array([s.n for s in batch], dtype=np.int32)
+    n = int(nat.sum())
+    x = np.ascontiguousarray(np.concatenate([s.x for s in batch]) if n else np.zeros((0, 3)), dtype=np.float64)
+    typ = np.empty(n, np.int32)
+    tag = np.empty(n, np.int32)
+    a = 0
+    for s in batch:
         tmap = np.array([elements.index(e) + 1 for e in s.elements], dtype=np.int32)
-        a, b = row_off[k], row_off[k + 1]
-        g0, g1 = n + ghost_off[k], n + ghost_off[k + 1]
-        t_all = tmap[s.type[own] - 1]
-        x[a:b] = xa[:s.n] + shifts[k]
-        x[g0:g1] = xa[s.n:] + shifts[k]
-        typ[a:b], typ[g0:g1] = t_all[:s.n], t_all[s.n:]
-        tag[a:b] = s.tag + a
-        tag[g0:g1] = s.tag[own[s.n:]] + a
-        owner[a:b] = np.arange(a, b)
-        owner[g0:g1] = own[s.n:] + a
-        img[g0:g1] = im[s.n:]
-    return Packed(x=x, type=typ, tag=tag, owner=owner, img=img, n=n, row_off=row_off, ghost_off=ghost_off,
-                  shift=shifts)
+        typ[a:a + s.n] = tmap[s.type - 1]
+        tag[a:a + s.n] = s.tag
+        a += s.n
+    lo = np.ascontiguousarray([s.lo for s in batch], dtype=np.float64)
+    hi = np.ascontiguousarray([s.hi for s in batch], dtype=np.float64)
+    per = np.array([1 if s.periodic else 0 for s in batch], dtype=np.int32)
+    N = C.c_longlong(0)
+    args = (K, nat.ctypes.data, x.ctypes.data, typ.ctypes.data, tag.ctypes.data, lo.ctypes.data, hi.ctypes.data,
+            per.ctypes.data, C.c_double(cutghost))
+    _lib.check(L.swr_batch_pack(*args, 0, None, None, None, None, None, None, None, None, C.byref(N)),
+               "swr_batch_pack")
+    Nv = N.value
+    xo = _buffer(bufs, "x", (Nv, 3), np.float64)
+    to = _buffer(bufs, "type", (Nv,), np.int32)
+    go = _buffer(bufs, "tag", (Nv,), np.int32)
+    oo = _buffer(bufs, "owner", (Nv,), np.int32)
+    io = _buffer(bufs, "img", (Nv, 3), np.int8)
+    row_off = np.empty(K + 1, np.int32)
+    ghost_off = np.empty(K + 1, np.int32)
+    shift = np.empty((K, 3), np.float64)
+    _lib.check(L.swr_batch_pack(*args, Nv, xo.ctypes.data, to.ctypes.data, go.ctypes.data, oo.ctypes.data,
+                                io.ctypes.data, row_off.ctypes.data, ghost_off.ctypes.data, shift.ctypes.data,
+                                C.byref(N)), "swr_batch_pack")
+    return Packed(x=xo, type=to, tag=go, owner=oo, img=io, n=n, row_off=row_off, ghost_off=ghost_off, shift=shift)
 
 
 class Batch:
@@ -107,10 +108,11 @@ class Batch:
         self.elements = list(elements)
         self.sys = _lib.System(ffield, self.elements, tol=tol, qeq_mixed=1 if mixed else 0)
         self.packed = None
+        self._bufs = {}
 
     def load(self, batch: list[systems.System], q=None):
         """New systems: QEq starts from zero history and list capacities are estimated afresh."""
-        P = pack(batch, self.elements)
+        P = pack(batch, self.elements, bufs=self._bufs)
         h = self.sys
         h.set_atoms(P.n, P.x, P.type, P.tag, owner=P.owner, img=P.img, q=q)
         _lib.check(h.L.swr_new_atoms(h.h), "swr_new_atoms")
@@ -144,3 +146,12 @@ class Batch:
     def iterations(self):
         it = self.sys.get("sys_iters")
         return it.reshape(-1, 2)
+
+    def histories(self):
+        """Per system, the QEq s/t histories (n_k x 5 each) the solve left:
+        the warm start of a follow-on evaluation of the same atoms."""
+        P = self.packed
+        sh = self.sys.get("s_hist").reshape(-1, 5)
+        th = self.sys.get("t_hist").reshape(-1, 5)
+        return [(sh[P.row_off[k]:P.row_off[k + 1]].copy(), th[P.row_off[k]:P.row_off[k + 1]].copy())
+                for k in range(len(P.row_off) - 1)]

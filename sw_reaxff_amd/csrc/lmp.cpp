@@ -132,6 +132,7 @@ struct swr_lmp {
   int periodic = 1;
   int n = 0;
   std::vector<double> x0, v0;
+  std::vector<double> s_hist0, t_hist0;  // optional warm start for setup (n x 5 each)
   std::vector<int> type0, tag0;
   swr_system *sys = nullptr;
   bool is_setup = false;
@@ -215,6 +216,11 @@ struct swr_lmp {
       if (t < 1 || t > (int)el.size()) { swr_set_error_msg("atom type without pair_coeff element"); return SWR_ERR_ARG; }
     if ((rc = swr_set_atoms(sys, n, n, tag0.data(), type0.data(), x0.data(), nullptr, nullptr, nullptr))) return rc;
     HIP_CHECK(hipMemcpy(sys->v, v0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    if (!s_hist0.empty()) {
+      if ((rc = swr_put(sys, "s_hist", s_hist0.data(), (int64_t)(sizeof(double) * s_hist0.size()))) ||
+          (rc = swr_put(sys, "t_hist", t_hist0.data(), (int64_t)(sizeof(double) * t_hist0.size()))))
+        return rc;
+    }
     ev.resize(16);
     for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
     step = 0;
@@ -331,6 +337,16 @@ int swr_lmp_create_atoms(swr_lmp *L, int n, const double *lo, const double *hi, 
   if (v) L->v0.assign(v, v + 3 * (size_t)n);
   L->type0.assign(type, type + n);
   L->tag0.assign(tag, tag + n);
+  return SWR_OK;
+}
+
+int swr_lmp_set_history(swr_lmp *L, int n, const double *s_hist, const double *t_hist) {
+  if (!L || !s_hist || !t_hist || n != L->n || L->is_setup) {
+    swr_set_error_msg("swr_lmp_set_history: needs the created atoms' count, before setup");
+    return SWR_ERR_ARG;
+  }
+  L->s_hist0.assign(s_hist, s_hist + 5 * (size_t)n);
+  L->t_hist0.assign(t_hist, t_hist + 5 * (size_t)n);
   return SWR_OK;
 }
 

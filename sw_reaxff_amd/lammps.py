@@ -67,6 +67,13 @@ class Lammps:
                                           1 if s.periodic else 0, ptr(x, C.c_double), ptr(v, C.c_double),
                                           ptr(t, C.c_int), ptr(g, C.c_int)), "create_atoms")
 
+    def set_history(self, s_hist, t_hist):
+        """QEq s/t histories (n x 5 each) that warm-start the setup solve."""
+        sh = np.ascontiguousarray(s_hist, dtype=np.float64)
+        th = np.ascontiguousarray(t_hist, dtype=np.float64)
+        assert sh.shape == th.shape and sh.ndim == 2 and sh.shape[1] == 5
+        check(self.L.swr_lmp_set_history(self.h, sh.shape[0], sh.ctypes.data, th.ctypes.data), "set_history")
+
     def setup(self):
         check(self.L.swr_lmp_setup(self.h), "setup")
 

@@ -12,7 +12,10 @@ follow-on MD step of a resident system.  The server:
     the device (ctypes releases the GIL during library calls);
   * keeps resident systems on the device between their steps (positions,
     velocities, Verlet list, s/t warm-start history in their own device
-    system); a follow-on request advances one MD step there;
+    system); a follow-on request advances one MD step there.  A new system
+    that is to stay resident is evaluated in its batch like any other; its
+    device system is set up when its first follow-on step arrives, warm-
+    started from the s/t histories its batched solve left;
   * admits work by estimated HBM footprint against a budget (bytes per atom
     dominated by the Verlet and far lists).
 
@@ -52,6 +55,7 @@ class Request:
     lane: int = -1
     batch_size: int = 0
     step_index: int = 0             # kind "step": which step of its resident system this was
+    hist: tuple | None = None       # resident's first evaluation: the s/t histories it left
 
     @property
     def latency(self):
@@ -98,23 +102,44 @@ def make_requests(count: int, rate: float, seed: int, lo: int = 20, hi: int = 20
     return reqs
 
 
-class _Resident:
-    """A system kept on the device between its MD steps."""
+def at_rate(reqs: list[Request], rate: float, unit_rate: float = 1.0) -> list[Request]:
+    """Fresh copies of requests generated at `unit_rate`, arriving at `rate`
+    (same systems and kinds, arrival times scaled)."""
+    f = unit_rate / rate
+    return [Request(rid=r.rid, arrival=r.arrival * f, kind=r.kind, system=r.system, resident=r.resident)
+            for r in reqs]
 
-    def __init__(self, s: systems.System, ffield, tol, mixed):
+
+class _Resident:
+    """A system kept on the device between its MD steps.  Its device system
+    is created and set up on first use (first() or the first step()),
+    warm-started from `hist` (the s/t histories of its first evaluation)
+    when given."""
+
+    def __init__(self, s: systems.System, ffield, tol, mixed, hist=None):
         self.lock = threading.Lock()
-        self.lmp = Lammps()
-        self.lmp.commands(standard_script(s.elements, ffield=ffield, tol=tol, mixed=mixed))
-        self.lmp.create_atoms(s)
+        self.s, self.ffield, self.tol, self.mixed, self.hist = s, ffield, tol, mixed, hist
+        self.lmp = None
         self.n = s.n
         self.bytes = footprint(s)
         self.steps = 0
 
+    def _ensure(self):
+        if self.lmp is None:
+            lmp = Lammps()
+            lmp.commands(standard_script(self.s.elements, ffield=self.ffield, tol=self.tol, mixed=self.mixed))
+            lmp.create_atoms(self.s)
+            if self.hist is not None:
+                lmp.set_history(*self.hist)
+            lmp.setup()
+            self.lmp = lmp
+
     def first(self):
-        self.lmp.setup()
+        self._ensure()
         return self.q()
 
     def step(self):
+        self._ensure()
         self.lmp.run(1)
         self.steps += 1
         return self.q()
@@ -139,6 +164,9 @@ class Server:
         self.closed = False
         self.t0 = 0.0
         self.errors: list[BaseException] = []
+        # seconds the workers spent per phase (summed over lanes)
+        self.busy = {"pack_load": 0.0, "evaluate": 0.0, "results": 0.0, "resident_steps": 0.0}
+        self.nsteps_first = 0
 
     # ---- admission
     def _take(self, lane):
@@ -189,42 +217,51 @@ class Server:
                     return
                 if kind == "step":
                     for r in reqs:
+                        t = time.perf_counter()
                         res = self.residents[r.resident]
                         with res.lock:
+                            if res.lmp is None:
+                                self.nsteps_first += 1
                             r.q = res.step()
                             r.step_index = res.steps
                         self._finish(r, lane, 1)
+                        self._account("resident_steps", t)
                     continue
-                # new systems: residents are set up in their own device system,
-                # the rest are evaluated together as one batch
-                batch = []
-                for r in reqs:
+                # new systems, residents included, are evaluated together as
+                # one batch; a resident keeps the s/t histories of its solve
+                t = time.perf_counter()
+                bt.load([r.system for r in reqs])
+                t = self._account("pack_load", t)
+                bt.evaluate()
+                bt.sync()
+                t = self._account("evaluate", t)
+                qs = bt.charges()
+                its = bt.iterations()
+                hists = bt.histories() if any(r.resident >= 0 for r in reqs) else None
+                for k, (r, q, it) in enumerate(zip(reqs, qs, its)):
+                    r.q = q.copy()
+                    r.iters = tuple(int(v) for v in it)
                     if r.resident >= 0:
-                        res = _Resident(r.system, self.ffield, self.tol, self.mixed)
-                        with res.lock:
-                            r.q = res.first()
+                        r.hist = hists[k]
+                        res = _Resident(r.system, self.ffield, self.tol, self.mixed, hist=r.hist)
                         with self.cv:
                             self.residents[r.resident] = res
                             self.in_use += res.bytes
-                        self._finish(r, lane, 1)
-                    else:
-                        batch.append(r)
-                if batch:
-                    bt.load([r.system for r in batch])
-                    bt.evaluate()
-                    qs = bt.charges()
-                    its = bt.iterations()
-                    for r, q, it in zip(batch, qs, its):
-                        r.q = q.copy()
-                        r.iters = tuple(int(v) for v in it)
-                    for r in batch:
-                        self._finish(r, lane, len(batch))
+                for r in reqs:
+                    self._finish(r, lane, len(reqs))
+                self._account("results", t)
                 self._release(nbytes)
         except BaseException as e:  # surface worker failures to run()
             self.errors.append(e)
             with self.cv:
                 self.closed = True
                 self.cv.notify_all()
+
+    def _account(self, phase, t):
+        now = time.perf_counter()
+        with self.cv:
+            self.busy[phase] += now - t
+        return now
 
     def _finish(self, r, lane, bsize):
         r.done = time.perf_counter() - self.t0

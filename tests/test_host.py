@@ -187,3 +187,71 @@ def test_setup_without_atoms_is_an_error(lmp):
     lmp.commands(standard_script(["H", "O"]))
     with pytest.raises(SwrError):
         lmp.setup()
+
+
+# ------------------------------------------------------------------ batch packing (host C++)
+def _pack_mix():
+    return [systems.cluster(60, seed=5), systems.water_box(64, seed=6),      # L ~ 12.4 A < 2 x cutghost
+            systems.water_box(300, seed=7), systems.cluster(400, seed=8), systems.water_box(100, seed=9)]
+
+
+@pytest.mark.parametrize("cut", [10.0, 12.0])
+def test_batch_pack_equals_numpy_restatement(built_lib, cut):
+    from sw_reaxff_amd import batch as B
+    from tests import batch_pack_ref as R
+    parts = _pack_mix()
+    P1 = R.pack(parts, B.ELEMENTS, cut)
+    bufs = {}
+    for _ in range(2):                                          # second pass reuses the buffers
+        P2 = B.pack(parts, cutghost=cut, bufs=bufs)
+        assert P1.n == P2.n
+        for f in ("x", "type", "tag", "owner", "img", "row_off", "ghost_off", "shift"):
+            assert np.array_equal(getattr(P1, f), getattr(P2, f)), f
+
+
+def test_batch_pack_separates_systems(built_lib):
+    from sw_reaxff_amd import batch as B
+    parts = _pack_mix()
+    P = B.pack(parts, cutghost=10.0)
+    # system id of every packed atom (locals, then ghosts)
+    sid = np.empty(P.x.shape[0], np.int64)
+    for k in range(len(parts)):
+        sid[P.row_off[k]:P.row_off[k + 1]] = k
+        sid[P.n + P.ghost_off[k]:P.n + P.ghost_off[k + 1]] = k
+    assert np.array_equal(sid[P.owner], sid)                    # ghosts copy their own system's atoms
+    for k in range(len(parts)):
+        a = P.x[sid == k]
+        b = P.x[sid != k]
+        d2 = ((a[:, None, :] - b[None, :, :]) ** 2).sum(-1)
+        assert d2.min() > 10.0 ** 2, k
+    # a ghost sits at its owner plus its periodic image
+    g = np.arange(P.n, P.x.shape[0])
+    L = np.array([parts[k].hi - parts[k].lo for k in sid[g]])
+    assert np.allclose(P.x[g], P.x[P.owner[g]] + P.img[g] * L, atol=1e-9)
+
+
+def test_batch_pack_rejects_bad_input(built_lib):
+    import ctypes as C
+    from sw_reaxff_amd import _lib
+    L = _lib.lib()
+    nat = np.array([3], np.int32)
+    x = np.zeros((3, 3))
+    t = np.ones(3, np.int32)
+    lo = np.zeros(3)
+    hi = np.zeros(3)                                             # empty periodic box
+    per = np.ones(1, np.int32)
+    N = C.c_longlong(0)
+    rc = L.swr_batch_pack(1, nat.ctypes.data, x.ctypes.data, t.ctypes.data, t.ctypes.data, lo.ctypes.data,
+                          hi.ctypes.data, per.ctypes.data, C.c_double(10.0), 0, None, None, None, None, None,
+                          None, None, None, C.byref(N))
+    assert rc != 0
+    hi = np.full(3, 20.0)
+    rc = L.swr_batch_pack(1, nat.ctypes.data, x.ctypes.data, t.ctypes.data, t.ctypes.data, lo.ctypes.data,
+                          hi.ctypes.data, per.ctypes.data, C.c_double(10.0), 0, None, None, None, None, None,
+                          None, None, None, C.byref(N))
+    assert rc == 0 and N.value == 3 * 8                          # the corner atom: 2 x 2 x 2 images
+    out = np.empty((N.value, 3))
+    rc = L.swr_batch_pack(1, nat.ctypes.data, x.ctypes.data, t.ctypes.data, t.ctypes.data, lo.ctypes.data,
+                          hi.ctypes.data, per.ctypes.data, C.c_double(10.0), N.value - 1, out.ctypes.data,
+                          None, None, None, None, None, None, None, C.byref(N))
+    assert rc != 0                                               # capacity below N
