@@ -722,6 +722,10 @@ int64_t swr_get_scalar(swr_system *S, const char *name) {
     return m;
   }
   if (k == "nsys") return S->nsys;
+  if (k == "mixed_outer") return S->mixed_outer;
+  // first outer residual of the last mixed solve, sqrt(u.r)/|b| x 1e15
+  if (k == "mixed_rho0_s_x1e15") return (int64_t)(S->mixed_rho0[0] * 1e15);
+  if (k == "mixed_rho0_t_x1e15") return (int64_t)(S->mixed_rho0[1] * 1e15);
   return -1;
 }
 

@@ -207,6 +207,8 @@ int swr_qeq_mixed_solve(swr_system *S) {
     MixState h;
     HIP_CHECK(hipMemcpyAsync(&h, ms, sizeof(h), hipMemcpyDeviceToHost, S->stream));
     HIP_CHECK(hipStreamSynchronize(S->stream));
+    S->mixed_outer = h.outer;
+    if (outer == 0) { S->mixed_rho0[0] = h.rho[0]; S->mixed_rho0[1] = h.rho[1]; }
     if (h.done) break;
     if ((rc = cg_solve(S, R, 4, 1))) return rc;
     hipLaunchKernelGGL(k_mixed_update, dim3(div_up(n, 256)), dim3(256), 0, S->stream, n, x,

@@ -212,6 +212,8 @@ struct swr_system {
   int mixed = 0;                   // mixed-precision QEq
   void *mixw = nullptr;            // mixed-precision workspace
   size_t mixw_bytes = 0;
+  int mixed_outer = 0;             // residual evaluations of the last mixed solve
+  double mixed_rho0[2] = {0, 0};   // its first outer residual sqrt(u.r)/|b| (s, t)
   int imax = 200;
 
   // ---- status
