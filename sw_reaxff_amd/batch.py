@@ -148,10 +148,15 @@ class Batch:
         return it.reshape(-1, 2)
 
     def histories(self):
-        """Per system, the QEq s/t histories (n_k x 5 each) the solve left:
-        the warm start of a follow-on evaluation of the same atoms."""
+        """Per system, warm-start s/t histories (n_k x 5 each) for a
+        follow-on evaluation of the same atoms: every slot holds the solution
+        this solve reached, so fix qeq/reax's extrapolation (cubic in s,
+        quadratic in t; fix_qeq_reax_sunway.cpp:711-714) returns it exactly.
+        (The raw history after one solve from zero holds it in slot 0 only,
+        which those formulas would extrapolate to 4 s and 3 t.)"""
         P = self.packed
-        sh = self.sys.get("s_hist").reshape(-1, 5)
-        th = self.sys.get("t_hist").reshape(-1, 5)
-        return [(sh[P.row_off[k]:P.row_off[k + 1]].copy(), th[P.row_off[k]:P.row_off[k + 1]].copy())
+        sh = self.sys.get("s_hist").reshape(-1, 5)[:, :1]
+        th = self.sys.get("t_hist").reshape(-1, 5)[:, :1]
+        return [(np.repeat(sh[P.row_off[k]:P.row_off[k + 1]], 5, axis=1),
+                 np.repeat(th[P.row_off[k]:P.row_off[k + 1]], 5, axis=1))
                 for k in range(len(P.row_off) - 1)]

@@ -171,7 +171,9 @@ __device__ __forceinline__ T2 mk2(T a, T b) {
 // scalars; c points at the system's s and t scalars).
 __device__ __forceinline__ bool cg_any_active(int PH, const CgScalars *__restrict__ c) {
   if (PH != PH_A1 && PH != PH_K) return true;
-  return (PH == PH_K && (c[0].active_b || c[1].active_b)) || c[0].active_a || c[1].active_a;
+  // all four flags requested at once (no short-circuit chain of loads)
+  const int ab = c[0].active_b | c[1].active_b, aa = c[0].active_a | c[1].active_a;
+  return ((PH == PH_K ? ab : 0) | aa) != 0;
 }
 
 // q = eta d + H d (sparse_matvec_C_para, fix_qeq_reax_sw64.c:424-466) of the
@@ -183,16 +185,19 @@ k_cg_spmv(int n, const DevParams *__restrict__ P, const int *__restrict__ type, 
           const int *__restrict__ numnbrs, const int *__restrict__ jlist, const VT *__restrict__ val, CgVecsT<T> V,
           int par_in, const CgScalars *__restrict__ cgs, const int *__restrict__ row_sys) {
   typedef typename Vec2<T>::type T2;
-  // one system: the whole launch is skipped once both solves have finished
-  // (uniform scalars, no per-row lookup before the gathers start)
-  if (!BATCH && !cg_any_active(PH, cgs)) return;
   const int w = wave_in_block(), lane = lane_id();
   const int i = blockIdx.x * RPB + w * 2 + (lane >> 5);
+  // The row's extent is requested together with the solve state, so the
+  // two latencies overlap (the state decides whether the row runs).
+  const int beg0 = i < n ? firstnbr[i] : 0, cnt0 = i < n ? numnbrs[i] : 0;
+  const int rs = BATCH && i < n ? row_sys[i] : 0;
+  // one system: the whole launch is skipped once both solves have finished;
   // a batch: rows of systems whose solves have finished are skipped (a
   // half-wave decides for its own row)
-  const bool live = i < n && (!BATCH || cg_any_active(PH, cgs + 2 * row_sys[i]));
+  if (!BATCH && !cg_any_active(PH, cgs) && cnt0 >= 0) return;
+  const bool live = i < n && (!BATCH || cg_any_active(PH, cgs + 2 * rs));
   const T2 *src = PH == PH_S1 ? V.x : PH == PH_S2 ? V.d[0] : PH == PH_S3 ? V.d[1] : (par_in ? V.d[1] : V.d[0]);
-  const int beg = live ? firstnbr[i] : 0, cnt = live ? numnbrs[i] : 0;
+  const int beg = live ? beg0 : 0, cnt = live ? cnt0 : 0;
   T2 q = row_spmv2_h<T, VT>(beg, cnt, jlist, val, src);
   if ((lane & 31) == 0 && live) {
     const T eta = (T)P->eta[type[i]];

@@ -10,7 +10,7 @@
 //   stop when sqrt(u.r)/|b| <= tol/100, u = M^-1 r   (the reference's test, FP64)
 //   solve H d = r in FP32 with the same pipelined CG (qeq_cg.h, FP64 dots),
 //     from d = 0, to the relative tolerance that brings the outer test
-//     below its target in one pass, clamped to [3e-4, 0.5] (the FP32
+//     below its target in one pass (0.9 of it), clamped to [3e-4, 0.5] (the FP32
 //     pipelined recurrences do not reliably reach smaller ones), at most
 //     50 iterations
 //   x += d                     (FP64, k_mixed_update)
@@ -111,9 +111,11 @@ k_resid_scalars(int nblk, const double *__restrict__ partials, int pstride, MixS
       if (isnan(rho)) atomicOr(&st->flags, SWR_F_NAN);
       continue;
     }
-    // sqrt(u.r) of the corrected x ~ sqrt(u_in.r_in) <= tol_in |r| must be
-    // <= tol |b|: tol_in = 0.5 tol |b| / |r|
-    const double tin = 0.5 * tol * bn / fmax(rn, 1e-300);
+    // the corrected x's residual is the inner one, and the inner test is
+    // sqrt(u_in.r_in) <= tol_in |r| on the same preconditioned norm: it
+    // meets the outer test sqrt(u.r) <= tol |b| at tol_in = tol |b| / |r|
+    // (0.9 of it, for the FP32 rounding of d)
+    const double tin = 0.9 * tol * bn / fmax(rn, 1e-300);
     inner[s].tol = fmin(0.5, fmax(INNER_TOL_MIN, tin));
   }
   m.done = (m.converged[0] && m.converged[1]) ? 1 : 0;
