@@ -300,7 +300,7 @@ def main():
     ap.add_argument("--requests", type=int, default=400, help="serve: requests in the stream")
     ap.add_argument("--rates", type=float, nargs="+", default=[500, 700, 900, 1000, 1100, 1200, 1400, 2000],
                     help="serve: Poisson arrival rates (1/s) to sweep")
-    ap.add_argument("--lanes", type=int, default=3, help="serve: concurrent batch workers")
+    ap.add_argument("--lanes", type=int, default=4, help="serve: concurrent batch workers")
     ap.add_argument("--max-batch-atoms", type=int, default=200_000, help="serve: atoms per batch at most")
     ap.add_argument("--plain-sample", type=int, default=128,
                     help="clusters timed one at a time for the plain-loop comparison")
