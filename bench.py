@@ -58,6 +58,7 @@ def counts(sysh):
     bi, be = sysh.get("bond_index"), sysh.get("bond_end")
     hi, he = sysh.get("hbond_index"), sysh.get("hbond_end")
     return dict(n=n, N=N, verlet=int(sysh.scalar("verlet_total")), far=int((fe - fi).sum()),
+                far_local=int((fe[:n] - fi[:n]).sum()),
                 bonds=int((be - bi).sum()), hbonds=int((he - hi).sum()), nnz=int(sysh.get("H_numnbrs").sum()))
 
 
@@ -93,6 +94,10 @@ def algorithmic_bytes(slot, c, mixed=False):
         return 8 * N * 20 + c["bonds"] * (56 + 200 + 8 + 16)
     if slot == "verlet":
         return 32 * N + 4 * c["verlet"] + 8 * N
+    if slot == "nonbonded":
+        # the local rows' far entries, packed atoms and far_index/end of the
+        # local atoms, fCdDelta gradient read and written
+        return 64 * c["far_local"] + n * (40 + 8 + 24 + 24)
     return None
 
 
@@ -400,6 +405,23 @@ def main():
         kernels[k] = {"ms_per_step": round(ms / ksteps, 4), "launches": cnt,
                       "avg_us": round(ms * 1e3 / wl, 2),
                       "gbs": round(b * wl / (ms * 1e-3) / 1e9, 1) if b else None}
+    # Widening beyond the metric's path: the non-bonded pass
+    # (vdW_Coulomb_Energy_Full_C) over the last step's far list, timed alone
+    sysh.ktimer_enable(True)
+    sysh.ktimer_read("nonbonded", reset=True)
+    for _ in range(10):
+        from sw_reaxff_amd import _lib as _L
+        _L.check(sysh.L.swr_vdw_coulomb_energy(sysh.h), "swr_vdw_coulomb_energy")
+    device_sync()
+    nb_ms, nb_cnt = sysh.ktimer_read("nonbonded")
+    sysh.ktimer_enable(False)
+    nb_avg = nb_ms / max(1, nb_cnt)
+    nb_b = algorithmic_bytes("nonbonded", c)
+    nonbonded = {"kernel": "k_vdw_coulomb", "avg_us": round(nb_avg * 1e3, 2), "launches": nb_cnt,
+                 "bytes_per_launch": nb_b, "achieved_gbs": round(nb_b / (nb_avg * 1e-3) / 1e9, 1),
+                 "frac": round(nb_b / (nb_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                 "energies_kcal_mol": [float(v) for v in sysh.get("nonbonded_energy")[:2]]}
+    log(f"non-bonded pass: {nonbonded}")
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline (oracle, budget {args.cpu_budget_s:.0f} s)")
@@ -415,6 +437,7 @@ def main():
         "atom_steps_per_s": round(n / (ms_step * 1e-3), 1),
         "cg_iters_per_step": [round(its[0], 2), round(its[1], 2)],
         "counts": c, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
+        "nonbonded_pass": nonbonded,
         "speedup_vs_cpu": round(cpu["value"] / ms_step, 2) if cpu else None,
         "host": platform.node(),
     }

@@ -190,6 +190,18 @@ int swr_pair_compute(swr_system *sys);
  * mixed-precision solve takes one system at a time. */
 int swr_set_batch(swr_system *sys, int nsys, const int *row_offsets);
 
+/* Non-bonded pass (vdW_Coulomb_Energy_Full_C, reaxc_nonbonded_sw64.c:260-500;
+ * host form :522-830; the vdw_type 1 CPE kernel reaxc_nonbonded_cpe.h:1-420):
+ * over the local atoms' far lists and the packed charges, the tapered
+ * shielded Morse vdW (+ inner wall for vdw_type 2/3) and shielded Coulomb
+ * energies.  Adds each local atom's gradient -(CEvd + CEclmb) dvec summed
+ * over its far entries to fCdDelta[i][0..2] (zeroed by swr_reset, as
+ * Reset_Workspace does), and leaves eng_vdwl, eng_coul and the virial (half
+ * of every full-list pair) in swr_get(sys, "nonbonded_energy") (8 doubles).
+ * Run after swr_write_reax_lists (or the fused list pass) of the step;
+ * enqueues only.  lgvdw is not supported (the pair_style default is off). */
+int swr_vdw_coulomb_energy(swr_system *sys);
+
 /* Host-side packing of nsys independent systems into one batch (no
  * reference counterpart; host code, no device calls).  System k has nat[k]
  * local atoms, listed in order in x (3 per atom), type (1-based, already in
@@ -224,7 +236,7 @@ int swr_device_synchronize(void);
  * hbond_index hbond_end hbonds hbond_far bo_dboc dDeltap_self bond_mark
  * Deltap Deltap_boc Delta Delta_e Delta_val vlpex nlp nlp_temp Clp Delta_lp
  * dDelta_lp Delta_lp_temp dDelta_lp_temp H_firstnbr H_numnbrs H_jlist H_val
- * Hdia_inv b_s b_t s t s_hist t_hist.  hbonds' ptr fields come back as
+ * Hdia_inv b_s b_t s t s_hist t_hist fCdDelta nonbonded_energy.  hbonds' ptr fields come back as
  * device addresses; hbond_far gives each hbond's far-list slot. */
 int64_t swr_get(swr_system *sys, const char *name, void *out);
 int     swr_put(swr_system *sys, const char *name, const void *in, int64_t bytes);

@@ -50,6 +50,7 @@ def lib():
         L.orc_qeq_pre_force.argtypes = [vp, ip, ip]
         L.orc_step.argtypes = [vp, ip, ip]
         L.orc_sparse_matvec.argtypes = [vp, dp, dp]
+        L.orc_vdw_coulomb.argtypes = [vp, dp]
         L.orc_get_array.restype = C.c_int64
         L.orc_get_array.argtypes = [vp, C.c_char_p, vp]
         L.orc_set_array.argtypes = [vp, C.c_char_p, vp, C.c_int64]
@@ -175,6 +176,13 @@ class Oracle:
         a, b = C.c_int(0), C.c_int(0)
         self.L.orc_qeq_pre_force(self.h, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def vdw_coulomb(self):
+        """Non-bonded pass over the current far list and packed charges: adds
+        the gradient into fCdDelta; returns (eng_vdwl, eng_coul, virial[6])."""
+        o = np.zeros(8)
+        self.L.orc_vdw_coulomb(self.h, _p(o, C.c_double))
+        return o[0], o[1], o[2:].copy()
 
     def spmv(self, xN):
         xN = np.ascontiguousarray(xN, dtype=np.float64)

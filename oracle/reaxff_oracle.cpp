@@ -93,6 +93,7 @@ struct orc_sys {
   std::vector<SBP> sbp;
   std::vector<std::vector<TBP>> tbp;
   double bo_cut = 0, nonb_low = 0, nonb_cut = 0;
+  int vdw_type = 0;
   double bond_cut = 5.0, hbond_cut = 7.5;
   double swa = 0, swb = 10, tol = 1e-6;
   double safezone = SAFE_ZONE, saferzone = SAFER_ZONE;
@@ -136,6 +137,9 @@ struct orc_sys {
   std::vector<int> bond_mark;
   std::vector<double> Deltap, Deltap_boc, Delta, Delta_e, Delta_val, vlpex, nlp,
       nlp_temp, Clp, Delta_lp, dDelta_lp, Delta_lp_temp, dDelta_lp_temp;
+
+  // ---- non-bonded (f2): workspace fCdDelta [N][4] (gradient xyz + CdDelta)
+  std::vector<double> fCdDelta;
 
   // ---- QEq (a11-a16)
   std::vector<int> Hfirst, Hnum, Hj;
@@ -202,6 +206,17 @@ bool read_ffield(orc_sys *S, const char *path) {
     a.p_val5 = tokd(t, 4); a.rcore2 = tokd(t, 5); a.ecore2 = tokd(t, 6);
     a.acore2 = tokd(t, 7);
   }
+  // vdWaals method (:241-292): 1 shielding only, 2 inner wall only, 3 both;
+  // an element disagreeing with an earlier one keeps the earlier setting,
+  // one with neither is an error
+  S->vdw_type = 0;
+  for (int i = 0; i < nt; i++) {
+    const SBP &a = S->sbp[i];
+    const bool wall = a.rcore2 > 0.01 && a.acore2 > 0.01, shield = a.gamma_w > 0.5;
+    if (!wall && !shield) { S->err = "No shielding or inner-wall set for element " + a.name; return false; }
+    const int want = wall ? (shield ? 3 : 2) : 1;
+    if (S->vdw_type == 0) S->vdw_type = want;
+  }
   for (int i = 0; i < nt; i++)              // :327-335
     if (S->sbp[i].mass < 21 && S->sbp[i].valency_val != S->sbp[i].valency_boc)
       S->sbp[i].valency_val = S->sbp[i].valency_boc;
@@ -240,6 +255,9 @@ bool read_ffield(orc_sys *S, const char *path) {
         b.r_vdW = 2.0 * std::sqrt(p1.r_vdw * p2.r_vdw);
         b.gamma_w = std::sqrt(p1.gamma_w * p2.gamma_w);
         b.gamma = std::pow(p1.gamma * p2.gamma, -1.5);
+        b.rcore = std::sqrt(p1.rcore2 * p2.rcore2);   // :445-452
+        b.ecore = std::sqrt(p1.ecore2 * p2.ecore2);
+        b.acore = std::sqrt(p1.acore2 * p2.acore2);
       }
     }
   t = next();                               // :488-545
@@ -612,6 +630,7 @@ int orc_reset(orc_sys *S) {
       S->Hindex[i] = (S->sbp[S->map[S->ltype[i]]].p_hbond == 1) ? S->numH++ : -1;
   S->bo_dboc.assign(2 * (size_t)N, 0.0);
   S->dDeltap_self.assign(3 * (size_t)N, 0.0);
+  S->fCdDelta.assign(4 * (size_t)N, 0.0);
   S->bindex.assign(N, 0); S->bend.assign(N, 0);
   int tot = 0;
   for (int i = 0; i < N; i++) { S->bindex[i] = S->bend[i] = tot; tot += S->num_bonds[i]; }
@@ -1097,6 +1116,105 @@ int orc_qeq_pre_force(orc_sys *S, int *its, int *itt) {
 
 // One step of the path in the reference's order (call stack: Verlet step ->
 // FixQEqReaxSunway::pre_force -> PairReaxCSunway::compute).
+// f2: vdW_Coulomb_Energy_Full_C, reaxc_nonbonded_sw64.c:260-500, in its
+// general host form :522-830 (the vdw_type 1 CPE kernel,
+// reaxc_nonbonded_cpe.h:1-420, evaluates the same expressions), with libm
+// pow/exp in place of p_powd/p_expd and lgvdw off (pair_reaxc_sunway.cpp:236).
+// Over local atoms i and every far entry j (types >= 0): the taper of
+// Init_Taper (reaxc_init_md_sunway.cpp:100-134, nonb_low..nonb_cut), the
+// shielded Morse vdW (+ inner core for vdw_type 2/3) and the tapered
+// shielded Coulomb; CEvd and CEclmb are (dE/dr)/r.  The gradient
+// -(CEvd + CEclmb) dvec is added to fCdDelta[i][0..2]; eng_vdwl, eng_coul and
+// the virial take half of every full-list pair.  out: eng_vdwl, eng_coul,
+// virial[6].
+int orc_vdw_coulomb(orc_sys *S, double *out) {
+  const int n = S->n;
+  const double C_ele = 332.06371;
+  const double p_vdW1 = S->gp[28], p_vdW1i = 1.0 / p_vdW1;
+  if ((int64_t)S->fCdDelta.size() < 4 * (int64_t)S->N) S->fCdDelta.assign(4 * (size_t)S->N, 0.0);
+  double tap[8];
+  {
+    const double swa = S->nonb_low, swb = S->nonb_cut;
+    const double d7 = std::pow(swb - swa, 7.0);
+    const double swa2 = swa * swa, swa3 = swa * swa * swa, swb2 = swb * swb, swb3 = swb * swb * swb;
+    tap[7] = 20.0 / d7;
+    tap[6] = -70.0 * (swa + swb) / d7;
+    tap[5] = 84.0 * (swa2 + 3.0 * swa * swb + swb2) / d7;
+    tap[4] = -35.0 * (swa3 + 9.0 * swa2 * swb + 9.0 * swa * swb2 + swb3) / d7;
+    tap[3] = 140.0 * (swa3 * swb + 3.0 * swa2 * swb2 + swa * swb3) / d7;
+    tap[2] = -210.0 * (swa3 * swb2 + swa2 * swb3) / d7;
+    tap[1] = 140.0 * swa3 * swb3 / d7;
+    tap[0] = (-35.0 * swa3 * swb2 * swb2 + 21.0 * swa2 * swb3 * swb2 + 7.0 * swa * swb3 * swb3 + swb3 * swb3 * swb) / d7;
+  }
+  const int vt = S->vdw_type;
+  std::vector<double> per(8 * (size_t)n, 0.0);   // per-atom energies + virial, summed in index order below
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < n; i++) {
+    const swr_atom_pack_t &ai = S->packed[i];
+    if (ai.type < 0) continue;
+    double *acc = &per[8 * (size_t)i];
+    double g[3] = {0, 0, 0};
+    for (int64_t pj = S->far_index[i]; pj < S->far_end[i]; pj++) {
+      const swr_far_neighbor_t &e = S->far[pj];
+      if (e.type < 0) continue;
+      const TBP &tw = S->tbp[ai.type][e.type];
+      // per-pair factors as the host form precomputes them (:386-402)
+      const double r_vdWinv = 1.0 / tw.r_vdW, alpha_div_rvdW = tw.alpha / tw.r_vdW;
+      const double powgi = std::pow(tw.gamma_w, -p_vdW1);
+      const double r = e.d, rinv = 1.0 / r;
+      double Tap = tap[7] * r + tap[6];
+      Tap = Tap * r + tap[5]; Tap = Tap * r + tap[4]; Tap = Tap * r + tap[3];
+      Tap = Tap * r + tap[2]; Tap = Tap * r + tap[1]; Tap = Tap * r + tap[0];
+      double dTap = 7 * tap[7] * r + 6 * tap[6];
+      dTap = dTap * r + 5 * tap[5]; dTap = dTap * r + 4 * tap[4];
+      dTap = dTap * r + 3 * tap[3]; dTap = dTap * r + 2 * tap[2];
+      dTap += tap[1] * rinv;
+      double e_vdW, CEvd, e_core = 0;
+      if (vt == 1 || vt == 3) {
+        const double powr = std::pow(r, p_vdW1);
+        const double fn13 = std::pow(powr + powgi, p_vdW1i);
+        const double exp2 = std::exp(0.5 * tw.alpha * (1.0 - fn13 * r_vdWinv));
+        const double exp1 = exp2 * exp2;
+        e_vdW = tw.D * (exp1 - 2.0 * exp2);
+        const double dfn13 = std::pow(powr + powgi, p_vdW1i - 1.0) * std::pow(r, p_vdW1 - 2.0);
+        CEvd = dTap * e_vdW - Tap * tw.D * alpha_div_rvdW * (exp1 - exp2) * dfn13;
+      } else {
+        const double exp2 = std::exp(0.5 * tw.alpha * (1.0 - r * r_vdWinv));
+        const double exp1 = exp2 * exp2;
+        e_vdW = tw.D * (exp1 - 2.0 * exp2);
+        CEvd = dTap * e_vdW - Tap * tw.D * alpha_div_rvdW * (exp1 - exp2) * rinv;
+      }
+      if (vt == 2 || vt == 3) {
+        const double rcoreinv = 1.0 / tw.rcore, a_div_rcore = tw.acore / tw.rcore;
+        e_core = tw.ecore * std::exp(tw.acore * (1.0 - r * rcoreinv));
+        const double de_core = -a_div_rcore * e_core;
+        CEvd += dTap * e_core + Tap * de_core * rinv;
+      }
+      const double dr3gamij_1 = r * r * r + tw.gamma;
+      const double dr3gamij_3 = std::pow(dr3gamij_1, 0.33333333333333);
+      const double qq = C_ele * ai.q * e.q;
+      const double e_ele = qq * Tap / dr3gamij_3;
+      const double CEclmb = qq * (dTap - Tap * r / dr3gamij_1) / dr3gamij_3;
+      const double fpair = -(CEvd + CEclmb);
+      acc[0] += 0.5 * Tap * (e_vdW + e_core);
+      acc[1] += 0.5 * e_ele;
+      const double *del = e.dvec;
+      acc[2] += 0.5 * del[0] * del[0] * fpair;
+      acc[3] += 0.5 * del[1] * del[1] * fpair;
+      acc[4] += 0.5 * del[2] * del[2] * fpair;
+      acc[5] += 0.5 * del[0] * del[1] * fpair;
+      acc[6] += 0.5 * del[0] * del[2] * fpair;
+      acc[7] += 0.5 * del[1] * del[2] * fpair;
+      for (int k = 0; k < 3; k++) g[k] += fpair * del[k];
+    }
+    for (int k = 0; k < 3; k++) S->fCdDelta[4 * (size_t)i + k] += g[k];
+  }
+  for (int k = 0; k < 8; k++) out[k] = 0.0;
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 8; k++) out[k] += per[8 * (size_t)i + k];
+  return 0;
+}
+
 int orc_step(orc_sys *S, int *its, int *itt) {
   int rc = orc_qeq_pre_force(S, its, itt);
   if (rc) return rc;
@@ -1131,7 +1249,7 @@ int64_t orc_get_array(orc_sys *S, const char *name, void *out) {
   A("dDelta_lp_temp", dDelta_lp_temp)
   A("H_firstnbr", Hfirst) A("H_numnbrs", Hnum) A("H_jlist", Hj) A("H_val", Hval)
   A("Hdia_inv", Hdia_inv) A("b_s", b_s) A("b_t", b_t) A("s", s) A("t", t)
-  A("s_hist", s_hist) A("t_hist", t_hist) A("q", q) A("x", x)
+  A("s_hist", s_hist) A("t_hist", t_hist) A("q", q) A("x", x) A("fCdDelta", fCdDelta)
 #undef A
   return -1;
 }
@@ -1160,6 +1278,7 @@ int64_t orc_get_scalar_i(orc_sys *S, const char *name) {
   if (k == "num_bonds_total") return S->num_bonds_total;
   if (k == "num_hbonds_total") return S->num_hbonds_total;
   if (k == "H_nnz") return (int64_t)S->Hj.size();
+  if (k == "vdw_type") return S->vdw_type;
   return -1;
 }
 

@@ -29,7 +29,7 @@ EXPORTS = [
     "swr_lmp_setup", "swr_lmp_run",
     "swr_lmp_system", "swr_lmp_stat", "swr_ktimer_enable", "swr_ktimer_read", "swr_device_synchronize",
     "swr_sparse_matvec_h", "swr_npair_full_bin_atomonly_build", "swr_sort_bonds", "swr_set_batch",
-    "swr_new_atoms", "swr_batch_pack",
+    "swr_new_atoms", "swr_batch_pack", "swr_vdw_coulomb_energy",
 ]
 
 
@@ -71,7 +71,7 @@ def lib():
         for nm in ["swr_write_reax_atoms_and_pack", "swr_reset", "swr_write_reax_lists", "swr_init_forces_noqeq_full",
                    "swr_write_reax_lists_init_forces", "swr_validate_lists", "swr_sort_bonds", "swr_bo",
                    "swr_qeq_init_matvec",
-                   "swr_calculate_q", "swr_pair_compute", "swr_sync"]:
+                   "swr_calculate_q", "swr_pair_compute", "swr_sync", "swr_vdw_coulomb_energy"]:
             getattr(L, nm).argtypes = [vp]
         L.swr_sparse_matvec.argtypes = [vp, vp, vp]
         L.swr_sparse_matvec_h.argtypes = [vp, dp, dp]

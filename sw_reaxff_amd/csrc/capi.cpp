@@ -93,6 +93,39 @@ void fill_params(swr_system *S, const std::vector<int> &map) {
   h.hbond_cut = S->ctl.hbond_cut;
   h.nonb_cut = ff.nonb_cut;
   h.nonb_cutsq = ff.nonb_cut * ff.nonb_cut;
+  // non-bonded pair factors and taper
+  h.vdw_type = ff.vdw_type;
+  h.p_vdW1 = ff.gp.size() > 28 ? ff.gp[28] : 0.0;
+  h.p_vdW1i = 1.0 / h.p_vdW1;
+  for (int e = 0; e < ff.ne() && e < SWR_MAX_ELEM; e++)
+    for (int f = 0; f < ff.ne() && f < SWR_MAX_ELEM; f++) {
+      const FfPair &p = ff.tb(e, f);
+      DevNb &d = h.nb[e * SWR_MAX_ELEM + f];
+      d.D = p.D;
+      d.alpha = p.alpha;
+      d.r_vdWinv = 1.0 / p.r_vdW;
+      d.alpha_div_rvdW = p.alpha / p.r_vdW;
+      d.powgi_vdW1 = std::pow(p.gamma_w, -h.p_vdW1);
+      d.gamma = p.gamma;
+      d.ecore = p.ecore;
+      d.acore = p.acore;
+      d.rcoreinv = p.rcore > 0 ? 1.0 / p.rcore : 0.0;
+      d.a_div_rcore = p.rcore > 0 ? p.acore / p.rcore : 0.0;
+    }
+  {
+    const double swa = ff.nonb_low, swb = ff.nonb_cut;
+    const double d7 = std::pow(swb - swa, 7.0);
+    const double swa2 = swa * swa, swa3 = swa * swa * swa, swb2 = swb * swb, swb3 = swb * swb * swb;
+    h.nb_tap[7] = 20.0 / d7;
+    h.nb_tap[6] = -70.0 * (swa + swb) / d7;
+    h.nb_tap[5] = 84.0 * (swa2 + 3.0 * swa * swb + swb2) / d7;
+    h.nb_tap[4] = -35.0 * (swa3 + 9.0 * swa2 * swb + 9.0 * swa * swb2 + swb3) / d7;
+    h.nb_tap[3] = 140.0 * (swa3 * swb + 3.0 * swa2 * swb2 + swa * swb3) / d7;
+    h.nb_tap[2] = -210.0 * (swa3 * swb2 + swa2 * swb3) / d7;
+    h.nb_tap[1] = 140.0 * swa3 * swb3 / d7;
+    h.nb_tap[0] = (-35.0 * swa3 * swb2 * swb2 + 21.0 * swa2 * swb3 * swb2 + 7.0 * swa * swb3 * swb3 +
+                   swb3 * swb3 * swb) / d7;
+  }
   // pair->extract("chi"/"eta"/"gamma") per LAMMPS type (pair_reaxc_sunway.cpp:1106-1128)
   for (int t = 1; t <= h.ntypes; t++) {
     const FfAtom &a = ff.atom[map[t]];
@@ -150,7 +183,7 @@ int swr_nbr_ensure_atoms(swr_system *S, int Ncap) {
       (rc = swr_alloc(&S->dDeltap_self, 3 * nn)) || (rc = swr_alloc(&S->bond_mark, nn)) ||
       (rc = swr_alloc(&S->bond_mark2, nn)) || (rc = swr_alloc(&S->counts, 2 * nn)) ||
       (rc = swr_alloc(&S->ws, WS_COUNT * nn)) || (rc = swr_alloc(&S->far_index, nn)) ||
-      (rc = swr_alloc(&S->far_end, nn)))
+      (rc = swr_alloc(&S->far_end, nn)) || (rc = swr_alloc(&S->fcd, 4 * nn)))
     return rc;
   HIP_CHECK(hipMemsetAsync(S->counts, 0, 2 * nn * sizeof(int), S->stream));
   S->Ncap = Ncap;
@@ -265,6 +298,7 @@ int swr_destroy(swr_system *S) {
                   S->voff, S->vcount, S->vlist, S->far, S->far_index, S->far_end, S->num_bonds, S->num_hbonds,
                   S->Hindex, S->bond_index, S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, S->Cdbo, S->Cdbopi,
                   S->Cdbopi2, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,
+                  S->fcd, S->nb_partials, S->nb_result,
                   S->bond_mark2, S->counts, S->ws, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32,
                   S->Hdia_inv, S->b_s, S->st, S->s_hist, S->t_hist, S->cgv, S->cgs, S->partials, S->counters,
                   S->status, S->scan_tmp, S->tmp_i, S->mixw, S->cand, S->cand_count, S->row_sys, S->tile_beg,
@@ -559,6 +593,13 @@ int swr_pair_compute(swr_system *S) {
   return SWR_OK;
 }
 
+int swr_vdw_coulomb_energy(swr_system *S) {
+  if (!S) return SWR_ERR_ARG;
+  int rc;
+  if ((rc = swr_quiesce(S))) return rc;
+  return swr_nonbonded_run(S);
+}
+
 int swr_sync(swr_system *S) {
   if (!S) return SWR_ERR_ARG;
   int rc = refresh_status(S);
@@ -621,6 +662,8 @@ bool lookup(swr_system *S, const std::string &k, View &v) {
   if (k == "bo_dboc") return set(S->bo_dboc, 16 * N);
   if (k == "dDeltap_self") return set(S->dDeltap_self, 24 * N);
   if (k == "bond_mark") return set(S->bond_mark, 4 * N);
+  if (k == "fCdDelta") return set(S->fcd, 32 * N);
+  if (k == "nonbonded_energy") return set(S->nb_result, S->nb_result ? 64 : 0);
   static const char *wsn[WS_COUNT] = {"Deltap", "Deltap_boc", "Delta", "Delta_e", "Delta_val", "vlpex", "nlp",
                                       "nlp_temp", "Clp", "Delta_lp", "dDelta_lp", "Delta_lp_temp", "dDelta_lp_temp"};
   for (int w = 0; w < WS_COUNT; w++)

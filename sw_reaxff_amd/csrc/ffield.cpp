@@ -93,6 +93,15 @@ std::string swr_read_ffield(const char *path, swr_ffield_t *ff) {
     a.p_ovun2 = num(t, 0); a.p_val3 = num(t, 1); a.valency_val = num(t, 3); a.p_val5 = num(t, 4);
     a.rcore2 = num(t, 5); a.ecore2 = num(t, 6); a.acore2 = num(t, 7);
   }
+  // vdWaals method (reaxc_ffield_sunway.cpp:241-292): the first element
+  // decides, later disagreeing elements keep it; neither shielding nor an
+  // inner wall is an error
+  ff->vdw_type = 0;
+  for (const FfAtom &a : ff->atom) {
+    const bool wall = a.rcore2 > 0.01 && a.acore2 > 0.01, shield = a.gamma_w > 0.5;
+    if (!wall && !shield) return "No shielding or inner-wall set for element " + a.name;
+    if (ff->vdw_type == 0) ff->vdw_type = wall ? (shield ? 3 : 2) : 1;
+  }
   for (FfAtom &a : ff->atom)
     if (a.mass < 21 && a.valency_val != a.valency_boc) a.valency_val = a.valency_boc;
 
@@ -126,6 +135,9 @@ std::string swr_read_ffield(const char *path, swr_ffield_t *ff) {
       p.r_vdW = 2.0 * std::sqrt(a.r_vdw * b.r_vdw);
       p.gamma_w = std::sqrt(a.gamma_w * b.gamma_w);
       p.gamma = std::pow(a.gamma * b.gamma, -1.5);
+      p.rcore = std::sqrt(a.rcore2 * b.rcore2);   // :445-452
+      p.ecore = std::sqrt(a.ecore2 * b.ecore2);
+      p.acore = std::sqrt(a.acore2 * b.acore2);
     }
   if (L.next(t) && !t.empty()) {
     const int noff = std::atoi(t[0].c_str());

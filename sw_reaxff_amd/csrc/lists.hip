@@ -745,6 +745,9 @@ int swr_lists_reset(swr_system *S) {
   if (rc) return rc;
   int *hflag = S->tmp_i, *hcap = hflag + N + 1, *hpos = hcap + N + 1, *hstart = hpos + N + 1;
   swr_kt_begin(S, KT_RESET);
+  // Reset_Workspace (reaxc_reset_tools_sunway.cpp:122-131) zeroes fCdDelta,
+  // which the non-bonded pass accumulates into
+  HIP_CHECK(hipMemsetAsync(S->fcd, 0, sizeof(double) * 4 * (size_t)N, S->stream));
   hipLaunchKernelGGL(k_reset_flags, dim3(div_up(N, 256)), dim3(256), 0, S->stream, S->packed, S->dp, n, N,
                      S->ctl.hbond_cut, S->num_hbonds, hflag, hcap);
   if ((rc = swr_scan_i32(S, hflag, hpos, N, &S->status->numH))) return rc;

@@ -20,6 +20,7 @@ struct FfPair {
   double p_boc3 = 0, p_boc4 = 0, p_boc5 = 0;
   double De_s = 0, De_p = 0, De_pp = 0, p_be1 = 0, p_be2 = 0, p_ovun1 = 0;
   double D = 0, alpha = 0, r_vdW = 0, gamma_w = 0, gamma = 0;
+  double rcore = 0, ecore = 0, acore = 0;   // inner wall (vdw_type 2, 3)
   double v13cor = 0, ovc = 0;
 };
 
@@ -28,6 +29,7 @@ struct swr_ffield_t {
   std::vector<FfAtom> atom;
   std::vector<FfPair> pair;         // [ne*ne]
   double bo_cut = 0, nonb_low = 0, nonb_cut = 0;
+  int vdw_type = 0;                 // 1 shielding, 2 inner wall, 3 both
   int ne() const { return (int)atom.size(); }
   const FfPair &tb(int i, int j) const { return pair[(size_t)i * atom.size() + j]; }
   FfPair &tb(int i, int j) { return pair[(size_t)i * atom.size() + j]; }

@@ -53,6 +53,12 @@ struct DevTbp {
   double v13cor, ovc;
 };
 
+struct DevNb {
+  double D, alpha, r_vdWinv, alpha_div_rvdW, powgi_vdW1;  // shielded Morse
+  double gamma;                                           // Coulomb shielding
+  double ecore, acore, rcoreinv, a_div_rcore;             // inner wall
+};
+
 struct DevParams {
   int nelem, ntypes;
   int map[SWR_MAX_TYPES + 1];
@@ -73,6 +79,14 @@ struct DevParams {
   double Tap[8];
   double swa, swb, swbsq;
   double tol;
+  // Non-bonded (vdW_Coulomb_Energy_Full_C), element-pair indexed: the
+  // factors its host form precomputes per pair (reaxc_nonbonded_sw64.c:
+  // 386-402), the Coulomb shielding gamma, and the taper on
+  // [nonb_low, nonb_cut] (Init_Taper, reaxc_init_md_sunway.cpp:100-134).
+  DevNb nb[SWR_MAX_ELEM * SWR_MAX_ELEM];
+  double nb_tap[8];
+  double p_vdW1, p_vdW1i;
+  int vdw_type;
 };
 
 // Small block of device-written results the host reads back asynchronously.
@@ -113,7 +127,7 @@ struct CgScalars {
 
 // Kernel timer slots.
 enum { KT_VERLET = 0, KT_FAR, KT_BUILD_H, KT_CG_ITER, KT_CG_SETUP, KT_BO, KT_VALIDATE, KT_SPMV, KT_RESET,
-       KT_QEQ_Q, KT_BINS, KT_MIXED_INNER, KT_MIXED_OUTER, KT_BONDS, KT_CG_SCALARS, KT_MARK, KT_CG_UPDATE, KT_COUNT };
+       KT_QEQ_Q, KT_BINS, KT_MIXED_INNER, KT_MIXED_OUTER, KT_BONDS, KT_CG_SCALARS, KT_MARK, KT_CG_UPDATE, KT_NONBONDED, KT_COUNT };
 
 struct swr_system {
   // ---- host-side configuration
@@ -173,6 +187,10 @@ struct swr_system {
   long long vtotal_local = -1;     // Verlet entries of the local rows (voff[n]), known at each build
 
   // ---- BO workspace
+  double *fcd = nullptr;           // fCdDelta [N][4]: gradient xyz + CdDelta (zeroed by Reset)
+  double *nb_partials = nullptr;   // non-bonded per-block energy/virial partials
+  size_t nb_partials_cap = 0;
+  double *nb_result = nullptr;     // eng_vdwl, eng_coul, virial[6] of the last non-bonded pass
   double *bo_dboc = nullptr;       // [N][2]
   double *dDeltap_self = nullptr;  // [N][3]
   int *bond_mark = nullptr, *bond_mark2 = nullptr;
@@ -271,6 +289,7 @@ int swr_lists_validate(swr_system *S);
 int swr_lists_mark(swr_system *S, int max_sweeps);
 int swr_mark_settle(swr_system *S);
 int swr_after_sync(swr_system *S);
+int swr_nonbonded_run(swr_system *S);
 int swr_quiesce(swr_system *S);
 constexpr int MARK_INNER = 8;         // bond_mark relaxation sweeps per launch
 constexpr int MARK_MAX_LAUNCH = 128;  // launches of one relaxation at most

@@ -123,6 +123,50 @@ class FField:
             pp = math.exp(t["p_bo5"] * (d / t["r_pp"]) ** t["p_bo6"])
         return s, pi, pp
 
+    def vdw_pair(self, a: str, b: str) -> dict:
+        """Non-bonded pair parameters: geometric means of the atom values
+        (r_vdW = 2 sqrt(r_vdw_a r_vdw_b)), the off-diagonal block's D, r_vdW
+        (2x its value) and alpha when > 0, the Coulomb shielding
+        (gamma_a gamma_b)^-1.5 and the inner-wall terms."""
+        A, B = self.atoms[a], self.atoms[b]
+        d = dict(D=math.sqrt(A["epsilon"] * B["epsilon"]), alpha=math.sqrt(A["alpha"] * B["alpha"]),
+                 r_vdW=2.0 * math.sqrt(A["r_vdw"] * B["r_vdw"]), gamma_w=math.sqrt(A["gamma_w"] * B["gamma_w"]),
+                 gamma=(A["gamma"] * B["gamma"]) ** -1.5, rcore=math.sqrt(A["rcore2"] * B["rcore2"]),
+                 ecore=math.sqrt(A["ecore2"] * B["ecore2"]), acore=math.sqrt(A["acore2"] * B["acore2"]))
+        od = self.offdiag.get((self.idx(a), self.idx(b)))
+        if od is not None:
+            if od["D"] > 0:
+                d["D"] = od["D"]
+            if od["r_vdw"] > 0:
+                d["r_vdW"] = 2.0 * od["r_vdw"]
+            if od["alpha"] > 0:
+                d["alpha"] = od["alpha"]
+        return d
+
+    def vdw_type(self) -> int:
+        """1 shielding, 2 inner wall, 3 both: the first element's setting."""
+        A = self.atoms[self.order[0]]
+        wall = A["rcore2"] > 0.01 and A["acore2"] > 0.01
+        shield = A["gamma_w"] > 0.5
+        return (3 if shield else 2) if wall else 1
+
+    def nonbonded_pair_energy(self, a: str, b: str, r: float, qa: float, qb: float):
+        """(E_vdW, E_Coulomb) of one pair at distance r: tapered shielded
+        Morse (+ inner wall) and tapered shielded Coulomb (ReaxFF)."""
+        P, p = self.vdw_pair(a, b), self.gp[28]
+        vt = self.vdw_type()
+        tap = self.taper_closed_form(r)
+        if vt in (1, 3):
+            f13 = (r ** p + P["gamma_w"] ** -p) ** (1.0 / p)
+        else:
+            f13 = r
+        x = P["alpha"] * (1.0 - f13 / P["r_vdW"])
+        ev = P["D"] * (math.exp(x) - 2.0 * math.exp(0.5 * x))
+        if vt in (2, 3):
+            ev += P["ecore"] * math.exp(P["acore"] * (1.0 - r / P["rcore"]))
+        ec = 332.06371 * qa * qb / (r ** 3 + P["gamma"]) ** (1.0 / 3.0)
+        return tap * ev, tap * ec
+
     def taper_closed_form(self, r: float) -> float:
         """7th-order taper for swa = 0: 20x^7 - 70x^6 + 84x^5 - 35x^4 + 1, x = r/swb."""
         x = r / self.gp[12]
