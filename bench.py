@@ -188,9 +188,10 @@ def copy_calibration(mib):
 
 def bench_clusters(args):
     """512 non-periodic water/methanol clusters (sizes log-uniform in [50, 3000]):
-    one batched evaluation (neighbour list, QEq, lists, BO; one launch per
-    stage for all clusters) against the plain per-system loop of the same
-    library, both from host arrays to results on the device."""
+    one batched evaluation (neighbour list, QEq, lists, BO and the non-bonded
+    energy + gradient; one launch per stage for all clusters) against the
+    plain per-system loop of the same library, both from host arrays to
+    results on the device."""
     from sw_reaxff_amd import batch as B
     count, lo, hi = 512, 50, 3000
     sizes = systems.cluster_sizes(count, lo, hi, seed=args.seed)
@@ -200,14 +201,14 @@ def bench_clusters(args):
     bt = B.Batch(tol=1e-6)
     log("batched: warm-up evaluation")
     bt.load(parts)
-    bt.evaluate()
+    bt.evaluate(nonbonded=True)
     bt.sync()
     lat = []
     for _ in range(max(1, args.steps)):
         device_sync()
         t0 = time.perf_counter()
         bt.load(parts)
-        its = bt.evaluate()
+        its = bt.evaluate(nonbonded=True)
         bt.sync()
         lat.append(time.perf_counter() - t0)
     t_batch = float(np.median(lat))
@@ -216,24 +217,25 @@ def bench_clusters(args):
     nplain = min(count, args.plain_sample)
     one = B.Batch(tol=1e-6)
     one.load([parts[0]])
-    one.evaluate()
+    one.evaluate(nonbonded=True)
     one.sync()
     device_sync()
     t0 = time.perf_counter()
     for k in range(nplain):
         one.load([parts[k]])
-        one.evaluate()
+        one.evaluate(nonbonded=True)
         one.sync()
     t_plain = (time.perf_counter() - t0) / nplain
     log(f"plain loop: {t_plain * 1e3:.2f} ms per system ({nplain} systems)")
     sps_batch, sps_plain = count / t_batch, 1.0 / t_plain
     out = {
-        "metric": "batched systems/s (nbr+BO+QEq, 512 clusters)", "value": round(sps_batch, 1),
+        "metric": "batched systems/s (nbr+BO+QEq + non-bonded E/F, 512 clusters)", "value": round(sps_batch, 1),
         "unit": "systems/s", "n_gpus": 1, "steps": len(lat), "warmup": 1,
         "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{count} water/methanol clusters, sizes log-uniform [{lo}, {hi}], "
-                               f"{natoms:,} atoms, 0.9 g/cm^3, 12 A vacuum padding, QEq tol 1e-6",
+                               f"{natoms:,} atoms, 0.9 g/cm^3, 12 A vacuum padding, QEq tol 1e-6, "
+                               "energy (vdW + Coulomb) and its gradient per cluster",
                    "systems": count, "atoms": natoms, "seed": args.seed, "parallelism": "single GPU"},
         "plain_loop_systems_per_s": round(sps_plain, 1), "speedup_vs_plain_loop": round(sps_batch / sps_plain, 2),
         "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
@@ -242,6 +244,48 @@ def bench_clusters(args):
         "plain_sample": nplain, "host": platform.node(),
     }
     print(json.dumps(out), flush=True)
+
+
+def mixed_errors(s, args):
+    """The mixed-precision run against an FP64 run at a tight tolerance
+    (1e-10), stepped side by side from the same atoms: the zero-force NVE
+    moves both identically, so every step compares the same positions.  Per
+    step: max |dq| (e) and the max error of the non-bonded gradient (the
+    forces this path computes, which carry the charges) relative to its
+    largest component; the FP64 path at the same tolerance as the mixed one
+    runs alongside for comparison."""
+    from sw_reaxff_amd import _lib as L
+    runs = []
+    for mixed, tol in ((True, args.tol), (False, 1e-10), (False, args.tol)):
+        lm = Lammps()
+        lm.commands(standard_script(s.elements, ffield=DEFAULT_FFIELD, every=args.every, dt=args.dt,
+                                    mixed=mixed, tol=tol))
+        lm.create_atoms(s)
+        lm.setup()
+        runs.append(lm)
+    n = s.n
+    dq, df, dq64, df64 = [], [], [], []
+    for step in range(args.error_steps):
+        out = []
+        for lm in runs:
+            lm.run(1)
+            h = lm.system
+            L.check(h.L.swr_vdw_coulomb_energy(h.h), "swr_vdw_coulomb_energy")
+            h.sync()
+            out.append((h.get("q")[:n], h.get("fCdDelta").reshape(-1, 4)[:n, :3]))
+        (qm, fm), (qd, fd), (q6, f6) = out
+        dq.append(float(np.max(np.abs(qm - qd))))
+        df.append(float(np.max(np.abs(fm - fd)) / np.max(np.abs(fd))))
+        dq64.append(float(np.max(np.abs(q6 - qd))))
+        df64.append(float(np.max(np.abs(f6 - fd)) / np.max(np.abs(fd))))
+    for lm in runs:
+        lm.close()
+    return {"steps": args.error_steps, "reference": "FP64 QEq at tol 1e-10, same trajectory",
+            "max_abs_dq_e": max(dq), "median_abs_dq_e": float(np.median(dq)), "max_rel_dF": max(df),
+            "fp64_at_same_tol": {"max_abs_dq_e": max(dq64), "median_abs_dq_e": float(np.median(dq64)),
+                                 "max_rel_dF": max(df64)},
+            "per_step_max_abs_dq_e": [float(f"{v:.3e}") for v in dq],
+            "per_step_max_rel_dF": [float(f"{v:.3e}") for v in df]}
 
 
 def bench_serve(args):
@@ -311,6 +355,8 @@ def main():
                     help="clusters timed one at a time for the plain-loop comparison")
     ap.add_argument("--mixed", action="store_true", help="mixed-precision QEq")
     ap.add_argument("--tol", type=float, default=1e-6, help="QEq tolerance (the reference's: 1e-6)")
+    ap.add_argument("--error-steps", type=int, default=200,
+                    help="mixed: steps of the side-by-side comparison with an FP64 run (0: skip)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--every", type=int, default=10)
     ap.add_argument("--dt", type=float, default=0.25)
@@ -422,6 +468,10 @@ def main():
                  "frac": round(nb_b / (nb_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                  "energies_kcal_mol": [float(v) for v in sysh.get("nonbonded_energy")[:2]]}
     log(f"non-bonded pass: {nonbonded}")
+    errors = None
+    if args.mixed and args.error_steps > 0:
+        errors = mixed_errors(s, args)
+        log(f"mixed vs FP64: {errors}")
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline (oracle, budget {args.cpu_budget_s:.0f} s)")
@@ -437,7 +487,7 @@ def main():
         "atom_steps_per_s": round(n / (ms_step * 1e-3), 1),
         "cg_iters_per_step": [round(its[0], 2), round(its[1], 2)],
         "counts": c, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
-        "nonbonded_pass": nonbonded,
+        "nonbonded_pass": nonbonded, "mixed_errors_vs_fp64": errors,
         "speedup_vs_cpu": round(cpu["value"] / ms_step, 2) if cpu else None,
         "host": platform.node(),
     }

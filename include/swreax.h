@@ -197,7 +197,8 @@ int swr_set_batch(swr_system *sys, int nsys, const int *row_offsets);
  * energies.  Adds each local atom's gradient -(CEvd + CEclmb) dvec summed
  * over its far entries to fCdDelta[i][0..2] (zeroed by swr_reset, as
  * Reset_Workspace does), and leaves eng_vdwl, eng_coul and the virial (half
- * of every full-list pair) in swr_get(sys, "nonbonded_energy") (8 doubles).
+ * of every full-list pair) in swr_get(sys, "nonbonded_energy") (8 doubles),
+ * and each local atom's half of its pairs' energy in "nonbonded_eatom".
  * Run after swr_write_reax_lists (or the fused list pass) of the step;
  * enqueues only.  lgvdw is not supported (the pair_style default is off). */
 int swr_vdw_coulomb_energy(swr_system *sys);
@@ -236,7 +237,8 @@ int swr_device_synchronize(void);
  * hbond_index hbond_end hbonds hbond_far bo_dboc dDeltap_self bond_mark
  * Deltap Deltap_boc Delta Delta_e Delta_val vlpex nlp nlp_temp Clp Delta_lp
  * dDelta_lp Delta_lp_temp dDelta_lp_temp H_firstnbr H_numnbrs H_jlist H_val
- * Hdia_inv b_s b_t s t s_hist t_hist fCdDelta nonbonded_energy.  hbonds' ptr fields come back as
+ * Hdia_inv b_s b_t s t s_hist t_hist fCdDelta nonbonded_energy nonbonded_eatom.  hbonds' ptr
+ * fields come back as
  * device addresses; hbond_far gives each hbond's far-list slot. */
 int64_t swr_get(swr_system *sys, const char *name, void *out);
 int     swr_put(swr_system *sys, const char *name, const void *in, int64_t bytes);

@@ -102,7 +102,8 @@ def pack(batch: list[systems.System], elements=ELEMENTS, cutghost: float = CUTGH
 
 
 class Batch:
-    """One device system holding a batch; evaluate() runs nbr + QEq + lists + BO once for all."""
+    """One device system holding a batch; evaluate() runs nbr + QEq + lists + BO (+ the
+    non-bonded pass) once for all."""
 
     def __init__(self, ffield=DEFAULT_FFIELD, elements=ELEMENTS, tol=1e-6, mixed=False):
         self.elements = list(elements)
@@ -121,13 +122,17 @@ class Batch:
         self.packed = P
         return P
 
-    def evaluate(self):
-        """Neighbour list, QEq (per-system), far/bond/hbond lists, BO; returns max CG iterations (s, t)."""
+    def evaluate(self, nonbonded=False):
+        """Neighbour list, QEq (per-system), far/bond/hbond lists, BO and, with
+        nonbonded, the vdW + Coulomb energies and gradient; returns max CG
+        iterations (s, t)."""
         h = self.sys
         _lib.check(h.L.swr_npair_full_bin_ghost_build(h.h, C.c_double(CUTGHOST)), "swr_npair_full_bin_ghost_build")
         a, b = C.c_int(0), C.c_int(0)
         _lib.check(h.L.swr_qeq_pre_force(h.h, C.byref(a), C.byref(b)), "swr_qeq_pre_force")
         _lib.check(h.L.swr_pair_compute(h.h), "swr_pair_compute")
+        if nonbonded:
+            _lib.check(h.L.swr_vdw_coulomb_energy(h.h), "swr_vdw_coulomb_energy")
         return a.value, b.value
 
     def sync(self):
@@ -142,6 +147,19 @@ class Batch:
         q = self.sys.get("q")
         P = self.packed
         return [q[P.row_off[k]:P.row_off[k + 1]] for k in range(len(P.row_off) - 1)]
+
+    def energies(self):
+        """Per system, its non-bonded (vdW + Coulomb) energy: the sum of its
+        atoms' shares (after evaluate(nonbonded=True))."""
+        e = self.sys.get("nonbonded_eatom")
+        P = self.packed
+        return np.array([e[P.row_off[k]:P.row_off[k + 1]].sum() for k in range(len(P.row_off) - 1)])
+
+    def gradients(self):
+        """Per system, the non-bonded energy gradient of its local atoms (n_k x 3)."""
+        g = self.sys.get("fCdDelta").reshape(-1, 4)
+        P = self.packed
+        return [g[P.row_off[k]:P.row_off[k + 1], :3] for k in range(len(P.row_off) - 1)]
 
     def iterations(self):
         it = self.sys.get("sys_iters")

@@ -298,7 +298,7 @@ int swr_destroy(swr_system *S) {
                   S->voff, S->vcount, S->vlist, S->far, S->far_index, S->far_end, S->num_bonds, S->num_hbonds,
                   S->Hindex, S->bond_index, S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, S->Cdbo, S->Cdbopi,
                   S->Cdbopi2, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,
-                  S->fcd, S->nb_partials, S->nb_result,
+                  S->fcd, S->nb_partials, S->nb_result, S->nb_eatom,
                   S->bond_mark2, S->counts, S->ws, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32,
                   S->Hdia_inv, S->b_s, S->st, S->s_hist, S->t_hist, S->cgv, S->cgs, S->partials, S->counters,
                   S->status, S->scan_tmp, S->tmp_i, S->mixw, S->cand, S->cand_count, S->row_sys, S->tile_beg,
@@ -664,6 +664,7 @@ bool lookup(swr_system *S, const std::string &k, View &v) {
   if (k == "bond_mark") return set(S->bond_mark, 4 * N);
   if (k == "fCdDelta") return set(S->fcd, 32 * N);
   if (k == "nonbonded_energy") return set(S->nb_result, S->nb_result ? 64 : 0);
+  if (k == "nonbonded_eatom") return set(S->nb_eatom, S->nb_eatom ? 8 * n : 0);
   static const char *wsn[WS_COUNT] = {"Deltap", "Deltap_boc", "Delta", "Delta_e", "Delta_val", "vlpex", "nlp",
                                       "nlp_temp", "Clp", "Delta_lp", "dDelta_lp", "Delta_lp_temp", "dDelta_lp_temp"};
   for (int w = 0; w < WS_COUNT; w++)

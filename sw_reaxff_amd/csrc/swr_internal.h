@@ -191,6 +191,8 @@ struct swr_system {
   double *nb_partials = nullptr;   // non-bonded per-block energy/virial partials
   size_t nb_partials_cap = 0;
   double *nb_result = nullptr;     // eng_vdwl, eng_coul, virial[6] of the last non-bonded pass
+  double *nb_eatom = nullptr;      // per local atom: its half of its pairs' vdW + Coulomb energy
+  size_t nb_eatom_cap = 0;
   double *bo_dboc = nullptr;       // [N][2]
   double *dDeltap_self = nullptr;  // [N][3]
   int *bond_mark = nullptr, *bond_mark2 = nullptr;

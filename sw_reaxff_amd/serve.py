@@ -1,7 +1,8 @@
 """Serving loop: evaluation requests of independent systems, continuously batched.
 
 Requests arrive over time; each is either a new system to evaluate once
-(neighbour list, QEq charges, far/bond/hbond lists, bond orders) or a
+(neighbour list, QEq charges, far/bond/hbond lists, bond orders, non-bonded
+energy and gradient) or a
 follow-on MD step of a resident system.  The server:
 
   * runs `lanes` workers, each owning one device system (its own HIP stream).
@@ -56,6 +57,7 @@ class Request:
     batch_size: int = 0
     step_index: int = 0             # kind "step": which step of its resident system this was
     hist: tuple | None = None       # resident's first evaluation: the s/t histories it left
+    energy: float | None = None     # new systems: non-bonded (vdW + Coulomb) energy, kcal/mol
 
     @property
     def latency(self):
@@ -232,14 +234,16 @@ class Server:
                 t = time.perf_counter()
                 bt.load([r.system for r in reqs])
                 t = self._account("pack_load", t)
-                bt.evaluate()
+                bt.evaluate(nonbonded=True)
                 bt.sync()
                 t = self._account("evaluate", t)
                 qs = bt.charges()
                 its = bt.iterations()
+                es = bt.energies()
                 hists = bt.histories() if any(r.resident >= 0 for r in reqs) else None
                 for k, (r, q, it) in enumerate(zip(reqs, qs, its)):
                     r.q = q.copy()
+                    r.energy = float(es[k])
                     r.iters = tuple(int(v) for v in it)
                     if r.resident >= 0:
                         r.hist = hists[k]

@@ -51,14 +51,16 @@ def test_alone_equals_in_the_mix(gpu_lib, tol, qtol):
     parts = mix()
     bt = B.Batch(tol=tol)
     bt.load(parts)
-    bt.evaluate()
+    bt.evaluate(nonbonded=True)
     bt.sync()
     q_mix = bt.charges()
     it_mix = bt.iterations()
+    e_mix = bt.energies()
+    g_mix = [g.copy() for g in bt.gradients()]
     for k, s in enumerate(parts):
         alone = B.Batch(tol=tol)
         alone.load([s])
-        alone.evaluate()
+        alone.evaluate(nonbonded=True)
         alone.sync()
         q1 = alone.charges()[0]
         it1 = alone.iterations()[0]
@@ -73,6 +75,15 @@ def test_alone_equals_in_the_mix(gpu_lib, tol, qtol):
         assert np.array_equal(c1, c2), k
         assert np.array_equal(n1, n2), k
         assert rel_err(bo2, bo1, 1e-12) <= 1e-9, k
+        # the non-bonded energy and gradient move with the charges (a charge
+        # change dq moves E by ~sum_j 332 q_j / r_ij * dq ~ 1e2..1e3 dq
+        # relative to E): bounds 100 x and 1000 x the charge tolerance
+        e1, g1 = alone.energies()[0], alone.gradients()[0]
+        de = abs(e1 - e_mix[k]) / abs(e1)
+        dg = np.max(np.abs(g1 - g_mix[k])) / np.abs(g1).max()
+        print(f"[batch] system {k}: E_nb {e1:.10e} rel diff {de:.2e}, gradient rel diff {dg:.2e}", flush=True)
+        assert de <= 100 * qtol, k
+        assert dg <= 1000 * qtol, k
 
 
 def test_batch_cluster_against_oracle(gpu_lib, oracle_lib, ffield):

@@ -16,11 +16,16 @@ pytestmark = pytest.mark.gpu
 Q_MIXED_ATOL = 1e-6
 
 
-def converged_charges(oracle_lib, ffield, elements, h):
+def converged_oracle(oracle_lib, ffield, elements, h):
     A = atoms_of(h)
     o = make_oracle(oracle_lib, ffield, elements, dict(A, q=np.zeros(A["N"])), verlet=verlet_of(h), tol=1e-10)
     o.step()
-    return o.get("q")[: A["n"]]
+    return o, A["n"]
+
+
+def converged_charges(oracle_lib, ffield, elements, h):
+    o, n = converged_oracle(oracle_lib, ffield, elements, h)
+    return o.get("q")[:n]
 
 
 def test_mixed_charges_within_1e6_of_converged_fp64(gpu_lib, oracle_lib, ffield):
@@ -39,20 +44,32 @@ def test_mixed_charges_within_1e6_of_converged_fp64(gpu_lib, oracle_lib, ffield)
 
 
 def test_mixed_md_steps(gpu_lib, oracle_lib, ffield):
-    """12 NVE steps (a rebuild at 10) with warm starts: every step within 1e-6 e."""
+    """12 NVE steps (a rebuild at 10) with warm starts: every step within 1e-6 e
+    of the converged FP64 charges, and the non-bonded gradient (the forces
+    the charges enter) within 1e-5 of its largest component of the one the
+    oracle gets from the converged charges."""
     s = water_c0(seed=22)
     lmp = harness(s, ffield, mixed=True)
     h = lmp.system
-    worst = 0.0
+    worst, worst_f = 0.0, 0.0
     for step in range(1, 13):
         lmp.run(1)
         n = h.scalar("n")
-        q_ref = converged_charges(oracle_lib, ffield, s.elements, h)
+        o, _ = converged_oracle(oracle_lib, ffield, s.elements, h)
+        q_ref = o.get("q")[:n]
         e = float(np.max(np.abs(h.get("q")[:n] - q_ref)))
         worst = max(worst, e)
-        print(f"[mixed] step {step}: max|dq| {e:.2e} matvecs {h.scalar('matvecs_s')},{h.scalar('matvecs_t')}",
-              flush=True)
+        o.vdw_coulomb()
+        f_ref = o.get("fCdDelta").reshape(-1, 4)[:n, :3]
+        gpu_lib.check(h.L.swr_vdw_coulomb_energy(h.h), "swr_vdw_coulomb_energy")
+        h.sync()
+        f = h.get("fCdDelta").reshape(-1, 4)[:n, :3]
+        ef = float(np.max(np.abs(f - f_ref)) / np.max(np.abs(f_ref)))
+        worst_f = max(worst_f, ef)
+        print(f"[mixed] step {step}: max|dq| {e:.2e} max|dF|/max|F| {ef:.2e} matvecs "
+              f"{h.scalar('matvecs_s')},{h.scalar('matvecs_t')}", flush=True)
     assert worst <= Q_MIXED_ATOL
+    assert worst_f <= 1e-5
 
 
 def test_mixed_spmv_uses_fp32_values(gpu_lib, ffield):

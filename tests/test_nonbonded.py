@@ -182,6 +182,8 @@ def _compare(lmp, oracle_lib, ff_path, s, xa, owner):
     assert ev == pytest.approx(oev, rel=1e-10)
     assert ec == pytest.approx(oec, rel=1e-10)
     assert np.max(np.abs(vir - ovir)) <= 1e-10 * np.max(np.abs(ovir))
+    ea = lmp.system.get("nonbonded_eatom")[: s.n]
+    assert ea.sum() == pytest.approx(ev + ec, rel=1e-12)
     g = fcd[: s.n, :3]
     assert np.max(np.abs(g - og)) <= 1e-10 * np.max(np.abs(og))
 
