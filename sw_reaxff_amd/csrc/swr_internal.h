@@ -233,6 +233,7 @@ struct swr_system {
   void *mixw = nullptr;            // mixed-precision workspace
   size_t mixw_bytes = 0;
   int mixed_outer = 0;             // residual evaluations of the last mixed solve
+  int mixed_last_inner = 0;        // iterations of the last inner FP32 solve (sizes the next launch batch)
   double mixed_rho0[2] = {0, 0};   // its first outer residual sqrt(u.r)/|b| (s, t)
   int imax = 200;
 
