@@ -297,15 +297,18 @@ def bench_serve(args):
     request per evaluation, all requests queued at the start."""
     from sw_reaxff_amd import serve
     from sw_reaxff_amd.batch import Batch
-    lanes = [Batch(tol=1e-6) for _ in range(args.lanes)]
+    mixed = not args.fp64
+    lanes = [Batch(tol=args.tol, mixed=mixed) for _ in range(args.lanes)]
     warm = serve.make_requests(24, rate=1e4, seed=args.seed + 1, hi=3000)
-    serve.Server(lanes=args.lanes, max_batch_atoms=args.max_batch_atoms, batches=lanes).run(warm)
+    serve.Server(lanes=args.lanes, max_batch_atoms=args.max_batch_atoms, batches=lanes, tol=args.tol,
+                 mixed=mixed).run(warm)
     runs = []
     base = serve.make_requests(args.requests, rate=1.0, seed=args.seed)   # systems generated once
     for rate in args.rates:
         reqs = serve.at_rate(base, rate)
         t0 = time.perf_counter()
-        srv = serve.Server(lanes=args.lanes, max_batch_atoms=args.max_batch_atoms, batches=lanes)
+        srv = serve.Server(lanes=args.lanes, max_batch_atoms=args.max_batch_atoms, batches=lanes, tol=args.tol,
+                           mixed=mixed)
         srv.run(reqs)
         wall = time.perf_counter() - t0
         summ = serve.summary(reqs, wall)
@@ -317,7 +320,7 @@ def bench_serve(args):
     # plain loop: one worker, batches of one, every request queued at t = 0
     reqs = serve.at_rate(base[:args.plain_sample], 1e9)
     t0 = time.perf_counter()
-    serve.Server(lanes=1, max_batch_atoms=1, batches=[lanes[0]]).run(reqs)
+    serve.Server(lanes=1, max_batch_atoms=1, batches=[lanes[0]], tol=args.tol, mixed=mixed).run(reqs)
     wall = time.perf_counter() - t0
     plain = len(reqs) / wall
     log(f"plain loop: {plain:.1f} systems/s ({len(reqs)} requests)")
@@ -327,10 +330,11 @@ def bench_serve(args):
         "metric": "serving systems/s at p99 <= 50 ms (nbr+BO+QEq, Poisson arrivals)", "value": best["systems_per_s"],
         "unit": "systems/s", "n_gpus": 1, "steps": args.requests, "warmup": len(warm),
         "ms_per_step": best["latency_ms"]["p50"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64+f32" if mixed else "f64", "data": "synthetic",
         "config": {"workload": f"{args.requests} requests per rate, Poisson arrivals, sizes log-uniform [20, 20000], "
                                "50% periodic water / 50% water-methanol clusters, 30% follow-on MD steps with "
-                               "resident s/t, 200 GB HBM admission, QEq tol 1e-6",
+                               f"resident s/t, 200 GB HBM admission, "
+                               f"{'mixed-precision ' if mixed else 'FP64 '}QEq tol {args.tol:g}",
                    "lanes": args.lanes, "max_batch_atoms": args.max_batch_atoms, "seed": args.seed,
                    "parallelism": "single GPU"},
         "best_rate": best["rate"], "p99_within_50ms": bool(ok), "latency_ms": best["latency_ms"],
@@ -354,6 +358,7 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=128,
                     help="clusters timed one at a time for the plain-loop comparison")
     ap.add_argument("--mixed", action="store_true", help="mixed-precision QEq")
+    ap.add_argument("--fp64", action="store_true", help="serve: FP64 QEq instead of the mixed-precision form")
     ap.add_argument("--tol", type=float, default=1e-6, help="QEq tolerance (the reference's: 1e-6)")
     ap.add_argument("--error-steps", type=int, default=200,
                     help="mixed: steps of the side-by-side comparison with an FP64 run (0: skip)")

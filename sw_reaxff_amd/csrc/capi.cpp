@@ -302,7 +302,8 @@ int swr_destroy(swr_system *S) {
                   S->bond_mark2, S->counts, S->ws, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32,
                   S->Hdia_inv, S->b_s, S->st, S->s_hist, S->t_hist, S->cgv, S->cgs, S->partials, S->counters,
                   S->status, S->scan_tmp, S->tmp_i, S->mixw, S->cand, S->cand_count, S->row_sys, S->tile_beg,
-                  S->tile_sys, S->sys_tile, S->sys_iters, S->done_count, S->qsum, S->mark_chg};
+                  S->tile_sys, S->sys_tile, S->sblk_beg, S->sys_sblk, S->sys_iters, S->done_count, S->qsum,
+                  S->mark_chg};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (S->status_host) (void)hipHostFree(S->status_host);
@@ -564,10 +565,6 @@ int swr_set_batch(swr_system *S, int nsys, const int *row_offsets) {
       swr_set_error_msg("swr_set_batch: offsets must not decrease");
       return SWR_ERR_ARG;
     }
-  if (S->mixed && nsys > 1) {
-    swr_set_error_msg("swr_set_batch: the mixed-precision solve runs one system at a time");
-    return SWR_ERR_STATE;
-  }
   S->sys_row.assign(row_offsets, row_offsets + nsys + 1);
   S->layout_dirty = true;
   return swr_qeq_layout(S);

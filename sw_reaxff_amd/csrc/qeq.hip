@@ -212,17 +212,22 @@ int swr_qeq_layout(swr_system *S) {
   const int n = S->n;
   if (S->sys_row.size() < 2 || S->sys_row.back() != n) S->sys_row = {0, n};
   const int nsys = (int)S->sys_row.size() - 1;
-  std::vector<int> row_sys(std::max(n, 1)), tile_beg, tile_sys, sys_tile(nsys + 1);
+  std::vector<int> row_sys(std::max(n, 1)), tile_beg, tile_sys, sys_tile(nsys + 1), sblk_beg, sys_sblk(nsys + 1);
   for (int s = 0; s < nsys; s++) {
     sys_tile[s] = (int)tile_sys.size();
     for (int r = S->sys_row[s]; r < S->sys_row[s + 1]; r += UT) {
       tile_beg.push_back(r);
       tile_sys.push_back(s);
     }
+    sys_sblk[s] = (int)sblk_beg.size();
+    for (int r = S->sys_row[s]; r < S->sys_row[s + 1]; r += RPB) sblk_beg.push_back(r);
     for (int r = S->sys_row[s]; r < S->sys_row[s + 1]; r++) row_sys[r] = s;
   }
   sys_tile[nsys] = (int)tile_sys.size();
   tile_beg.push_back(n);
+  const int nsblk = (int)sblk_beg.size();
+  sys_sblk[nsys] = nsblk;
+  sblk_beg.push_back(n);
   if (tile_sys.empty()) {  // no rows: one empty tile keeps every grid non-empty
     tile_beg = {0, 0};
     tile_sys = {0};
@@ -233,6 +238,7 @@ int swr_qeq_layout(swr_system *S) {
   int rc;
   if ((rc = swr_alloc(&S->row_sys, row_sys.size())) || (rc = swr_alloc(&S->tile_beg, tile_beg.size())) ||
       (rc = swr_alloc(&S->tile_sys, tile_sys.size())) || (rc = swr_alloc(&S->sys_tile, sys_tile.size())) ||
+      (rc = swr_alloc(&S->sblk_beg, sblk_beg.size())) || (rc = swr_alloc(&S->sys_sblk, sys_sblk.size())) ||
       (rc = swr_alloc(&S->sys_iters, 2 * (size_t)nsys)) || (rc = swr_alloc(&S->cgs, 4 * (size_t)nsys)) ||
       (rc = swr_alloc(&S->qsum, 3 * (size_t)nsys)))
     return rc;
@@ -243,9 +249,13 @@ int swr_qeq_layout(swr_system *S) {
                            S->stream));
   HIP_CHECK(hipMemcpyAsync(S->sys_tile, sys_tile.data(), sizeof(int) * sys_tile.size(), hipMemcpyHostToDevice,
                            S->stream));
+  HIP_CHECK(hipMemcpyAsync(S->sblk_beg, sblk_beg.data(), sizeof(int) * sblk_beg.size(), hipMemcpyHostToDevice,
+                           S->stream));
+  HIP_CHECK(hipMemcpyAsync(S->sys_sblk, sys_sblk.data(), sizeof(int) * sys_sblk.size(), hipMemcpyHostToDevice,
+                           S->stream));
   HIP_CHECK(hipMemsetAsync(S->sys_iters, 0, sizeof(int) * 2 * nsys, S->stream));
   HIP_CHECK(hipMemsetAsync(S->cgs, 0, sizeof(CgScalars) * 4 * nsys, S->stream));
-  const int ps = std::max(div_up(std::max(n, 1), RPB), ntiles);
+  const int ps = std::max(std::max(div_up(std::max(n, 1), RPB), nsblk), ntiles);
   if (ps > S->cg_pstride || !S->partials) {
     if ((rc = swr_alloc(&S->partials, (size_t)ps * 8))) return rc;
     S->cg_pstride = ps;
@@ -254,6 +264,7 @@ int swr_qeq_layout(swr_system *S) {
   HIP_CHECK(hipStreamSynchronize(S->stream));
   S->nsys = nsys;
   S->ntiles = ntiles;
+  S->nsblk = nsblk;
   S->h_sys_iters.assign(2 * nsys, 0);
   S->layout_dirty = false;
   return SWR_OK;

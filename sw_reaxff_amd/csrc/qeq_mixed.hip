@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 
 namespace {
 
@@ -44,7 +45,7 @@ constexpr double OUTER_TOL_FACTOR = 0.01;  // outer target = tol * this
 constexpr double INNER_TOL_MIN = 3e-4;
 constexpr int INNER_IMAX = 50;
 
-// Outer state (device), read back by the host after every residual.
+// Outer state of one system (device).
 struct MixState {
   int done;            // both right-hand sides meet the tolerance
   int outer;           // residual evaluations so far
@@ -53,18 +54,62 @@ struct MixState {
   double rho[2];       // sqrt(u.r)/|b| of the latest residual
 };
 
+// Inner FP32 PCG state of one right-hand side of one system.
+struct PcgState {
+  double rz, alpha, beta, bnorm, tol;
+  int iter, active;
+};
+
+// Counters over the batch of systems, read back by the host between launch
+// batches (with system 0's MixState right after them).
+struct MixCtl {
+  int outer_ndone;     // systems whose refinement has finished
+  int inner_ndone;     // systems whose inner s and t solves have both finished
+  int inner_itmax;     // most inner iterations of a system in the current inner solve
+  int pad;
+};
+
+// Workspace layout: MixCtl at 0, MixState[nsys] from byte 64, PcgState[2 nsys]
+// and the float2 vectors each from the next 256-byte boundary.
+constexpr size_t MS_OFF = 64;
+inline size_t pcg_off(int nsys) { return (MS_OFF + sizeof(MixState) * (size_t)nsys + 255) & ~(size_t)255; }
+inline size_t vec_off(int nsys) { return (pcg_off(nsys) + sizeof(PcgState) * 2 * (size_t)nsys + 255) & ~(size_t)255; }
+
+// Rows of a half-wave-per-row block: a batch uses the per-system block
+// layout (sblk_beg), one system the plain 8-row blocks.
+template <bool BATCH>
+__device__ __forceinline__ void sblk_rows(int n, const int *__restrict__ sblk_beg, int &r0, int &r1) {
+  if (BATCH) {
+    r0 = sblk_beg[blockIdx.x];
+    r1 = sblk_beg[blockIdx.x + 1];
+  } else {
+    r0 = blockIdx.x * RPB;
+    r1 = min(n, r0 + RPB);
+  }
+}
+
+__device__ __forceinline__ bool pcg_live(const PcgState *__restrict__ c) { return (c[0].active | c[1].active) != 0; }
+
 // r = b - (eta x + H x) in FP64 over FP32 values; u = M^-1 r; the dots
 // u.r (s, t) and r.r (s, t), b.b (s, t); r is also written as the FP32
-// right-hand side of the inner solve.  Half-wave per row as in k_cg.
+// right-hand side of the inner solve.  Half-wave per row as in k_cg.  Rows
+// of systems whose refinement has finished are skipped (not on the first
+// pass, whose state is the previous solve's).
+template <bool BATCH>
 __global__ void __launch_bounds__(QT)
 k_resid(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
         const int *__restrict__ numnbrs, const int *__restrict__ jlist, const float *__restrict__ val32,
         const double2 *__restrict__ x, const double2 *__restrict__ b, const double *__restrict__ Hd,
-        float2 *__restrict__ r32, double *__restrict__ partials, int pstride) {
+        float2 *__restrict__ r32, double *__restrict__ partials, int pstride, const int *__restrict__ sblk_beg,
+        const int *__restrict__ row_sys, const MixState *__restrict__ ms, int first) {
+  int r0, r1;
+  sblk_rows<BATCH>(n, sblk_beg, r0, r1);
+  if (BATCH && !first && r0 < r1 && ms[row_sys[r0]].done) return;
   const int w = wave_in_block(), lane = lane_id();
-  const int i = blockIdx.x * RPB + w * 2 + (lane >> 5);
-  const bool owner = (lane & 31) == 0 && i < n;
-  const int beg = i < n ? firstnbr[i] : 0, cnt = i < n ? numnbrs[i] : 0;
+  const int i = r0 + w * 2 + (lane >> 5);
+  const bool in = i < r1;
+  const bool owner = (lane & 31) == 0 && in;
+  const int beg = in ? firstnbr[i] : 0, cnt = in ? numnbrs[i] : 0;
   double2 q = row_spmv2_h<double, float>(beg, cnt, jlist, val32, x);
   double d[6] = {0, 0, 0, 0, 0, 0};
   if (owner) {
@@ -80,16 +125,21 @@ k_resid(int n, const DevParams *__restrict__ P, const int *__restrict__ type, co
   block_partials<6, QT>(d, partials, pstride, blockIdx.x);
 }
 
-// Outer test and the inner tolerances (one workgroup).  A right-hand side
-// that meets the test gets tol = -1 in the inner scalars, so its inner solve
-// does nothing and its d stays 0.
+// Outer test and the inner tolerances, one workgroup per system (over its
+// own blocks' partials, in a fixed order).  A right-hand side that meets the
+// test gets tol = -1 in the inner scalars, so its inner solve does nothing
+// and its d stays 0.
 __global__ void __launch_bounds__(RT)
-k_resid_scalars(int nblk, const double *__restrict__ partials, int pstride, MixState *__restrict__ ms,
-                CgScalars *__restrict__ inner, double tol, int first, DevStatus *st, int *__restrict__ iters) {
+k_resid_scalars(const double *__restrict__ partials, int pstride, const int *__restrict__ sys_sblk,
+                MixCtl *__restrict__ ctl, MixState *__restrict__ msa, CgScalars *__restrict__ inner_all, double tol,
+                int first, DevStatus *st, int *__restrict__ iters_all) {
+  const int sys = blockIdx.x;
+  MixState m = msa[sys];
+  if (!first && m.done) return;
   double tot[6];
-  reduce_partials<6>(partials, pstride, 0, nblk, tot);
+  reduce_partials<6>(partials, pstride, sys_sblk[sys], sys_sblk[sys + 1], tot);
   if (threadIdx.x != 0) return;
-  MixState m = *ms;
+  CgScalars *inner = inner_all + 2 * sys;
   if (first) {
     m.done = 0; m.outer = 0;
     m.converged[0] = m.converged[1] = 0;
@@ -127,12 +177,16 @@ k_resid_scalars(int nblk, const double *__restrict__ partials, int pstride, MixS
     atomicOr(&st->flags, SWR_F_CG_MAXIT);
   }
   if (m.done) {
-    iters[0] = m.matvecs[0];
-    iters[1] = m.matvecs[1];
+    // a system that stops here gets no inner solve: both tolerances are -1
+    inner[0].tol = inner[1].tol = -1.0;
+    iters_all[2 * sys] = m.matvecs[0];
+    iters_all[2 * sys + 1] = m.matvecs[1];
+    atomicAdd(&ctl->outer_ndone, 1);
   }
-  *ms = m;
+  msa[sys] = m;
 }
 
+// x += d over all rows (d is 0 in systems whose inner solve did not run).
 __global__ void k_mixed_update(int n, double2 *__restrict__ x, const float2 *__restrict__ d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -148,28 +202,24 @@ __global__ void k_mixed_update(int n, double2 *__restrict__ x, const float2 *__r
 // standard form, FP32 vectors (s and t side by side as float2), FP64 dots
 // and scalars.  Unlike the pipelined recurrence it needs no setup products
 // and keeps converging at the FP64 rate over the one to two orders of
-// magnitude an inner solve asks for.  Per iteration: q = H p with p.q
-// (k_pcg_spmv), alpha (k_pcg_alpha), d += alpha p, r -= alpha q, z = M^-1 r
-// with r.z (k_pcg_update), beta and the stopping test sqrt(r.z) <= tol |b|
-// on the outer test's norm (k_pcg_beta), p = z + beta p (k_pcg_p).  A
-// right-hand side that stops is frozen; launches after both stopped exit at
-// once.
-struct PcgState {
-  double rz, alpha, beta, bnorm, tol;
-  int iter, active;
-};
-struct PcgCtl {
-  PcgState st[2];
-  int done;
-};
+// magnitude an inner solve asks for.  One iteration: q = eta p + H p with
+// the p.q partials (k_pcg_spmv), alpha (k_pcg_alpha), d += alpha p,
+// r -= alpha q, z = M^-1 r with r.z (k_pcg_update), beta and the stopping
+// test sqrt(r.z) <= tol |b| on the outer test's norm (k_pcg_beta),
+// p = z + beta p (k_pcg_p).  Each system has its own scalars (one
+// workgroup per system sums its own blocks' partials in a fixed order, so a
+// system takes the same iterations alone and in a batch); a right-hand side
+// that stops is frozen, rows of a system whose solves both stopped are
+// skipped, and launches after every system stopped exit at once.
 
 __global__ void __launch_bounds__(UT)
-k_pcg_init(int n, const float2 *__restrict__ rhs, const double *__restrict__ Hd, float2 *__restrict__ d,
+k_pcg_init(CgLayout L, const float2 *__restrict__ rhs, const double *__restrict__ Hd, float2 *__restrict__ d,
            float2 *__restrict__ r, float2 *__restrict__ z, float2 *__restrict__ p, double *__restrict__ partials,
            int pstride) {
-  const int i = blockIdx.x * UT + threadIdx.x;
+  const int tile = blockIdx.x;
+  const int i = L.tile_beg[tile] + threadIdx.x;
   double dd[4] = {0, 0, 0, 0};
-  if (i < n) {
+  if (i < L.tile_beg[tile + 1]) {
     const float2 b = rhs[i];
     const float h = (float)Hd[i];
     const float2 zz = make_float2(h * b.x, h * b.y);
@@ -180,39 +230,47 @@ k_pcg_init(int n, const float2 *__restrict__ rhs, const double *__restrict__ Hd,
     dd[0] = (double)b.x * zz.x; dd[1] = (double)b.y * zz.y;
     dd[2] = (double)b.x * b.x; dd[3] = (double)b.y * b.y;
   }
-  block_partials<4, UT>(dd, partials, pstride, blockIdx.x);
+  block_partials<4, UT>(dd, partials, pstride, tile);
 }
 
 __global__ void __launch_bounds__(RT)
-k_pcg_init_scalars(int nblk, const double *__restrict__ partials, int pstride, const CgScalars *__restrict__ cgs,
-                   PcgCtl *__restrict__ ctl) {
+k_pcg_init_scalars(const double *__restrict__ partials, int pstride, CgLayout L,
+                   const CgScalars *__restrict__ inner_all, PcgState *__restrict__ pst, MixCtl *__restrict__ ctl) {
+  const int sys = blockIdx.x;
   double tot[4];
-  reduce_partials<4>(partials, pstride, 0, nblk, tot);
+  reduce_partials<4>(partials, pstride, L.sys_tile[sys], L.sys_tile[sys + 1], tot);
   if (threadIdx.x != 0) return;
+  PcgState *c = pst + 2 * sys;
   for (int s = 0; s < 2; s++) {
-    PcgState &c = ctl->st[s];
-    c.rz = tot[s];
-    c.bnorm = sqrt(tot[2 + s]);
-    c.tol = cgs[s].tol;
-    c.iter = 0;
-    c.alpha = c.beta = 0.0;
-    c.active = (c.tol > 0.0 && c.bnorm > 0.0 && sqrt(fmax(c.rz, 0.0)) > c.tol * c.bnorm) ? 1 : 0;
+    c[s].rz = tot[s];
+    c[s].bnorm = sqrt(tot[2 + s]);
+    c[s].tol = inner_all[2 * sys + s].tol;
+    c[s].iter = 0;
+    c[s].alpha = c[s].beta = 0.0;
+    c[s].active = (c[s].tol > 0.0 && c[s].bnorm > 0.0 && sqrt(fmax(c[s].rz, 0.0)) > c[s].tol * c[s].bnorm) ? 1 : 0;
   }
-  ctl->done = (ctl->st[0].active || ctl->st[1].active) ? 0 : 1;
+  if (!pcg_live(c)) atomicAdd(&ctl->inner_ndone, 1);
 }
 
+template <bool BATCH>
 __global__ void __launch_bounds__(QT)
-k_pcg_spmv(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
-           const int *__restrict__ numnbrs, const int *__restrict__ jlist, const float *__restrict__ val,
-           const float2 *__restrict__ p, float2 *__restrict__ q, const PcgCtl *__restrict__ ctl,
-           double *__restrict__ partials, int pstride) {
+k_pcg_spmv(int n, int nsys, const DevParams *__restrict__ P, const int *__restrict__ type,
+           const int *__restrict__ firstnbr, const int *__restrict__ numnbrs, const int *__restrict__ jlist,
+           const float *__restrict__ val, const float2 *__restrict__ p, float2 *__restrict__ q,
+           const MixCtl *__restrict__ ctl, const PcgState *__restrict__ pst, double *__restrict__ partials,
+           int pstride, const int *__restrict__ sblk_beg, const int *__restrict__ row_sys) {
+  int r0, r1;
+  sblk_rows<BATCH>(n, sblk_beg, r0, r1);
   const int w = wave_in_block(), lane = lane_id();
-  const int i = blockIdx.x * RPB + w * 2 + (lane >> 5);
-  const int beg0 = i < n ? firstnbr[i] : 0, cnt0 = i < n ? numnbrs[i] : 0;
-  if (ctl->done && cnt0 >= 0) return;
+  const int i = r0 + w * 2 + (lane >> 5);
+  const bool in = i < r1;
+  // the row's extent is requested together with the solve state
+  const int beg0 = in ? firstnbr[i] : 0, cnt0 = in ? numnbrs[i] : 0;
+  if (ctl->inner_ndone >= nsys && cnt0 >= 0) return;
+  if (BATCH && r0 < r1 && !pcg_live(pst + 2 * row_sys[r0])) return;
   const float2 sum = row_spmv2_h<float, float>(beg0, cnt0, jlist, val, p);
   double dd[2] = {0, 0};
-  if ((lane & 31) == 0 && i < n) {
+  if ((lane & 31) == 0 && in) {
     const float eta = (float)P->eta[type[i]];
     const float2 pi = p[i];
     const float2 qq = make_float2(eta * pi.x + sum.x, eta * pi.y + sum.y);
@@ -224,26 +282,28 @@ k_pcg_spmv(int n, const DevParams *__restrict__ P, const int *__restrict__ type,
 }
 
 __global__ void __launch_bounds__(RT)
-k_pcg_alpha(int nblk, const double *__restrict__ partials, int pstride, PcgCtl *__restrict__ ctl) {
-  if (ctl->done) return;
+k_pcg_alpha(const double *__restrict__ partials, int pstride, const int *__restrict__ sys_sblk,
+            PcgState *__restrict__ pst) {
+  const int sys = blockIdx.x;
+  PcgState *c = pst + 2 * sys;
+  if (!pcg_live(c)) return;
   double tot[2];
-  reduce_partials<2>(partials, pstride, 0, nblk, tot);
+  reduce_partials<2>(partials, pstride, sys_sblk[sys], sys_sblk[sys + 1], tot);
   if (threadIdx.x != 0) return;
-  for (int s = 0; s < 2; s++) {
-    PcgState &c = ctl->st[s];
-    c.alpha = c.active ? c.rz / tot[s] : 0.0;
-  }
+  for (int s = 0; s < 2; s++) c[s].alpha = c[s].active ? c[s].rz / tot[s] : 0.0;
 }
 
 __global__ void __launch_bounds__(UT)
-k_pcg_update(int n, const double *__restrict__ Hd, float2 *__restrict__ d, float2 *__restrict__ r,
+k_pcg_update(CgLayout L, const double *__restrict__ Hd, float2 *__restrict__ d, float2 *__restrict__ r,
              float2 *__restrict__ z, const float2 *__restrict__ p, const float2 *__restrict__ q,
-             const PcgCtl *__restrict__ ctl, double *__restrict__ partials, int pstride) {
-  if (ctl->done) return;
-  const int i = blockIdx.x * UT + threadIdx.x;
-  const float as = (float)ctl->st[0].alpha, at = (float)ctl->st[1].alpha;
+             const PcgState *__restrict__ pst, double *__restrict__ partials, int pstride) {
+  const int tile = blockIdx.x;
+  const PcgState *c = pst + 2 * L.tile_sys[tile];
+  if (!pcg_live(c)) return;
+  const int i = L.tile_beg[tile] + threadIdx.x;
+  const float as = (float)c[0].alpha, at = (float)c[1].alpha;
   double dd[2] = {0, 0};
-  if (i < n) {
+  if (i < L.tile_beg[tile + 1]) {
     const float2 pp = p[i], qq = q[i];
     float2 dv = d[i], rv = r[i];
     dv.x += as * pp.x; dv.y += at * pp.y;
@@ -256,33 +316,40 @@ k_pcg_update(int n, const double *__restrict__ Hd, float2 *__restrict__ d, float
     dd[0] = (double)rv.x * zz.x;
     dd[1] = (double)rv.y * zz.y;
   }
-  block_partials<2, UT>(dd, partials, pstride, blockIdx.x);
+  block_partials<2, UT>(dd, partials, pstride, tile);
 }
 
 __global__ void __launch_bounds__(RT)
-k_pcg_beta(int nblk, const double *__restrict__ partials, int pstride, PcgCtl *__restrict__ ctl, int imax) {
-  if (ctl->done) return;
+k_pcg_beta(const double *__restrict__ partials, int pstride, CgLayout L, PcgState *__restrict__ pst,
+           MixCtl *__restrict__ ctl, int imax) {
+  const int sys = blockIdx.x;
+  PcgState *c = pst + 2 * sys;
+  if (!pcg_live(c)) return;
   double tot[2];
-  reduce_partials<2>(partials, pstride, 0, nblk, tot);
+  reduce_partials<2>(partials, pstride, L.sys_tile[sys], L.sys_tile[sys + 1], tot);
   if (threadIdx.x != 0) return;
   for (int s = 0; s < 2; s++) {
-    PcgState &c = ctl->st[s];
-    if (!c.active) { c.beta = 0.0; continue; }
-    c.iter += 1;
-    c.beta = tot[s] / c.rz;
-    c.rz = tot[s];
-    c.active = (c.iter < imax && sqrt(fmax(c.rz, 0.0)) > c.tol * c.bnorm) ? 1 : 0;
+    if (!c[s].active) { c[s].beta = 0.0; continue; }
+    c[s].iter += 1;
+    c[s].beta = tot[s] / c[s].rz;
+    c[s].rz = tot[s];
+    c[s].active = (c[s].iter < imax && sqrt(fmax(c[s].rz, 0.0)) > c[s].tol * c[s].bnorm) ? 1 : 0;
   }
-  ctl->done = (ctl->st[0].active || ctl->st[1].active) ? 0 : 1;
+  if (!pcg_live(c)) {
+    atomicAdd(&ctl->inner_ndone, 1);
+    atomicMax(&ctl->inner_itmax, max(c[0].iter, c[1].iter));
+  }
 }
 
 __global__ void __launch_bounds__(UT)
-k_pcg_p(int n, const float2 *__restrict__ z, float2 *__restrict__ p, const PcgCtl *__restrict__ ctl) {
-  if (ctl->done) return;
-  const int i = blockIdx.x * UT + threadIdx.x;
-  if (i >= n) return;
-  const float bs = (float)ctl->st[0].beta, bt = (float)ctl->st[1].beta;
-  const bool as = ctl->st[0].active != 0, at = ctl->st[1].active != 0;
+k_pcg_p(CgLayout L, const float2 *__restrict__ z, float2 *__restrict__ p, const PcgState *__restrict__ pst) {
+  const int tile = blockIdx.x;
+  const PcgState *c = pst + 2 * L.tile_sys[tile];
+  if (!pcg_live(c)) return;
+  const int i = L.tile_beg[tile] + threadIdx.x;
+  if (i >= L.tile_beg[tile + 1]) return;
+  const float bs = (float)c[0].beta, bt = (float)c[1].beta;
+  const bool as = c[0].active != 0, at = c[1].active != 0;
   const float2 zz = z[i];
   float2 pp = p[i];
   if (as) pp.x = zz.x + bs * pp.x;
@@ -290,23 +357,41 @@ k_pcg_p(int n, const float2 *__restrict__ z, float2 *__restrict__ p, const PcgCt
   p[i] = pp;
 }
 
-// Writes the iteration counts into the inner scalars the outer accounting reads.
-__global__ void k_pcg_finish(const PcgCtl *__restrict__ ctl, CgScalars *__restrict__ cgs, int *__restrict__ iters) {
+// Writes each system's iteration counts into the inner scalars the outer
+// accounting reads; the status block gets the largest.
+__global__ void __launch_bounds__(256)
+k_pcg_finish(int nsys, const PcgState *__restrict__ pst, CgScalars *__restrict__ inner_all,
+             int *__restrict__ status_iters) {
+  __shared__ int mx[2][256];
+  int m0 = 0, m1 = 0;
+  for (int s = threadIdx.x; s < nsys; s += 256) {
+    const int i0 = pst[2 * s].iter, i1 = pst[2 * s + 1].iter;
+    inner_all[2 * s].iter = i0;
+    inner_all[2 * s + 1].iter = i1;
+    m0 = max(m0, i0);
+    m1 = max(m1, i1);
+  }
+  mx[0][threadIdx.x] = m0;
+  mx[1][threadIdx.x] = m1;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) {
+      mx[0][threadIdx.x] = max(mx[0][threadIdx.x], mx[0][threadIdx.x + k]);
+      mx[1][threadIdx.x] = max(mx[1][threadIdx.x], mx[1][threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
-    cgs[0].iter = ctl->st[0].iter;
-    cgs[1].iter = ctl->st[1].iter;
-    iters[0] = ctl->st[0].iter;
-    iters[1] = ctl->st[1].iter;
+    status_iters[0] = mx[0][0];
+    status_iters[1] = mx[1][0];
   }
 }
 
-// Inner vectors (float2) and the outer state, carved from one allocation.
-enum { MV_X = 0, MV_R, MV_U, MV_W, MV_Z, MV_SS, MV_V, MV_P, MV_M, MV_D0, MV_D1, MV_B, MV_Q, MV_COUNT };
+// Inner vectors (float2), carved from the workspace after the states.
+enum { MV_D = 0, MV_R, MV_Z, MV_P, MV_Q, MV_B, MV_COUNT };
 
-// Layout: MixState at the start, the float2 vectors from byte 256.
 int ensure_mixw(swr_system *S) {
-  static_assert(sizeof(MixState) <= 256, "MixState must fit its slot");
-  const size_t need = 256 + sizeof(float2) * MV_COUNT * (size_t)S->ncap;
+  const size_t need = vec_off(S->nsys) + sizeof(float2) * MV_COUNT * (size_t)S->ncap;
   if (S->mixw && S->mixw_bytes >= need) return SWR_OK;
   if (S->mixw) (void)hipFree(S->mixw);
   S->mixw = nullptr;
@@ -328,71 +413,88 @@ int swr_qeq_mixed_solve(swr_system *S) {
     return SWR_ERR_STATE;
   }
   if ((rc = swr_qeq_layout(S)) || (rc = ensure_mixw(S))) return rc;
-  if (S->nsys != 1) {
-    swr_set_error_msg("the mixed-precision solve runs one system at a time");
-    return SWR_ERR_STATE;
-  }
-  static_assert(sizeof(MixState) <= 128 && sizeof(PcgCtl) <= 128, "mixed-precision state must fit its slots");
+  static_assert(sizeof(MixCtl) <= MS_OFF, "the batch counters must fit their slot");
   const int n = S->n;
-  const int nblk = div_up(n, RPB);    // half-wave-per-row kernels
-  const int ublk = div_up(n, UT);     // thread-per-row kernels
+  const int nsys = S->nsys;
+  const bool batch = nsys > 1;
+  const CgLayout L = swr_qeq_cg_layout(S);
+  const int nblk = batch ? S->nsblk : div_up(n, RPB);   // half-wave-per-row kernels
   const int ps = S->cg_pstride;
-  MixState *ms = (MixState *)S->mixw;
-  PcgCtl *pc = (PcgCtl *)((char *)S->mixw + 128);
-  float2 *base = (float2 *)((char *)S->mixw + 256);
+  char *w = (char *)S->mixw;
+  MixCtl *ctl = (MixCtl *)w;
+  MixState *ms = (MixState *)(w + MS_OFF);
+  PcgState *pst = (PcgState *)(w + pcg_off(nsys));
+  float2 *base = (float2 *)(w + vec_off(nsys));
   const size_t c = S->ncap;
-  float2 *dv = base + MV_X * c, *rv = base + MV_R * c, *zv = base + MV_Z * c, *pv = base + MV_P * c,
+  float2 *dv = base + MV_D * c, *rv = base + MV_R * c, *zv = base + MV_Z * c, *pv = base + MV_P * c,
          *qv = base + MV_Q * c;
-  float2 *rhs = base + MV_B * c;  // the inner solve's b: the FP32 residual
-  CgScalars *inner = S->cgs + 2;  // after the FP64 solve's s, t: the inner tolerances
+  float2 *rhs = base + MV_B * c;      // the inner solve's b: the FP32 residual
+  CgScalars *inner = S->cgs + 2 * nsys;  // after the FP64 solve's s, t: the inner tolerances
   double2 *x = (double2 *)S->st;
   const double2 *b = (const double2 *)S->b_s;
-  for (int outer = 0;; outer++) {
+  if (nblk < 1 || L.ntiles < 1) return SWR_OK;
+  HIP_CHECK(hipMemsetAsync(ctl, 0, sizeof(MixCtl), S->stream));
+  struct { MixCtl c; char pad[MS_OFF - sizeof(MixCtl)]; MixState m0; } h;
+  static_assert(offsetof(decltype(h), m0) == MS_OFF, "host copy mirrors the workspace");
+  int outer = 0;
+  for (;; outer++) {
     swr_kt_begin(S, KT_MIXED_OUTER);
-    hipLaunchKernelGGL(k_resid, dim3(nblk), dim3(QT), 0, S->stream, n, S->dp, S->type, S->H_firstnbr, S->H_numnbrs,
-                       S->H_jlist, S->H_val32, (const double2 *)x, b, S->Hdia_inv, rhs, S->partials, ps);
-    hipLaunchKernelGGL(k_resid_scalars, dim3(1), dim3(RT), 0, S->stream, nblk, S->partials, ps, ms, inner,
-                       S->ctl.tol * OUTER_TOL_FACTOR, outer == 0 ? 1 : 0, S->status, S->sys_iters);
+    if (batch)
+      hipLaunchKernelGGL(k_resid<true>, dim3(nblk), dim3(QT), 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
+                         S->H_numnbrs, S->H_jlist, S->H_val32, (const double2 *)x, b, S->Hdia_inv, rhs, S->partials,
+                         ps, S->sblk_beg, S->row_sys, ms, outer == 0 ? 1 : 0);
+    else
+      hipLaunchKernelGGL(k_resid<false>, dim3(nblk), dim3(QT), 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
+                         S->H_numnbrs, S->H_jlist, S->H_val32, (const double2 *)x, b, S->Hdia_inv, rhs, S->partials,
+                         ps, S->sblk_beg, S->row_sys, ms, outer == 0 ? 1 : 0);
+    hipLaunchKernelGGL(k_resid_scalars, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, S->sys_sblk, ctl, ms,
+                       inner, S->ctl.tol * OUTER_TOL_FACTOR, outer == 0 ? 1 : 0, S->status, S->sys_iters);
     swr_kt_end(S, KT_MIXED_OUTER);
     HIP_CHECK(hipGetLastError());
-    MixState h;
-    HIP_CHECK(hipMemcpyAsync(&h, ms, sizeof(h), hipMemcpyDeviceToHost, S->stream));
+    HIP_CHECK(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, S->stream));
     HIP_CHECK(hipStreamSynchronize(S->stream));
-    S->mixed_outer = h.outer;
-    if (outer == 0) { S->mixed_rho0[0] = h.rho[0]; S->mixed_rho0[1] = h.rho[1]; }
-    if (h.done) break;
+    if (outer == 0) { S->mixed_rho0[0] = h.m0.rho[0]; S->mixed_rho0[1] = h.m0.rho[1]; }
+    if (h.c.outer_ndone >= nsys || outer + 1 >= MAX_OUTER) break;
     // inner FP32 PCG from d = 0; iterations launched in batches sized from
-    // the last solve, the host checking the done flag between batches
+    // the last solve, the host checking the done count between batches
     swr_kt_begin(S, KT_MIXED_INNER);
-    hipLaunchKernelGGL(k_pcg_init, dim3(ublk), dim3(UT), 0, S->stream, n, rhs, S->Hdia_inv, dv, rv, zv, pv,
+    HIP_CHECK(hipMemsetAsync(&ctl->inner_ndone, 0, 2 * sizeof(int), S->stream));
+    hipLaunchKernelGGL(k_pcg_init, dim3(L.ntiles), dim3(UT), 0, S->stream, L, rhs, S->Hdia_inv, dv, rv, zv, pv,
                        S->partials, ps);
-    hipLaunchKernelGGL(k_pcg_init_scalars, dim3(1), dim3(RT), 0, S->stream, ublk, S->partials, ps, inner, pc);
+    hipLaunchKernelGGL(k_pcg_init_scalars, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, L, inner, pst, ctl);
     int it = 0;
-    int batch = std::max(2, S->mixed_last_inner + 1);
+    int nb = std::max(2, S->mixed_last_inner + 1);
     while (it < INNER_IMAX) {
-      for (int k = 0; k < batch && it < INNER_IMAX; k++, it++) {
-        hipLaunchKernelGGL(k_pcg_spmv, dim3(nblk), dim3(QT), 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
-                           S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, pc, S->partials, ps);
-        hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(RT), 0, S->stream, nblk, S->partials, ps, pc);
-        hipLaunchKernelGGL(k_pcg_update, dim3(ublk), dim3(UT), 0, S->stream, n, S->Hdia_inv, dv, rv, zv,
-                           (const float2 *)pv, (const float2 *)qv, pc, S->partials, ps);
-        hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(RT), 0, S->stream, ublk, S->partials, ps, pc, INNER_IMAX);
-        hipLaunchKernelGGL(k_pcg_p, dim3(ublk), dim3(UT), 0, S->stream, n, (const float2 *)zv, pv, pc);
+      for (int k = 0; k < nb && it < INNER_IMAX; k++, it++) {
+        if (batch)
+          hipLaunchKernelGGL(k_pcg_spmv<true>, dim3(nblk), dim3(QT), 0, S->stream, n, nsys, S->dp, S->type,
+                             S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
+                             S->partials, ps, S->sblk_beg, S->row_sys);
+        else
+          hipLaunchKernelGGL(k_pcg_spmv<false>, dim3(nblk), dim3(QT), 0, S->stream, n, nsys, S->dp, S->type,
+                             S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
+                             S->partials, ps, S->sblk_beg, S->row_sys);
+        hipLaunchKernelGGL(k_pcg_alpha, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, S->sys_sblk, pst);
+        hipLaunchKernelGGL(k_pcg_update, dim3(L.ntiles), dim3(UT), 0, S->stream, L, S->Hdia_inv, dv, rv, zv,
+                           (const float2 *)pv, (const float2 *)qv, pst, S->partials, ps);
+        hipLaunchKernelGGL(k_pcg_beta, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, L, pst, ctl, INNER_IMAX);
+        hipLaunchKernelGGL(k_pcg_p, dim3(L.ntiles), dim3(UT), 0, S->stream, L, (const float2 *)zv, pv, pst);
       }
       HIP_CHECK(hipGetLastError());
-      PcgCtl hc;
-      HIP_CHECK(hipMemcpyAsync(&hc, pc, sizeof(hc), hipMemcpyDeviceToHost, S->stream));
+      MixCtl hc;
+      HIP_CHECK(hipMemcpyAsync(&hc, ctl, sizeof(hc), hipMemcpyDeviceToHost, S->stream));
       HIP_CHECK(hipStreamSynchronize(S->stream));
-      if (hc.done) {
-        S->mixed_last_inner = std::max(hc.st[0].iter, hc.st[1].iter);
+      if (hc.inner_ndone >= nsys) {
+        S->mixed_last_inner = hc.inner_itmax;
         break;
       }
-      batch = 4;
+      nb = 4;
     }
-    hipLaunchKernelGGL(k_pcg_finish, dim3(1), dim3(64), 0, S->stream, pc, inner, S->status->inner_iters);
+    hipLaunchKernelGGL(k_pcg_finish, dim3(1), dim3(256), 0, S->stream, nsys, pst, inner, S->status->inner_iters);
     swr_kt_end(S, KT_MIXED_INNER);
     hipLaunchKernelGGL(k_mixed_update, dim3(div_up(n, 256)), dim3(256), 0, S->stream, n, x, (const float2 *)dv);
     HIP_CHECK(hipGetLastError());
   }
+  S->mixed_outer = outer + 1;
   return SWR_OK;
 }

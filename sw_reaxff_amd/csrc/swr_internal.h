@@ -224,6 +224,12 @@ struct swr_system {
   int *tile_beg = nullptr;         // [ntiles + 1]
   int *tile_sys = nullptr;         // [ntiles]
   int *sys_tile = nullptr;         // [nsys + 1]
+  // half-wave-per-row kernels over a batch: block b runs rows
+  // [sblk_beg[b], sblk_beg[b+1]) (at most 8, one system), system s owns
+  // blocks [sys_sblk[s], sys_sblk[s+1])
+  int *sblk_beg = nullptr;         // [nsblk + 1]
+  int *sys_sblk = nullptr;         // [nsys + 1]
+  int nsblk = 0;
   int *sys_iters = nullptr;        // [2 nsys] iterations of s, t per system (device)
   std::vector<int> h_sys_iters;    // host copy after each solve
   int *done_count = nullptr;       // systems finished in the running solve

@@ -210,7 +210,7 @@ class Server:
 
     # ---- workers
     def _worker(self, lane):
-        bt = self.batches[lane] or Batch(ffield=self.ffield, elements=ELEMENTS, tol=self.tol)
+        bt = self.batches[lane] or Batch(ffield=self.ffield, elements=ELEMENTS, tol=self.tol, mixed=self.mixed)
         self.batches[lane] = bt
         try:
             while True:

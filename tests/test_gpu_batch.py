@@ -86,6 +86,41 @@ def test_alone_equals_in_the_mix(gpu_lib, tol, qtol):
         assert dg <= 1000 * qtol, k
 
 
+def test_mixed_batch(gpu_lib):
+    """Mixed-precision QEq over a batch (per-system outer and inner scalars):
+    every system's charges lie within 1e-6 e of the FP64 batch solved to
+    1e-10, and equal its charges alone within 1e-7 e (alone, the single-
+    system kernels run; the FP32 rounding of a translated H entry can flip
+    its last bit), with residual + inner iteration counts within 3."""
+    parts = mix()
+    ref = B.Batch(tol=1e-10)
+    ref.load(parts)
+    ref.evaluate()
+    ref.sync()
+    q_ref = [q.copy() for q in ref.charges()]
+    bt = B.Batch(tol=1e-6, mixed=True)
+    bt.load(parts)
+    bt.evaluate()
+    bt.sync()
+    q_mix = bt.charges()
+    it_mix = bt.iterations()
+    for k, s in enumerate(parts):
+        alone = B.Batch(tol=1e-6, mixed=True)
+        alone.load([s])
+        alone.evaluate()
+        alone.sync()
+        q1 = alone.charges()[0]
+        it1 = alone.iterations()[0]
+        da = float(np.max(np.abs(q1 - q_mix[k])))
+        dr = float(np.max(np.abs(q_mix[k] - q_ref[k])))
+        print(f"[batch mixed] system {k} n={s.n} periodic={s.periodic} matvecs alone {tuple(it1)} mix "
+              f"{tuple(it_mix[k])} max|dq| alone-mix {da:.2e} vs FP64(1e-10) {dr:.2e}", flush=True)
+        assert np.all(np.abs(np.asarray(it1) - np.asarray(it_mix[k])) <= 3), k
+        assert da <= 1e-7, k
+        assert dr <= 1e-6, k
+        assert abs(q_mix[k].sum()) < 1e-8, k
+
+
 def test_batch_cluster_against_oracle(gpu_lib, oracle_lib, ffield):
     parts = mix()
     bt = B.Batch(tol=1e-10)

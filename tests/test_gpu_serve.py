@@ -14,7 +14,9 @@ agree with the lone run within 1e-5 e at tol 1e-6 and within 1e-8 e at tol
 1e-10.  A resident's first evaluation runs in a batch like any new system;
 its device system, set up at its first follow-on step from the s/t
 histories that batched solve left, is replayed from the same histories and
-stepped the same way, so its step charges agree to 1e-12 e.
+stepped the same way, so its step charges agree to 1e-12 e.  With
+mixed-precision QEq (per-system FP64 refinement to tol/100 around the FP32
+inner solve) served and lone charges agree within 1e-7 e.
 """
 import numpy as np
 import pytest
@@ -25,16 +27,16 @@ from sw_reaxff_amd.batch import Batch
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("tol,qtol", [(1e-6, 1e-5), (1e-10, 1e-8)])
-def test_served_results_equal_alone(gpu_lib, ffield, tol, qtol):
+@pytest.mark.parametrize("tol,qtol,mixed", [(1e-6, 1e-5, False), (1e-10, 1e-8, False), (1e-6, 1e-7, True)])
+def test_served_results_equal_alone(gpu_lib, ffield, tol, qtol, mixed):
     # arrivals faster than one evaluation, so batches of several systems form
     reqs = serve.make_requests(30, rate=5000.0, seed=11, lo=20, hi=1500, frac_step=0.3, frac_resident=0.25)
-    srv = serve.Server(lanes=2, hbm_budget=200e9, ffield=ffield, tol=tol)
+    srv = serve.Server(lanes=2, hbm_budget=200e9, ffield=ffield, tol=tol, mixed=mixed)
     srv.run(reqs)
     assert all(r.q is not None for r in reqs)
     assert any(r.kind == "step" for r in reqs) and any(r.batch_size > 1 for r in reqs)
     print(f"[serve] {serve.summary(reqs, max(r.done for r in reqs))}", flush=True)
-    alone = Batch(ffield=ffield, tol=tol)
+    alone = Batch(ffield=ffield, tol=tol, mixed=mixed)
     worst = 0.0
     for r in reqs:
         if r.kind == "new":
@@ -54,7 +56,7 @@ def test_served_results_equal_alone(gpu_lib, ffield, tol, qtol):
         first = next(r for r in reqs if r.kind == "new" and r.resident == rid)
         steps = sorted((r for r in reqs if r.kind == "step" and r.resident == rid), key=lambda r: r.step_index)
         assert first.hist is not None
-        res = serve._Resident(first.system, ffield, tol, False, hist=first.hist)
+        res = serve._Resident(first.system, ffield, tol, mixed, hist=first.hist)
         for k, r in enumerate(steps, 1):
             assert r.step_index == k
             assert np.max(np.abs(res.step() - r.q)) <= 1e-12, (rid, k)
