@@ -299,16 +299,17 @@ def bench_serve(args):
     from sw_reaxff_amd.batch import Batch
     mixed = not args.fp64
     lanes = [Batch(tol=args.tol, mixed=mixed) for _ in range(args.lanes)]
+    serve.warm_lanes(lanes, args.max_batch_atoms)
     warm = serve.make_requests(24, rate=1e4, seed=args.seed + 1, hi=3000)
     serve.Server(lanes=args.lanes, max_batch_atoms=args.max_batch_atoms, batches=lanes, tol=args.tol,
-                 mixed=mixed).run(warm)
+                 mixed=mixed, resident=args.resident).run(warm)
     runs = []
     base = serve.make_requests(args.requests, rate=1.0, seed=args.seed)   # systems generated once
     for rate in args.rates:
         reqs = serve.at_rate(base, rate)
         t0 = time.perf_counter()
         srv = serve.Server(lanes=args.lanes, max_batch_atoms=args.max_batch_atoms, batches=lanes, tol=args.tol,
-                           mixed=mixed)
+                           mixed=mixed, resident=args.resident)
         srv.run(reqs)
         wall = time.perf_counter() - t0
         summ = serve.summary(reqs, wall)
@@ -320,7 +321,8 @@ def bench_serve(args):
     # plain loop: one worker, batches of one, every request queued at t = 0
     reqs = serve.at_rate(base[:args.plain_sample], 1e9)
     t0 = time.perf_counter()
-    serve.Server(lanes=1, max_batch_atoms=1, batches=[lanes[0]], tol=args.tol, mixed=mixed).run(reqs)
+    serve.Server(lanes=1, max_batch_atoms=1, batches=[lanes[0]], tol=args.tol, mixed=mixed,
+                 resident=args.resident).run(reqs)
     wall = time.perf_counter() - t0
     plain = len(reqs) / wall
     log(f"plain loop: {plain:.1f} systems/s ({len(reqs)} requests)")
@@ -336,6 +338,7 @@ def bench_serve(args):
                                f"resident s/t, 200 GB HBM admission, "
                                f"{'mixed-precision ' if mixed else 'FP64 '}QEq tol {args.tol:g}",
                    "lanes": args.lanes, "max_batch_atoms": args.max_batch_atoms, "seed": args.seed,
+                   "residents": args.resident,
                    "parallelism": "single GPU"},
         "best_rate": best["rate"], "p99_within_50ms": bool(ok), "latency_ms": best["latency_ms"],
         "runs": runs, "plain_loop_systems_per_s": round(plain, 1),
@@ -359,6 +362,9 @@ def main():
                     help="clusters timed one at a time for the plain-loop comparison")
     ap.add_argument("--mixed", action="store_true", help="mixed-precision QEq")
     ap.add_argument("--fp64", action="store_true", help="serve: FP64 QEq instead of the mixed-precision form")
+    ap.add_argument("--resident", default="batched", choices=["batched", "device"],
+                    help="serve: follow-on steps in the batches (warm start in device memory) or on each "
+                         "resident's own device system")
     ap.add_argument("--tol", type=float, default=1e-6, help="QEq tolerance (the reference's: 1e-6)")
     ap.add_argument("--error-steps", type=int, default=200,
                     help="mixed: steps of the side-by-side comparison with an FP64 run (0: skip)")

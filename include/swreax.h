@@ -187,8 +187,28 @@ int swr_pair_compute(swr_system *sys);
  * system's charges are those it gets alone.  Per-system iteration counts:
  * swr_get(sys, "sys_iters", ...) (2 ints per system: s, t).  Reset to one
  * system whenever swr_set_atoms changes the local atom count.  The
- * mixed-precision solve takes one system at a time. */
+ * mixed-precision solve keeps per-system refinement and inner-solve state
+ * the same way. */
 int swr_set_batch(swr_system *sys, int nsys, const int *row_offsets);
+
+/* Device-resident QEq warm start of one system between evaluations (no
+ * reference counterpart: fix qeq/reax keeps s_hist/t_hist in its own arrays
+ * across the steps of one run, fix_qeq_reax_sunway.cpp:711-714; a serving
+ * loop evaluates a resident system's successive steps in different batches).
+ * swr_hist_save copies the s and t histories (5 per row each) of local rows
+ * [row0, row0+n) into *store, a device buffer that outlives the batch
+ * (allocated when *store is NULL or smaller than n rows, else reused);
+ * fill != 0 sets all five slots of each row to slot 0 (the solution just
+ * reached), so the extrapolation of the next solve starts from it.
+ * swr_hist_load copies a store's n rows into rows [row0, row0+n) of sys
+ * (after swr_set_atoms/swr_new_atoms, before the solve).  Both enqueue on
+ * sys's stream only; a store is read by a load on another system's stream
+ * only after the saving system was synchronised.  swr_hist_free releases a
+ * store no call in flight uses. */
+typedef struct swr_hist_store swr_hist_store;
+int swr_hist_save(swr_system *sys, int row0, int n, int fill, swr_hist_store **store);
+int swr_hist_load(swr_system *sys, int row0, int n, const swr_hist_store *store);
+int swr_hist_free(swr_hist_store *store);
 
 /* Non-bonded pass (vdW_Coulomb_Energy_Full_C, reaxc_nonbonded_sw64.c:260-500;
  * host form :522-830; the vdw_type 1 CPE kernel reaxc_nonbonded_cpe.h:1-420):

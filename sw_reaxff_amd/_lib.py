@@ -29,7 +29,8 @@ EXPORTS = [
     "swr_lmp_setup", "swr_lmp_run",
     "swr_lmp_system", "swr_lmp_stat", "swr_ktimer_enable", "swr_ktimer_read", "swr_device_synchronize",
     "swr_sparse_matvec_h", "swr_npair_full_bin_atomonly_build", "swr_sort_bonds", "swr_set_batch",
-    "swr_new_atoms", "swr_batch_pack", "swr_vdw_coulomb_energy",
+    "swr_new_atoms", "swr_batch_pack", "swr_vdw_coulomb_energy", "swr_hist_save", "swr_hist_load",
+    "swr_hist_free",
 ]
 
 
@@ -90,6 +91,9 @@ def lib():
         L.swr_lmp_create_atoms.argtypes = [vp, C.c_int, dp, dp, C.c_int, dp, dp, ip, ip]
         L.swr_lmp_set_history.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p]
         L.swr_batch_pack.argtypes = [C.c_int] + [C.c_void_p] * 7 + [C.c_double, C.c_longlong] + [C.c_void_p] * 9
+        L.swr_hist_save.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+        L.swr_hist_load.argtypes = [vp, C.c_int, C.c_int, vp]
+        L.swr_hist_free.argtypes = [vp]
         L.swr_lmp_setup.argtypes = [vp]
         L.swr_lmp_run.argtypes = [vp, C.c_int]
         L.swr_lmp_system.restype = vp

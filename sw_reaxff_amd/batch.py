@@ -101,6 +101,20 @@ def pack(batch: list[systems.System], elements=ELEMENTS, cutghost: float = CUTGH
     return Packed(x=xo, type=to, tag=go, owner=oo, img=io, n=n, row_off=row_off, ghost_off=ghost_off, shift=shift)
 
 
+class HistStore:
+    """A system's QEq warm start (s/t histories, 5 per atom each) kept in device
+    memory between evaluations in different batches (swr_hist_save/load)."""
+
+    def __init__(self):
+        self.h = C.c_void_p(None)
+        self.n = 0
+
+    def __del__(self):
+        if self.h.value and _lib is not None:
+            _lib.lib().swr_hist_free(self.h)
+            self.h = C.c_void_p(None)
+
+
 class Batch:
     """One device system holding a batch; evaluate() runs nbr + QEq + lists + BO (+ the
     non-bonded pass) once for all."""
@@ -111,16 +125,35 @@ class Batch:
         self.packed = None
         self._bufs = {}
 
-    def load(self, batch: list[systems.System], q=None):
-        """New systems: QEq starts from zero history and list capacities are estimated afresh."""
+    def load(self, batch: list[systems.System], q=None, hist=None):
+        """New atoms: list capacities are estimated afresh and QEq starts from
+        zero history, except for systems k with a HistStore hist[k] (their
+        solve starts from its extrapolation)."""
         P = pack(batch, self.elements, bufs=self._bufs)
         h = self.sys
         h.set_atoms(P.n, P.x, P.type, P.tag, owner=P.owner, img=P.img, q=q)
         _lib.check(h.L.swr_new_atoms(h.h), "swr_new_atoms")
         off = np.ascontiguousarray(P.row_off, dtype=np.int32)
         _lib.check(h.L.swr_set_batch(h.h, len(batch), off.ctypes.data_as(C.POINTER(C.c_int))), "swr_set_batch")
+        for k, st in enumerate(hist or []):
+            if st is not None:
+                assert st.n == batch[k].n, "warm start of another system"
+                _lib.check(h.L.swr_hist_load(h.h, int(P.row_off[k]), st.n, st.h), "swr_hist_load")
         self.packed = P
         return P
+
+    def save_history(self, k, store=None, fill=False):
+        """Copies system k's s/t histories after the solve into a HistStore
+        (reused when given); fill sets all five slots to the solution (a
+        first evaluation, whose history holds it in slot 0 only).  Enqueues
+        only: sync() before another batch loads it."""
+        store = store or HistStore()
+        P = self.packed
+        n = int(P.row_off[k + 1] - P.row_off[k])
+        _lib.check(self.sys.L.swr_hist_save(self.sys.h, int(P.row_off[k]), n, 1 if fill else 0, C.byref(store.h)),
+                   "swr_hist_save")
+        store.n = n
+        return store
 
     def evaluate(self, nonbonded=False):
         """Neighbour list, QEq (per-system), far/bond/hbond lists, BO and, with

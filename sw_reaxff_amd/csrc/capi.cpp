@@ -267,7 +267,7 @@ int swr_create(const char *ffield_path, int ntypes, const char *const *elements,
   S->imax = S->ctl.qeq_imax > 0 ? S->ctl.qeq_imax : 200;
   fill_params(S, map);
   hipError_t e;
-  if ((e = hipGetDevice(&S->device)) != hipSuccess || (e = hipStreamCreate(&S->stream)) != hipSuccess) {
+  if ((e = hipGetDevice(&S->device)) != hipSuccess || (e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking)) != hipSuccess) {
     swr_set_error_msg(std::string("HIP init: ") + hipGetErrorString(e));
     delete S;
     return SWR_ERR_HIP;
@@ -283,9 +283,15 @@ int swr_create(const char *ffield_path, int ntypes, const char *const *elements,
     return SWR_ERR_NOMEM;
   }
   std::memset(S->status_host, 0, sizeof(DevStatus));
-  (void)hipMemcpy(S->dp, &S->hp, sizeof(DevParams), hipMemcpyHostToDevice);
-  (void)hipMemset(S->status, 0, sizeof(DevStatus));
-  (void)hipMemset(S->counters, 0, 4 * sizeof(unsigned int));
+  // the system's stream does not wait for the null stream: everything goes on it
+  if (hipMemcpyAsync(S->dp, &S->hp, sizeof(DevParams), hipMemcpyHostToDevice, S->stream) != hipSuccess ||
+      hipMemsetAsync(S->status, 0, sizeof(DevStatus), S->stream) != hipSuccess ||
+      hipMemsetAsync(S->counters, 0, 4 * sizeof(unsigned int), S->stream) != hipSuccess ||
+      hipStreamSynchronize(S->stream) != hipSuccess) {
+    swr_set_error_msg("HIP init: parameter upload failed");
+    swr_destroy(S);
+    return SWR_ERR_HIP;
+  }
   *out = S;
   return SWR_OK;
 }
@@ -373,8 +379,9 @@ int swr_new_atoms(swr_system *S) {
 int swr_set_capacities(swr_system *S, const int *num_bonds, const int *num_hbonds) {
   if (!S || !num_bonds || !num_hbonds) return SWR_ERR_ARG;
   if (int rc = swr_quiesce(S)) return rc;
-  HIP_CHECK(hipMemcpy(S->num_bonds, num_bonds, sizeof(int) * S->N, hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(S->num_hbonds, num_hbonds, sizeof(int) * S->N, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpyAsync(S->num_bonds, num_bonds, sizeof(int) * S->N, hipMemcpyHostToDevice, S->stream));
+  HIP_CHECK(hipMemcpyAsync(S->num_hbonds, num_hbonds, sizeof(int) * S->N, hipMemcpyHostToDevice, S->stream));
+  HIP_CHECK(hipStreamSynchronize(S->stream));
   S->capacities_set = true;
   S->next_totals_known = false;
   return SWR_OK;
@@ -386,13 +393,11 @@ static int nbr_build_common(swr_system *S, double cutneigh, int local_only) {
     double cutmax = std::max(std::max(S->ff.nonb_cut, S->ctl.hbond_cut), 2 * S->ctl.bond_cut);
     cutneigh = cutmax + S->ctl.skin;
   }
-  // bin grid over the atoms' bounding box (host copy of x for the extent)
-  std::vector<double> xh(3 * (size_t)S->N);
-  HIP_CHECK(hipMemcpy(xh.data(), S->x, xh.size() * sizeof(double), hipMemcpyDeviceToHost));
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (int i = 0; i < S->N; i++)
-    for (int d = 0; d < 3; d++) { lo[d] = std::min(lo[d], xh[3 * i + d]); hi[d] = std::max(hi[d], xh[3 * i + d]); }
-  int rc = swr_nbr_set_grid(S, lo, hi, 0.0, cutneigh);
+  // bin grid over the atoms' bounding box (reduced on the device)
+  double lo[3], hi[3];
+  int rc = swr_nbr_bbox(S, lo, hi);
+  if (rc) return rc;
+  rc = swr_nbr_set_grid(S, lo, hi, 0.0, cutneigh);
   if (rc) return rc;
   return swr_nbr_build(S, local_only);
 }
@@ -500,7 +505,10 @@ int swr_bo(swr_system *S) {
 int swr_qeq_init_matvec(swr_system *S) {
   if (!S || !S->vlist || S->n <= 0) return SWR_ERR_STATE;
   long long need = S->vtotal_local;
-  if (need < 0) HIP_CHECK(hipMemcpy(&need, S->voff + S->n, sizeof(long long), hipMemcpyDeviceToHost));
+  if (need < 0) {
+    HIP_CHECK(hipMemcpyAsync(&need, S->voff + S->n, sizeof(long long), hipMemcpyDeviceToHost, S->stream));
+    HIP_CHECK(hipStreamSynchronize(S->stream));
+  }
   int rc;
   if (need > S->Hcap) {
     long long cap = (long long)(need * 1.15) + 1024;
@@ -694,7 +702,10 @@ int64_t swr_get(swr_system *S, const char *name, void *out) {
     int64_t cnt = S->hbond_cap;
     if (!out) return 8 * cnt;
     std::vector<swr_hbond_t> hb(cnt);
-    if (cnt) HIP_CHECK(hipMemcpy(hb.data(), S->hbonds, sizeof(swr_hbond_t) * cnt, hipMemcpyDeviceToHost));
+    if (cnt) {
+      HIP_CHECK(hipMemcpyAsync(hb.data(), S->hbonds, sizeof(swr_hbond_t) * cnt, hipMemcpyDeviceToHost, S->stream));
+      HIP_CHECK(hipStreamSynchronize(S->stream));
+    }
     int64_t *o = (int64_t *)out;
     for (int64_t i = 0; i < cnt; i++) {
       intptr_t off = (intptr_t)hb[i].ptr - (intptr_t)S->far;
@@ -710,11 +721,13 @@ int64_t swr_get(swr_system *S, const char *name, void *out) {
   if (v.stride_kind == 3) {  // host-side table
     std::memcpy(out, v.ptr, v.bytes);
   } else if (v.stride_kind == 0) {
-    HIP_CHECK(hipMemcpy(out, v.ptr, v.bytes, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpyAsync(out, v.ptr, v.bytes, hipMemcpyDeviceToHost, S->stream));
+    HIP_CHECK(hipStreamSynchronize(S->stream));
   } else {
     int64_t cnt = v.bytes / 8;
     std::vector<double> tmp(2 * cnt);
-    HIP_CHECK(hipMemcpy(tmp.data(), v.ptr, 16 * cnt, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpyAsync(tmp.data(), v.ptr, 16 * cnt, hipMemcpyDeviceToHost, S->stream));
+    HIP_CHECK(hipStreamSynchronize(S->stream));
     double *o = (double *)out;
     for (int64_t i = 0; i < cnt; i++) o[i] = tmp[2 * i + (v.stride_kind - 1)];
   }
@@ -728,7 +741,8 @@ int swr_put(swr_system *S, const char *name, const void *in, int64_t bytes) {
   if (!lookup(S, k, v) || bytes > v.bytes || v.stride_kind != 0) return SWR_ERR_ARG;
   if (int rc = swr_quiesce(S)) return rc;
   HIP_CHECK(hipStreamSynchronize(S->stream));
-  HIP_CHECK(hipMemcpy((void *)v.ptr, in, bytes, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpyAsync((void *)v.ptr, in, bytes, hipMemcpyHostToDevice, S->stream));
+  HIP_CHECK(hipStreamSynchronize(S->stream));
   if (k == "num_bonds") S->capacities_set = true;
   S->next_totals_known = false;
   return SWR_OK;

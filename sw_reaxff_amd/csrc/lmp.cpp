@@ -215,7 +215,8 @@ struct swr_lmp {
     for (int t : type0)
       if (t < 1 || t > (int)el.size()) { swr_set_error_msg("atom type without pair_coeff element"); return SWR_ERR_ARG; }
     if ((rc = swr_set_atoms(sys, n, n, tag0.data(), type0.data(), x0.data(), nullptr, nullptr, nullptr))) return rc;
-    HIP_CHECK(hipMemcpy(sys->v, v0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpyAsync(sys->v, v0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, sys->stream));
+    HIP_CHECK(hipStreamSynchronize(sys->stream));
     if (!s_hist0.empty()) {
       if ((rc = swr_put(sys, "s_hist", s_hist0.data(), (int64_t)(sizeof(double) * s_hist0.size()))) ||
           (rc = swr_put(sys, "t_hist", t_hist0.data(), (int64_t)(sizeof(double) * t_hist0.size()))))

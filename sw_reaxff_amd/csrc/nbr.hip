@@ -230,6 +230,51 @@ static std::vector<int4> stencil_rows(const double bs[3], double cut) {
 }
 
 // ---------------------------------------------------------------- host API
+namespace {
+constexpr int BBOX_BLOCKS = 64;
+
+// Per-block min and max of the atoms' coordinates (grid-stride over N).
+__global__ void __launch_bounds__(256) k_bbox(const double *__restrict__ x, int N, double *__restrict__ part) {
+  __shared__ double s[6][256];
+  double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256)
+    for (int d = 0; d < 3; d++) {
+      const double c = x[3 * (size_t)i + d];
+      v[d] = fmin(v[d], c);
+      v[3 + d] = fmax(v[3 + d], c);
+    }
+  for (int k = 0; k < 6; k++) s[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int k = 0; k < 6; k++)
+        s[k][threadIdx.x] = k < 3 ? fmin(s[k][threadIdx.x], s[k][threadIdx.x + o])
+                                  : fmax(s[k][threadIdx.x], s[k][threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+}  // namespace
+
+// Bounding box of all N atoms (lo, hi on the host).
+int swr_nbr_bbox(swr_system *S, double lo[3], double hi[3]) {
+  int rc = swr_ensure_tmp(S, sizeof(double) * 6 * BBOX_BLOCKS);
+  if (rc) return rc;
+  double *part = (double *)S->tmp_i;
+  double h[6 * BBOX_BLOCKS];
+  hipLaunchKernelGGL(k_bbox, dim3(BBOX_BLOCKS), dim3(256), 0, S->stream, S->x, S->N, part);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, S->stream));
+  HIP_CHECK(hipStreamSynchronize(S->stream));
+  for (int d = 0; d < 3; d++) { lo[d] = 1e300; hi[d] = -1e300; }
+  for (int b = 0; b < BBOX_BLOCKS; b++)
+    for (int d = 0; d < 3; d++) {
+      lo[d] = std::min(lo[d], h[6 * b + d]);
+      hi[d] = std::max(hi[d], h[6 * b + 3 + d]);
+    }
+  return SWR_OK;
+}
+
 int swr_nbr_set_grid(swr_system *S, const double lo[3], const double hi[3], double cutghost, double cutneigh) {
   // Grid covering [lo - cutghost, hi + cutghost] with bins of ~cutneigh/2.
   const double target = 0.5 * cutneigh;
@@ -253,9 +298,11 @@ int swr_nbr_set_grid(swr_system *S, const double lo[3], const double hi[3], doub
   S->nbins = nb;
   std::vector<int4> rows = stencil_rows(S->binsize, cutneigh);
   S->nrows = (int)rows.size();
-  int rc = swr_alloc(&S->stencil_rows, rows.size());
+  int rc = swr_reserve(&S->stencil_rows, S->cap_stencil, rows.size());
   if (rc) return rc;
-  HIP_CHECK(hipMemcpy(S->stencil_rows, rows.data(), rows.size() * sizeof(int4), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpyAsync(S->stencil_rows, rows.data(), rows.size() * sizeof(int4), hipMemcpyHostToDevice,
+                           S->stream));
+  HIP_CHECK(hipStreamSynchronize(S->stream));  // rows is a local
   return SWR_OK;
 }
 
@@ -334,7 +381,8 @@ int swr_nbr_build(swr_system *S, int local_only) {
   const int N = S->N;
   const int nrow = local_only ? S->n : N;
   int rc;
-  if ((rc = swr_alloc(&S->bin_of, N)) || (rc = swr_alloc(&S->bp_id, N)) || (rc = swr_alloc(&S->bp_pos, N)))
+  if ((rc = swr_reserve(&S->bin_of, S->cap_bin_of, N)) || (rc = swr_reserve(&S->bp_id, S->cap_bp_id, N)) ||
+      (rc = swr_reserve(&S->bp_pos, S->cap_bp_pos, N)))
     return rc;
   swr_kt_begin(S, KT_BINS);
   HIP_CHECK(hipMemsetAsync(S->bin_count, 0, sizeof(int) * (S->nbins + 1), S->stream));
@@ -348,7 +396,8 @@ int swr_nbr_build(swr_system *S, int local_only) {
   hipLaunchKernelGGL(k_bin_sort, dim3(div_up(S->nbins, 128)), dim3(128), 0, S->stream, S->nbins, S->bin_start,
                      S->bp_id, S->x, S->bp_pos);
   swr_kt_end(S, KT_BINS);
-  if ((rc = swr_alloc(&S->vcount, N)) || (rc = swr_alloc(&S->voff, (size_t)N + 1))) return rc;
+  if ((rc = swr_reserve(&S->vcount, S->cap_vcount, N)) || (rc = swr_reserve(&S->voff, S->cap_voff, (size_t)N + 1)))
+    return rc;
   const double cutsq = S->cutneigh * S->cutneigh;
   const int nblk = div_up(N, WAVES_PER_BLOCK);
   swr_kt_begin(S, KT_VERLET);
@@ -366,7 +415,8 @@ int swr_nbr_build(swr_system *S, int local_only) {
   }
   S->vtotal = total;
   // local rows' share (the H matrix capacity), read once per build
-  HIP_CHECK(hipMemcpy(&S->vtotal_local, S->voff + S->n, sizeof(long long), hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpyAsync(&S->vtotal_local, S->voff + S->n, sizeof(long long), hipMemcpyDeviceToHost, S->stream));
+  HIP_CHECK(hipStreamSynchronize(S->stream));
   swr_kt_begin(S, KT_VERLET);
   hipLaunchKernelGGL(k_verlet<true>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->x, N, S->bin_of, S->bin_start,
                      S->bp_pos, S->bp_id, S->stencil_rows, S->nrows, S->nbin[0], S->nbin[1], S->nbin[2], cutsq,

@@ -236,11 +236,15 @@ int swr_qeq_layout(swr_system *S) {
   }
   const int ntiles = (int)tile_sys.size();
   int rc;
-  if ((rc = swr_alloc(&S->row_sys, row_sys.size())) || (rc = swr_alloc(&S->tile_beg, tile_beg.size())) ||
-      (rc = swr_alloc(&S->tile_sys, tile_sys.size())) || (rc = swr_alloc(&S->sys_tile, sys_tile.size())) ||
-      (rc = swr_alloc(&S->sblk_beg, sblk_beg.size())) || (rc = swr_alloc(&S->sys_sblk, sys_sblk.size())) ||
-      (rc = swr_alloc(&S->sys_iters, 2 * (size_t)nsys)) || (rc = swr_alloc(&S->cgs, 4 * (size_t)nsys)) ||
-      (rc = swr_alloc(&S->qsum, 3 * (size_t)nsys)))
+  if ((rc = swr_reserve(&S->row_sys, S->cap_row_sys, row_sys.size())) ||
+      (rc = swr_reserve(&S->tile_beg, S->cap_tile_beg, tile_beg.size())) ||
+      (rc = swr_reserve(&S->tile_sys, S->cap_tile_sys, tile_sys.size())) ||
+      (rc = swr_reserve(&S->sys_tile, S->cap_sys_tile, sys_tile.size())) ||
+      (rc = swr_reserve(&S->sblk_beg, S->cap_sblk_beg, sblk_beg.size())) ||
+      (rc = swr_reserve(&S->sys_sblk, S->cap_sys_sblk, sys_sblk.size())) ||
+      (rc = swr_reserve(&S->sys_iters, S->cap_sys_iters, 2 * (size_t)nsys)) ||
+      (rc = swr_reserve(&S->cgs, S->cap_cgs, 4 * (size_t)nsys)) ||
+      (rc = swr_reserve(&S->qsum, S->cap_qsum, 3 * (size_t)nsys)))
     return rc;
   HIP_CHECK(hipMemcpyAsync(S->row_sys, row_sys.data(), sizeof(int) * row_sys.size(), hipMemcpyHostToDevice, S->stream));
   HIP_CHECK(hipMemcpyAsync(S->tile_beg, tile_beg.data(), sizeof(int) * tile_beg.size(), hipMemcpyHostToDevice,
@@ -317,5 +321,72 @@ int swr_qeq_calc_q(swr_system *S) {
                      S->qsum, S->owner, S->row_sys, S->q, S->s_hist, S->t_hist);
   swr_kt_end(S, KT_QEQ_Q);
   HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+// ---------------------------------------------------------------- warm-start store
+struct swr_hist_store {
+  double *dev = nullptr;   // [2][cap][5]: s rows, then t rows
+  int cap = 0;
+};
+
+namespace {
+// slots 1..4 of each history row = slot 0 (rows of s and t alike)
+__global__ void k_hist_fill(int rows, double *__restrict__ h) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  double *r = h + 5 * (size_t)i;
+  const double v = r[0];
+  r[1] = v; r[2] = v; r[3] = v; r[4] = v;
+}
+}  // namespace
+
+extern "C" int swr_hist_save(swr_system *S, int row0, int n, int fill, swr_hist_store **store) {
+  if (!S || !store || row0 < 0 || n < 0 || row0 + n > S->n) {
+    swr_set_error_msg("swr_hist_save: rows out of range");
+    return SWR_ERR_ARG;
+  }
+  swr_hist_store *h = *store;
+  if (!h) *store = h = new swr_hist_store();
+  if (h->cap < n) {
+    if (h->dev) HIP_CHECK(hipFree(h->dev));
+    h->dev = nullptr;
+    h->cap = 0;
+    if (hipMalloc(&h->dev, sizeof(double) * 10 * (size_t)n) != hipSuccess) {
+      swr_set_error_msg("swr_hist_save: hipMalloc failed");
+      return SWR_ERR_NOMEM;
+    }
+    h->cap = n;
+  }
+  if (n == 0) return SWR_OK;
+  const size_t bytes = sizeof(double) * 5 * (size_t)n;
+  HIP_CHECK(hipMemcpyAsync(h->dev, S->s_hist + 5 * (size_t)row0, bytes, hipMemcpyDeviceToDevice, S->stream));
+  HIP_CHECK(hipMemcpyAsync(h->dev + 5 * (size_t)h->cap, S->t_hist + 5 * (size_t)row0, bytes,
+                           hipMemcpyDeviceToDevice, S->stream));
+  if (fill) {
+    hipLaunchKernelGGL(k_hist_fill, dim3(div_up(n, 256)), dim3(256), 0, S->stream, n, h->dev);
+    hipLaunchKernelGGL(k_hist_fill, dim3(div_up(n, 256)), dim3(256), 0, S->stream, n, h->dev + 5 * (size_t)h->cap);
+    HIP_CHECK(hipGetLastError());
+  }
+  return SWR_OK;
+}
+
+extern "C" int swr_hist_load(swr_system *S, int row0, int n, const swr_hist_store *h) {
+  if (!S || !h || row0 < 0 || n < 0 || row0 + n > S->n || n > h->cap) {
+    swr_set_error_msg("swr_hist_load: rows out of range or store too small");
+    return SWR_ERR_ARG;
+  }
+  if (n == 0) return SWR_OK;
+  const size_t bytes = sizeof(double) * 5 * (size_t)n;
+  HIP_CHECK(hipMemcpyAsync(S->s_hist + 5 * (size_t)row0, h->dev, bytes, hipMemcpyDeviceToDevice, S->stream));
+  HIP_CHECK(hipMemcpyAsync(S->t_hist + 5 * (size_t)row0, h->dev + 5 * (size_t)h->cap, bytes,
+                           hipMemcpyDeviceToDevice, S->stream));
+  return SWR_OK;
+}
+
+extern "C" int swr_hist_free(swr_hist_store *h) {
+  if (!h) return SWR_OK;
+  if (h->dev) (void)hipFree(h->dev);
+  delete h;
   return SWR_OK;
 }

@@ -152,6 +152,7 @@ struct swr_system {
   int nbin[3] = {0, 0, 0};
   double binlo[3] = {0, 0, 0}, binsize[3] = {0, 0, 0}, bininv[3] = {0, 0, 0};
   int nbins = 0, binscap = 0;
+  size_t cap_stencil = 0, cap_bin_of = 0, cap_bp_id = 0, cap_bp_pos = 0, cap_vcount = 0, cap_voff = 0;
   int *bin_of = nullptr, *bin_count = nullptr, *bin_start = nullptr, *bin_fill = nullptr;
   double4 *bp_pos = nullptr;     // atoms sorted by bin: x, y, z, unused
   int *bp_id = nullptr;
@@ -230,6 +231,9 @@ struct swr_system {
   int *sblk_beg = nullptr;         // [nsblk + 1]
   int *sys_sblk = nullptr;         // [nsys + 1]
   int nsblk = 0;
+  // capacities of the layout arrays (grown, never shrunk)
+  size_t cap_row_sys = 0, cap_tile_beg = 0, cap_tile_sys = 0, cap_sys_tile = 0, cap_sblk_beg = 0,
+         cap_sys_sblk = 0, cap_sys_iters = 0, cap_cgs = 0, cap_qsum = 0;
   int *sys_iters = nullptr;        // [2 nsys] iterations of s, t per system (device)
   std::vector<int> h_sys_iters;    // host copy after each solve
   int *done_count = nullptr;       // systems finished in the running solve
@@ -284,6 +288,7 @@ int swr_ensure_tmp(swr_system *S, size_t bytes);
 
 // Stage implementations (nbr.hip, lists.hip, bo.hip, qeq.hip, qeq_mixed.hip).
 int swr_nbr_set_grid(swr_system *S, const double lo[3], const double hi[3], double cutghost, double cutneigh);
+int swr_nbr_bbox(swr_system *S, double lo[3], double hi[3]);
 int swr_nbr_build(swr_system *S, int local_only);
 int swr_nbr_borders(swr_system *S, const double lo[3], const double hi[3], double cutghost);
 int swr_nbr_forward_x(swr_system *S, const double lo[3], const double hi[3]);
@@ -321,6 +326,19 @@ inline int swr_alloc(T **p, size_t count) {
     return SWR_ERR_NOMEM;
   }
   return SWR_OK;
+}
+
+// Grows *p to at least count elements (a quarter more, so a slowly growing
+// size does not reallocate every call); keeps it when it is large enough.
+// hipFree waits for the whole device, so per-call buffers must not be
+// freed and reallocated on every call: other systems' streams would stall.
+template <class T>
+inline int swr_reserve(T **p, size_t &cap, size_t count) {
+  if (*p && count <= cap) return SWR_OK;
+  const size_t want = count + count / 4 + 16;
+  const int rc = swr_alloc(p, want);
+  cap = rc ? 0 : want;
+  return rc;
 }
 
 template <class T>

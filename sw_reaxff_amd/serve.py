@@ -2,21 +2,28 @@
 
 Requests arrive over time; each is either a new system to evaluate once
 (neighbour list, QEq charges, far/bond/hbond lists, bond orders, non-bonded
-energy and gradient) or a
-follow-on MD step of a resident system.  The server:
+energy and gradient) or a follow-on MD step of a resident system.  The
+server:
 
   * runs `lanes` workers, each owning one device system (its own HIP stream).
-    A free worker takes every new-system request queued at that moment that
-    fits the HBM budget and evaluates them as ONE batch (Batch: one launch
-    per stage for all of them); requests that arrive while a batch is on the
-    GPU wait for the next free worker, and the workers' batches overlap on
-    the device (ctypes releases the GIL during library calls);
-  * keeps resident systems on the device between their steps (positions,
-    velocities, Verlet list, s/t warm-start history in their own device
-    system); a follow-on request advances one MD step there.  A new system
-    that is to stay resident is evaluated in its batch like any other; its
-    device system is set up when its first follow-on step arrives, warm-
-    started from the s/t histories its batched solve left;
+    A free worker takes the queued requests in arrival order that fit the
+    HBM budget and the batch size cap and evaluates them as ONE batch
+    (Batch: one launch per stage for all of them); requests that arrive
+    while a batch is on the GPU wait for the next free worker, and the
+    workers' batches overlap on the device (ctypes releases the GIL during
+    library calls);
+  * resident systems (resident="batched", the default): a follow-on step
+    advances the system one NVE step (F = 0: x += v dt, wrapped into a
+    periodic box) and evaluates it in the next batch like any other request,
+    its QEq solve warm-started from s/t histories kept in device memory
+    between its steps (HistStore: saved from the batch that evaluated its
+    last step, loaded into the batch of its next one; all five slots hold
+    the solution after its first evaluation).  Steps of one system run in
+    order, one per batch;
+  * resident="device": each resident system instead keeps its own device
+    system (positions, velocities, Verlet list with skin, s/t history) set
+    up at its first follow-on step from the histories its batched solve
+    left, and each further step runs there (one system per launch);
   * admits work by estimated HBM footprint against a budget (bytes per atom
     dominated by the Verlet and far lists).
 
@@ -29,7 +36,7 @@ from __future__ import annotations
 import math
 import threading
 import time
-from dataclasses import dataclass
+from dataclasses import dataclass, replace
 
 import numpy as np
 
@@ -112,11 +119,34 @@ def at_rate(reqs: list[Request], rate: float, unit_rate: float = 1.0) -> list[Re
             for r in reqs]
 
 
+def advance(s: systems.System, dt: float) -> systems.System:
+    """One NVE step with F = 0 (initial_integrate: x += dt v; the final half
+    kick adds nothing), periodic positions wrapped into [lo, hi) as
+    Domain::pbc does."""
+    x = s.x + dt * s.v
+    if s.periodic:
+        L = s.hi - s.lo
+        x = np.where(x < s.lo, x + L, x)
+        over = x >= s.hi
+        x = np.where(over, np.maximum(x - L, s.lo), x)
+    return replace(s, x=x)
+
+
+class _ResidentState:
+    """A resident system between its steps (resident="batched"): positions and
+    velocities on the host, its QEq warm start in device memory."""
+
+    def __init__(self, s: systems.System, store):
+        self.s, self.store = s, store
+        self.steps = 0
+        self.busy = False                # a step of it is in a batch on the GPU
+
+
 class _Resident:
-    """A system kept on the device between its MD steps.  Its device system
-    is created and set up on first use (first() or the first step()),
-    warm-started from `hist` (the s/t histories of its first evaluation)
-    when given."""
+    """A system kept on its own device system between its MD steps
+    (resident="device").  The device system is created and set up on first
+    use (first() or the first step()), warm-started from `hist` (the s/t
+    histories of its first evaluation) when given."""
 
     def __init__(self, s: systems.System, ffield, tol, mixed, hist=None):
         self.lock = threading.Lock()
@@ -152,16 +182,19 @@ class _Resident:
 
 class Server:
     def __init__(self, lanes: int = 2, hbm_budget: float = 200e9, max_batch_atoms: int = 400_000,
-                 ffield: str = DEFAULT_FFIELD, tol: float = 1e-6, mixed: bool = False, batches=None):
+                 ffield: str = DEFAULT_FFIELD, tol: float = 1e-6, mixed: bool = False, batches=None,
+                 resident: str = "batched", dt: float = 0.25):
+        assert resident in ("batched", "device")
         self.lanes = lanes
         self.batches = batches or [None] * lanes   # reusable per-lane Batch objects (device buffers kept)
         self.budget = hbm_budget
         self.max_batch_atoms = max_batch_atoms
         self.ffield, self.tol, self.mixed = ffield, tol, mixed
+        self.resident_mode, self.dt = resident, dt
         self.pending: list[Request] = []
         self.cv = threading.Condition()
-        self.in_use = 0.0                         # admitted bytes (batches in flight + residents)
-        self.residents: dict[int, _Resident] = {}
+        self.in_use = 0.0                         # admitted bytes (batches in flight + device-mode residents)
+        self.residents: dict = {}                 # id -> _ResidentState ("batched") or _Resident ("device")
         self.finished: list[Request] = []
         self.closed = False
         self.t0 = 0.0
@@ -172,33 +205,46 @@ class Server:
 
     # ---- admission
     def _take(self, lane):
-        """Requests for a lane: all queued follow-on steps, or a batch of new
-        systems that fits the budget and the batch size cap."""
+        """Requests for a lane.  Batched residents: the queued requests in
+        arrival order that fit the budget and the batch size cap, new systems
+        and steps alike (a step whose system is still being evaluated waits
+        for the next batch).  Device residents: all queued steps, else a
+        batch of new systems."""
         with self.cv:
             while True:
                 if self.pending:
-                    steps = [r for r in self.pending if r.kind == "step" and r.resident in self.residents]
-                    if steps:
-                        for r in steps:
-                            self.pending.remove(r)
-                        return "step", steps, 0.0
-                    news = [r for r in self.pending if r.kind == "new"]
-                    take, atoms, nbytes = [], 0, 0.0
-                    for r in news:
-                        b = footprint(r.system)
-                        if take and (atoms + r.system.n > self.max_batch_atoms or
-                                     self.in_use + nbytes + b > self.budget):
+                    if self.resident_mode == "device":
+                        steps = [r for r in self.pending if r.kind == "step" and r.resident in self.residents]
+                        if steps:
+                            for r in steps:
+                                self.pending.remove(r)
+                            return "step", steps, 0.0
+                    take, atoms, nbytes, seen = [], 0, 0.0, set()
+                    for r in self.pending:
+                        if r.kind == "step":
+                            if self.resident_mode == "device":
+                                continue
+                            st = self.residents.get(r.resident)
+                            if st is None or st.busy or r.resident in seen:
+                                continue
+                            n, b = st.s.n, footprint(st.s)
+                        else:
+                            n, b = r.system.n, footprint(r.system)
+                        if take and (atoms + n > self.max_batch_atoms or self.in_use + nbytes + b > self.budget):
                             break
                         if not take and self.in_use + b > self.budget:
                             break
+                        if r.kind == "step":
+                            seen.add(r.resident)
+                            st.busy = True
                         take.append(r)
-                        atoms += r.system.n
+                        atoms += n
                         nbytes += b
                     if take:
                         for r in take:
                             self.pending.remove(r)
                         self.in_use += nbytes
-                        return "new", take, nbytes
+                        return "batch", take, nbytes
                 if self.closed and not self.pending:
                     return None, [], 0.0
                 self.cv.wait(0.002)
@@ -218,48 +264,82 @@ class Server:
                 if kind is None:
                     return
                 if kind == "step":
-                    for r in reqs:
-                        t = time.perf_counter()
-                        res = self.residents[r.resident]
-                        with res.lock:
-                            if res.lmp is None:
-                                self.nsteps_first += 1
-                            r.q = res.step()
-                            r.step_index = res.steps
-                        self._finish(r, lane, 1)
-                        self._account("resident_steps", t)
+                    self._device_steps(lane, reqs)
                     continue
-                # new systems, residents included, are evaluated together as
-                # one batch; a resident keeps the s/t histories of its solve
-                t = time.perf_counter()
-                bt.load([r.system for r in reqs])
-                t = self._account("pack_load", t)
-                bt.evaluate(nonbonded=True)
-                bt.sync()
-                t = self._account("evaluate", t)
-                qs = bt.charges()
-                its = bt.iterations()
-                es = bt.energies()
-                hists = bt.histories() if any(r.resident >= 0 for r in reqs) else None
-                for k, (r, q, it) in enumerate(zip(reqs, qs, its)):
-                    r.q = q.copy()
-                    r.energy = float(es[k])
-                    r.iters = tuple(int(v) for v in it)
-                    if r.resident >= 0:
-                        r.hist = hists[k]
-                        res = _Resident(r.system, self.ffield, self.tol, self.mixed, hist=r.hist)
-                        with self.cv:
-                            self.residents[r.resident] = res
-                            self.in_use += res.bytes
-                for r in reqs:
-                    self._finish(r, lane, len(reqs))
-                self._account("results", t)
-                self._release(nbytes)
+                self._batch(lane, bt, reqs, nbytes)
         except BaseException as e:  # surface worker failures to run()
             self.errors.append(e)
             with self.cv:
                 self.closed = True
                 self.cv.notify_all()
+
+    def _device_steps(self, lane, reqs):
+        for r in reqs:
+            t = time.perf_counter()
+            res = self.residents[r.resident]
+            with res.lock:
+                if res.lmp is None:
+                    self.nsteps_first += 1
+                r.q = res.step()
+                r.step_index = res.steps
+            self._finish(r, lane, 1)
+            self._account("resident_steps", t)
+
+    def _batch(self, lane, bt, reqs, nbytes):
+        """One batch: new systems and (batched mode) resident steps, the
+        latter advanced one step and warm-started from their stores."""
+        batched = self.resident_mode == "batched"
+        t = time.perf_counter()
+        syss, hists = [], []
+        for r in reqs:
+            if r.kind == "step":
+                st = self.residents[r.resident]
+                r.system = advance(st.s, self.dt)
+                hists.append(st.store)
+            else:
+                hists.append(None)
+            syss.append(r.system)
+        bt.load(syss, hist=hists if batched else None)
+        t = self._account("pack_load", t)
+        bt.evaluate(nonbonded=True)
+        # warm starts for the residents' next steps, enqueued before the sync
+        stores = {}
+        if batched:
+            for k, r in enumerate(reqs):
+                if r.kind == "step":
+                    bt.save_history(k, self.residents[r.resident].store)
+                elif r.resident >= 0:
+                    stores[k] = bt.save_history(k, fill=True)
+        bt.sync()
+        t = self._account("evaluate", t)
+        qs = bt.charges()
+        its = bt.iterations()
+        es = bt.energies()
+        hists = bt.histories() if not batched and any(r.resident >= 0 for r in reqs) else None
+        for k, (r, q, it) in enumerate(zip(reqs, qs, its)):
+            r.q = q.copy()
+            r.energy = float(es[k])
+            r.iters = tuple(int(v) for v in it)
+        with self.cv:
+            for k, r in enumerate(reqs):
+                if r.kind == "step":
+                    st = self.residents[r.resident]
+                    st.s = r.system
+                    st.steps += 1
+                    r.step_index = st.steps
+                    st.busy = False
+                elif r.resident >= 0:
+                    if batched:
+                        self.residents[r.resident] = _ResidentState(r.system, stores[k])
+                    else:
+                        r.hist = hists[k]
+                        res = _Resident(r.system, self.ffield, self.tol, self.mixed, hist=r.hist)
+                        self.residents[r.resident] = res
+                        self.in_use += res.bytes
+        for r in reqs:
+            self._finish(r, lane, len(reqs))
+        self._account("results", t)
+        self._release(nbytes)
 
     def _account(self, phase, t):
         now = time.perf_counter()
@@ -296,6 +376,19 @@ class Server:
         if self.errors:
             raise self.errors[0]
         return requests
+
+
+def warm_lanes(batches, max_batch_atoms: int, seed: int = 7):
+    """Evaluates one batch of about max_batch_atoms local atoms (periodic
+    20,000-atom water boxes, the largest requests) on each lane's Batch, so
+    its device buffers start at the size of the largest batch the server
+    admits: growing a buffer frees the old one, which waits for the whole
+    device and so stalls every lane."""
+    big = [make_system("water", 20000, seed=seed + k) for k in range(max(1, max_batch_atoms // 20000))]
+    for bt in batches:
+        bt.load(big)
+        bt.evaluate(nonbonded=True)
+        bt.sync()
 
 
 def summary(reqs: list[Request], wall: float):

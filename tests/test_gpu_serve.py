@@ -11,12 +11,16 @@ those last-bit differences (~1e7) and can stop an iteration or two apart
 from the lone run, which moves charges by up to the solver's own truncation
 (~5e-6 e at the reference tolerance 1e-6).  Served charges must therefore
 agree with the lone run within 1e-5 e at tol 1e-6 and within 1e-8 e at tol
-1e-10.  A resident's first evaluation runs in a batch like any new system;
-its device system, set up at its first follow-on step from the s/t
-histories that batched solve left, is replayed from the same histories and
-stepped the same way, so its step charges agree to 1e-12 e.  With
-mixed-precision QEq (per-system FP64 refinement to tol/100 around the FP32
-inner solve) served and lone charges agree within 1e-7 e.
+1e-10; with mixed-precision QEq (per-system FP64 refinement to tol/100
+around the FP32 inner solve) within 1e-7 e.
+
+Batched residents: each step's positions are the last step's advanced by
+dt v (checked), evaluated in a batch with the warm start the last step left
+in device memory; the replay evaluates the same positions alone, chaining
+its own warm starts the same way, to the same tolerance.  Device residents
+(their own device system, set up at the first follow-on step from the s/t
+histories the batched solve left) are replayed from the same histories and
+stepped the same way, so their step charges agree to 1e-12 e.
 """
 import numpy as np
 import pytest
@@ -27,15 +31,16 @@ from sw_reaxff_amd.batch import Batch
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("tol,qtol,mixed", [(1e-6, 1e-5, False), (1e-10, 1e-8, False), (1e-6, 1e-7, True)])
-def test_served_results_equal_alone(gpu_lib, ffield, tol, qtol, mixed):
+@pytest.mark.parametrize("tol,qtol,mixed,mode", [(1e-6, 1e-5, False, "batched"), (1e-10, 1e-8, False, "batched"),
+                                                  (1e-6, 1e-7, True, "batched"), (1e-6, 1e-5, False, "device")])
+def test_served_results_equal_alone(gpu_lib, ffield, tol, qtol, mixed, mode):
     # arrivals faster than one evaluation, so batches of several systems form
     reqs = serve.make_requests(30, rate=5000.0, seed=11, lo=20, hi=1500, frac_step=0.3, frac_resident=0.25)
-    srv = serve.Server(lanes=2, hbm_budget=200e9, ffield=ffield, tol=tol, mixed=mixed)
+    srv = serve.Server(lanes=2, hbm_budget=200e9, ffield=ffield, tol=tol, mixed=mixed, resident=mode)
     srv.run(reqs)
     assert all(r.q is not None for r in reqs)
     assert any(r.kind == "step" for r in reqs) and any(r.batch_size > 1 for r in reqs)
-    print(f"[serve] {serve.summary(reqs, max(r.done for r in reqs))}", flush=True)
+    print(f"[serve] {mode} {serve.summary(reqs, max(r.done for r in reqs))}", flush=True)
     alone = Batch(ffield=ffield, tol=tol, mixed=mixed)
     worst = 0.0
     for r in reqs:
@@ -52,11 +57,31 @@ def test_served_results_equal_alone(gpu_lib, ffield, tol, qtol, mixed):
             assert dq <= qtol, (r.rid, r.system.n, r.batch_size, dq, r.iters, its)
     print(f"[serve] tol {tol}: worst served-vs-alone charge difference {worst:.2e}", flush=True)
     # residents: replay alone in the order the server stepped them
+    worst = 0.0
     for rid in sorted({r.resident for r in reqs if r.resident >= 0}):
         first = next(r for r in reqs if r.kind == "new" and r.resident == rid)
         steps = sorted((r for r in reqs if r.kind == "step" and r.resident == rid), key=lambda r: r.step_index)
-        assert first.hist is not None
-        res = serve._Resident(first.system, ffield, tol, mixed, hist=first.hist)
+        if mode == "device":
+            assert first.hist is not None
+            res = serve._Resident(first.system, ffield, tol, mixed, hist=first.hist)
+            for k, r in enumerate(steps, 1):
+                assert r.step_index == k
+                assert np.max(np.abs(res.step() - r.q)) <= 1e-12, (rid, k)
+            continue
+        alone.load([first.system])
+        alone.evaluate()
+        store = alone.save_history(0, fill=True)
+        alone.sync()
+        prev = first.system
         for k, r in enumerate(steps, 1):
             assert r.step_index == k
-            assert np.max(np.abs(res.step() - r.q)) <= 1e-12, (rid, k)
+            assert np.array_equal(r.system.x, serve.advance(prev, srv.dt).x), (rid, k)
+            alone.load([r.system], hist=[store])
+            alone.evaluate()
+            alone.save_history(0, store)
+            alone.sync()
+            dq = float(np.max(np.abs(alone.charges()[0] - r.q)))
+            worst = max(worst, dq)
+            assert dq <= qtol, (rid, k, dq)
+            prev = r.system
+    print(f"[serve] {mode} tol {tol}: worst resident step served-vs-alone charge difference {worst:.2e}", flush=True)
