@@ -354,7 +354,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="water96k", choices=sorted(CONFIGS) + ["clusters512", "serve"])
     ap.add_argument("--requests", type=int, default=400, help="serve: requests in the stream")
-    ap.add_argument("--rates", type=float, nargs="+", default=[500, 700, 900, 1000, 1100, 1200, 1400, 2000],
+    ap.add_argument("--rates", type=float, nargs="+", default=[1000, 1500, 2000, 2200, 2400, 2600],
                     help="serve: Poisson arrival rates (1/s) to sweep")
     ap.add_argument("--lanes", type=int, default=4, help="serve: concurrent batch workers")
     ap.add_argument("--max-batch-atoms", type=int, default=200_000, help="serve: atoms per batch at most")

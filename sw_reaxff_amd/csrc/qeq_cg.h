@@ -57,29 +57,36 @@ struct CgVecsT {
 };
 
 // Sums partials[k*pstride + b] over b in [b0, b1) for k < ND with one
-// workgroup: thread t adds b = b0+t, b0+t+RT, ... in order, then a fixed LDS
-// tree combines the RT sums.  The order depends only on the range.
-template <int ND>
+// workgroup of NT threads: thread t adds b = b0+t, b0+t+NT, ... in order,
+// each wave combines its lanes by a fixed butterfly, then the wave sums are
+// added in wave order (one barrier) by thread 0, the only thread whose out
+// holds the sums.  The order depends only on the range and NT.
+template <int ND, int NT = RT>
 __device__ void reduce_partials(const double *__restrict__ partials, int pstride, int b0, int b1, double *out) {
-  __shared__ double sr[ND][RT];
+  constexpr int NW = NT / SWR_WAVE;
+  __shared__ double sr[NW][ND];
   double acc[ND];
 #pragma unroll
   for (int k = 0; k < ND; k++) acc[k] = 0.0;
 #pragma unroll 4
-  for (int b = b0 + (int)threadIdx.x; b < b1; b += RT)
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += NT)
 #pragma unroll
     for (int k = 0; k < ND; k++) acc[k] += partials[(size_t)k * pstride + b];
+  const int w = wave_in_block();
 #pragma unroll
-  for (int k = 0; k < ND; k++) sr[k][threadIdx.x] = acc[k];
-  __syncthreads();
-  for (int s = RT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-#pragma unroll
-      for (int k = 0; k < ND; k++) sr[k][threadIdx.x] += sr[k][threadIdx.x + s];
-    __syncthreads();
+  for (int k = 0; k < ND; k++) {
+    const double v = wave_sum(acc[k]);
+    if (lane_id() == 0) sr[w][k] = v;
   }
+  __syncthreads();
 #pragma unroll
-  for (int k = 0; k < ND; k++) out[k] = sr[k][0];
+  for (int k = 0; k < ND; k++) {
+    double t = 0.0;
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int ww = 0; ww < NW; ww++) t += sr[ww][k];
+    out[k] = t;
+  }
 }
 
 // Block sum of per-thread values d[k] in a fixed order (wave tree, then

@@ -44,6 +44,10 @@ constexpr int MAX_OUTER = 20;
 constexpr double OUTER_TOL_FACTOR = 0.01;  // outer target = tol * this
 constexpr double INNER_TOL_MIN = 3e-4;
 constexpr int INNER_IMAX = 50;
+// Threads of the per-system scalar kernels in a batch: most systems of a
+// batch are small, and a 256-thread workgroup finds room on a busy CU far
+// sooner than a 1024-thread one (one system alone uses RT).
+constexpr int BT = 256;
 
 // Outer state of one system (device).
 struct MixState {
@@ -129,7 +133,8 @@ k_resid(int n, const DevParams *__restrict__ P, const int *__restrict__ type, co
 // own blocks' partials, in a fixed order).  A right-hand side that meets the
 // test gets tol = -1 in the inner scalars, so its inner solve does nothing
 // and its d stays 0.
-__global__ void __launch_bounds__(RT)
+template <int NT>
+__global__ void __launch_bounds__(NT)
 k_resid_scalars(const double *__restrict__ partials, int pstride, const int *__restrict__ sys_sblk,
                 MixCtl *__restrict__ ctl, MixState *__restrict__ msa, CgScalars *__restrict__ inner_all, double tol,
                 int first, DevStatus *st, int *__restrict__ iters_all) {
@@ -137,7 +142,7 @@ k_resid_scalars(const double *__restrict__ partials, int pstride, const int *__r
   MixState m = msa[sys];
   if (!first && m.done) return;
   double tot[6];
-  reduce_partials<6>(partials, pstride, sys_sblk[sys], sys_sblk[sys + 1], tot);
+  reduce_partials<6, NT>(partials, pstride, sys_sblk[sys], sys_sblk[sys + 1], tot);
   if (threadIdx.x != 0) return;
   CgScalars *inner = inner_all + 2 * sys;
   if (first) {
@@ -233,12 +238,13 @@ k_pcg_init(CgLayout L, const float2 *__restrict__ rhs, const double *__restrict_
   block_partials<4, UT>(dd, partials, pstride, tile);
 }
 
-__global__ void __launch_bounds__(RT)
+template <int NT>
+__global__ void __launch_bounds__(NT)
 k_pcg_init_scalars(const double *__restrict__ partials, int pstride, CgLayout L,
                    const CgScalars *__restrict__ inner_all, PcgState *__restrict__ pst, MixCtl *__restrict__ ctl) {
   const int sys = blockIdx.x;
   double tot[4];
-  reduce_partials<4>(partials, pstride, L.sys_tile[sys], L.sys_tile[sys + 1], tot);
+  reduce_partials<4, NT>(partials, pstride, L.sys_tile[sys], L.sys_tile[sys + 1], tot);
   if (threadIdx.x != 0) return;
   PcgState *c = pst + 2 * sys;
   for (int s = 0; s < 2; s++) {
@@ -281,14 +287,15 @@ k_pcg_spmv(int n, int nsys, const DevParams *__restrict__ P, const int *__restri
   block_partials<2, QT>(dd, partials, pstride, blockIdx.x);
 }
 
-__global__ void __launch_bounds__(RT)
+template <int NT>
+__global__ void __launch_bounds__(NT)
 k_pcg_alpha(const double *__restrict__ partials, int pstride, const int *__restrict__ sys_sblk,
             PcgState *__restrict__ pst) {
   const int sys = blockIdx.x;
   PcgState *c = pst + 2 * sys;
   if (!pcg_live(c)) return;
   double tot[2];
-  reduce_partials<2>(partials, pstride, sys_sblk[sys], sys_sblk[sys + 1], tot);
+  reduce_partials<2, NT>(partials, pstride, sys_sblk[sys], sys_sblk[sys + 1], tot);
   if (threadIdx.x != 0) return;
   for (int s = 0; s < 2; s++) c[s].alpha = c[s].active ? c[s].rz / tot[s] : 0.0;
 }
@@ -319,14 +326,15 @@ k_pcg_update(CgLayout L, const double *__restrict__ Hd, float2 *__restrict__ d, 
   block_partials<2, UT>(dd, partials, pstride, tile);
 }
 
-__global__ void __launch_bounds__(RT)
+template <int NT>
+__global__ void __launch_bounds__(NT)
 k_pcg_beta(const double *__restrict__ partials, int pstride, CgLayout L, PcgState *__restrict__ pst,
            MixCtl *__restrict__ ctl, int imax) {
   const int sys = blockIdx.x;
   PcgState *c = pst + 2 * sys;
   if (!pcg_live(c)) return;
   double tot[2];
-  reduce_partials<2>(partials, pstride, L.sys_tile[sys], L.sys_tile[sys + 1], tot);
+  reduce_partials<2, NT>(partials, pstride, L.sys_tile[sys], L.sys_tile[sys + 1], tot);
   if (threadIdx.x != 0) return;
   for (int s = 0; s < 2; s++) {
     if (!c[s].active) { c[s].beta = 0.0; continue; }
@@ -447,8 +455,12 @@ int swr_qeq_mixed_solve(swr_system *S) {
       hipLaunchKernelGGL(k_resid<false>, dim3(nblk), dim3(QT), 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
                          S->H_numnbrs, S->H_jlist, S->H_val32, (const double2 *)x, b, S->Hdia_inv, rhs, S->partials,
                          ps, S->sblk_beg, S->row_sys, ms, outer == 0 ? 1 : 0);
-    hipLaunchKernelGGL(k_resid_scalars, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, S->sys_sblk, ctl, ms,
-                       inner, S->ctl.tol * OUTER_TOL_FACTOR, outer == 0 ? 1 : 0, S->status, S->sys_iters);
+    if (batch)
+      hipLaunchKernelGGL(k_resid_scalars<BT>, dim3(nsys), dim3(BT), 0, S->stream, S->partials, ps, S->sys_sblk, ctl,
+                         ms, inner, S->ctl.tol * OUTER_TOL_FACTOR, outer == 0 ? 1 : 0, S->status, S->sys_iters);
+    else
+      hipLaunchKernelGGL(k_resid_scalars<RT>, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, S->sys_sblk, ctl,
+                         ms, inner, S->ctl.tol * OUTER_TOL_FACTOR, outer == 0 ? 1 : 0, S->status, S->sys_iters);
     swr_kt_end(S, KT_MIXED_OUTER);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, S->stream));
@@ -461,7 +473,12 @@ int swr_qeq_mixed_solve(swr_system *S) {
     HIP_CHECK(hipMemsetAsync(&ctl->inner_ndone, 0, 2 * sizeof(int), S->stream));
     hipLaunchKernelGGL(k_pcg_init, dim3(L.ntiles), dim3(UT), 0, S->stream, L, rhs, S->Hdia_inv, dv, rv, zv, pv,
                        S->partials, ps);
-    hipLaunchKernelGGL(k_pcg_init_scalars, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, L, inner, pst, ctl);
+    if (batch)
+      hipLaunchKernelGGL(k_pcg_init_scalars<BT>, dim3(nsys), dim3(BT), 0, S->stream, S->partials, ps, L, inner, pst,
+                         ctl);
+    else
+      hipLaunchKernelGGL(k_pcg_init_scalars<RT>, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, L, inner, pst,
+                         ctl);
     int it = 0;
     int nb = std::max(2, S->mixed_last_inner + 1);
     while (it < INNER_IMAX) {
@@ -474,10 +491,18 @@ int swr_qeq_mixed_solve(swr_system *S) {
           hipLaunchKernelGGL(k_pcg_spmv<false>, dim3(nblk), dim3(QT), 0, S->stream, n, nsys, S->dp, S->type,
                              S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
                              S->partials, ps, S->sblk_beg, S->row_sys);
-        hipLaunchKernelGGL(k_pcg_alpha, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, S->sys_sblk, pst);
+        if (batch)
+          hipLaunchKernelGGL(k_pcg_alpha<BT>, dim3(nsys), dim3(BT), 0, S->stream, S->partials, ps, S->sys_sblk, pst);
+        else
+          hipLaunchKernelGGL(k_pcg_alpha<RT>, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, S->sys_sblk, pst);
         hipLaunchKernelGGL(k_pcg_update, dim3(L.ntiles), dim3(UT), 0, S->stream, L, S->Hdia_inv, dv, rv, zv,
                            (const float2 *)pv, (const float2 *)qv, pst, S->partials, ps);
-        hipLaunchKernelGGL(k_pcg_beta, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, L, pst, ctl, INNER_IMAX);
+        if (batch)
+          hipLaunchKernelGGL(k_pcg_beta<BT>, dim3(nsys), dim3(BT), 0, S->stream, S->partials, ps, L, pst, ctl,
+                             INNER_IMAX);
+        else
+          hipLaunchKernelGGL(k_pcg_beta<RT>, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, L, pst, ctl,
+                             INNER_IMAX);
         hipLaunchKernelGGL(k_pcg_p, dim3(L.ntiles), dim3(UT), 0, S->stream, L, (const float2 *)zv, pv, pst);
       }
       HIP_CHECK(hipGetLastError());
