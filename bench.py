@@ -246,6 +246,39 @@ def bench_clusters(args):
     print(json.dumps(out), flush=True)
 
 
+# The mixed-precision solve's outer target is tol times this (qeq_mixed.hip).
+MIXED_OUTER_FACTOR = 1e-3
+QEQ_SLOTS = {False: ("cg_setup", "cg_iter", "cg_update", "cg_scalars", "qeq_q"),
+             True: ("mixed_inner", "mixed_outer", "qeq_q")}
+
+
+def qeq_leg(s, args, mixed, tol, steps):
+    """One more run of the same box (its own setup and warmup) with kernel
+    timers on: ms/step and the QEq solve's share (H build excluded: the same
+    kernel in both forms)."""
+    lm = Lammps()
+    lm.commands(standard_script(s.elements, ffield=DEFAULT_FFIELD, every=args.every, dt=args.dt, mixed=mixed,
+                                tol=tol))
+    lm.create_atoms(s)
+    lm.setup()
+    lm.run(args.warmup)
+    h = lm.system
+    h.ktimer_enable(True)
+    for slot in SLOTS:
+        h.ktimer_read(slot, reset=True)
+    its0 = (lm.stat("total_iters_s"), lm.stat("total_iters_t"))
+    device_sync()
+    t0 = time.perf_counter()
+    lm.run(steps)
+    device_sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    qeq = sum(h.ktimer_read(k)[0] for k in QEQ_SLOTS[mixed]) / steps
+    its = ((lm.stat("total_iters_s") - its0[0]) / steps, (lm.stat("total_iters_t") - its0[1]) / steps)
+    lm.close()
+    return {"form": "mixed" if mixed else "fp64", "tol": tol, "ms_per_step_timed": round(ms, 4),
+            "qeq_ms_per_step": round(qeq, 4), "iters_per_step": [round(its[0], 2), round(its[1], 2)]}
+
+
 def mixed_errors(s, args):
     """The mixed-precision run against an FP64 run at a tight tolerance
     (1e-10), stepped side by side from the same atoms: the zero-force NVE
@@ -256,7 +289,10 @@ def mixed_errors(s, args):
     runs alongside for comparison."""
     from sw_reaxff_amd import _lib as L
     runs = []
-    for mixed, tol in ((True, args.tol), (False, 1e-10), (False, args.tol)):
+    legs = [(True, args.tol), (False, 1e-10), (False, args.tol)]
+    if s.n <= 200_000:   # four systems of the 768,000-atom box do not fit one GPU at once
+        legs.append((False, args.tol * MIXED_OUTER_FACTOR))
+    for mixed, tol in legs:
         lm = Lammps()
         lm.commands(standard_script(s.elements, ffield=DEFAULT_FFIELD, every=args.every, dt=args.dt,
                                     mixed=mixed, tol=tol))
@@ -264,7 +300,7 @@ def mixed_errors(s, args):
         lm.setup()
         runs.append(lm)
     n = s.n
-    dq, df, dq64, df64 = [], [], [], []
+    dq, df, dq64, df64, dqm, dfm = [], [], [], [], [], []
     for step in range(args.error_steps):
         out = []
         for lm in runs:
@@ -273,17 +309,26 @@ def mixed_errors(s, args):
             L.check(h.L.swr_vdw_coulomb_energy(h.h), "swr_vdw_coulomb_energy")
             h.sync()
             out.append((h.get("q")[:n], h.get("fCdDelta").reshape(-1, 4)[:n, :3]))
-        (qm, fm), (qd, fd), (q6, f6) = out
+        (qm, fm), (qd, fd), (q6, f6) = out[:3]
         dq.append(float(np.max(np.abs(qm - qd))))
         df.append(float(np.max(np.abs(fm - fd)) / np.max(np.abs(fd))))
         dq64.append(float(np.max(np.abs(q6 - qd))))
         df64.append(float(np.max(np.abs(f6 - fd)) / np.max(np.abs(fd))))
+        if len(out) > 3:
+            q9, f9 = out[3]
+            dqm.append(float(np.max(np.abs(q9 - qd))))
+            dfm.append(float(np.max(np.abs(f9 - fd)) / np.max(np.abs(fd))))
     for lm in runs:
         lm.close()
+    matched = None
+    if dqm:
+        matched = {"tol": args.tol * MIXED_OUTER_FACTOR, "max_abs_dq_e": max(dqm),
+                   "median_abs_dq_e": float(np.median(dqm)), "max_rel_dF": max(dfm)}
     return {"steps": args.error_steps, "reference": "FP64 QEq at tol 1e-10, same trajectory",
             "max_abs_dq_e": max(dq), "median_abs_dq_e": float(np.median(dq)), "max_rel_dF": max(df),
             "fp64_at_same_tol": {"max_abs_dq_e": max(dq64), "median_abs_dq_e": float(np.median(dq64)),
                                  "max_rel_dF": max(df64)},
+            "fp64_at_mixed_outer_target": matched,
             "per_step_max_abs_dq_e": [float(f"{v:.3e}") for v in dq],
             "per_step_max_rel_dF": [float(f"{v:.3e}") for v in df]}
 
@@ -479,15 +524,27 @@ def main():
                  "frac": round(nb_b / (nb_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                  "energies_kcal_mol": [float(v) for v in sysh.get("nonbonded_energy")[:2]]}
     log(f"non-bonded pass: {nonbonded}")
-    errors = None
-    if args.mixed and args.error_steps > 0:
-        errors = mixed_errors(s, args)
-        log(f"mixed vs FP64: {errors}")
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline (oracle, budget {args.cpu_budget_s:.0f} s)")
         cpu = cpu_baseline(sysh, s.elements, DEFAULT_FFIELD, args.dt, args.every, args.cpu_budget_s)
         log(f"CPU {cpu['value']} ms/step on {cpu['cores']} threads")
+    errors, qeq_cmp = None, None
+    if args.mixed:
+        qeq_mixed = sum(kt[k][0] for k in QEQ_SLOTS[True] if k in kt) / ksteps
+        lmp.close()   # the comparisons hold several systems of this size at once
+        if args.error_steps > 0:
+            errors = mixed_errors(s, args)
+            log(f"mixed vs FP64: {errors}")
+        # the QEq solve against FP64 at the reference tolerance and at the
+        # mixed solve's own outer target (the FP64 tolerance that reaches
+        # the same accuracy), each timed in a run of its own
+        legs = [qeq_leg(s, args, False, args.tol, ksteps),
+                qeq_leg(s, args, False, args.tol * MIXED_OUTER_FACTOR, ksteps)]
+        qeq_cmp = {"mixed_qeq_ms_per_step": round(qeq_mixed, 4), "fp64": legs,
+                   "speedup_vs_fp64_same_tol": round(legs[0]["qeq_ms_per_step"] / qeq_mixed, 3),
+                   "speedup_vs_fp64_at_outer_target": round(legs[1]["qeq_ms_per_step"] / qeq_mixed, 3)}
+        log(f"QEq mixed vs FP64: {qeq_cmp}")
     out = {
         "metric": METRIC, "value": round(ms_step, 4), "unit": "ms/step", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": False, "scaling": "weak",
@@ -498,7 +555,7 @@ def main():
         "atom_steps_per_s": round(n / (ms_step * 1e-3), 1),
         "cg_iters_per_step": [round(its[0], 2), round(its[1], 2)],
         "counts": c, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
-        "nonbonded_pass": nonbonded, "mixed_errors_vs_fp64": errors,
+        "nonbonded_pass": nonbonded, "mixed_errors_vs_fp64": errors, "mixed_qeq_vs_fp64": qeq_cmp,
         "speedup_vs_cpu": round(cpu["value"] / ms_step, 2) if cpu else None,
         "host": platform.node(),
     }

@@ -57,7 +57,7 @@ typedef struct swr_control_t {
   int mincap, maxfar;
   double swa, swb, tol;
   double skin;          /* neighbor skin; cutneigh = max(nonb,hbond,2*bond) + skin */
-  int qeq_mixed;        /* 1: FP32 H + FP32 inner CG inside FP64 refinement    */
+  int qeq_mixed;        /* 1: FP32 inner CG on an FP32 copy of H, FP64 refinement */
   int qeq_imax;         /* CG iteration cap, 200 in the reference              */
 } swr_control_t;
 

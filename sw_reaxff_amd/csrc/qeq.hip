@@ -184,7 +184,7 @@ int swr_qeq_build_H(swr_system *S) {
   swr_kt_begin(S, KT_BUILD_H);
   hipLaunchKernelGGL(k_build_H, dim3(div_up(n, QW)), dim3(QT), 0, S->stream, n, S->dp, S->x, S->type, S->owner,
                      S->voff, S->vlist, S->H_firstnbr, S->H_numnbrs, S->H_jlist,
-                     S->mixed ? (double *)nullptr : S->H_val, S->mixed ? S->H_val32 : (float *)nullptr, S->Hdia_inv, (double2 *)S->b_s, (double2 *)S->st,
+                     S->H_val, S->mixed ? S->H_val32 : (float *)nullptr, S->Hdia_inv, (double2 *)S->b_s, (double2 *)S->st,
                      S->s_hist, S->t_hist);
   swr_kt_end(S, KT_BUILD_H);
   HIP_CHECK(hipGetLastError());

@@ -11,7 +11,7 @@ those last-bit differences (~1e7) and can stop an iteration or two apart
 from the lone run, which moves charges by up to the solver's own truncation
 (~5e-6 e at the reference tolerance 1e-6).  Served charges must therefore
 agree with the lone run within 1e-5 e at tol 1e-6 and within 1e-8 e at tol
-1e-10; with mixed-precision QEq (per-system FP64 refinement to tol/100
+1e-10; with mixed-precision QEq (per-system FP64 refinement to tol/1000
 around the FP32 inner solve) within 1e-7 e.
 
 Batched residents: each step's positions are the last step's advanced by
