@@ -59,6 +59,7 @@ typedef struct swr_control_t {
   double skin;          /* neighbor skin; cutneigh = max(nonb,hbond,2*bond) + skin */
   int qeq_mixed;        /* 1: FP32 inner CG on an FP32 copy of H, FP64 refinement */
   int qeq_imax;         /* CG iteration cap, 200 in the reference              */
+  int enobonds;         /* pair_style keyword enobonds (1 = yes, the default)  */
 } swr_control_t;
 
 typedef struct swr_system swr_system;
@@ -223,6 +224,30 @@ int swr_hist_free(swr_hist_store *store);
  * enqueues only.  lgvdw is not supported (the pair_style default is off). */
 int swr_vdw_coulomb_energy(swr_system *sys);
 
+/* Bond and atom energies: Atom_Energy (reaxc_multi_body_sunway.cpp:41-57),
+ * i.e. Merge_Bonds_Atom_Energy_C_New (reaxc_multi_body_sw64.c:21-337; the
+ * separate Bonds() returns at once, reaxc_bonds_sunway.cpp:49-52).  Over the
+ * local atoms: polarization, lone-pair (+ C-C correction), over- and
+ * under-coordination energies and, for each bond a local atom owns (smaller
+ * orig_id, position order between images of one atom), the bond energy and
+ * the terminal triple-bond stabilisation.  Adds to Cdbo/Cdbopi/Cdbopi2 of
+ * the bond slots and to CdDelta (fCdDelta[i][3]) of the atoms and their
+ * bonded neighbours (ghosts included); leaves e_bond, e_lp, e_ov, e_un,
+ * e_pol in swr_get(sys, "bonded_energy") (5 doubles; e_pol is the plain sum
+ * over atoms, the value the reference adds to eng_coul).  Run after swr_bo
+ * of the step; enqueues only.  The pair_style keyword enobonds is
+ * swr_control_t.enobonds. */
+int swr_atom_energy(swr_system *sys);
+
+/* Add_All_dBond_to_Forces_C (reaxc_forces_sw64.c:417-589): over all N atoms
+ * and their bonds, the chain rule of the bond-order derivatives with the
+ * coefficients Cdbo/Cdbopi/Cdbopi2 of both halves of each bond and the
+ * CdDelta of both atoms, added to fCdDelta[.][0..2] (the energy gradient;
+ * ghost entries are to be summed onto their owners, as LAMMPS' reverse
+ * communication does).  Run after the terms that produce those
+ * coefficients (swr_atom_energy); enqueues only. */
+int swr_add_dbond_forces(swr_system *sys);
+
 /* Host-side packing of nsys independent systems into one batch (no
  * reference counterpart; host code, no device calls).  System k has nat[k]
  * local atoms, listed in order in x (3 per atom), type (1-based, already in
@@ -257,7 +282,7 @@ int swr_device_synchronize(void);
  * hbond_index hbond_end hbonds hbond_far bo_dboc dDeltap_self bond_mark
  * Deltap Deltap_boc Delta Delta_e Delta_val vlpex nlp nlp_temp Clp Delta_lp
  * dDelta_lp Delta_lp_temp dDelta_lp_temp H_firstnbr H_numnbrs H_jlist H_val
- * Hdia_inv b_s b_t s t s_hist t_hist fCdDelta nonbonded_energy nonbonded_eatom.  hbonds' ptr
+ * Hdia_inv b_s b_t s t s_hist t_hist fCdDelta nonbonded_energy nonbonded_eatom bonded_energy.  hbonds' ptr
  * fields come back as
  * device addresses; hbond_far gives each hbond's far-list slot. */
 int64_t swr_get(swr_system *sys, const char *name, void *out);

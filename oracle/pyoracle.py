@@ -51,6 +51,8 @@ def lib():
         L.orc_step.argtypes = [vp, ip, ip]
         L.orc_sparse_matvec.argtypes = [vp, dp, dp]
         L.orc_vdw_coulomb.argtypes = [vp, dp]
+        L.orc_atom_energy.argtypes = [vp, C.c_int, dp]
+        L.orc_dbond_forces.argtypes = [vp]
         L.orc_get_array.restype = C.c_int64
         L.orc_get_array.argtypes = [vp, C.c_char_p, vp]
         L.orc_set_array.argtypes = [vp, C.c_char_p, vp, C.c_int64]
@@ -183,6 +185,19 @@ class Oracle:
         o = np.zeros(8)
         self.L.orc_vdw_coulomb(self.h, _p(o, C.c_double))
         return o[0], o[1], o[2:].copy()
+
+    def atom_energy(self, enobonds=1):
+        """Atom_Energy (bond, lone-pair, over/under-coordination, polarization
+        energies) over the current bonds: accumulates Cdbo/Cdbopi/Cdbopi2 and
+        CdDelta (fCdDelta[:, 3]); returns (e_bond, e_lp, e_ov, e_un, e_pol)."""
+        o = np.zeros(5)
+        self.L.orc_atom_energy(self.h, int(enobonds), _p(o, C.c_double))
+        return o
+
+    def dbond_forces(self):
+        """Add_All_dBond_to_Forces: adds the bond-order chain rule to
+        fCdDelta[:, 0..2] over all N atoms."""
+        self.L.orc_dbond_forces(self.h)
 
     def spmv(self, xN):
         xN = np.ascontiguousarray(xN, dtype=np.float64)

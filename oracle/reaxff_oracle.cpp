@@ -1215,6 +1215,230 @@ int orc_vdw_coulomb(orc_sys *S, double *out) {
   return 0;
 }
 
+// Whether local atom i owns bond (i, j) for the bond energy: the smaller
+// orig_id, or for two images of one atom the one lower in z, then y, then x
+// (Merge_Bonds_Atom_Energy_C_New, reaxc_multi_body_sw64.c:245-255).
+static bool owns_bond(const orc_sys *S, int i, int j) {
+  const swr_atom_pack_t &a = S->packed[i], &b = S->packed[j];
+  if (a.orig_id > b.orig_id) return false;
+  if (a.orig_id == b.orig_id) {
+    if (b.x[2] < a.x[2]) return false;
+    if (b.x[2] == a.x[2] && b.x[1] < a.x[1]) return false;
+    if (b.x[2] == a.x[2] && b.x[1] == a.x[1] && b.x[0] < a.x[0]) return false;
+  }
+  return true;
+}
+
+// Atom_Energy (reaxc_multi_body_sunway.cpp:41-57; Bonds() returns at once,
+// reaxc_bonds_sunway.cpp:49-52) = Merge_Bonds_Atom_Energy_C_New
+// (reaxc_multi_body_sw64.c:21-337), the host loop over local atoms: the
+// polarization energy, lone pair (+ the C-C lone-pair correction), over- and
+// under-coordination, and for each bond the atom owns the bond energy and the
+// terminal triple-bond stabilisation.  Accumulates Cdbo/Cdbopi/Cdbopi2 of
+// the atom's bond slots and CdDelta (fCdDelta[.][3]) of the atom and its
+// bonded neighbours, in the reference's order.  Departures: a neighbour of
+// type < 0 is skipped in the last bond loop as in the first (the reference
+// indexes tbp with it there); e_pol is the plain sum over atoms (the value
+// the reference adds to eng_coul; its data->my_en.e_pol adds the running
+// sum at every atom, :97-98).  out: e_bond (ebond + estriph), e_lp (e_lp +
+// e_lph), e_ov, e_un, e_pol.
+int orc_atom_energy(orc_sys *S, int enobonds, double *out) {
+  const int n = S->n;
+  const std::vector<double> &gp = S->gp;
+  const double KCALpMOL_to_EV = 23.02;        // reaxc_defs_sunway.h:68
+  const double p_lp3 = gp[5], p_ovun3 = gp[32], p_ovun4 = gp[31], p_ovun6 = gp[6], p_ovun7 = gp[8],
+               p_ovun8 = gp[9];
+  const double gp3 = gp[3], gp4 = gp[4], gp7 = gp[7], gp10 = gp[10];
+  const int gp37 = (int)gp[37];
+  if ((int64_t)S->fCdDelta.size() < 4 * (int64_t)S->N) S->fCdDelta.assign(4 * (size_t)S->N, 0.0);
+  double *fc = S->fCdDelta.data();
+  double e_lph = 0, e_lp = 0, e_ov = 0, e_un_sum = 0, ebond = 0, estriph = 0, e_pol = 0;
+  for (int i = 0; i < n; i++) {
+    const swr_atom_pack_t &ai = S->packed[i];
+    const int ti = ai.type;
+    if (ti < 0) continue;
+    const SBP &si = S->sbp[ti];
+    const double q = ai.q;
+    e_pol += KCALpMOL_to_EV * (si.chi * q + (si.eta / 2.) * SQR(q));
+    // lone pair
+    const double p_lp2 = si.p_lp2;
+    const double expvd2 = std::exp(-75 * S->Delta_lp[i]);
+    const double inv_expvd2 = 1. / (1. + expvd2);
+    int numbonds = 0;
+    const double p_ovun2 = si.p_ovun2;
+    double sum_ovun1 = 0, sum_ovun2 = 0;
+    const double dfvl = si.mass > 21.0 ? 0.0 : 1.0;   // 1st-row elements only
+    for (int pj = S->bindex[i]; pj < S->bend[i]; ++pj) {
+      numbonds++;
+      const int j = S->bonds[pj].nbr;
+      const int tj = S->packed[j].type;
+      if (tj < 0) continue;
+      const TBP &tw = S->tbp[ti][tj];
+      sum_ovun1 += tw.p_ovun1 * tw.De_s * S->BO[pj];
+      sum_ovun2 += (S->Delta[j] - dfvl * S->Delta_lp_temp[j]) * (S->BOpi[2 * (size_t)pj] + S->BOpi[2 * (size_t)pj + 1]);
+    }
+    const double dElp = p_lp2 * inv_expvd2 + 75 * p_lp2 * S->Delta_lp[i] * expvd2 * SQR(inv_expvd2);
+    const double CElp = dElp * S->dDelta_lp[i];
+    if (p_lp3 > 0.001 && si.name == "C") {
+      for (int pj = S->bindex[i]; pj < S->bend[i]; ++pj) {
+        const int j = S->bonds[pj].nbr;
+        const int tj = S->packed[j].type;
+        if (tj < 0) continue;
+        if (S->sbp[tj].name == "C") {
+          const double Di = S->Delta[i];
+          const double vov3 = S->BO[pj] - Di - 0.040 * std::pow(Di, 4.);
+          if (vov3 > 3.) {
+            e_lph += p_lp3 * SQR(vov3 - 3.0);
+            const double deahu2dbo = 2. * p_lp3 * (vov3 - 3.);
+            const double deahu2dsbo = 2. * p_lp3 * (vov3 - 3.) * (-1. - 0.16 * std::pow(Di, 3.));
+            S->Cdbo[pj] += deahu2dbo;
+            fc[4 * (size_t)i + 3] += deahu2dsbo;
+          }
+        }
+      }
+    }
+    // over-coordination
+    const double exp_ovun1 = p_ovun3 * std::exp(p_ovun4 * sum_ovun2);
+    const double inv_exp_ovun1 = 1.0 / (1 + exp_ovun1);
+    const double Delta_lpcorr = S->Delta[i] - (dfvl * S->Delta_lp_temp[i]) * inv_exp_ovun1;
+    const double exp_ovun2 = std::exp(p_ovun2 * Delta_lpcorr);
+    const double inv_exp_ovun2 = 1.0 / (1.0 + exp_ovun2);
+    const double DlpVi = 1.0 / (Delta_lpcorr + si.valency + 1e-8);
+    const double CEover1 = Delta_lpcorr * DlpVi * inv_exp_ovun2;
+    e_ov += sum_ovun1 * CEover1;
+    const double CEover2 =
+        sum_ovun1 * DlpVi * inv_exp_ovun2 * (1.0 - Delta_lpcorr * (DlpVi + p_ovun2 * exp_ovun2 * inv_exp_ovun2));
+    const double CEover3 = CEover2 * (1.0 - dfvl * S->dDelta_lp[i] * inv_exp_ovun1);
+    const double CEover4 = CEover2 * (dfvl * S->Delta_lp_temp[i]) * p_ovun4 * exp_ovun1 * SQR(inv_exp_ovun1);
+    // under-coordination
+    const double p_ovun5 = si.p_ovun5;
+    const double exp_ovun2n = 1.0 / exp_ovun2;
+    const double exp_ovun6 = std::exp(p_ovun6 * Delta_lpcorr);
+    const double exp_ovun8 = p_ovun7 * std::exp(p_ovun8 * sum_ovun2);
+    const double inv_exp_ovun2n = 1.0 / (1.0 + exp_ovun2n);
+    const double inv_exp_ovun8 = 1.0 / (1.0 + exp_ovun8);
+    double e_un = 0.0;
+    fc[4 * (size_t)i + 3] += CEover3;
+    const bool under = numbonds > 0 || enobonds;
+    if (under) e_un = -p_ovun5 * (1.0 - exp_ovun6) * inv_exp_ovun2n * inv_exp_ovun8;
+    const double CEunder1 =
+        inv_exp_ovun2n * (p_ovun5 * p_ovun6 * exp_ovun6 * inv_exp_ovun8 + p_ovun2 * e_un * exp_ovun2n);
+    const double CEunder2 = -e_un * p_ovun8 * exp_ovun8 * inv_exp_ovun8;
+    const double CEunder3 = CEunder1 * (1.0 - dfvl * S->dDelta_lp[i] * inv_exp_ovun1);
+    const double CEunder4 =
+        CEunder1 * (dfvl * S->Delta_lp_temp[i]) * p_ovun4 * exp_ovun1 * SQR(inv_exp_ovun1) + CEunder2;
+    if (under) {
+      e_un_sum += e_un;
+      fc[4 * (size_t)i + 3] += CEunder3;
+      fc[4 * (size_t)i + 3] += CElp;
+      e_lp += p_lp2 * S->Delta_lp[i] * inv_expvd2;
+    }
+    for (int pj = S->bindex[i]; pj < S->bend[i]; ++pj) {
+      const int j = S->bonds[pj].nbr;
+      const int tj = S->packed[j].type;
+      if (tj < 0) continue;
+      const TBP &tw = S->tbp[ti][tj];
+      const double pi_sum = S->BOpi[2 * (size_t)pj] + S->BOpi[2 * (size_t)pj + 1];
+      S->Cdbo[pj] += CEover1 * tw.p_ovun1 * tw.De_s;
+      const double ftmp2 = (1.0 - dfvl * S->dDelta_lp[j]) * pi_sum;
+      fc[4 * (size_t)j + 3] += CEover4 * ftmp2;
+      const double ftmp = CEover4 * (S->Delta[j] - dfvl * S->Delta_lp_temp[j]);
+      S->Cdbopi[pj] += ftmp;
+      S->Cdbopi2[pj] += ftmp;
+      fc[4 * (size_t)j + 3] += CEunder4 * ftmp2;
+      const double fu = CEunder4 * (S->Delta[j] - dfvl * S->Delta_lp_temp[j]);
+      S->Cdbopi[pj] += fu;
+      S->Cdbopi2[pj] += fu;
+      if (!owns_bond(S, i, j)) continue;
+      const SBP &sj = S->sbp[tj];
+      const double BO_s = S->bo_data[pj].BO_s;
+      const double pow_BOs_be2 = BO_s == 0.0 ? 0.0 : std::pow(BO_s, tw.p_be2);
+      const double exp_be12 = std::exp(tw.p_be1 * (1.0 - pow_BOs_be2));
+      const double CEbo = -tw.De_s * exp_be12 * (1.0 - tw.p_be1 * tw.p_be2 * pow_BOs_be2);
+      ebond += -tw.De_s * BO_s * exp_be12 - tw.De_p * S->BOpi[2 * (size_t)pj] - tw.De_pp * S->BOpi[2 * (size_t)pj + 1];
+      S->Cdbo[pj] += CEbo;
+      S->Cdbopi[pj] -= (CEbo + tw.De_p);
+      S->Cdbopi2[pj] -= (CEbo + tw.De_pp);
+      const double BOij = S->BO[pj];
+      if (BOij >= 1.00 && (gp37 == 2 || (si.mass == 12.0000 && sj.mass == 15.9990) ||
+                           (sj.mass == 12.0000 && si.mass == 15.9990))) {
+        const double exphu = std::exp(-gp7 * SQR(BOij - 2.50));
+        const double exphua1 = std::exp(-gp3 * (S->bo_dboc[2 * (size_t)i] - BOij));
+        const double exphub1 = std::exp(-gp3 * (S->bo_dboc[2 * (size_t)j] - BOij));
+        const double exphuov = std::exp(gp4 * (S->Delta[i] + S->Delta[j]));
+        const double hulpov = 1.0 / (1.0 + 25.0 * exphuov);
+        estriph += gp10 * exphu * hulpov * (exphua1 + exphub1);
+        const double decobdbo = gp10 * exphu * hulpov * (exphua1 + exphub1) * (gp3 - 2.0 * gp7 * (BOij - 2.50));
+        const double decobdboua =
+            -gp10 * exphu * hulpov * (gp3 * exphua1 + 25.0 * gp4 * exphuov * hulpov * (exphua1 + exphub1));
+        const double decobdboub =
+            -gp10 * exphu * hulpov * (gp3 * exphub1 + 25.0 * gp4 * exphuov * hulpov * (exphua1 + exphub1));
+        S->Cdbo[pj] += decobdbo;
+        fc[4 * (size_t)i + 3] += decobdboua;
+        fc[4 * (size_t)j + 3] += decobdboub;
+      }
+    }
+  }
+  out[0] = ebond + estriph;
+  out[1] = e_lp + e_lph;
+  out[2] = e_ov;
+  out[3] = e_un_sum;
+  out[4] = e_pol;
+  return 0;
+}
+
+// Add_All_dBond_to_Forces_C (reaxc_forces_sw64.c:417-589), its host loop
+// (the #else branch :495-586; the CPE form :1370-1570 evaluates the same
+// terms): over all N atoms and their bonds, the chain rule of the bond-order
+// derivatives with the coefficients Cdbo/Cdbopi/Cdbopi2 of both halves of
+// each bond and the CdDelta of both atoms; adds the atom's own terms to
+// fCdDelta[i][0..2] and buf_c dBOp to each bonded neighbour.  fCdDelta[.][3]
+// is only read.
+int orc_dbond_forces(orc_sys *S) {
+  const int N = S->N;
+  double *fc = S->fCdDelta.data();
+  std::vector<double> bufd;
+  std::vector<int> bufj;
+  for (int i = 0; i < N; ++i) {
+    double buf_c = 0.0;
+    bufd.clear();
+    bufj.clear();
+    const double *ds = &S->dDeltap_self[3 * (size_t)i];
+    for (int pj = S->bindex[i]; pj < S->bend[i]; ++pj) {
+      const swr_bond_t &b = S->bonds[pj];
+      const int j = b.nbr, sym = b.sym_index;
+      const swr_bond_order_t &bo = S->bo_data[pj];
+      const double cdbo = S->Cdbo[pj] + S->Cdbo[sym];
+      const double cdbopi = S->Cdbopi[pj] + S->Cdbopi[sym];
+      const double cdbopi2 = S->Cdbopi2[pj] + S->Cdbopi2[sym];
+      const double C1dbo = bo.C1dbo * cdbo, C2dbo = bo.C2dbo * cdbo;
+      const double C1dbopi = bo.C1dbopi * cdbopi, C2dbopi = bo.C2dbopi * cdbopi, C3dbopi = bo.C3dbopi * cdbopi;
+      const double C1dbopi2 = bo.C1dbopi2 * cdbopi2, C2dbopi2 = bo.C2dbopi2 * cdbopi2,
+                   C3dbopi2 = bo.C3dbopi2 * cdbopi2;
+      const double cdd = fc[4 * (size_t)i + 3] + fc[4 * (size_t)j + 3];
+      const double C1dDelta = bo.C1dbo * cdd, C2dDelta = bo.C2dbo * cdd;
+      double t[3];
+      for (int k = 0; k < 3; k++) t[k] = C1dbo * bo.dBOp[k];
+      for (int k = 0; k < 3; k++) t[k] += C2dbo * ds[k];
+      for (int k = 0; k < 3; k++) t[k] += C1dDelta * bo.dBOp[k];
+      for (int k = 0; k < 3; k++) t[k] += C2dDelta * ds[k];
+      for (int k = 0; k < 3; k++) t[k] += C1dbopi * bo.dln_BOp_pi[k];
+      for (int k = 0; k < 3; k++) t[k] += C2dbopi * bo.dBOp[k];
+      for (int k = 0; k < 3; k++) t[k] += C3dbopi * ds[k];
+      for (int k = 0; k < 3; k++) t[k] += C1dbopi2 * bo.dln_BOp_pi2[k];
+      for (int k = 0; k < 3; k++) t[k] += C2dbopi2 * bo.dBOp[k];
+      for (int k = 0; k < 3; k++) t[k] += C3dbopi2 * ds[k];
+      for (int k = 0; k < 3; k++) fc[4 * (size_t)i + k] += t[k];
+      buf_c += (-C2dbo - C2dDelta - C3dbopi - C3dbopi2);
+      bufj.push_back(j);
+      for (int k = 0; k < 3; k++) bufd.push_back(bo.dBOp[k]);
+    }
+    for (size_t m = 0; m < bufj.size(); m++)
+      for (int k = 0; k < 3; k++) fc[4 * (size_t)bufj[m] + k] += buf_c * bufd[3 * m + k];
+  }
+  return 0;
+}
+
 int orc_step(orc_sys *S, int *its, int *itt) {
   int rc = orc_qeq_pre_force(S, its, itt);
   if (rc) return rc;

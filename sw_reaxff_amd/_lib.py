@@ -30,7 +30,7 @@ EXPORTS = [
     "swr_lmp_system", "swr_lmp_stat", "swr_ktimer_enable", "swr_ktimer_read", "swr_device_synchronize",
     "swr_sparse_matvec_h", "swr_npair_full_bin_atomonly_build", "swr_sort_bonds", "swr_set_batch",
     "swr_new_atoms", "swr_batch_pack", "swr_vdw_coulomb_energy", "swr_hist_save", "swr_hist_load",
-    "swr_hist_free",
+    "swr_hist_free", "swr_atom_energy", "swr_add_dbond_forces",
 ]
 
 
@@ -44,7 +44,7 @@ class Control(C.Structure):
     _fields_ = [("bond_cut", C.c_double), ("hbond_cut", C.c_double), ("thb_cut", C.c_double),
                 ("safezone", C.c_double), ("saferzone", C.c_double), ("mincap", C.c_int), ("maxfar", C.c_int),
                 ("swa", C.c_double), ("swb", C.c_double), ("tol", C.c_double), ("skin", C.c_double),
-                ("qeq_mixed", C.c_int), ("qeq_imax", C.c_int)]
+                ("qeq_mixed", C.c_int), ("qeq_imax", C.c_int), ("enobonds", C.c_int)]
 
 
 _lib = None
@@ -72,7 +72,8 @@ def lib():
         for nm in ["swr_write_reax_atoms_and_pack", "swr_reset", "swr_write_reax_lists", "swr_init_forces_noqeq_full",
                    "swr_write_reax_lists_init_forces", "swr_validate_lists", "swr_sort_bonds", "swr_bo",
                    "swr_qeq_init_matvec",
-                   "swr_calculate_q", "swr_pair_compute", "swr_sync", "swr_vdw_coulomb_energy"]:
+                   "swr_calculate_q", "swr_pair_compute", "swr_sync", "swr_vdw_coulomb_energy",
+                   "swr_atom_energy", "swr_add_dbond_forces"]:
             getattr(L, nm).argtypes = [vp]
         L.swr_sparse_matvec.argtypes = [vp, vp, vp]
         L.swr_sparse_matvec_h.argtypes = [vp, dp, dp]

@@ -1,0 +1,431 @@
+// bonded.hip — bond and atom energies and the bond-order chain rule of the
+// forces:
+//
+//   Atom_Energy (reaxc_multi_body_sunway.cpp:41-57) = Merge_Bonds_Atom_Energy_C_New
+//   (reaxc_multi_body_sw64.c:21-337): per local atom the polarization, lone-pair
+//   (+ C-C correction), over- and under-coordination energies, and for each
+//   bond the atom owns the bond energy and the terminal triple-bond
+//   stabilisation; the coefficients Cdbo/Cdbopi/Cdbopi2 of each bond slot and
+//   CdDelta (fCdDelta[.][3]) of the atoms.  (Bonds() itself returns at once,
+//   reaxc_bonds_sunway.cpp:49-52.)
+//
+//   Add_All_dBond_to_Forces_C (reaxc_forces_sw64.c:417-589): over all N atoms
+//   and their bonds, the chain rule of the bond-order derivatives into
+//   fCdDelta[.][0..2] (the energy gradient, as the non-bonded pass writes it).
+//
+// The reference loops over atoms and adds into the neighbours' entries as it
+// goes.  Here every kernel writes only entries of its own atom, so no
+// floating-point atomics are needed and results are the same on every run:
+//
+//   k_atom_energy     one thread per local atom: energies (per-atom rows,
+//                     summed in a fixed order by k_sum_rows), its own bond
+//                     slots' coefficients, its own CdDelta, and the two
+//                     per-atom factors (CEover4, CEunder4) its neighbours need;
+//   k_atom_energy_nbr one thread per atom (local and ghost): the CdDelta terms
+//                     the reference adds to atom j from each bonded local atom
+//                     i, read through the bond's reverse slot (sym_index), so
+//                     the values are those i's own slot holds;
+//   k_dbond_own       one thread per atom: its own chain-rule terms and the
+//                     factor buf_c it passes to its neighbours;
+//   k_dbond_nbr       one thread per atom: sum over its bonds of the
+//                     neighbour's buf_c times the neighbour's dBOp toward it.
+//
+// Each atom's terms are added in the reference's order for that atom; only
+// the interleaving of its own and its neighbours' terms differs, which moves
+// results at the rounding level.
+#include "swr_internal.h"
+#include "swr_device.h"
+
+namespace {
+
+constexpr int BT = 256;
+constexpr double KCALpMOL_to_EV = 23.02;   // reaxc_defs_sunway.h:68
+
+__device__ __forceinline__ double sqr(double x) { return x * x; }
+
+// Local atom i owns bond (i, j) for the bond energy: the smaller orig_id, or
+// for two images of one atom the lower in z, then y, then x (:245-255).
+__device__ __forceinline__ bool owns_bond(const swr_atom_pack_t &a, const swr_atom_pack_t &b) {
+  if (a.orig_id > b.orig_id) return false;
+  if (a.orig_id == b.orig_id) {
+    if (b.x[2] < a.x[2]) return false;
+    if (b.x[2] == a.x[2] && b.x[1] < a.x[1]) return false;
+    if (b.x[2] == a.x[2] && b.x[1] == a.x[1] && b.x[0] < a.x[0]) return false;
+  }
+  return true;
+}
+
+// Terminal triple-bond stabilisation of bond (i, j) owned by i (:283-308):
+// returns whether it applies; dbo, dua, dub are the terms for Cdbo of i's
+// slot, CdDelta of i and CdDelta of j; e the energy.
+__device__ __forceinline__ bool triple_bond(const DevParams *__restrict__ P, int ti, int tj, double BOij,
+                                            double dboc_i, double dboc_j, double Delta_i, double Delta_j, double &e,
+                                            double &dbo, double &dua, double &dub) {
+  const double mi = P->mass[ti], mj = P->mass[tj];
+  if (!(BOij >= 1.00 && (P->gp37 == 2 || (mi == 12.0000 && mj == 15.9990) || (mj == 12.0000 && mi == 15.9990))))
+    return false;
+  const double gp3 = P->gp3, gp4 = P->gp4, gp7 = P->gp7, gp10 = P->gp10;
+  const double exphu = exp(-gp7 * sqr(BOij - 2.50));
+  const double exphua1 = exp(-gp3 * (dboc_i - BOij));
+  const double exphub1 = exp(-gp3 * (dboc_j - BOij));
+  const double exphuov = exp(gp4 * (Delta_i + Delta_j));
+  const double hulpov = 1.0 / (1.0 + 25.0 * exphuov);
+  e = gp10 * exphu * hulpov * (exphua1 + exphub1);
+  dbo = gp10 * exphu * hulpov * (exphua1 + exphub1) * (gp3 - 2.0 * gp7 * (BOij - 2.50));
+  dua = -gp10 * exphu * hulpov * (gp3 * exphua1 + 25.0 * gp4 * exphuov * hulpov * (exphua1 + exphub1));
+  dub = -gp10 * exphu * hulpov * (gp3 * exphub1 + 25.0 * gp4 * exphuov * hulpov * (exphua1 + exphub1));
+  return true;
+}
+
+struct BeArgs {
+  const DevParams *P;
+  const swr_atom_pack_t *atoms;
+  const int *bidx, *bend;
+  const swr_bond_t *bonds;
+  const swr_bond_order_t *bo;
+  const double *BO, *BOpi;       // BOpi: [slot][2]
+  const double *Delta, *Delta_lp, *dDelta_lp, *Delta_lp_temp;
+  const double *bo_dboc;         // [N][2]
+  double *Cdbo, *Cdbopi, *Cdbopi2;
+  double *fcd;                   // [N][4]
+  double *ce;                    // [2][stride]: CEover4, CEunder4
+  double *part;                  // [5][stride]
+  size_t stride;
+};
+
+__global__ void __launch_bounds__(BT) k_atom_energy(int n, BeArgs A) {
+  const int i = blockIdx.x * BT + threadIdx.x;
+  if (i >= n) return;
+  const DevParams *__restrict__ P = A.P;
+  const swr_atom_pack_t ai = A.atoms[i];
+  const int ti = ai.type;
+  double e_bond = 0, e_lp = 0, e_ov = 0, e_un = 0, e_pol = 0;
+  double ceo4 = 0, ceu4 = 0;
+  if (ti >= 0) {
+    const double q = ai.q;
+    e_pol = KCALpMOL_to_EV * (P->chi_e[ti] * q + (P->eta_e[ti] / 2.) * sqr(q));
+    const int b0 = A.bidx[i], b1 = A.bend[i];
+    // lone pair
+    const double p_lp2 = P->p_lp2[ti];
+    const double Dlp = A.Delta_lp[i];
+    const double expvd2 = exp(-75 * Dlp);
+    const double inv_expvd2 = 1. / (1. + expvd2);
+    const double p_ovun2 = P->p_ovun2[ti];
+    const double dfvl = P->mass[ti] > 21.0 ? 0.0 : 1.0;   // 1st-row elements only
+    int numbonds = 0;
+    double sum_ovun1 = 0, sum_ovun2 = 0;
+    for (int pj = b0; pj < b1; ++pj) {
+      numbonds++;
+      const int j = A.bonds[pj].nbr;
+      const int tj = A.atoms[j].type;
+      if (tj < 0) continue;
+      const DevBe &tw = P->be[ti * SWR_MAX_ELEM + tj];
+      sum_ovun1 += tw.p_ovun1 * tw.De_s * A.BO[pj];
+      sum_ovun2 += (A.Delta[j] - dfvl * A.Delta_lp_temp[j]) * (A.BOpi[2 * (size_t)pj] + A.BOpi[2 * (size_t)pj + 1]);
+    }
+    const double dElp = p_lp2 * inv_expvd2 + 75 * p_lp2 * Dlp * expvd2 * sqr(inv_expvd2);
+    const double CElp = dElp * A.dDelta_lp[i];
+    double cdd = A.fcd[4 * (size_t)i + 3];
+    const double Di = A.Delta[i];
+    if (P->p_lp3 > 0.001 && P->is_carbon[ti]) {
+      for (int pj = b0; pj < b1; ++pj) {
+        const int j = A.bonds[pj].nbr;
+        const int tj = A.atoms[j].type;
+        if (tj < 0 || !P->is_carbon[tj]) continue;
+        const double vov3 = A.BO[pj] - Di - 0.040 * pow(Di, 4.);
+        if (vov3 > 3.) {
+          e_lp += P->p_lp3 * sqr(vov3 - 3.0);
+          const double deahu2dbo = 2. * P->p_lp3 * (vov3 - 3.);
+          const double deahu2dsbo = 2. * P->p_lp3 * (vov3 - 3.) * (-1. - 0.16 * pow(Di, 3.));
+          A.Cdbo[pj] += deahu2dbo;
+          cdd += deahu2dsbo;
+        }
+      }
+    }
+    // over-coordination
+    const double p_ovun4 = P->p_ovun4;
+    const double exp_ovun1 = P->p_ovun3 * exp(p_ovun4 * sum_ovun2);
+    const double inv_exp_ovun1 = 1.0 / (1 + exp_ovun1);
+    const double Dlpt = A.Delta_lp_temp[i];
+    const double Delta_lpcorr = Di - (dfvl * Dlpt) * inv_exp_ovun1;
+    const double exp_ovun2 = exp(p_ovun2 * Delta_lpcorr);
+    const double inv_exp_ovun2 = 1.0 / (1.0 + exp_ovun2);
+    const double DlpVi = 1.0 / (Delta_lpcorr + P->valency[ti] + 1e-8);
+    const double CEover1 = Delta_lpcorr * DlpVi * inv_exp_ovun2;
+    e_ov = sum_ovun1 * CEover1;
+    const double CEover2 =
+        sum_ovun1 * DlpVi * inv_exp_ovun2 * (1.0 - Delta_lpcorr * (DlpVi + p_ovun2 * exp_ovun2 * inv_exp_ovun2));
+    const double dDlp = A.dDelta_lp[i];
+    const double CEover3 = CEover2 * (1.0 - dfvl * dDlp * inv_exp_ovun1);
+    ceo4 = CEover2 * (dfvl * Dlpt) * p_ovun4 * exp_ovun1 * sqr(inv_exp_ovun1);
+    // under-coordination
+    const double p_ovun5 = P->p_ovun5[ti];
+    const double exp_ovun2n = 1.0 / exp_ovun2;
+    const double exp_ovun6 = exp(P->p_ovun6 * Delta_lpcorr);
+    const double exp_ovun8 = P->p_ovun7 * exp(P->p_ovun8 * sum_ovun2);
+    const double inv_exp_ovun2n = 1.0 / (1.0 + exp_ovun2n);
+    const double inv_exp_ovun8 = 1.0 / (1.0 + exp_ovun8);
+    cdd += CEover3;
+    const bool under = numbonds > 0 || P->enobonds;
+    double eu = 0.0;
+    if (under) eu = -p_ovun5 * (1.0 - exp_ovun6) * inv_exp_ovun2n * inv_exp_ovun8;
+    const double CEunder1 =
+        inv_exp_ovun2n * (p_ovun5 * P->p_ovun6 * exp_ovun6 * inv_exp_ovun8 + p_ovun2 * eu * exp_ovun2n);
+    const double CEunder2 = -eu * P->p_ovun8 * exp_ovun8 * inv_exp_ovun8;
+    const double CEunder3 = CEunder1 * (1.0 - dfvl * dDlp * inv_exp_ovun1);
+    ceu4 = CEunder1 * (dfvl * Dlpt) * p_ovun4 * exp_ovun1 * sqr(inv_exp_ovun1) + CEunder2;
+    if (under) {
+      e_un = eu;
+      cdd += CEunder3;
+      cdd += CElp;
+      e_lp += p_lp2 * Dlp * inv_expvd2;
+    }
+    // per bond: over/under-coordination coefficients of this slot, bond
+    // energy and triple-bond stabilisation of the bonds this atom owns
+    for (int pj = b0; pj < b1; ++pj) {
+      const int j = A.bonds[pj].nbr;
+      const swr_atom_pack_t aj = A.atoms[j];
+      const int tj = aj.type;
+      if (tj < 0) continue;
+      const DevBe &tw = P->be[ti * SWR_MAX_ELEM + tj];
+      double cdbo = A.Cdbo[pj], cdbopi = A.Cdbopi[pj], cdbopi2 = A.Cdbopi2[pj];
+      cdbo += CEover1 * tw.p_ovun1 * tw.De_s;
+      const double dj = A.Delta[j] - dfvl * A.Delta_lp_temp[j];
+      const double ftmp = ceo4 * dj;
+      cdbopi += ftmp;
+      cdbopi2 += ftmp;
+      const double fu = ceu4 * dj;
+      cdbopi += fu;
+      cdbopi2 += fu;
+      if (owns_bond(ai, aj)) {
+        const double BO_s = A.bo[pj].BO_s;
+        const double pow_BOs_be2 = BO_s == 0.0 ? 0.0 : pow(BO_s, tw.p_be2);
+        const double exp_be12 = exp(tw.p_be1 * (1.0 - pow_BOs_be2));
+        const double CEbo = -tw.De_s * exp_be12 * (1.0 - tw.p_be1 * tw.p_be2 * pow_BOs_be2);
+        e_bond += -tw.De_s * BO_s * exp_be12 - tw.De_p * A.BOpi[2 * (size_t)pj] - tw.De_pp * A.BOpi[2 * (size_t)pj + 1];
+        cdbo += CEbo;
+        cdbopi -= (CEbo + tw.De_p);
+        cdbopi2 -= (CEbo + tw.De_pp);
+        double e3, dbo, dua, dub;
+        if (triple_bond(P, ti, tj, A.BO[pj], A.bo_dboc[2 * (size_t)i], A.bo_dboc[2 * (size_t)j], Di, A.Delta[j], e3,
+                        dbo, dua, dub)) {
+          e_bond += e3;
+          cdbo += dbo;
+          cdd += dua;
+        }
+      }
+      A.Cdbo[pj] = cdbo;
+      A.Cdbopi[pj] = cdbopi;
+      A.Cdbopi2[pj] = cdbopi2;
+    }
+    A.fcd[4 * (size_t)i + 3] = cdd;
+  }
+  A.ce[i] = ceo4;
+  A.ce[A.stride + i] = ceu4;
+  A.part[i] = e_bond;
+  A.part[A.stride + i] = e_lp;
+  A.part[2 * A.stride + i] = e_ov;
+  A.part[3 * A.stride + i] = e_un;
+  A.part[4 * A.stride + i] = e_pol;
+}
+
+// CdDelta terms of atom j from each bonded local atom i (:270-279 and the
+// triple-bond term :305), read through i's slot toward j.
+__global__ void __launch_bounds__(BT) k_atom_energy_nbr(int n, int N, BeArgs A) {
+  const int j = blockIdx.x * BT + threadIdx.x;
+  if (j >= N) return;
+  const DevParams *__restrict__ P = A.P;
+  const swr_atom_pack_t aj = A.atoms[j];
+  const int tj = aj.type;
+  if (tj < 0) return;
+  const int b0 = A.bidx[j], b1 = A.bend[j];
+  if (b0 >= b1) return;
+  double cdd = A.fcd[4 * (size_t)j + 3];
+  const double dDlp_j = A.dDelta_lp[j];
+  for (int pk = b0; pk < b1; ++pk) {
+    const swr_bond_t b = A.bonds[pk];
+    const int i = b.nbr;
+    if (i >= n) continue;
+    const swr_atom_pack_t ai = A.atoms[i];
+    const int ti = ai.type;
+    if (ti < 0) continue;
+    const int pj = b.sym_index;   // i's slot toward j
+    const double dfvl = P->mass[ti] > 21.0 ? 0.0 : 1.0;
+    const double ftmp2 = (1.0 - dfvl * dDlp_j) * (A.BOpi[2 * (size_t)pj] + A.BOpi[2 * (size_t)pj + 1]);
+    cdd += A.ce[i] * ftmp2;
+    cdd += A.ce[A.stride + i] * ftmp2;
+    if (owns_bond(ai, aj)) {
+      double e3, dbo, dua, dub;
+      if (triple_bond(P, ti, tj, A.BO[pj], A.bo_dboc[2 * (size_t)i], A.bo_dboc[2 * (size_t)j], A.Delta[i],
+                      A.Delta[j], e3, dbo, dua, dub))
+        cdd += dub;
+    }
+  }
+  A.fcd[4 * (size_t)j + 3] = cdd;
+}
+
+// Fixed-order sums of the per-atom energy rows: block r sums row r.
+__global__ void __launch_bounds__(1024) k_sum_rows(int n, const double *__restrict__ part, size_t stride,
+                                                   double *__restrict__ out) {
+  __shared__ double s[1024];
+  const double *row = part + blockIdx.x * stride;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) acc += row[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+struct DbArgs {
+  const swr_bond_t *bonds;
+  const swr_bond_order_t *bo;
+  const int *bidx, *bend;
+  const double *Cdbo, *Cdbopi, *Cdbopi2;
+  const double *dDeltap_self;    // [N][3]
+  double *fcd;                   // [N][4]
+  double *bufc;                  // [N]
+};
+
+// The atom's own terms (the reference's forces on i, :532-548) and buf_c.
+__global__ void __launch_bounds__(BT) k_dbond_own(int N, DbArgs A) {
+  const int i = blockIdx.x * BT + threadIdx.x;
+  if (i >= N) return;
+  const int b0 = A.bidx[i], b1 = A.bend[i];
+  double buf_c = 0.0;
+  if (b0 < b1) {
+    const double ds0 = A.dDeltap_self[3 * (size_t)i], ds1 = A.dDeltap_self[3 * (size_t)i + 1],
+                 ds2 = A.dDeltap_self[3 * (size_t)i + 2];
+    const double cd_i = A.fcd[4 * (size_t)i + 3];
+    double f0 = A.fcd[4 * (size_t)i], f1 = A.fcd[4 * (size_t)i + 1], f2 = A.fcd[4 * (size_t)i + 2];
+    for (int pj = b0; pj < b1; ++pj) {
+      const swr_bond_t b = A.bonds[pj];
+      const int j = b.nbr, sym = b.sym_index;
+      const swr_bond_order_t &bo = A.bo[pj];
+      const double cdbo = A.Cdbo[pj] + A.Cdbo[sym];
+      const double cdbopi = A.Cdbopi[pj] + A.Cdbopi[sym];
+      const double cdbopi2 = A.Cdbopi2[pj] + A.Cdbopi2[sym];
+      const double C1dbo = bo.C1dbo * cdbo, C2dbo = bo.C2dbo * cdbo;
+      const double C1dbopi = bo.C1dbopi * cdbopi, C2dbopi = bo.C2dbopi * cdbopi, C3dbopi = bo.C3dbopi * cdbopi;
+      const double C1dbopi2 = bo.C1dbopi2 * cdbopi2, C2dbopi2 = bo.C2dbopi2 * cdbopi2,
+                   C3dbopi2 = bo.C3dbopi2 * cdbopi2;
+      const double cdd = cd_i + A.fcd[4 * (size_t)j + 3];
+      const double C1dDelta = bo.C1dbo * cdd, C2dDelta = bo.C2dbo * cdd;
+      const double dbx = bo.dBOp[0], dby = bo.dBOp[1], dbz = bo.dBOp[2];
+      double t0 = C1dbo * dbx, t1 = C1dbo * dby, t2 = C1dbo * dbz;
+      t0 += C2dbo * ds0; t1 += C2dbo * ds1; t2 += C2dbo * ds2;
+      t0 += C1dDelta * dbx; t1 += C1dDelta * dby; t2 += C1dDelta * dbz;
+      t0 += C2dDelta * ds0; t1 += C2dDelta * ds1; t2 += C2dDelta * ds2;
+      t0 += C1dbopi * bo.dln_BOp_pi[0]; t1 += C1dbopi * bo.dln_BOp_pi[1]; t2 += C1dbopi * bo.dln_BOp_pi[2];
+      t0 += C2dbopi * dbx; t1 += C2dbopi * dby; t2 += C2dbopi * dbz;
+      t0 += C3dbopi * ds0; t1 += C3dbopi * ds1; t2 += C3dbopi * ds2;
+      t0 += C1dbopi2 * bo.dln_BOp_pi2[0]; t1 += C1dbopi2 * bo.dln_BOp_pi2[1]; t2 += C1dbopi2 * bo.dln_BOp_pi2[2];
+      t0 += C2dbopi2 * dbx; t1 += C2dbopi2 * dby; t2 += C2dbopi2 * dbz;
+      t0 += C3dbopi2 * ds0; t1 += C3dbopi2 * ds1; t2 += C3dbopi2 * ds2;
+      f0 += t0; f1 += t1; f2 += t2;
+      buf_c += (-C2dbo - C2dDelta - C3dbopi - C3dbopi2);
+    }
+    A.fcd[4 * (size_t)i] = f0;
+    A.fcd[4 * (size_t)i + 1] = f1;
+    A.fcd[4 * (size_t)i + 2] = f2;
+  }
+  A.bufc[i] = buf_c;
+}
+
+// The terms the reference adds to k from each bonded atom i (forces on k,
+// :580-584): buf_c of i times i's dBOp toward k.
+__global__ void __launch_bounds__(BT) k_dbond_nbr(int N, DbArgs A) {
+  const int k = blockIdx.x * BT + threadIdx.x;
+  if (k >= N) return;
+  const int b0 = A.bidx[k], b1 = A.bend[k];
+  if (b0 >= b1) return;
+  double f0 = A.fcd[4 * (size_t)k], f1 = A.fcd[4 * (size_t)k + 1], f2 = A.fcd[4 * (size_t)k + 2];
+  for (int pk = b0; pk < b1; ++pk) {
+    const swr_bond_t b = A.bonds[pk];
+    const double c = A.bufc[b.nbr];
+    const double *d = A.bo[b.sym_index].dBOp;
+    f0 += c * d[0];
+    f1 += c * d[1];
+    f2 += c * d[2];
+  }
+  A.fcd[4 * (size_t)k] = f0;
+  A.fcd[4 * (size_t)k + 1] = f1;
+  A.fcd[4 * (size_t)k + 2] = f2;
+}
+
+}  // namespace
+
+int swr_atom_energy_run(swr_system *S) {
+  const int n = S->n, N = S->N;
+  if (!S->bonds || !S->bo_data || !S->fcd || N <= 0) {
+    swr_set_error_msg("Atom_Energy needs the bonds and bond orders of the step (swr_bo first)");
+    return SWR_ERR_STATE;
+  }
+  int rc;
+  const size_t stride = (size_t)S->Ncap;
+  if ((rc = swr_reserve(&S->be_part, S->cap_be_part, 5 * stride)) ||
+      (rc = swr_reserve(&S->be_ce, S->cap_be_ce, 2 * stride)) ||
+      (rc = swr_reserve(&S->be_result, S->cap_be_result, 8)))
+    return rc;
+  S->be_stride = stride;
+  BeArgs A;
+  A.P = S->dp;
+  A.atoms = S->packed;
+  A.bidx = S->bond_index;
+  A.bend = S->bond_end;
+  A.bonds = S->bonds;
+  A.bo = S->bo_data;
+  A.BO = S->BO;
+  A.BOpi = S->BOpi;
+  const double *ws = S->ws;
+  A.Delta = ws + (size_t)WS_DELTA * S->Ncap;
+  A.Delta_lp = ws + (size_t)WS_DELTA_LP * S->Ncap;
+  A.dDelta_lp = ws + (size_t)WS_DDELTA_LP * S->Ncap;
+  A.Delta_lp_temp = ws + (size_t)WS_DELTA_LP_TEMP * S->Ncap;
+  A.bo_dboc = S->bo_dboc;
+  A.Cdbo = S->Cdbo;
+  A.Cdbopi = S->Cdbopi;
+  A.Cdbopi2 = S->Cdbopi2;
+  A.fcd = S->fcd;
+  A.ce = S->be_ce;
+  A.part = S->be_part;
+  A.stride = stride;
+  swr_kt_begin(S, KT_ATOM_ENERGY);
+  if (n > 0) hipLaunchKernelGGL(k_atom_energy, dim3(div_up(n, BT)), dim3(BT), 0, S->stream, n, A);
+  hipLaunchKernelGGL(k_atom_energy_nbr, dim3(div_up(N, BT)), dim3(BT), 0, S->stream, n, N, A);
+  hipLaunchKernelGGL(k_sum_rows, dim3(5), dim3(1024), 0, S->stream, n, (const double *)S->be_part, stride,
+                     S->be_result);
+  swr_kt_end(S, KT_ATOM_ENERGY);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_dbond_forces_run(swr_system *S) {
+  const int N = S->N;
+  if (!S->bonds || !S->bo_data || !S->fcd || N <= 0) {
+    swr_set_error_msg("Add_All_dBond_to_Forces needs the bonds and bond orders of the step");
+    return SWR_ERR_STATE;
+  }
+  int rc;
+  if ((rc = swr_reserve(&S->be_bufc, S->cap_be_bufc, (size_t)S->Ncap))) return rc;
+  DbArgs A;
+  A.bonds = S->bonds;
+  A.bo = S->bo_data;
+  A.bidx = S->bond_index;
+  A.bend = S->bond_end;
+  A.Cdbo = S->Cdbo;
+  A.Cdbopi = S->Cdbopi;
+  A.Cdbopi2 = S->Cdbopi2;
+  A.dDeltap_self = S->dDeltap_self;
+  A.fcd = S->fcd;
+  A.bufc = S->be_bufc;
+  swr_kt_begin(S, KT_DBOND);
+  hipLaunchKernelGGL(k_dbond_own, dim3(div_up(N, BT)), dim3(BT), 0, S->stream, N, A);
+  hipLaunchKernelGGL(k_dbond_nbr, dim3(div_up(N, BT)), dim3(BT), 0, S->stream, N, A);
+  swr_kt_end(S, KT_DBOND);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}

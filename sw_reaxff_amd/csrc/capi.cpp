@@ -93,6 +93,26 @@ void fill_params(swr_system *S, const std::vector<int> &map) {
   h.hbond_cut = S->ctl.hbond_cut;
   h.nonb_cut = ff.nonb_cut;
   h.nonb_cutsq = ff.nonb_cut * ff.nonb_cut;
+  // bond and atom energies
+  for (int e = 0; e < ff.ne() && e < SWR_MAX_ELEM; e++) {
+    const FfAtom &a = ff.atom[e];
+    h.p_lp2[e] = a.p_lp2; h.p_ovun2[e] = a.p_ovun2; h.p_ovun5[e] = a.p_ovun5;
+    h.chi_e[e] = a.chi; h.eta_e[e] = a.eta;
+    h.is_carbon[e] = a.name == "C" ? 1 : 0;
+    for (int f = 0; f < ff.ne() && f < SWR_MAX_ELEM; f++) {
+      const FfPair &p = ff.tb(e, f);
+      DevBe &d = h.be[e * SWR_MAX_ELEM + f];
+      d.De_s = p.De_s; d.De_p = p.De_p; d.De_pp = p.De_pp; d.p_be1 = p.p_be1; d.p_be2 = p.p_be2;
+      d.p_ovun1 = p.p_ovun1;
+    }
+  }
+  {
+    auto g = [&](size_t k) { return k < ff.gp.size() ? ff.gp[k] : 0.0; };
+    h.p_lp3 = g(5); h.p_ovun3 = g(32); h.p_ovun4 = g(31); h.p_ovun6 = g(6); h.p_ovun7 = g(8);
+    h.p_ovun8 = g(9); h.gp3 = g(3); h.gp4 = g(4); h.gp7 = g(7); h.gp10 = g(10);
+    h.gp37 = (int)g(37);
+    h.enobonds = S->ctl.enobonds;
+  }
   // non-bonded pair factors and taper
   h.vdw_type = ff.vdw_type;
   h.p_vdW1 = ff.gp.size() > 28 ? ff.gp[28] : 0.0;
@@ -230,6 +250,7 @@ void swr_control_default(swr_control_t *c) {
   c->skin = 2.0;
   c->qeq_mixed = 0;
   c->qeq_imax = 200;    // fix_qeq_reax_sunway.cpp:998
+  c->enobonds = 1;      // pair_reaxc_sunway.cpp:237
 }
 
 int swr_create(const char *ffield_path, int ntypes, const char *const *elements, const swr_control_t *ctl,
@@ -304,7 +325,7 @@ int swr_destroy(swr_system *S) {
                   S->voff, S->vcount, S->vlist, S->far, S->far_index, S->far_end, S->num_bonds, S->num_hbonds,
                   S->Hindex, S->bond_index, S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, S->Cdbo, S->Cdbopi,
                   S->Cdbopi2, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,
-                  S->fcd, S->nb_partials, S->nb_result, S->nb_eatom,
+                  S->fcd, S->nb_partials, S->nb_result, S->nb_eatom, S->be_part, S->be_ce, S->be_bufc, S->be_result,
                   S->bond_mark2, S->counts, S->ws, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32,
                   S->Hdia_inv, S->b_s, S->st, S->s_hist, S->t_hist, S->cgv, S->cgs, S->partials, S->counters,
                   S->status, S->scan_tmp, S->tmp_i, S->mixw, S->cand, S->cand_count, S->row_sys, S->tile_beg,
@@ -605,6 +626,20 @@ int swr_vdw_coulomb_energy(swr_system *S) {
   return swr_nonbonded_run(S);
 }
 
+int swr_atom_energy(swr_system *S) {
+  if (!S) return SWR_ERR_ARG;
+  int rc;
+  if ((rc = swr_quiesce(S))) return rc;
+  return swr_atom_energy_run(S);
+}
+
+int swr_add_dbond_forces(swr_system *S) {
+  if (!S) return SWR_ERR_ARG;
+  int rc;
+  if ((rc = swr_quiesce(S))) return rc;
+  return swr_dbond_forces_run(S);
+}
+
 int swr_sync(swr_system *S) {
   if (!S) return SWR_ERR_ARG;
   int rc = refresh_status(S);
@@ -670,6 +705,7 @@ bool lookup(swr_system *S, const std::string &k, View &v) {
   if (k == "fCdDelta") return set(S->fcd, 32 * N);
   if (k == "nonbonded_energy") return set(S->nb_result, S->nb_result ? 64 : 0);
   if (k == "nonbonded_eatom") return set(S->nb_eatom, S->nb_eatom ? 8 * n : 0);
+  if (k == "bonded_energy") return set(S->be_result, S->be_result ? 40 : 0);
   static const char *wsn[WS_COUNT] = {"Deltap", "Deltap_boc", "Delta", "Delta_e", "Delta_val", "vlpex", "nlp",
                                       "nlp_temp", "Clp", "Delta_lp", "dDelta_lp", "Delta_lp_temp", "dDelta_lp_temp"};
   for (int w = 0; w < WS_COUNT; w++)

@@ -57,6 +57,7 @@ struct PairReaxCAMD {
       } else if (key == "enobonds" || key == "lgvdw") {
         if (v != "yes" && v != "no") return "Illegal pair_style reax/c command";
         if (key == "lgvdw" && v == "yes") return "lgvdw yes is not supported on this path";
+        if (key == "enobonds") ctl.enobonds = v == "yes" ? 1 : 0;
       } else if (key == "safezone") {
         ctl.safezone = std::atof(v.c_str());
         if (ctl.safezone < 0.0) return "Illegal pair_style reax/c safezone command";

@@ -59,6 +59,11 @@ struct DevNb {
   double ecore, acore, rcoreinv, a_div_rcore;             // inner wall
 };
 
+// Bond and atom energies (Merge_Bonds_Atom_Energy_C_New), element-pair indexed.
+struct DevBe {
+  double De_s, De_p, De_pp, p_be1, p_be2, p_ovun1;
+};
+
 struct DevParams {
   int nelem, ntypes;
   int map[SWR_MAX_TYPES + 1];
@@ -87,6 +92,14 @@ struct DevParams {
   double nb_tap[8];
   double p_vdW1, p_vdW1i;
   int vdw_type;
+  // Bond and atom energies: element parameters, the pair block, and the
+  // general parameters l[3..10], l[31], l[32], l[37] they read
+  double p_lp2[SWR_MAX_ELEM], p_ovun2[SWR_MAX_ELEM], p_ovun5[SWR_MAX_ELEM], chi_e[SWR_MAX_ELEM],
+      eta_e[SWR_MAX_ELEM];
+  int is_carbon[SWR_MAX_ELEM];
+  DevBe be[SWR_MAX_ELEM * SWR_MAX_ELEM];
+  double p_lp3, p_ovun3, p_ovun4, p_ovun6, p_ovun7, p_ovun8, gp3, gp4, gp7, gp10;
+  int gp37, enobonds;
 };
 
 // Small block of device-written results the host reads back asynchronously.
@@ -127,7 +140,8 @@ struct CgScalars {
 
 // Kernel timer slots.
 enum { KT_VERLET = 0, KT_FAR, KT_BUILD_H, KT_CG_ITER, KT_CG_SETUP, KT_BO, KT_VALIDATE, KT_SPMV, KT_RESET,
-       KT_QEQ_Q, KT_BINS, KT_MIXED_INNER, KT_MIXED_OUTER, KT_BONDS, KT_CG_SCALARS, KT_MARK, KT_CG_UPDATE, KT_NONBONDED, KT_COUNT };
+       KT_QEQ_Q, KT_BINS, KT_MIXED_INNER, KT_MIXED_OUTER, KT_BONDS, KT_CG_SCALARS, KT_MARK, KT_CG_UPDATE, KT_NONBONDED, KT_ATOM_ENERGY,
+       KT_DBOND, KT_COUNT };
 
 struct swr_system {
   // ---- host-side configuration
@@ -194,6 +208,13 @@ struct swr_system {
   double *nb_result = nullptr;     // eng_vdwl, eng_coul, virial[6] of the last non-bonded pass
   double *nb_eatom = nullptr;      // per local atom: its half of its pairs' vdW + Coulomb energy
   size_t nb_eatom_cap = 0;
+  // bond and atom energies, dBO forces
+  double *be_part = nullptr;       // [5][cap] per local atom energies
+  double *be_ce = nullptr;         // [2][cap] CEover4, CEunder4 per local atom
+  double *be_bufc = nullptr;       // [cap] per atom buf_c of the dBO chain rule
+  double *be_result = nullptr;     // e_bond, e_lp, e_ov, e_un, e_pol of the last Atom_Energy
+  size_t cap_be_part = 0, cap_be_ce = 0, cap_be_bufc = 0, cap_be_result = 0;
+  size_t be_stride = 0;            // row stride of be_part/be_ce (Ncap when they were sized)
   double *bo_dboc = nullptr;       // [N][2]
   double *dDeltap_self = nullptr;  // [N][3]
   int *bond_mark = nullptr, *bond_mark2 = nullptr;
@@ -304,6 +325,8 @@ int swr_lists_mark(swr_system *S, int max_sweeps);
 int swr_mark_settle(swr_system *S);
 int swr_after_sync(swr_system *S);
 int swr_nonbonded_run(swr_system *S);
+int swr_atom_energy_run(swr_system *S);
+int swr_dbond_forces_run(swr_system *S);
 int swr_quiesce(swr_system *S);
 constexpr int MARK_INNER = 8;         // bond_mark relaxation sweeps per launch
 constexpr int MARK_MAX_LAUNCH = 128;  // launches of one relaxation at most
