@@ -234,8 +234,10 @@ int swr_vdw_coulomb_energy(swr_system *sys);
  * the bond slots and to CdDelta (fCdDelta[i][3]) of the atoms and their
  * bonded neighbours (ghosts included); leaves e_bond, e_lp, e_ov, e_un,
  * e_pol in swr_get(sys, "bonded_energy") (5 doubles; e_pol is the plain sum
- * over atoms, the value the reference adds to eng_coul).  Run after swr_bo
- * of the step; enqueues only.  The pair_style keyword enobonds is
+ * over atoms, the value the reference adds to eng_coul) and each local
+ * atom's share in "bonded_eatom" (5 rows of be_stride doubles, the row
+ * stride being swr_get_scalar(sys, "bonded_stride")).  Run after swr_bo of
+ * the step; enqueues only.  The pair_style keyword enobonds is
  * swr_control_t.enobonds. */
 int swr_atom_energy(swr_system *sys);
 

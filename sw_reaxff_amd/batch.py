@@ -155,10 +155,11 @@ class Batch:
         store.n = n
         return store
 
-    def evaluate(self, nonbonded=False):
-        """Neighbour list, QEq (per-system), far/bond/hbond lists, BO and, with
-        nonbonded, the vdW + Coulomb energies and gradient; returns max CG
-        iterations (s, t)."""
+    def evaluate(self, nonbonded=False, bonded=False):
+        """Neighbour list, QEq (per-system), far/bond/hbond lists, BO; with
+        nonbonded the vdW + Coulomb energies and gradient, with bonded the
+        bond and atom energies and the bond-order chain rule of the gradient;
+        returns max CG iterations (s, t)."""
         h = self.sys
         _lib.check(h.L.swr_npair_full_bin_ghost_build(h.h, C.c_double(CUTGHOST)), "swr_npair_full_bin_ghost_build")
         a, b = C.c_int(0), C.c_int(0)
@@ -166,6 +167,9 @@ class Batch:
         _lib.check(h.L.swr_pair_compute(h.h), "swr_pair_compute")
         if nonbonded:
             _lib.check(h.L.swr_vdw_coulomb_energy(h.h), "swr_vdw_coulomb_energy")
+        if bonded:
+            _lib.check(h.L.swr_atom_energy(h.h), "swr_atom_energy")
+            _lib.check(h.L.swr_add_dbond_forces(h.h), "swr_add_dbond_forces")
         return a.value, b.value
 
     def sync(self):
@@ -188,11 +192,27 @@ class Batch:
         P = self.packed
         return np.array([e[P.row_off[k]:P.row_off[k + 1]].sum() for k in range(len(P.row_off) - 1)])
 
-    def gradients(self):
-        """Per system, the non-bonded energy gradient of its local atoms (n_k x 3)."""
-        g = self.sys.get("fCdDelta").reshape(-1, 4)
+    def bonded_energies(self):
+        """Per system, (e_bond, e_lp, e_ov, e_un, e_pol) summed over its local
+        atoms (after evaluate(bonded=True))."""
+        h = self.sys
+        stride = h.scalar("bonded_stride")
+        e = h.get("bonded_eatom").reshape(5, stride)
         P = self.packed
-        return [g[P.row_off[k]:P.row_off[k + 1], :3] for k in range(len(P.row_off) - 1)]
+        return np.array([e[:, P.row_off[k]:P.row_off[k + 1]].sum(axis=1) for k in range(len(P.row_off) - 1)])
+
+    def gradients(self):
+        """Per system, the energy gradient of its local atoms (n_k x 3) from
+        the passes evaluate() ran, ghost entries summed onto their owners (as
+        LAMMPS' reverse communication does)."""
+        h = self.sys
+        P = self.packed
+        N = h.scalar("N")
+        g = h.get("fCdDelta").reshape(-1, 4)[:N, :3]
+        tot = g[:P.n].copy()
+        if N > P.n:
+            np.add.at(tot, h.get("owner")[P.n:N], g[P.n:N])
+        return [tot[P.row_off[k]:P.row_off[k + 1]] for k in range(len(P.row_off) - 1)]
 
     def iterations(self):
         it = self.sys.get("sys_iters")

@@ -36,6 +36,8 @@
 #include "swr_internal.h"
 #include "swr_device.h"
 
+#include <algorithm>
+
 namespace {
 
 constexpr int BT = 256;
@@ -365,7 +367,7 @@ int swr_atom_energy_run(swr_system *S) {
     return SWR_ERR_STATE;
   }
   int rc;
-  const size_t stride = (size_t)S->Ncap;
+  const size_t stride = (size_t)std::max(S->ncap, 1);
   if ((rc = swr_reserve(&S->be_part, S->cap_be_part, 5 * stride)) ||
       (rc = swr_reserve(&S->be_ce, S->cap_be_ce, 2 * stride)) ||
       (rc = swr_reserve(&S->be_result, S->cap_be_result, 8)))

@@ -706,6 +706,7 @@ bool lookup(swr_system *S, const std::string &k, View &v) {
   if (k == "nonbonded_energy") return set(S->nb_result, S->nb_result ? 64 : 0);
   if (k == "nonbonded_eatom") return set(S->nb_eatom, S->nb_eatom ? 8 * n : 0);
   if (k == "bonded_energy") return set(S->be_result, S->be_result ? 40 : 0);
+  if (k == "bonded_eatom") return set(S->be_part, S->be_part ? 40 * (int64_t)S->be_stride : 0);
   static const char *wsn[WS_COUNT] = {"Deltap", "Deltap_boc", "Delta", "Delta_e", "Delta_val", "vlpex", "nlp",
                                       "nlp_temp", "Clp", "Delta_lp", "dDelta_lp", "Delta_lp_temp", "dDelta_lp_temp"};
   for (int w = 0; w < WS_COUNT; w++)
@@ -814,6 +815,7 @@ int64_t swr_get_scalar(swr_system *S, const char *name) {
   }
   if (k == "nsys") return S->nsys;
   if (k == "mixed_outer") return S->mixed_outer;
+  if (k == "bonded_stride") return (int64_t)S->be_stride;
   // first outer residual of the last mixed solve, sqrt(u.r)/|b| x 1e15
   if (k == "mixed_rho0_s_x1e15") return (int64_t)(S->mixed_rho0[0] * 1e15);
   if (k == "mixed_rho0_t_x1e15") return (int64_t)(S->mixed_rho0[1] * 1e15);

@@ -1,8 +1,9 @@
 """Serving loop: evaluation requests of independent systems, continuously batched.
 
 Requests arrive over time; each is either a new system to evaluate once
-(neighbour list, QEq charges, far/bond/hbond lists, bond orders, non-bonded
-energy and gradient) or a follow-on MD step of a resident system.  The
+(neighbour list, QEq charges, far/bond/hbond lists, bond orders, the
+non-bonded and bond/atom energies and their gradient) or a follow-on MD
+step of a resident system.  The
 server:
 
   * runs `lanes` workers, each owning one device system (its own HIP stream).
@@ -64,7 +65,7 @@ class Request:
     batch_size: int = 0
     step_index: int = 0             # kind "step": which step of its resident system this was
     hist: tuple | None = None       # resident's first evaluation: the s/t histories it left
-    energy: float | None = None     # new systems: non-bonded (vdW + Coulomb) energy, kcal/mol
+    energy: float | None = None     # vdW + Coulomb + bond/atom (bond, lone pair, over/under, polarization) energy
 
     @property
     def latency(self):
@@ -301,7 +302,7 @@ class Server:
             syss.append(r.system)
         bt.load(syss, hist=hists if batched else None)
         t = self._account("pack_load", t)
-        bt.evaluate(nonbonded=True)
+        bt.evaluate(nonbonded=True, bonded=True)
         # warm starts for the residents' next steps, enqueued before the sync
         stores = {}
         if batched:
@@ -314,7 +315,7 @@ class Server:
         t = self._account("evaluate", t)
         qs = bt.charges()
         its = bt.iterations()
-        es = bt.energies()
+        es = bt.energies() + bt.bonded_energies().sum(axis=1)
         hists = bt.histories() if not batched and any(r.resident >= 0 for r in reqs) else None
         for k, (r, q, it) in enumerate(zip(reqs, qs, its)):
             r.q = q.copy()
@@ -387,7 +388,7 @@ def warm_lanes(batches, max_batch_atoms: int, seed: int = 7):
     big = [make_system("water", 20000, seed=seed + k) for k in range(max(1, max_batch_atoms // 20000))]
     for bt in batches:
         bt.load(big)
-        bt.evaluate(nonbonded=True)
+        bt.evaluate(nonbonded=True, bonded=True)
         bt.sync()
 
 

@@ -51,16 +51,17 @@ def test_alone_equals_in_the_mix(gpu_lib, tol, qtol):
     parts = mix()
     bt = B.Batch(tol=tol)
     bt.load(parts)
-    bt.evaluate(nonbonded=True)
+    bt.evaluate(nonbonded=True, bonded=True)
     bt.sync()
     q_mix = bt.charges()
     it_mix = bt.iterations()
     e_mix = bt.energies()
+    eb_mix = bt.bonded_energies()
     g_mix = [g.copy() for g in bt.gradients()]
     for k, s in enumerate(parts):
         alone = B.Batch(tol=tol)
         alone.load([s])
-        alone.evaluate(nonbonded=True)
+        alone.evaluate(nonbonded=True, bonded=True)
         alone.sync()
         q1 = alone.charges()[0]
         it1 = alone.iterations()[0]
@@ -81,9 +82,16 @@ def test_alone_equals_in_the_mix(gpu_lib, tol, qtol):
         e1, g1 = alone.energies()[0], alone.gradients()[0]
         de = abs(e1 - e_mix[k]) / abs(e1)
         dg = np.max(np.abs(g1 - g_mix[k])) / np.abs(g1).max()
-        print(f"[batch] system {k}: E_nb {e1:.10e} rel diff {de:.2e}, gradient rel diff {dg:.2e}", flush=True)
+        # bond and atom energies do not depend on the charges (but for e_pol)
+        eb1 = alone.bonded_energies()[0]
+        deb = np.abs(eb1[:4] - eb_mix[k][:4]) / np.maximum(np.abs(eb1[:4]), 1e-3)
+        dpol = abs(eb1[4] - eb_mix[k][4]) / max(abs(eb1[4]), 1.0)
+        print(f"[batch] system {k}: E_nb {e1:.10e} rel diff {de:.2e}, bond/atom {eb1} rel diff {deb.max():.2e} "
+              f"pol {dpol:.2e}, gradient rel diff {dg:.2e}", flush=True)
         assert de <= 100 * qtol, k
         assert dg <= 1000 * qtol, k
+        assert deb.max() <= 1e-10, k
+        assert dpol <= 100 * qtol, k
 
 
 def test_mixed_batch(gpu_lib):
