@@ -367,7 +367,7 @@ int swr_atom_energy_run(swr_system *S) {
     return SWR_ERR_STATE;
   }
   int rc;
-  const size_t stride = (size_t)std::max(S->ncap, 1);
+  const size_t stride = (size_t)std::max(n, 1);   // rows as long as the local atoms: the getter copies 5n
   if ((rc = swr_reserve(&S->be_part, S->cap_be_part, 5 * stride)) ||
       (rc = swr_reserve(&S->be_ce, S->cap_be_ce, 2 * stride)) ||
       (rc = swr_reserve(&S->be_result, S->cap_be_result, 8)))

@@ -214,7 +214,7 @@ struct swr_system {
   double *be_bufc = nullptr;       // [cap] per atom buf_c of the dBO chain rule
   double *be_result = nullptr;     // e_bond, e_lp, e_ov, e_un, e_pol of the last Atom_Energy
   size_t cap_be_part = 0, cap_be_ce = 0, cap_be_bufc = 0, cap_be_result = 0;
-  size_t be_stride = 0;            // row stride of be_part/be_ce (the local capacity when they were sized)
+  size_t be_stride = 0;            // row stride of be_part/be_ce (the local atom count of the last pass)
   double *bo_dboc = nullptr;       // [N][2]
   double *dDeltap_self = nullptr;  // [N][3]
   int *bond_mark = nullptr, *bond_mark2 = nullptr;
