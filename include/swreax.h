@@ -241,6 +241,17 @@ int swr_vdw_coulomb_energy(swr_system *sys);
  * swr_control_t.enobonds. */
 int swr_atom_energy(swr_system *sys);
 
+/* Hydrogen bonds: Hydrogen_Bonds (reaxc_forces_sunway.cpp:123-129) =
+ * Hydrogen_Bonds_C (reaxc_hydrogen_bonds_sw64.c:26-283): for every local H
+ * atom j, bond j-i to an acceptor with BO >= 0.01 and hydrogen-bond partner
+ * k of j (orig_id(k) != orig_id(i), r0_hb(i,j,k) > 0 in the force field's
+ * hydrogen-bond block): the energy, CEhb1 into Cdbo of j's slot, and the
+ * angle and distance terms into fCdDelta[.][0..2] of i, j and k.  Leaves
+ * e_hb in swr_get(sys, "hbond_energy") and each local H atom's share in
+ * "hbond_eatom".  Run after swr_atom_energy and before
+ * swr_add_dbond_forces, as the reference orders them; enqueues only. */
+int swr_hydrogen_bonds(swr_system *sys);
+
 /* Add_All_dBond_to_Forces_C (reaxc_forces_sw64.c:417-589): over all N atoms
  * and their bonds, the chain rule of the bond-order derivatives with the
  * coefficients Cdbo/Cdbopi/Cdbopi2 of both halves of each bond and the
@@ -284,7 +295,8 @@ int swr_device_synchronize(void);
  * hbond_index hbond_end hbonds hbond_far bo_dboc dDeltap_self bond_mark
  * Deltap Deltap_boc Delta Delta_e Delta_val vlpex nlp nlp_temp Clp Delta_lp
  * dDelta_lp Delta_lp_temp dDelta_lp_temp H_firstnbr H_numnbrs H_jlist H_val
- * Hdia_inv b_s b_t s t s_hist t_hist fCdDelta nonbonded_energy nonbonded_eatom bonded_energy.  hbonds' ptr
+ * Hdia_inv b_s b_t s t s_hist t_hist fCdDelta nonbonded_energy nonbonded_eatom bonded_energy
+ * bonded_eatom hbond_energy hbond_eatom.  hbonds' ptr
  * fields come back as
  * device addresses; hbond_far gives each hbond's far-list slot. */
 int64_t swr_get(swr_system *sys, const char *name, void *out);

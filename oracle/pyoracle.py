@@ -53,6 +53,8 @@ def lib():
         L.orc_vdw_coulomb.argtypes = [vp, dp]
         L.orc_atom_energy.argtypes = [vp, C.c_int, dp]
         L.orc_dbond_forces.argtypes = [vp]
+        L.orc_hydrogen_bonds.restype = C.c_double
+        L.orc_hydrogen_bonds.argtypes = [vp]
         L.orc_get_array.restype = C.c_int64
         L.orc_get_array.argtypes = [vp, C.c_char_p, vp]
         L.orc_set_array.argtypes = [vp, C.c_char_p, vp, C.c_int64]
@@ -193,6 +195,11 @@ class Oracle:
         o = np.zeros(5)
         self.L.orc_atom_energy(self.h, int(enobonds), _p(o, C.c_double))
         return o
+
+    def hydrogen_bonds(self):
+        """Hydrogen_Bonds over the current bonds and hbond lists: adds CEhb1 to
+        Cdbo and the angle/distance terms to fCdDelta[:, 0..2]; returns e_hb."""
+        return float(self.L.orc_hydrogen_bonds(self.h))
 
     def dbond_forces(self):
         """Add_All_dBond_to_Forces: adds the bond-order chain rule to

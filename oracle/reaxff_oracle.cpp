@@ -56,6 +56,10 @@ struct SBP {
   double p_ovun2, p_val3, valency_val, p_val5, rcore2, ecore2, acore2;
 };
 
+struct HBP {
+  double r0_hb, p_hb1, p_hb2, p_hb3;
+};
+
 struct TBP {
   double p_bo1, p_bo2, p_bo3, p_bo4, p_bo5, p_bo6;
   double r_s, r_p, r_pp;
@@ -92,6 +96,7 @@ struct orc_sys {
   std::vector<double> gp;
   std::vector<SBP> sbp;
   std::vector<std::vector<TBP>> tbp;
+  std::vector<HBP> hbp;          // [nt][nt][nt], r0_hb < 0: no hydrogen bond
   double bo_cut = 0, nonb_low = 0, nonb_cut = 0;
   int vdw_type = 0;
   double bond_cut = 5.0, hbond_cut = 7.5;
@@ -156,7 +161,8 @@ namespace {
 
 // ------------------------------------------------------------------------
 // Force field: Read_Force_Field, reaxc_ffield_sunway.cpp:35-386 (the part
-// this path consumes: globals, atoms, bonds, off-diagonal overrides).
+// this path consumes: globals, atoms, bonds, off-diagonal overrides, and the
+// hydrogen-bond block :654-686 after skipping the angle and torsion blocks).
 // ------------------------------------------------------------------------
 bool read_ffield(orc_sys *S, const char *path) {
   FILE *fp = std::fopen(path, "r");
@@ -274,6 +280,23 @@ bool read_ffield(orc_sys *S, const char *path) {
       if ((val = tokd(t, 6)) > 0.0) S->tbp[j][kk].r_p = S->tbp[kk][j].r_p = val;
       if ((val = tokd(t, 7)) > 0.0) S->tbp[j][kk].r_pp = S->tbp[kk][j].r_pp = val;
     }
+  }
+  // valence-angle and torsion blocks (:547-...) are skipped by their counts;
+  // the hydrogen-bond block: r0_hb = -1 unless listed, then j k m r0 p1 p2 p3
+  S->hbp.assign((size_t)nt * nt * nt, HBP{-1.0, 0.0, 0.0, 0.0});
+  for (int blk = 0; blk < 2; blk++) {
+    t = next();
+    const int l = t.empty() ? 0 : std::atoi(t[0].c_str());
+    for (int k = 0; k < l; k++) next();
+  }
+  t = next();
+  const int nh = t.empty() ? 0 : std::atoi(t[0].c_str());
+  for (int k = 0; k < nh; k++) {
+    t = next();
+    if (t.size() < 7) break;
+    const int a = std::atoi(t[0].c_str()) - 1, b = std::atoi(t[1].c_str()) - 1, c = std::atoi(t[2].c_str()) - 1;
+    if (a < nt && c < nt && a >= 0 && b >= 0 && b < nt && c >= 0)
+      S->hbp[((size_t)a * nt + b) * nt + c] = HBP{tokd(t, 3), tokd(t, 4), tokd(t, 5), tokd(t, 6)};
   }
   return true;
 }
@@ -1385,6 +1408,93 @@ int orc_atom_energy(orc_sys *S, int enobonds, double *out) {
   out[3] = e_un_sum;
   out[4] = e_pol;
   return 0;
+}
+
+// Hydrogen_Bonds (reaxc_forces_sunway.cpp:123-129) = Hydrogen_Bonds_C
+// (reaxc_hydrogen_bonds_sw64.c:26-283), its host loop (the #else branch
+// :145-253; the CPE form reaxc_hydrogen_bonds_cpe.h evaluates the same
+// terms): for every local H atom j (p_hbond 1), every bond j-i to an
+// acceptor (p_hbond 2) with BO >= HB_THRESHOLD = 0.01 (reaxc_defs_sunway.h:91)
+// and every hydrogen-bond partner k of j with orig_id(k) != orig_id(i) and
+// r0_hb(i, j, k) > 0: the energy, CEhb1 into Cdbo of j's slot, and the
+// angle and distance terms into fCdDelta of i, j and k.  The angle through
+// Calculate_Theta / Calculate_dCos_Theta (:336-372) with libm acos and pow
+// where the reference calls xacos and p_powd.  Departure: an atom of type
+// < 0 is skipped before its p_hbond is read.  Returns e_hb.
+double orc_hydrogen_bonds(orc_sys *S) {
+  const int n = S->n;
+  const int nt = (int)S->sbp.size();
+  const double HB_THRESHOLD = 1e-2;
+  double *fc = S->fCdDelta.data();
+  double e_hb_total = 0.0;
+  std::vector<int> hblist;
+  for (int j = 0; j < n; j++) {
+    const int tj = S->packed[j].type;
+    if (tj < 0 || S->sbp[tj].p_hbond != 1) continue;
+    hblist.clear();
+    for (int pi = S->bindex[j]; pi < S->bend[j]; ++pi) {
+      const int i = S->bonds[pi].nbr;
+      const int ti = S->packed[i].type;
+      if (ti < 0) continue;
+      if (S->sbp[ti].p_hbond == 2 && S->BO[pi] >= HB_THRESHOLD) hblist.push_back(pi);
+    }
+    const int H = S->Hindex[j];
+    if (H < 0) continue;
+    for (int pk = S->hindex[H]; pk < S->hend[H]; ++pk) {
+      const int k = S->hbonds[pk].nbr;
+      const int tk = S->packed[k].type;
+      if (tk < 0) continue;
+      const swr_far_neighbor_t &fe = S->far[S->hbond_far[pk]];
+      const double r_jk = fe.d;
+      double dvec_jk[3];
+      for (int c = 0; c < 3; c++) dvec_jk[c] = S->hbonds[pk].scl * fe.dvec[c];
+      for (const int pi : hblist) {
+        const swr_bond_t &bij = S->bonds[pi];
+        const int i = bij.nbr;
+        if (S->packed[i].orig_id == S->packed[k].orig_id) continue;
+        const int ti = S->packed[i].type;
+        const HBP &hbp = S->hbp[((size_t)ti * nt + tj) * nt + tk];
+        if (hbp.r0_hb <= 0.0) continue;
+        // Calculate_Theta
+        const double *dvec_ji = bij.dvec;
+        const double d_ji = bij.d;
+        const double dot = dvec_ji[0] * dvec_jk[0] + dvec_ji[1] * dvec_jk[1] + dvec_ji[2] * dvec_jk[2];
+        double cos_theta = dot / (d_ji * r_jk);
+        if (cos_theta > 1.) cos_theta = 1.0;
+        if (cos_theta < -1.) cos_theta = -1.0;
+        const double theta = std::acos(cos_theta);
+        // Calculate_dCos_Theta
+        const double sqr_d_ji = SQR(d_ji), sqr_d_jk = SQR(r_jk);
+        const double inv_dists = 1.0 / (d_ji * r_jk);
+        const double inv_dists3 = std::pow(inv_dists, 3.0);
+        const double Cdot_inv3 = dot * inv_dists3;
+        double di[3], dj[3], dk[3];
+        for (int c = 0; c < 3; c++) {
+          di[c] = dvec_jk[c] * inv_dists - Cdot_inv3 * sqr_d_jk * dvec_ji[c];
+          dj[c] = -(dvec_jk[c] + dvec_ji[c]) * inv_dists + Cdot_inv3 * (sqr_d_jk * dvec_ji[c] + sqr_d_ji * dvec_jk[c]);
+          dk[c] = dvec_ji[c] * inv_dists - Cdot_inv3 * sqr_d_ji * dvec_jk[c];
+        }
+        const double sin_theta2 = std::sin(theta / 2.0);
+        double sin_xhz4 = SQR(sin_theta2);
+        sin_xhz4 *= sin_xhz4;
+        const double cos_xhz1 = (1.0 - cos_theta);
+        const double exp_hb2 = std::exp(-hbp.p_hb2 * S->BO[pi]);
+        const double exp_hb3 = std::exp(-hbp.p_hb3 * (hbp.r0_hb / r_jk + r_jk / hbp.r0_hb - 2.0));
+        const double e_hb = hbp.p_hb1 * (1.0 - exp_hb2) * exp_hb3 * sin_xhz4;
+        e_hb_total += e_hb;
+        const double CEhb1 = hbp.p_hb1 * hbp.p_hb2 * exp_hb2 * exp_hb3 * sin_xhz4;
+        const double CEhb2 = -hbp.p_hb1 / 2.0 * (1.0 - exp_hb2) * exp_hb3 * cos_xhz1;
+        const double CEhb3 = -hbp.p_hb3 * (-hbp.r0_hb / SQR(r_jk) + 1.0 / hbp.r0_hb) * e_hb;
+        S->Cdbo[pi] += CEhb1;
+        for (int c = 0; c < 3; c++) fc[4 * (size_t)i + c] += CEhb2 * di[c];
+        for (int c = 0; c < 3; c++) fc[4 * (size_t)j + c] += CEhb2 * dj[c];
+        for (int c = 0; c < 3; c++) fc[4 * (size_t)k + c] += CEhb2 * dk[c];
+        for (int c = 0; c < 3; c++) fc[4 * (size_t)j + c] += -CEhb3 / r_jk * dvec_jk[c];
+        for (int c = 0; c < 3; c++) fc[4 * (size_t)k + c] += CEhb3 / r_jk * dvec_jk[c];
+      }
+    }
+  }
+  return e_hb_total;
 }
 
 // Add_All_dBond_to_Forces_C (reaxc_forces_sw64.c:417-589), its host loop

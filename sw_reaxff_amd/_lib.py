@@ -30,7 +30,7 @@ EXPORTS = [
     "swr_lmp_system", "swr_lmp_stat", "swr_ktimer_enable", "swr_ktimer_read", "swr_device_synchronize",
     "swr_sparse_matvec_h", "swr_npair_full_bin_atomonly_build", "swr_sort_bonds", "swr_set_batch",
     "swr_new_atoms", "swr_batch_pack", "swr_vdw_coulomb_energy", "swr_hist_save", "swr_hist_load",
-    "swr_hist_free", "swr_atom_energy", "swr_add_dbond_forces",
+    "swr_hist_free", "swr_atom_energy", "swr_add_dbond_forces", "swr_hydrogen_bonds",
 ]
 
 
@@ -73,7 +73,7 @@ def lib():
                    "swr_write_reax_lists_init_forces", "swr_validate_lists", "swr_sort_bonds", "swr_bo",
                    "swr_qeq_init_matvec",
                    "swr_calculate_q", "swr_pair_compute", "swr_sync", "swr_vdw_coulomb_energy",
-                   "swr_atom_energy", "swr_add_dbond_forces"]:
+                   "swr_atom_energy", "swr_add_dbond_forces", "swr_hydrogen_bonds"]:
             getattr(L, nm).argtypes = [vp]
         L.swr_sparse_matvec.argtypes = [vp, vp, vp]
         L.swr_sparse_matvec_h.argtypes = [vp, dp, dp]

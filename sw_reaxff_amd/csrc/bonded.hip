@@ -358,6 +358,225 @@ __global__ void __launch_bounds__(BT) k_dbond_nbr(int N, DbArgs A) {
   A.fcd[4 * (size_t)k + 2] = f2;
 }
 
+// ---------------------------------------------------------------- hydrogen bonds
+// Hydrogen_Bonds_C (reaxc_hydrogen_bonds_sw64.c:26-283, host loop :145-253):
+// a triple is a local H atom j (p_hbond 1), a bond j-i to an acceptor
+// (p_hbond 2) with BO >= 0.01, and a hydrogen-bond partner k of j with
+// orig_id(k) != orig_id(i) and r0_hb(i, j, k) > 0.  Its energy and CEhb1
+// (into Cdbo of j's slot) and the angle/distance terms into fCdDelta of i,
+// j and k.  Three kernels, each writing only its own atom's entries and
+// each recomputing the triples that touch its atom: k_hb_own (the H atom
+// j: energy, Cdbo, its own terms), k_hb_i (the bonded acceptor i, through
+// its bonds to local H atoms and their hydrogen-bond lists) and k_hb_k (the
+// partner acceptor k, through its far list: k is in j's hydrogen-bond list
+// exactly when j is a local H atom and their far entry has d <= hbond_cut,
+// the rule the list pass applies, with the same d and the opposite dvec).
+// A wave per atom, lanes over hydrogen-bond or far entries; the lane sums
+// are combined by a fixed butterfly.
+constexpr double HB_THRESHOLD = 1e-2;   // reaxc_defs_sunway.h:91
+
+struct HbArgs {
+  const DevParams *P;
+  const swr_atom_pack_t *atoms;
+  const int *bidx, *bend;
+  const swr_bond_t *bonds;
+  const double *BO;
+  const int *Hindex, *hidx, *hend;
+  const swr_hbond_t *hbonds;
+  const int *far_index, *far_end;
+  const swr_far_neighbor_t *far;
+  double *Cdbo, *fcd, *part;
+  double hbond_cut;
+};
+
+// One triple (Calculate_Theta, Calculate_dCos_Theta :336-372, energy and
+// coefficients :224-235); the cube of 1/(d_ji d_jk) by two products where
+// the reference calls p_powd.
+struct HbTriple {
+  double e, ce1, ce2, ce3;
+  double di[3], dj[3], dk[3];
+};
+
+__device__ __forceinline__ void hb_triple(const DevHb &hbp, double BOij, const double *dvec_ji, double d_ji,
+                                          const double *dvec_jk, double r_jk, HbTriple &t) {
+  const double dot = dvec_ji[0] * dvec_jk[0] + dvec_ji[1] * dvec_jk[1] + dvec_ji[2] * dvec_jk[2];
+  double cos_theta = dot / (d_ji * r_jk);
+  if (cos_theta > 1.) cos_theta = 1.0;
+  if (cos_theta < -1.) cos_theta = -1.0;
+  const double sqr_d_ji = d_ji * d_ji, sqr_d_jk = r_jk * r_jk;
+  const double inv_dists = 1.0 / (d_ji * r_jk);
+  const double inv_dists3 = inv_dists * inv_dists * inv_dists;
+  const double Cdot_inv3 = dot * inv_dists3;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    t.di[c] = dvec_jk[c] * inv_dists - Cdot_inv3 * sqr_d_jk * dvec_ji[c];
+    t.dj[c] = -(dvec_jk[c] + dvec_ji[c]) * inv_dists + Cdot_inv3 * (sqr_d_jk * dvec_ji[c] + sqr_d_ji * dvec_jk[c]);
+    t.dk[c] = dvec_ji[c] * inv_dists - Cdot_inv3 * sqr_d_ji * dvec_jk[c];
+  }
+  // sin^4(theta/2) = ((1 - cos theta)/2)^2: the reference goes through acos
+  // and sin; the identity gives the same value to rounding without either
+  const double cos_xhz1 = (1.0 - cos_theta);
+  double sin_xhz4 = 0.5 * cos_xhz1;
+  sin_xhz4 *= sin_xhz4;
+  const double exp_hb2 = exp(-hbp.p_hb2 * BOij);
+  const double exp_hb3 = exp(-hbp.p_hb3 * (hbp.r0_hb / r_jk + r_jk / hbp.r0_hb - 2.0));
+  t.e = hbp.p_hb1 * (1.0 - exp_hb2) * exp_hb3 * sin_xhz4;
+  t.ce1 = hbp.p_hb1 * hbp.p_hb2 * exp_hb2 * exp_hb3 * sin_xhz4;
+  t.ce2 = -hbp.p_hb1 / 2.0 * (1.0 - exp_hb2) * exp_hb3 * cos_xhz1;
+  t.ce3 = -hbp.p_hb3 * (-hbp.r0_hb / (r_jk * r_jk) + 1.0 / hbp.r0_hb) * t.e;
+}
+
+__device__ __forceinline__ const DevHb &hb_params(const DevParams *P, int ti, int tj, int tk) {
+  return P->hb[(ti * SWR_MAX_ELEM + tj) * SWR_MAX_ELEM + tk];
+}
+
+// The H atom j: energy, CEhb1 of its acceptor bonds, its own terms.
+__global__ void __launch_bounds__(BT) k_hb_own(int n, HbArgs A) {
+  const int j = blockIdx.x * (BT / SWR_WAVE) + wave_in_block();
+  if (j >= n) return;
+  const int lane = lane_id();
+  const DevParams *__restrict__ P = A.P;
+  const swr_atom_pack_t aj = A.atoms[j];
+  const int tj = aj.type;
+  const int H = (tj >= 0 && P->p_hbond[tj] == 1) ? A.Hindex[j] : -1;
+  if (H < 0) {
+    if (lane == 0) A.part[j] = 0.0;
+    return;
+  }
+  const int h0 = A.hidx[H], h1 = A.hend[H];
+  double e = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  for (int pi = A.bidx[j]; pi < A.bend[j]; ++pi) {
+    const swr_bond_t bij = A.bonds[pi];
+    const int i = bij.nbr;
+    const int ti = A.atoms[i].type;
+    const double BOij = A.BO[pi];
+    if (ti < 0 || P->p_hbond[ti] != 2 || !(BOij >= HB_THRESHOLD)) continue;
+    const int oi = A.atoms[i].orig_id;
+    double ce1 = 0.0;
+    for (int pk = h0 + lane; pk < h1; pk += SWR_WAVE) {
+      const swr_hbond_t hb = A.hbonds[pk];
+      const int k = hb.nbr;
+      const swr_atom_pack_t ak = A.atoms[k];
+      if (ak.type < 0 || ak.orig_id == oi) continue;
+      const DevHb &hbp = hb_params(P, ti, tj, ak.type);
+      if (hbp.r0_hb <= 0.0) continue;
+      const swr_far_neighbor_t *fe = hb.ptr;
+      const double r_jk = fe->d;
+      const double dvec_jk[3] = {hb.scl * fe->dvec[0], hb.scl * fe->dvec[1], hb.scl * fe->dvec[2]};
+      HbTriple t;
+      hb_triple(hbp, BOij, bij.dvec, bij.d, dvec_jk, r_jk, t);
+      e += t.e;
+      ce1 += t.ce1;
+      f0 += t.ce2 * t.dj[0] - t.ce3 / r_jk * dvec_jk[0];
+      f1 += t.ce2 * t.dj[1] - t.ce3 / r_jk * dvec_jk[1];
+      f2 += t.ce2 * t.dj[2] - t.ce3 / r_jk * dvec_jk[2];
+    }
+    ce1 = wave_sum(ce1);
+    if (lane == 0) A.Cdbo[pi] += ce1;
+  }
+  e = wave_sum(e);
+  f0 = wave_sum(f0);
+  f1 = wave_sum(f1);
+  f2 = wave_sum(f2);
+  if (lane == 0) {
+    A.part[j] = e;
+    A.fcd[4 * (size_t)j] += f0;
+    A.fcd[4 * (size_t)j + 1] += f1;
+    A.fcd[4 * (size_t)j + 2] += f2;
+  }
+}
+
+// The acceptor i bonded to local H atoms j: the di terms of j's triples.
+__global__ void __launch_bounds__(BT) k_hb_i(int n, int N, HbArgs A) {
+  const int i = blockIdx.x * (BT / SWR_WAVE) + wave_in_block();
+  if (i >= N) return;
+  const int lane = lane_id();
+  const DevParams *__restrict__ P = A.P;
+  const swr_atom_pack_t ai = A.atoms[i];
+  const int ti = ai.type;
+  if (ti < 0 || P->p_hbond[ti] != 2) return;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  for (int pb = A.bidx[i]; pb < A.bend[i]; ++pb) {
+    const swr_bond_t b = A.bonds[pb];
+    const int j = b.nbr;
+    if (j >= n) continue;
+    const int tj = A.atoms[j].type;
+    if (tj < 0 || P->p_hbond[tj] != 1) continue;
+    const int pi = b.sym_index;                // j's slot toward i
+    const double BOij = A.BO[pi];
+    if (!(BOij >= HB_THRESHOLD)) continue;
+    const int H = A.Hindex[j];
+    if (H < 0) continue;
+    const swr_bond_t bij = A.bonds[pi];
+    for (int pk = A.hidx[H] + lane; pk < A.hend[H]; pk += SWR_WAVE) {
+      const swr_hbond_t hb = A.hbonds[pk];
+      const swr_atom_pack_t ak = A.atoms[hb.nbr];
+      if (ak.type < 0 || ak.orig_id == ai.orig_id) continue;
+      const DevHb &hbp = hb_params(P, ti, tj, ak.type);
+      if (hbp.r0_hb <= 0.0) continue;
+      const swr_far_neighbor_t *fe = hb.ptr;
+      const double r_jk = fe->d;
+      const double dvec_jk[3] = {hb.scl * fe->dvec[0], hb.scl * fe->dvec[1], hb.scl * fe->dvec[2]};
+      HbTriple t;
+      hb_triple(hbp, BOij, bij.dvec, bij.d, dvec_jk, r_jk, t);
+      f0 += t.ce2 * t.di[0];
+      f1 += t.ce2 * t.di[1];
+      f2 += t.ce2 * t.di[2];
+    }
+  }
+  f0 = wave_sum(f0);
+  f1 = wave_sum(f1);
+  f2 = wave_sum(f2);
+  if (lane == 0) {
+    A.fcd[4 * (size_t)i] += f0;
+    A.fcd[4 * (size_t)i + 1] += f1;
+    A.fcd[4 * (size_t)i + 2] += f2;
+  }
+}
+
+// The partner acceptor k: the dk and distance terms of every triple whose
+// H atom has k in its hydrogen-bond list (found through k's far list).
+__global__ void __launch_bounds__(BT) k_hb_k(int n, int N, HbArgs A) {
+  const int k = blockIdx.x * (BT / SWR_WAVE) + wave_in_block();
+  if (k >= N) return;
+  const int lane = lane_id();
+  const DevParams *__restrict__ P = A.P;
+  const swr_atom_pack_t ak = A.atoms[k];
+  const int tk = ak.type;
+  if (tk < 0 || P->p_hbond[tk] != 2 || !(A.hbond_cut > 0)) return;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  for (int p = A.far_index[k] + lane; p < A.far_end[k]; p += SWR_WAVE) {
+    const swr_far_neighbor_t fe = A.far[p];
+    const int j = fe.nbr;
+    if (j >= n || fe.d > A.hbond_cut) continue;
+    const int tj = A.atoms[j].type;
+    if (tj < 0 || P->p_hbond[tj] != 1) continue;
+    const double r_jk = fe.d;
+    const double dvec_jk[3] = {-fe.dvec[0], -fe.dvec[1], -fe.dvec[2]};   // x_k - x_j
+    for (int pi = A.bidx[j]; pi < A.bend[j]; ++pi) {
+      const swr_bond_t bij = A.bonds[pi];
+      const swr_atom_pack_t ai = A.atoms[bij.nbr];
+      const double BOij = A.BO[pi];
+      if (ai.type < 0 || P->p_hbond[ai.type] != 2 || !(BOij >= HB_THRESHOLD) || ai.orig_id == ak.orig_id) continue;
+      const DevHb &hbp = hb_params(P, ai.type, tj, tk);
+      if (hbp.r0_hb <= 0.0) continue;
+      HbTriple t;
+      hb_triple(hbp, BOij, bij.dvec, bij.d, dvec_jk, r_jk, t);
+      f0 += t.ce2 * t.dk[0] + t.ce3 / r_jk * dvec_jk[0];
+      f1 += t.ce2 * t.dk[1] + t.ce3 / r_jk * dvec_jk[1];
+      f2 += t.ce2 * t.dk[2] + t.ce3 / r_jk * dvec_jk[2];
+    }
+  }
+  f0 = wave_sum(f0);
+  f1 = wave_sum(f1);
+  f2 = wave_sum(f2);
+  if (lane == 0) {
+    A.fcd[4 * (size_t)k] += f0;
+    A.fcd[4 * (size_t)k + 1] += f1;
+    A.fcd[4 * (size_t)k + 2] += f2;
+  }
+}
+
 }  // namespace
 
 int swr_atom_energy_run(swr_system *S) {
@@ -428,6 +647,46 @@ int swr_dbond_forces_run(swr_system *S) {
   hipLaunchKernelGGL(k_dbond_own, dim3(div_up(N, BT)), dim3(BT), 0, S->stream, N, A);
   hipLaunchKernelGGL(k_dbond_nbr, dim3(div_up(N, BT)), dim3(BT), 0, S->stream, N, A);
   swr_kt_end(S, KT_DBOND);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_hydrogen_bonds_run(swr_system *S) {
+  const int n = S->n, N = S->N;
+  if (!S->bonds || !S->BO || !S->fcd || N <= 0 || !S->far) {
+    swr_set_error_msg("Hydrogen_Bonds needs the bonds, bond orders and lists of the step");
+    return SWR_ERR_STATE;
+  }
+  int rc;
+  if ((rc = swr_reserve(&S->hb_part, S->cap_hb_part, (size_t)std::max(n, 1))) ||
+      (rc = swr_reserve(&S->hb_result, S->cap_hb_result, 1)))
+    return rc;
+  HbArgs A;
+  A.P = S->dp;
+  A.atoms = S->packed;
+  A.bidx = S->bond_index;
+  A.bend = S->bond_end;
+  A.bonds = S->bonds;
+  A.BO = S->BO;
+  A.Hindex = S->Hindex;
+  A.hidx = S->hbond_index;
+  A.hend = S->hbond_end;
+  A.hbonds = S->hbonds;
+  A.far_index = S->far_index;
+  A.far_end = S->far_end;
+  A.far = S->far;
+  A.Cdbo = S->Cdbo;
+  A.fcd = S->fcd;
+  A.part = S->hb_part;
+  A.hbond_cut = S->ctl.hbond_cut;
+  constexpr int WPB = BT / SWR_WAVE;
+  swr_kt_begin(S, KT_HBONDS);
+  if (n > 0) hipLaunchKernelGGL(k_hb_own, dim3(div_up(n, WPB)), dim3(BT), 0, S->stream, n, A);
+  hipLaunchKernelGGL(k_hb_i, dim3(div_up(N, WPB)), dim3(BT), 0, S->stream, n, N, A);
+  hipLaunchKernelGGL(k_hb_k, dim3(div_up(N, WPB)), dim3(BT), 0, S->stream, n, N, A);
+  hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(1024), 0, S->stream, n, (const double *)S->hb_part, (size_t)0,
+                     S->hb_result);
+  swr_kt_end(S, KT_HBONDS);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }

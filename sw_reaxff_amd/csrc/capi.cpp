@@ -113,6 +113,14 @@ void fill_params(swr_system *S, const std::vector<int> &map) {
     h.gp37 = (int)g(37);
     h.enobonds = S->ctl.enobonds;
   }
+  for (int a = 0; a < ff.ne() && a < SWR_MAX_ELEM; a++)
+    for (int b = 0; b < ff.ne() && b < SWR_MAX_ELEM; b++)
+      for (int c = 0; c < ff.ne() && c < SWR_MAX_ELEM; c++) {
+        DevHb &d = h.hb[(a * SWR_MAX_ELEM + b) * SWR_MAX_ELEM + c];
+        if (ff.hb.empty()) { d.r0_hb = -1.0; continue; }
+        const FfHbond &f = ff.hbp(a, b, c);
+        d.r0_hb = f.r0_hb; d.p_hb1 = f.p_hb1; d.p_hb2 = f.p_hb2; d.p_hb3 = f.p_hb3;
+      }
   // non-bonded pair factors and taper
   h.vdw_type = ff.vdw_type;
   h.p_vdW1 = ff.gp.size() > 28 ? ff.gp[28] : 0.0;
@@ -325,7 +333,8 @@ int swr_destroy(swr_system *S) {
                   S->voff, S->vcount, S->vlist, S->far, S->far_index, S->far_end, S->num_bonds, S->num_hbonds,
                   S->Hindex, S->bond_index, S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, S->Cdbo, S->Cdbopi,
                   S->Cdbopi2, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,
-                  S->fcd, S->nb_partials, S->nb_result, S->nb_eatom, S->be_part, S->be_ce, S->be_bufc, S->be_result,
+                  S->fcd, S->nb_partials, S->nb_result, S->nb_eatom, S->be_part, S->be_ce, S->be_bufc, S->be_result, S->hb_part,
+                  S->hb_result,
                   S->bond_mark2, S->counts, S->ws, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32,
                   S->Hdia_inv, S->b_s, S->st, S->s_hist, S->t_hist, S->cgv, S->cgs, S->partials, S->counters,
                   S->status, S->scan_tmp, S->tmp_i, S->mixw, S->cand, S->cand_count, S->row_sys, S->tile_beg,
@@ -640,6 +649,13 @@ int swr_add_dbond_forces(swr_system *S) {
   return swr_dbond_forces_run(S);
 }
 
+int swr_hydrogen_bonds(swr_system *S) {
+  if (!S) return SWR_ERR_ARG;
+  int rc;
+  if ((rc = swr_quiesce(S))) return rc;
+  return swr_hydrogen_bonds_run(S);
+}
+
 int swr_sync(swr_system *S) {
   if (!S) return SWR_ERR_ARG;
   int rc = refresh_status(S);
@@ -707,6 +723,8 @@ bool lookup(swr_system *S, const std::string &k, View &v) {
   if (k == "nonbonded_eatom") return set(S->nb_eatom, S->nb_eatom ? 8 * n : 0);
   if (k == "bonded_energy") return set(S->be_result, S->be_result ? 40 : 0);
   if (k == "bonded_eatom") return set(S->be_part, S->be_part ? 40 * (int64_t)S->be_stride : 0);
+  if (k == "hbond_energy") return set(S->hb_result, S->hb_result ? 8 : 0);
+  if (k == "hbond_eatom") return set(S->hb_part, S->hb_part ? 8 * n : 0);
   static const char *wsn[WS_COUNT] = {"Deltap", "Deltap_boc", "Delta", "Delta_e", "Delta_val", "vlpex", "nlp",
                                       "nlp_temp", "Clp", "Delta_lp", "dDelta_lp", "Delta_lp_temp", "dDelta_lp_temp"};
   for (int w = 0; w < WS_COUNT; w++)

@@ -11,7 +11,8 @@
 //   * pair radii are arithmetic means, p_boc3/4/5 geometric means,
 //     gamma = (g_i g_j)^-1.5, then positive off-diagonal values override
 //     D, r_vdW (x2), alpha, r_s, r_p, r_pp                              (:402-545).
-// Angle, torsion and hbond blocks follow but are not needed here.
+// Angle and torsion blocks are skipped; the hydrogen-bond block is read
+// (:654-686).
 #include "swr_ffield.h"
 
 #include <cctype>
@@ -156,6 +157,24 @@ std::string swr_read_ffield(const char *path, swr_ffield_t *ff) {
         if ((v = num(t, 7)) > 0.0) p.r_pp = v;
       }
     }
+  }
+  // valence-angle and torsion blocks: skipped by their counts; then the
+  // hydrogen-bond block (:654-686): r0_hb = -1 unless listed
+  ff->hb.assign((size_t)ne * ne * ne, FfHbond());
+  for (int blk = 0; blk < 2; blk++) {
+    if (!L.next(t) || t.empty()) return "";
+    const int l = std::atoi(t[0].c_str());
+    for (int k = 0; k < l; k++)
+      if (!L.next(t)) return "";
+  }
+  if (!L.next(t) || t.empty()) return "";
+  const int nh = std::atoi(t[0].c_str());
+  for (int k = 0; k < nh; k++) {
+    if (!L.next(t) || t.size() < 7) return "truncated hydrogen-bond block";
+    const int a = std::atoi(t[0].c_str()) - 1, b = std::atoi(t[1].c_str()) - 1, c = std::atoi(t[2].c_str()) - 1;
+    if (a < 0 || b < 0 || c < 0 || a >= ne || b >= ne || c >= ne) continue;
+    FfHbond &h = ff->hb[((size_t)a * ne + b) * ne + c];
+    h.r0_hb = num(t, 3); h.p_hb1 = num(t, 4); h.p_hb2 = num(t, 5); h.p_hb3 = num(t, 6);
   }
   return "";
 }

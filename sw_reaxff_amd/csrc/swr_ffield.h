@@ -24,8 +24,13 @@ struct FfPair {
   double v13cor = 0, ovc = 0;
 };
 
+struct FfHbond {
+  double r0_hb = -1.0, p_hb1 = 0, p_hb2 = 0, p_hb3 = 0;   // r0_hb <= 0: no hydrogen bond
+};
+
 struct swr_ffield_t {
   std::vector<double> gp;           // general parameters l[0..]
+  std::vector<FfHbond> hb;          // [ne*ne*ne] (acceptor i, hydrogen j, acceptor k)
   std::vector<FfAtom> atom;
   std::vector<FfPair> pair;         // [ne*ne]
   double bo_cut = 0, nonb_low = 0, nonb_cut = 0;
@@ -33,6 +38,7 @@ struct swr_ffield_t {
   int ne() const { return (int)atom.size(); }
   const FfPair &tb(int i, int j) const { return pair[(size_t)i * atom.size() + j]; }
   FfPair &tb(int i, int j) { return pair[(size_t)i * atom.size() + j]; }
+  const FfHbond &hbp(int i, int j, int k) const { return hb[((size_t)i * atom.size() + j) * atom.size() + k]; }
 };
 
 // Returns an empty string on success, else the reason.

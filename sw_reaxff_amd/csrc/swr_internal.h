@@ -59,6 +59,11 @@ struct DevNb {
   double ecore, acore, rcoreinv, a_div_rcore;             // inner wall
 };
 
+// Hydrogen-bond parameters (acceptor i, hydrogen j, acceptor k); r0_hb <= 0: none.
+struct DevHb {
+  double r0_hb, p_hb1, p_hb2, p_hb3;
+};
+
 // Bond and atom energies (Merge_Bonds_Atom_Energy_C_New), element-pair indexed.
 struct DevBe {
   double De_s, De_p, De_pp, p_be1, p_be2, p_ovun1;
@@ -100,6 +105,7 @@ struct DevParams {
   DevBe be[SWR_MAX_ELEM * SWR_MAX_ELEM];
   double p_lp3, p_ovun3, p_ovun4, p_ovun6, p_ovun7, p_ovun8, gp3, gp4, gp7, gp10;
   int gp37, enobonds;
+  DevHb hb[SWR_MAX_ELEM * SWR_MAX_ELEM * SWR_MAX_ELEM];
 };
 
 // Small block of device-written results the host reads back asynchronously.
@@ -141,7 +147,7 @@ struct CgScalars {
 // Kernel timer slots.
 enum { KT_VERLET = 0, KT_FAR, KT_BUILD_H, KT_CG_ITER, KT_CG_SETUP, KT_BO, KT_VALIDATE, KT_SPMV, KT_RESET,
        KT_QEQ_Q, KT_BINS, KT_MIXED_INNER, KT_MIXED_OUTER, KT_BONDS, KT_CG_SCALARS, KT_MARK, KT_CG_UPDATE, KT_NONBONDED, KT_ATOM_ENERGY,
-       KT_DBOND, KT_COUNT };
+       KT_DBOND, KT_HBONDS, KT_COUNT };
 
 struct swr_system {
   // ---- host-side configuration
@@ -214,6 +220,9 @@ struct swr_system {
   double *be_bufc = nullptr;       // [cap] per atom buf_c of the dBO chain rule
   double *be_result = nullptr;     // e_bond, e_lp, e_ov, e_un, e_pol of the last Atom_Energy
   size_t cap_be_part = 0, cap_be_ce = 0, cap_be_bufc = 0, cap_be_result = 0;
+  double *hb_part = nullptr;        // [n] per local H atom hydrogen-bond energy
+  double *hb_result = nullptr;     // e_hb of the last Hydrogen_Bonds
+  size_t cap_hb_part = 0, cap_hb_result = 0;
   size_t be_stride = 0;            // row stride of be_part/be_ce (the local atom count of the last pass)
   double *bo_dboc = nullptr;       // [N][2]
   double *dDeltap_self = nullptr;  // [N][3]
@@ -327,6 +336,7 @@ int swr_after_sync(swr_system *S);
 int swr_nonbonded_run(swr_system *S);
 int swr_atom_energy_run(swr_system *S);
 int swr_dbond_forces_run(swr_system *S);
+int swr_hydrogen_bonds_run(swr_system *S);
 int swr_quiesce(swr_system *S);
 constexpr int MARK_INNER = 8;         // bond_mark relaxation sweeps per launch
 constexpr int MARK_MAX_LAUNCH = 128;  // launches of one relaxation at most

@@ -74,6 +74,19 @@ class FField:
             vals = [float(v) for v in t[2:8]]
             self.offdiag[(i, j)] = self.offdiag[(j, i)] = dict(zip(["D", "r_vdw", "alpha", "r_s", "r_p", "r_pp"], vals))
             k += 1
+        # valence-angle and torsion blocks (a count line, then that many
+        # lines each), then the hydrogen bonds: acceptor, H, acceptor (1-based)
+        # and r0_hb, p_hb1, p_hb2, p_hb3
+        for _ in range(2):
+            k += 1 + int(lines[k].split()[0])
+        nh = int(lines[k].split()[0])
+        k += 1
+        self.hbonds = {}
+        for _ in range(nh):
+            t = lines[k].split()
+            self.hbonds[(int(t[0]) - 1, int(t[1]) - 1, int(t[2]) - 1)] = dict(
+                zip(["r0_hb", "p_hb1", "p_hb2", "p_hb3"], [float(v) for v in t[3:7]]))
+            k += 1
 
     # --- derived quantities (standard ReaxFF definitions)
     @property
@@ -85,6 +98,10 @@ class FField:
 
     def atom(self, el: str) -> dict:
         return self.atoms[el]
+
+    def hbond(self, i: str, j: str, k: str):
+        """Hydrogen-bond parameters of acceptor i, hydrogen j, acceptor k (None if absent)."""
+        return self.hbonds.get((self.idx(i), self.idx(j), self.idx(k)))
 
     def eta(self, el: str) -> float:
         return 2.0 * self.atoms[el]["eta_file"]

@@ -1,12 +1,13 @@
-"""Bond and atom energies (Merge_Bonds_Atom_Energy_C_New) and the bond-order
-chain rule of the forces (Add_All_dBond_to_Forces_C): oracle pins on the CPU
-and HIP-vs-oracle parity on the GPU.
+"""Bond and atom energies (Merge_Bonds_Atom_Energy_C_New), hydrogen bonds
+(Hydrogen_Bonds_C) and the bond-order chain rule of the forces
+(Add_All_dBond_to_Forces_C): oracle pins on the CPU and HIP-vs-oracle parity
+on the GPU.
 
 Pins (independent of the oracle's code):
-  * finite differences of e_bond + e_lp + e_ov + e_un (charges do not enter
-    them) reproduce the gradient the chain rule leaves in fCdDelta[:, :3],
-    on water/methanol clusters and compressed clusters whose bonds reach the
-    over-coordination branch;
+  * finite differences of e_bond + e_lp + e_ov + e_un + e_hb (charges do not
+    enter them) reproduce the gradient the hydrogen-bond terms and the chain
+    rule leave in fCdDelta[:, :3], on water/methanol clusters, compressed
+    clusters whose bonds reach the over-coordination branch, and water;
   * the gradient of an isolated cluster sums to zero and has no torque;
   * e_bond equals the published bond-energy expression summed once per bond
     over the oracle's corrected bond orders, with the bond parameters read
@@ -14,7 +15,11 @@ Pins (independent of the oracle's code):
     cluster whose bond orders all stay below 1 (the terminal triple-bond
     term, which this force field's gp37 = 2 applies to every bond of order
     >= 1, is then absent; the compressed clusters exercise it);
-  * e_pol equals sum 23.02 (chi q + eta/2 q^2) with the reader's chi, eta.
+  * e_pol equals sum 23.02 (chi q + eta/2 q^2) with the reader's chi, eta;
+  * e_hb of a water dimer equals the published hydrogen-bond expression
+    summed over every H, every O it is bonded to (BO >= 0.01, the oracle's
+    bond orders) and every other O within the cutoff, with the reader's
+    parameters and the angle and distance from the positions.
 The lone-pair, over- and under-coordination expressions are pinned only
 through those finite differences and the GPU-vs-oracle parity: no closed
 form or reference output covers them.  The C-C lone-pair correction is
@@ -51,7 +56,8 @@ def _oracle(oracle_lib, ffield, s, x=None, q=None):
 
 
 def _energy(oracle_lib, ffield, s, x):
-    return float(_oracle(oracle_lib, ffield, s, x).atom_energy()[:4].sum())
+    o = _oracle(oracle_lib, ffield, s, x)
+    return float(o.atom_energy()[:4].sum()) + o.hydrogen_bonds()
 
 
 def compressed(s, f):
@@ -77,6 +83,7 @@ def test_gradient_matches_finite_differences(oracle_lib, ffield, case):
         s = water_cluster(8, seed=2)
     o = _oracle(oracle_lib, ffield, s)
     e = o.atom_energy()
+    ehb = o.hydrogen_bonds()
     o.dbond_forces()
     g = o.get("fCdDelta").reshape(-1, 4)[: s.n, :3]
     gmax = np.abs(g).max()
@@ -90,7 +97,7 @@ def test_gradient_matches_finite_differences(oracle_lib, ffield, case):
             xm[a, k] -= h
             fd = (_energy(oracle_lib, ffield, s, xp) - _energy(oracle_lib, ffield, s, xm)) / (2 * h)
             worst = max(worst, abs(fd - g[a, k]) / gmax)
-    print(f"[bonded] {case} n={s.n} energies {e} max|g| {gmax:.3e} worst FD error {worst:.2e} of max|g|")
+    print(f"[bonded] {case} n={s.n} energies {e} e_hb {ehb:.4e} max|g| {gmax:.3e} worst FD error {worst:.2e}")
     assert worst <= 1e-6
 
 
@@ -98,6 +105,7 @@ def test_gradient_has_no_net_force_or_torque(oracle_lib, ffield):
     s = systems.cluster(60, seed=11)
     o = _oracle(oracle_lib, ffield, s)
     o.atom_energy()
+    assert o.hydrogen_bonds() != 0.0
     o.dbond_forces()
     g = o.get("fCdDelta").reshape(-1, 4)[: s.n, :3]
     scale = np.abs(g).max()
@@ -140,6 +148,42 @@ def test_polarization_energy_closed_form(oracle_lib, ffield):
     assert e[4] == pytest.approx(want, rel=1e-12)
 
 
+def test_hydrogen_bond_closed_form(oracle_lib, ffield):
+    """Two water molecules 2.9 A apart (O...O), one H pointing at the other O."""
+    ff = FField(ffield)
+    ang = np.radians(104.52)
+    w = np.array([[0.0, 0.0, 0.0], [0.9572, 0.0, 0.0], [0.9572 * np.cos(ang), 0.9572 * np.sin(ang), 0.0]])
+    rot = np.array([[np.cos(0.7), 0.0, np.sin(0.7)], [0.0, 1.0, 0.0], [-np.sin(0.7), 0.0, np.cos(0.7)]])
+    x = np.vstack([w, w @ rot.T + np.array([2.9, 0.2, 0.1])])
+    s = systems.System(x=x, v=np.zeros_like(x), type=np.array([1, 2, 2, 1, 2, 2], np.int32),
+                       tag=np.arange(1, 7, dtype=np.int32), elements=["O", "H"], lo=x.min(0) - 20, hi=x.max(0) + 20,
+                       periodic=False)
+    o = _oracle(oracle_lib, ffield, s)
+    o.atom_energy()
+    got = o.hydrogen_bonds()
+    bi, be, nbr, BO = o.get("bond_index"), o.get("bond_end"), o.get("bonds")["nbr"], o.get("BO")
+    p = ff.hbond("O", "H", "O")
+    want, terms = 0.0, 0
+    for j in (1, 2, 4, 5):                                  # the H atoms
+        for q in range(bi[j], be[j]):
+            oi = int(nbr[q])
+            if oi not in (0, 3) or BO[q] < 0.01:            # bonds to an O with BO >= 0.01
+                continue
+            for ok in (0, 3):                               # every other O within the cutoff
+                b = x[ok] - x[j]
+                r = np.linalg.norm(b)
+                if ok == oi or r > 7.5:
+                    continue
+                a = x[oi] - x[j]
+                theta = np.arccos(np.dot(a, b) / (np.linalg.norm(a) * r))
+                want += (p["p_hb1"] * (1.0 - np.exp(-p["p_hb2"] * BO[q]))
+                         * np.exp(-p["p_hb3"] * (p["r0_hb"] / r + r / p["r0_hb"] - 2.0)) * np.sin(theta / 2.0) ** 4)
+                terms += 1
+    assert terms >= 4
+    assert got == pytest.approx(want, rel=1e-12)
+    assert got < -1.0        # a real hydrogen bond
+
+
 # ---------------------------------------------------------------- GPU parity
 def _bond_key(bi, be, nbr, n_rows):
     keys = []
@@ -154,12 +198,14 @@ def _gpu_vs_oracle(gpu_lib, oracle_lib, ffield, lmp, elements, stype, stag):
     from sw_reaxff_amd import _lib
     h = lmp.system
     _lib.check(h.L.swr_atom_energy(h.h), "swr_atom_energy")
+    _lib.check(h.L.swr_hydrogen_bonds(h.h), "swr_hydrogen_bonds")
     h.sync()
     cd_gpu = h.get("fCdDelta").reshape(-1, 4)[:, 3].copy()
     cdbo_gpu = [h.get(k).copy() for k in ("Cdbo", "Cdbopi", "Cdbopi2")]
+    ghb_gpu = h.get("fCdDelta").reshape(-1, 4)[:, :3].copy()
     _lib.check(h.L.swr_add_dbond_forces(h.h), "swr_add_dbond_forces")
     h.sync()
-    e_gpu = h.get("bonded_energy")
+    e_gpu = np.append(h.get("bonded_energy"), h.get("hbond_energy"))
     g_gpu = h.get("fCdDelta").reshape(-1, 4)[:, :3]
     n, N = h.scalar("n"), h.scalar("N")
     xa = h.get("x").reshape(-1, 3)[:N]
@@ -169,13 +215,15 @@ def _gpu_vs_oracle(gpu_lib, oracle_lib, ffield, lmp, elements, stype, stag):
                 q=h.get("q")[:N])
     o.build_verlet(12.0)
     run_stages(o)
-    e_o = o.atom_energy()
+    e_o = np.append(o.atom_energy(), o.hydrogen_bonds())
     cd_o = o.get("fCdDelta").reshape(-1, 4)[:, 3].copy()
     cdbo_o = [o.get(k).copy() for k in ("Cdbo", "Cdbopi", "Cdbopi2")]
+    ghb_o = o.get("fCdDelta").reshape(-1, 4)[:, :3].copy()
     o.dbond_forces()
     g_o = o.get("fCdDelta").reshape(-1, 4)[:, :3]
     print(f"[bonded] n={n} N={N} gpu {e_gpu} oracle {e_o}", flush=True)
-    for k in range(5):
+    assert np.max(np.abs(ghb_gpu[:N] - ghb_o[:N])) <= 1e-10 * max(np.abs(ghb_o[:N]).max(), 1e-300)
+    for k in range(6):
         assert e_gpu[k] == pytest.approx(e_o[k], rel=1e-10, abs=1e-9), k
     # per bond, matched by (atom, partner)
     kg = _bond_key(h.get("bond_index"), h.get("bond_end"), h.get("bonds")["nbr"], N)
