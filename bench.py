@@ -569,7 +569,7 @@ def main():
               "avg_us": round(be_avg * 1e3, 2), "atom_energy_us": round(ae_ms * 1e3 / max(1, ae_cnt), 2),
               "dbond_us": round(db_ms * 1e3 / max(1, db_cnt), 2), "launches": ae_cnt, "bytes_per_pass": be_b,
               "achieved_gbs": round(be_b / (be_avg * 1e-3) / 1e9, 1),
-              "hbonds": {"kernels": "k_hb_own, k_hb_i, k_hb_k, k_sum_rows",
+              "hbonds": {"kernels": "k_hb, k_hb_fold, k_sum_rows",
                          "avg_us": round(hb_ms * 1e3 / max(1, hb_cnt), 2),
                          "e_hb_kcal_mol": float(sysh.get("hbond_energy")[0])},
               "energies_kcal_mol": dict(zip(("e_bond", "e_lp", "e_ov", "e_un", "e_pol"),

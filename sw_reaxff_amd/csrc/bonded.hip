@@ -364,16 +364,16 @@ __global__ void __launch_bounds__(BT) k_dbond_nbr(int N, DbArgs A) {
 // (p_hbond 2) with BO >= 0.01, and a hydrogen-bond partner k of j with
 // orig_id(k) != orig_id(i) and r0_hb(i, j, k) > 0.  Its energy and CEhb1
 // (into Cdbo of j's slot) and the angle/distance terms into fCdDelta of i,
-// j and k.  Three kernels, each writing only its own atom's entries and
-// each recomputing the triples that touch its atom: k_hb_own (the H atom
-// j: energy, Cdbo, its own terms), k_hb_i (the bonded acceptor i, through
-// its bonds to local H atoms and their hydrogen-bond lists) and k_hb_k (the
-// partner acceptor k, through its far list: k is in j's hydrogen-bond list
-// exactly when j is a local H atom and their far entry has d <= hbond_cut,
-// the rule the list pass applies, with the same d and the opposite dvec).
-// A wave per atom, lanes over hydrogen-bond or far entries; the lane sums
-// are combined by a fixed butterfly.
+// j and k.  k_hb computes each triple once, a wave per H atom with its lanes
+// over the atom's hydrogen-bond entries: the energy, CEhb1 and j's own
+// terms are summed by a fixed butterfly; the terms of i (summed over the
+// entries, one set per bond) and of k (summed over the bonds, one set per
+// entry) go into 64-bit fixed-point accumulators (2^-40 resolution) by
+// integer atomics, whose sums do not depend on the order they arrive in;
+// k_hb_fold adds them to fCdDelta and clears them.  The results are the same
+// on every run.
 constexpr double HB_THRESHOLD = 1e-2;   // reaxc_defs_sunway.h:91
+constexpr double HB_FIX = 1099511627776.0;   // 2^40
 
 struct HbArgs {
   const DevParams *P;
@@ -383,10 +383,8 @@ struct HbArgs {
   const double *BO;
   const int *Hindex, *hidx, *hend;
   const swr_hbond_t *hbonds;
-  const int *far_index, *far_end;
-  const swr_far_neighbor_t *far;
   double *Cdbo, *fcd, *part;
-  double hbond_cut;
+  unsigned long long *acc;   // [N][3] fixed-point terms of i and k
 };
 
 // One triple (Calculate_Theta, Calculate_dCos_Theta :336-372, energy and
@@ -426,12 +424,11 @@ __device__ __forceinline__ void hb_triple(const DevHb &hbp, double BOij, const d
   t.ce3 = -hbp.p_hb3 * (-hbp.r0_hb / (r_jk * r_jk) + 1.0 / hbp.r0_hb) * t.e;
 }
 
-__device__ __forceinline__ const DevHb &hb_params(const DevParams *P, int ti, int tj, int tk) {
-  return P->hb[(ti * SWR_MAX_ELEM + tj) * SWR_MAX_ELEM + tk];
+__device__ __forceinline__ void fix_add(unsigned long long *a, double v) {
+  atomicAdd(a, (unsigned long long)(long long)llrint(v * HB_FIX));
 }
 
-// The H atom j: energy, CEhb1 of its acceptor bonds, its own terms.
-__global__ void __launch_bounds__(BT) k_hb_own(int n, HbArgs A) {
+__global__ void __launch_bounds__(BT) k_hb(int n, HbArgs A) {
   const int j = blockIdx.x * (BT / SWR_WAVE) + wave_in_block();
   if (j >= n) return;
   const int lane = lane_id();
@@ -452,13 +449,13 @@ __global__ void __launch_bounds__(BT) k_hb_own(int n, HbArgs A) {
     const double BOij = A.BO[pi];
     if (ti < 0 || P->p_hbond[ti] != 2 || !(BOij >= HB_THRESHOLD)) continue;
     const int oi = A.atoms[i].orig_id;
-    double ce1 = 0.0;
+    double ce1 = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
     for (int pk = h0 + lane; pk < h1; pk += SWR_WAVE) {
       const swr_hbond_t hb = A.hbonds[pk];
       const int k = hb.nbr;
       const swr_atom_pack_t ak = A.atoms[k];
       if (ak.type < 0 || ak.orig_id == oi) continue;
-      const DevHb &hbp = hb_params(P, ti, tj, ak.type);
+      const DevHb &hbp = P->hb[(ti * SWR_MAX_ELEM + tj) * SWR_MAX_ELEM + ak.type];
       if (hbp.r0_hb <= 0.0) continue;
       const swr_far_neighbor_t *fe = hb.ptr;
       const double r_jk = fe->d;
@@ -470,9 +467,25 @@ __global__ void __launch_bounds__(BT) k_hb_own(int n, HbArgs A) {
       f0 += t.ce2 * t.dj[0] - t.ce3 / r_jk * dvec_jk[0];
       f1 += t.ce2 * t.dj[1] - t.ce3 / r_jk * dvec_jk[1];
       f2 += t.ce2 * t.dj[2] - t.ce3 / r_jk * dvec_jk[2];
+      g0 += t.ce2 * t.di[0];
+      g1 += t.ce2 * t.di[1];
+      g2 += t.ce2 * t.di[2];
+      unsigned long long *ak3 = A.acc + 3 * (size_t)k;
+      fix_add(ak3, t.ce2 * t.dk[0] + t.ce3 / r_jk * dvec_jk[0]);
+      fix_add(ak3 + 1, t.ce2 * t.dk[1] + t.ce3 / r_jk * dvec_jk[1]);
+      fix_add(ak3 + 2, t.ce2 * t.dk[2] + t.ce3 / r_jk * dvec_jk[2]);
     }
     ce1 = wave_sum(ce1);
-    if (lane == 0) A.Cdbo[pi] += ce1;
+    g0 = wave_sum(g0);
+    g1 = wave_sum(g1);
+    g2 = wave_sum(g2);
+    if (lane == 0) {
+      A.Cdbo[pi] += ce1;
+      unsigned long long *ai3 = A.acc + 3 * (size_t)i;
+      fix_add(ai3, g0);
+      fix_add(ai3 + 1, g1);
+      fix_add(ai3 + 2, g2);
+    }
   }
   e = wave_sum(e);
   f0 = wave_sum(f0);
@@ -486,94 +499,18 @@ __global__ void __launch_bounds__(BT) k_hb_own(int n, HbArgs A) {
   }
 }
 
-// The acceptor i bonded to local H atoms j: the di terms of j's triples.
-__global__ void __launch_bounds__(BT) k_hb_i(int n, int N, HbArgs A) {
-  const int i = blockIdx.x * (BT / SWR_WAVE) + wave_in_block();
-  if (i >= N) return;
-  const int lane = lane_id();
-  const DevParams *__restrict__ P = A.P;
-  const swr_atom_pack_t ai = A.atoms[i];
-  const int ti = ai.type;
-  if (ti < 0 || P->p_hbond[ti] != 2) return;
-  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  for (int pb = A.bidx[i]; pb < A.bend[i]; ++pb) {
-    const swr_bond_t b = A.bonds[pb];
-    const int j = b.nbr;
-    if (j >= n) continue;
-    const int tj = A.atoms[j].type;
-    if (tj < 0 || P->p_hbond[tj] != 1) continue;
-    const int pi = b.sym_index;                // j's slot toward i
-    const double BOij = A.BO[pi];
-    if (!(BOij >= HB_THRESHOLD)) continue;
-    const int H = A.Hindex[j];
-    if (H < 0) continue;
-    const swr_bond_t bij = A.bonds[pi];
-    for (int pk = A.hidx[H] + lane; pk < A.hend[H]; pk += SWR_WAVE) {
-      const swr_hbond_t hb = A.hbonds[pk];
-      const swr_atom_pack_t ak = A.atoms[hb.nbr];
-      if (ak.type < 0 || ak.orig_id == ai.orig_id) continue;
-      const DevHb &hbp = hb_params(P, ti, tj, ak.type);
-      if (hbp.r0_hb <= 0.0) continue;
-      const swr_far_neighbor_t *fe = hb.ptr;
-      const double r_jk = fe->d;
-      const double dvec_jk[3] = {hb.scl * fe->dvec[0], hb.scl * fe->dvec[1], hb.scl * fe->dvec[2]};
-      HbTriple t;
-      hb_triple(hbp, BOij, bij.dvec, bij.d, dvec_jk, r_jk, t);
-      f0 += t.ce2 * t.di[0];
-      f1 += t.ce2 * t.di[1];
-      f2 += t.ce2 * t.di[2];
+// fCdDelta += the fixed-point terms; the accumulators are left at zero.
+__global__ void __launch_bounds__(BT) k_hb_fold(int N, unsigned long long *__restrict__ acc,
+                                                double *__restrict__ fcd) {
+  const int a = blockIdx.x * BT + threadIdx.x;
+  if (a >= N) return;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const long long v = (long long)acc[3 * (size_t)a + c];
+    if (v != 0) {
+      fcd[4 * (size_t)a + c] += (double)v * (1.0 / HB_FIX);
+      acc[3 * (size_t)a + c] = 0ull;
     }
-  }
-  f0 = wave_sum(f0);
-  f1 = wave_sum(f1);
-  f2 = wave_sum(f2);
-  if (lane == 0) {
-    A.fcd[4 * (size_t)i] += f0;
-    A.fcd[4 * (size_t)i + 1] += f1;
-    A.fcd[4 * (size_t)i + 2] += f2;
-  }
-}
-
-// The partner acceptor k: the dk and distance terms of every triple whose
-// H atom has k in its hydrogen-bond list (found through k's far list).
-__global__ void __launch_bounds__(BT) k_hb_k(int n, int N, HbArgs A) {
-  const int k = blockIdx.x * (BT / SWR_WAVE) + wave_in_block();
-  if (k >= N) return;
-  const int lane = lane_id();
-  const DevParams *__restrict__ P = A.P;
-  const swr_atom_pack_t ak = A.atoms[k];
-  const int tk = ak.type;
-  if (tk < 0 || P->p_hbond[tk] != 2 || !(A.hbond_cut > 0)) return;
-  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  for (int p = A.far_index[k] + lane; p < A.far_end[k]; p += SWR_WAVE) {
-    const swr_far_neighbor_t fe = A.far[p];
-    const int j = fe.nbr;
-    if (j >= n || fe.d > A.hbond_cut) continue;
-    const int tj = A.atoms[j].type;
-    if (tj < 0 || P->p_hbond[tj] != 1) continue;
-    const double r_jk = fe.d;
-    const double dvec_jk[3] = {-fe.dvec[0], -fe.dvec[1], -fe.dvec[2]};   // x_k - x_j
-    for (int pi = A.bidx[j]; pi < A.bend[j]; ++pi) {
-      const swr_bond_t bij = A.bonds[pi];
-      const swr_atom_pack_t ai = A.atoms[bij.nbr];
-      const double BOij = A.BO[pi];
-      if (ai.type < 0 || P->p_hbond[ai.type] != 2 || !(BOij >= HB_THRESHOLD) || ai.orig_id == ak.orig_id) continue;
-      const DevHb &hbp = hb_params(P, ai.type, tj, tk);
-      if (hbp.r0_hb <= 0.0) continue;
-      HbTriple t;
-      hb_triple(hbp, BOij, bij.dvec, bij.d, dvec_jk, r_jk, t);
-      f0 += t.ce2 * t.dk[0] + t.ce3 / r_jk * dvec_jk[0];
-      f1 += t.ce2 * t.dk[1] + t.ce3 / r_jk * dvec_jk[1];
-      f2 += t.ce2 * t.dk[2] + t.ce3 / r_jk * dvec_jk[2];
-    }
-  }
-  f0 = wave_sum(f0);
-  f1 = wave_sum(f1);
-  f2 = wave_sum(f2);
-  if (lane == 0) {
-    A.fcd[4 * (size_t)k] += f0;
-    A.fcd[4 * (size_t)k + 1] += f1;
-    A.fcd[4 * (size_t)k + 2] += f2;
   }
 }
 
@@ -653,14 +590,19 @@ int swr_dbond_forces_run(swr_system *S) {
 
 int swr_hydrogen_bonds_run(swr_system *S) {
   const int n = S->n, N = S->N;
-  if (!S->bonds || !S->BO || !S->fcd || N <= 0 || !S->far) {
-    swr_set_error_msg("Hydrogen_Bonds needs the bonds, bond orders and lists of the step");
+  if (!S->bonds || !S->BO || !S->fcd || N <= 0 || !S->hbonds) {
+    swr_set_error_msg("Hydrogen_Bonds needs the bonds, bond orders and hydrogen-bond lists of the step");
     return SWR_ERR_STATE;
   }
   int rc;
+  const size_t old_acc = S->cap_hb_acc;
   if ((rc = swr_reserve(&S->hb_part, S->cap_hb_part, (size_t)std::max(n, 1))) ||
-      (rc = swr_reserve(&S->hb_result, S->cap_hb_result, 1)))
+      (rc = swr_reserve(&S->hb_result, S->cap_hb_result, 1)) ||
+      (rc = swr_reserve(&S->hb_acc, S->cap_hb_acc, 3 * (size_t)N)))
     return rc;
+  // a new accumulator starts at zero; afterwards k_hb_fold leaves it so
+  if (S->cap_hb_acc != old_acc)
+    HIP_CHECK(hipMemsetAsync(S->hb_acc, 0, sizeof(unsigned long long) * S->cap_hb_acc, S->stream));
   HbArgs A;
   A.P = S->dp;
   A.atoms = S->packed;
@@ -672,18 +614,14 @@ int swr_hydrogen_bonds_run(swr_system *S) {
   A.hidx = S->hbond_index;
   A.hend = S->hbond_end;
   A.hbonds = S->hbonds;
-  A.far_index = S->far_index;
-  A.far_end = S->far_end;
-  A.far = S->far;
   A.Cdbo = S->Cdbo;
   A.fcd = S->fcd;
   A.part = S->hb_part;
-  A.hbond_cut = S->ctl.hbond_cut;
+  A.acc = S->hb_acc;
   constexpr int WPB = BT / SWR_WAVE;
   swr_kt_begin(S, KT_HBONDS);
-  if (n > 0) hipLaunchKernelGGL(k_hb_own, dim3(div_up(n, WPB)), dim3(BT), 0, S->stream, n, A);
-  hipLaunchKernelGGL(k_hb_i, dim3(div_up(N, WPB)), dim3(BT), 0, S->stream, n, N, A);
-  hipLaunchKernelGGL(k_hb_k, dim3(div_up(N, WPB)), dim3(BT), 0, S->stream, n, N, A);
+  if (n > 0) hipLaunchKernelGGL(k_hb, dim3(div_up(n, WPB)), dim3(BT), 0, S->stream, n, A);
+  hipLaunchKernelGGL(k_hb_fold, dim3(div_up(N, BT)), dim3(BT), 0, S->stream, N, S->hb_acc, S->fcd);
   hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(1024), 0, S->stream, n, (const double *)S->hb_part, (size_t)0,
                      S->hb_result);
   swr_kt_end(S, KT_HBONDS);

@@ -222,7 +222,8 @@ struct swr_system {
   size_t cap_be_part = 0, cap_be_ce = 0, cap_be_bufc = 0, cap_be_result = 0;
   double *hb_part = nullptr;        // [n] per local H atom hydrogen-bond energy
   double *hb_result = nullptr;     // e_hb of the last Hydrogen_Bonds
-  size_t cap_hb_part = 0, cap_hb_result = 0;
+  unsigned long long *hb_acc = nullptr;   // [N][3] fixed-point hydrogen-bond terms (zero between calls)
+  size_t cap_hb_part = 0, cap_hb_result = 0, cap_hb_acc = 0;
   size_t be_stride = 0;            // row stride of be_part/be_ce (the local atom count of the last pass)
   double *bo_dboc = nullptr;       // [N][2]
   double *dDeltap_self = nullptr;  // [N][3]
