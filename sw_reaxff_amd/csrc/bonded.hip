@@ -371,7 +371,9 @@ __global__ void __launch_bounds__(BT) k_dbond_nbr(int N, DbArgs A) {
 // entry) go into 64-bit fixed-point accumulators (2^-40 resolution) by
 // integer atomics, whose sums do not depend on the order they arrive in;
 // k_hb_fold adds them to fCdDelta and clears them.  The results are the same
-// on every run.
+// on every run.  The accumulators hold +-2^23 kcal/mol/A per atom; one
+// triple's terms are bounded by p_hb1 times the angle derivative (a few
+// kcal/mol/A at hydrogen-bond distances), far inside that range.
 constexpr double HB_THRESHOLD = 1e-2;   // reaxc_defs_sunway.h:91
 constexpr double HB_FIX = 1099511627776.0;   // 2^40
 
