@@ -261,6 +261,33 @@ int swr_hydrogen_bonds(swr_system *sys);
  * coefficients (swr_atom_energy); enqueues only. */
 int swr_add_dbond_forces(swr_system *sys);
 
+/* Forces on the local atoms: what PairReaxCSunway::compute does with the
+ * gradient (pair_reaxc_sunway.cpp:746-760, f[k] += -fCdDelta[k] over all N
+ * atoms) followed by LAMMPS' reverse communication, which adds each ghost's
+ * force onto the local atom it images (owner[] of swr_set_atoms, or the
+ * harness's borders).  f[i] = -(fCdDelta[i] + sum over the ghosts g of i of
+ * fCdDelta[g]), summed in ascending ghost order; read back with
+ * swr_get(sys, "f") ([n][3]).  Run after the terms that fill fCdDelta
+ * (swr_vdw_coulomb_energy, swr_hydrogen_bonds, swr_add_dbond_forces);
+ * enqueues only. */
+int swr_compute_forces(swr_system *sys);
+
+/* atom->mass: mass[1..ntypes] per atom type (mass[0] unused, as in LAMMPS).
+ * Defaults to the force field's mass of each type's element. */
+int swr_set_masses(swr_system *sys, int ntypes, const double *mass);
+
+/* FixNVESunway::initial_integrate / final_integrate =
+ * fix_nve_initial_integrate_para / fix_nve_final_integrate_para
+ * (fix_nve_sw64.c:25-173; per-type mass branch): for every local atom
+ * v += dtf/m f, x += dtv v (initial) and v += dtf/m f (final), with f from
+ * the last swr_compute_forces and v from swr_put(sys, "v", ...).  As in the
+ * reference, dtv = dt and dtf = dt/2 * ftm2v (fix_nve_sunway.cpp:56-57;
+ * ftm2v = 1/48.88821291^2 in real units).  Ghost positions are refreshed by
+ * the caller's forward communication or rebuild, as in LAMMPS.  Fails with
+ * "All masses are not set" when a type has no positive mass; enqueues only. */
+int swr_fix_nve_initial_integrate(swr_system *sys, double dtv, double dtf);
+int swr_fix_nve_final_integrate(swr_system *sys, double dtv, double dtf);
+
 /* Host-side packing of nsys independent systems into one batch (no
  * reference counterpart; host code, no device calls).  System k has nat[k]
  * local atoms, listed in order in x (3 per atom), type (1-based, already in

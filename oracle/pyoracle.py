@@ -230,3 +230,37 @@ class Oracle:
 
     def scalar(self, name):
         return self.L.orc_get_scalar_i(self.h, name.encode())
+
+
+# ---------------------------------------------------------------- forces, NVE
+# Plain numpy restatements; pinned in tests/test_md.py by finite differences
+# of the periodic energy (forces), a constant-force closed form and time
+# reversibility (NVE).
+
+def forces(fcd, owner, n):
+    """Forces on the n local atoms: PairReaxCSunway::compute adds -fCdDelta
+    to atom->f of every atom, ghosts included (pair_reaxc_sunway.cpp:746-760),
+    and LAMMPS' reverse communication adds each ghost's force onto the local
+    atom it images (owner[g])."""
+    g = np.asarray(fcd, np.float64).reshape(-1, 4)[:, :3]
+    N = owner.shape[0]
+    f = -g[:n].copy()
+    np.add.at(f, np.asarray(owner[n:N], np.int64), -g[n:N])
+    return f
+
+
+def nve_initial(x, v, f, mass, dtv, dtf):
+    """fix_nve_initial_integrate_para, per-type mass branch
+    (fix_nve_sw64.c:62-99): v += dtf/m f; x += dtv v.  mass is per atom."""
+    dtfm = (dtf / np.asarray(mass, np.float64))[:, None]
+    v = v + dtfm * f
+    return x + dtv * v, v
+
+
+def nve_final(v, f, mass, dtf):
+    """fix_nve_final_integrate_para, per-type mass branch
+    (fix_nve_sw64.c:138-171): v += dtf/m f."""
+    return v + (dtf / np.asarray(mass, np.float64))[:, None] * f
+
+
+FTM2V = 1.0 / 48.88821291 / 48.88821291   # force->ftm2v, LAMMPS real units

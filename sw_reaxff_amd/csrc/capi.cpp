@@ -41,6 +41,9 @@ void fill_params(swr_system *S, const std::vector<int> &map) {
   h.nelem = ff.ne();
   h.ntypes = (int)map.size() - 1;
   for (size_t t = 0; t < map.size(); t++) h.map[t] = map[t];
+  // atom->mass defaults to the force field's element masses (the data files
+  // of the reference list the same values); swr_set_masses overrides them
+  for (size_t t = 1; t < map.size(); t++) h.type_mass[t] = map[t] >= 0 ? ff.atom[map[t]].mass : 0.0;
   for (int e = 0; e < ff.ne() && e < SWR_MAX_ELEM; e++) {
     const FfAtom &a = ff.atom[e];
     h.r_s[e] = a.r_s; h.r_pi[e] = a.r_pi; h.r_pi_pi[e] = a.r_pi_pi;
@@ -334,7 +337,7 @@ int swr_destroy(swr_system *S) {
                   S->Hindex, S->bond_index, S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, S->Cdbo, S->Cdbopi,
                   S->Cdbopi2, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,
                   S->fcd, S->nb_partials, S->nb_result, S->nb_eatom, S->be_part, S->be_ce, S->be_bufc, S->be_result, S->hb_part,
-                  S->hb_result, S->hb_acc,
+                  S->hb_result, S->hb_acc, S->f, S->gcsr_cnt, S->gcsr_start, S->gcsr_list,
                   S->bond_mark2, S->counts, S->ws, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32,
                   S->Hdia_inv, S->b_s, S->st, S->s_hist, S->t_hist, S->cgv, S->cgs, S->partials, S->counters,
                   S->status, S->scan_tmp, S->tmp_i, S->mixw, S->cand, S->cand_count, S->row_sys, S->tile_beg,
@@ -370,6 +373,8 @@ int swr_set_atoms(swr_system *S, int n, int N, const int *tag, const int *type, 
   S->layout_dirty = true;
   S->next_totals_known = false;
   S->vtotal_local = -1;
+  S->gcsr_valid = false;
+  S->have_f = false;
   S->n = n;
   S->N = N;
   HIP_CHECK(hipMemcpyAsync(S->x, x, sizeof(double) * 3 * N, hipMemcpyHostToDevice, S->stream));
@@ -656,6 +661,45 @@ int swr_hydrogen_bonds(swr_system *S) {
   return swr_hydrogen_bonds_run(S);
 }
 
+int swr_compute_forces(swr_system *S) {
+  if (!S) return SWR_ERR_ARG;
+  int rc;
+  if ((rc = swr_quiesce(S))) return rc;
+  return swr_forces_run(S);
+}
+
+int swr_set_masses(swr_system *S, int ntypes, const double *mass) {
+  if (!S || !mass || ntypes != S->hp.ntypes) {
+    swr_set_error_msg("swr_set_masses: needs mass[1..ntypes] for every atom type");
+    return SWR_ERR_ARG;
+  }
+  for (int t = 1; t <= ntypes; t++)
+    if (!(mass[t] > 0.0)) {
+      swr_set_error_msg("Invalid mass value");
+      return SWR_ERR_ARG;
+    }
+  int rc;
+  if ((rc = swr_quiesce(S))) return rc;
+  for (int t = 1; t <= ntypes; t++) S->hp.type_mass[t] = mass[t];
+  HIP_CHECK(hipMemcpyAsync(S->dp, &S->hp, sizeof(DevParams), hipMemcpyHostToDevice, S->stream));
+  HIP_CHECK(hipStreamSynchronize(S->stream));  // S->hp may change again before the copy ran
+  return SWR_OK;
+}
+
+int swr_fix_nve_initial_integrate(swr_system *S, double dtv, double dtf) {
+  if (!S) return SWR_ERR_ARG;
+  int rc;
+  if ((rc = swr_quiesce(S))) return rc;
+  return swr_nve_run(S, 0, dtv, dtf);
+}
+
+int swr_fix_nve_final_integrate(swr_system *S, double dtv, double dtf) {
+  if (!S) return SWR_ERR_ARG;
+  int rc;
+  if ((rc = swr_quiesce(S))) return rc;
+  return swr_nve_run(S, 1, dtv, dtf);
+}
+
 int swr_sync(swr_system *S) {
   if (!S) return SWR_ERR_ARG;
   int rc = refresh_status(S);
@@ -725,6 +769,7 @@ bool lookup(swr_system *S, const std::string &k, View &v) {
   if (k == "bonded_eatom") return set(S->be_part, S->be_part ? 40 * (int64_t)S->be_stride : 0);
   if (k == "hbond_energy") return set(S->hb_result, S->hb_result ? 8 : 0);
   if (k == "hbond_eatom") return set(S->hb_part, S->hb_part ? 8 * n : 0);
+  if (k == "f") return set(S->f, S->have_f ? 24 * n : 0);
   static const char *wsn[WS_COUNT] = {"Deltap", "Deltap_boc", "Delta", "Delta_e", "Delta_val", "vlpex", "nlp",
                                       "nlp_temp", "Clp", "Delta_lp", "dDelta_lp", "Delta_lp_temp", "dDelta_lp_temp"};
   for (int w = 0; w < WS_COUNT; w++)

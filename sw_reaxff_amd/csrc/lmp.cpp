@@ -3,9 +3,14 @@
 // It provides what LAMMPS provides to the reference's styles on this path:
 // the box, ghost images (Comm::borders / forward comm, done on the device),
 // the re-neighbour trigger (neigh_modify every k) and velocity-Verlet NVE.
-// The forces of the full ReaxFF energy are outside this path, so NVE moves
-// atoms with the velocities they have (zero force); positions still change
-// every step, and every step recomputes the lists, bond orders and charges.
+// By default the pair style runs only the path (lists, bond orders, charges)
+// and NVE moves atoms with the velocities they have (zero force); positions
+// still change every step, and every step recomputes the lists, bond orders
+// and charges.  The harness command `forces yes` (not a LAMMPS command:
+// LAMMPS always computes forces) adds the energy terms built here (bond,
+// lone-pair, over-/under-coordination, hydrogen bonds, van der Waals,
+// Coulomb, polarization; no valence angles or torsions) and integrates with
+// their forces, as FixNVESunway does; `mass T m` sets atom->mass.
 //
 // The style classes keep the reference's command surface and call order:
 //   PairReaxCAMD   <-> PairReaxCSunway   (pair_reaxc_sunway.cpp)
@@ -129,6 +134,8 @@ struct swr_lmp {
   double skin = 2.0;
   int every = 10;
   double dt = 0.25;
+  bool forces = false;
+  std::vector<double> mass_set;   // [type] from `mass` commands, 0 = force-field default
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   int periodic = 1;
   int n = 0;
@@ -200,6 +207,30 @@ struct swr_lmp {
     mark(9);
     if ((rc = swr_bo(sys))) return rc;
     mark(10);
+    if (forces) {
+      // Compute_Bonded_Forces, Compute_NonBonded_Forces, Compute_Total_Force
+      // (reaxc_forces_sunway.cpp), then the copy into atom->f + reverse comm
+      if ((rc = swr_atom_energy(sys)) || (rc = swr_hydrogen_bonds(sys)) || (rc = swr_vdw_coulomb_energy(sys)) ||
+          (rc = swr_add_dbond_forces(sys)) || (rc = swr_compute_forces(sys)))
+        return rc;
+    }
+    return SWR_OK;
+  }
+
+  // force->ftm2v in real units (LAMMPS force.cpp / update.cpp)
+  static constexpr double FTM2V = 1.0 / 48.88821291 / 48.88821291;
+
+  // potential energy of the terms computed (eng_vdwl + eng_coul)
+  int potential_energy(double *pe) {
+    int rc;
+    if ((rc = swr_sync(sys))) return rc;
+    double nb[8], be[5], hb = 0.0;
+    if (!forces || swr_get(sys, "nonbonded_energy", nb) != (int64_t)sizeof(nb) ||
+        swr_get(sys, "bonded_energy", be) != (int64_t)sizeof(be) || swr_get(sys, "hbond_energy", &hb) != 8) {
+      swr_set_error_msg("the potential energy needs `forces yes`");
+      return SWR_ERR_STATE;
+    }
+    *pe = nb[0] + nb[1] + be[0] + be[1] + be[2] + be[3] + be[4] + hb;
     return SWR_OK;
   }
 
@@ -213,6 +244,12 @@ struct swr_lmp {
     for (auto &e : pair.elements) el.push_back(e.c_str());
     int rc = swr_create(pair.ffield.c_str(), (int)el.size(), el.data(), &ctl, &sys);
     if (rc) return rc;
+    if (!mass_set.empty()) {
+      std::vector<double> m(sys->hp.type_mass, sys->hp.type_mass + sys->hp.ntypes + 1);
+      for (size_t t = 1; t < mass_set.size() && t < m.size(); t++)
+        if (mass_set[t] > 0.0) m[t] = mass_set[t];
+      if ((rc = swr_set_masses(sys, sys->hp.ntypes, m.data()))) return rc;
+    }
     for (int t : type0)
       if (t < 1 || t > (int)el.size()) { swr_set_error_msg("atom type without pair_coeff element"); return SWR_ERR_ARG; }
     if ((rc = swr_set_atoms(sys, n, n, tag0.data(), type0.data(), x0.data(), nullptr, nullptr, nullptr))) return rc;
@@ -237,7 +274,11 @@ struct swr_lmp {
     int rc;
     step++;
     mark(0);
-    if ((rc = swr_nbr_nve(sys, dt))) return rc;  // initial_integrate (+ final: F = 0)
+    if (forces) {
+      if ((rc = swr_fix_nve_initial_integrate(sys, dt, 0.5 * dt * FTM2V))) return rc;
+    } else if ((rc = swr_nbr_nve(sys, dt))) {
+      return rc;  // initial_integrate (+ final: F = 0)
+    }
     mark(1);
     if (step - last_rebuild >= every) {
       if ((rc = neighbor_build())) return rc;
@@ -246,7 +287,9 @@ struct swr_lmp {
       mark(2);
       if (periodic && (rc = swr_nbr_forward_x(sys, lo, hi))) return rc;
     }
-    return force_stages();
+    if ((rc = force_stages())) return rc;
+    if (forces && (rc = swr_fix_nve_final_integrate(sys, dt, 0.5 * dt * FTM2V))) return rc;
+    return SWR_OK;
   }
 
   int run(int nsteps) {
@@ -313,6 +356,20 @@ int swr_lmp_command(swr_lmp *L, const char *cmd) {
   } else if (c == "timestep") {
     if (a.size() != 1) err = "Illegal timestep command";
     else L->dt = std::atof(a[0].c_str());
+  } else if (c == "forces") {
+    if (a.size() != 1 || (a[0] != "yes" && a[0] != "no")) err = "Illegal forces command";
+    else if (L->is_setup) err = "forces must be set before the run";
+    else L->forces = a[0] == "yes";
+  } else if (c == "mass") {
+    const int t = a.size() == 2 ? std::atoi(a[0].c_str()) : 0;
+    const double m = a.size() == 2 ? std::atof(a[1].c_str()) : 0.0;
+    if (t < 1 || t > SWR_MAX_TYPES) err = "Illegal mass command";
+    else if (!(m > 0.0)) err = "Invalid mass value";
+    else if (L->is_setup) err = "mass must be set before the run";
+    else {
+      if ((int)L->mass_set.size() <= t) L->mass_set.resize(t + 1, 0.0);
+      L->mass_set[t] = m;
+    }
   } else if (c == "profile") {
     L->profile = (a.size() == 1 && a[0] == "yes") ? 1 : 0;
   } else if (c == "units" || c == "atom_style") {
@@ -362,6 +419,11 @@ double swr_lmp_stat(swr_lmp *L, const char *name) {
   if (!L || !name) return NAN;
   std::string k = name;
   if (k == "step") return (double)L->step;
+  if (k == "pe") {
+    double pe = NAN;
+    if (!L->sys || L->potential_energy(&pe)) return NAN;
+    return pe;
+  }
   if (k == "wall_ms") return L->wall_ms;
   if (k == "steps_run") return (double)L->steps_run;
   if (k == "iters_s") return L->iters_s;

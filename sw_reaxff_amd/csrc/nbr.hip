@@ -311,6 +311,7 @@ int swr_nbr_ensure_atoms(swr_system *S, int Ncap);
 // Builds ghost images of the n local atoms (already remapped into the box).
 int swr_nbr_borders(swr_system *S, const double lo[3], const double hi[3], double cutghost) {
   S->next_totals_known = false;  // new ghosts take their owners' capacities
+  S->gcsr_valid = false;
   const int n = S->n;
   int ncur = n;
   hipLaunchKernelGGL(k_init_local_maps, dim3(div_up(n, 256)), dim3(256), 0, S->stream, S->owner, S->img, n);

@@ -72,6 +72,7 @@ struct DevBe {
 struct DevParams {
   int nelem, ntypes;
   int map[SWR_MAX_TYPES + 1];
+  double type_mass[SWR_MAX_TYPES + 1];   // atom->mass per LAMMPS type (fix nve)
   double r_s[SWR_MAX_ELEM], r_pi[SWR_MAX_ELEM], r_pi_pi[SWR_MAX_ELEM];
   double valency[SWR_MAX_ELEM], valency_e[SWR_MAX_ELEM], valency_boc[SWR_MAX_ELEM];
   double valency_val[SWR_MAX_ELEM], nlp_opt[SWR_MAX_ELEM], mass[SWR_MAX_ELEM];
@@ -224,6 +225,15 @@ struct swr_system {
   double *hb_result = nullptr;     // e_hb of the last Hydrogen_Bonds
   unsigned long long *hb_acc = nullptr;   // [N][3] fixed-point hydrogen-bond terms (zero between calls)
   size_t cap_hb_part = 0, cap_hb_result = 0, cap_hb_acc = 0;
+  // forces on local atoms (ghost rows of fCdDelta summed onto owners) and the
+  // ghost-per-owner CSR that sums them in a fixed order
+  double *f = nullptr;                     // [n][3]
+  size_t cap_f = 0;
+  bool have_f = false;
+  int *gcsr_cnt = nullptr, *gcsr_start = nullptr, *gcsr_list = nullptr;
+  size_t cap_gcsr_cnt = 0, cap_gcsr_start = 0, cap_gcsr_list = 0;
+  bool gcsr_valid = false;
+  int gcsr_n = 0, gcsr_N = 0;
   size_t be_stride = 0;            // row stride of be_part/be_ce (the local atom count of the last pass)
   double *bo_dboc = nullptr;       // [N][2]
   double *dDeltap_self = nullptr;  // [N][3]
@@ -338,6 +348,8 @@ int swr_nonbonded_run(swr_system *S);
 int swr_atom_energy_run(swr_system *S);
 int swr_dbond_forces_run(swr_system *S);
 int swr_hydrogen_bonds_run(swr_system *S);
+int swr_forces_run(swr_system *S);
+int swr_nve_run(swr_system *S, int stage, double dtv, double dtf);
 int swr_quiesce(swr_system *S);
 constexpr int MARK_INNER = 8;         // bond_mark relaxation sweeps per launch
 constexpr int MARK_MAX_LAUNCH = 128;  // launches of one relaxation at most

@@ -89,8 +89,12 @@ class Lammps:
 
 
 def standard_script(elements, ffield=DEFAULT_FFIELD, skin=2.0, every=10, dt=0.25, tol=1e-6, mixed=False,
-                    profile=False):
-    """The LAMMPS input this path is benchmarked with (pair_style NULL defaults)."""
+                    profile=False, forces=False):
+    """The LAMMPS input this path is benchmarked with (pair_style NULL defaults).
+
+    forces=True adds the harness command `forces yes`: the energy terms built
+    here and NVE with their forces (without it NVE moves atoms at constant
+    velocity and only the path runs)."""
     return "\n".join([
         "units real",
         "atom_style charge",
@@ -101,4 +105,4 @@ def standard_script(elements, ffield=DEFAULT_FFIELD, skin=2.0, every=10, dt=0.25
         f"neigh_modify every {every}",
         f"timestep {dt}",
         f"profile {'yes' if profile else 'no'}",
-    ])
+    ] + (["forces yes"] if forces else []))

@@ -19,10 +19,10 @@ BO_RTOL = 1e-10
 Q_ATOL = 1e-6
 
 
-def harness(system, ffield, steps=0, every=10, dt=0.25, mixed=False, profile=False, tol=1e-6):
+def harness(system, ffield, steps=0, every=10, dt=0.25, mixed=False, profile=False, tol=1e-6, forces=False):
     lmp = Lammps()
     lmp.commands(standard_script(system.elements, ffield=ffield, every=every, dt=dt, mixed=mixed,
-                                 profile=profile, tol=tol))
+                                 profile=profile, tol=tol, forces=forces))
     lmp.create_atoms(system)
     lmp.setup()
     if steps:

@@ -1,0 +1,297 @@
+"""Forces on local atoms and velocity-Verlet NVE (fix nve).
+
+Oracle pins (CPU):
+  * the fold of ghost gradients onto owners (PairReaxCSunway::compute's
+    f[k] += -fCdDelta[k] plus LAMMPS' reverse communication) reproduces minus
+    the finite-difference derivative of a periodic water box's energy (bond,
+    lone-pair, over/under-coordination, hydrogen-bond, van der Waals and
+    Coulomb at fixed charges) with respect to a local atom's position, its
+    images moving with it;
+  * NVE (fix_nve_sw64.c) is exact for a constant force (x = x0 + v0 t +
+    a t^2 / 2) and time-reversible for a linear force.
+GPU parity: swr_compute_forces == the oracle's fold of the oracle's gradient
+(1e-10 relative) and == the fold of the device's own gradient (1e-13), one
+harness step of swr_fix_nve_* == the restated update with a `mass`
+override, and 20 harness MD steps == the oracle's NVE trajectory (positions,
+velocities, energy).  On the CPU the oracle's energy error is second order
+in the time step where the bond set is fixed (forces consistent with the
+energy along a trajectory).
+"""
+import numpy as np
+import pytest
+
+from sw_reaxff_amd import systems
+
+from .test_bonded import run_stages
+
+
+# ---------------------------------------------------------------- NVE pins
+def test_nve_constant_force_closed_form(oracle_lib):
+    po = oracle_lib
+    rng = np.random.default_rng(1)
+    n = 50
+    x0, v0, f = rng.normal(size=(n, 3)), rng.normal(size=(n, 3)) * 0.01, rng.normal(size=(n, 3))
+    m = rng.uniform(1.0, 16.0, size=n)
+    dt, k = 0.25, 40
+    dtf = 0.5 * dt * po.FTM2V
+    x, v = x0, v0
+    for _ in range(k):
+        x, v = po.nve_initial(x, v, f, m, dt, dtf)
+        v = po.nve_final(v, f, m, dtf)
+    a = f * po.FTM2V / m[:, None]
+    t = k * dt
+    assert np.allclose(x, x0 + v0 * t + 0.5 * a * t * t, rtol=1e-12, atol=1e-12)
+    assert np.allclose(v, v0 + a * t, rtol=1e-12, atol=1e-14)
+
+
+def test_nve_is_time_reversible(oracle_lib):
+    po = oracle_lib
+    rng = np.random.default_rng(2)
+    n = 30
+    K = rng.uniform(100.0, 500.0, size=(n, 1))
+    xeq = rng.normal(size=(n, 3))
+    x0 = xeq + 0.1 * rng.normal(size=(n, 3))
+    v0 = 0.01 * rng.normal(size=(n, 3))
+    m = rng.uniform(1.0, 16.0, size=n)
+    dt = 0.5
+    dtf = 0.5 * dt * po.FTM2V
+
+    def run(x, v, steps):
+        f = -K * (x - xeq)
+        for _ in range(steps):
+            x, v = po.nve_initial(x, v, f, m, dt, dtf)
+            f = -K * (x - xeq)
+            v = po.nve_final(v, f, m, dtf)
+        return x, v
+
+    x1, v1 = run(x0, v0, 60)
+    assert np.abs(x1 - x0).max() > 1e-2          # it moved
+    x2, v2 = run(x1, -v1, 60)
+    assert np.allclose(x2, x0, rtol=0, atol=1e-11)
+    assert np.allclose(-v2, v0, rtol=0, atol=1e-13)
+
+
+# ---------------------------------------------------------------- force fold pin
+def _periodic_oracle(oracle_lib, ffield, s, xl, ql, cutghost):
+    xa, owner = systems.borders(xl, s.lo, s.hi, cutghost)
+    o = oracle_lib.Oracle(ffield, s.elements, threads=8)
+    o.set_atoms(s.n, xa, s.type[owner].astype(np.int32), s.tag[owner].astype(np.int32), owner.astype(np.int32),
+                q=ql[owner])
+    o.build_verlet(12.0)
+    run_stages(o)
+    e = float(o.atom_energy()[:4].sum()) + o.hydrogen_bonds()
+    ev, ec, _ = o.vdw_coulomb()
+    return o, owner, e + ev + ec
+
+
+def test_forces_match_finite_differences_periodic(oracle_lib, ffield):
+    po = oracle_lib
+    s = systems.water_box(64, seed=6)
+    rng = np.random.default_rng(4)
+    q = rng.uniform(-0.4, 0.4, size=s.n)
+    q -= q.mean()
+    cutghost = 10.0
+    o, owner, e0 = _periodic_oracle(oracle_lib, ffield, s, s.x, q, cutghost)
+    o.dbond_forces()
+    f = po.forces(o.get("fCdDelta"), owner, s.n)
+    assert owner.size > 4 * s.n                   # most terms reach ghosts
+    fmax = np.abs(f).max()
+    h = 1e-5
+    worst = 0.0
+    for a in rng.choice(s.n, size=5, replace=False):
+        for k in range(3):
+            xp, xm = s.x.copy(), s.x.copy()
+            xp[a, k] += h
+            xm[a, k] -= h
+            ep = _periodic_oracle(oracle_lib, ffield, s, xp, q, cutghost)[2]
+            em = _periodic_oracle(oracle_lib, ffield, s, xm, q, cutghost)[2]
+            worst = max(worst, abs(-(ep - em) / (2 * h) - f[a, k]) / fmax)
+    print(f"[md] periodic water n={s.n} N={owner.size} E={e0:.6e} max|f| {fmax:.3e} worst FD error {worst:.2e}")
+    assert worst <= 1e-6
+    # the fold matters: without the ghost rows the forces are wrong
+    g_local = -o.get("fCdDelta").reshape(-1, 4)[: s.n, :3]
+    assert np.abs(g_local - f).max() > 1e-3 * fmax
+
+
+def test_forces_fold_sums_every_image(oracle_lib):
+    po = oracle_lib
+    n, N = 3, 9
+    owner = np.array([0, 1, 2, 0, 0, 2, 1, 0, 2], np.int32)
+    fcd = np.zeros((N, 4))
+    fcd[:, 0] = np.arange(N) + 1.0
+    fcd[:, 3] = 100.0                              # CdDelta column is not a force
+    f = po.forces(fcd, owner, n)
+    assert f[:, 0].tolist() == [-(1 + 4 + 5 + 8), -(2 + 7), -(3 + 6 + 9)]
+    assert not f[:, 1:].any()
+
+
+# ---------------------------------------------------------------- GPU
+def _harness(s, ffield, tol, extra=(), dt=0.25, qeq=True):
+    from sw_reaxff_amd.lammps import Lammps, standard_script
+    lmp = Lammps()
+    script = standard_script(s.elements, ffield=ffield, tol=tol, dt=dt, forces=True)
+    if not qeq:
+        script = "\n".join(ln for ln in script.split("\n") if "qeq" not in ln).replace(
+            "pair_style reax/c/amd NULL", "pair_style reax/c/amd NULL checkqeq no")
+    lmp.commands(script)
+    for c in extra:
+        lmp.command(c)
+    lmp.create_atoms(s)
+    lmp.setup()
+    return lmp
+
+
+@pytest.mark.gpu
+def test_gpu_forces_match_oracle(gpu_lib, oracle_lib, ffield):
+    po = oracle_lib
+    s = systems.water_box(648, seed=3)
+    lmp = _harness(s, ffield, 1e-10)
+    h = lmp.system
+    h.sync()
+    n, N = h.scalar("n"), h.scalar("N")
+    f_gpu = h.get("f").reshape(-1, 3)
+    assert f_gpu.shape == (n, 3)
+    owner = h.get("owner")[:N]
+    fcd_gpu = h.get("fCdDelta")[: 4 * N]
+    f_fold = po.forces(fcd_gpu, owner, n)
+    assert np.max(np.abs(f_gpu - f_fold)) <= 1e-13 * np.abs(f_fold).max()
+    xa = h.get("x").reshape(-1, 3)[:N]
+    o = oracle_lib.Oracle(ffield, s.elements, threads=8)
+    o.set_atoms(n, xa, s.type[owner].astype(np.int32), s.tag[owner].astype(np.int32), owner.astype(np.int32),
+                q=h.get("q")[:N])
+    o.build_verlet(12.0)
+    run_stages(o)
+    o.atom_energy()
+    o.hydrogen_bonds()
+    o.vdw_coulomb()
+    o.dbond_forces()
+    f_o = po.forces(o.get("fCdDelta"), owner, n)
+    err = np.max(np.abs(f_gpu - f_o)) / np.abs(f_o).max()
+    print(f"[md] water box n={n} N={N} max|f| {np.abs(f_o).max():.3e} GPU vs oracle {err:.2e}", flush=True)
+    assert err <= 1e-10
+
+
+@pytest.mark.gpu
+def test_gpu_nve_step_matches_restatement(gpu_lib, oracle_lib, ffield):
+    po = oracle_lib
+    s = systems.water_box(648, seed=8)
+    mH = 2.014                                    # a `mass` override for H
+    lmp = _harness(s, ffield, 1e-8, extra=(f"mass {s.elements.index('H') + 1} {mH}",))
+    h = lmp.system
+    h.sync()
+    n = h.scalar("n")
+    x0 = h.get("x").reshape(-1, 3)[:n].copy()
+    v0 = h.get("v").reshape(-1, 3).copy()
+    f0 = h.get("f").reshape(-1, 3).copy()
+    m = np.where(s.type == s.elements.index("H") + 1, mH, s.masses)
+    dt = 0.25
+    dtf = 0.5 * dt * po.FTM2V
+    lmp.run(1)                                    # no rebuild: positions stay unwrapped
+    h.sync()
+    x1 = h.get("x").reshape(-1, 3)[:n]
+    v1 = h.get("v").reshape(-1, 3)
+    f1 = h.get("f").reshape(-1, 3)
+    xw, vh = po.nve_initial(x0, v0, f0, m, dt, dtf)
+    vw = po.nve_final(vh, f1, m, dtf)
+    assert np.abs(f0).max() > 1.0 and np.abs(x1 - x0).max() > 1e-4
+    assert np.max(np.abs(x1 - xw)) <= 1e-13 * np.abs(xw).max()
+    assert np.max(np.abs(v1 - vw)) <= 1e-13 * np.abs(vw).max()
+
+
+def _oracle_md(oracle_lib, ffield, s, dt, nsteps, every, cutghost=10.0):
+    """NVE with the oracle's forces (charges fixed at zero): borders and the
+    whole list + bond-order + energy pass every step; returns the local
+    positions (wrapped into the box when periodic), velocities and total
+    energy minus its start, every `every` steps."""
+    po = oracle_lib
+    m = s.masses
+    L = s.hi - s.lo
+
+    def energy_forces(x):
+        if s.periodic:
+            xa, owner = systems.borders(s.lo + np.mod(x - s.lo, L), s.lo, s.hi, cutghost)
+        else:
+            xa, owner = x, np.arange(s.n, dtype=np.int32)
+        o = oracle_lib.Oracle(ffield, s.elements, threads=8)
+        o.set_atoms(s.n, xa, s.type[owner].astype(np.int32), s.tag[owner].astype(np.int32),
+                    owner.astype(np.int32), q=np.zeros(owner.size))
+        o.build_verlet(12.0)
+        run_stages(o)
+        e = float(o.atom_energy()[:4].sum()) + o.hydrogen_bonds()
+        ev, ec, _ = o.vdw_coulomb()
+        o.dbond_forces()
+        return e + ev + ec, po.forces(o.get("fCdDelta"), owner, s.n)
+
+    def ke(v):
+        return 0.5 * float((m[:, None] * v * v).sum()) / po.FTM2V
+
+    dtf = 0.5 * dt * po.FTM2V
+    x, v = s.x.copy(), s.v.copy()
+    e, f = energy_forces(x)
+    e0 = e + ke(v)
+    dev = []
+    for k in range(nsteps):
+        x, v = po.nve_initial(x, v, f, m, dt, dtf)
+        e, f = energy_forces(x)
+        v = po.nve_final(v, f, m, dtf)
+        if (k + 1) % every == 0:
+            dev.append(e + ke(v) - e0)
+    if s.periodic:
+        x = s.lo + np.mod(x - s.lo, L)
+    return x, v, np.array(dev)
+
+
+def test_oracle_nve_energy_error_is_second_order(oracle_lib, ffield):
+    """On a water cluster whose bond set stays fixed the energy of the terms
+    built here is smooth, and velocity Verlet with the oracle's forces keeps
+    it within dt^2 g(t) of its start: halving dt quarters every deviation.
+    (A force missing a term or with a wrong factor leaves a deviation that
+    does not shrink with dt.)  Where bonds enter and leave the list at the
+    bond-order cutoff, or pairs cross the hydrogen-bond cutoff, the energy
+    jumps; a 648-water box then also shows a deviation independent of dt,
+    the same on the GPU as here."""
+    from .test_bonded import water_cluster
+    s = water_cluster(8, seed=2)
+    devs = [np.abs(_oracle_md(oracle_lib, ffield, s, dt, int(round(2.0 / dt)), int(round(0.5 / dt)))[2])
+            for dt in (0.1, 0.05, 0.025)]
+    print(f"[md] cluster n={s.n} |E-E0| at 0.5..2 fs for dt 0.1/0.05/0.025: {[d.tolist() for d in devs]}")
+    assert devs[0].max() > 1e-3
+    for a, b in zip(devs, devs[1:]):
+        assert np.all((a / b > 3.6) & (a / b < 4.4))
+
+
+@pytest.mark.gpu
+def test_gpu_md_trajectory_matches_oracle(gpu_lib, oracle_lib, ffield):
+    """20 steps of 0.1 fs with `forces yes` (zero charges) through the harness
+    (device borders, rebuild every 10 steps, forward communication between)
+    against the oracle's own NVE with borders and lists rebuilt every step."""
+    po = oracle_lib
+    s = systems.water_box(64, seed=5)
+    dt, nsteps = 0.1, 20
+    lmp = _harness(s, ffield, 1e-8, dt=dt, qeq=False)
+    h = lmp.system
+    m = s.masses
+
+    def total():
+        v = h.get("v").reshape(-1, 3)
+        return lmp.stat("pe") + 0.5 * float((m[:, None] * v * v).sum()) / po.FTM2V
+
+    e0 = total()
+    dev = []
+    for _ in range(nsteps // 5):
+        lmp.run(5)
+        dev.append(total() - e0)
+    n = h.scalar("n")
+    L = s.hi - s.lo
+    x_gpu = s.lo + np.mod(h.get("x").reshape(-1, 3)[:n] - s.lo, L)
+    v_gpu = h.get("v").reshape(-1, 3)
+    x_o, v_o, dev_o = _oracle_md(oracle_lib, ffield, s, dt, nsteps, 5)
+    dx = np.abs(x_gpu - x_o)
+    dx = np.minimum(dx, L - dx)
+    print(f"[md] trajectory n={s.n} max|dx| {dx.max():.2e} max|dv|/max|v| "
+          f"{np.abs(v_gpu - v_o).max() / np.abs(v_o).max():.2e} E-E0 gpu {np.round(dev, 6).tolist()} "
+          f"oracle {np.round(dev_o, 6).tolist()}", flush=True)
+    assert np.abs(x_gpu - s.x).max() > 1e-3
+    assert dx.max() <= 1e-9
+    assert np.abs(v_gpu - v_o).max() <= 1e-9 * np.abs(v_o).max()
+    assert np.allclose(dev, dev_o, rtol=0, atol=1e-6)
