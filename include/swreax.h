@@ -60,6 +60,7 @@ typedef struct swr_control_t {
   int qeq_mixed;        /* 1: FP32 inner CG on an FP32 copy of H, FP64 refinement */
   int qeq_imax;         /* CG iteration cap, 200 in the reference              */
   int enobonds;         /* pair_style keyword enobonds (1 = yes, the default)  */
+  double thb_cutsq;     /* valence-angle BO product cutoff, 0.00001 (:219)    */
 } swr_control_t;
 
 typedef struct swr_system swr_system;
@@ -251,6 +252,22 @@ int swr_atom_energy(swr_system *sys);
  * "hbond_eatom".  Run after swr_atom_energy and before
  * swr_add_dbond_forces, as the reference orders them; enqueues only. */
 int swr_hydrogen_bonds(swr_system *sys);
+
+/* Valence angles: the valence part of Merge_Torsion_Valence_Angles
+ * (reaxc_torsion_angles_sw64.c:577-1117, host form :812-1117), which
+ * Torsion_Angles calls (reaxc_torsion_angles_sunway.cpp:538-559;
+ * Valence_Angles itself returns at once, reaxc_valence_angles_sunway.cpp:92-96).  For every
+ * local atom j and pair of its bonds j-k, j-h with BO > thb_cut and
+ * BO_jk BO_hj > thb_cutsq, and every parameter set of the force field's
+ * valence-angle block for (k, j, h) with |p_val1| > 0.001: angle, penalty
+ * and coalition energies, their coefficients into Cdbo/Cdbopi/Cdbopi2 of j's
+ * bonds and CdDelta of j, k, h, and the angle derivative into
+ * fCdDelta[.][0..2] of j, k, h.  Leaves e_ang, e_pen, e_coa in
+ * swr_get(sys, "valence_energy") and each local atom's share in
+ * "valence_eatom" (3 rows of n doubles).  The torsion and conjugation part
+ * of the same function is not built.  Run after swr_bo and before
+ * swr_add_dbond_forces; enqueues only. */
+int swr_valence_angles(swr_system *sys);
 
 /* Add_All_dBond_to_Forces_C (reaxc_forces_sw64.c:417-589): over all N atoms
  * and their bonds, the chain rule of the bond-order derivatives with the

@@ -55,6 +55,7 @@ def lib():
         L.orc_dbond_forces.argtypes = [vp]
         L.orc_hydrogen_bonds.restype = C.c_double
         L.orc_hydrogen_bonds.argtypes = [vp]
+        L.orc_valence_angles.argtypes = [vp, dp]
         L.orc_get_array.restype = C.c_int64
         L.orc_get_array.argtypes = [vp, C.c_char_p, vp]
         L.orc_set_array.argtypes = [vp, C.c_char_p, vp, C.c_int64]
@@ -200,6 +201,14 @@ class Oracle:
         """Hydrogen_Bonds over the current bonds and hbond lists: adds CEhb1 to
         Cdbo and the angle/distance terms to fCdDelta[:, 0..2]; returns e_hb."""
         return float(self.L.orc_hydrogen_bonds(self.h))
+
+    def valence_angles(self):
+        """Valence part of Merge_Torsion_Valence_Angles over the current bonds:
+        adds to Cdbo/Cdbopi/Cdbopi2, CdDelta and fCdDelta[:, 0..2]; returns
+        (e_ang, e_pen, e_coa)."""
+        o = np.zeros(3)
+        self.L.orc_valence_angles(self.h, _p(o, C.c_double))
+        return o
 
     def dbond_forces(self):
         """Add_All_dBond_to_Forces: adds the bond-order chain rule to

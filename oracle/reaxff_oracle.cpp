@@ -60,6 +60,12 @@ struct HBP {
   double r0_hb, p_hb1, p_hb2, p_hb3;
 };
 
+// three_body_parameters (reaxc_ctypes_sunway.h); REAX_MAX_3BODY_PARAM = 5
+struct THBP {
+  double theta_00, p_val1, p_val2, p_coa1, p_val7, p_pen1, p_val4;
+};
+constexpr int MAX_3BODY_PARAM = 5;
+
 struct TBP {
   double p_bo1, p_bo2, p_bo3, p_bo4, p_bo5, p_bo6;
   double r_s, r_p, r_pp;
@@ -97,6 +103,9 @@ struct orc_sys {
   std::vector<SBP> sbp;
   std::vector<std::vector<TBP>> tbp;
   std::vector<HBP> hbp;          // [nt][nt][nt], r0_hb < 0: no hydrogen bond
+  std::vector<int> thb_cnt;      // [nt][nt][nt]
+  std::vector<THBP> thbp;        // [nt][nt][nt][MAX_3BODY_PARAM], zero-filled
+  double thb_cut = 0.001, thb_cutsq = 0.00001;   // pair_reaxc_sunway.cpp:218-219
   double bo_cut = 0, nonb_low = 0, nonb_cut = 0;
   int vdw_type = 0;
   double bond_cut = 5.0, hbond_cut = 7.5;
@@ -281,13 +290,33 @@ bool read_ffield(orc_sys *S, const char *path) {
       if ((val = tokd(t, 7)) > 0.0) S->tbp[j][kk].r_pp = S->tbp[kk][j].r_pp = val;
     }
   }
-  // valence-angle and torsion blocks (:547-...) are skipped by their counts;
-  // the hydrogen-bond block: r0_hb = -1 unless listed, then j k m r0 p1 p2 p3
+  // valence-angle block (:520-572): line "j k m" writes prm[cnt(j,k,m)] of
+  // both (j,k,m) and (m,k,j) and counts both up; then the torsion block,
+  // skipped by its count; then the hydrogen-bond block: r0_hb = -1 unless
+  // listed, then j k m r0 p1 p2 p3
   S->hbp.assign((size_t)nt * nt * nt, HBP{-1.0, 0.0, 0.0, 0.0});
-  for (int blk = 0; blk < 2; blk++) {
+  S->thb_cnt.assign((size_t)nt * nt * nt, 0);
+  S->thbp.assign((size_t)nt * nt * nt * MAX_3BODY_PARAM, THBP{0, 0, 0, 0, 0, 0, 0});
+  {
     t = next();
     const int l = t.empty() ? 0 : std::atoi(t[0].c_str());
-    for (int k = 0; k < l; k++) next();
+    for (int k = 0; k < l; k++) {
+      t = next();
+      if (t.size() < 10) continue;
+      const int a = std::atoi(t[0].c_str()) - 1, b = std::atoi(t[1].c_str()) - 1, c = std::atoi(t[2].c_str()) - 1;
+      if (a < 0 || b < 0 || c < 0 || a >= nt || b >= nt || c >= nt) continue;
+      const size_t abc = ((size_t)a * nt + b) * nt + c, cba = ((size_t)c * nt + b) * nt + a;
+      const int cnt = S->thb_cnt[abc];
+      if (cnt >= MAX_3BODY_PARAM || S->thb_cnt[cba] >= MAX_3BODY_PARAM) continue;
+      S->thb_cnt[abc]++;
+      S->thb_cnt[cba]++;
+      const THBP v{tokd(t, 3), tokd(t, 4), tokd(t, 5), tokd(t, 6), tokd(t, 7), tokd(t, 8), tokd(t, 9)};
+      S->thbp[abc * MAX_3BODY_PARAM + cnt] = v;
+      S->thbp[cba * MAX_3BODY_PARAM + cnt] = v;
+    }
+    t = next();
+    const int l2 = t.empty() ? 0 : std::atoi(t[0].c_str());
+    for (int k = 0; k < l2; k++) next();
   }
   t = next();
   const int nh = t.empty() ? 0 : std::atoi(t[0].c_str());
@@ -1495,6 +1524,182 @@ double orc_hydrogen_bonds(orc_sys *S) {
     }
   }
   return e_hb_total;
+}
+
+// Valence part of Merge_Torsion_Valence_Angles, host form
+// (reaxc_torsion_angles_sw64.c:812-1117): for every atom j < n with bonds,
+// SBO and its derivatives from j's bonds (:833-884), then for every bond
+// pair (pk, ph > pk) with BO > thb_cut each and product > thb_cutsq and
+// every parameter set of thbp[type k][type j][type h] with |p_val1| > 0.001
+// (:957-1115): angle, penalty and coalition energies and their coefficients,
+// added to Cdbo/Cdbopi/Cdbopi2, CdDelta and fCdDelta in the reference's
+// order.  DEG2RAD with constPI = 3.14159265 (reaxc_defs_sunway.h:55,61);
+// theta by libm acos where the reference calls xacos.  Returns e_ang,
+// e_pen, e_coa in out[0..2].
+int orc_valence_angles(orc_sys *S, double *out) {
+  const int n = S->n;
+  const int nt = (int)S->sbp.size();
+  const double constPI = 3.14159265;
+  auto DEG2RAD = [&](double a) { return a * constPI / 180.0; };
+  const std::vector<double> &gp = S->gp;
+  auto g = [&](size_t k) { return k < gp.size() ? gp[k] : 0.0; };
+  const double p_val6 = g(14), p_val8 = g(33), p_val9 = g(16), p_val10 = g(17);
+  const double p_pen2 = g(19), p_pen3 = g(20), p_pen4 = g(21);
+  const double p_coa2 = g(2), p_coa3 = g(38), p_coa4 = g(30);
+  const double thb_cut = S->thb_cut, thb_cutsq = S->thb_cutsq;
+  double *fc = S->fCdDelta.data();
+  double e_ang_t = 0.0, e_pen_t = 0.0, e_coa_t = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const int type_j = S->packed[j].type;
+    const int start_j = S->bindex[j], end_j = S->bend[j];
+    if (type_j < 0 || end_j <= start_j) continue;
+    const double Dboc_j = S->bo_dboc[2 * (size_t)j + 1];
+    const double p_val3 = S->sbp[type_j].p_val3, p_val5 = S->sbp[type_j].p_val5;
+    double SBOp = 0, prod_SBO = 1;
+    for (int t = start_j; t < end_j; ++t) {
+      SBOp += S->BOpi[2 * (size_t)t] + S->BOpi[2 * (size_t)t + 1];
+      double temp = SQR(S->BO[t]);
+      temp *= temp;
+      temp *= temp;
+      prod_SBO *= std::exp(-temp);
+    }
+    double vlpadj, dSBO2;
+    if (S->vlpex[j] >= 0) {
+      vlpadj = 0;
+      dSBO2 = prod_SBO - 1;
+    } else {
+      vlpadj = S->nlp[j];
+      dSBO2 = (prod_SBO - 1) * (1 - p_val8 * S->dDelta_lp[j]);
+    }
+    const double SBO = SBOp + (1 - prod_SBO) * (-Dboc_j - p_val8 * vlpadj);
+    const double dSBO1 = -8 * prod_SBO * (Dboc_j + p_val8 * vlpadj);
+    double SBO2, CSBO2;
+    if (SBO <= 0) {
+      SBO2 = 0; CSBO2 = 0;
+    } else if (SBO > 0 && SBO <= 1) {
+      SBO2 = std::pow(SBO, p_val9); CSBO2 = p_val9 * std::pow(SBO, p_val9 - 1);
+    } else if (SBO > 1 && SBO < 2) {
+      SBO2 = 2 - std::pow(2 - SBO, p_val9); CSBO2 = p_val9 * std::pow(2 - SBO, p_val9 - 1);
+    } else {
+      SBO2 = 2; CSBO2 = 0;
+    }
+    const double expval6 = std::exp(p_val6 * Dboc_j);
+    for (int pk = start_j; pk < end_j; ++pk) {
+      const swr_bond_t &bjk = S->bonds[pk];
+      const int k = bjk.nbr;
+      const double BOA_jk = S->BO[pk] - thb_cut;
+      if (!(BOA_jk > 0.0)) continue;
+      if (S->bend[k] <= S->bindex[k]) continue;
+      const int type_k = S->packed[k].type;
+      for (int ph = start_j; ph < end_j; ++ph) {
+        if (ph == pk) continue;
+        const swr_bond_t &bhj = S->bonds[ph];
+        const double BOA_hj = S->BO[ph] - thb_cut;
+        const int h = bhj.nbr;
+        const int type_h = S->packed[h].type;
+        // Calculate_Theta / Calculate_dCos_Theta with the j-k bond first
+        const double dot = bjk.dvec[0] * bhj.dvec[0] + bjk.dvec[1] * bhj.dvec[1] + bjk.dvec[2] * bhj.dvec[2];
+        double cos_theta = dot / (bjk.d * bhj.d);
+        if (cos_theta > 1.) cos_theta = 1.0;
+        if (cos_theta < -1.) cos_theta = -1.0;
+        const double theta = std::acos(cos_theta);
+        const double sqr_d_ji = SQR(bjk.d), sqr_d_jk = SQR(bhj.d);
+        const double inv_dists = 1.0 / (bjk.d * bhj.d);
+        const double Cdot_inv3 = dot * std::pow(inv_dists, 3.0);
+        double dcos_di[3], dcos_dj[3], dcos_dk[3];
+        for (int c = 0; c < 3; ++c) {
+          dcos_di[c] = bhj.dvec[c] * inv_dists - Cdot_inv3 * sqr_d_jk * bjk.dvec[c];
+          dcos_dj[c] = -(bhj.dvec[c] + bjk.dvec[c]) * inv_dists +
+                       Cdot_inv3 * (sqr_d_jk * bjk.dvec[c] + sqr_d_ji * bhj.dvec[c]);
+          dcos_dk[c] = bjk.dvec[c] * inv_dists - Cdot_inv3 * sqr_d_ji * bhj.dvec[c];
+        }
+        double sin_theta = std::sin(theta);
+        if (sin_theta < 1.0e-5) sin_theta = 1.0e-5;
+        if (!(ph > pk && S->BO[pk] > thb_cut && S->BO[ph] > thb_cut && S->BO[pk] * S->BO[ph] > thb_cutsq))
+          continue;
+        if (type_k < 0 || type_h < 0) continue;
+        const size_t tri = ((size_t)type_k * nt + type_j) * nt + type_h;
+        for (int cnt = 0; cnt < S->thb_cnt[tri] && cnt < MAX_3BODY_PARAM; ++cnt) {
+          const THBP &q = S->thbp[tri * MAX_3BODY_PARAM + cnt];
+          if (!(std::fabs(q.p_val1) > 0.001)) continue;
+          const double p_val1 = q.p_val1, p_val2 = q.p_val2, p_val4 = q.p_val4, p_val7 = q.p_val7;
+          const double theta_00 = q.theta_00;
+          const double exp3jk = std::exp(-p_val3 * std::pow(BOA_jk, p_val4));
+          const double f7_jk = 1.0 - exp3jk;
+          const double Cf7jk = p_val3 * p_val4 * std::pow(BOA_jk, p_val4 - 1.0) * exp3jk;
+          const double exp3hj = std::exp(-p_val3 * std::pow(BOA_hj, p_val4));
+          const double f7_hj = 1.0 - exp3hj;
+          const double Cf7hj = p_val3 * p_val4 * std::pow(BOA_hj, p_val4 - 1.0) * exp3hj;
+          const double expval7 = std::exp(-p_val7 * Dboc_j);
+          const double trm8 = 1.0 + expval6 + expval7;
+          const double f8_Dj = p_val5 - ((p_val5 - 1.0) * (2.0 + expval6) / trm8);
+          const double Cf8j = ((1.0 - p_val5) / SQR(trm8)) *
+                              (p_val6 * expval6 * trm8 - (2.0 + expval6) * (p_val6 * expval6 - p_val7 * expval7));
+          const double theta_0 = DEG2RAD(180.0 - theta_00 * (1.0 - std::exp(-p_val10 * (2.0 - SBO2))));
+          const double expval2theta = std::exp(-p_val2 * SQR(theta_0 - theta));
+          const double expval12theta = p_val1 >= 0 ? p_val1 * (1.0 - expval2theta) : p_val1 * -expval2theta;
+          const double CEval1 = Cf7jk * f7_hj * f8_Dj * expval12theta;
+          const double CEval2 = Cf7hj * f7_jk * f8_Dj * expval12theta;
+          const double CEval3 = Cf8j * f7_hj * f7_jk * expval12theta;
+          const double CEval4 = -2.0 * p_val1 * p_val2 * f7_jk * f7_hj * f8_Dj * expval2theta * (theta_0 - theta);
+          const double Ctheta_0 = p_val10 * DEG2RAD(theta_00) * std::exp(-p_val10 * (2.0 - SBO2));
+          const double CEval5 = -CEval4 * Ctheta_0 * CSBO2;
+          const double CEval6 = CEval5 * dSBO1;
+          const double CEval7 = CEval5 * dSBO2;
+          const double CEval8 = -CEval4 / sin_theta;
+          const double e_ang = f7_jk * f7_hj * f8_Dj * expval12theta;
+          e_ang_t += e_ang;
+          const double p_pen1 = q.p_pen1;
+          const double exp_pen2jk = std::exp(-p_pen2 * SQR(BOA_jk - 2.0));
+          const double exp_pen2hj = std::exp(-p_pen2 * SQR(BOA_hj - 2.0));
+          const double exp_pen3 = std::exp(-p_pen3 * S->Delta[j]);
+          const double exp_pen4 = std::exp(p_pen4 * S->Delta[j]);
+          const double trm_pen34 = 1.0 + exp_pen3 + exp_pen4;
+          const double f9_Dj = (2.0 + exp_pen3) / trm_pen34;
+          const double Cf9j = (-p_pen3 * exp_pen3 * trm_pen34 - (2.0 + exp_pen3) * (-p_pen3 * exp_pen3 + p_pen4 * exp_pen4)) /
+                              SQR(trm_pen34);
+          const double e_pen = p_pen1 * f9_Dj * exp_pen2jk * exp_pen2hj;
+          e_pen_t += e_pen;
+          const double CEpen1 = e_pen * Cf9j / f9_Dj;
+          const double temp = -2.0 * p_pen2 * e_pen;
+          const double CEpen2 = temp * (BOA_jk - 2.0), CEpen3 = temp * (BOA_hj - 2.0);
+          const double p_coa1 = q.p_coa1;
+          const double exp_coa2 = std::exp(p_coa2 * S->Delta_val[j]);
+          const double bk0 = S->bo_dboc[2 * (size_t)k], bh0 = S->bo_dboc[2 * (size_t)h];
+          const double e_coa = p_coa1 / (1. + exp_coa2) * std::exp(-p_coa3 * SQR(bk0 - BOA_jk)) *
+                               std::exp(-p_coa3 * SQR(bh0 - BOA_hj)) * std::exp(-p_coa4 * SQR(BOA_jk - 1.5)) *
+                               std::exp(-p_coa4 * SQR(BOA_hj - 1.5));
+          e_coa_t += e_coa;
+          const double CEcoa1 = -2 * p_coa4 * (BOA_jk - 1.5) * e_coa;
+          const double CEcoa2 = -2 * p_coa4 * (BOA_hj - 1.5) * e_coa;
+          const double CEcoa3 = -p_coa2 * exp_coa2 * e_coa / (1 + exp_coa2);
+          const double CEcoa4 = -2 * p_coa3 * (bk0 - BOA_jk) * e_coa;
+          const double CEcoa5 = -2 * p_coa3 * (bh0 - BOA_hj) * e_coa;
+          S->Cdbo[pk] += (CEval1 + CEpen2 + (CEcoa1 - CEcoa4));
+          S->Cdbo[ph] += (CEval2 + CEpen3 + (CEcoa2 - CEcoa5));
+          fc[4 * (size_t)j + 3] += ((CEval3 + CEval7) + CEpen1 + CEcoa3);
+          fc[4 * (size_t)k + 3] += CEcoa4;
+          fc[4 * (size_t)h + 3] += CEcoa5;
+          for (int t = start_j; t < end_j; ++t) {
+            const double bo = S->BO[t];
+            const double c3 = bo * bo * bo;
+            S->Cdbo[t] += CEval6 * (c3 * c3 * bo);
+            S->Cdbopi[t] += CEval5;
+            S->Cdbopi2[t] += CEval5;
+          }
+          for (int c = 0; c < 3; ++c) {
+            fc[4 * (size_t)k + c] += CEval8 * dcos_di[c];
+            fc[4 * (size_t)j + c] += CEval8 * dcos_dj[c];
+            fc[4 * (size_t)h + c] += CEval8 * dcos_dk[c];
+          }
+        }
+      }
+    }
+  }
+  out[0] = e_ang_t;
+  out[1] = e_pen_t;
+  out[2] = e_coa_t;
+  return 0;
 }
 
 // Add_All_dBond_to_Forces_C (reaxc_forces_sw64.c:417-589), its host loop

@@ -516,6 +516,223 @@ __global__ void __launch_bounds__(BT) k_hb_fold(int N, unsigned long long *__res
   }
 }
 
+// ---------------------------------------------------------------- valence angles
+// The valence part of Merge_Torsion_Valence_Angles (host form
+// reaxc_torsion_angles_sw64.c:812-1117): for every local atom j and pair of
+// its bonds j-k (pk) and j-h (ph > pk) with BO above thb_cut and their
+// product above thb_cutsq, and every parameter set of (type k, type j,
+// type h) with |p_val1| > 0.001: the angle, penalty and coalition energies;
+// their coefficients into Cdbo of both bonds and of all of j's bonds
+// (CEval6 BO^7), Cdbopi/Cdbopi2 of j's bonds (CEval5), CdDelta of j, k and
+// h, and the angle derivative into fCdDelta[.][0..2] of k, j and h.  One
+// thread per j writes j's own bond slots and row; the terms of k and h go
+// into 64-bit fixed-point accumulators by integer atomics (order-independent
+// sums), folded into fCdDelta by k_va_fold.
+constexpr double DEG2RAD_PI = 3.14159265;   // constPI, reaxc_defs_sunway.h:61 (DEG2RAD :55)
+
+struct VaArgs {
+  const DevParams *P;
+  const swr_atom_pack_t *atoms;
+  const int *bidx, *bend;
+  const swr_bond_t *bonds;
+  const double *BO, *BOpi, *bo_dboc;
+  const double *Delta, *Delta_val, *vlpex, *nlp, *dDelta_lp;
+  double *Cdbo, *Cdbopi, *Cdbopi2, *fcd, *part;
+  size_t stride;
+  unsigned long long *acc;   // [N][4]
+};
+
+__global__ void __launch_bounds__(BT) k_valence(int n, VaArgs A) {
+  const int j = blockIdx.x * BT + threadIdx.x;
+  if (j >= n) return;
+  const DevParams *__restrict__ P = A.P;
+  const int tj = A.atoms[j].type;
+  const int b0 = A.bidx[j], b1 = A.bend[j];
+  double e_ang = 0.0, e_pen = 0.0, e_coa = 0.0;
+  if (tj >= 0 && b1 > b0) {
+    const double thb_cut = P->thb_cut, thb_cutsq = P->thb_cutsq;
+    const double Dboc_j = A.bo_dboc[2 * (size_t)j + 1];
+    const double p_val3 = P->p_val3[tj], p_val5 = P->p_val5[tj];
+    const double p_val6 = P->p_val6, p_val8 = P->p_val8, p_val9 = P->p_val9, p_val10 = P->p_val10;
+    double SBOp = 0.0, prod_SBO = 1.0;
+    for (int t = b0; t < b1; ++t) {
+      SBOp += A.BOpi[2 * (size_t)t] + A.BOpi[2 * (size_t)t + 1];
+      double b2 = A.BO[t] * A.BO[t];
+      b2 *= b2;
+      b2 *= b2;
+      prod_SBO *= exp(-b2);
+    }
+    double vlpadj, dSBO2;
+    if (A.vlpex[j] >= 0) {
+      vlpadj = 0.0;
+      dSBO2 = prod_SBO - 1.0;
+    } else {
+      vlpadj = A.nlp[j];
+      dSBO2 = (prod_SBO - 1.0) * (1.0 - p_val8 * A.dDelta_lp[j]);
+    }
+    const double SBO = SBOp + (1.0 - prod_SBO) * (-Dboc_j - p_val8 * vlpadj);
+    const double dSBO1 = -8.0 * prod_SBO * (Dboc_j + p_val8 * vlpadj);
+    double SBO2, CSBO2;
+    if (SBO <= 0.0) {
+      SBO2 = 0.0; CSBO2 = 0.0;
+    } else if (SBO <= 1.0) {
+      SBO2 = pow(SBO, p_val9); CSBO2 = p_val9 * pow(SBO, p_val9 - 1.0);
+    } else if (SBO < 2.0) {
+      SBO2 = 2.0 - pow(2.0 - SBO, p_val9); CSBO2 = p_val9 * pow(2.0 - SBO, p_val9 - 1.0);
+    } else {
+      SBO2 = 2.0; CSBO2 = 0.0;
+    }
+    const double expval6 = exp(p_val6 * Dboc_j);
+    const double Delta_j = A.Delta[j];
+    const double exp_pen3 = exp(-P->p_pen3 * Delta_j), exp_pen4 = exp(P->p_pen4 * Delta_j);
+    const double trm_pen34 = 1.0 + exp_pen3 + exp_pen4;
+    const double f9_Dj = (2.0 + exp_pen3) / trm_pen34;
+    const double Cf9j = (-P->p_pen3 * exp_pen3 * trm_pen34 - (2.0 + exp_pen3) * (-P->p_pen3 * exp_pen3 + P->p_pen4 * exp_pen4)) /
+                        (trm_pen34 * trm_pen34);
+    const double exp_coa2 = exp(P->p_coa2 * A.Delta_val[j]);
+    double ce5 = 0.0, ce6 = 0.0, cdj = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    for (int pk = b0; pk < b1; ++pk) {
+      const double BO_jk = A.BO[pk];
+      if (!(BO_jk > thb_cut)) continue;
+      const swr_bond_t bjk = A.bonds[pk];
+      const int k = bjk.nbr;
+      if (A.bend[k] <= A.bidx[k]) continue;
+      const int tk = A.atoms[k].type;
+      const double BOA_jk = BO_jk - thb_cut;
+      for (int ph = pk + 1; ph < b1; ++ph) {
+        const double BO_hj = A.BO[ph];
+        if (!(BO_hj > thb_cut) || !(BO_jk * BO_hj > thb_cutsq)) continue;
+        const int th = A.atoms[A.bonds[ph].nbr].type;
+        if (tk < 0 || th < 0) continue;
+        const int tri = (tk * SWR_MAX_ELEM + tj) * SWR_MAX_ELEM + th;
+        const int nset = P->thb_n[tri];
+        if (nset == 0) continue;
+        const swr_bond_t bhj = A.bonds[ph];
+        const int h = bhj.nbr;
+        const double BOA_hj = BO_hj - thb_cut;
+        // Calculate_Theta / Calculate_dCos_Theta (:1510-1547) with the j-k
+        // bond first: di belongs to k, dk to h
+        const double dot = bjk.dvec[0] * bhj.dvec[0] + bjk.dvec[1] * bhj.dvec[1] + bjk.dvec[2] * bhj.dvec[2];
+        double cos_t = dot / (bjk.d * bhj.d);
+        if (cos_t > 1.0) cos_t = 1.0;
+        if (cos_t < -1.0) cos_t = -1.0;
+        const double theta = acos(cos_t);
+        double sin_t = sin(theta);
+        if (sin_t < 1.0e-5) sin_t = 1.0e-5;
+        const double sq_jk = bjk.d * bjk.d, sq_hj = bhj.d * bhj.d;
+        const double inv_d = 1.0 / (bjk.d * bhj.d);
+        const double cinv3 = dot * (inv_d * inv_d * inv_d);
+        const double bo_k0 = A.bo_dboc[2 * (size_t)k], bo_h0 = A.bo_dboc[2 * (size_t)h];
+        double ck[4] = {0.0, 0.0, 0.0, 0.0}, chh[4] = {0.0, 0.0, 0.0, 0.0};
+        double cdbo_jk = 0.0, cdbo_hj = 0.0;
+        for (int c = 0; c < nset; ++c) {
+          const DevThb &q = P->thb[tri][c];
+          const double p_val1 = q.p_val1, p_val2 = q.p_val2, p_val4 = q.p_val4, p_val7 = q.p_val7;
+          const double exp3jk = exp(-p_val3 * pow(BOA_jk, p_val4));
+          const double f7_jk = 1.0 - exp3jk;
+          const double Cf7jk = p_val3 * p_val4 * pow(BOA_jk, p_val4 - 1.0) * exp3jk;
+          const double exp3hj = exp(-p_val3 * pow(BOA_hj, p_val4));
+          const double f7_hj = 1.0 - exp3hj;
+          const double Cf7hj = p_val3 * p_val4 * pow(BOA_hj, p_val4 - 1.0) * exp3hj;
+          const double expval7 = exp(-p_val7 * Dboc_j);
+          const double trm8 = 1.0 + expval6 + expval7;
+          const double f8_Dj = p_val5 - ((p_val5 - 1.0) * (2.0 + expval6) / trm8);
+          const double Cf8j = ((1.0 - p_val5) / (trm8 * trm8)) *
+                              (p_val6 * expval6 * trm8 - (2.0 + expval6) * (p_val6 * expval6 - p_val7 * expval7));
+          const double exp10 = exp(-p_val10 * (2.0 - SBO2));
+          const double theta_0 = (180.0 - q.theta_00 * (1.0 - exp10)) * DEG2RAD_PI / 180.0;
+          const double expval2theta = exp(-p_val2 * (theta_0 - theta) * (theta_0 - theta));
+          const double expval12theta = p_val1 >= 0 ? p_val1 * (1.0 - expval2theta) : p_val1 * -expval2theta;
+          const double CEval1 = Cf7jk * f7_hj * f8_Dj * expval12theta;
+          const double CEval2 = Cf7hj * f7_jk * f8_Dj * expval12theta;
+          const double CEval3 = Cf8j * f7_hj * f7_jk * expval12theta;
+          const double CEval4 = -2.0 * p_val1 * p_val2 * f7_jk * f7_hj * f8_Dj * expval2theta * (theta_0 - theta);
+          const double Ctheta_0 = p_val10 * (q.theta_00 * DEG2RAD_PI / 180.0) * exp10;
+          const double CEval5 = -CEval4 * Ctheta_0 * CSBO2;
+          const double CEval8 = -CEval4 / sin_t;
+          e_ang += f7_jk * f7_hj * f8_Dj * expval12theta;
+          // penalty
+          const double exp_pen2jk = exp(-P->p_pen2 * (BOA_jk - 2.0) * (BOA_jk - 2.0));
+          const double exp_pen2hj = exp(-P->p_pen2 * (BOA_hj - 2.0) * (BOA_hj - 2.0));
+          const double ep = q.p_pen1 * f9_Dj * exp_pen2jk * exp_pen2hj;
+          e_pen += ep;
+          const double CEpen1 = ep * Cf9j / f9_Dj;
+          const double tp = -2.0 * P->p_pen2 * ep;
+          const double CEpen2 = tp * (BOA_jk - 2.0), CEpen3 = tp * (BOA_hj - 2.0);
+          // coalition
+          const double ec = q.p_coa1 / (1. + exp_coa2) * exp(-P->p_coa3 * (bo_k0 - BOA_jk) * (bo_k0 - BOA_jk)) *
+                            exp(-P->p_coa3 * (bo_h0 - BOA_hj) * (bo_h0 - BOA_hj)) *
+                            exp(-P->p_coa4 * (BOA_jk - 1.5) * (BOA_jk - 1.5)) *
+                            exp(-P->p_coa4 * (BOA_hj - 1.5) * (BOA_hj - 1.5));
+          e_coa += ec;
+          const double CEcoa1 = -2 * P->p_coa4 * (BOA_jk - 1.5) * ec;
+          const double CEcoa2 = -2 * P->p_coa4 * (BOA_hj - 1.5) * ec;
+          const double CEcoa3 = -P->p_coa2 * exp_coa2 * ec / (1 + exp_coa2);
+          const double CEcoa4 = -2 * P->p_coa3 * (bo_k0 - BOA_jk) * ec;
+          const double CEcoa5 = -2 * P->p_coa3 * (bo_h0 - BOA_hj) * ec;
+          cdbo_jk += CEval1 + CEpen2 + (CEcoa1 - CEcoa4);
+          cdbo_hj += CEval2 + CEpen3 + (CEcoa2 - CEcoa5);
+          cdj += (CEval3 + CEval5 * dSBO2) + CEpen1 + CEcoa3;
+          ck[3] += CEcoa4;
+          chh[3] += CEcoa5;
+          ce5 += CEval5;
+          ce6 += CEval5 * dSBO1;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const double di = bhj.dvec[d] * inv_d - cinv3 * sq_hj * bjk.dvec[d];
+            const double dj = -(bhj.dvec[d] + bjk.dvec[d]) * inv_d + cinv3 * (sq_hj * bjk.dvec[d] + sq_jk * bhj.dvec[d]);
+            const double dk = bjk.dvec[d] * inv_d - cinv3 * sq_jk * bhj.dvec[d];
+            ck[d] += CEval8 * di;
+            chh[d] += CEval8 * dk;
+            if (d == 0) g0 += CEval8 * dj;
+            if (d == 1) g1 += CEval8 * dj;
+            if (d == 2) g2 += CEval8 * dj;
+          }
+        }
+        A.Cdbo[pk] += cdbo_jk;
+        A.Cdbo[ph] += cdbo_hj;
+        unsigned long long *ak = A.acc + 4 * (size_t)k, *ah = A.acc + 4 * (size_t)h;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          fix_add(ak + d, ck[d]);
+          fix_add(ah + d, chh[d]);
+        }
+      }
+    }
+    if (ce5 != 0.0 || ce6 != 0.0)
+      for (int t = b0; t < b1; ++t) {
+        const double bo = A.BO[t];
+        const double bo3 = bo * bo * bo;
+        A.Cdbo[t] += ce6 * (bo3 * bo3 * bo);
+        A.Cdbopi[t] += ce5;
+        A.Cdbopi2[t] += ce5;
+      }
+    A.fcd[4 * (size_t)j] += g0;
+    A.fcd[4 * (size_t)j + 1] += g1;
+    A.fcd[4 * (size_t)j + 2] += g2;
+    A.fcd[4 * (size_t)j + 3] += cdj;
+  }
+  A.part[j] = e_ang;
+  A.part[A.stride + j] = e_pen;
+  A.part[2 * A.stride + j] = e_coa;
+}
+
+// fCdDelta (all four columns) += the fixed-point terms; the accumulators are
+// left at zero.
+__global__ void __launch_bounds__(BT) k_va_fold(int N, unsigned long long *__restrict__ acc,
+                                                double *__restrict__ fcd) {
+  const int a = blockIdx.x * BT + threadIdx.x;
+  if (a >= N) return;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const long long v = (long long)acc[4 * (size_t)a + c];
+    if (v != 0) {
+      fcd[4 * (size_t)a + c] += (double)v * (1.0 / HB_FIX);
+      acc[4 * (size_t)a + c] = 0ull;
+    }
+  }
+}
+
 }  // namespace
 
 int swr_atom_energy_run(swr_system *S) {
@@ -627,6 +844,54 @@ int swr_hydrogen_bonds_run(swr_system *S) {
   hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(1024), 0, S->stream, n, (const double *)S->hb_part, (size_t)0,
                      S->hb_result);
   swr_kt_end(S, KT_HBONDS);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_valence_angles_run(swr_system *S) {
+  const int n = S->n, N = S->N;
+  if (!S->bonds || !S->BO || !S->fcd || N <= 0) {
+    swr_set_error_msg("Valence_Angles needs the bonds and bond orders of the step (swr_bo first)");
+    return SWR_ERR_STATE;
+  }
+  int rc;
+  const size_t stride = (size_t)std::max(n, 1);
+  const size_t old_acc = S->cap_va_acc;
+  if ((rc = swr_reserve(&S->va_part, S->cap_va_part, 3 * stride)) ||
+      (rc = swr_reserve(&S->va_result, S->cap_va_result, 4)) ||
+      (rc = swr_reserve(&S->va_acc, S->cap_va_acc, 4 * (size_t)N)))
+    return rc;
+  if (S->cap_va_acc != old_acc)
+    HIP_CHECK(hipMemsetAsync(S->va_acc, 0, sizeof(unsigned long long) * S->cap_va_acc, S->stream));
+  S->va_stride = stride;
+  VaArgs A;
+  A.P = S->dp;
+  A.atoms = S->packed;
+  A.bidx = S->bond_index;
+  A.bend = S->bond_end;
+  A.bonds = S->bonds;
+  A.BO = S->BO;
+  A.BOpi = S->BOpi;
+  A.bo_dboc = S->bo_dboc;
+  const double *ws = S->ws;
+  A.Delta = ws + (size_t)WS_DELTA * S->Ncap;
+  A.Delta_val = ws + (size_t)WS_DELTA_VAL * S->Ncap;
+  A.vlpex = ws + (size_t)WS_VLPEX * S->Ncap;
+  A.nlp = ws + (size_t)WS_NLP * S->Ncap;
+  A.dDelta_lp = ws + (size_t)WS_DDELTA_LP * S->Ncap;
+  A.Cdbo = S->Cdbo;
+  A.Cdbopi = S->Cdbopi;
+  A.Cdbopi2 = S->Cdbopi2;
+  A.fcd = S->fcd;
+  A.part = S->va_part;
+  A.stride = stride;
+  A.acc = S->va_acc;
+  swr_kt_begin(S, KT_VALENCE);
+  if (n > 0) hipLaunchKernelGGL(k_valence, dim3(div_up(n, BT)), dim3(BT), 0, S->stream, n, A);
+  hipLaunchKernelGGL(k_va_fold, dim3(div_up(N, BT)), dim3(BT), 0, S->stream, N, S->va_acc, S->fcd);
+  hipLaunchKernelGGL(k_sum_rows, dim3(3), dim3(1024), 0, S->stream, n, (const double *)S->va_part, stride,
+                     S->va_result);
+  swr_kt_end(S, KT_VALENCE);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }

@@ -115,7 +115,32 @@ void fill_params(swr_system *S, const std::vector<int> &map) {
     h.p_ovun8 = g(9); h.gp3 = g(3); h.gp4 = g(4); h.gp7 = g(7); h.gp10 = g(10);
     h.gp37 = (int)g(37);
     h.enobonds = S->ctl.enobonds;
+    h.p_val6 = g(14); h.p_val8 = g(33); h.p_val9 = g(16); h.p_val10 = g(17);
+    h.p_pen2 = g(19); h.p_pen3 = g(20); h.p_pen4 = g(21);
+    h.p_coa2 = g(2); h.p_coa3 = g(38); h.p_coa4 = g(30);
+    h.thb_cut = S->ctl.thb_cut; h.thb_cutsq = S->ctl.thb_cutsq;
   }
+  for (int e = 0; e < ff.ne() && e < SWR_MAX_ELEM; e++) {
+    h.p_val3[e] = ff.atom[e].p_val3;
+    h.p_val5[e] = ff.atom[e].p_val5;
+  }
+  // valence-angle sets the energy uses (Merge_Torsion_Valence_Angles skips
+  // |p_val1| <= 0.001, reaxc_torsion_angles_sw64.c:960)
+  for (int a = 0; a < ff.ne() && a < SWR_MAX_ELEM; a++)
+    for (int b = 0; b < ff.ne() && b < SWR_MAX_ELEM; b++)
+      for (int c = 0; c < ff.ne() && c < SWR_MAX_ELEM; c++) {
+        const int d = (a * SWR_MAX_ELEM + b) * SWR_MAX_ELEM + c;
+        h.thb_n[d] = 0;
+        if (ff.thb_cnt.empty()) continue;
+        const size_t t = ff.tri(a, b, c);
+        for (int k = 0; k < ff.thb_cnt[t] && k < SWR_FF_MAX_THB; k++) {
+          const FfAngle &f = ff.thb[t * SWR_FF_MAX_THB + k];
+          if (!(std::fabs(f.p_val1) > 0.001)) continue;
+          DevThb &o = h.thb[d][h.thb_n[d]++];
+          o.theta_00 = f.theta_00; o.p_val1 = f.p_val1; o.p_val2 = f.p_val2; o.p_coa1 = f.p_coa1;
+          o.p_val7 = f.p_val7; o.p_pen1 = f.p_pen1; o.p_val4 = f.p_val4;
+        }
+      }
   for (int a = 0; a < ff.ne() && a < SWR_MAX_ELEM; a++)
     for (int b = 0; b < ff.ne() && b < SWR_MAX_ELEM; b++)
       for (int c = 0; c < ff.ne() && c < SWR_MAX_ELEM; c++) {
@@ -251,6 +276,7 @@ void swr_control_default(swr_control_t *c) {
   c->bond_cut = 5.0;    // pair_reaxc_sunway.cpp:216
   c->hbond_cut = 7.5;   // :217
   c->thb_cut = 0.001;   // :218
+  c->thb_cutsq = 0.00001;   // :219
   c->safezone = 1.2;    // SAFE_ZONE
   c->saferzone = 1.4;   // SAFER_ZONE
   c->mincap = 50;       // MIN_CAP
@@ -337,7 +363,7 @@ int swr_destroy(swr_system *S) {
                   S->Hindex, S->bond_index, S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, S->Cdbo, S->Cdbopi,
                   S->Cdbopi2, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,
                   S->fcd, S->nb_partials, S->nb_result, S->nb_eatom, S->be_part, S->be_ce, S->be_bufc, S->be_result, S->hb_part,
-                  S->hb_result, S->hb_acc, S->f, S->gcsr_cnt, S->gcsr_start, S->gcsr_list,
+                  S->hb_result, S->hb_acc, S->va_part, S->va_result, S->va_acc, S->f, S->gcsr_cnt, S->gcsr_start, S->gcsr_list,
                   S->bond_mark2, S->counts, S->ws, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32,
                   S->Hdia_inv, S->b_s, S->st, S->s_hist, S->t_hist, S->cgv, S->cgs, S->partials, S->counters,
                   S->status, S->scan_tmp, S->tmp_i, S->mixw, S->cand, S->cand_count, S->row_sys, S->tile_beg,
@@ -661,6 +687,13 @@ int swr_hydrogen_bonds(swr_system *S) {
   return swr_hydrogen_bonds_run(S);
 }
 
+int swr_valence_angles(swr_system *S) {
+  if (!S) return SWR_ERR_ARG;
+  int rc;
+  if ((rc = swr_quiesce(S))) return rc;
+  return swr_valence_angles_run(S);
+}
+
 int swr_compute_forces(swr_system *S) {
   if (!S) return SWR_ERR_ARG;
   int rc;
@@ -770,6 +803,8 @@ bool lookup(swr_system *S, const std::string &k, View &v) {
   if (k == "hbond_energy") return set(S->hb_result, S->hb_result ? 8 : 0);
   if (k == "hbond_eatom") return set(S->hb_part, S->hb_part ? 8 * n : 0);
   if (k == "f") return set(S->f, S->have_f ? 24 * n : 0);
+  if (k == "valence_energy") return set(S->va_result, S->va_result ? 24 : 0);
+  if (k == "valence_eatom") return set(S->va_part, S->va_part ? 24 * (int64_t)S->va_stride : 0);
   static const char *wsn[WS_COUNT] = {"Deltap", "Deltap_boc", "Delta", "Delta_e", "Delta_val", "vlpex", "nlp",
                                       "nlp_temp", "Clp", "Delta_lp", "dDelta_lp", "Delta_lp_temp", "dDelta_lp_temp"};
   for (int w = 0; w < WS_COUNT; w++)

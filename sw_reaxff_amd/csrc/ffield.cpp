@@ -11,7 +11,7 @@
 //   * pair radii are arithmetic means, p_boc3/4/5 geometric means,
 //     gamma = (g_i g_j)^-1.5, then positive off-diagonal values override
 //     D, r_vdW (x2), alpha, r_s, r_p, r_pp                              (:402-545).
-// Angle and torsion blocks are skipped; the hydrogen-bond block is read
+// The torsion block is skipped; the valence-angle and hydrogen-bond blocks are read
 // (:654-686).
 #include "swr_ffield.h"
 
@@ -158,10 +158,33 @@ std::string swr_read_ffield(const char *path, swr_ffield_t *ff) {
       }
     }
   }
-  // valence-angle and torsion blocks: skipped by their counts; then the
-  // hydrogen-bond block (:654-686): r0_hb = -1 unless listed
+  // valence-angle block (:520-572): a line "a b c" adds one parameter set
+  // to (a,b,c) and to (c,b,a) at the count (a,b,c) had, and counts both up
+  // (so a == c counts twice and leaves a zero set, which p_val1 = 0 skips)
   ff->hb.assign((size_t)ne * ne * ne, FfHbond());
-  for (int blk = 0; blk < 2; blk++) {
+  ff->thb.assign((size_t)ne * ne * ne * SWR_FF_MAX_THB, FfAngle());
+  ff->thb_cnt.assign((size_t)ne * ne * ne, 0);
+  if (!L.next(t) || t.empty()) return "";
+  const int nang = std::atoi(t[0].c_str());
+  for (int k = 0; k < nang; k++) {
+    if (!L.next(t)) return "truncated valence-angle block";
+    if (t.size() < 10) return "bad valence-angle line";
+    const int a = std::atoi(t[0].c_str()) - 1, b = std::atoi(t[1].c_str()) - 1, c = std::atoi(t[2].c_str()) - 1;
+    if (a < 0 || b < 0 || c < 0 || a >= ne || b >= ne || c >= ne) continue;
+    const int cnt = ff->thb_cnt[ff->tri(a, b, c)];
+    if (cnt >= SWR_FF_MAX_THB || ff->thb_cnt[ff->tri(c, b, a)] >= SWR_FF_MAX_THB)
+      return "too many valence-angle parameter sets for one triple";
+    ff->thb_cnt[ff->tri(a, b, c)]++;
+    ff->thb_cnt[ff->tri(c, b, a)]++;
+    FfAngle g;
+    g.theta_00 = num(t, 3); g.p_val1 = num(t, 4); g.p_val2 = num(t, 5); g.p_coa1 = num(t, 6);
+    g.p_val7 = num(t, 7); g.p_pen1 = num(t, 8); g.p_val4 = num(t, 9);
+    ff->thb[ff->tri(a, b, c) * SWR_FF_MAX_THB + cnt] = g;
+    ff->thb[ff->tri(c, b, a) * SWR_FF_MAX_THB + cnt] = g;
+  }
+  // torsion block: skipped by its count; then the hydrogen-bond block
+  // (:654-686): r0_hb = -1 unless listed
+  {
     if (!L.next(t) || t.empty()) return "";
     const int l = std::atoi(t[0].c_str());
     for (int k = 0; k < l; k++)

@@ -28,8 +28,16 @@ struct FfHbond {
   double r0_hb = -1.0, p_hb1 = 0, p_hb2 = 0, p_hb3 = 0;   // r0_hb <= 0: no hydrogen bond
 };
 
+// One valence-angle parameter set (reaxc_ffield_sunway.cpp:526-572).
+struct FfAngle {
+  double theta_00 = 0, p_val1 = 0, p_val2 = 0, p_coa1 = 0, p_val7 = 0, p_pen1 = 0, p_val4 = 0;
+};
+constexpr int SWR_FF_MAX_THB = 5;   // parameter sets per (i, j, k) triple, REAX_MAX_3BODY_PARAM
+
 struct swr_ffield_t {
   std::vector<double> gp;           // general parameters l[0..]
+  std::vector<FfAngle> thb;         // [ne*ne*ne][SWR_FF_MAX_THB] (end atom i, centre j, end atom k)
+  std::vector<int> thb_cnt;         // [ne*ne*ne]
   std::vector<FfHbond> hb;          // [ne*ne*ne] (acceptor i, hydrogen j, acceptor k)
   std::vector<FfAtom> atom;
   std::vector<FfPair> pair;         // [ne*ne]
@@ -39,6 +47,7 @@ struct swr_ffield_t {
   const FfPair &tb(int i, int j) const { return pair[(size_t)i * atom.size() + j]; }
   FfPair &tb(int i, int j) { return pair[(size_t)i * atom.size() + j]; }
   const FfHbond &hbp(int i, int j, int k) const { return hb[((size_t)i * atom.size() + j) * atom.size() + k]; }
+  size_t tri(int i, int j, int k) const { return ((size_t)i * atom.size() + j) * atom.size() + k; }
 };
 
 // Returns an empty string on success, else the reason.

@@ -64,6 +64,13 @@ struct DevHb {
   double r0_hb, p_hb1, p_hb2, p_hb3;
 };
 
+// Valence-angle parameter sets of one (end, centre, end) element triple,
+// only those with |p_val1| > 0.001 (the ones the energy uses), in file order.
+constexpr int SWR_MAX_THB = 5;
+struct DevThb {
+  double theta_00, p_val1, p_val2, p_coa1, p_val7, p_pen1, p_val4;
+};
+
 // Bond and atom energies (Merge_Bonds_Atom_Energy_C_New), element-pair indexed.
 struct DevBe {
   double De_s, De_p, De_pp, p_be1, p_be2, p_ovun1;
@@ -107,6 +114,12 @@ struct DevParams {
   double p_lp3, p_ovun3, p_ovun4, p_ovun6, p_ovun7, p_ovun8, gp3, gp4, gp7, gp10;
   int gp37, enobonds;
   DevHb hb[SWR_MAX_ELEM * SWR_MAX_ELEM * SWR_MAX_ELEM];
+  // Valence angles: element p_val3/p_val5, general l[2,14,16,17,19,20,21,
+  // 30,33,38], control thb_cut/thb_cutsq
+  double p_val3[SWR_MAX_ELEM], p_val5[SWR_MAX_ELEM];
+  double p_val6, p_val8, p_val9, p_val10, p_pen2, p_pen3, p_pen4, p_coa2, p_coa3, p_coa4, thb_cut, thb_cutsq;
+  int thb_n[SWR_MAX_ELEM * SWR_MAX_ELEM * SWR_MAX_ELEM];
+  DevThb thb[SWR_MAX_ELEM * SWR_MAX_ELEM * SWR_MAX_ELEM][SWR_MAX_THB];
 };
 
 // Small block of device-written results the host reads back asynchronously.
@@ -148,7 +161,7 @@ struct CgScalars {
 // Kernel timer slots.
 enum { KT_VERLET = 0, KT_FAR, KT_BUILD_H, KT_CG_ITER, KT_CG_SETUP, KT_BO, KT_VALIDATE, KT_SPMV, KT_RESET,
        KT_QEQ_Q, KT_BINS, KT_MIXED_INNER, KT_MIXED_OUTER, KT_BONDS, KT_CG_SCALARS, KT_MARK, KT_CG_UPDATE, KT_NONBONDED, KT_ATOM_ENERGY,
-       KT_DBOND, KT_HBONDS, KT_COUNT };
+       KT_DBOND, KT_HBONDS, KT_VALENCE, KT_COUNT };
 
 struct swr_system {
   // ---- host-side configuration
@@ -225,6 +238,11 @@ struct swr_system {
   double *hb_result = nullptr;     // e_hb of the last Hydrogen_Bonds
   unsigned long long *hb_acc = nullptr;   // [N][3] fixed-point hydrogen-bond terms (zero between calls)
   size_t cap_hb_part = 0, cap_hb_result = 0, cap_hb_acc = 0;
+  // valence angles: per-atom energies [3][va_stride], totals, fixed-point
+  // terms of the end atoms [N][4] (zero between calls)
+  double *va_part = nullptr, *va_result = nullptr;
+  unsigned long long *va_acc = nullptr;
+  size_t cap_va_part = 0, cap_va_result = 0, cap_va_acc = 0, va_stride = 0;
   // forces on local atoms (ghost rows of fCdDelta summed onto owners) and the
   // ghost-per-owner CSR that sums them in a fixed order
   double *f = nullptr;                     // [n][3]
@@ -348,6 +366,7 @@ int swr_nonbonded_run(swr_system *S);
 int swr_atom_energy_run(swr_system *S);
 int swr_dbond_forces_run(swr_system *S);
 int swr_hydrogen_bonds_run(swr_system *S);
+int swr_valence_angles_run(swr_system *S);
 int swr_forces_run(swr_system *S);
 int swr_nve_run(swr_system *S, int stage, double dtv, double dtf);
 int swr_quiesce(swr_system *S);

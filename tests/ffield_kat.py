@@ -74,11 +74,24 @@ class FField:
             vals = [float(v) for v in t[2:8]]
             self.offdiag[(i, j)] = self.offdiag[(j, i)] = dict(zip(["D", "r_vdw", "alpha", "r_s", "r_p", "r_pp"], vals))
             k += 1
-        # valence-angle and torsion blocks (a count line, then that many
-        # lines each), then the hydrogen bonds: acceptor, H, acceptor (1-based)
+        # valence angles: end, centre, end (1-based) and theta_00, p_val1,
+        # p_val2, p_coa1, p_val7, p_pen1, p_val4 (each line also serves the
+        # reversed triple); the torsion block (a count line, then that many
+        # lines); then the hydrogen bonds: acceptor, H, acceptor (1-based)
         # and r0_hb, p_hb1, p_hb2, p_hb3
-        for _ in range(2):
-            k += 1 + int(lines[k].split()[0])
+        na = int(lines[k].split()[0])
+        k += 1
+        self.angles = {}
+        for _ in range(na):
+            t = lines[k].split()
+            a, b, c = int(t[0]) - 1, int(t[1]) - 1, int(t[2]) - 1
+            prm = dict(zip(["theta_00", "p_val1", "p_val2", "p_coa1", "p_val7", "p_pen1", "p_val4"],
+                           [float(v) for v in t[3:10]]))
+            self.angles.setdefault((a, b, c), []).append(prm)
+            if c != a:
+                self.angles.setdefault((c, b, a), []).append(prm)
+            k += 1
+        k += 1 + int(lines[k].split()[0])
         nh = int(lines[k].split()[0])
         k += 1
         self.hbonds = {}
@@ -102,6 +115,10 @@ class FField:
     def hbond(self, i: str, j: str, k: str):
         """Hydrogen-bond parameters of acceptor i, hydrogen j, acceptor k (None if absent)."""
         return self.hbonds.get((self.idx(i), self.idx(j), self.idx(k)))
+
+    def angle(self, i: str, j: str, k: str):
+        """Valence-angle parameter sets of end i, centre j, end k."""
+        return self.angles.get((self.idx(i), self.idx(j), self.idx(k)), [])
 
     def eta(self, el: str) -> float:
         return 2.0 * self.atoms[el]["eta_file"]

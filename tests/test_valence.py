@@ -1,0 +1,221 @@
+"""Valence angles (the valence part of Merge_Torsion_Valence_Angles).
+
+Oracle pins (CPU):
+  * finite differences of e_ang + e_pen + e_coa reproduce the gradient the
+    oracle's valence pass plus the bond-order chain rule give (water/methanol
+    clusters, a compressed one, nitromethane beside propane for the penalty
+    and coalition terms, water);
+  * zero net force and torque;
+  * an isolated water molecule: e_ang equals the published angle expression
+    f7(BO_a) f7(BO_b) f8(Delta_j) p_val1 (1 - exp(-p_val2 (theta_0 -
+    theta)^2)) summed over its three triples, with theta from the positions,
+    the parameters read from the raw ffield text and theta_0 from SBO; these
+    triples have no penalty or coalition parameters, so those energies
+    vanish.
+GPU parity: energies, Cdbo/Cdbopi/Cdbopi2 per bond, CdDelta and the
+gradient after the chain rule at 1e-10 relative (water box, clusters).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from sw_reaxff_amd import systems
+
+from .ffield_kat import FField
+from .test_bonded import _bond_key, compressed, run_stages, water_cluster
+
+
+def _oracle(oracle_lib, ffield, s, x=None):
+    x = s.x if x is None else x
+    o = oracle_lib.Oracle(ffield, s.elements, threads=1)
+    o.set_atoms(s.n, np.asarray(x, np.float64), s.type.astype(np.int32), s.tag.astype(np.int32),
+                np.arange(s.n, dtype=np.int32))
+    o.build_verlet(12.0)
+    run_stages(o)
+    return o
+
+
+def _energy(oracle_lib, ffield, s, x):
+    return float(_oracle(oracle_lib, ffield, s, x).valence_angles().sum())
+
+
+def nitromethane_propane(seed):
+    """Nitromethane (O-N-O: penalty and coalition terms) beside a propane
+    carbon chain (C-C-C: penalty), slightly distorted."""
+    els = ["C", "H", "O", "N"]
+    x = np.array([[0.0, 0.0, 0.0], [1.49, 0.0, 0.0], [2.10, 1.07, 0.0], [2.10, -1.07, 0.0],
+                  [-0.36, 1.03, 0.0], [-0.36, -0.51, 0.89], [-0.36, -0.51, -0.89],
+                  [0.0, 4.5, 0.0], [1.53, 4.5, 0.0], [2.04, 5.94, 0.0],
+                  [-0.37, 3.48, 0.0], [-0.37, 5.0, 0.89], [1.90, 3.98, 0.89], [3.14, 5.94, 0.0]])
+    t = np.array([1, 4, 3, 3, 2, 2, 2, 1, 1, 1, 2, 2, 2, 2])
+    x = x + np.random.default_rng(seed).normal(scale=0.05, size=x.shape) + 10.0
+    return systems.System(x=x, v=np.zeros_like(x), type=t, tag=np.arange(1, len(t) + 1), elements=els,
+                          lo=np.zeros(3), hi=np.full(3, 30.0), periodic=False)
+
+
+@pytest.mark.parametrize("case", ["cluster30", "compressed", "nitro", "water"])
+def test_valence_gradient_matches_finite_differences(oracle_lib, ffield, case):
+    if case == "cluster30":
+        s = systems.cluster(30, seed=5)
+    elif case == "compressed":
+        s = compressed(systems.cluster(24, seed=8), 0.9)
+    elif case == "nitro":
+        s = nitromethane_propane(3)
+    else:
+        s = water_cluster(8, seed=2)
+    o = _oracle(oracle_lib, ffield, s)
+    e = o.valence_angles()
+    o.dbond_forces()
+    g = o.get("fCdDelta").reshape(-1, 4)[: s.n, :3]
+    gmax = np.abs(g).max()
+    assert e[0] != 0.0
+    if case == "nitro":
+        assert e[1] != 0.0 and e[2] != 0.0
+    rng = np.random.default_rng(7)
+    h = 1e-5
+    worst = 0.0
+    for a in rng.choice(s.n, size=6, replace=False):
+        for k in range(3):
+            xp, xm = s.x.copy(), s.x.copy()
+            xp[a, k] += h
+            xm[a, k] -= h
+            fd = (_energy(oracle_lib, ffield, s, xp) - _energy(oracle_lib, ffield, s, xm)) / (2 * h)
+            worst = max(worst, abs(fd - g[a, k]) / gmax)
+    print(f"[valence] {case} n={s.n} e_ang/e_pen/e_coa {e} max|g| {gmax:.3e} worst FD error {worst:.2e}")
+    assert worst <= 1e-6
+
+
+def test_valence_gradient_has_no_net_force_or_torque(oracle_lib, ffield):
+    s = systems.cluster(60, seed=11)
+    o = _oracle(oracle_lib, ffield, s)
+    o.valence_angles()
+    o.dbond_forces()
+    g = o.get("fCdDelta").reshape(-1, 4)[: s.n, :3]
+    scale = np.abs(g).max()
+    assert np.abs(g.sum(axis=0)).max() <= 1e-11 * scale * s.n
+    torque = np.cross(s.x - s.x.mean(axis=0), g).sum(axis=0)
+    assert np.abs(torque).max() <= 1e-10 * scale * s.n
+
+
+def test_water_molecule_angle_energy_closed_form(oracle_lib, ffield):
+    """Every triple of an isolated water molecule: H-O-H centred on O and,
+    through the weak H-H bond (BO ~ 0.012 > thb_cut), O-H-H centred on each
+    H, each f7(BO_a) f7(BO_b) f8(Delta_boc) p_val1 (1 - exp(-p_val2
+    (theta_0 - theta)^2)) with theta from the positions."""
+    ff = FField(ffield)
+    els = ["H", "O"]
+    x = systems.water_body()[[0, 1, 2]] + 5.0
+    s = systems.System(x=x, v=np.zeros_like(x), type=np.array([2, 1, 1]), tag=np.array([1, 2, 3]),
+                       elements=els, lo=np.zeros(3), hi=np.full(3, 20.0), periodic=False)
+    o = _oracle(oracle_lib, ffield, s)
+    e = o.valence_angles()
+    bi, be = o.get("bond_index"), o.get("bond_end")
+    nbr = o.get("bonds")["nbr"]
+    BO = o.get("BO")
+    BOpi = o.get("BOpi").reshape(-1, 2)
+    Dboc_all = o.get("bo_dboc").reshape(-1, 2)[:, 1]
+    vlpex, nlp = o.get("vlpex"), o.get("nlp")
+    gp = ff.gp
+    thb_cut, thb_cutsq = 0.001, 0.00001
+    pi_reference = 3.14159265                   # the reference's constPI
+    el = ["O", "H", "H"]
+    want, ntriples = 0.0, 0
+    for j in range(3):
+        a = ff.atom(el[j])
+        p_val3, p_val5 = a["p_val3"], a["p_val5"]
+        slots = list(range(bi[j], be[j]))
+        SBOp = sum(BOpi[p].sum() for p in slots)
+        prod = math.prod(math.exp(-BO[p] ** 8) for p in slots)
+        vlpadj = 0.0 if vlpex[j] >= 0 else nlp[j]
+        SBO = SBOp + (1 - prod) * (-Dboc_all[j] - gp[33] * vlpadj)
+        if SBO <= 0:
+            SBO2 = 0.0
+        elif SBO <= 1:
+            SBO2 = SBO ** gp[16]
+        elif SBO < 2:
+            SBO2 = 2 - (2 - SBO) ** gp[16]
+        else:
+            SBO2 = 2.0
+        for u in range(len(slots)):
+            for w in range(u + 1, len(slots)):
+                pa, pb = slots[u], slots[w]
+                if not (BO[pa] > thb_cut and BO[pb] > thb_cut and BO[pa] * BO[pb] > thb_cutsq):
+                    continue
+                ka, kb = int(nbr[pa]), int(nbr[pb])
+                v1, v2 = x[ka] - x[j], x[kb] - x[j]
+                theta = math.acos(v1 @ v2 / (np.linalg.norm(v1) * np.linalg.norm(v2)))
+                for prm in ff.angle(el[ka], el[j], el[kb]):
+                    if abs(prm["p_val1"]) <= 0.001:
+                        continue
+                    ntriples += 1
+                    theta0 = (180.0 - prm["theta_00"] * (1 - math.exp(-gp[17] * (2 - SBO2)))) * pi_reference / 180.0
+                    f7a = 1 - math.exp(-p_val3 * (BO[pa] - thb_cut) ** prm["p_val4"])
+                    f7b = 1 - math.exp(-p_val3 * (BO[pb] - thb_cut) ** prm["p_val4"])
+                    e6, e7 = math.exp(gp[14] * Dboc_all[j]), math.exp(-prm["p_val7"] * Dboc_all[j])
+                    f8 = p_val5 - (p_val5 - 1) * (2 + e6) / (1 + e6 + e7)
+                    want += f7a * f7b * f8 * prm["p_val1"] * (1 - math.exp(-prm["p_val2"] * (theta0 - theta) ** 2))
+    print(f"[valence] water molecule: {ntriples} triples, e_ang {e[0]:.10e} closed form {want:.10e}")
+    assert ntriples == 3
+    assert e[0] == pytest.approx(want, rel=1e-12)
+    assert e[1] == 0.0 and e[2] == 0.0      # no penalty or coalition parameters for these triples
+
+
+# ---------------------------------------------------------------- GPU parity
+def _gpu_vs_oracle(oracle_lib, ffield, lmp, elements, stype, stag):
+    from sw_reaxff_amd import _lib
+    h = lmp.system
+    _lib.check(h.L.swr_valence_angles(h.h), "swr_valence_angles")
+    h.sync()
+    cd_gpu = h.get("fCdDelta").reshape(-1, 4)[:, 3].copy()
+    cdbo_gpu = [h.get(k).copy() for k in ("Cdbo", "Cdbopi", "Cdbopi2")]
+    gv_gpu = h.get("fCdDelta").reshape(-1, 4)[:, :3].copy()
+    _lib.check(h.L.swr_add_dbond_forces(h.h), "swr_add_dbond_forces")
+    h.sync()
+    e_gpu = h.get("valence_energy")
+    ea = h.get("valence_eatom")
+    g_gpu = h.get("fCdDelta").reshape(-1, 4)[:, :3]
+    n, N = h.scalar("n"), h.scalar("N")
+    xa = h.get("x").reshape(-1, 3)[:N]
+    owner = h.get("owner")[:N]
+    o = oracle_lib.Oracle(ffield, elements, threads=8)
+    o.set_atoms(n, xa, stype[owner].astype(np.int32), stag[owner].astype(np.int32), owner.astype(np.int32),
+                q=h.get("q")[:N])
+    o.build_verlet(12.0)
+    run_stages(o)
+    e_o = o.valence_angles()
+    cd_o = o.get("fCdDelta").reshape(-1, 4)[:, 3].copy()
+    cdbo_o = [o.get(k).copy() for k in ("Cdbo", "Cdbopi", "Cdbopi2")]
+    gv_o = o.get("fCdDelta").reshape(-1, 4)[:, :3].copy()
+    o.dbond_forces()
+    g_o = o.get("fCdDelta").reshape(-1, 4)[:, :3]
+    print(f"[valence] n={n} N={N} gpu {e_gpu} oracle {e_o}", flush=True)
+    for k in range(3):
+        assert e_gpu[k] == pytest.approx(e_o[k], rel=1e-10, abs=1e-9), k
+    assert ea.reshape(3, -1)[:, :n].sum(axis=1) == pytest.approx(e_gpu, rel=1e-12, abs=1e-12)
+    kg = _bond_key(h.get("bond_index"), h.get("bond_end"), h.get("bonds")["nbr"], N)
+    ko = _bond_key(o.get("bond_index"), o.get("bond_end"), o.get("bonds")["nbr"], N)
+    assert kg.size == ko.size
+    for a, b in zip(cdbo_gpu, cdbo_o):
+        scale = np.abs(b[ko]).max()
+        assert np.max(np.abs(a[kg] - b[ko])) <= 1e-10 * scale
+    assert np.max(np.abs(cd_gpu[:N] - cd_o[:N])) <= 1e-10 * np.abs(cd_o[:N]).max()
+    assert np.max(np.abs(gv_gpu[:N] - gv_o[:N])) <= 1e-10 * np.abs(gv_o[:N]).max()
+    assert np.max(np.abs(g_gpu[:N] - g_o[:N])) <= 1e-10 * np.abs(g_o[:N]).max()
+
+
+@pytest.mark.gpu
+def test_gpu_valence_water_box(gpu_lib, oracle_lib, ffield):
+    from .parity import harness
+    s = systems.water_box(648, seed=3)
+    lmp = harness(s, ffield, tol=1e-10)
+    _gpu_vs_oracle(oracle_lib, ffield, lmp, s.elements, s.type, s.tag)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("f", [1.0, 0.9])
+def test_gpu_valence_cluster(gpu_lib, oracle_lib, ffield, f):
+    from .parity import harness
+    s = compressed(systems.cluster(700, seed=9), f)
+    lmp = harness(s, ffield, tol=1e-10)
+    _gpu_vs_oracle(oracle_lib, ffield, lmp, s.elements, s.type, s.tag)
