@@ -552,15 +552,17 @@ def main():
     # and the bond/atom energies with the bond-order chain rule of the
     # gradient (Atom_Energy + Add_All_dBond_to_Forces) over the same bonds
     sysh.ktimer_enable(True)
-    for slot in ("atom_energy", "hbonds", "dbond"):
+    for slot in ("atom_energy", "valence", "hbonds", "dbond"):
         sysh.ktimer_read(slot, reset=True)
     for _ in range(10):
         _L.check(sysh.L.swr_atom_energy(sysh.h), "swr_atom_energy")
+        _L.check(sysh.L.swr_valence_angles(sysh.h), "swr_valence_angles")
         _L.check(sysh.L.swr_hydrogen_bonds(sysh.h), "swr_hydrogen_bonds")
         _L.check(sysh.L.swr_add_dbond_forces(sysh.h), "swr_add_dbond_forces")
     device_sync()
     ae_ms, ae_cnt = sysh.ktimer_read("atom_energy")
     hb_ms, hb_cnt = sysh.ktimer_read("hbonds")
+    va_ms, va_cnt = sysh.ktimer_read("valence")
     db_ms, db_cnt = sysh.ktimer_read("dbond")
     sysh.ktimer_enable(False)
     be_b = algorithmic_bytes("bonded", c)
@@ -569,6 +571,9 @@ def main():
               "avg_us": round(be_avg * 1e3, 2), "atom_energy_us": round(ae_ms * 1e3 / max(1, ae_cnt), 2),
               "dbond_us": round(db_ms * 1e3 / max(1, db_cnt), 2), "launches": ae_cnt, "bytes_per_pass": be_b,
               "achieved_gbs": round(be_b / (be_avg * 1e-3) / 1e9, 1),
+              "valence": {"kernels": "k_valence, k_va_fold, k_sum_rows",
+                          "avg_us": round(va_ms * 1e3 / max(1, va_cnt), 2),
+                          "energies_kcal_mol": [float(v) for v in sysh.get("valence_energy")]},
               "hbonds": {"kernels": "k_hb, k_hb_fold, k_sum_rows",
                          "avg_us": round(hb_ms * 1e3 / max(1, hb_cnt), 2),
                          "e_hb_kcal_mol": float(sysh.get("hbond_energy")[0])},

@@ -583,6 +583,7 @@ __global__ void __launch_bounds__(BT) k_valence(int n, VaArgs A) {
       SBO2 = 2.0; CSBO2 = 0.0;
     }
     const double expval6 = exp(p_val6 * Dboc_j);
+    const double exp10 = exp(-p_val10 * (2.0 - SBO2));   // theta_0 and Ctheta_0 of every set
     const double Delta_j = A.Delta[j];
     const double exp_pen3 = exp(-P->p_pen3 * Delta_j), exp_pen4 = exp(P->p_pen4 * Delta_j);
     const double trm_pen34 = 1.0 + exp_pen3 + exp_pen4;
@@ -623,23 +624,35 @@ __global__ void __launch_bounds__(BT) k_valence(int n, VaArgs A) {
         const double inv_d = 1.0 / (bjk.d * bhj.d);
         const double cinv3 = dot * (inv_d * inv_d * inv_d);
         const double bo_k0 = A.bo_dboc[2 * (size_t)k], bo_h0 = A.bo_dboc[2 * (size_t)h];
+        // the penalty and coalition factors of the pair, the same for every
+        // set; only evaluated when a set has p_pen1 or p_coa1 (a term with a
+        // zero prefactor is zero either way)
+        const bool need_pc = P->thb_pc[tri] != 0;
+        const double pen2 = need_pc ? exp(-P->p_pen2 * (BOA_jk - 2.0) * (BOA_jk - 2.0)) *
+                                          exp(-P->p_pen2 * (BOA_hj - 2.0) * (BOA_hj - 2.0))
+                                    : 0.0;
+        const double coa = need_pc ? 1.0 / (1. + exp_coa2) * exp(-P->p_coa3 * (bo_k0 - BOA_jk) * (bo_k0 - BOA_jk)) *
+                                         exp(-P->p_coa3 * (bo_h0 - BOA_hj) * (bo_h0 - BOA_hj)) *
+                                         exp(-P->p_coa4 * (BOA_jk - 1.5) * (BOA_jk - 1.5)) *
+                                         exp(-P->p_coa4 * (BOA_hj - 1.5) * (BOA_hj - 1.5))
+                                   : 0.0;
         double ck[4] = {0.0, 0.0, 0.0, 0.0}, chh[4] = {0.0, 0.0, 0.0, 0.0};
         double cdbo_jk = 0.0, cdbo_hj = 0.0;
         for (int c = 0; c < nset; ++c) {
           const DevThb &q = P->thb[tri][c];
           const double p_val1 = q.p_val1, p_val2 = q.p_val2, p_val4 = q.p_val4, p_val7 = q.p_val7;
-          const double exp3jk = exp(-p_val3 * pow(BOA_jk, p_val4));
+          const double pjk = pow(BOA_jk, p_val4 - 1.0), phj = pow(BOA_hj, p_val4 - 1.0);
+          const double exp3jk = exp(-p_val3 * (pjk * BOA_jk));
           const double f7_jk = 1.0 - exp3jk;
-          const double Cf7jk = p_val3 * p_val4 * pow(BOA_jk, p_val4 - 1.0) * exp3jk;
-          const double exp3hj = exp(-p_val3 * pow(BOA_hj, p_val4));
+          const double Cf7jk = p_val3 * p_val4 * pjk * exp3jk;
+          const double exp3hj = exp(-p_val3 * (phj * BOA_hj));
           const double f7_hj = 1.0 - exp3hj;
-          const double Cf7hj = p_val3 * p_val4 * pow(BOA_hj, p_val4 - 1.0) * exp3hj;
+          const double Cf7hj = p_val3 * p_val4 * phj * exp3hj;
           const double expval7 = exp(-p_val7 * Dboc_j);
           const double trm8 = 1.0 + expval6 + expval7;
           const double f8_Dj = p_val5 - ((p_val5 - 1.0) * (2.0 + expval6) / trm8);
           const double Cf8j = ((1.0 - p_val5) / (trm8 * trm8)) *
                               (p_val6 * expval6 * trm8 - (2.0 + expval6) * (p_val6 * expval6 - p_val7 * expval7));
-          const double exp10 = exp(-p_val10 * (2.0 - SBO2));
           const double theta_0 = (180.0 - q.theta_00 * (1.0 - exp10)) * DEG2RAD_PI / 180.0;
           const double expval2theta = exp(-p_val2 * (theta_0 - theta) * (theta_0 - theta));
           const double expval12theta = p_val1 >= 0 ? p_val1 * (1.0 - expval2theta) : p_val1 * -expval2theta;
@@ -652,18 +665,13 @@ __global__ void __launch_bounds__(BT) k_valence(int n, VaArgs A) {
           const double CEval8 = -CEval4 / sin_t;
           e_ang += f7_jk * f7_hj * f8_Dj * expval12theta;
           // penalty
-          const double exp_pen2jk = exp(-P->p_pen2 * (BOA_jk - 2.0) * (BOA_jk - 2.0));
-          const double exp_pen2hj = exp(-P->p_pen2 * (BOA_hj - 2.0) * (BOA_hj - 2.0));
-          const double ep = q.p_pen1 * f9_Dj * exp_pen2jk * exp_pen2hj;
+          const double ep = q.p_pen1 * f9_Dj * pen2;
           e_pen += ep;
           const double CEpen1 = ep * Cf9j / f9_Dj;
           const double tp = -2.0 * P->p_pen2 * ep;
           const double CEpen2 = tp * (BOA_jk - 2.0), CEpen3 = tp * (BOA_hj - 2.0);
           // coalition
-          const double ec = q.p_coa1 / (1. + exp_coa2) * exp(-P->p_coa3 * (bo_k0 - BOA_jk) * (bo_k0 - BOA_jk)) *
-                            exp(-P->p_coa3 * (bo_h0 - BOA_hj) * (bo_h0 - BOA_hj)) *
-                            exp(-P->p_coa4 * (BOA_jk - 1.5) * (BOA_jk - 1.5)) *
-                            exp(-P->p_coa4 * (BOA_hj - 1.5) * (BOA_hj - 1.5));
+          const double ec = q.p_coa1 * coa;
           e_coa += ec;
           const double CEcoa1 = -2 * P->p_coa4 * (BOA_jk - 1.5) * ec;
           const double CEcoa2 = -2 * P->p_coa4 * (BOA_hj - 1.5) * ec;

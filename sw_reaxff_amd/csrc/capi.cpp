@@ -131,6 +131,7 @@ void fill_params(swr_system *S, const std::vector<int> &map) {
       for (int c = 0; c < ff.ne() && c < SWR_MAX_ELEM; c++) {
         const int d = (a * SWR_MAX_ELEM + b) * SWR_MAX_ELEM + c;
         h.thb_n[d] = 0;
+        h.thb_pc[d] = 0;
         if (ff.thb_cnt.empty()) continue;
         const size_t t = ff.tri(a, b, c);
         for (int k = 0; k < ff.thb_cnt[t] && k < SWR_FF_MAX_THB; k++) {
@@ -139,6 +140,7 @@ void fill_params(swr_system *S, const std::vector<int> &map) {
           DevThb &o = h.thb[d][h.thb_n[d]++];
           o.theta_00 = f.theta_00; o.p_val1 = f.p_val1; o.p_val2 = f.p_val2; o.p_coa1 = f.p_coa1;
           o.p_val7 = f.p_val7; o.p_pen1 = f.p_pen1; o.p_val4 = f.p_val4;
+          if (f.p_pen1 != 0.0 || f.p_coa1 != 0.0) h.thb_pc[d] = 1;
         }
       }
   for (int a = 0; a < ff.ne() && a < SWR_MAX_ELEM; a++)

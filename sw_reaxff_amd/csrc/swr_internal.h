@@ -119,6 +119,7 @@ struct DevParams {
   double p_val3[SWR_MAX_ELEM], p_val5[SWR_MAX_ELEM];
   double p_val6, p_val8, p_val9, p_val10, p_pen2, p_pen3, p_pen4, p_coa2, p_coa3, p_coa4, thb_cut, thb_cutsq;
   int thb_n[SWR_MAX_ELEM * SWR_MAX_ELEM * SWR_MAX_ELEM];
+  int thb_pc[SWR_MAX_ELEM * SWR_MAX_ELEM * SWR_MAX_ELEM];   // 1: a set has p_pen1 or p_coa1 != 0
   DevThb thb[SWR_MAX_ELEM * SWR_MAX_ELEM * SWR_MAX_ELEM][SWR_MAX_THB];
 };
 
