@@ -552,17 +552,19 @@ def main():
     # and the bond/atom energies with the bond-order chain rule of the
     # gradient (Atom_Energy + Add_All_dBond_to_Forces) over the same bonds
     sysh.ktimer_enable(True)
-    for slot in ("atom_energy", "valence", "hbonds", "dbond"):
+    for slot in ("atom_energy", "valence", "torsion", "hbonds", "dbond"):
         sysh.ktimer_read(slot, reset=True)
     for _ in range(10):
         _L.check(sysh.L.swr_atom_energy(sysh.h), "swr_atom_energy")
         _L.check(sysh.L.swr_valence_angles(sysh.h), "swr_valence_angles")
+        _L.check(sysh.L.swr_torsion_angles(sysh.h), "swr_torsion_angles")
         _L.check(sysh.L.swr_hydrogen_bonds(sysh.h), "swr_hydrogen_bonds")
         _L.check(sysh.L.swr_add_dbond_forces(sysh.h), "swr_add_dbond_forces")
     device_sync()
     ae_ms, ae_cnt = sysh.ktimer_read("atom_energy")
     hb_ms, hb_cnt = sysh.ktimer_read("hbonds")
     va_ms, va_cnt = sysh.ktimer_read("valence")
+    to_ms, to_cnt = sysh.ktimer_read("torsion")
     db_ms, db_cnt = sysh.ktimer_read("dbond")
     sysh.ktimer_enable(False)
     be_b = algorithmic_bytes("bonded", c)
@@ -574,6 +576,9 @@ def main():
               "valence": {"kernels": "k_valence, k_va_fold, k_sum_rows",
                           "avg_us": round(va_ms * 1e3 / max(1, va_cnt), 2),
                           "energies_kcal_mol": [float(v) for v in sysh.get("valence_energy")]},
+              "torsion": {"kernels": "k_torsion, k_va_fold, k_to_slot_fold, k_sum_rows",
+                          "avg_us": round(to_ms * 1e3 / max(1, to_cnt), 2),
+                          "energies_kcal_mol": [float(v) for v in sysh.get("torsion_energy")]},
               "hbonds": {"kernels": "k_hb, k_hb_fold, k_sum_rows",
                          "avg_us": round(hb_ms * 1e3 / max(1, hb_cnt), 2),
                          "e_hb_kcal_mol": float(sysh.get("hbond_energy")[0])},

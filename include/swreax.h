@@ -264,10 +264,24 @@ int swr_hydrogen_bonds(swr_system *sys);
  * bonds and CdDelta of j, k, h, and the angle derivative into
  * fCdDelta[.][0..2] of j, k, h.  Leaves e_ang, e_pen, e_coa in
  * swr_get(sys, "valence_energy") and each local atom's share in
- * "valence_eatom" (3 rows of n doubles).  The torsion and conjugation part
- * of the same function is not built.  Run after swr_bo and before
+ * "valence_eatom" (3 rows of n doubles).  The torsion part of the same
+ * function is swr_torsion_angles.  Run after swr_bo and before
  * swr_add_dbond_forces; enqueues only. */
 int swr_valence_angles(swr_system *sys);
+
+/* Torsions: the torsion part of Merge_Torsion_Valence_Angles
+ * (reaxc_torsion_angles_sw64.c:812-1408; Calculate_Omega :21-104).  For every
+ * local atom j and bond j-k (BO > thb_cut), once per physical bond (the
+ * smaller orig_id first, images of one atom by position), every other bond
+ * j-i and bond k-l (l != i) with BO > thb_cut, BO_ij BO_jk BO_kl > thb_cut
+ * and a torsion entry for (i, j, k, l) in the force field's torsion block:
+ * torsion and four-body conjugation energies, their coefficients into
+ * Cdbopi of j-k, Cdbo of j-i, j-k, k-l and CdDelta of j and k, and the angle
+ * and dihedral derivatives into fCdDelta[.][0..2] of i, j, k, l.  Leaves
+ * e_tor, e_con in swr_get(sys, "torsion_energy") and each local atom's share
+ * in "torsion_eatom" (2 rows of n doubles).  Run after swr_bo and before
+ * swr_add_dbond_forces; enqueues only. */
+int swr_torsion_angles(swr_system *sys);
 
 /* Add_All_dBond_to_Forces_C (reaxc_forces_sw64.c:417-589): over all N atoms
  * and their bonds, the chain rule of the bond-order derivatives with the

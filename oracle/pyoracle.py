@@ -56,6 +56,7 @@ def lib():
         L.orc_hydrogen_bonds.restype = C.c_double
         L.orc_hydrogen_bonds.argtypes = [vp]
         L.orc_valence_angles.argtypes = [vp, dp]
+        L.orc_torsion_angles.argtypes = [vp, dp]
         L.orc_get_array.restype = C.c_int64
         L.orc_get_array.argtypes = [vp, C.c_char_p, vp]
         L.orc_set_array.argtypes = [vp, C.c_char_p, vp, C.c_int64]
@@ -208,6 +209,14 @@ class Oracle:
         (e_ang, e_pen, e_coa)."""
         o = np.zeros(3)
         self.L.orc_valence_angles(self.h, _p(o, C.c_double))
+        return o
+
+    def torsion_angles(self):
+        """Torsion part of Merge_Torsion_Valence_Angles over the current
+        bonds: adds to Cdbo/Cdbopi, CdDelta and fCdDelta[:, 0..2]; returns
+        (e_tor, e_con)."""
+        o = np.zeros(2)
+        self.L.orc_torsion_angles(self.h, _p(o, C.c_double))
         return o
 
     def dbond_forces(self):

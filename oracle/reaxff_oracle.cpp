@@ -66,6 +66,12 @@ struct THBP {
 };
 constexpr int MAX_3BODY_PARAM = 5;
 
+// four_body_header cnt + four_body_parameters prm[0]
+struct FBP {
+  int cnt;
+  double V1, V2, V3, p_tor1, p_cot1;
+};
+
 struct TBP {
   double p_bo1, p_bo2, p_bo3, p_bo4, p_bo5, p_bo6;
   double r_s, r_p, r_pp;
@@ -106,6 +112,7 @@ struct orc_sys {
   std::vector<int> thb_cnt;      // [nt][nt][nt]
   std::vector<THBP> thbp;        // [nt][nt][nt][MAX_3BODY_PARAM], zero-filled
   double thb_cut = 0.001, thb_cutsq = 0.00001;   // pair_reaxc_sunway.cpp:218-219
+  std::vector<FBP> fbp;          // [nt][nt][nt][nt]
   double bo_cut = 0, nonb_low = 0, nonb_cut = 0;
   int vdw_type = 0;
   double bond_cut = 5.0, hbond_cut = 7.5;
@@ -314,9 +321,36 @@ bool read_ffield(orc_sys *S, const char *path) {
       S->thbp[abc * MAX_3BODY_PARAM + cnt] = v;
       S->thbp[cba * MAX_3BODY_PARAM + cnt] = v;
     }
+    // torsion block (:574-652): explicit quadruples set and flag both
+    // orientations; "0 j k 0" sets cnt of every (p,j,k,o), (o,k,j,p) and the
+    // values where not flagged
+    S->fbp.assign((size_t)nt * nt * nt * nt, FBP{0, 0, 0, 0, 0, 0});
+    std::vector<char> tor_flag((size_t)nt * nt * nt * nt, 0);
+    auto Q = [&](int a, int b, int c, int d) { return (((size_t)a * nt + b) * nt + c) * nt + d; };
     t = next();
     const int l2 = t.empty() ? 0 : std::atoi(t[0].c_str());
-    for (int k = 0; k < l2; k++) next();
+    for (int k = 0; k < l2; k++) {
+      t = next();
+      if (t.size() < 9) continue;
+      const int j = std::atoi(t[0].c_str()) - 1, kk = std::atoi(t[1].c_str()) - 1, m = std::atoi(t[2].c_str()) - 1,
+                nn = std::atoi(t[3].c_str()) - 1;
+      const FBP v{1, tokd(t, 4), tokd(t, 5), tokd(t, 6), tokd(t, 7), tokd(t, 8)};
+      if (j >= 0 && nn >= 0) {
+        if (j < nt && kk < nt && m < nt && nn < nt) {
+          tor_flag[Q(j, kk, m, nn)] = tor_flag[Q(nn, m, kk, j)] = 1;
+          S->fbp[Q(j, kk, m, nn)] = v;
+          S->fbp[Q(nn, m, kk, j)] = v;
+        }
+      } else if (kk < nt && m < nt && kk >= 0 && m >= 0) {
+        for (int p = 0; p < nt; p++)
+          for (int o = 0; o < nt; o++) {
+            S->fbp[Q(p, kk, m, o)].cnt = 1;
+            S->fbp[Q(o, m, kk, p)].cnt = 1;
+            if (tor_flag[Q(p, kk, m, o)] == 0) S->fbp[Q(p, kk, m, o)] = v;
+            if (tor_flag[Q(o, m, kk, p)] == 0) S->fbp[Q(o, m, kk, p)] = v;
+          }
+      }
+    }
   }
   t = next();
   const int nh = t.empty() ? 0 : std::atoi(t[0].c_str());
@@ -1524,6 +1558,226 @@ double orc_hydrogen_bonds(orc_sys *S) {
     }
   }
   return e_hb_total;
+}
+
+// Calculate_Omega (reaxc_torsion_angles_sw64.c:21-104): the dihedral i-j-k-l
+// from the bond vectors, and the derivatives of cos(omega).
+double orc_omega(const double *dvec_ij, double r_ij, const double *dvec_jk, double r_jk, const double *dvec_kl,
+                 double r_kl, const double *dvec_li, double r_li, double theta_ijk, const double *ijk_di,
+                 const double *ijk_dj, const double *ijk_dk, double theta_jkl, const double *jkl_di,
+                 const double *jkl_dj, const double *jkl_dk, double *d_i, double *d_j, double *d_k, double *d_l) {
+  const double MIN_SINE = 1e-10;
+  auto dot = [](const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+  double sin_ijk = std::sin(theta_ijk), cos_ijk = std::cos(theta_ijk);
+  double sin_jkl = std::sin(theta_jkl), cos_jkl = std::cos(theta_jkl);
+  const double unnorm_cos_omega = -dot(dvec_ij, dvec_jk) * dot(dvec_jk, dvec_kl) + SQR(r_jk) * dot(dvec_ij, dvec_kl);
+  const double cross[3] = {dvec_jk[1] * dvec_kl[2] - dvec_jk[2] * dvec_kl[1],
+                           dvec_jk[2] * dvec_kl[0] - dvec_jk[0] * dvec_kl[2],
+                           dvec_jk[0] * dvec_kl[1] - dvec_jk[1] * dvec_kl[0]};
+  const double unnorm_sin_omega = -r_jk * dot(dvec_ij, cross);
+  const double omega = std::atan2(unnorm_sin_omega, unnorm_cos_omega);
+  const double htra = r_ij + cos_ijk * (r_kl * cos_jkl - r_jk);
+  const double htrb = r_jk - r_ij * cos_ijk - r_kl * cos_jkl;
+  const double htrc = r_kl + cos_jkl * (r_ij * cos_ijk - r_jk);
+  const double hthd = r_ij * sin_ijk * (r_jk - r_kl * cos_jkl);
+  const double hthe = r_kl * sin_jkl * (r_jk - r_ij * cos_ijk);
+  const double hnra = r_kl * sin_ijk * sin_jkl;
+  const double hnrc = r_ij * sin_ijk * sin_jkl;
+  const double hnhd = r_ij * r_kl * cos_ijk * sin_jkl;
+  const double hnhe = r_ij * r_kl * sin_ijk * cos_jkl;
+  double poem = 2.0 * r_ij * r_kl * sin_ijk * sin_jkl;
+  if (poem < 1e-20) poem = 1e-20;
+  const double tel = SQR(r_ij) + SQR(r_jk) + SQR(r_kl) - SQR(r_li) -
+                     2.0 * (r_ij * r_jk * cos_ijk - r_ij * r_kl * cos_ijk * cos_jkl + r_jk * r_kl * cos_jkl);
+  double arg = tel / poem;
+  if (arg > 1.0) arg = 1.0;
+  if (arg < -1.0) arg = -1.0;
+  if (sin_ijk >= 0 && sin_ijk <= MIN_SINE) sin_ijk = MIN_SINE;
+  else if (sin_ijk <= 0 && sin_ijk >= -MIN_SINE) sin_ijk = -MIN_SINE;
+  if (sin_jkl >= 0 && sin_jkl <= MIN_SINE) sin_jkl = MIN_SINE;
+  else if (sin_jkl <= 0 && sin_jkl >= -MIN_SINE) sin_jkl = -MIN_SINE;
+  const double s = 2.0 / poem;
+  for (int c = 0; c < 3; c++) {
+    d_i[c] = s * ((htra - arg * hnra) / r_ij * dvec_ij[c] - dvec_li[c] - (hthd - arg * hnhd) / sin_ijk * ijk_dk[c]);
+    d_j[c] = s * (-(htra - arg * hnra) / r_ij * dvec_ij[c] - htrb / r_jk * dvec_jk[c] -
+                  (hthd - arg * hnhd) / sin_ijk * ijk_dj[c] - (hthe - arg * hnhe) / sin_jkl * jkl_di[c]);
+    d_k[c] = s * (-(htrc - arg * hnrc) / r_kl * dvec_kl[c] + htrb / r_jk * dvec_jk[c] -
+                  (hthd - arg * hnhd) / sin_ijk * ijk_di[c] - (hthe - arg * hnhe) / sin_jkl * jkl_dj[c]);
+    d_l[c] = s * ((htrc - arg * hnrc) / r_kl * dvec_kl[c] + dvec_li[c] - (hthe - arg * hnhe) / sin_jkl * jkl_dk[c]);
+  }
+  return omega;
+}
+
+// Torsion part of Merge_Torsion_Valence_Angles, host form
+// (reaxc_torsion_angles_sw64.c:812-1408): for every local j and bond j-k
+// (pk) with BO > thb_cut (k with bonds), taken once per physical bond
+// (orig_id(j) < orig_id(k), or equal with k after j by z, y, x), every other
+// bond j-i (ph) with BO > thb_cut and every bond k-l (pw) other than k-j with
+// i != l, a torsion term for (type i, j, k, l), BO_kl > thb_cut and
+// BO_ij BO_jk BO_kl > thb_cut: the torsion and conjugation energies and their
+// coefficients into Cdbopi of j-k, Cdbo of j-i, j-k and k-l, CdDelta of j
+// and k, and the angle and dihedral derivatives into fCdDelta of i, j, k, l.
+// theta by libm acos where the reference calls xacos; MIN_SINE 1e-10 (:12).
+// Returns e_tor, e_con in out[0..1].
+int orc_torsion_angles(orc_sys *S, double *out) {
+  const int n = S->n;
+  const int nt = (int)S->sbp.size();
+  const double MIN_SINE = 1e-10;
+  auto g = [&](size_t k) { return k < S->gp.size() ? S->gp[k] : 0.0; };
+  const double p_tor2 = g(23), p_tor3 = g(24), p_tor4 = g(25), p_cot2 = g(27);
+  const double thb_cut = S->thb_cut;
+  double *fc = S->fCdDelta.data();
+  double e_tor_t = 0.0, e_con_t = 0.0;
+  auto theta_of = [](const swr_bond_t &a, const swr_bond_t &b, double *di, double *dj, double *dk) {
+    const double dot = a.dvec[0] * b.dvec[0] + a.dvec[1] * b.dvec[1] + a.dvec[2] * b.dvec[2];
+    double c = dot / (a.d * b.d);
+    if (c > 1.) c = 1.0;
+    if (c < -1.) c = -1.0;
+    const double sa = SQR(a.d), sb = SQR(b.d);
+    const double inv = 1.0 / (a.d * b.d);
+    const double ci3 = dot * std::pow(inv, 3.0);
+    for (int q = 0; q < 3; q++) {
+      di[q] = b.dvec[q] * inv - ci3 * sb * a.dvec[q];
+      dj[q] = -(b.dvec[q] + a.dvec[q]) * inv + ci3 * (sb * a.dvec[q] + sa * b.dvec[q]);
+      dk[q] = a.dvec[q] * inv - ci3 * sa * b.dvec[q];
+    }
+    return std::acos(c);
+  };
+  auto add4 = [&](int a, double s, const double *v) {
+    for (int q = 0; q < 3; q++) fc[4 * (size_t)a + q] += s * v[q];
+  };
+  for (int j = 0; j < n; ++j) {
+    const int type_j = S->packed[j].type;
+    const int start_j = S->bindex[j], end_j = S->bend[j];
+    if (type_j < 0 || end_j <= start_j) continue;
+    const double Delta_j = S->bo_dboc[2 * (size_t)j + 1];
+    for (int pk = start_j; pk < end_j; ++pk) {
+      const swr_bond_t &bjk = S->bonds[pk];
+      const int k = bjk.nbr;
+      const double BOA_jk = S->BO[pk] - thb_cut;
+      if (!(BOA_jk > 0.0)) continue;
+      const int start_k = S->bindex[k], end_k = S->bend[k];
+      if (end_k <= start_k) continue;
+      const swr_atom_pack_t &aj = S->packed[j], &ak = S->packed[k];
+      if (aj.orig_id > ak.orig_id) continue;
+      if (aj.orig_id == ak.orig_id) {
+        if (ak.x[2] < aj.x[2]) continue;
+        if (ak.x[2] == aj.x[2] && ak.x[1] < aj.x[1]) continue;
+        if (ak.x[2] == aj.x[2] && ak.x[1] == aj.x[1] && ak.x[0] < aj.x[0]) continue;
+      }
+      const int pj = bjk.sym_index;
+      const swr_bond_t &bkj = S->bonds[pj];
+      const int type_k = ak.type;
+      const double Delta_k = S->bo_dboc[2 * (size_t)k + 1];
+      const double r_jk = bjk.d;
+      const double exp_tor2_jk = std::exp(-p_tor2 * BOA_jk);
+      const double exp_cot2_jk = std::exp(-p_cot2 * SQR(BOA_jk - 1.5));
+      const double exp_tor3_DjDk = std::exp(-p_tor3 * (Delta_j + Delta_k));
+      const double exp_tor4_DjDk = std::exp(p_tor4 * (Delta_j + Delta_k));
+      const double exp_tor34_inv = 1.0 / (1.0 + exp_tor3_DjDk + exp_tor4_DjDk);
+      const double f11_DjDk = (2.0 + exp_tor3_DjDk) * exp_tor34_inv;
+      for (int ph = start_j; ph < end_j; ++ph) {
+        if (ph == pk) continue;
+        const swr_bond_t &bij = S->bonds[ph];
+        if (!(S->BO[ph] > thb_cut)) continue;
+        const int i = bij.nbr;
+        const int type_i = S->packed[i].type;
+        double ijk_di[3], ijk_dj[3], ijk_dk[3];   // of k, j, i
+        const double theta_ijk = theta_of(bjk, bij, ijk_di, ijk_dj, ijk_dk);
+        const double r_ij = bij.d;
+        const double BOA_ij = S->BO[ph] - thb_cut;
+        const double sin_ijk = std::sin(theta_ijk), cos_ijk = std::cos(theta_ijk);
+        double tan_ijk_i;
+        if (sin_ijk >= 0 && sin_ijk <= MIN_SINE) tan_ijk_i = cos_ijk / MIN_SINE;
+        else if (sin_ijk <= 0 && sin_ijk >= -MIN_SINE) tan_ijk_i = cos_ijk / -MIN_SINE;
+        else tan_ijk_i = cos_ijk / sin_ijk;
+        const double exp_tor2_ij = std::exp(-p_tor2 * BOA_ij);
+        const double exp_cot2_ij = std::exp(-p_cot2 * SQR(BOA_ij - 1.5));
+        for (int pw = start_k; pw < end_k; ++pw) {
+          if (pw == pj) continue;
+          const swr_bond_t &bkl = S->bonds[pw];
+          const int l = bkl.nbr;
+          const int type_l = S->packed[l].type;
+          double jkl_di[3], jkl_dj[3], jkl_dk[3];   // of j, k, l
+          const double theta_jkl = theta_of(bkj, bkl, jkl_di, jkl_dj, jkl_dk);
+          if (type_i < 0 || type_l < 0) continue;
+          const FBP &fb = S->fbp[(((size_t)type_i * nt + type_j) * nt + type_k) * nt + type_l];
+          if (!(i != l && fb.cnt && S->BO[pw] > thb_cut && S->BO[ph] * S->BO[pk] * S->BO[pw] > thb_cut)) continue;
+          const double r_kl = bkl.d;
+          const double BOA_kl = S->BO[pw] - thb_cut;
+          const double sin_jkl = std::sin(theta_jkl), cos_jkl = std::cos(theta_jkl);
+          double tan_jkl_i;
+          if (sin_jkl >= 0 && sin_jkl <= MIN_SINE) tan_jkl_i = cos_jkl / MIN_SINE;
+          else if (sin_jkl <= 0 && sin_jkl >= -MIN_SINE) tan_jkl_i = cos_jkl / -MIN_SINE;
+          else tan_jkl_i = cos_jkl / sin_jkl;
+          double dvec_li[3];
+          for (int q = 0; q < 3; q++) dvec_li[q] = S->packed[i].x[q] - S->packed[l].x[q];
+          const double r_li = std::sqrt(SQR(dvec_li[0]) + SQR(dvec_li[1]) + SQR(dvec_li[2]));
+          double dw_i[3], dw_j[3], dw_k[3], dw_l[3];
+          const double omega = orc_omega(bij.dvec, r_ij, bjk.dvec, r_jk, bkl.dvec, r_kl, dvec_li, r_li, theta_ijk,
+                                         ijk_di, ijk_dj, ijk_dk, theta_jkl, jkl_di, jkl_dj, jkl_dk, dw_i, dw_j,
+                                         dw_k, dw_l);
+          const double cos_omega = std::cos(omega), cos2omega = std::cos(2. * omega),
+                       cos3omega = std::cos(3. * omega);
+          const double BOpi_jk = S->BOpi[2 * (size_t)pk];
+          const double exp_tor1 = std::exp(fb.p_tor1 * SQR(2.0 - BOpi_jk - f11_DjDk));
+          const double exp_tor2_kl = std::exp(-p_tor2 * BOA_kl);
+          const double exp_cot2_kl = std::exp(-p_cot2 * SQR(BOA_kl - 1.5));
+          const double fn10 = (1.0 - exp_tor2_ij) * (1.0 - exp_tor2_jk) * (1.0 - exp_tor2_kl);
+          const double CV = 0.5 * (fb.V1 * (1.0 + cos_omega) + fb.V2 * exp_tor1 * (1.0 - cos2omega) +
+                                   fb.V3 * (1.0 + cos3omega));
+          const double e_tor = fn10 * sin_ijk * sin_jkl * CV;
+          e_tor_t += e_tor;
+          const double dfn11 = (-p_tor3 * exp_tor3_DjDk +
+                                (p_tor3 * exp_tor3_DjDk - p_tor4 * exp_tor4_DjDk) * (2.0 + exp_tor3_DjDk) *
+                                    exp_tor34_inv) *
+                               exp_tor34_inv;
+          const double CEtors1 = sin_ijk * sin_jkl * CV;
+          const double CEtors2 = -fn10 * 2.0 * fb.p_tor1 * fb.V2 * exp_tor1 * (2.0 - BOpi_jk - f11_DjDk) *
+                                 (1.0 - SQR(cos_omega)) * sin_ijk * sin_jkl;
+          const double CEtors3 = CEtors2 * dfn11;
+          const double CEtors4 = CEtors1 * p_tor2 * exp_tor2_ij * (1.0 - exp_tor2_jk) * (1.0 - exp_tor2_kl);
+          const double CEtors5 = CEtors1 * p_tor2 * (1.0 - exp_tor2_ij) * exp_tor2_jk * (1.0 - exp_tor2_kl);
+          const double CEtors6 = CEtors1 * p_tor2 * (1.0 - exp_tor2_ij) * (1.0 - exp_tor2_jk) * exp_tor2_kl;
+          const double cmn = -fn10 * CV;
+          const double CEtors7 = cmn * sin_jkl * tan_ijk_i;
+          const double CEtors8 = cmn * sin_ijk * tan_jkl_i;
+          const double CEtors9 = fn10 * sin_ijk * sin_jkl *
+                                 (0.5 * fb.V1 - 2.0 * fb.V2 * exp_tor1 * cos_omega +
+                                  1.5 * fb.V3 * (cos2omega + 2.0 * SQR(cos_omega)));
+          const double fn12 = exp_cot2_ij * exp_cot2_jk * exp_cot2_kl;
+          const double e_con = fb.p_cot1 * fn12 * (1.0 + (SQR(cos_omega) - 1.0) * sin_ijk * sin_jkl);
+          e_con_t += e_con;
+          const double Cconj = -2.0 * fn12 * fb.p_cot1 * p_cot2 * (1.0 + (SQR(cos_omega) - 1.0) * sin_ijk * sin_jkl);
+          const double CEconj1 = Cconj * (BOA_ij - 1.5e0);
+          const double CEconj2 = Cconj * (BOA_jk - 1.5e0);
+          const double CEconj3 = Cconj * (BOA_kl - 1.5e0);
+          const double CEconj4 = -fb.p_cot1 * fn12 * (SQR(cos_omega) - 1.0) * sin_jkl * tan_ijk_i;
+          const double CEconj5 = -fb.p_cot1 * fn12 * (SQR(cos_omega) - 1.0) * sin_ijk * tan_jkl_i;
+          const double CEconj6 = 2.0 * fb.p_cot1 * fn12 * cos_omega * sin_ijk * sin_jkl;
+          S->Cdbopi[pk] += CEtors2;
+          fc[4 * (size_t)j + 3] += CEtors3;
+          fc[4 * (size_t)k + 3] += CEtors3;
+          S->Cdbo[ph] += (CEtors4 + CEconj1);
+          S->Cdbo[pk] += (CEtors5 + CEconj2);
+          S->Cdbo[pw] += (CEtors6 + CEconj3);
+          add4(i, CEtors7 + CEconj4, ijk_dk);
+          add4(j, CEtors7 + CEconj4, ijk_dj);
+          add4(k, CEtors7 + CEconj4, ijk_di);
+          add4(j, CEtors8 + CEconj5, jkl_di);
+          add4(k, CEtors8 + CEconj5, jkl_dj);
+          add4(l, CEtors8 + CEconj5, jkl_dk);
+          add4(i, CEtors9 + CEconj6, dw_i);
+          add4(j, CEtors9 + CEconj6, dw_j);
+          add4(k, CEtors9 + CEconj6, dw_k);
+          add4(l, CEtors9 + CEconj6, dw_l);
+        }
+      }
+    }
+  }
+  out[0] = e_tor_t;
+  out[1] = e_con_t;
+  return 0;
 }
 
 // Valence part of Merge_Torsion_Valence_Angles, host form

@@ -31,7 +31,7 @@ EXPORTS = [
     "swr_sparse_matvec_h", "swr_npair_full_bin_atomonly_build", "swr_sort_bonds", "swr_set_batch",
     "swr_new_atoms", "swr_batch_pack", "swr_vdw_coulomb_energy", "swr_hist_save", "swr_hist_load",
     "swr_hist_free", "swr_atom_energy", "swr_add_dbond_forces", "swr_hydrogen_bonds",
-    "swr_compute_forces", "swr_set_masses", "swr_valence_angles", "swr_fix_nve_initial_integrate", "swr_fix_nve_final_integrate",
+    "swr_compute_forces", "swr_set_masses", "swr_valence_angles", "swr_torsion_angles", "swr_fix_nve_initial_integrate", "swr_fix_nve_final_integrate",
 ]
 
 
@@ -75,7 +75,7 @@ def lib():
                    "swr_qeq_init_matvec",
                    "swr_calculate_q", "swr_pair_compute", "swr_sync", "swr_vdw_coulomb_energy",
                    "swr_atom_energy", "swr_add_dbond_forces", "swr_hydrogen_bonds", "swr_compute_forces",
-                   "swr_valence_angles"]:
+                   "swr_valence_angles", "swr_torsion_angles"]:
             getattr(L, nm).argtypes = [vp]
         L.swr_set_masses.argtypes = [vp, C.c_int, dp]
         L.swr_fix_nve_initial_integrate.argtypes = [vp, C.c_double, C.c_double]

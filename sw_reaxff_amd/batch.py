@@ -158,7 +158,7 @@ class Batch:
     def evaluate(self, nonbonded=False, bonded=False):
         """Neighbour list, QEq (per-system), far/bond/hbond lists, BO; with
         nonbonded the vdW + Coulomb energies and gradient, with bonded the
-        bond, atom, valence-angle and hydrogen-bond energies and the
+        bond, atom, valence-angle, torsion and hydrogen-bond energies and the
         bond-order chain rule of the gradient;
         returns max CG iterations (s, t)."""
         h = self.sys
@@ -171,6 +171,7 @@ class Batch:
         if bonded:
             _lib.check(h.L.swr_atom_energy(h.h), "swr_atom_energy")
             _lib.check(h.L.swr_valence_angles(h.h), "swr_valence_angles")
+            _lib.check(h.L.swr_torsion_angles(h.h), "swr_torsion_angles")
             _lib.check(h.L.swr_hydrogen_bonds(h.h), "swr_hydrogen_bonds")
             _lib.check(h.L.swr_add_dbond_forces(h.h), "swr_add_dbond_forces")
         return a.value, b.value
@@ -197,11 +198,12 @@ class Batch:
 
     def bonded_energies(self):
         """Per system, (e_bond, e_lp, e_ov, e_un, e_pol, e_hb, e_ang, e_pen,
-        e_coa) summed over its local atoms (after evaluate(bonded=True))."""
+        e_coa, e_tor, e_con) summed over its local atoms (after
+        evaluate(bonded=True))."""
         h = self.sys
         stride, n = h.scalar("bonded_stride"), h.scalar("n")
         e = np.vstack([h.get("bonded_eatom").reshape(5, stride)[:, :n], h.get("hbond_eatom")[None, :n],
-                       h.get("valence_eatom").reshape(3, -1)[:, :n]])
+                       h.get("valence_eatom").reshape(3, -1)[:, :n], h.get("torsion_eatom").reshape(2, -1)[:, :n]])
         P = self.packed
         return np.array([e[:, P.row_off[k]:P.row_off[k + 1]].sum(axis=1) for k in range(len(P.row_off) - 1)])
 

@@ -741,6 +741,267 @@ __global__ void __launch_bounds__(BT) k_va_fold(int N, unsigned long long *__res
   }
 }
 
+// ---------------------------------------------------------------- torsions
+// The torsion part of Merge_Torsion_Valence_Angles (host form
+// reaxc_torsion_angles_sw64.c:812-1408; Calculate_Omega :21-104).  One
+// thread per local atom j over its bonds j-k taken once per physical bond
+// (orig_id order, then position for images of one atom), the other bonds
+// j-i and the bonds k-l of k: torsion and conjugation energies; j's own bond
+// slots (Cdbopi and Cdbo of j-k, Cdbo of j-i) and row written directly; the
+// CdDelta and derivative terms of k, i, l and the Cdbo of k's slot k-l into
+// 64-bit fixed-point accumulators by integer atomics, folded by k_to_fold.
+struct ToArgs {
+  const DevParams *P;
+  const swr_atom_pack_t *atoms;
+  const int *bidx, *bend;
+  const swr_bond_t *bonds;
+  const double *BO, *BOpi, *bo_dboc;
+  double *Cdbo, *Cdbopi, *fcd, *part;
+  size_t stride;
+  unsigned long long *acc;    // [N][4]
+  unsigned long long *sacc;   // [bond slots]
+};
+
+// Calculate_Theta + Calculate_dCos_Theta of bonds a (first) and b from one
+// centre: di belongs to a's far atom, dj to the centre, dk to b's.
+__device__ __forceinline__ double theta_dcos(const swr_bond_t &a, const swr_bond_t &b, double *di, double *dj,
+                                             double *dk) {
+  const double dot = a.dvec[0] * b.dvec[0] + a.dvec[1] * b.dvec[1] + a.dvec[2] * b.dvec[2];
+  double c = dot / (a.d * b.d);
+  if (c > 1.0) c = 1.0;
+  if (c < -1.0) c = -1.0;
+  const double sa = a.d * a.d, sb = b.d * b.d;
+  const double inv = 1.0 / (a.d * b.d);
+  const double ci3 = dot * (inv * inv * inv);
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    di[q] = b.dvec[q] * inv - ci3 * sb * a.dvec[q];
+    dj[q] = -(b.dvec[q] + a.dvec[q]) * inv + ci3 * (sb * a.dvec[q] + sa * b.dvec[q]);
+    dk[q] = a.dvec[q] * inv - ci3 * sa * b.dvec[q];
+  }
+  return acos(c);
+}
+
+__device__ __forceinline__ double tan_inv(double s, double c) {
+  constexpr double MIN_SINE = 1e-10;   // reaxc_torsion_angles_sw64.c:12
+  if (s >= 0 && s <= MIN_SINE) return c / MIN_SINE;
+  if (s <= 0 && s >= -MIN_SINE) return c / -MIN_SINE;
+  return c / s;
+}
+
+__global__ void __launch_bounds__(BT) k_torsion(int n, ToArgs A) {
+  const int j = blockIdx.x * BT + threadIdx.x;
+  if (j >= n) return;
+  const DevParams *__restrict__ P = A.P;
+  const swr_atom_pack_t aj = A.atoms[j];
+  const int tj = aj.type;
+  const int b0 = A.bidx[j], b1 = A.bend[j];
+  double e_tor = 0.0, e_con = 0.0;
+  double fj[4] = {0.0, 0.0, 0.0, 0.0};
+  if (tj >= 0 && b1 > b0) {
+    constexpr double MIN_SINE = 1e-10;
+    const double thb_cut = P->thb_cut;
+    const double p_tor2 = P->p_tor2, p_tor3 = P->p_tor3, p_tor4 = P->p_tor4, p_cot2 = P->p_cot2;
+    const double Delta_j = A.bo_dboc[2 * (size_t)j + 1];
+    for (int pk = b0; pk < b1; ++pk) {
+      const double BO_jk = A.BO[pk];
+      if (!(BO_jk - thb_cut > 0.0)) continue;
+      const swr_bond_t bjk = A.bonds[pk];
+      const int k = bjk.nbr;
+      const int k0 = A.bidx[k], k1 = A.bend[k];
+      if (k1 <= k0) continue;
+      const swr_atom_pack_t ak = A.atoms[k];
+      if (aj.orig_id > ak.orig_id) continue;
+      if (aj.orig_id == ak.orig_id) {
+        if (ak.x[2] < aj.x[2]) continue;
+        if (ak.x[2] == aj.x[2] && ak.x[1] < aj.x[1]) continue;
+        if (ak.x[2] == aj.x[2] && ak.x[1] == aj.x[1] && ak.x[0] < aj.x[0]) continue;
+      }
+      const int tk = ak.type;
+      if (tk < 0) continue;
+      const int pj = bjk.sym_index;
+      const swr_bond_t bkj = A.bonds[pj];
+      const double BOA_jk = BO_jk - thb_cut;
+      const double Delta_k = A.bo_dboc[2 * (size_t)k + 1];
+      const double r_jk = bjk.d;
+      const double exp_tor2_jk = exp(-p_tor2 * BOA_jk);
+      const double exp_cot2_jk = exp(-p_cot2 * (BOA_jk - 1.5) * (BOA_jk - 1.5));
+      const double exp_tor3_DjDk = exp(-p_tor3 * (Delta_j + Delta_k));
+      const double exp_tor4_DjDk = exp(p_tor4 * (Delta_j + Delta_k));
+      const double exp_tor34_inv = 1.0 / (1.0 + exp_tor3_DjDk + exp_tor4_DjDk);
+      const double f11_DjDk = (2.0 + exp_tor3_DjDk) * exp_tor34_inv;
+      const double dfn11 = (-p_tor3 * exp_tor3_DjDk +
+                            (p_tor3 * exp_tor3_DjDk - p_tor4 * exp_tor4_DjDk) * (2.0 + exp_tor3_DjDk) * exp_tor34_inv) *
+                           exp_tor34_inv;
+      const double BOpi_jk = A.BOpi[2 * (size_t)pk];
+      double cdbopi_jk = 0.0, cdbo_jk = 0.0, cd_k = 0.0;
+      double fk[3] = {0.0, 0.0, 0.0};
+      for (int ph = b0; ph < b1; ++ph) {
+        if (ph == pk) continue;
+        const double BO_ij = A.BO[ph];
+        if (!(BO_ij > thb_cut)) continue;
+        const swr_bond_t bij = A.bonds[ph];
+        const int i = bij.nbr;
+        const int ti = A.atoms[i].type;
+        if (ti < 0) continue;
+        double ijk_di[3], ijk_dj[3], ijk_dk[3];   // of k, j, i
+        const double theta_ijk = theta_dcos(bjk, bij, ijk_di, ijk_dj, ijk_dk);
+        const double r_ij = bij.d;
+        const double BOA_ij = BO_ij - thb_cut;
+        double sin_ijk = sin(theta_ijk);
+        const double cos_ijk = cos(theta_ijk);
+        const double tan_ijk_i = tan_inv(sin_ijk, cos_ijk);
+        const double exp_tor2_ij = exp(-p_tor2 * BOA_ij);
+        const double exp_cot2_ij = exp(-p_cot2 * (BOA_ij - 1.5) * (BOA_ij - 1.5));
+        const double xi[3] = {A.atoms[i].x[0], A.atoms[i].x[1], A.atoms[i].x[2]};
+        double cdbo_ij = 0.0;
+        double fi[3] = {0.0, 0.0, 0.0};
+        for (int pw = k0; pw < k1; ++pw) {
+          if (pw == pj) continue;
+          const double BO_kl = A.BO[pw];
+          if (!(BO_kl > thb_cut) || !(BO_ij * BO_jk * BO_kl > thb_cut)) continue;
+          const swr_bond_t bkl = A.bonds[pw];
+          const int l = bkl.nbr;
+          if (i == l) continue;
+          const swr_atom_pack_t al = A.atoms[l];
+          if (al.type < 0) continue;
+          const DevTor &fb = P->tor[((ti * SWR_MAX_ELEM + tj) * SWR_MAX_ELEM + tk) * SWR_MAX_ELEM + al.type];
+          if (!fb.cnt) continue;
+          double jkl_di[3], jkl_dj[3], jkl_dk[3];   // of j, k, l
+          const double theta_jkl = theta_dcos(bkj, bkl, jkl_di, jkl_dj, jkl_dk);
+          const double r_kl = bkl.d;
+          const double BOA_kl = BO_kl - thb_cut;
+          double sin_jkl = sin(theta_jkl);
+          const double cos_jkl = cos(theta_jkl);
+          const double tan_jkl_i = tan_inv(sin_jkl, cos_jkl);
+          const double dvec_li[3] = {xi[0] - al.x[0], xi[1] - al.x[1], xi[2] - al.x[2]};
+          const double r_li = sqrt(dvec_li[0] * dvec_li[0] + dvec_li[1] * dvec_li[1] + dvec_li[2] * dvec_li[2]);
+          // Calculate_Omega
+          const double *dij = bij.dvec, *djk = bjk.dvec, *dkl = bkl.dvec;
+          const double d_ij_jk = dij[0] * djk[0] + dij[1] * djk[1] + dij[2] * djk[2];
+          const double d_jk_kl = djk[0] * dkl[0] + djk[1] * dkl[1] + djk[2] * dkl[2];
+          const double d_ij_kl = dij[0] * dkl[0] + dij[1] * dkl[1] + dij[2] * dkl[2];
+          const double cr[3] = {djk[1] * dkl[2] - djk[2] * dkl[1], djk[2] * dkl[0] - djk[0] * dkl[2],
+                                djk[0] * dkl[1] - djk[1] * dkl[0]};
+          const double uc = -d_ij_jk * d_jk_kl + r_jk * r_jk * d_ij_kl;
+          const double us = -r_jk * (dij[0] * cr[0] + dij[1] * cr[1] + dij[2] * cr[2]);
+          const double omega = atan2(us, uc);
+          const double htra = r_ij + cos_ijk * (r_kl * cos_jkl - r_jk);
+          const double htrb = r_jk - r_ij * cos_ijk - r_kl * cos_jkl;
+          const double htrc = r_kl + cos_jkl * (r_ij * cos_ijk - r_jk);
+          const double hthd = r_ij * sin_ijk * (r_jk - r_kl * cos_jkl);
+          const double hthe = r_kl * sin_jkl * (r_jk - r_ij * cos_ijk);
+          const double hnra = r_kl * sin_ijk * sin_jkl;
+          const double hnrc = r_ij * sin_ijk * sin_jkl;
+          const double hnhd = r_ij * r_kl * cos_ijk * sin_jkl;
+          const double hnhe = r_ij * r_kl * sin_ijk * cos_jkl;
+          double poem = 2.0 * r_ij * r_kl * sin_ijk * sin_jkl;
+          if (poem < 1e-20) poem = 1e-20;
+          const double tel = r_ij * r_ij + r_jk * r_jk + r_kl * r_kl - r_li * r_li -
+                             2.0 * (r_ij * r_jk * cos_ijk - r_ij * r_kl * cos_ijk * cos_jkl + r_jk * r_kl * cos_jkl);
+          double arg = tel / poem;
+          if (arg > 1.0) arg = 1.0;
+          if (arg < -1.0) arg = -1.0;
+          double s_ijk = sin_ijk, s_jkl = sin_jkl;   // clamped copies for the divisions
+          if (s_ijk >= 0 && s_ijk <= MIN_SINE) s_ijk = MIN_SINE;
+          else if (s_ijk <= 0 && s_ijk >= -MIN_SINE) s_ijk = -MIN_SINE;
+          if (s_jkl >= 0 && s_jkl <= MIN_SINE) s_jkl = MIN_SINE;
+          else if (s_jkl <= 0 && s_jkl >= -MIN_SINE) s_jkl = -MIN_SINE;
+          const double sc = 2.0 / poem;
+          const double a_ij = (htra - arg * hnra) / r_ij, a_kl = (htrc - arg * hnrc) / r_kl, a_jk = htrb / r_jk;
+          const double t_d = (hthd - arg * hnhd) / s_ijk, t_e = (hthe - arg * hnhe) / s_jkl;
+          // torsion and conjugation energies
+          const double cos_omega = cos(omega), cos2omega = cos(2. * omega), cos3omega = cos(3. * omega);
+          const double exp_tor1 = exp(fb.p_tor1 * (2.0 - BOpi_jk - f11_DjDk) * (2.0 - BOpi_jk - f11_DjDk));
+          const double exp_tor2_kl = exp(-p_tor2 * BOA_kl);
+          const double exp_cot2_kl = exp(-p_cot2 * (BOA_kl - 1.5) * (BOA_kl - 1.5));
+          const double fn10 = (1.0 - exp_tor2_ij) * (1.0 - exp_tor2_jk) * (1.0 - exp_tor2_kl);
+          const double CV = 0.5 * (fb.V1 * (1.0 + cos_omega) + fb.V2 * exp_tor1 * (1.0 - cos2omega) +
+                                   fb.V3 * (1.0 + cos3omega));
+          e_tor += fn10 * sin_ijk * sin_jkl * CV;
+          const double CEtors1 = sin_ijk * sin_jkl * CV;
+          const double CEtors2 = -fn10 * 2.0 * fb.p_tor1 * fb.V2 * exp_tor1 * (2.0 - BOpi_jk - f11_DjDk) *
+                                 (1.0 - cos_omega * cos_omega) * sin_ijk * sin_jkl;
+          const double CEtors3 = CEtors2 * dfn11;
+          const double CEtors4 = CEtors1 * p_tor2 * exp_tor2_ij * (1.0 - exp_tor2_jk) * (1.0 - exp_tor2_kl);
+          const double CEtors5 = CEtors1 * p_tor2 * (1.0 - exp_tor2_ij) * exp_tor2_jk * (1.0 - exp_tor2_kl);
+          const double CEtors6 = CEtors1 * p_tor2 * (1.0 - exp_tor2_ij) * (1.0 - exp_tor2_jk) * exp_tor2_kl;
+          const double cmn = -fn10 * CV;
+          const double CEtors7 = cmn * sin_jkl * tan_ijk_i;
+          const double CEtors8 = cmn * sin_ijk * tan_jkl_i;
+          const double CEtors9 = fn10 * sin_ijk * sin_jkl *
+                                 (0.5 * fb.V1 - 2.0 * fb.V2 * exp_tor1 * cos_omega +
+                                  1.5 * fb.V3 * (cos2omega + 2.0 * cos_omega * cos_omega));
+          const double fn12 = exp_cot2_ij * exp_cot2_jk * exp_cot2_kl;
+          const double conj = 1.0 + (cos_omega * cos_omega - 1.0) * sin_ijk * sin_jkl;
+          e_con += fb.p_cot1 * fn12 * conj;
+          const double Cconj = -2.0 * fn12 * fb.p_cot1 * p_cot2 * conj;
+          const double CEconj4 = -fb.p_cot1 * fn12 * (cos_omega * cos_omega - 1.0) * sin_jkl * tan_ijk_i;
+          const double CEconj5 = -fb.p_cot1 * fn12 * (cos_omega * cos_omega - 1.0) * sin_ijk * tan_jkl_i;
+          const double CEconj6 = 2.0 * fb.p_cot1 * fn12 * cos_omega * sin_ijk * sin_jkl;
+          cdbopi_jk += CEtors2;
+          fj[3] += CEtors3;
+          cd_k += CEtors3;
+          cdbo_ij += CEtors4 + Cconj * (BOA_ij - 1.5e0);
+          cdbo_jk += CEtors5 + Cconj * (BOA_jk - 1.5e0);
+          fix_add(A.sacc + pw, CEtors6 + Cconj * (BOA_kl - 1.5e0));
+          const double c7 = CEtors7 + CEconj4, c8 = CEtors8 + CEconj5, c9 = CEtors9 + CEconj6;
+          double fl[3];
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            const double w_i = sc * (a_ij * dij[q] - dvec_li[q] - t_d * ijk_dk[q]);
+            const double w_j = sc * (-a_ij * dij[q] - a_jk * djk[q] - t_d * ijk_dj[q] - t_e * jkl_di[q]);
+            const double w_k = sc * (-a_kl * dkl[q] + a_jk * djk[q] - t_d * ijk_di[q] - t_e * jkl_dj[q]);
+            const double w_l = sc * (a_kl * dkl[q] + dvec_li[q] - t_e * jkl_dk[q]);
+            fi[q] += c7 * ijk_dk[q] + c9 * w_i;
+            fj[q] += c7 * ijk_dj[q] + c8 * jkl_di[q] + c9 * w_j;
+            fk[q] += c7 * ijk_di[q] + c8 * jkl_dj[q] + c9 * w_k;
+            fl[q] = c8 * jkl_dk[q] + c9 * w_l;
+          }
+          unsigned long long *al4 = A.acc + 4 * (size_t)l;
+          fix_add(al4, fl[0]);
+          fix_add(al4 + 1, fl[1]);
+          fix_add(al4 + 2, fl[2]);
+        }
+        if (cdbo_ij != 0.0) A.Cdbo[ph] += cdbo_ij;
+        if (fi[0] != 0.0 || fi[1] != 0.0 || fi[2] != 0.0) {
+          unsigned long long *ai4 = A.acc + 4 * (size_t)i;
+          fix_add(ai4, fi[0]);
+          fix_add(ai4 + 1, fi[1]);
+          fix_add(ai4 + 2, fi[2]);
+        }
+      }
+      if (cdbopi_jk != 0.0) A.Cdbopi[pk] += cdbopi_jk;
+      if (cdbo_jk != 0.0) A.Cdbo[pk] += cdbo_jk;
+      if (cd_k != 0.0 || fk[0] != 0.0 || fk[1] != 0.0 || fk[2] != 0.0) {
+        unsigned long long *ak4 = A.acc + 4 * (size_t)k;
+        fix_add(ak4, fk[0]);
+        fix_add(ak4 + 1, fk[1]);
+        fix_add(ak4 + 2, fk[2]);
+        fix_add(ak4 + 3, cd_k);
+      }
+    }
+    A.fcd[4 * (size_t)j] += fj[0];
+    A.fcd[4 * (size_t)j + 1] += fj[1];
+    A.fcd[4 * (size_t)j + 2] += fj[2];
+    A.fcd[4 * (size_t)j + 3] += fj[3];
+  }
+  A.part[j] = e_tor;
+  A.part[A.stride + j] = e_con;
+}
+
+// Cdbo of the bond slots += their fixed-point terms; accumulators left at zero.
+__global__ void __launch_bounds__(BT) k_to_slot_fold(int nslots, unsigned long long *__restrict__ sacc,
+                                                     double *__restrict__ Cdbo) {
+  const int s = blockIdx.x * BT + threadIdx.x;
+  if (s >= nslots) return;
+  const long long v = (long long)sacc[s];
+  if (v != 0) {
+    Cdbo[s] += (double)v * (1.0 / HB_FIX);
+    sacc[s] = 0ull;
+  }
+}
+
 }  // namespace
 
 int swr_atom_energy_run(swr_system *S) {
@@ -900,6 +1161,54 @@ int swr_valence_angles_run(swr_system *S) {
   hipLaunchKernelGGL(k_sum_rows, dim3(3), dim3(1024), 0, S->stream, n, (const double *)S->va_part, stride,
                      S->va_result);
   swr_kt_end(S, KT_VALENCE);
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_torsion_angles_run(swr_system *S) {
+  const int n = S->n, N = S->N;
+  if (!S->bonds || !S->BO || !S->fcd || N <= 0) {
+    swr_set_error_msg("Torsion_Angles needs the bonds and bond orders of the step (swr_bo first)");
+    return SWR_ERR_STATE;
+  }
+  int rc;
+  const size_t stride = (size_t)std::max(n, 1);
+  const int nslots = S->bond_cap;
+  const size_t old_acc = S->cap_to_acc, old_sacc = S->cap_to_sacc;
+  if ((rc = swr_reserve(&S->to_part, S->cap_to_part, 2 * stride)) ||
+      (rc = swr_reserve(&S->to_result, S->cap_to_result, 4)) ||
+      (rc = swr_reserve(&S->to_acc, S->cap_to_acc, 4 * (size_t)N)) ||
+      (rc = swr_reserve(&S->to_sacc, S->cap_to_sacc, (size_t)std::max(nslots, 1))))
+    return rc;
+  if (S->cap_to_acc != old_acc)
+    HIP_CHECK(hipMemsetAsync(S->to_acc, 0, sizeof(unsigned long long) * S->cap_to_acc, S->stream));
+  if (S->cap_to_sacc != old_sacc)
+    HIP_CHECK(hipMemsetAsync(S->to_sacc, 0, sizeof(unsigned long long) * S->cap_to_sacc, S->stream));
+  S->to_stride = stride;
+  ToArgs A;
+  A.P = S->dp;
+  A.atoms = S->packed;
+  A.bidx = S->bond_index;
+  A.bend = S->bond_end;
+  A.bonds = S->bonds;
+  A.BO = S->BO;
+  A.BOpi = S->BOpi;
+  A.bo_dboc = S->bo_dboc;
+  A.Cdbo = S->Cdbo;
+  A.Cdbopi = S->Cdbopi;
+  A.fcd = S->fcd;
+  A.part = S->to_part;
+  A.stride = stride;
+  A.acc = S->to_acc;
+  A.sacc = S->to_sacc;
+  swr_kt_begin(S, KT_TORSION);
+  if (n > 0) hipLaunchKernelGGL(k_torsion, dim3(div_up(n, BT)), dim3(BT), 0, S->stream, n, A);
+  hipLaunchKernelGGL(k_va_fold, dim3(div_up(N, BT)), dim3(BT), 0, S->stream, N, S->to_acc, S->fcd);
+  if (nslots > 0)
+    hipLaunchKernelGGL(k_to_slot_fold, dim3(div_up(nslots, BT)), dim3(BT), 0, S->stream, nslots, S->to_sacc, S->Cdbo);
+  hipLaunchKernelGGL(k_sum_rows, dim3(2), dim3(1024), 0, S->stream, n, (const double *)S->to_part, stride,
+                     S->to_result);
+  swr_kt_end(S, KT_TORSION);
   HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }

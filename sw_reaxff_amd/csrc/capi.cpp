@@ -119,7 +119,17 @@ void fill_params(swr_system *S, const std::vector<int> &map) {
     h.p_pen2 = g(19); h.p_pen3 = g(20); h.p_pen4 = g(21);
     h.p_coa2 = g(2); h.p_coa3 = g(38); h.p_coa4 = g(30);
     h.thb_cut = S->ctl.thb_cut; h.thb_cutsq = S->ctl.thb_cutsq;
+    h.p_tor2 = g(23); h.p_tor3 = g(24); h.p_tor4 = g(25); h.p_cot2 = g(27);
   }
+  for (int a = 0; a < ff.ne() && a < SWR_MAX_ELEM; a++)
+    for (int b = 0; b < ff.ne() && b < SWR_MAX_ELEM; b++)
+      for (int c = 0; c < ff.ne() && c < SWR_MAX_ELEM; c++)
+        for (int d = 0; d < ff.ne() && d < SWR_MAX_ELEM; d++) {
+          DevTor &o = h.tor[((a * SWR_MAX_ELEM + b) * SWR_MAX_ELEM + c) * SWR_MAX_ELEM + d];
+          if (ff.fbp.empty()) { o.cnt = 0; continue; }
+          const FfTorsion &f = ff.fbp[ff.quad(a, b, c, d)];
+          o.cnt = f.cnt; o.V1 = f.V1; o.V2 = f.V2; o.V3 = f.V3; o.p_tor1 = f.p_tor1; o.p_cot1 = f.p_cot1;
+        }
   for (int e = 0; e < ff.ne() && e < SWR_MAX_ELEM; e++) {
     h.p_val3[e] = ff.atom[e].p_val3;
     h.p_val5[e] = ff.atom[e].p_val5;
@@ -365,7 +375,8 @@ int swr_destroy(swr_system *S) {
                   S->Hindex, S->bond_index, S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, S->Cdbo, S->Cdbopi,
                   S->Cdbopi2, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,
                   S->fcd, S->nb_partials, S->nb_result, S->nb_eatom, S->be_part, S->be_ce, S->be_bufc, S->be_result, S->hb_part,
-                  S->hb_result, S->hb_acc, S->va_part, S->va_result, S->va_acc, S->f, S->gcsr_cnt, S->gcsr_start, S->gcsr_list,
+                  S->hb_result, S->hb_acc, S->va_part, S->va_result, S->va_acc, S->to_part, S->to_result, S->to_acc,
+                  S->to_sacc, S->f, S->gcsr_cnt, S->gcsr_start, S->gcsr_list,
                   S->bond_mark2, S->counts, S->ws, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32,
                   S->Hdia_inv, S->b_s, S->st, S->s_hist, S->t_hist, S->cgv, S->cgs, S->partials, S->counters,
                   S->status, S->scan_tmp, S->tmp_i, S->mixw, S->cand, S->cand_count, S->row_sys, S->tile_beg,
@@ -696,6 +707,13 @@ int swr_valence_angles(swr_system *S) {
   return swr_valence_angles_run(S);
 }
 
+int swr_torsion_angles(swr_system *S) {
+  if (!S) return SWR_ERR_ARG;
+  int rc;
+  if ((rc = swr_quiesce(S))) return rc;
+  return swr_torsion_angles_run(S);
+}
+
 int swr_compute_forces(swr_system *S) {
   if (!S) return SWR_ERR_ARG;
   int rc;
@@ -807,6 +825,8 @@ bool lookup(swr_system *S, const std::string &k, View &v) {
   if (k == "f") return set(S->f, S->have_f ? 24 * n : 0);
   if (k == "valence_energy") return set(S->va_result, S->va_result ? 24 : 0);
   if (k == "valence_eatom") return set(S->va_part, S->va_part ? 24 * (int64_t)S->va_stride : 0);
+  if (k == "torsion_energy") return set(S->to_result, S->to_result ? 16 : 0);
+  if (k == "torsion_eatom") return set(S->to_part, S->to_part ? 16 * (int64_t)S->to_stride : 0);
   static const char *wsn[WS_COUNT] = {"Deltap", "Deltap_boc", "Delta", "Delta_e", "Delta_val", "vlpex", "nlp",
                                       "nlp_temp", "Clp", "Delta_lp", "dDelta_lp", "Delta_lp_temp", "dDelta_lp_temp"};
   for (int w = 0; w < WS_COUNT; w++)

@@ -182,13 +182,37 @@ std::string swr_read_ffield(const char *path, swr_ffield_t *ff) {
     ff->thb[ff->tri(a, b, c) * SWR_FF_MAX_THB + cnt] = g;
     ff->thb[ff->tri(c, b, a) * SWR_FF_MAX_THB + cnt] = g;
   }
-  // torsion block: skipped by its count; then the hydrogen-bond block
-  // (:654-686): r0_hb = -1 unless listed
+  // torsion block (:574-652): "i j k l" (all non-zero) sets (i,j,k,l) and
+  // (l,k,j,i) and marks them; "0 j k 0" gives every (p,j,k,o) and (o,k,j,p)
+  // a term, with these values where no explicit line set them
+  ff->fbp.assign((size_t)ne * ne * ne * ne, FfTorsion());
   {
+    std::vector<char> flag((size_t)ne * ne * ne * ne, 0);
     if (!L.next(t) || t.empty()) return "";
     const int l = std::atoi(t[0].c_str());
-    for (int k = 0; k < l; k++)
-      if (!L.next(t)) return "";
+    for (int k = 0; k < l; k++) {
+      if (!L.next(t)) return "truncated torsion block";
+      if (t.size() < 9) return "bad torsion line";
+      const int a = std::atoi(t[0].c_str()) - 1, b = std::atoi(t[1].c_str()) - 1, c = std::atoi(t[2].c_str()) - 1,
+                d = std::atoi(t[3].c_str()) - 1;
+      FfTorsion v;
+      v.cnt = 1; v.V1 = num(t, 4); v.V2 = num(t, 5); v.V3 = num(t, 6); v.p_tor1 = num(t, 7); v.p_cot1 = num(t, 8);
+      if (a >= 0 && d >= 0) {
+        if (a < ne && b < ne && c < ne && d < ne && b >= 0 && c >= 0) {
+          flag[ff->quad(a, b, c, d)] = flag[ff->quad(d, c, b, a)] = 1;
+          ff->fbp[ff->quad(a, b, c, d)] = v;
+          ff->fbp[ff->quad(d, c, b, a)] = v;
+        }
+      } else if (b >= 0 && c >= 0 && b < ne && c < ne) {
+        for (int p = 0; p < ne; p++)
+          for (int o = 0; o < ne; o++) {
+            FfTorsion &f1 = ff->fbp[ff->quad(p, b, c, o)], &f2 = ff->fbp[ff->quad(o, c, b, p)];
+            f1.cnt = f2.cnt = 1;
+            if (!flag[ff->quad(p, b, c, o)]) f1 = v;
+            if (!flag[ff->quad(o, c, b, p)]) f2 = v;
+          }
+      }
+    }
   }
   if (!L.next(t) || t.empty()) return "";
   const int nh = std::atoi(t[0].c_str());

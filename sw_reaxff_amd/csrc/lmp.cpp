@@ -9,7 +9,8 @@
 // and charges.  The harness command `forces yes` (not a LAMMPS command:
 // LAMMPS always computes forces) adds the energy terms built here (bond,
 // lone-pair, over-/under-coordination, valence angle, penalty, coalition,
-// hydrogen bonds, van der Waals, Coulomb, polarization; no torsions) and integrates with
+// torsion, four-body conjugation, hydrogen bonds, van der Waals, Coulomb,
+// polarization: the full ReaxFF energy of the reference) and integrates with
 // their forces, as FixNVESunway does; `mass T m` sets atom->mass.
 //
 // The style classes keep the reference's command surface and call order:
@@ -210,7 +211,8 @@ struct swr_lmp {
     if (forces) {
       // Compute_Bonded_Forces, Compute_NonBonded_Forces, Compute_Total_Force
       // (reaxc_forces_sunway.cpp), then the copy into atom->f + reverse comm
-      if ((rc = swr_atom_energy(sys)) || (rc = swr_valence_angles(sys)) || (rc = swr_hydrogen_bonds(sys)) ||
+      if ((rc = swr_atom_energy(sys)) || (rc = swr_valence_angles(sys)) || (rc = swr_torsion_angles(sys)) ||
+          (rc = swr_hydrogen_bonds(sys)) ||
           (rc = swr_vdw_coulomb_energy(sys)) ||
           (rc = swr_add_dbond_forces(sys)) || (rc = swr_compute_forces(sys)))
         return rc;
@@ -225,14 +227,15 @@ struct swr_lmp {
   int potential_energy(double *pe) {
     int rc;
     if ((rc = swr_sync(sys))) return rc;
-    double nb[8], be[5], va[3], hb = 0.0;
+    double nb[8], be[5], va[3], to[2], hb = 0.0;
     if (!forces || swr_get(sys, "nonbonded_energy", nb) != (int64_t)sizeof(nb) ||
         swr_get(sys, "bonded_energy", be) != (int64_t)sizeof(be) || swr_get(sys, "hbond_energy", &hb) != 8 ||
-        swr_get(sys, "valence_energy", va) != (int64_t)sizeof(va)) {
+        swr_get(sys, "valence_energy", va) != (int64_t)sizeof(va) ||
+        swr_get(sys, "torsion_energy", to) != (int64_t)sizeof(to)) {
       swr_set_error_msg("the potential energy needs `forces yes`");
       return SWR_ERR_STATE;
     }
-    *pe = nb[0] + nb[1] + be[0] + be[1] + be[2] + be[3] + be[4] + hb + va[0] + va[1] + va[2];
+    *pe = nb[0] + nb[1] + be[0] + be[1] + be[2] + be[3] + be[4] + hb + va[0] + va[1] + va[2] + to[0] + to[1];
     return SWR_OK;
   }
 

@@ -34,10 +34,17 @@ struct FfAngle {
 };
 constexpr int SWR_FF_MAX_THB = 5;   // parameter sets per (i, j, k) triple, REAX_MAX_3BODY_PARAM
 
+// Torsion parameters of one (i, j, k, l) quadruple (reaxc_ffield_sunway.cpp:574-652).
+struct FfTorsion {
+  int cnt = 0;   // 1: the quadruple has a torsion term
+  double V1 = 0, V2 = 0, V3 = 0, p_tor1 = 0, p_cot1 = 0;
+};
+
 struct swr_ffield_t {
   std::vector<double> gp;           // general parameters l[0..]
   std::vector<FfAngle> thb;         // [ne*ne*ne][SWR_FF_MAX_THB] (end atom i, centre j, end atom k)
   std::vector<int> thb_cnt;         // [ne*ne*ne]
+  std::vector<FfTorsion> fbp;       // [ne*ne*ne*ne]
   std::vector<FfHbond> hb;          // [ne*ne*ne] (acceptor i, hydrogen j, acceptor k)
   std::vector<FfAtom> atom;
   std::vector<FfPair> pair;         // [ne*ne]
@@ -48,6 +55,7 @@ struct swr_ffield_t {
   FfPair &tb(int i, int j) { return pair[(size_t)i * atom.size() + j]; }
   const FfHbond &hbp(int i, int j, int k) const { return hb[((size_t)i * atom.size() + j) * atom.size() + k]; }
   size_t tri(int i, int j, int k) const { return ((size_t)i * atom.size() + j) * atom.size() + k; }
+  size_t quad(int i, int j, int k, int l) const { return tri(i, j, k) * atom.size() + l; }
 };
 
 // Returns an empty string on success, else the reason.

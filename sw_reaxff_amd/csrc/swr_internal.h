@@ -67,6 +67,12 @@ struct DevHb {
 // Valence-angle parameter sets of one (end, centre, end) element triple,
 // only those with |p_val1| > 0.001 (the ones the energy uses), in file order.
 constexpr int SWR_MAX_THB = 5;
+
+// Torsion parameters of one (i, j, k, l) element quadruple; cnt 0: no term.
+struct DevTor {
+  int cnt;
+  double V1, V2, V3, p_tor1, p_cot1;
+};
 struct DevThb {
   double theta_00, p_val1, p_val2, p_coa1, p_val7, p_pen1, p_val4;
 };
@@ -121,6 +127,9 @@ struct DevParams {
   int thb_n[SWR_MAX_ELEM * SWR_MAX_ELEM * SWR_MAX_ELEM];
   int thb_pc[SWR_MAX_ELEM * SWR_MAX_ELEM * SWR_MAX_ELEM];   // 1: a set has p_pen1 or p_coa1 != 0
   DevThb thb[SWR_MAX_ELEM * SWR_MAX_ELEM * SWR_MAX_ELEM][SWR_MAX_THB];
+  // Torsions: general l[23,24,25,27] and the quadruple parameters
+  double p_tor2, p_tor3, p_tor4, p_cot2;
+  DevTor tor[SWR_MAX_ELEM * SWR_MAX_ELEM * SWR_MAX_ELEM * SWR_MAX_ELEM];
 };
 
 // Small block of device-written results the host reads back asynchronously.
@@ -162,7 +171,7 @@ struct CgScalars {
 // Kernel timer slots.
 enum { KT_VERLET = 0, KT_FAR, KT_BUILD_H, KT_CG_ITER, KT_CG_SETUP, KT_BO, KT_VALIDATE, KT_SPMV, KT_RESET,
        KT_QEQ_Q, KT_BINS, KT_MIXED_INNER, KT_MIXED_OUTER, KT_BONDS, KT_CG_SCALARS, KT_MARK, KT_CG_UPDATE, KT_NONBONDED, KT_ATOM_ENERGY,
-       KT_DBOND, KT_HBONDS, KT_VALENCE, KT_COUNT };
+       KT_DBOND, KT_HBONDS, KT_VALENCE, KT_TORSION, KT_COUNT };
 
 struct swr_system {
   // ---- host-side configuration
@@ -244,6 +253,11 @@ struct swr_system {
   double *va_part = nullptr, *va_result = nullptr;
   unsigned long long *va_acc = nullptr;
   size_t cap_va_part = 0, cap_va_result = 0, cap_va_acc = 0, va_stride = 0;
+  // torsions: per-atom energies [2][to_stride], totals, fixed-point terms of
+  // atoms [N][4] and of bond slots [bond_cap] (zero between calls)
+  double *to_part = nullptr, *to_result = nullptr;
+  unsigned long long *to_acc = nullptr, *to_sacc = nullptr;
+  size_t cap_to_part = 0, cap_to_result = 0, cap_to_acc = 0, cap_to_sacc = 0, to_stride = 0;
   // forces on local atoms (ghost rows of fCdDelta summed onto owners) and the
   // ghost-per-owner CSR that sums them in a fixed order
   double *f = nullptr;                     // [n][3]
@@ -368,6 +382,7 @@ int swr_atom_energy_run(swr_system *S);
 int swr_dbond_forces_run(swr_system *S);
 int swr_hydrogen_bonds_run(swr_system *S);
 int swr_valence_angles_run(swr_system *S);
+int swr_torsion_angles_run(swr_system *S);
 int swr_forces_run(swr_system *S);
 int swr_nve_run(swr_system *S, int stage, double dtv, double dtf);
 int swr_quiesce(swr_system *S);

@@ -139,7 +139,8 @@ int swr_scan_i32(swr_system *S, const int *in, int *out, int n, int *total_dev) 
 const char *const swr_kt_names[KT_COUNT] = {"verlet", "far", "build_H", "cg_iter", "cg_setup", "bo",
                                             "validate", "spmv", "reset", "qeq_q", "bins", "mixed_inner",
                                             "mixed_outer", "bonds", "cg_scalars", "mark", "cg_update", "nonbonded",
-                                            "atom_energy", "dbond", "hbonds", "valence"};
+                                            "atom_energy", "dbond", "hbonds", "valence",
+                                            "torsion"};
 
 // Records a start event for kernel slot k on the system stream.
 void swr_kt_begin(swr_system *S, int k) {

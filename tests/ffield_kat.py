@@ -91,7 +91,22 @@ class FField:
             if c != a:
                 self.angles.setdefault((c, b, a), []).append(prm)
             k += 1
-        k += 1 + int(lines[k].split()[0])
+        # torsions: "i j k l V1 V2 V3 p_tor1 p_cot1" (1-based); an explicit
+        # line serves (i,j,k,l) and (l,k,j,i), a "0 j k 0" line every end
+        # pair not given explicitly
+        nt4 = int(lines[k].split()[0])
+        k += 1
+        explicit, wild = {}, {}
+        for _ in range(nt4):
+            t = lines[k].split()
+            a, b, c, d = (int(v) - 1 for v in t[:4])
+            prm = dict(zip(["V1", "V2", "V3", "p_tor1", "p_cot1"], [float(v) for v in t[4:9]]))
+            if a >= 0 and d >= 0:
+                explicit[(a, b, c, d)] = explicit[(d, c, b, a)] = prm
+            else:
+                wild[(b, c)] = wild[(c, b)] = prm
+            k += 1
+        self.torsions_explicit, self.torsions_wild = explicit, wild
         nh = int(lines[k].split()[0])
         k += 1
         self.hbonds = {}
@@ -119,6 +134,11 @@ class FField:
     def angle(self, i: str, j: str, k: str):
         """Valence-angle parameter sets of end i, centre j, end k."""
         return self.angles.get((self.idx(i), self.idx(j), self.idx(k)), [])
+
+    def torsion(self, i: str, j: str, k: str, l: str):
+        """Torsion parameters of i-j-k-l (None if the quadruple has no term)."""
+        key = (self.idx(i), self.idx(j), self.idx(k), self.idx(l))
+        return self.torsions_explicit.get(key, self.torsions_wild.get(key[1:3]))
 
     def eta(self, el: str) -> float:
         return 2.0 * self.atoms[el]["eta_file"]

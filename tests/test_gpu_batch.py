@@ -84,7 +84,7 @@ def test_alone_equals_in_the_mix(gpu_lib, tol, qtol):
         dg = np.max(np.abs(g1 - g_mix[k])) / np.abs(g1).max()
         # bond, atom and hydrogen-bond energies do not depend on the charges (but for e_pol)
         eb1 = alone.bonded_energies()[0]
-        sel = [0, 1, 2, 3, 5, 6, 7, 8]
+        sel = [0, 1, 2, 3, 5, 6, 7, 8, 9, 10]
         deb = np.abs(eb1[sel] - eb_mix[k][sel]) / np.maximum(np.abs(eb1[sel]), 1e-3)
         dpol = abs(eb1[4] - eb_mix[k][4]) / max(abs(eb1[4]), 1.0)
         print(f"[batch] system {k}: E_nb {e1:.10e} rel diff {de:.2e}, bond/atom {eb1} rel diff {deb.max():.2e} "

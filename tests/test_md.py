@@ -4,8 +4,8 @@ Oracle pins (CPU):
   * the fold of ghost gradients onto owners (PairReaxCSunway::compute's
     f[k] += -fCdDelta[k] plus LAMMPS' reverse communication) reproduces minus
     the finite-difference derivative of a periodic water box's energy (bond,
-    lone-pair, over/under-coordination, valence-angle, hydrogen-bond, van der
-    Waals and Coulomb at fixed charges) with respect to a local atom's position, its
+    lone-pair, over/under-coordination, valence-angle, torsion, hydrogen-bond,
+    van der Waals and Coulomb at fixed charges) with respect to a local atom's position, its
     images moving with it;
   * NVE (fix_nve_sw64.c) is exact for a constant force (x = x0 + v0 t +
     a t^2 / 2) and time-reversible for a linear force.
@@ -79,7 +79,7 @@ def _periodic_oracle(oracle_lib, ffield, s, xl, ql, cutghost):
                 q=ql[owner])
     o.build_verlet(12.0)
     run_stages(o)
-    e = float(o.atom_energy()[:4].sum()) + float(o.valence_angles().sum()) + o.hydrogen_bonds()
+    e = float(o.atom_energy()[:4].sum()) + float(o.valence_angles().sum()) + float(o.torsion_angles().sum()) + o.hydrogen_bonds()
     ev, ec, _ = o.vdw_coulomb()
     return o, owner, e + ev + ec
 
@@ -163,6 +163,7 @@ def test_gpu_forces_match_oracle(gpu_lib, oracle_lib, ffield):
     run_stages(o)
     o.atom_energy()
     o.valence_angles()
+    o.torsion_angles()
     o.hydrogen_bonds()
     o.vdw_coulomb()
     o.dbond_forces()
@@ -218,7 +219,7 @@ def _oracle_md(oracle_lib, ffield, s, dt, nsteps, every, cutghost=10.0):
                     owner.astype(np.int32), q=np.zeros(owner.size))
         o.build_verlet(12.0)
         run_stages(o)
-        e = float(o.atom_energy()[:4].sum()) + float(o.valence_angles().sum()) + o.hydrogen_bonds()
+        e = float(o.atom_energy()[:4].sum()) + float(o.valence_angles().sum()) + float(o.torsion_angles().sum()) + o.hydrogen_bonds()
         ev, ec, _ = o.vdw_coulomb()
         o.dbond_forces()
         return e + ev + ec, po.forces(o.get("fCdDelta"), owner, s.n)

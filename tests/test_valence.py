@@ -14,6 +14,10 @@ Oracle pins (CPU):
     vanish.
 GPU parity: energies, Cdbo/Cdbopi/Cdbopi2 per bond, CdDelta and the
 gradient after the chain rule at 1e-10 relative (water box, clusters).
+
+Torsions (the torsion part of the same function): finite differences of
+e_tor + e_con (the same four systems), zero net force and torque, and the
+same GPU parity.
 """
 import math
 
@@ -219,3 +223,119 @@ def test_gpu_valence_cluster(gpu_lib, oracle_lib, ffield, f):
     s = compressed(systems.cluster(700, seed=9), f)
     lmp = harness(s, ffield, tol=1e-10)
     _gpu_vs_oracle(oracle_lib, ffield, lmp, s.elements, s.type, s.tag)
+
+
+# ---------------------------------------------------------------- torsions
+def _tor_energy(oracle_lib, ffield, s, x):
+    return float(_oracle(oracle_lib, ffield, s, x).torsion_angles().sum())
+
+
+@pytest.mark.parametrize("case", ["cluster30", "compressed", "nitro", "water"])
+def test_torsion_gradient_matches_finite_differences(oracle_lib, ffield, case):
+    """Torsion part of Merge_Torsion_Valence_Angles: finite differences of
+    e_tor + e_con against the gradient of the oracle's torsion pass plus the
+    bond-order chain rule."""
+    if case == "cluster30":
+        s = systems.cluster(30, seed=5)
+    elif case == "compressed":
+        s = compressed(systems.cluster(24, seed=8), 0.9)
+    elif case == "nitro":
+        s = nitromethane_propane(3)
+    else:
+        s = water_cluster(8, seed=2)
+    o = _oracle(oracle_lib, ffield, s)
+    e = o.torsion_angles()
+    o.dbond_forces()
+    g = o.get("fCdDelta").reshape(-1, 4)[: s.n, :3]
+    gmax = np.abs(g).max()
+    assert e[0] != 0.0
+    if case != "water":                   # water's quadruples have p_cot1 = 0
+        assert e[1] != 0.0
+    rng = np.random.default_rng(9)
+    h = 1e-5
+    worst = 0.0
+    for a in rng.choice(s.n, size=6, replace=False):
+        for k in range(3):
+            xp, xm = s.x.copy(), s.x.copy()
+            xp[a, k] += h
+            xm[a, k] -= h
+            fd = (_tor_energy(oracle_lib, ffield, s, xp) - _tor_energy(oracle_lib, ffield, s, xm)) / (2 * h)
+            worst = max(worst, abs(fd - g[a, k]) / gmax)
+    print(f"[torsion] {case} n={s.n} e_tor/e_con {e} max|g| {gmax:.3e} worst FD error {worst:.2e}")
+    assert worst <= 1e-6
+
+
+def test_torsion_gradient_has_no_net_force_or_torque(oracle_lib, ffield):
+    s = systems.cluster(60, seed=11)
+    o = _oracle(oracle_lib, ffield, s)
+    assert o.torsion_angles()[0] != 0.0
+    o.dbond_forces()
+    g = o.get("fCdDelta").reshape(-1, 4)[: s.n, :3]
+    scale = np.abs(g).max()
+    assert np.abs(g.sum(axis=0)).max() <= 1e-11 * scale * s.n
+    torque = np.cross(s.x - s.x.mean(axis=0), g).sum(axis=0)
+    assert np.abs(torque).max() <= 1e-10 * scale * s.n
+
+
+def _gpu_tor_vs_oracle(oracle_lib, ffield, lmp, elements, stype, stag):
+    from sw_reaxff_amd import _lib
+    h = lmp.system
+    _lib.check(h.L.swr_torsion_angles(h.h), "swr_torsion_angles")
+    h.sync()
+    cd_gpu = h.get("fCdDelta").reshape(-1, 4)[:, 3].copy()
+    cdbo_gpu = [h.get(k).copy() for k in ("Cdbo", "Cdbopi")]
+    gt_gpu = h.get("fCdDelta").reshape(-1, 4)[:, :3].copy()
+    _lib.check(h.L.swr_add_dbond_forces(h.h), "swr_add_dbond_forces")
+    h.sync()
+    e_gpu = h.get("torsion_energy")
+    ea = h.get("torsion_eatom")
+    g_gpu = h.get("fCdDelta").reshape(-1, 4)[:, :3]
+    n, N = h.scalar("n"), h.scalar("N")
+    xa = h.get("x").reshape(-1, 3)[:N]
+    owner = h.get("owner")[:N]
+    o = oracle_lib.Oracle(ffield, elements, threads=8)
+    o.set_atoms(n, xa, stype[owner].astype(np.int32), stag[owner].astype(np.int32), owner.astype(np.int32),
+                q=h.get("q")[:N])
+    o.build_verlet(12.0)
+    run_stages(o)
+    e_o = o.torsion_angles()
+    cd_o = o.get("fCdDelta").reshape(-1, 4)[:, 3].copy()
+    cdbo_o = [o.get(k).copy() for k in ("Cdbo", "Cdbopi")]
+    gt_o = o.get("fCdDelta").reshape(-1, 4)[:, :3].copy()
+    o.dbond_forces()
+    g_o = o.get("fCdDelta").reshape(-1, 4)[:, :3]
+    print(f"[torsion] n={n} N={N} gpu {e_gpu} oracle {e_o}", flush=True)
+    for k in range(2):
+        assert e_gpu[k] == pytest.approx(e_o[k], rel=1e-10, abs=1e-9), k
+    assert ea.reshape(2, -1)[:, :n].sum(axis=1) == pytest.approx(e_gpu, rel=1e-12, abs=1e-12)
+    kg = _bond_key(h.get("bond_index"), h.get("bond_end"), h.get("bonds")["nbr"], N)
+    ko = _bond_key(o.get("bond_index"), o.get("bond_end"), o.get("bonds")["nbr"], N)
+    assert kg.size == ko.size
+    # terms of atoms other than the centre, and the k-l slot's Cdbo, are
+    # summed in 64-bit fixed point (2^-40 resolution): an absolute floor of
+    # 1e-10 covers ~100 roundings of 2^-41 where the values themselves are
+    # tiny (water's torsion CdDelta is ~1e-8)
+    fix = 1e-10
+    for a, b in zip(cdbo_gpu, cdbo_o):
+        scale = np.abs(b[ko]).max()
+        assert np.max(np.abs(a[kg] - b[ko])) <= 1e-10 * scale + fix
+    assert np.max(np.abs(cd_gpu[:N] - cd_o[:N])) <= 1e-10 * np.abs(cd_o[:N]).max() + fix
+    assert np.max(np.abs(gt_gpu[:N] - gt_o[:N])) <= 1e-10 * np.abs(gt_o[:N]).max() + fix
+    assert np.max(np.abs(g_gpu[:N] - g_o[:N])) <= 1e-10 * np.abs(g_o[:N]).max()
+
+
+@pytest.mark.gpu
+def test_gpu_torsion_water_box(gpu_lib, oracle_lib, ffield):
+    from .parity import harness
+    s = systems.water_box(648, seed=3)
+    lmp = harness(s, ffield, tol=1e-10)
+    _gpu_tor_vs_oracle(oracle_lib, ffield, lmp, s.elements, s.type, s.tag)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("f", [1.0, 0.9])
+def test_gpu_torsion_cluster(gpu_lib, oracle_lib, ffield, f):
+    from .parity import harness
+    s = compressed(systems.cluster(700, seed=9), f)
+    lmp = harness(s, ffield, tol=1e-10)
+    _gpu_tor_vs_oracle(oracle_lib, ffield, lmp, s.elements, s.type, s.tag)
