@@ -743,13 +743,15 @@ __global__ void __launch_bounds__(BT) k_va_fold(int N, unsigned long long *__res
 
 // ---------------------------------------------------------------- torsions
 // The torsion part of Merge_Torsion_Valence_Angles (host form
-// reaxc_torsion_angles_sw64.c:812-1408; Calculate_Omega :21-104).  One
-// thread per local atom j over its bonds j-k taken once per physical bond
-// (orig_id order, then position for images of one atom), the other bonds
-// j-i and the bonds k-l of k: torsion and conjugation energies; j's own bond
-// slots (Cdbopi and Cdbo of j-k, Cdbo of j-i) and row written directly; the
-// CdDelta and derivative terms of k, i, l and the Cdbo of k's slot k-l into
-// 64-bit fixed-point accumulators by integer atomics, folded by k_to_fold.
+// reaxc_torsion_angles_sw64.c:812-1408; Calculate_Omega :21-104).  Four
+// lanes per local atom j, each over every fourth bond j-k of j (taken once
+// per physical bond: orig_id order, then position for images of one atom),
+// the other bonds j-i and the bonds k-l of k: torsion and conjugation
+// energies; the slot's own Cdbopi and Cdbo written directly; j's row and
+// energies summed over the lanes by a fixed shuffle tree; Cdbo of j-i and
+// k-l and the CdDelta and derivative terms of i, k, l into 64-bit
+// fixed-point accumulators by integer atomics, folded by k_va_fold /
+// k_to_slot_fold.
 struct ToArgs {
   const DevParams *P;
   const swr_atom_pack_t *atoms;
@@ -789,36 +791,39 @@ __device__ __forceinline__ double tan_inv(double s, double c) {
   return c / s;
 }
 
+constexpr int TO_LANES = 4;   // lanes per centre atom j, each over every 4th bond slot of j
+
 __global__ void __launch_bounds__(BT) k_torsion(int n, ToArgs A) {
-  const int j = blockIdx.x * BT + threadIdx.x;
-  if (j >= n) return;
+  const int tid = blockIdx.x * BT + threadIdx.x;
+  const int j = tid / TO_LANES, t = tid % TO_LANES;
   const DevParams *__restrict__ P = A.P;
-  const swr_atom_pack_t aj = A.atoms[j];
-  const int tj = aj.type;
-  const int b0 = A.bidx[j], b1 = A.bend[j];
   double e_tor = 0.0, e_con = 0.0;
   double fj[4] = {0.0, 0.0, 0.0, 0.0};
-  if (tj >= 0 && b1 > b0) {
+  // no early return: every lane reaches the group reduction below
+  const swr_atom_pack_t aj = A.atoms[j < n ? j : 0];
+  const int tj = j < n ? aj.type : -1;
+  const int b0 = j < n ? A.bidx[j] : 0, b1 = j < n ? A.bend[j] : 0;
+  if (tj >= 0) {
     constexpr double MIN_SINE = 1e-10;
     const double thb_cut = P->thb_cut;
     const double p_tor2 = P->p_tor2, p_tor3 = P->p_tor3, p_tor4 = P->p_tor4, p_cot2 = P->p_cot2;
     const double Delta_j = A.bo_dboc[2 * (size_t)j + 1];
-    for (int pk = b0; pk < b1; ++pk) {
+    for (int pk = b0 + t; pk < b1; pk += TO_LANES) do {
       const double BO_jk = A.BO[pk];
-      if (!(BO_jk - thb_cut > 0.0)) continue;
+      if (!(BO_jk - thb_cut > 0.0)) break;
       const swr_bond_t bjk = A.bonds[pk];
       const int k = bjk.nbr;
       const int k0 = A.bidx[k], k1 = A.bend[k];
-      if (k1 <= k0) continue;
+      if (k1 <= k0) break;
       const swr_atom_pack_t ak = A.atoms[k];
-      if (aj.orig_id > ak.orig_id) continue;
+      if (aj.orig_id > ak.orig_id) break;
       if (aj.orig_id == ak.orig_id) {
-        if (ak.x[2] < aj.x[2]) continue;
-        if (ak.x[2] == aj.x[2] && ak.x[1] < aj.x[1]) continue;
-        if (ak.x[2] == aj.x[2] && ak.x[1] == aj.x[1] && ak.x[0] < aj.x[0]) continue;
+        if (ak.x[2] < aj.x[2]) break;
+        if (ak.x[2] == aj.x[2] && ak.x[1] < aj.x[1]) break;
+        if (ak.x[2] == aj.x[2] && ak.x[1] == aj.x[1] && ak.x[0] < aj.x[0]) break;
       }
       const int tk = ak.type;
-      if (tk < 0) continue;
+      if (tk < 0) break;
       const int pj = bjk.sym_index;
       const swr_bond_t bkj = A.bonds[pj];
       const double BOA_jk = BO_jk - thb_cut;
@@ -963,7 +968,7 @@ __global__ void __launch_bounds__(BT) k_torsion(int n, ToArgs A) {
           fix_add(al4 + 1, fl[1]);
           fix_add(al4 + 2, fl[2]);
         }
-        if (cdbo_ij != 0.0) A.Cdbo[ph] += cdbo_ij;
+        if (cdbo_ij != 0.0) fix_add(A.sacc + ph, cdbo_ij);
         if (fi[0] != 0.0 || fi[1] != 0.0 || fi[2] != 0.0) {
           unsigned long long *ai4 = A.acc + 4 * (size_t)i;
           fix_add(ai4, fi[0]);
@@ -980,14 +985,25 @@ __global__ void __launch_bounds__(BT) k_torsion(int n, ToArgs A) {
         fix_add(ak4 + 2, fk[2]);
         fix_add(ak4 + 3, cd_k);
       }
-    }
-    A.fcd[4 * (size_t)j] += fj[0];
-    A.fcd[4 * (size_t)j + 1] += fj[1];
-    A.fcd[4 * (size_t)j + 2] += fj[2];
-    A.fcd[4 * (size_t)j + 3] += fj[3];
+    } while (false);
   }
-  A.part[j] = e_tor;
-  A.part[A.stride + j] = e_con;
+  // the group's lanes combined by a fixed xor tree (the same order on every
+  // run); lane 0 writes j's row and energies
+#pragma unroll
+  for (int m = 1; m < TO_LANES; m <<= 1) {
+    e_tor += __shfl_xor(e_tor, m, TO_LANES);
+    e_con += __shfl_xor(e_con, m, TO_LANES);
+#pragma unroll
+    for (int q = 0; q < 4; q++) fj[q] += __shfl_xor(fj[q], m, TO_LANES);
+  }
+  if (j < n && t == 0) {
+    if (tj >= 0) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) A.fcd[4 * (size_t)j + q] += fj[q];
+    }
+    A.part[j] = e_tor;
+    A.part[A.stride + j] = e_con;
+  }
 }
 
 // Cdbo of the bond slots += their fixed-point terms; accumulators left at zero.
@@ -1202,7 +1218,8 @@ int swr_torsion_angles_run(swr_system *S) {
   A.acc = S->to_acc;
   A.sacc = S->to_sacc;
   swr_kt_begin(S, KT_TORSION);
-  if (n > 0) hipLaunchKernelGGL(k_torsion, dim3(div_up(n, BT)), dim3(BT), 0, S->stream, n, A);
+  if (n > 0)
+    hipLaunchKernelGGL(k_torsion, dim3(div_up(n * TO_LANES, BT)), dim3(BT), 0, S->stream, n, A);
   hipLaunchKernelGGL(k_va_fold, dim3(div_up(N, BT)), dim3(BT), 0, S->stream, N, S->to_acc, S->fcd);
   if (nslots > 0)
     hipLaunchKernelGGL(k_to_slot_fold, dim3(div_up(nslots, BT)), dim3(BT), 0, S->stream, nslots, S->to_sacc, S->Cdbo);
