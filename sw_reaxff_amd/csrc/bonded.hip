@@ -849,15 +849,13 @@ __global__ void __launch_bounds__(BT) k_torsion(int n, ToArgs A) {
         const int i = bij.nbr;
         const int ti = A.atoms[i].type;
         if (ti < 0) continue;
+        // the i-j-k angle terms, evaluated at the first k-l bond that makes
+        // a torsion with this j-i bond (most j-i bonds make none)
         double ijk_di[3], ijk_dj[3], ijk_dk[3];   // of k, j, i
-        const double theta_ijk = theta_dcos(bjk, bij, ijk_di, ijk_dj, ijk_dk);
+        double theta_ijk = 0.0, sin_ijk = 0.0, cos_ijk = 0.0, tan_ijk_i = 0.0, exp_tor2_ij = 0.0, exp_cot2_ij = 0.0;
+        bool have_ijk = false;
         const double r_ij = bij.d;
         const double BOA_ij = BO_ij - thb_cut;
-        double sin_ijk = sin(theta_ijk);
-        const double cos_ijk = cos(theta_ijk);
-        const double tan_ijk_i = tan_inv(sin_ijk, cos_ijk);
-        const double exp_tor2_ij = exp(-p_tor2 * BOA_ij);
-        const double exp_cot2_ij = exp(-p_cot2 * (BOA_ij - 1.5) * (BOA_ij - 1.5));
         const double xi[3] = {A.atoms[i].x[0], A.atoms[i].x[1], A.atoms[i].x[2]};
         double cdbo_ij = 0.0;
         double fi[3] = {0.0, 0.0, 0.0};
@@ -872,6 +870,15 @@ __global__ void __launch_bounds__(BT) k_torsion(int n, ToArgs A) {
           if (al.type < 0) continue;
           const DevTor &fb = P->tor[((ti * SWR_MAX_ELEM + tj) * SWR_MAX_ELEM + tk) * SWR_MAX_ELEM + al.type];
           if (!fb.cnt) continue;
+          if (!have_ijk) {
+            theta_ijk = theta_dcos(bjk, bij, ijk_di, ijk_dj, ijk_dk);
+            sin_ijk = sin(theta_ijk);
+            cos_ijk = cos(theta_ijk);
+            tan_ijk_i = tan_inv(sin_ijk, cos_ijk);
+            exp_tor2_ij = exp(-p_tor2 * BOA_ij);
+            exp_cot2_ij = exp(-p_cot2 * (BOA_ij - 1.5) * (BOA_ij - 1.5));
+            have_ijk = true;
+          }
           double jkl_di[3], jkl_dj[3], jkl_dk[3];   // of j, k, l
           const double theta_jkl = theta_dcos(bkj, bkl, jkl_di, jkl_dj, jkl_dk);
           const double r_kl = bkl.d;
