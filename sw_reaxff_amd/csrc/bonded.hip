@@ -378,6 +378,7 @@ constexpr double HB_THRESHOLD = 1e-2;   // reaxc_defs_sunway.h:91
 constexpr double HB_FIX = 1099511627776.0;   // 2^40
 
 struct HbArgs {
+  unsigned int *flags;   // DevStatus::flags
   const DevParams *P;
   const swr_atom_pack_t *atoms;
   const int *bidx, *bend;
@@ -426,8 +427,16 @@ __device__ __forceinline__ void hb_triple(const DevHb &hbp, double BOij, const d
   t.ce3 = -hbp.p_hb3 * (-hbp.r0_hb / (r_jk * r_jk) + 1.0 / hbp.r0_hb) * t.e;
 }
 
-__device__ __forceinline__ void fix_add(unsigned long long *a, double v) {
-  atomicAdd(a, (unsigned long long)(long long)llrint(v * HB_FIX));
+// Adds v in units of 2^-40 to a fixed-point accumulator.  A term beyond
+// +-2^23 (or not finite) is not added: it raises SWR_F_FIXED_RANGE, which
+// the next swr_sync reports as SWR_ERR_NUMERIC, rather than wrapping.
+__device__ __forceinline__ void fix_add(unsigned int *flags, unsigned long long *a, double v) {
+  const double s = v * HB_FIX;
+  if (!(fabs(s) < 9.0e18)) {
+    atomicOr(flags, SWR_F_FIXED_RANGE);
+    return;
+  }
+  atomicAdd(a, (unsigned long long)(long long)llrint(s));
 }
 
 __global__ void __launch_bounds__(BT) k_hb(int n, HbArgs A) {
@@ -473,9 +482,9 @@ __global__ void __launch_bounds__(BT) k_hb(int n, HbArgs A) {
       g1 += t.ce2 * t.di[1];
       g2 += t.ce2 * t.di[2];
       unsigned long long *ak3 = A.acc + 3 * (size_t)k;
-      fix_add(ak3, t.ce2 * t.dk[0] + t.ce3 / r_jk * dvec_jk[0]);
-      fix_add(ak3 + 1, t.ce2 * t.dk[1] + t.ce3 / r_jk * dvec_jk[1]);
-      fix_add(ak3 + 2, t.ce2 * t.dk[2] + t.ce3 / r_jk * dvec_jk[2]);
+      fix_add(A.flags, ak3, t.ce2 * t.dk[0] + t.ce3 / r_jk * dvec_jk[0]);
+      fix_add(A.flags, ak3 + 1, t.ce2 * t.dk[1] + t.ce3 / r_jk * dvec_jk[1]);
+      fix_add(A.flags, ak3 + 2, t.ce2 * t.dk[2] + t.ce3 / r_jk * dvec_jk[2]);
     }
     ce1 = wave_sum(ce1);
     g0 = wave_sum(g0);
@@ -484,9 +493,9 @@ __global__ void __launch_bounds__(BT) k_hb(int n, HbArgs A) {
     if (lane == 0) {
       A.Cdbo[pi] += ce1;
       unsigned long long *ai3 = A.acc + 3 * (size_t)i;
-      fix_add(ai3, g0);
-      fix_add(ai3 + 1, g1);
-      fix_add(ai3 + 2, g2);
+      fix_add(A.flags, ai3, g0);
+      fix_add(A.flags, ai3 + 1, g1);
+      fix_add(A.flags, ai3 + 2, g2);
     }
   }
   e = wave_sum(e);
@@ -531,6 +540,7 @@ __global__ void __launch_bounds__(BT) k_hb_fold(int N, unsigned long long *__res
 constexpr double DEG2RAD_PI = 3.14159265;   // constPI, reaxc_defs_sunway.h:61 (DEG2RAD :55)
 
 struct VaArgs {
+  unsigned int *flags;   // DevStatus::flags
   const DevParams *P;
   const swr_atom_pack_t *atoms;
   const int *bidx, *bend;
@@ -702,8 +712,8 @@ __global__ void __launch_bounds__(BT) k_valence(int n, VaArgs A) {
         unsigned long long *ak = A.acc + 4 * (size_t)k, *ah = A.acc + 4 * (size_t)h;
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
-          fix_add(ak + d, ck[d]);
-          fix_add(ah + d, chh[d]);
+          fix_add(A.flags, ak + d, ck[d]);
+          fix_add(A.flags, ah + d, chh[d]);
         }
       }
     }
@@ -753,6 +763,7 @@ __global__ void __launch_bounds__(BT) k_va_fold(int N, unsigned long long *__res
 // fixed-point accumulators by integer atomics, folded by k_va_fold /
 // k_to_slot_fold.
 struct ToArgs {
+  unsigned int *flags;   // DevStatus::flags
   const DevParams *P;
   const swr_atom_pack_t *atoms;
   const int *bidx, *bend;
@@ -956,7 +967,7 @@ __global__ void __launch_bounds__(BT) k_torsion(int n, ToArgs A) {
           cd_k += CEtors3;
           cdbo_ij += CEtors4 + Cconj * (BOA_ij - 1.5e0);
           cdbo_jk += CEtors5 + Cconj * (BOA_jk - 1.5e0);
-          fix_add(A.sacc + pw, CEtors6 + Cconj * (BOA_kl - 1.5e0));
+          fix_add(A.flags, A.sacc + pw, CEtors6 + Cconj * (BOA_kl - 1.5e0));
           const double c7 = CEtors7 + CEconj4, c8 = CEtors8 + CEconj5, c9 = CEtors9 + CEconj6;
           double fl[3];
 #pragma unroll
@@ -971,26 +982,26 @@ __global__ void __launch_bounds__(BT) k_torsion(int n, ToArgs A) {
             fl[q] = c8 * jkl_dk[q] + c9 * w_l;
           }
           unsigned long long *al4 = A.acc + 4 * (size_t)l;
-          fix_add(al4, fl[0]);
-          fix_add(al4 + 1, fl[1]);
-          fix_add(al4 + 2, fl[2]);
+          fix_add(A.flags, al4, fl[0]);
+          fix_add(A.flags, al4 + 1, fl[1]);
+          fix_add(A.flags, al4 + 2, fl[2]);
         }
-        if (cdbo_ij != 0.0) fix_add(A.sacc + ph, cdbo_ij);
+        if (cdbo_ij != 0.0) fix_add(A.flags, A.sacc + ph, cdbo_ij);
         if (fi[0] != 0.0 || fi[1] != 0.0 || fi[2] != 0.0) {
           unsigned long long *ai4 = A.acc + 4 * (size_t)i;
-          fix_add(ai4, fi[0]);
-          fix_add(ai4 + 1, fi[1]);
-          fix_add(ai4 + 2, fi[2]);
+          fix_add(A.flags, ai4, fi[0]);
+          fix_add(A.flags, ai4 + 1, fi[1]);
+          fix_add(A.flags, ai4 + 2, fi[2]);
         }
       }
       if (cdbopi_jk != 0.0) A.Cdbopi[pk] += cdbopi_jk;
       if (cdbo_jk != 0.0) A.Cdbo[pk] += cdbo_jk;
       if (cd_k != 0.0 || fk[0] != 0.0 || fk[1] != 0.0 || fk[2] != 0.0) {
         unsigned long long *ak4 = A.acc + 4 * (size_t)k;
-        fix_add(ak4, fk[0]);
-        fix_add(ak4 + 1, fk[1]);
-        fix_add(ak4 + 2, fk[2]);
-        fix_add(ak4 + 3, cd_k);
+        fix_add(A.flags, ak4, fk[0]);
+        fix_add(A.flags, ak4 + 1, fk[1]);
+        fix_add(A.flags, ak4 + 2, fk[2]);
+        fix_add(A.flags, ak4 + 3, cd_k);
       }
     } while (false);
   }
@@ -1129,6 +1140,7 @@ int swr_hydrogen_bonds_run(swr_system *S) {
   A.fcd = S->fcd;
   A.part = S->hb_part;
   A.acc = S->hb_acc;
+  A.flags = &S->status->flags;
   constexpr int WPB = BT / SWR_WAVE;
   swr_kt_begin(S, KT_HBONDS);
   if (n > 0) hipLaunchKernelGGL(k_hb, dim3(div_up(n, WPB)), dim3(BT), 0, S->stream, n, A);
@@ -1178,6 +1190,7 @@ int swr_valence_angles_run(swr_system *S) {
   A.part = S->va_part;
   A.stride = stride;
   A.acc = S->va_acc;
+  A.flags = &S->status->flags;
   swr_kt_begin(S, KT_VALENCE);
   if (n > 0) hipLaunchKernelGGL(k_valence, dim3(div_up(n, BT)), dim3(BT), 0, S->stream, n, A);
   hipLaunchKernelGGL(k_va_fold, dim3(div_up(N, BT)), dim3(BT), 0, S->stream, N, S->va_acc, S->fcd);
@@ -1223,6 +1236,7 @@ int swr_torsion_angles_run(swr_system *S) {
   A.part = S->to_part;
   A.stride = stride;
   A.acc = S->to_acc;
+  A.flags = &S->status->flags;
   A.sacc = S->to_sacc;
   swr_kt_begin(S, KT_TORSION);
   if (n > 0)

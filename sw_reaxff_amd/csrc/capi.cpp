@@ -226,7 +226,7 @@ int status_to_rc(unsigned int f) {
   if (f & (SWR_F_BOND_CAP | SWR_F_BOND_TOTAL | SWR_F_BOND_LDS)) return SWR_ERR_BOND_CAP;
   if (f & (SWR_F_HBOND_CAP | SWR_F_HBOND_TOTAL)) return SWR_ERR_HBOND_CAP;
   if (f & SWR_F_BIN) return SWR_ERR_BIN;
-  if (f & SWR_F_NAN) return SWR_ERR_NUMERIC;
+  if (f & (SWR_F_NAN | SWR_F_FIXED_RANGE)) return SWR_ERR_NUMERIC;
   if (f & SWR_F_MARK_DEPTH) return SWR_ERR_STATE;
   return SWR_OK;
 }

@@ -29,6 +29,7 @@
 #define SWR_F_MARK_DEPTH (1u << 5)   // bond_mark relaxation did not converge
 #define SWR_F_BIN        (1u << 6)   // an atom fell outside the bin grid
 #define SWR_F_NAN        (1u << 7)   // non-finite value in CG scalars
+#define SWR_F_FIXED_RANGE (1u << 9)  // a bonded term beyond the fixed-point accumulators' range
 #define SWR_F_CG_MAXIT   (1u << 8)   // a QEq solve stopped at its iteration limit (a warning, as in the reference)
 
 #define SWR_MAX_ATOM_BONDS 32        // bonds one atom may hold while building its list (the reference: MAX_LEN 20, MAXBOND 35)
