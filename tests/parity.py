@@ -13,8 +13,8 @@ import numpy as np
 from sw_reaxff_amd import systems
 from sw_reaxff_amd.lammps import Lammps, standard_script
 
-# Tolerances: lists bit-exact as sets, bond orders
-# 1e-10 relative, charges 1e-6 e.
+# Parity tolerances of the path: lists bit-exact as sets, bond orders 1e-10
+# relative, charges 1e-6 e.
 BO_RTOL = 1e-10
 Q_ATOL = 1e-6
 
