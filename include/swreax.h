@@ -374,9 +374,9 @@ int swr_ktimer_read(swr_system *sys, const char *slot, double *ms, long long *co
  * Supplies what LAMMPS would around the path: box, ghost images (borders /
  * forward comm on the device), the reneighbour trigger and NVE.  Commands
  * follow LAMMPS syntax for the subset this path needs:
- *   pair_style reax/c/amd NULL [checkqeq yes|no] [safezone v] [mincap n] [maxfar n]
+ *   pair_style reax/c/sunway NULL [checkqeq yes|no] [safezone v] [mincap n] [maxfar n]
  *   pair_coeff * * <ffield> <element per type>
- *   fix <id> all qeq/reax/amd <nevery> <swa> <swb> <tol> reax/c [mixed]
+ *   fix <id> all qeq/reax/sunway <nevery> <swa> <swb> <tol> reax/c [mixed]
  *   neighbor <skin> bin
  *   neigh_modify every <k>
  *   timestep <dt_fs> */

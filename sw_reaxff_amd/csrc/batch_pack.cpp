@@ -86,6 +86,13 @@ extern "C" int swr_batch_pack(int nsys, const int *nat, const double *x, const i
     if (periodic[k]) {
       for (int d = 0; d < 3; d++)
         if (!(hk[d] > lk[d])) { swr_set_error_msg("swr_batch_pack: empty periodic box"); return SWR_ERR_ARG; }
+      // one image per side and dimension covers the ghost shell only when
+      // the box is at least cutghost long (LAMMPS then needs several swaps)
+      for (int d = 0; d < 3; d++)
+        if (hk[d] - lk[d] < cutghost) {
+          swr_set_error_msg("swr_batch_pack: periodic box shorter than the ghost cutoff");
+          return SWR_ERR_ARG;
+        }
       const Slabs B(lk, hk, cutghost);
       for (int i = 0; i < nat[k]; i++) {
         long long copies = 1;

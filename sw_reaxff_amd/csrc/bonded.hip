@@ -368,12 +368,10 @@ __global__ void __launch_bounds__(BT) k_dbond_nbr(int N, DbArgs A) {
 // over the atom's hydrogen-bond entries: the energy, CEhb1 and j's own
 // terms are summed by a fixed butterfly; the terms of i (summed over the
 // entries, one set per bond) and of k (summed over the bonds, one set per
-// entry) go into 64-bit fixed-point accumulators (2^-40 resolution) by
-// integer atomics, whose sums do not depend on the order they arrive in;
-// k_hb_fold adds them to fCdDelta and clears them.  The results are the same
-// on every run.  The accumulators hold +-2^23 kcal/mol/A per atom; one
-// triple's terms are bounded by p_hb1 times the angle derivative (a few
-// kcal/mol/A at hydrogen-bond distances), far inside that range.
+// entry) go into fixed-point accumulators (fix_add: 2^-40 resolution below
+// 2^20, sums up to 2^42) by integer atomics, whose sums do not depend on the
+// order they arrive in; k_hb_fold adds them to fCdDelta and clears them.
+// The results are the same on every run.
 constexpr double HB_THRESHOLD = 1e-2;   // reaxc_defs_sunway.h:91
 constexpr double HB_FIX = 1099511627776.0;   // 2^40
 
@@ -387,7 +385,8 @@ struct HbArgs {
   const int *Hindex, *hidx, *hend;
   const swr_hbond_t *hbonds;
   double *Cdbo, *fcd, *part;
-  unsigned long long *acc;   // [N][3] fixed-point terms of i and k
+  unsigned long long *acc;   // [2][N][3] fixed-point terms of i and k (fine, then coarse words)
+  size_t coff;               // 3N: offset of the coarse words
 };
 
 // One triple (Calculate_Theta, Calculate_dCos_Theta :336-372, energy and
@@ -427,16 +426,41 @@ __device__ __forceinline__ void hb_triple(const DevHb &hbp, double BOij, const d
   t.ce3 = -hbp.p_hb3 * (-hbp.r0_hb / (r_jk * r_jk) + 1.0 / hbp.r0_hb) * t.e;
 }
 
-// Adds v in units of 2^-40 to a fixed-point accumulator.  A term beyond
-// +-2^23 (or not finite) is not added: it raises SWR_F_FIXED_RANGE, which
-// the next swr_sync reports as SWR_ERR_NUMERIC, rather than wrapping.
-__device__ __forceinline__ void fix_add(unsigned int *flags, unsigned long long *a, double v) {
-  const double s = v * HB_FIX;
-  if (!(fabs(s) < 9.0e18)) {
-    atomicOr(flags, SWR_F_FIXED_RANGE);
-    return;
+// One fixed-point accumulator is a pair of 64-bit words: a fine word in
+// units of 2^-40 and a coarse word in units of 2^-20, coff elements after
+// it.  A term below 2^20 in magnitude goes to the fine word whole; a larger
+// one is split exactly into a multiple of 2^-20 (coarse word) and a
+// remainder below 2^-21 (fine word), so the pair holds sums up to 2^42 and
+// small terms keep the 2^-40 resolution.  Both words are integers, so the
+// sums do not depend on the order the adds arrive in.  A non-finite term,
+// a term beyond 2^42, or an add that wraps a word (detected from the value
+// atomicAdd returns) raises SWR_F_FIXED_RANGE, which the next swr_sync
+// reports as SWR_ERR_NUMERIC.
+constexpr double FIX_COARSE = 1048576.0;   // 2^20: coarse units and the split threshold
+
+__device__ __forceinline__ void fix_word_add(unsigned int *flags, unsigned long long *a, long long d) {
+  const long long old = (long long)atomicAdd(a, (unsigned long long)d);
+  const long long sum = (long long)((unsigned long long)old + (unsigned long long)d);
+  if (((old ^ sum) & (d ^ sum)) < 0) atomicOr(flags, SWR_F_FIXED_RANGE);   // signed wrap
+}
+
+__device__ __forceinline__ void fix_add(unsigned int *flags, unsigned long long *a, size_t coff, double v) {
+  if (!(fabs(v) < FIX_COARSE)) {
+    const double hi = rint(v * FIX_COARSE);
+    if (!(fabs(hi) < 4.0e18)) {   // not finite, or beyond 2^42
+      atomicOr(flags, SWR_F_FIXED_RANGE);
+      return;
+    }
+    fix_word_add(flags, a + coff, (long long)hi);
+    v -= hi * (1.0 / FIX_COARSE);   // exact: |v - hi 2^-20| <= 2^-21
   }
-  atomicAdd(a, (unsigned long long)(long long)llrint(s));
+  const long long f = llrint(v * HB_FIX);
+  if (f != 0) fix_word_add(flags, a, f);
+}
+
+// The value of the accumulator pair at a (coarse word coff elements on).
+__device__ __forceinline__ double fix_value(const unsigned long long *a, size_t coff) {
+  return (double)(long long)a[0] * (1.0 / HB_FIX) + (double)(long long)a[coff] * (1.0 / FIX_COARSE);
 }
 
 __global__ void __launch_bounds__(BT) k_hb(int n, HbArgs A) {
@@ -482,9 +506,9 @@ __global__ void __launch_bounds__(BT) k_hb(int n, HbArgs A) {
       g1 += t.ce2 * t.di[1];
       g2 += t.ce2 * t.di[2];
       unsigned long long *ak3 = A.acc + 3 * (size_t)k;
-      fix_add(A.flags, ak3, t.ce2 * t.dk[0] + t.ce3 / r_jk * dvec_jk[0]);
-      fix_add(A.flags, ak3 + 1, t.ce2 * t.dk[1] + t.ce3 / r_jk * dvec_jk[1]);
-      fix_add(A.flags, ak3 + 2, t.ce2 * t.dk[2] + t.ce3 / r_jk * dvec_jk[2]);
+      fix_add(A.flags, ak3, A.coff, t.ce2 * t.dk[0] + t.ce3 / r_jk * dvec_jk[0]);
+      fix_add(A.flags, ak3 + 1, A.coff, t.ce2 * t.dk[1] + t.ce3 / r_jk * dvec_jk[1]);
+      fix_add(A.flags, ak3 + 2, A.coff, t.ce2 * t.dk[2] + t.ce3 / r_jk * dvec_jk[2]);
     }
     ce1 = wave_sum(ce1);
     g0 = wave_sum(g0);
@@ -493,9 +517,9 @@ __global__ void __launch_bounds__(BT) k_hb(int n, HbArgs A) {
     if (lane == 0) {
       A.Cdbo[pi] += ce1;
       unsigned long long *ai3 = A.acc + 3 * (size_t)i;
-      fix_add(A.flags, ai3, g0);
-      fix_add(A.flags, ai3 + 1, g1);
-      fix_add(A.flags, ai3 + 2, g2);
+      fix_add(A.flags, ai3, A.coff, g0);
+      fix_add(A.flags, ai3 + 1, A.coff, g1);
+      fix_add(A.flags, ai3 + 2, A.coff, g2);
     }
   }
   e = wave_sum(e);
@@ -515,12 +539,14 @@ __global__ void __launch_bounds__(BT) k_hb_fold(int N, unsigned long long *__res
                                                 double *__restrict__ fcd) {
   const int a = blockIdx.x * BT + threadIdx.x;
   if (a >= N) return;
+  const size_t coff = 3 * (size_t)N;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    const long long v = (long long)acc[3 * (size_t)a + c];
-    if (v != 0) {
-      fcd[4 * (size_t)a + c] += (double)v * (1.0 / HB_FIX);
-      acc[3 * (size_t)a + c] = 0ull;
+    unsigned long long *w = acc + 3 * (size_t)a + c;
+    if (w[0] != 0ull || w[coff] != 0ull) {
+      fcd[4 * (size_t)a + c] += fix_value(w, coff);
+      w[0] = 0ull;
+      w[coff] = 0ull;
     }
   }
 }
@@ -549,7 +575,8 @@ struct VaArgs {
   const double *Delta, *Delta_val, *vlpex, *nlp, *dDelta_lp;
   double *Cdbo, *Cdbopi, *Cdbopi2, *fcd, *part;
   size_t stride;
-  unsigned long long *acc;   // [N][4]
+  unsigned long long *acc;   // [2][N][4] (fine, then coarse words)
+  size_t coff;               // 4N
 };
 
 __global__ void __launch_bounds__(BT) k_valence(int n, VaArgs A) {
@@ -712,8 +739,8 @@ __global__ void __launch_bounds__(BT) k_valence(int n, VaArgs A) {
         unsigned long long *ak = A.acc + 4 * (size_t)k, *ah = A.acc + 4 * (size_t)h;
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
-          fix_add(A.flags, ak + d, ck[d]);
-          fix_add(A.flags, ah + d, chh[d]);
+          fix_add(A.flags, ak + d, A.coff, ck[d]);
+          fix_add(A.flags, ah + d, A.coff, chh[d]);
         }
       }
     }
@@ -741,12 +768,14 @@ __global__ void __launch_bounds__(BT) k_va_fold(int N, unsigned long long *__res
                                                 double *__restrict__ fcd) {
   const int a = blockIdx.x * BT + threadIdx.x;
   if (a >= N) return;
+  const size_t coff = 4 * (size_t)N;
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    const long long v = (long long)acc[4 * (size_t)a + c];
-    if (v != 0) {
-      fcd[4 * (size_t)a + c] += (double)v * (1.0 / HB_FIX);
-      acc[4 * (size_t)a + c] = 0ull;
+    unsigned long long *w = acc + 4 * (size_t)a + c;
+    if (w[0] != 0ull || w[coff] != 0ull) {
+      fcd[4 * (size_t)a + c] += fix_value(w, coff);
+      w[0] = 0ull;
+      w[coff] = 0ull;
     }
   }
 }
@@ -771,8 +800,9 @@ struct ToArgs {
   const double *BO, *BOpi, *bo_dboc;
   double *Cdbo, *Cdbopi, *fcd, *part;
   size_t stride;
-  unsigned long long *acc;    // [N][4]
-  unsigned long long *sacc;   // [bond slots]
+  unsigned long long *acc;    // [2][N][4] (fine, then coarse words)
+  unsigned long long *sacc;   // [2][bond slots]
+  size_t coff, scoff;         // 4N, bond slots
 };
 
 // Calculate_Theta + Calculate_dCos_Theta of bonds a (first) and b from one
@@ -967,7 +997,7 @@ __global__ void __launch_bounds__(BT) k_torsion(int n, ToArgs A) {
           cd_k += CEtors3;
           cdbo_ij += CEtors4 + Cconj * (BOA_ij - 1.5e0);
           cdbo_jk += CEtors5 + Cconj * (BOA_jk - 1.5e0);
-          fix_add(A.flags, A.sacc + pw, CEtors6 + Cconj * (BOA_kl - 1.5e0));
+          fix_add(A.flags, A.sacc + pw, A.scoff, CEtors6 + Cconj * (BOA_kl - 1.5e0));
           const double c7 = CEtors7 + CEconj4, c8 = CEtors8 + CEconj5, c9 = CEtors9 + CEconj6;
           double fl[3];
 #pragma unroll
@@ -982,26 +1012,26 @@ __global__ void __launch_bounds__(BT) k_torsion(int n, ToArgs A) {
             fl[q] = c8 * jkl_dk[q] + c9 * w_l;
           }
           unsigned long long *al4 = A.acc + 4 * (size_t)l;
-          fix_add(A.flags, al4, fl[0]);
-          fix_add(A.flags, al4 + 1, fl[1]);
-          fix_add(A.flags, al4 + 2, fl[2]);
+          fix_add(A.flags, al4, A.coff, fl[0]);
+          fix_add(A.flags, al4 + 1, A.coff, fl[1]);
+          fix_add(A.flags, al4 + 2, A.coff, fl[2]);
         }
-        if (cdbo_ij != 0.0) fix_add(A.flags, A.sacc + ph, cdbo_ij);
+        if (cdbo_ij != 0.0) fix_add(A.flags, A.sacc + ph, A.scoff, cdbo_ij);
         if (fi[0] != 0.0 || fi[1] != 0.0 || fi[2] != 0.0) {
           unsigned long long *ai4 = A.acc + 4 * (size_t)i;
-          fix_add(A.flags, ai4, fi[0]);
-          fix_add(A.flags, ai4 + 1, fi[1]);
-          fix_add(A.flags, ai4 + 2, fi[2]);
+          fix_add(A.flags, ai4, A.coff, fi[0]);
+          fix_add(A.flags, ai4 + 1, A.coff, fi[1]);
+          fix_add(A.flags, ai4 + 2, A.coff, fi[2]);
         }
       }
       if (cdbopi_jk != 0.0) A.Cdbopi[pk] += cdbopi_jk;
       if (cdbo_jk != 0.0) A.Cdbo[pk] += cdbo_jk;
       if (cd_k != 0.0 || fk[0] != 0.0 || fk[1] != 0.0 || fk[2] != 0.0) {
         unsigned long long *ak4 = A.acc + 4 * (size_t)k;
-        fix_add(A.flags, ak4, fk[0]);
-        fix_add(A.flags, ak4 + 1, fk[1]);
-        fix_add(A.flags, ak4 + 2, fk[2]);
-        fix_add(A.flags, ak4 + 3, cd_k);
+        fix_add(A.flags, ak4, A.coff, fk[0]);
+        fix_add(A.flags, ak4 + 1, A.coff, fk[1]);
+        fix_add(A.flags, ak4 + 2, A.coff, fk[2]);
+        fix_add(A.flags, ak4 + 3, A.coff, cd_k);
       }
     } while (false);
   }
@@ -1029,10 +1059,11 @@ __global__ void __launch_bounds__(BT) k_to_slot_fold(int nslots, unsigned long l
                                                      double *__restrict__ Cdbo) {
   const int s = blockIdx.x * BT + threadIdx.x;
   if (s >= nslots) return;
-  const long long v = (long long)sacc[s];
-  if (v != 0) {
-    Cdbo[s] += (double)v * (1.0 / HB_FIX);
-    sacc[s] = 0ull;
+  unsigned long long *w = sacc + s;
+  if (w[0] != 0ull || w[nslots] != 0ull) {
+    Cdbo[s] += fix_value(w, (size_t)nslots);
+    w[0] = 0ull;
+    w[nslots] = 0ull;
   }
 }
 
@@ -1120,7 +1151,7 @@ int swr_hydrogen_bonds_run(swr_system *S) {
   const size_t old_acc = S->cap_hb_acc;
   if ((rc = swr_reserve(&S->hb_part, S->cap_hb_part, (size_t)std::max(n, 1))) ||
       (rc = swr_reserve(&S->hb_result, S->cap_hb_result, 1)) ||
-      (rc = swr_reserve(&S->hb_acc, S->cap_hb_acc, 3 * (size_t)N)))
+      (rc = swr_reserve(&S->hb_acc, S->cap_hb_acc, 6 * (size_t)N)))
     return rc;
   // a new accumulator starts at zero; afterwards k_hb_fold leaves it so
   if (S->cap_hb_acc != old_acc)
@@ -1140,6 +1171,7 @@ int swr_hydrogen_bonds_run(swr_system *S) {
   A.fcd = S->fcd;
   A.part = S->hb_part;
   A.acc = S->hb_acc;
+  A.coff = 3 * (size_t)N;
   A.flags = &S->status->flags;
   constexpr int WPB = BT / SWR_WAVE;
   swr_kt_begin(S, KT_HBONDS);
@@ -1163,7 +1195,7 @@ int swr_valence_angles_run(swr_system *S) {
   const size_t old_acc = S->cap_va_acc;
   if ((rc = swr_reserve(&S->va_part, S->cap_va_part, 3 * stride)) ||
       (rc = swr_reserve(&S->va_result, S->cap_va_result, 4)) ||
-      (rc = swr_reserve(&S->va_acc, S->cap_va_acc, 4 * (size_t)N)))
+      (rc = swr_reserve(&S->va_acc, S->cap_va_acc, 8 * (size_t)N)))
     return rc;
   if (S->cap_va_acc != old_acc)
     HIP_CHECK(hipMemsetAsync(S->va_acc, 0, sizeof(unsigned long long) * S->cap_va_acc, S->stream));
@@ -1190,6 +1222,7 @@ int swr_valence_angles_run(swr_system *S) {
   A.part = S->va_part;
   A.stride = stride;
   A.acc = S->va_acc;
+  A.coff = 4 * (size_t)N;
   A.flags = &S->status->flags;
   swr_kt_begin(S, KT_VALENCE);
   if (n > 0) hipLaunchKernelGGL(k_valence, dim3(div_up(n, BT)), dim3(BT), 0, S->stream, n, A);
@@ -1213,8 +1246,8 @@ int swr_torsion_angles_run(swr_system *S) {
   const size_t old_acc = S->cap_to_acc, old_sacc = S->cap_to_sacc;
   if ((rc = swr_reserve(&S->to_part, S->cap_to_part, 2 * stride)) ||
       (rc = swr_reserve(&S->to_result, S->cap_to_result, 4)) ||
-      (rc = swr_reserve(&S->to_acc, S->cap_to_acc, 4 * (size_t)N)) ||
-      (rc = swr_reserve(&S->to_sacc, S->cap_to_sacc, (size_t)std::max(nslots, 1))))
+      (rc = swr_reserve(&S->to_acc, S->cap_to_acc, 8 * (size_t)N)) ||
+      (rc = swr_reserve(&S->to_sacc, S->cap_to_sacc, 2 * (size_t)std::max(nslots, 1))))
     return rc;
   if (S->cap_to_acc != old_acc)
     HIP_CHECK(hipMemsetAsync(S->to_acc, 0, sizeof(unsigned long long) * S->cap_to_acc, S->stream));
@@ -1236,8 +1269,10 @@ int swr_torsion_angles_run(swr_system *S) {
   A.part = S->to_part;
   A.stride = stride;
   A.acc = S->to_acc;
+  A.coff = 4 * (size_t)N;
   A.flags = &S->status->flags;
   A.sacc = S->to_sacc;
+  A.scoff = (size_t)std::max(nslots, 1);
   swr_kt_begin(S, KT_TORSION);
   if (n > 0)
     hipLaunchKernelGGL(k_torsion, dim3(div_up(n * TO_LANES, BT)), dim3(BT), 0, S->stream, n, A);

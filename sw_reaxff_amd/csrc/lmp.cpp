@@ -14,8 +14,11 @@
 // their forces, as FixNVESunway does; `mass T m` sets atom->mass.
 //
 // The style classes keep the reference's command surface and call order:
-//   PairReaxCAMD   <-> PairReaxCSunway   (pair_reaxc_sunway.cpp)
-//   FixQEqReaxAMD  <-> FixQEqReaxSunway  (fix_qeq_reax_sunway.cpp)
+//   PairReaxC  = pair_style reax/c/sunway  (PairReaxCSunway, pair_reaxc_sunway.h:26)
+//   FixQEqReax = fix qeq/reax/sunway       (FixQEqReaxSunway, fix_qeq_reax_sunway.h:26)
+// The un-suffixed names reax/c and qeq/reax resolve to them, as they do in
+// LAMMPS under `-sf sunway` / `suffix sunway` (run.sh:2); reax/c/amd and
+// qeq/reax/amd are accepted as aliases.
 // One step: initial_integrate, neighbor (pbc + borders + list build every k
 // steps, else forward comm), FixQEqReax::pre_force, PairReaxC::compute.
 #include <hip/hip_runtime.h>
@@ -33,6 +36,12 @@
 
 namespace {
 
+// A style name as LAMMPS resolves it: the base name, its /sunway variant
+// (the kept surface), or the /amd alias.
+bool is_style(const std::string &name, const std::string &base) {
+  return name == base || name == base + "/sunway" || name == base + "/amd";
+}
+
 std::vector<std::string> split(const std::string &s) {
   std::istringstream is(s);
   std::vector<std::string> t;
@@ -41,16 +50,16 @@ std::vector<std::string> split(const std::string &s) {
   return t;
 }
 
-// pair_style reax/c/amd <NULL> [keywords]  (PairReaxCSunway::settings,
+// pair_style reax/c/sunway <NULL> [keywords]  (PairReaxCSunway::settings,
 // pair_reaxc_sunway.cpp:202-290) and pair_coeff * * ffield el... (:294-362).
-struct PairReaxCAMD {
+struct PairReaxC {
   swr_control_t ctl;
   bool have_style = false, have_coeff = false;
   int qeqflag = 1;
   std::string ffield;
   std::vector<std::string> elements;
 
-  PairReaxCAMD() { swr_control_default(&ctl); }
+  PairReaxC() { swr_control_default(&ctl); }
 
   std::string settings(const std::vector<std::string> &a) {
     if (a.empty()) return "Illegal pair_style command";
@@ -95,9 +104,9 @@ struct PairReaxCAMD {
   }
 };
 
-// fix ID group qeq/reax/amd nevery swa swb tol reax/c [mixed]
+// fix ID group qeq/reax/sunway nevery swa swb tol reax/c [mixed]
 // (FixQEqReaxSunway ctor, fix_qeq_reax_sunway.cpp:76-99).
-struct FixQEqReaxAMD {
+struct FixQEqReax {
   bool present = false;
   int nevery = 1;
   double swa = 0.0, swb = 10.0, tol = 1e-6;
@@ -130,8 +139,8 @@ const char *kStageNames[ST_COUNT] = {"integrate", "neighbor", "comm",     "qeq_H
 }  // namespace
 
 struct swr_lmp {
-  PairReaxCAMD pair;
-  FixQEqReaxAMD qeq;
+  PairReaxC pair;
+  FixQEqReax qeq;
   double skin = 2.0;
   int every = 10;
   double dt = 0.25;
@@ -344,13 +353,13 @@ int swr_lmp_command(swr_lmp *L, const char *cmd) {
   const std::string &c = t[0];
   std::vector<std::string> a(t.begin() + 1, t.end());
   if (c == "pair_style") {
-    if (a.empty() || (a[0] != "reax/c/amd" && a[0] != "reax/c")) err = "Unknown pair style";
+    if (a.empty() || !is_style(a[0], "reax/c")) err = "Unknown pair style";
     else err = L->pair.settings(std::vector<std::string>(a.begin() + 1, a.end()));
   } else if (c == "pair_coeff") {
     err = L->pair.coeff(a);
   } else if (c == "fix") {
-    if (a.size() >= 3 && (a[2] == "qeq/reax/amd" || a[2] == "qeq/reax")) err = L->qeq.parse(a);
-    else if (a.size() >= 3 && a[2] == "nve") err = "";
+    if (a.size() >= 3 && is_style(a[2], "qeq/reax")) err = L->qeq.parse(a);
+    else if (a.size() >= 3 && is_style(a[2], "nve")) err = "";
     else err = "Unknown fix style";
   } else if (c == "neighbor") {
     if (a.size() != 2 || a[1] != "bin") err = "Illegal neighbor command";
@@ -375,6 +384,10 @@ int swr_lmp_command(swr_lmp *L, const char *cmd) {
       if ((int)L->mass_set.size() <= t) L->mass_set.resize(t + 1, 0.0);
       L->mass_set[t] = m;
     }
+  } else if (c == "suffix") {
+    // `suffix sunway` (the -sf sunway of run.sh:2): the un-suffixed style
+    // names already resolve to the sunway styles here
+    if (a.size() != 1) err = "Illegal suffix command";
   } else if (c == "profile") {
     L->profile = (a.size() == 1 && a[0] == "yes") ? 1 : 0;
   } else if (c == "units" || c == "atom_style") {

@@ -310,6 +310,13 @@ int swr_nbr_ensure_atoms(swr_system *S, int Ncap);
 
 // Builds ghost images of the n local atoms (already remapped into the box).
 int swr_nbr_borders(swr_system *S, const double lo[3], const double hi[3], double cutghost) {
+  // one image per side and dimension (a single swap each way) covers the
+  // ghost shell only when every box length is at least cutghost
+  for (int d = 0; d < 3; d++)
+    if (!(hi[d] - lo[d] >= cutghost)) {
+      swr_set_error_msg("periodic box shorter than the ghost cutoff in a dimension");
+      return SWR_ERR_ARG;
+    }
   S->next_totals_known = false;  // new ghosts take their owners' capacities
   S->gcsr_valid = false;
   const int n = S->n;

@@ -2,9 +2,9 @@
 
     lmp = Lammps()
     lmp.commands('''
-        pair_style reax/c/amd NULL
+        pair_style reax/c/sunway NULL
         pair_coeff * * ffield.reax H O
-        fix 2 all qeq/reax/amd 1 0.0 10.0 1.0e-6 reax/c
+        fix 2 all qeq/reax/sunway 1 0.0 10.0 1.0e-6 reax/c
         neighbor 2.0 bin
         neigh_modify every 10
         timestep 0.25
@@ -12,9 +12,11 @@
     lmp.create_atoms(system)      # a sw_reaxff_amd.systems.System
     lmp.run(100)
 
-Command names and arguments follow `pair_style reax/c/sunway` and
-`fix qeq/reax/sunway` (pair_reaxc_sunway.cpp:202-362,
-fix_qeq_reax_sunway.cpp:76-99).
+Command names and arguments are those of `pair_style reax/c/sunway` and
+`fix qeq/reax/sunway` (PairStyle pair_reaxc_sunway.h:26, settings
+pair_reaxc_sunway.cpp:202-362; FixStyle fix_qeq_reax_sunway.h:26, arguments
+fix_qeq_reax_sunway.cpp:76-99); `reax/c` and `qeq/reax` resolve to them as
+under `-sf sunway`.
 """
 from __future__ import annotations
 
@@ -98,9 +100,9 @@ def standard_script(elements, ffield=DEFAULT_FFIELD, skin=2.0, every=10, dt=0.25
     return "\n".join([
         "units real",
         "atom_style charge",
-        "pair_style reax/c/amd NULL",
+        "pair_style reax/c/sunway NULL",
         f"pair_coeff * * {ffield} {' '.join(elements)}",
-        f"fix 2 all qeq/reax/amd 1 0.0 10.0 {tol:g} reax/c" + (" mixed" if mixed else ""),
+        f"fix 2 all qeq/reax/sunway 1 0.0 10.0 {tol:g} reax/c" + (" mixed" if mixed else ""),
         f"neighbor {skin} bin",
         f"neigh_modify every {every}",
         f"timestep {dt}",

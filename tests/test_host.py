@@ -10,6 +10,8 @@ import pytest
 
 from sw_reaxff_amd import systems
 
+from .conftest import FFIELD
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "sw_reaxff_amd", "libswreax.so")
 
@@ -181,6 +183,31 @@ def test_command_errors(lmp, cmd, msg):
     assert msg in str(ei.value)
 
 
+@pytest.mark.parametrize("pair,fix", [
+    ("reax/c/sunway", "qeq/reax/sunway"),      # PairStyle pair_reaxc_sunway.h:26, FixStyle fix_qeq_reax_sunway.h:26
+    ("reax/c", "qeq/reax"),                    # the names a -sf sunway input uses
+])
+def test_sunway_style_names(built_lib, pair, fix):
+    """The reference's command surface: its style names and the suffix
+    command are accepted with the same arguments and errors."""
+    from sw_reaxff_amd._lib import SwrError
+    from sw_reaxff_amd.lammps import Lammps, standard_script
+    L = Lammps()
+    L.command("suffix sunway")
+    L.command(f"pair_style {pair} NULL checkqeq yes safezone 1.6 mincap 100")
+    L.command(f"pair_coeff * * {FFIELD} H O")
+    L.command(f"fix 2 all {fix} 1 0.0 10.0 1e-6 reax/c")
+    L.command("fix 1 all nve/sunway")
+    with pytest.raises(SwrError) as ei:
+        L.command(f"fix 3 all {fix} 1 0.0 10.0 1e-6 reax/c dual")
+    assert "Dual keyword" in str(ei.value)
+    with pytest.raises(SwrError) as ei:
+        L.command(f"pair_style {pair}/omp NULL")
+    assert "Unknown pair style" in str(ei.value)
+    script = standard_script(["H", "O"])
+    assert "pair_style reax/c/sunway NULL" in script and "qeq/reax/sunway" in script
+
+
 def test_setup_without_atoms_is_an_error(lmp):
     from sw_reaxff_amd._lib import SwrError
     from sw_reaxff_amd.lammps import standard_script
@@ -255,3 +282,13 @@ def test_batch_pack_rejects_bad_input(built_lib):
                           hi.ctypes.data, per.ctypes.data, C.c_double(10.0), N.value - 1, out.ctypes.data,
                           None, None, None, None, None, None, None, C.byref(N))
     assert rc != 0                                               # capacity below N
+    hi = np.array([20.0, 20.0, 9.5])                             # L_z below the 10 A ghost cut
+    rc = L.swr_batch_pack(1, nat.ctypes.data, x.ctypes.data, t.ctypes.data, t.ctypes.data, lo.ctypes.data,
+                          hi.ctypes.data, per.ctypes.data, C.c_double(10.0), 0, None, None, None, None, None,
+                          None, None, None, C.byref(N))
+    assert rc == 1 and "shorter than the ghost cutoff" in L.swr_last_error().decode()
+    per0 = np.zeros(1, np.int32)                                 # the same box, not periodic: no images
+    rc = L.swr_batch_pack(1, nat.ctypes.data, x.ctypes.data, t.ctypes.data, t.ctypes.data, lo.ctypes.data,
+                          hi.ctypes.data, per0.ctypes.data, C.c_double(10.0), 0, None, None, None, None, None,
+                          None, None, None, C.byref(N))
+    assert rc == 0 and N.value == 3

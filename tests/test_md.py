@@ -132,7 +132,7 @@ def _harness(s, ffield, tol, extra=(), dt=0.25, qeq=True):
     script = standard_script(s.elements, ffield=ffield, tol=tol, dt=dt, forces=True)
     if not qeq:
         script = "\n".join(ln for ln in script.split("\n") if "qeq" not in ln).replace(
-            "pair_style reax/c/amd NULL", "pair_style reax/c/amd NULL checkqeq no")
+            "pair_style reax/c/sunway NULL", "pair_style reax/c/sunway NULL checkqeq no")
     lmp.commands(script)
     for c in extra:
         lmp.command(c)
