@@ -131,60 +131,104 @@ def pmc_traffic(kernel_slot, config):
     return None
 
 
-def cpu_baseline(sysh, elements, ffield, lmp_dt, every, budget_s):
-    """Times the FP64 oracle on this host on the GPU's state, moving atoms like the GPU loop."""
-    from oracle import pyoracle  # test-infrastructure checker, only for this baseline leg
-    if not os.path.exists(pyoracle.LIB_PATH):
-        import subprocess
-        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, stdout=subprocess.DEVNULL)
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    n, N = sysh.scalar("n"), sysh.scalar("N")
-    x = sysh.get("x").reshape(-1, 3).copy()
-    typ, tag, owner = sysh.get("type"), sysh.get("tag"), sysh.get("owner")
-    img = sysh.get("img").reshape(-1, 3).astype(np.float64)
-    v = sysh.get("v").reshape(-1, 3)
-    lo_hi = None
-    o = pyoracle.Oracle(ffield, elements, threads=threads)
-    o.set_atoms(n, x, typ, tag, owner, q=sysh.get("q"))
-    o.set("s_hist", sysh.get("s_hist"))
-    o.set("t_hist", sysh.get("t_hist"))
-    o.set("num_bonds", sysh.get("num_bonds"))
-    o.set("num_hbonds", sysh.get("num_hbonds"))
-    # box lengths from the ghost images
-    gh = np.nonzero(np.abs(img).sum(1) > 0)[0]
-    L = np.zeros(3)
-    for d in range(3):
-        k = gh[img[gh, d] != 0][0]
-        L[d] = (x[k, d] - x[owner[k], d]) / img[k, d]
-    t0 = time.perf_counter()
-    o.build_verlet(12.0)
-    t_verlet = time.perf_counter() - t0
-    steps, t_steps, iters = 0, 0.0, []
-    while True:
-        x[:n] += lmp_dt * v
-        x[n:] = x[owner[n:]] + img[n:] * L
-        o.set_atoms(n, x, typ, tag, owner, q=o.get("q"))
-        t0 = time.perf_counter()
-        its = o.step()
-        t_steps += time.perf_counter() - t0
-        iters.append(its)
-        steps += 1
-        if t_steps + t_verlet > budget_s or steps >= 10:
-            break
-    ms = (t_steps / steps + t_verlet / every) * 1e3
-    cpu_model = ""
+def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return dict(value=round(ms, 3), unit="ms/step", cores=threads, kind="port",
-                sample=f"{steps} oracle steps of the same {n}-atom box on the GPU run's state "
-                       f"(+ one Verlet build amortised over {every} steps); CG iterations {iters}",
-                nproc=os.cpu_count(), cpu=cpu_model, verlet_build_s=round(t_verlet, 3))
+    return ""
+
+
+def _gpu_forces(sysh):
+    """The energy terms and forces of the GPU's last step (the passes the
+    harness runs under `forces yes`), for the parity check."""
+    L = sysh.L
+    for nm in ("swr_atom_energy", "swr_valence_angles", "swr_torsion_angles", "swr_hydrogen_bonds",
+               "swr_vdw_coulomb_energy", "swr_add_dbond_forces", "swr_compute_forces"):
+        _lib.check(getattr(L, nm)(sysh.h), nm)
+    sysh.sync()
+
+
+def parity_and_cpu_baseline(lmp, sysh, elements, ffield, every, budget_s, samples, one_core=True, oracle_tol=1e-6):
+    """The oracle (the FP64 CPU restatement, compiled from oracle/) on the
+    GPU's own state, after the timed region: the GPU runs `samples` more
+    steps one at a time, and after each the oracle runs the same step from
+    the same atoms, Verlet list, capacities and QEq history.  Reports the
+    parity of each step (max |dq|, bond orders and coefficients, the lists,
+    CG iterations; energies and forces on the first) and the oracle's time
+    per step on all host cores and on one core."""
+    from oracle import compare, pyoracle  # test-infrastructure checker, only for this leg
+    if not os.path.exists(pyoracle.LIB_PATH):
+        import subprocess
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, stdout=subprocess.DEVNULL)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = pyoracle.Oracle(ffield, elements, threads=threads, tol=oracle_tol)
+    A = compare.atoms_of(sysh)
+    n = A["n"]
+    o.set_atoms(n, A["x"], A["type"], A["tag"], A["owner"])
+    t0 = time.perf_counter()
+    o.build_verlet(12.0)
+    t_verlet = time.perf_counter() - t0
+    t_steps, steps, iters, par = 0.0, 0, [], []
+    ef = None
+    hist = None
+    while steps < samples:
+        hist = compare.history_of(sysh)
+        lmp.run(1)
+        compare.feed_oracle(o, sysh, hist)
+        t0 = time.perf_counter()
+        its = o.step()
+        t_steps += time.perf_counter() - t0
+        steps += 1
+        git = (int(sysh.scalar("matvecs_s")), int(sysh.scalar("matvecs_t")))
+        iters.append((its, git))
+        try:
+            st = compare.compare_state(sysh, o, 1e-6, far=(steps == 1))    # charges within 1e-6 e
+            st["ok"] = True
+        except AssertionError as e:
+            st = {"ok": False, "error": str(e)[:300]}
+        par.append(st)
+        if steps == 1:
+            _gpu_forces(sysh)
+            de, df = compare.compare_energy_forces(sysh, o, A["owner"], n)
+            ef = {"max_rel_dE": float(f"{de:.3e}"), "max_rel_dF": float(f"{df:.3e}")}
+        if t_steps + t_verlet > budget_s:
+            break
+    ms = (t_steps / steps + t_verlet / every) * 1e3
+    cpu = dict(value=round(ms, 3), unit="ms/step", cores=threads, kind="port",
+               sample=f"{steps} oracle steps of the same {n}-atom box on the GPU run's state (+ one Verlet "
+                      f"build amortised over {every} steps)",
+               nproc=os.cpu_count(), cpu=_cpu_model(), verlet_build_s=round(t_verlet, 3))
+    if one_core:
+        # the last step again on one core: a Verlet build and one oracle step
+        pyoracle.lib().orc_set_threads(1)
+        o1 = pyoracle.Oracle(ffield, elements, threads=1)
+        compare.feed_oracle(o1, sysh, hist)
+        t0 = time.perf_counter()
+        o1.build_verlet(12.0)
+        t1v = time.perf_counter() - t0
+        compare.feed_oracle(o1, sysh, hist)
+        t0 = time.perf_counter()
+        o1.step()
+        t1s = time.perf_counter() - t0
+        pyoracle.lib().orc_set_threads(threads)
+        cpu["one_core"] = {"value": round((t1s + t1v / every) * 1e3, 3), "unit": "ms/step", "cores": 1,
+                           "sample": "1 oracle step of the same box + one Verlet build amortised over "
+                                     f"{every} steps", "verlet_build_s": round(t1v, 3)}
+    ok = all(p.get("ok") for p in par)
+    parity = {"reference": f"FP64 oracle (CPU restatement) at QEq tol {oracle_tol:g}, same atoms, Verlet list, "
+                           "capacities and QEq history",
+              "steps": steps, "ok": ok,
+              "max_abs_dq_e": max((p["dq"] for p in par if p.get("ok")), default=None),
+              "max_rel_dBO": max((p["bo_rel"] for p in par if p.get("ok")), default=None),
+              "lists_equal": ok, "far_entries_bitwise": par[0].get("far_dq") is not None if par else None,
+              "cg_iterations_oracle_gpu": [[list(a), list(b)] for a, b in iters],
+              "energies_forces_first_step": ef,
+              "failures": [p["error"] for p in par if not p.get("ok")]}
+    return cpu, parity
 
 
 def copy_calibration(mib):
@@ -201,14 +245,16 @@ def copy_calibration(mib):
                       "launches": 5}), flush=True)
 
 
-def bench_clusters(args):
+def bench_clusters(args, nsteps=None, plain_sample=None):
     """512 non-periodic water/methanol clusters (sizes log-uniform in [50, 3000]):
-    one batched evaluation (neighbour list, QEq, lists, BO and the non-bonded
-    energy + gradient; one launch per stage for all clusters) against the
-    plain per-system loop of the same library, both from host arrays to
-    results on the device."""
+    one batched evaluation (neighbour list, QEq, lists, BO and the full
+    ReaxFF energy + gradient; one launch per stage for all clusters) against
+    the plain per-system loop of the same library, both from host arrays to
+    results on the device.  Returns the JSON record."""
     from sw_reaxff_amd import batch as B
     count, lo, hi = 512, 50, 3000
+    nsteps = max(1, args.steps) if nsteps is None else nsteps
+    plain_sample = args.plain_sample if plain_sample is None else plain_sample
     sizes = systems.cluster_sizes(count, lo, hi, seed=args.seed)
     log(f"building {count} clusters, {int(sizes.sum()):,} atoms (seed {args.seed})")
     parts = [systems.cluster(int(k), seed=args.seed + 1 + i) for i, k in enumerate(sizes)]
@@ -219,7 +265,7 @@ def bench_clusters(args):
     bt.evaluate(nonbonded=True, bonded=True)
     bt.sync()
     lat = []
-    for _ in range(max(1, args.steps)):
+    for _ in range(nsteps):
         device_sync()
         t0 = time.perf_counter()
         bt.load(parts)
@@ -229,7 +275,7 @@ def bench_clusters(args):
     t_batch = float(np.median(lat))
     log(f"batched: {t_batch * 1e3:.1f} ms per batch of {count} (CG iterations max {its})")
     # plain loop: one system at a time through the same library (sampled)
-    nplain = min(count, args.plain_sample)
+    nplain = min(count, plain_sample)
     one = B.Batch(tol=1e-6)
     one.load([parts[0]])
     one.evaluate(nonbonded=True, bonded=True)
@@ -243,16 +289,17 @@ def bench_clusters(args):
     t_plain = (time.perf_counter() - t0) / nplain
     log(f"plain loop: {t_plain * 1e3:.2f} ms per system ({nplain} systems)")
     sps_batch, sps_plain = count / t_batch, 1.0 / t_plain
-    out = {
-        "metric": "batched systems/s (nbr+BO+QEq + non-bonded and bond/atom E/F, 512 clusters)",
+    return {
+        "metric": "batched systems/s (nbr+BO+QEq + the full ReaxFF energy and gradient, 512 clusters)",
         "value": round(sps_batch, 1),
         "unit": "systems/s", "n_gpus": 1, "steps": len(lat), "warmup": 1,
         "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{count} water/methanol clusters, sizes log-uniform [{lo}, {hi}], "
                                f"{natoms:,} atoms, 0.9 g/cm^3, 12 A vacuum padding, QEq tol 1e-6, "
-                               "energy (vdW + Coulomb, bond, lone pair, over/under-coordination, "
-                               "polarization) and its gradient per cluster",
+                               "the full ReaxFF energy (vdW, Coulomb, bond, lone pair, over/under-coordination, "
+                               "polarization, valence angle, penalty, coalition, torsion, conjugation, hydrogen "
+                               "bonds) and its gradient per cluster",
                    "systems": count, "atoms": natoms, "seed": args.seed, "parallelism": "single GPU"},
         "plain_loop_systems_per_s": round(sps_plain, 1), "speedup_vs_plain_loop": round(sps_batch / sps_plain, 2),
         "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
@@ -260,7 +307,6 @@ def bench_clusters(args):
         "atom_evals_per_s": round(natoms / t_batch, 1), "cg_iters_max": list(its),
         "plain_sample": nplain, "host": platform.node(),
     }
-    print(json.dumps(out), flush=True)
 
 
 # The mixed-precision solve's outer target is tol times this (qeq_mixed.hip).
@@ -350,13 +396,14 @@ def mixed_errors(s, args):
             "per_step_max_rel_dF": [float(f"{v:.3e}") for v in df]}
 
 
-def bench_serve(args):
+def bench_serve(args, rates=None, requests=None, reps=None, pool=None):
     """A Poisson stream of evaluation requests (sizes log-uniform in [20, 20000],
     half periodic water boxes, half clusters, 30% follow-on MD steps of
     resident systems) through the continuously batching server at a sweep of
     arrival rates; the value is the highest throughput whose p99 latency stays
     within 50 ms.  The plain loop is the same server with one worker and one
-    request per evaluation, all requests queued at the start."""
+    request per evaluation, all requests queued at the start.  Returns the
+    JSON record."""
     from sw_reaxff_amd import serve
     from sw_reaxff_amd.batch import Batch
     mixed = not args.fp64
@@ -365,11 +412,18 @@ def bench_serve(args):
     warm = serve.make_requests(24, rate=1e4, seed=args.seed + 1, hi=3000)
     serve.Server(lanes=args.lanes, max_batch_atoms=args.max_batch_atoms, batches=lanes, tol=args.tol,
                  mixed=mixed, resident=args.resident).run(warm)
+    rates = args.rates if rates is None else rates
+    requests = args.requests if requests is None else requests
+    nreps = args.reps if reps is None else reps
+    pool = args.pool if pool is None else pool
     runs = []
-    base = serve.make_requests(args.requests, rate=1.0, seed=args.seed)   # systems generated once
-    for rate in args.rates:
+    # the stream's systems are generated once (drawn from `pool` distinct
+    # systems) and replayed at every rate
+    log(f"serving: {requests} requests from {pool} distinct systems")
+    base = serve.make_requests(requests, rate=1.0, seed=args.seed, pool=pool)
+    for rate in rates:
         reps = []
-        for rep in range(args.reps):
+        for rep in range(nreps):
             reqs = serve.at_rate(base, rate)
             t0 = time.perf_counter()
             srv = serve.Server(lanes=args.lanes, max_batch_atoms=args.max_batch_atoms, batches=lanes, tol=args.tol,
@@ -381,7 +435,7 @@ def bench_serve(args):
             summ["lane_busy_s"] = {k: round(v, 3) for k, v in srv.busy.items()}
             summ["wall_s"] = round(wall, 3)
             reps.append(summ)
-            log(f"rate {rate:.0f}/s (run {rep + 1}/{args.reps}): {summ}")
+            log(f"rate {rate:.0f}/s (run {rep + 1}/{nreps}): {summ}")
         # a rate counts with its slowest repetition: lowest throughput, highest p99
         worst = min(reps, key=lambda r: r["systems_per_s"])
         worst = dict(worst, latency_ms=max((r["latency_ms"] for r in reps), key=lambda l: l["p99"]),
@@ -399,10 +453,11 @@ def bench_serve(args):
     best = max(ok, key=lambda r: r["systems_per_s"]) if ok else max(runs, key=lambda r: r["systems_per_s"])
     out = {
         "metric": "serving systems/s at p99 <= 50 ms (nbr+BO+QEq, Poisson arrivals)", "value": best["systems_per_s"],
-        "unit": "systems/s", "n_gpus": 1, "steps": args.requests, "warmup": len(warm),
+        "unit": "systems/s", "n_gpus": 1, "steps": requests, "warmup": len(warm),
         "ms_per_step": best["latency_ms"]["p50"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64+f32" if mixed else "f64", "data": "synthetic",
-        "config": {"workload": f"{args.requests} requests per rate, Poisson arrivals, sizes log-uniform [20, 20000], "
+        "config": {"workload": f"{requests} requests per rate ({pool} distinct systems), Poisson arrivals, "
+                               "sizes log-uniform [20, 20000], "
                                "50% periodic water / 50% water-methanol clusters, 30% follow-on MD steps with "
                                f"resident s/t, 200 GB HBM admission, "
                                f"{'mixed-precision ' if mixed else 'FP64 '}QEq tol {args.tol:g}",
@@ -413,7 +468,7 @@ def bench_serve(args):
         "runs": runs, "plain_loop_systems_per_s": round(plain, 1),
         "speedup_vs_plain_loop": round(best["systems_per_s"] / plain, 2), "host": platform.node(),
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -422,7 +477,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="water96k", choices=sorted(CONFIGS) + ["clusters512", "serve"])
-    ap.add_argument("--requests", type=int, default=400, help="serve: requests in the stream")
+    ap.add_argument("--requests", type=int, default=4000, help="serve: requests in the stream (per rate)")
+    ap.add_argument("--pool", type=int, default=400, help="serve: distinct systems the requests draw from")
     ap.add_argument("--rates", type=float, nargs="+", default=[1000, 1500, 1800, 2000, 2200, 2400],
                     help="serve: Poisson arrival rates (1/s) to sweep")
     ap.add_argument("--lanes", type=int, default=4, help="serve: concurrent batch workers")
@@ -438,10 +494,17 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6, help="QEq tolerance (the reference's: 1e-6)")
     ap.add_argument("--error-steps", type=int, default=200,
                     help="mixed: steps of the side-by-side comparison with an FP64 run (0: skip)")
+    ap.add_argument("--no-legs", dest="legs", action="store_false",
+                    help="water96k: skip the short clusters512 and serving runs whose results join the line")
+    ap.add_argument("--serve-rate", type=float, default=1500.0, help="arrival rate of the short serving run")
+    ap.add_argument("--serve-requests", type=int, default=2000, help="requests of the short serving run")
+    ap.add_argument("--serve-pool", type=int, default=150, help="distinct systems of the short serving run")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--every", type=int, default=10)
     ap.add_argument("--dt", type=float, default=0.25)
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
+    ap.add_argument("--parity-steps", type=int, default=3,
+                    help="GPU steps after the timed region checked against the oracle (also the CPU sample)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (read MIB, write MIB) five times: the known-size "
@@ -455,10 +518,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.config == "clusters512":
-        bench_clusters(args)
+        print(json.dumps(bench_clusters(args)), flush=True)
         return
     if args.config == "serve":
-        bench_serve(args)
+        print(json.dumps(bench_serve(args)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this path has no multi-GPU form (asked for --gpus {args.gpus}).",
@@ -587,11 +650,14 @@ def main():
               "energies_kcal_mol": dict(zip(("e_bond", "e_lp", "e_ov", "e_un", "e_pol"),
                                             [float(v) for v in sysh.get("bonded_energy")]))}
     log(f"bonded pass: {bonded}")
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu:
-        log(f"CPU baseline (oracle, budget {args.cpu_budget_s:.0f} s)")
-        cpu = cpu_baseline(sysh, s.elements, DEFAULT_FFIELD, args.dt, args.every, args.cpu_budget_s)
-        log(f"CPU {cpu['value']} ms/step on {cpu['cores']} threads")
+        log(f"oracle on the GPU's state: parity and CPU baseline (budget {args.cpu_budget_s:.0f} s)")
+        # the mixed-precision charges are checked against the converged FP64 solution
+        cpu, parity = parity_and_cpu_baseline(lmp, sysh, s.elements, DEFAULT_FFIELD, args.every,
+                                              args.cpu_budget_s, args.parity_steps, one_core=not args.mixed,
+                                              oracle_tol=1e-10 if args.mixed else args.tol)
+        log(f"CPU {cpu['value']} ms/step on {cpu['cores']} threads; {cpu.get('one_core')}; parity {parity}")
     errors, qeq_cmp = None, None
     if args.mixed:
         qeq_mixed = sum(kt[k][0] for k in QEQ_SLOTS[True] if k in kt) / ksteps
@@ -608,6 +674,23 @@ def main():
                    "speedup_vs_fp64_same_tol": round(legs[0]["qeq_ms_per_step"] / qeq_mixed, 3),
                    "speedup_vs_fp64_at_outer_target": round(legs[1]["qeq_ms_per_step"] / qeq_mixed, 3)}
         log(f"QEq mixed vs FP64: {qeq_cmp}")
+    # the metric's second half (batched systems/s, serving p50/p99) as extra
+    # keys of the same line: short runs of the clusters512 and serve workloads
+    legs = {}
+    if args.legs and not args.mixed:
+        lmp.close()
+        cl = bench_clusters(args, nsteps=3, plain_sample=64)
+        legs["clusters512"] = {k: cl[k] for k in ("value", "unit", "ms_per_step", "batch_latency_ms",
+                                                  "plain_loop_systems_per_s", "speedup_vs_plain_loop",
+                                                  "atom_evals_per_s", "cg_iters_max", "plain_sample")}
+        legs["clusters512"]["workload"] = cl["config"]["workload"]
+        sv = bench_serve(args, rates=[args.serve_rate], requests=args.serve_requests, reps=1,
+                         pool=args.serve_pool)
+        legs["serving"] = {k: sv[k] for k in ("value", "unit", "best_rate", "latency_ms", "p99_within_50ms",
+                                              "plain_loop_systems_per_s", "speedup_vs_plain_loop")}
+        legs["serving"]["workload"] = sv["config"]["workload"]
+        legs["serving"]["lanes"] = sv["config"]["lanes"]
+        legs["serving"]["residents"] = sv["config"]["residents"]
     out = {
         "metric": METRIC, "value": round(ms_step, 4), "unit": "ms/step", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": False, "scaling": "weak",
@@ -617,9 +700,11 @@ def main():
                    "atoms": n, "atoms_with_ghosts": c["N"], "seed": args.seed, "parallelism": "single GPU"},
         "atom_steps_per_s": round(n / (ms_step * 1e-3), 1),
         "cg_iters_per_step": [round(its[0], 2), round(its[1], 2)],
-        "counts": c, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu,
+        "counts": c, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu, "parity_vs_oracle": parity,
         "nonbonded_pass": nonbonded, "bonded_pass": bonded, "mixed_errors_vs_fp64": errors, "mixed_qeq_vs_fp64": qeq_cmp,
         "speedup_vs_cpu": round(cpu["value"] / ms_step, 2) if cpu else None,
+        "speedup_vs_cpu_one_core": round(cpu["one_core"]["value"] / ms_step, 2) if cpu and cpu.get("one_core") else None,
+        **legs,
         "host": platform.node(),
     }
     print(json.dumps(out), flush=True)

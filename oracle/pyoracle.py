@@ -47,6 +47,7 @@ def lib():
             getattr(L, nm).argtypes = [vp]
         L.orc_write_reax_lists.argtypes = [vp]
         L.orc_qeq_cg.argtypes = [vp, C.c_int]
+        L.orc_set_cg_imax.argtypes = [vp, C.c_int]
         L.orc_qeq_pre_force.argtypes = [vp, ip, ip]
         L.orc_step.argtypes = [vp, ip, ip]
         L.orc_sparse_matvec.argtypes = [vp, dp, dp]
@@ -177,6 +178,15 @@ class Oracle:
         if rc:
             raise RuntimeError(f"oracle step failed rc={rc}")
         return a.value, b.value
+
+    def set_cg_imax(self, imax):
+        """CG_v2's iteration cap (200 in the reference)."""
+        self.L.orc_set_cg_imax(self.h, int(imax))
+
+    def cg(self, which=0):
+        """One CG_v2 solve (0: s with b_s, 1: t with b_t) from the current
+        s or t; returns the reference's loop counter i."""
+        return self.L.orc_qeq_cg(self.h, int(which))
 
     def pre_force(self):
         a, b = C.c_int(0), C.c_int(0)

@@ -168,6 +168,7 @@ struct orc_sys {
   std::vector<double> Hdia_inv, b_s, b_t, s, t, s_hist, t_hist;  // hist [n][5]
   std::vector<double> p, qv, r, d, u, w, m, v, z, ss;
   int matvecs_s = 0, matvecs_t = 0;
+  int cg_imax = 200;   // CG_v2's imax (fix_qeq_reax_sunway.cpp:998)
   bool qeq_storage_init = false;
 
   std::string err;
@@ -1109,7 +1110,7 @@ int orc_sparse_matvec(orc_sys *S, const double *x_N, double *b_n) {
 // in atom order (one rank's loop, then an identity allreduce).
 static int cg_v2(orc_sys *S, std::vector<double> &bv, std::vector<double> &xv) {
   const int nn = S->n, N = S->N;
-  const int imax = 200;
+  const int imax = S->cg_imax;
   auto &r = S->r, &u = S->u, &w = S->w, &m = S->m, &v = S->v, &z = S->z, &ss = S->ss, &p = S->p,
        &d = S->d, &q = S->qv;
   for (auto *vec : {&r, &u, &w, &m, &v, &z, &ss, &p, &d, &q}) vec->assign(N, 0.0);
@@ -1167,6 +1168,8 @@ static int cg_v2(orc_sys *S, std::vector<double> &bv, std::vector<double> &xv) {
   }
   return i;
 }
+
+void orc_set_cg_imax(orc_sys *S, int imax) { S->cg_imax = imax > 1 ? imax : 1; }
 
 int orc_qeq_cg(orc_sys *S, int which) {
   if (which == 0) return S->matvecs_s = cg_v2(S, S->b_s, S->s);

@@ -71,6 +71,9 @@ int orc_bo(orc_sys *s);
 /* a11-a16: QEq. */
 int orc_qeq_init_matvec(orc_sys *s);
 int orc_qeq_cg(orc_sys *s, int which /* 0: s, 1: t */);
+/* CG_v2's iteration cap imax (200 in the reference): the solve returns
+ * x after imax - 1 updates when it has not converged before. */
+void orc_set_cg_imax(orc_sys *s, int imax);
 int orc_qeq_calculate_q(orc_sys *s);
 int orc_qeq_pre_force(orc_sys *s, int *iters_s, int *iters_t);
 int orc_sparse_matvec(orc_sys *s, const double *x_N, double *b_n);

@@ -276,13 +276,19 @@ int swr_nbr_bbox(swr_system *S, double lo[3], double hi[3]) {
 }
 
 int swr_nbr_set_grid(swr_system *S, const double lo[3], const double hi[3], double cutghost, double cutneigh) {
-  // Grid covering [lo - cutghost, hi + cutghost] with bins of ~cutneigh/2.
+  // Grid covering [lo - cutghost, hi + cutghost] with bins of at least
+  // cutneigh/2 (a flat or linear set of atoms gets one bin of that size
+  // across its thin dimensions: the stencil then stays within +-2 bins).
   const double target = 0.5 * cutneigh;
+  if (!(target > 0.0)) {
+    swr_set_error_msg("neighbour cutoff must be positive");
+    return SWR_ERR_ARG;
+  }
   for (int d = 0; d < 3; d++) {
     double a = lo[d] - cutghost - 1e-6, b = hi[d] + cutghost + 1e-6;
     int nb = std::max(1, (int)std::floor((b - a) / target));
     S->binlo[d] = a;
-    S->binsize[d] = (b - a) / nb;
+    S->binsize[d] = std::max((b - a) / nb, target);
     S->bininv[d] = 1.0 / S->binsize[d];
     S->nbin[d] = nb;
   }

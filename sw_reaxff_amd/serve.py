@@ -14,17 +14,23 @@ server:
     workers' batches overlap on the device (ctypes releases the GIL during
     library calls);
   * resident systems (resident="batched", the default): a follow-on step
-    advances the system one NVE step (F = 0: x += v dt, wrapped into a
-    periodic box) and evaluates it in the next batch like any other request,
-    its QEq solve warm-started from s/t histories kept in device memory
-    between its steps (HistStore: saved from the batch that evaluated its
-    last step, loaded into the batch of its next one; all five slots hold
-    the solution after its first evaluation).  Steps of one system run in
-    order, one per batch;
+    is one velocity-Verlet step of fix nve with the ReaxFF forces: the
+    half kick and drift with the forces its last evaluation computed
+    (nve_initial), the evaluation of the new positions in the next batch
+    like any other request (its QEq solve warm-started from s/t histories
+    kept in device memory between its steps: HistStore, saved from the
+    batch that evaluated its last step, loaded into the batch of its next
+    one; all five slots hold the solution after its first evaluation), and
+    the final half kick with the new forces (nve_final).  The few hundred
+    bytes per atom of positions, velocities and forces go with the batch's
+    host packing; the Verlet list is built per batch, whose composition
+    changes with every batch.  Steps of one system run in order, one per
+    batch;
   * resident="device": each resident system instead keeps its own device
-    system (positions, velocities, Verlet list with skin, s/t history) set
-    up at its first follow-on step from the histories its batched solve
-    left, and each further step runs there (one system per launch);
+    system (positions, velocities, forces, Verlet list with skin, s/t
+    history) set up at its first follow-on step from the histories its
+    batched solve left, and each further step runs there with the harness'
+    fix nve and forces (one system per launch);
   * admits work by estimated HBM footprint against a budget (bytes per atom
     dominated by the Verlet and far lists).
 
@@ -65,7 +71,8 @@ class Request:
     batch_size: int = 0
     step_index: int = 0             # kind "step": which step of its resident system this was
     hist: tuple | None = None       # resident's first evaluation: the s/t histories it left
-    energy: float | None = None     # vdW + Coulomb + bond/atom (bond, lone pair, over/under, polarization) energy
+    energy: float | None = None     # the full ReaxFF energy of the evaluation (kcal/mol)
+    f: np.ndarray | None = None     # residents' evaluations: the forces on the local atoms (kcal/mol/A)
 
     @property
     def latency(self):
@@ -89,21 +96,32 @@ def make_system(kind: str, natoms: int, seed: int) -> systems.System:
 
 
 def make_requests(count: int, rate: float, seed: int, lo: int = 20, hi: int = 20000, frac_step: float = 0.3,
-                  frac_resident: float = 0.2, dt_sizes=None):
+                  frac_resident: float = 0.2, pool: int = 0):
     """Poisson arrivals at `rate` per second; sizes log-uniform in [lo, hi];
     half periodic water boxes, half water/methanol clusters; a fraction of
     requests are follow-on MD steps of a resident system (new systems become
-    resident with probability frac_resident)."""
+    resident with probability frac_resident).  With pool > 0 the new
+    requests draw their systems from `pool` distinct systems generated first
+    (same size and kind distribution; building thousands of distinct
+    clusters in Python takes minutes), otherwise each is generated anew."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    systems_pool = []
+    for _ in range(pool):
+        size = int(math.exp(rng.uniform(math.log(lo), math.log(hi))))
+        kind = "water" if rng.random() < 0.5 else "cluster"
+        systems_pool.append(make_system(kind, size, seed=int(rng.integers(1 << 30))))
     reqs, residents = [], []
     for k in range(count):
         if residents and rng.random() < frac_step:
             reqs.append(Request(rid=k, arrival=float(t[k]), kind="step", resident=int(rng.choice(residents))))
             continue
-        size = int(math.exp(rng.uniform(math.log(lo), math.log(hi))))
-        kind = "water" if rng.random() < 0.5 else "cluster"
-        s = make_system(kind, size, seed=int(rng.integers(1 << 30)))
+        if systems_pool:
+            s = systems_pool[int(rng.integers(len(systems_pool)))]
+        else:
+            size = int(math.exp(rng.uniform(math.log(lo), math.log(hi))))
+            kind = "water" if rng.random() < 0.5 else "cluster"
+            s = make_system(kind, size, seed=int(rng.integers(1 << 30)))
         r = Request(rid=k, arrival=float(t[k]), kind="new", system=s)
         if rng.random() < frac_resident:
             r.resident = len(residents)
@@ -120,25 +138,41 @@ def at_rate(reqs: list[Request], rate: float, unit_rate: float = 1.0) -> list[Re
             for r in reqs]
 
 
-def advance(s: systems.System, dt: float) -> systems.System:
-    """One NVE step with F = 0 (initial_integrate: x += dt v; the final half
-    kick adds nothing), periodic positions wrapped into [lo, hi) as
-    Domain::pbc does."""
-    x = s.x + dt * s.v
-    if s.periodic:
-        L = s.hi - s.lo
-        x = np.where(x < s.lo, x + L, x)
-        over = x >= s.hi
-        x = np.where(over, np.maximum(x - L, s.lo), x)
-    return replace(s, x=x)
+# force->ftm2v in LAMMPS real units: (kcal/mol/A) / amu -> A/fs^2
+FTM2V = 1.0 / 48.88821291 / 48.88821291
+
+
+def _wrap(x, s: systems.System):
+    """Periodic positions wrapped into [lo, hi) as Domain::pbc does."""
+    if not s.periodic:
+        return x
+    L = s.hi - s.lo
+    x = np.where(x < s.lo, x + L, x)
+    over = x >= s.hi
+    return np.where(over, np.maximum(x - L, s.lo), x)
+
+
+def nve_initial(s: systems.System, f: np.ndarray, dt: float) -> systems.System:
+    """FixNVESunway::initial_integrate (fix_nve_sw64.c:25-99, per-type mass
+    branch): v += dtf/m f, x += dt v with dtf = dt/2 ftm2v
+    (fix_nve_sunway.cpp:56-57), then periodic wrapping."""
+    dtf = 0.5 * dt * FTM2V
+    v = s.v + (dtf / s.masses)[:, None] * f
+    return replace(s, x=_wrap(s.x + dt * v, s), v=v)
+
+
+def nve_final(s: systems.System, f: np.ndarray, dt: float) -> systems.System:
+    """FixNVESunway::final_integrate (fix_nve_sw64.c:102-173): v += dtf/m f."""
+    return replace(s, v=s.v + (0.5 * dt * FTM2V / s.masses)[:, None] * f)
 
 
 class _ResidentState:
-    """A resident system between its steps (resident="batched"): positions and
-    velocities on the host, its QEq warm start in device memory."""
+    """A resident system between its steps (resident="batched"): positions,
+    velocities (after the last step's final kick) and the forces at those
+    positions on the host, its QEq warm start in device memory."""
 
-    def __init__(self, s: systems.System, store):
-        self.s, self.store = s, store
+    def __init__(self, s: systems.System, f: np.ndarray, store):
+        self.s, self.f, self.store = s, f, store
         self.steps = 0
         self.busy = False                # a step of it is in a batch on the GPU
 
@@ -160,7 +194,8 @@ class _Resident:
     def _ensure(self):
         if self.lmp is None:
             lmp = Lammps()
-            lmp.commands(standard_script(self.s.elements, ffield=self.ffield, tol=self.tol, mixed=self.mixed))
+            lmp.commands(standard_script(self.s.elements, ffield=self.ffield, tol=self.tol, mixed=self.mixed,
+                                         forces=True))
             lmp.create_atoms(self.s)
             if self.hist is not None:
                 lmp.set_history(*self.hist)
@@ -295,7 +330,7 @@ class Server:
         for r in reqs:
             if r.kind == "step":
                 st = self.residents[r.resident]
-                r.system = advance(st.s, self.dt)
+                r.system = nve_initial(st.s, st.f, self.dt)
                 hists.append(st.store)
             else:
                 hists.append(None)
@@ -317,21 +352,25 @@ class Server:
         its = bt.iterations()
         es = bt.energies() + bt.bonded_energies().sum(axis=1)
         hists = bt.histories() if not batched and any(r.resident >= 0 for r in reqs) else None
+        grads = bt.gradients() if batched and any(r.resident >= 0 for r in reqs) else None
         for k, (r, q, it) in enumerate(zip(reqs, qs, its)):
             r.q = q.copy()
             r.energy = float(es[k])
             r.iters = tuple(int(v) for v in it)
+            if grads is not None and r.resident >= 0:
+                r.f = -grads[k]
         with self.cv:
             for k, r in enumerate(reqs):
                 if r.kind == "step":
                     st = self.residents[r.resident]
-                    st.s = r.system
+                    st.s = nve_final(r.system, r.f, self.dt)
+                    st.f = r.f
                     st.steps += 1
                     r.step_index = st.steps
                     st.busy = False
                 elif r.resident >= 0:
                     if batched:
-                        self.residents[r.resident] = _ResidentState(r.system, stores[k])
+                        self.residents[r.resident] = _ResidentState(r.system, r.f, stores[k])
                     else:
                         r.hist = hists[k]
                         res = _Resident(r.system, self.ffield, self.tol, self.mixed, hist=r.hist)

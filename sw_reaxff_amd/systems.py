@@ -161,7 +161,13 @@ def cluster(natoms: int, seed: int, density: float = 0.9, min_heavy: float = 2.2
     mass_tot = sum(WATER_MASS if k == 0 else METHANOL_MASS for k in kinds)
     vol = mass_tot / AVOGADRO / density * 1e24
     radius = (3 * vol / (4 * math.pi)) ** (1.0 / 3.0)
-    heavy = np.empty((0, 3))
+    # placed heavy atoms in a hash of cubic cells of edge min_heavy: every
+    # heavy atom closer than min_heavy to a candidate lies in the 27 cells
+    # around it, so the test below rejects exactly the placements a check
+    # against all placed heavy atoms would
+    cells: dict = {}
+    inv = 1.0 / min_heavy
+    lim = min_heavy * min_heavy
     xs, ts = [], []
     for k in kinds:
         body = wb if k == 0 else mb
@@ -178,11 +184,27 @@ def cluster(natoms: int, seed: int, density: float = 0.9, min_heavy: float = 2.2
             c *= r_try
             Rm = _random_rotations(rng, 1)[0]
             p = c + body @ Rm.T
-            if heavy.shape[0]:
-                dd = ((heavy[None, :, :] - p[hv][:, None, :]) ** 2).sum(-1)
-                if dd.min() < min_heavy * min_heavy:
-                    continue
-            heavy = np.vstack([heavy, p[hv]])
+            ph = p[hv]
+            keys = [tuple(v) for v in np.floor(ph * inv).astype(np.int64).tolist()]
+            clash = False
+            for a, (ix, iy, iz) in zip(ph, keys):
+                for dx in (-1, 0, 1):
+                    for dy in (-1, 0, 1):
+                        for dz in (-1, 0, 1):
+                            near = cells.get((ix + dx, iy + dy, iz + dz))
+                            if near is not None and (((np.asarray(near) - a) ** 2).sum(-1) < lim).any():
+                                clash = True
+                                break
+                        if clash:
+                            break
+                    if clash:
+                        break
+                if clash:
+                    break
+            if clash:
+                continue
+            for a, key in zip(ph, keys):
+                cells.setdefault(key, []).append(a)
             xs.append(p)
             ts.append(types)
             placed = True

@@ -22,6 +22,14 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: longer CPU test")
 
 
+def pytest_runtest_logreport(report):
+    """With SWR_INSTAFAIL=1 a failing test's traceback is printed when it
+    fails, not only in the final summary (a long GPU run that is cut short
+    keeps what it found)."""
+    if report.failed and os.environ.get("SWR_INSTAFAIL"):
+        print(f"\n[instafail] {report.nodeid} ({report.when})\n{report.longreprtext[-4000:]}", flush=True)
+
+
 def _ensure_built(path, make_dir):
     if not os.path.exists(path):
         subprocess.run(["make", "-C", make_dir, "-j8"], check=True, stdout=subprocess.DEVNULL)

@@ -8,14 +8,14 @@ Verlet order, so that summation orders agree.
 """
 from __future__ import annotations
 
-import numpy as np
-
+from oracle.compare import (BO_FIELDS, BO_RTOL, WS_ARRAYS, atoms_of, compare_energy_forces,  # noqa: F401
+                            compare_state, feed_oracle, gpu_energy_forces, history_of, oracle_energy_forces,
+                            ranges, rel_err, verlet_of)
 from sw_reaxff_amd import systems
 from sw_reaxff_amd.lammps import Lammps, standard_script
 
-# Parity tolerances of the path: lists bit-exact as sets, bond orders 1e-10
-# relative, charges 1e-6 e.
-BO_RTOL = 1e-10
+# Parity tolerance of the charges at the reference tolerance: 1e-6 e (lists
+# bit-exact as sets and bond orders 1e-10 relative: BO_RTOL).
 Q_ATOL = 1e-6
 
 
@@ -30,18 +30,6 @@ def harness(system, ffield, steps=0, every=10, dt=0.25, mixed=False, profile=Fal
     return lmp
 
 
-def atoms_of(sysh):
-    x = sysh.get("x").reshape(-1, 3)
-    return dict(n=sysh.scalar("n"), N=sysh.scalar("N"), x=x, type=sysh.get("type"), tag=sysh.get("tag"),
-                owner=sysh.get("owner"), q=sysh.get("q"))
-
-
-def verlet_of(sysh):
-    off = sysh.get("verlet_offsets")
-    lst = sysh.get("verlet_list")
-    return off, lst[: off[-1]]
-
-
 def per_atom(index, end, arr, i):
     return arr[index[i]:end[i]]
 
@@ -52,13 +40,6 @@ def make_oracle(oracle_lib, ffield, elements, A, verlet=None, threads=8, tol=1e-
     if verlet is not None:
         o.set_verlet(*verlet)
     return o
-
-
-def rel_err(a, b, floor=1e-300):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
-    den = np.maximum(np.abs(b), floor)
-    return np.max(np.abs(a - b) / den) if a.size else 0.0
 
 
 def water_c0(seed=1):
