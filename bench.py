@@ -264,6 +264,7 @@ def bench_clusters(args, nsteps=None, plain_sample=None):
     bt.load(parts)
     bt.evaluate(nonbonded=True, bonded=True)
     bt.sync()
+    tune = bt.sys.autotune(reps=3) if args.autotune else None
     lat = []
     for _ in range(nsteps):
         device_sync()
@@ -305,7 +306,7 @@ def bench_clusters(args, nsteps=None, plain_sample=None):
         "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                              "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "atom_evals_per_s": round(natoms / t_batch, 1), "cg_iters_max": list(its),
-        "plain_sample": nplain, "host": platform.node(),
+        "plain_sample": nplain, "autotune": tune, "host": platform.node(),
     }
 
 
@@ -409,6 +410,14 @@ def bench_serve(args, rates=None, requests=None, reps=None, pool=None):
     mixed = not args.fp64
     lanes = [Batch(tol=args.tol, mixed=mixed) for _ in range(args.lanes)]
     serve.warm_lanes(lanes, args.max_batch_atoms)
+    tune = None
+    if args.autotune:
+        # the shapes tuned on one lane's warm-up batch, given to every lane
+        tune = lanes[0].sys.autotune(reps=3)
+        for bt in lanes[1:]:
+            for fam, rec in tune.items():
+                if isinstance(rec, dict) and "chosen" in rec:
+                    bt.sys.set_shape(fam, rec["chosen"])
     warm = serve.make_requests(24, rate=1e4, seed=args.seed + 1, hi=3000)
     serve.Server(lanes=args.lanes, max_batch_atoms=args.max_batch_atoms, batches=lanes, tol=args.tol,
                  mixed=mixed, resident=args.resident).run(warm)
@@ -466,7 +475,8 @@ def bench_serve(args, rates=None, requests=None, reps=None, pool=None):
                    "parallelism": "single GPU"},
         "best_rate": best["rate"], "p99_within_50ms": bool(ok), "latency_ms": best["latency_ms"],
         "runs": runs, "plain_loop_systems_per_s": round(plain, 1),
-        "speedup_vs_plain_loop": round(best["systems_per_s"] / plain, 2), "host": platform.node(),
+        "speedup_vs_plain_loop": round(best["systems_per_s"] / plain, 2), "autotune": tune,
+        "host": platform.node(),
     }
     return out
 
@@ -494,6 +504,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6, help="QEq tolerance (the reference's: 1e-6)")
     ap.add_argument("--error-steps", type=int, default=200,
                     help="mixed: steps of the side-by-side comparison with an FP64 run (0: skip)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="launch with the default shapes instead of the autotuner's")
     ap.add_argument("--no-legs", dest="legs", action="store_false",
                     help="water96k: skip the short clusters512 and serving runs whose results join the line")
     ap.add_argument("--serve-rate", type=float, default=1500.0, help="arrival rate of the short serving run")
@@ -539,6 +551,13 @@ def main():
     sysh = lmp.system
     log(f"warmup {args.warmup} steps")
     lmp.run(args.warmup)
+    # launch shapes chosen by the library's autotuner on this box, before the
+    # timed region (the choices and per-shape times go into the line)
+    tune = None
+    if args.autotune:
+        tune = sysh.autotune(reps=5)
+        log(f"autotune: {tune}")
+        lmp.run(2)
     its0 = (lmp.stat("total_iters_s"), lmp.stat("total_iters_t"))
     device_sync()
     # the headline: whole steps, no per-kernel event records in the stream
@@ -701,6 +720,7 @@ def main():
         "atom_steps_per_s": round(n / (ms_step * 1e-3), 1),
         "cg_iters_per_step": [round(its[0], 2), round(its[1], 2)],
         "counts": c, "kernels": kernels, "roofline": roofline, "cpu_baseline": cpu, "parity_vs_oracle": parity,
+        "autotune": tune,
         "nonbonded_pass": nonbonded, "bonded_pass": bonded, "mixed_errors_vs_fp64": errors, "mixed_qeq_vs_fp64": qeq_cmp,
         "speedup_vs_cpu": round(cpu["value"] / ms_step, 2) if cpu else None,
         "speedup_vs_cpu_one_core": round(cpu["one_core"]["value"] / ms_step, 2) if cpu and cpu.get("one_core") else None,

@@ -339,6 +339,26 @@ int swr_batch_pack(int nsys, const int *nat, const double *x, const int *type, c
                    double *x_out, int *type_out, int *tag_out, int *owner_out, signed char *img_out,
                    int *row_off, int *ghost_off, double *shift, long long *N_out);
 
+/* ---- autotuner of launch shapes (no reference counterpart: the Sunway
+ * kernels have fixed ISTEP/SNSTEP tiles).  Families: "spmv_fp64" (the FP64
+ * CG's SpMV), "spmv_fp32" (the mixed-precision inner SpMV), "spmv_resid"
+ * (its FP64 residual) with shapes G8U4 G16U2 G16U4 G32U2 (lanes per H row,
+ * groups of 4 entries in flight per lane), and "far" (the far pass) with
+ * G4W2 G4W4 G6W2 G8W4 (gatherer and writer waves per workgroup).
+ * swr_autotune times every shape the system's current state allows
+ * (reps launches each, HIP events) and keeps the fastest; run it between
+ * steps.  A shape changes the order of sums inside an H row, never a list.
+ * With the environment variable SWR_TUNE_CACHE naming a file, the choices
+ * are appended there (device, family, log2 of the local atom count) and
+ * swr_autotune_load applies those matching the system (returns how many). */
+int     swr_autotune(swr_system *sys, int reps);
+int     swr_autotune_load(swr_system *sys);
+int     swr_set_shape(swr_system *sys, const char *family, const char *shape);
+int     swr_get_shape(swr_system *sys, const char *family, char *out, int len);
+/* JSON of the last swr_autotune (microseconds per shape, the choice);
+ * returns its length, copies at most len - 1 bytes into out. */
+int64_t swr_tune_report(swr_system *sys, char *out, int64_t len);
+
 /* Join the stream; returns the first device-raised error. */
 int swr_sync(swr_system *sys);
 /* hipDeviceSynchronize on the current device (a barrier for callers that

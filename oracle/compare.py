@@ -25,6 +25,16 @@ def rel_err(a, b, floor=1e-300):
     return np.max(np.abs(a - b) / den) if a.size else 0.0
 
 
+def rel_err_scaled(a, b, frac=1e-5):
+    """rel_err with a floor at frac times the array's largest magnitude:
+    entries that are differences of much larger terms (Delta' = sum BO -
+    valency near zero, a coefficient that cancels) are compared absolutely
+    at rtol * frac of that magnitude, a few units in its last place."""
+    b = np.asarray(b, dtype=np.float64)
+    scale = float(np.abs(b).max()) if b.size else 0.0
+    return rel_err(a, b, max(frac * scale, 1e-300))
+
+
 def worst_entry(a, b, floor=1e-300):
     """The entry that sets rel_err: (index, a, b, largest |b|)."""
     a = np.asarray(a, dtype=np.float64).ravel()
@@ -114,19 +124,22 @@ def compare_state(h, o, q_atol, bo_rtol=BO_RTOL, far=True):
     own = np.repeat(np.arange(N), (ge - gb).astype(np.int64))
     assert np.array_equal(gB["dbond_index"] - gb[own], oB["dbond_index"] - ob[own])
     assert np.array_equal(gB["sym_index"] - gb[gB["nbr"]], oB["sym_index"] - ob[oB["nbr"]])
+    # bond orders strictly relative (floor 1e-12: BO'-bo_cut of a weak bond);
+    # their coefficients and the per-atom sums relative down to 1e-5 of
+    # each array's largest magnitude, absolute below it
     worst = {"BO": rel_err(h.get("BO")[gs], o.get("BO")[os_], 1e-12),
              "BOpi": rel_err(h.get("BOpi").reshape(-1, 2)[gs], o.get("BOpi").reshape(-1, 2)[os_], 1e-12)}
     gbo, obo = h.get("bo_data")[gs], o.get("bo_data")[os_]
     for f in BO_FIELDS:
-        worst[f] = rel_err(gbo[f], obo[f], 1e-12)
+        worst[f] = rel_err_scaled(gbo[f], obo[f])
     for nm in WS_ARRAYS:
-        worst[nm] = rel_err(h.get(nm), o.get(nm), 1e-12)
+        worst[nm] = rel_err_scaled(h.get(nm), o.get(nm))
     st["bo_rel"] = float(max(worst.values()))
     if st["bo_rel"] > bo_rtol:
         arrays = {"BO": (h.get("BO")[gs], o.get("BO")[os_])}
         arrays.update({f: (gbo[f], obo[f]) for f in BO_FIELDS})
         arrays.update({nm: (h.get(nm), o.get(nm)) for nm in WS_ARRAYS})
-        bad = {k: (v, worst_entry(*arrays[k], 1e-12) if k in arrays else None)
+        bad = {k: (v, worst_entry(*arrays[k]) if k in arrays else None)
                for k, v in worst.items() if v > bo_rtol}
         raise AssertionError(f"relative differences beyond {bo_rtol}: {bad}")
     assert np.array_equal(h.get("bond_mark")[:N], o.get("bond_mark")[:N]), "bond_mark"

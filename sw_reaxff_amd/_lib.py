@@ -32,6 +32,7 @@ EXPORTS = [
     "swr_new_atoms", "swr_batch_pack", "swr_vdw_coulomb_energy", "swr_hist_save", "swr_hist_load",
     "swr_hist_free", "swr_atom_energy", "swr_add_dbond_forces", "swr_hydrogen_bonds",
     "swr_compute_forces", "swr_set_masses", "swr_valence_angles", "swr_torsion_angles", "swr_fix_nve_initial_integrate", "swr_fix_nve_final_integrate",
+    "swr_autotune", "swr_autotune_load", "swr_set_shape", "swr_get_shape", "swr_tune_report",
 ]
 
 
@@ -109,6 +110,12 @@ def lib():
         L.swr_ktimer_enable.argtypes = [vp, C.c_int]
         L.swr_device_synchronize.argtypes = []
         L.swr_ktimer_read.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
+        L.swr_autotune.argtypes = [vp, C.c_int]
+        L.swr_autotune_load.argtypes = [vp]
+        L.swr_set_shape.argtypes = [vp, C.c_char_p, C.c_char_p]
+        L.swr_get_shape.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_int]
+        L.swr_tune_report.restype = i64
+        L.swr_tune_report.argtypes = [vp, C.c_char_p, i64]
         _lib = L
     return _lib
 
@@ -235,6 +242,28 @@ class System:
 
     def scalar(self, name):
         return self.L.swr_get_scalar(self.h, name.encode())
+
+    # ---- autotuner of launch shapes
+    def autotune(self, reps=5):
+        """Times every launch shape of the tunable kernels on the current
+        state and keeps the fastest; returns the report (dict)."""
+        check(self.L.swr_autotune(self.h, int(reps)), "swr_autotune")
+        return self.tune_report()
+
+    def tune_report(self):
+        import json
+        nb = self.L.swr_tune_report(self.h, None, 0)
+        buf = C.create_string_buffer(int(nb) + 1)
+        self.L.swr_tune_report(self.h, buf, nb + 1)
+        return json.loads(buf.value.decode()) if nb > 0 else {}
+
+    def set_shape(self, family, shape):
+        check(self.L.swr_set_shape(self.h, family.encode(), shape.encode()), f"swr_set_shape {family} {shape}")
+
+    def shape(self, family):
+        buf = C.create_string_buffer(32)
+        check(self.L.swr_get_shape(self.h, family.encode(), buf, 32), f"swr_get_shape {family}")
+        return buf.value.decode()
 
     def ktimer_enable(self, on=True):
         check(self.L.swr_ktimer_enable(self.h, 1 if on else 0), "swr_ktimer_enable")

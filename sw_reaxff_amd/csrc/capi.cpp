@@ -370,9 +370,10 @@ int swr_destroy(swr_system *S) {
   if (!S) return SWR_OK;
   if (S->stream) (void)hipStreamSynchronize(S->stream);
   void *ptrs[] = {S->dp, S->x, S->v, S->q, S->type, S->tag, S->owner, S->img, S->mask, S->packed,
-                  S->bin_of, S->bin_count, S->bin_start, S->bin_fill, S->bp_pos, S->bp_id, S->stencil_rows,
+                  S->bin_of, S->bin_count, S->bin_start, S->rs_keys, S->rs_vals, S->rs_hist, S->bp_pos, S->bp_id, S->stencil_rows,
                   S->voff, S->vcount, S->vlist, S->far, S->far_index, S->far_end, S->num_bonds, S->num_hbonds,
-                  S->Hindex, S->bond_index, S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, S->Cdbo, S->Cdbopi,
+                  S->Hindex, S->bond_index, S->bond_end, S->bonds, S->bond_nbr, S->bo_data, S->BO, S->BOpi, S->Cdbo,
+                  S->Cdbopi,
                   S->Cdbopi2, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,
                   S->fcd, S->nb_partials, S->nb_result, S->nb_eatom, S->be_part, S->be_ce, S->be_bufc, S->be_result, S->hb_part,
                   S->hb_result, S->hb_acc, S->va_part, S->va_result, S->va_acc, S->to_part, S->to_result, S->to_acc,
@@ -489,7 +490,8 @@ static int ensure_bond_lists(swr_system *S, int need_bonds, int need_hbonds) {
   int rc;
   if (need_bonds > S->bond_cap) {
     int cap = std::max((int)(need_bonds * S->ctl.safezone), S->ctl.mincap * 25);
-    if ((rc = swr_alloc(&S->bonds, cap)) || (rc = swr_alloc(&S->bo_data, cap)) || (rc = swr_alloc(&S->BO, cap)) ||
+    if ((rc = swr_alloc(&S->bonds, cap)) || (rc = swr_alloc(&S->bond_nbr, cap)) ||
+        (rc = swr_alloc(&S->bo_data, cap)) || (rc = swr_alloc(&S->BO, cap)) ||
         (rc = swr_alloc(&S->BOpi, 2 * (size_t)cap)) || (rc = swr_alloc(&S->Cdbo, cap)) ||
         (rc = swr_alloc(&S->Cdbopi, cap)) || (rc = swr_alloc(&S->Cdbopi2, cap)))
       return rc;
@@ -583,6 +585,9 @@ int swr_qeq_init_matvec(swr_system *S) {
     HIP_CHECK(hipMemcpyAsync(&need, S->voff + S->n, sizeof(long long), hipMemcpyDeviceToHost, S->stream));
     HIP_CHECK(hipStreamSynchronize(S->stream));
   }
+  // H rows start at their Verlet offset + 4 i, rounded up to 4 entries
+  // (qeq.hip): the last row ends by (voff[n] + 4 n + 3) & ~3
+  need = (need + 4LL * S->n + 3) & ~3LL;
   int rc;
   if (need > S->Hcap) {
     long long cap = (long long)(need * 1.15) + 1024;

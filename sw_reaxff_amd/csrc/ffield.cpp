@@ -11,8 +11,8 @@
 //   * pair radii are arithmetic means, p_boc3/4/5 geometric means,
 //     gamma = (g_i g_j)^-1.5, then positive off-diagonal values override
 //     D, r_vdW (x2), alpha, r_s, r_p, r_pp                              (:402-545).
-// The torsion block is skipped; the valence-angle and hydrogen-bond blocks are read
-// (:654-686).
+// The valence-angle, torsion and hydrogen-bond blocks are read as well
+// (:547-686).
 #include "swr_ffield.h"
 
 #include <cctype>

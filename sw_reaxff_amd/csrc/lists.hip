@@ -13,6 +13,9 @@
 // Far slots of atom i are [voff[i], voff[i+1]) — bounded by its Verlet
 // count — so no separate counting pass is needed (the reference uses
 // i*maxfar, pair_reaxc_sw64.c:256).
+#include <type_traits>
+#include <vector>
+
 #include "swr_internal.h"
 #include "swr_device.h"
 
@@ -281,10 +284,9 @@ __device__ __forceinline__ void lds_barrier() {
 // phase's staged entries out as contiguous 16-byte nontemporal stores.  One
 // LDS barrier per phase; double-buffered staging.  Entry order and content
 // are those of k_far (a far list is its Verlet row filtered in order).
-constexpr int FG = 4;                        // gatherer waves per block
-constexpr int FWR = 2;                       // writer waves per block
-constexpr int FT = (FG + FWR) * SWR_WAVE;    // threads per block
-
+// The split (FG gatherers, FWR writers per block) is a template parameter;
+// the autotuner picks among the FarShape variants below.
+template <int FG, int FWR>
 struct FarStage {
   double2 far[2][FG][4 * SWR_WAVE];  // staged entries, 4 pieces each
   int2 hb[2][FG][SWR_WAVE];          // staged hbonds: (nbr, far slot)
@@ -295,21 +297,38 @@ struct FarStage {
   int nphase[FG];
 };
 
+// Gatherer/writer splits the autotuner chooses from.
+template <int SH> struct FarShape;
+template <> struct FarShape<SWR_FAR_G4W2> { static constexpr int FG = 4, FWR = 2; };
+template <> struct FarShape<SWR_FAR_G4W4> { static constexpr int FG = 4, FWR = 4; };
+template <> struct FarShape<SWR_FAR_G6W2> { static constexpr int FG = 6, FWR = 2; };
+template <> struct FarShape<SWR_FAR_G8W4> { static constexpr int FG = 8, FWR = 4; };
+
+template <class F>
+void far_dispatch(int sh, F &&f) {
+  switch (sh) {
+    case SWR_FAR_G4W4: f(std::integral_constant<int, SWR_FAR_G4W4>()); break;
+    case SWR_FAR_G6W2: f(std::integral_constant<int, SWR_FAR_G6W2>()); break;
+    case SWR_FAR_G8W4: f(std::integral_constant<int, SWR_FAR_G8W4>()); break;
+    default: f(std::integral_constant<int, SWR_FAR_G4W2>()); break;
+  }
+}
+
 // A gatherer's neighbour record for one chunk: the id and the packed atom.
 struct FarNbr {
   int j;
   swr_atom_pack_t a;
 };
 
-template <bool CAND, int HB>
-__global__ void __launch_bounds__(FT)
+template <bool CAND, int HB, int FG, int FWR>
+__global__ void __launch_bounds__((FG + FWR) * SWR_WAVE)
 k_far_ws(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__restrict__ atoms,
          const long long *__restrict__ voff, const int *__restrict__ vlist, swr_far_neighbor_t *__restrict__ far,
          int *__restrict__ far_index, int *__restrict__ far_end, int *__restrict__ cand,
          int *__restrict__ cand_count, const int *__restrict__ Hindex, const int *__restrict__ num_hbonds,
          const int *__restrict__ hbond_index, int *__restrict__ hbond_end, swr_hbond_t *__restrict__ hbonds,
          int *__restrict__ hb_count, int hbond_cap, DevStatus *st) {
-  __shared__ FarStage S;
+  __shared__ FarStage<FG, FWR> S;
   const int w = __builtin_amdgcn_readfirstlane(wave_in_block());
   const int lane = lane_id();
   const bool gatherer = w < FG;
@@ -507,7 +526,8 @@ __global__ void __launch_bounds__(LT)
 k_bonds(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__restrict__ atoms,
         const swr_far_neighbor_t *__restrict__ far, const int *__restrict__ cand,
         const int *__restrict__ cand_count, const int *__restrict__ num_bonds, const int *__restrict__ bond_index,
-        int *__restrict__ bond_end, swr_bond_t *__restrict__ bonds, swr_bond_order_t *__restrict__ bo_data,
+        int *__restrict__ bond_end, swr_bond_t *__restrict__ bonds, int *__restrict__ bond_nbr,
+        swr_bond_order_t *__restrict__ bo_data,
         double *__restrict__ BO, double *__restrict__ BOpi, double *__restrict__ Cdbo, double *__restrict__ Cdbopi,
         double *__restrict__ Cdbopi2, double *__restrict__ bo_dboc, double *__restrict__ dDeltap_self,
         int *__restrict__ bond_mark, double *__restrict__ Deltap, double *__restrict__ Deltap_boc,
@@ -591,6 +611,7 @@ k_bonds(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__
     b.rel_box[0] = b.rel_box[1] = b.rel_box[2] = 0;
     b.sym_index = 0;
     bonds[p] = b;
+    bond_nbr[p] = rc.nbr;
     bo_data[p] = o;
     BO[p] = t.BO - bo_cut;
     BOpi[2 * (size_t)p] = t.pi;
@@ -678,7 +699,7 @@ __global__ void k_validate(int n, int N, const int *__restrict__ bond_index, con
 // has reached it.  chg[launch] is set when the launch changed a mark; a launch
 // after an unchanged one does nothing.
 __global__ void k_bond_mark(int n, int N, int launch, int inner, const int *__restrict__ bond_index,
-                            const int *__restrict__ bond_end, const swr_bond_t *__restrict__ bonds,
+                            const int *__restrict__ bond_end, const int *__restrict__ bond_nbr,
                             int *__restrict__ mark, int *__restrict__ chg) {
   if (launch > 0 && __hip_atomic_load(&chg[launch - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
   const int g = n + blockIdx.x * blockDim.x + threadIdx.x;
@@ -689,7 +710,7 @@ __global__ void k_bond_mark(int n, int N, int launch, int inner, const int *__re
   for (int it = 0; it < inner; it++) {
     int best = cur;
     for (int p = p0; p < p1; p++) {
-      const int j = bonds[p].nbr;
+      const int j = bond_nbr[p];
       if (j < g) {
         const int v = __hip_atomic_load(&mark[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
         if (v < best) best = v;
@@ -781,19 +802,27 @@ int swr_lists_run(swr_system *S, int mode) {
                      S->hbond_end, S->hbonds, hb_count, S->hbond_cap, S->status)
 #define BOND_ARGS(k)                                                                                              \
   hipLaunchKernelGGL(k, dim3(div_up(N, 2 * LW)), dim3(LT), 0, S->stream, S->n, N, S->dp, S->packed, S->far, S->cand,          \
-                     S->cand_count, S->num_bonds, S->bond_index, S->bond_end, S->bonds, S->bo_data, S->BO, S->BOpi, \
+                     S->cand_count, S->num_bonds, S->bond_index, S->bond_end, S->bonds, S->bond_nbr, S->bo_data,    \
+                     S->BO, S->BOpi,                                                                               \
                      S->Cdbo, S->Cdbopi, S->Cdbopi2, S->bo_dboc, S->dDeltap_self, S->bond_mark,                    \
                      ws + (size_t)WS_DELTAP * S->Ncap, ws + (size_t)WS_DELTAP_BOC * S->Ncap, S->counts,            \
                      S->bond_cap, S->status)
-#define FARWS_ARGS(k)                                                                                            \
-  hipLaunchKernelGGL(k, dim3(div_up(N, FG)), dim3(FT), 0, S->stream, S->n, N, S->dp, S->packed, S->voff, S->vlist,  \
-                     S->far, S->far_index, S->far_end, S->cand, S->cand_count, S->Hindex, S->num_hbonds,            \
-                     S->hbond_index, S->hbond_end, S->hbonds, hb_count, S->hbond_cap, S->status)
   swr_kt_begin(S, KT_FAR);
-  if (mode == 0) FARWS_ARGS((k_far_ws<false, 0>));
-  else if (mode == 1) FARWS_ARGS((k_far_ws<true, 2>));
-  else if (mode == 2) FAR_ARGS((k_far<true, false, true, 2>));
-  else FARWS_ARGS((k_far_ws<true, 1>));
+  if (mode == 2) {
+    FAR_ARGS((k_far<true, false, true, 2>));
+  } else {
+    far_dispatch(S->far_shape, [&](auto shc) {
+      constexpr int FG = FarShape<decltype(shc)::value>::FG, FWR = FarShape<decltype(shc)::value>::FWR;
+      const dim3 gb(div_up(N, FG)), tb((FG + FWR) * SWR_WAVE);
+#define FARWS_ARGS(k)                                                                                            \
+  hipLaunchKernelGGL(k, gb, tb, 0, S->stream, S->n, N, S->dp, S->packed, S->voff, S->vlist, S->far, S->far_index,    \
+                     S->far_end, S->cand, S->cand_count, S->Hindex, S->num_hbonds, S->hbond_index, S->hbond_end,     \
+                     S->hbonds, hb_count, S->hbond_cap, S->status)
+      if (mode == 0) FARWS_ARGS((k_far_ws<false, 0, FG, FWR>));
+      else if (mode == 1) FARWS_ARGS((k_far_ws<true, 2, FG, FWR>));
+      else FARWS_ARGS((k_far_ws<true, 1, FG, FWR>));
+    });
+  }
   swr_kt_end(S, KT_FAR);
   if (mode != 0) {
     swr_kt_begin(S, KT_BONDS);
@@ -805,6 +834,47 @@ int swr_lists_run(swr_system *S, int mode) {
 #undef FARWS_ARGS
 #undef BOND_ARGS
   HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+// Autotuner hook: the average time of `reps` launches of the far pass (the
+// fused far + bond-candidate + hbond form) in gatherer/writer shape sh.
+// Each launch follows a Reset, as in a step (the pass appends to the hbond
+// lists Reset starts); the caller re-runs the pair stages afterwards.
+int swr_lists_time_far(swr_system *S, int sh, int reps, float *ms) {
+  int rc = swr_quiesce(S);
+  if (rc) return rc;
+  const int N = S->N;
+  if (N <= 0 || !S->far || !S->packed) return SWR_ERR_STATE;
+  if ((size_t)N * MCAND > S->cand_cap) {
+    if ((rc = swr_alloc(&S->cand, (size_t)S->Ncap * MCAND)) || (rc = swr_alloc(&S->cand_count, S->Ncap))) return rc;
+    S->cand_cap = (size_t)S->Ncap * MCAND;
+  }
+  int *hb_count = S->counts + S->Ncap;
+  std::vector<hipEvent_t> ev(2 * (size_t)std::max(reps, 1));
+  for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+  for (int r = 0; r < reps; r++) {
+    if ((rc = swr_lists_reset(S))) return rc;
+    (void)hipEventRecord(ev[2 * r], S->stream);
+    far_dispatch(sh, [&](auto shc) {
+      constexpr int FG = FarShape<decltype(shc)::value>::FG, FWR = FarShape<decltype(shc)::value>::FWR;
+      hipLaunchKernelGGL((k_far_ws<true, 2, FG, FWR>), dim3(div_up(N, FG)), dim3((FG + FWR) * SWR_WAVE), 0,
+                         S->stream, S->n, N, S->dp, S->packed, S->voff, S->vlist, S->far, S->far_index, S->far_end,
+                         S->cand, S->cand_count, S->Hindex, S->num_hbonds, S->hbond_index, S->hbond_end, S->hbonds,
+                         hb_count, S->hbond_cap, S->status);
+    });
+    (void)hipEventRecord(ev[2 * r + 1], S->stream);
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(S->stream));
+  double tot = 0.0;
+  for (int r = 0; r < reps; r++) {
+    float t = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&t, ev[2 * r], ev[2 * r + 1]));
+    tot += t;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  *ms = (float)(tot / std::max(reps, 1));
   return SWR_OK;
 }
 
@@ -865,7 +935,7 @@ int swr_lists_mark(swr_system *S, int max_sweeps) {
   hipLaunchKernelGGL(k_mark_init, dim3(1), dim3(MARK_MAX_LAUNCH), 0, S->stream, S->mark_chg, MARK_MAX_LAUNCH);
   for (int k = 0; k < L; k++)
     hipLaunchKernelGGL(k_bond_mark, dim3(div_up(N - n, 256)), dim3(256), 0, S->stream, n, N, k, MARK_INNER,
-                       S->bond_index, S->bond_end, S->bonds, S->bond_mark, S->mark_chg);
+                       S->bond_index, S->bond_end, S->bond_nbr, S->bond_mark, S->mark_chg);
   HIP_CHECK(hipMemcpyAsync(S->mark_chg_host, S->mark_chg, sizeof(int) * L, hipMemcpyDeviceToHost, S->stream));
   swr_kt_end(S, KT_MARK);
   HIP_CHECK(hipGetLastError());
@@ -891,7 +961,7 @@ int swr_mark_settle(swr_system *S) {
     const int l0 = launch;
     for (int k = 0; k < 4 && launch < MARK_MAX_LAUNCH; k++, launch++)
       hipLaunchKernelGGL(k_bond_mark, dim3(div_up(N - n, 256)), dim3(256), 0, S->stream, n, N, launch, MARK_INNER,
-                         S->bond_index, S->bond_end, S->bonds, S->bond_mark, S->mark_chg);
+                         S->bond_index, S->bond_end, S->bond_nbr, S->bond_mark, S->mark_chg);
     HIP_CHECK(hipMemcpyAsync(S->mark_chg_host + l0, S->mark_chg + l0, sizeof(int) * (launch - l0),
                              hipMemcpyDeviceToHost, S->stream));
     HIP_CHECK(hipStreamSynchronize(S->stream));

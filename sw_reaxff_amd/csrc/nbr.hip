@@ -120,9 +120,9 @@ __device__ __forceinline__ int coord_bin(double v, double lo, double inv, int nb
   return b;
 }
 
-__global__ void k_bin_assign(const double *__restrict__ x, int N, double lx, double ly, double lz, double ix,
-                             double iy, double iz, int nx, int ny, int nz, int *__restrict__ bin_of,
-                             int *__restrict__ bin_count, DevStatus *st) {
+// Bin of every atom (bin = (bz * ny + by) * nx + bx).
+__global__ void k_bin_keys(const double *__restrict__ x, int N, double lx, double ly, double lz, double ix,
+                           double iy, double iz, int nx, int ny, int nz, int *__restrict__ bin_of, DevStatus *st) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   int oor = 0;
@@ -130,36 +130,104 @@ __global__ void k_bin_assign(const double *__restrict__ x, int N, double lx, dou
   int by = coord_bin(x[3 * i + 1], ly, iy, ny, &oor);
   int bz = coord_bin(x[3 * i + 2], lz, iz, nz, &oor);
   if (oor) atomicOr(&st->flags, SWR_F_BIN);
-  int b = (bz * ny + by) * nx + bx;
-  bin_of[i] = b;
-  atomicAdd(&bin_count[b], 1);
+  bin_of[i] = (bz * ny + by) * nx + bx;
 }
 
-__global__ void k_bin_scatter(const int *__restrict__ bin_of, int N, const int *__restrict__ bin_start,
-                              int *__restrict__ bin_fill, int *__restrict__ bp_id) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  int b = bin_of[i];
-  int slot = bin_start[b] + atomicAdd(&bin_fill[b], 1);
-  bp_id[slot] = i;
+// ---- counting sort of the atoms by bin, staged in LDS
+// A stable LSD radix sort of (bin, atom) pairs with 8-bit digits: per pass
+// every block takes RS_CHUNK consecutive pairs (in the previous pass's
+// order; atom index order in the first), counts their digits in an LDS
+// histogram (rs_hist: hist[d * nblk + blk]), an exclusive scan over
+// (digit, block) gives each block's start per digit, and rs_scatter writes
+// each pair at its block's start for its digit plus its rank among the
+// block's earlier pairs with that digit, ranked in LDS in input order (a
+// wave ranks its lanes by matching digit bits with ballots; waves and
+// rounds are added in order).  Stability makes the atoms of each bin come
+// out in ascending index order, with no atomics on global memory, so the
+// binning is the same on every run.
+constexpr int RS_T = 256;                   // threads per block
+constexpr int RS_ROUNDS = 4;                // pairs per thread
+constexpr int RS_CHUNK = RS_T * RS_ROUNDS;  // pairs per block
+constexpr int RS_RADIX = 256;
+
+__global__ void __launch_bounds__(RS_T) k_rs_hist(const int *__restrict__ keys, int N, int shift, int nblk,
+                                                  int *__restrict__ hist) {
+  __shared__ int h[RS_RADIX];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * RS_CHUNK;
+#pragma unroll
+  for (int r = 0; r < RS_ROUNDS; r++) {
+    const int k = base + r * RS_T + threadIdx.x;
+    if (k < N) atomicAdd(&h[(keys[k] >> shift) & (RS_RADIX - 1)], 1);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// Sort each bin's atoms by index (bins hold ~20 atoms) and gather positions.
-__global__ void k_bin_sort(int nbins, const int *__restrict__ bin_start, int *__restrict__ bp_id,
-                           const double *__restrict__ x, double4 *__restrict__ bp_pos) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
-  int s = bin_start[b], e = bin_start[b + 1];
-  for (int k = s + 1; k < e; k++) {
-    int v = bp_id[k];
-    int m = k - 1;
-    while (m >= s && bp_id[m] > v) { bp_id[m + 1] = bp_id[m]; m--; }
-    bp_id[m + 1] = v;
+// vals_in == nullptr: the values are the input positions (atom indices).
+__global__ void __launch_bounds__(RS_T) k_rs_scatter(const int *__restrict__ keys_in, const int *__restrict__ vals_in,
+                                                     int N, int shift, int nblk, const int *__restrict__ off,
+                                                     int *__restrict__ keys_out, int *__restrict__ vals_out) {
+  constexpr int NW = RS_T / SWR_WAVE;
+  __shared__ int base[RS_RADIX];       // this block's next slot per digit
+  __shared__ int wcnt[NW][RS_RADIX];   // per-wave digit counts of the current round
+  const int w = wave_in_block(), lane = lane_id();
+  base[threadIdx.x] = off[threadIdx.x * nblk + blockIdx.x];
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (SWR_WAVE - lane));
+  for (int r = 0; r < RS_ROUNDS; r++) {
+#pragma unroll
+    for (int d = lane; d < RS_RADIX; d += SWR_WAVE) wcnt[w][d] = 0;
+    const int k = blockIdx.x * RS_CHUNK + r * RS_T + threadIdx.x;
+    const bool valid = k < N;
+    const int key = valid ? keys_in[k] : 0;
+    const int val = valid ? (vals_in ? vals_in[k] : k) : 0;
+    const int dig = (key >> shift) & (RS_RADIX - 1);
+    // lanes of this wave holding the same digit
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const unsigned long long m = __ballot((dig >> b) & 1);
+      peers &= ((dig >> b) & 1) ? m : ~m;
+    }
+    const int rank = __popcll(peers & below);
+    __syncthreads();   // wcnt zeroed, base of the previous round final
+    if (valid && rank == 0) wcnt[w][dig] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      int pos = base[dig] + rank;
+      for (int v = 0; v < w; v++) pos += wcnt[v][dig];
+      keys_out[pos] = key;
+      vals_out[pos] = val;
+    }
+    __syncthreads();
+    int add = 0;
+#pragma unroll
+    for (int v = 0; v < NW; v++) add += wcnt[v][threadIdx.x];
+    base[threadIdx.x] += add;
+    __syncthreads();
   }
-  for (int k = s; k < e; k++) {
-    int j = bp_id[k];
-    bp_pos[k] = make_double4(x[3 * j], x[3 * j + 1], x[3 * j + 2], 0.0);
-  }
+}
+
+// Atoms per bin from the sorted keys: each run of equal keys writes its
+// length (bins without atoms keep the zero they were cleared to).
+__global__ void k_bin_runs(const int *__restrict__ skeys, int N, int *__restrict__ bin_count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  const int key = skeys[k];
+  if (k > 0 && skeys[k - 1] == key) return;
+  int e = k + 1;
+  while (e < N && skeys[e] == key) e++;
+  bin_count[key] = e - k;
+}
+
+// Positions of the sorted atoms (x, y, z, unused).
+__global__ void k_bin_pos(int N, const int *__restrict__ bp_id, const double *__restrict__ x,
+                          double4 *__restrict__ bp_pos) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  const int j = bp_id[k];
+  bp_pos[k] = make_double4(x[3 * j], x[3 * j + 1], x[3 * j + 2], 0.0);
 }
 
 // ---------------------------------------------------------------- Verlet
@@ -296,9 +364,7 @@ int swr_nbr_set_grid(swr_system *S, const double lo[3], const double hi[3], doub
   int nb = S->nbin[0] * S->nbin[1] * S->nbin[2];
   if (nb + 1 > S->binscap) {
     int rc;
-    if ((rc = swr_alloc(&S->bin_count, nb + 1)) || (rc = swr_alloc(&S->bin_start, nb + 2)) ||
-        (rc = swr_alloc(&S->bin_fill, nb + 1)))
-      return rc;
+    if ((rc = swr_alloc(&S->bin_count, nb + 1)) || (rc = swr_alloc(&S->bin_start, nb + 2))) return rc;
     S->binscap = nb + 1;
   }
   S->nbins = nb;
@@ -398,17 +464,42 @@ int swr_nbr_build(swr_system *S, int local_only) {
   if ((rc = swr_reserve(&S->bin_of, S->cap_bin_of, N)) || (rc = swr_reserve(&S->bp_id, S->cap_bp_id, N)) ||
       (rc = swr_reserve(&S->bp_pos, S->cap_bp_pos, N)))
     return rc;
+  // counting sort by bin, staged in LDS (k_rs_*): passes of 8 bits up to the
+  // largest bin id; ping-pong between the key/value buffers so that the
+  // last pass writes bp_id
+  const int rs_nblk = div_up(N, RS_CHUNK);
+  int passes = 1;
+  while (passes < 4 && (S->nbins - 1) >> (8 * passes)) passes++;
+  if ((rc = swr_reserve(&S->rs_keys, S->cap_rs_keys, 2 * (size_t)N)) ||
+      (rc = swr_reserve(&S->rs_vals, S->cap_rs_vals, (size_t)N)) ||
+      (rc = swr_reserve(&S->rs_hist, S->cap_rs_hist, 2 * (size_t)RS_RADIX * rs_nblk + 1)))
+    return rc;
   swr_kt_begin(S, KT_BINS);
-  HIP_CHECK(hipMemsetAsync(S->bin_count, 0, sizeof(int) * (S->nbins + 1), S->stream));
-  HIP_CHECK(hipMemsetAsync(S->bin_fill, 0, sizeof(int) * (S->nbins + 1), S->stream));
-  hipLaunchKernelGGL(k_bin_assign, dim3(div_up(N, 256)), dim3(256), 0, S->stream, S->x, N, S->binlo[0], S->binlo[1],
+  hipLaunchKernelGGL(k_bin_keys, dim3(div_up(N, 256)), dim3(256), 0, S->stream, S->x, N, S->binlo[0], S->binlo[1],
                      S->binlo[2], S->bininv[0], S->bininv[1], S->bininv[2], S->nbin[0], S->nbin[1], S->nbin[2],
-                     S->bin_of, S->bin_count, S->status);
+                     S->bin_of, S->status);
+  {
+    int *hist = S->rs_hist, *off = S->rs_hist + (size_t)RS_RADIX * rs_nblk;
+    const int *kin = S->bin_of, *vin = nullptr;
+    int *kbuf[2] = {S->rs_keys, S->rs_keys + N};
+    for (int p = 0; p < passes; p++) {
+      // the last pass writes the values into bp_id, earlier ones alternate
+      // so that no pass reads the buffer it writes
+      int *vout = (p == passes - 1) ? S->bp_id : ((passes - 1 - p) % 2 ? S->rs_vals : S->bp_id);
+      int *kout = kbuf[p & 1];
+      hipLaunchKernelGGL(k_rs_hist, dim3(rs_nblk), dim3(RS_T), 0, S->stream, kin, N, 8 * p, rs_nblk, hist);
+      if ((rc = swr_scan_i32(S, hist, off, RS_RADIX * rs_nblk, nullptr))) return rc;
+      hipLaunchKernelGGL(k_rs_scatter, dim3(rs_nblk), dim3(RS_T), 0, S->stream, kin, vin, N, 8 * p, rs_nblk,
+                         (const int *)off, kout, vout);
+      kin = kout;
+      vin = vout;
+    }
+    HIP_CHECK(hipMemsetAsync(S->bin_count, 0, sizeof(int) * (S->nbins + 1), S->stream));
+    hipLaunchKernelGGL(k_bin_runs, dim3(div_up(N, 256)), dim3(256), 0, S->stream, kin, N, S->bin_count);
+  }
   if ((rc = swr_scan_i32(S, S->bin_count, S->bin_start, S->nbins + 1, nullptr))) return rc;
-  hipLaunchKernelGGL(k_bin_scatter, dim3(div_up(N, 256)), dim3(256), 0, S->stream, S->bin_of, N, S->bin_start,
-                     S->bin_fill, S->bp_id);
-  hipLaunchKernelGGL(k_bin_sort, dim3(div_up(S->nbins, 128)), dim3(128), 0, S->stream, S->nbins, S->bin_start,
-                     S->bp_id, S->x, S->bp_pos);
+  hipLaunchKernelGGL(k_bin_pos, dim3(div_up(N, 256)), dim3(256), 0, S->stream, N, (const int *)S->bp_id, S->x,
+                     S->bp_pos);
   swr_kt_end(S, KT_BINS);
   if ((rc = swr_reserve(&S->vcount, S->cap_vcount, N)) || (rc = swr_reserve(&S->voff, S->cap_voff, (size_t)N + 1)))
     return rc;

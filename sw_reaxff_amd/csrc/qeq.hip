@@ -6,8 +6,11 @@
 //   val = Tap(r) * 14.4 / (r^3 + shld[ti][tj])^0.3333333333333
 // with Tap by Horner (calculate_H, fix_qeq_reax_sunway.cpp:965-981).  Row i
 // holds jlist/val[firstnbr[i] .. firstnbr[i]+numnbrs[i]); firstnbr[i] is the
-// row's Verlet offset (the reference uses a 600*i stride,
-// fix_qeq_reax_sw64.c:828).  Ghost columns are folded onto their owning
+// row's Verlet offset moved up by 4 i and rounded up to a multiple of 4
+// entries (the reference uses a 600*i stride, fix_qeq_reax_sw64.c:828): rows
+// start 16-byte aligned, are padded to a multiple of 4 entries (column i,
+// value 0), and never overlap (a row holds at most its Verlet row's count),
+// so the SpMV reads them with 16-byte loads.  Ghost columns are folded onto their owning
 // local atom: on one domain the forward comm of x makes x[ghost] ==
 // x[owner], so the SpMV result is unchanged and x needs only n entries.
 //
@@ -24,6 +27,15 @@ using namespace swr_cg;
 constexpr double EV_TO_KCAL_PER_MOL = 14.4;
 
 // ---------------------------------------------------------------- H build
+// First slot of H row i: its Verlet offset plus 4 i, rounded up to a
+// multiple of 4.  Row i (at most voff[i+1] - voff[i] entries, padded to a
+// multiple of 4) ends at or before row i+1 starts, since
+// ceil(a/4) + ceil(c/4) <= ceil((a + c + 4)/4); the matrix needs
+// h_row_start(voff[n], n) slots.
+__device__ __host__ __forceinline__ long long h_row_start(long long voff_i, long long i) {
+  return (voff_i + 4 * i + 3) & ~3LL;
+}
+
 __global__ void __launch_bounds__(QT)
 k_build_H(int n, const DevParams *__restrict__ P, const double *__restrict__ x, const int *__restrict__ type,
           const int *__restrict__ owner, const long long *__restrict__ voff, const int *__restrict__ vlist,
@@ -41,6 +53,7 @@ k_build_H(int n, const DevParams *__restrict__ P, const double *__restrict__ x, 
 #pragma unroll
   for (int k = 0; k < 8; k++) T[k] = P->Tap[k];
   const long long beg = voff[i], end = voff[i + 1];
+  const long long dst = h_row_start(beg, i);
   int cnt = 0;
   for (long long base = beg; base < end; base += SWR_WAVE) {
     const long long k = base + lane;
@@ -75,15 +88,22 @@ k_build_H(int n, const DevParams *__restrict__ P, const double *__restrict__ x, 
     }
     const unsigned long long m = __ballot(hit);
     if (hit) {
-      const long long slot = beg + cnt + ballot_rank(m);
+      const long long slot = dst + cnt + ballot_rank(m);
       jlist[slot] = jo;
       if (val) val[slot] = hv;
       if (val32) val32[slot] = (float)hv;
     }
     cnt += __popcll(m);
   }
+  // padding to a multiple of 4 entries: column i with value 0
+  if (lane < ((cnt + 3) & ~3) - cnt) {
+    const long long slot = dst + cnt + lane;
+    jlist[slot] = i;
+    if (val) val[slot] = 0.0;
+    if (val32) val32[slot] = 0.0f;
+  }
   if (lane == 0) {
-    firstnbr[i] = (int)beg;
+    firstnbr[i] = (int)dst;
     numnbrs[i] = cnt;
     // init_matvec vectors (fix_qeq_reax_sunway.cpp:700-716)
     Hdia_inv[i] = 1. / P->eta[ti];
@@ -286,6 +306,44 @@ CgLayout swr_qeq_cg_layout(swr_system *S) {
   return L;
 }
 
+// Autotuner hook: the average time of `reps` launches of the FP64 CG's SpMV
+// in shape sh on the current H and vectors (the setup-phase instance, which
+// multiplies d0 into q and always runs; both are scratch between solves).
+int swr_qeq_time_spmv(swr_system *S, int sh, int reps, float *ms) {
+  int rc = swr_qeq_layout(S);
+  if (rc) return rc;
+  if (!S->H_val || S->n <= 0) return SWR_ERR_STATE;
+  CgVecsT<double> V = make_vecs(S);
+  const int n = S->n, nsp = div_up(n, RPB);
+  const CgScalars *cgs = S->cgs;
+  const int *row_sys = S->row_sys;
+  hipEvent_t e[2];
+  HIP_CHECK(hipEventCreate(&e[0]));
+  HIP_CHECK(hipEventCreate(&e[1]));
+  spmv_dispatch(sh, [&](auto shc) {
+    constexpr int SH = decltype(shc)::value;
+    const dim3 gb(nsp), tb(8 * SpmvShape<SH>::G);
+    for (int r = -1; r < reps; r++) {   // one untimed launch first
+      if (r == 0) (void)hipEventRecord(e[0], S->stream);
+      if (S->nsys > 1)
+        hipLaunchKernelGGL((k_cg_spmv<PH_S2, true, double, double, SH>), gb, tb, 0, S->stream, n, S->dp, S->type,
+                           S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, V, 0, cgs, row_sys);
+      else
+        hipLaunchKernelGGL((k_cg_spmv<PH_S2, false, double, double, SH>), gb, tb, 0, S->stream, n, S->dp, S->type,
+                           S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, V, 0, cgs, row_sys);
+    }
+    (void)hipEventRecord(e[1], S->stream);
+  });
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventSynchronize(e[1]));
+  float t = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&t, e[0], e[1]));
+  *ms = t / std::max(reps, 1);
+  (void)hipEventDestroy(e[0]);
+  (void)hipEventDestroy(e[1]);
+  return SWR_OK;
+}
+
 // Runs the FP64 solves of every system to convergence (first batch of
 // first_batch iteration launches, then batches of 4 between host checks).
 int swr_qeq_cg_run(swr_system *S, int first_batch) {
@@ -305,6 +363,8 @@ int swr_qeq_cg_run(swr_system *S, int first_batch) {
   R.kt_update = KT_CG_UPDATE;
   R.kt_setup = KT_CG_SETUP;
   R.kt_scalars = KT_CG_SCALARS;
+  R.partials = S->partials;
+  R.pstride = S->cg_pstride;
   return cg_solve(S, R, first_batch, 0);
 }
 

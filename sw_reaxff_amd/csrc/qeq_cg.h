@@ -5,8 +5,9 @@
 //
 // Vectors hold the s and t systems side by side (T2 = double2 or float2), so
 // each H entry is read once and feeds both solves.  An iteration is three
-// launches: k_cg_spmv (q = H d, half a wave per row, nothing else so the
-// gathers run at full occupancy), k_cg_upd (one thread per row, coalesced:
+// launches: k_cg_spmv (q = H d, G lanes per row with 16-byte loads of the
+// aligned rows, nothing else so the gathers run at full occupancy; the
+// shape is the autotuner's), k_cg_upd (one thread per row, coalesced:
 // the reference's B and C steps of iteration i -- x/p/ss/v/z updates -- and
 // the A step of iteration i+1 -- r/u/w updates, the two dots, d = M^-1 w) and
 // k_cg_scalars.  Each k_cg_upd workgroup writes its partial dots; the
@@ -16,6 +17,8 @@
 // scalars are FP64 whatever T is.  Each solve stops by the reference's test
 // sqrt(u.r)/|b| <= tol (or i = imax) and is frozen while the other runs.
 #pragma once
+
+#include <type_traits>
 
 #include "swr_internal.h"
 #include "swr_device.h"
@@ -127,43 +130,101 @@ __device__ inline void cg_after_A(CgScalars &c, double l0, double l1, int imax) 
   c.active_a = (c.iter < imax && sqrt(l0) / c.b_norm > c.tol) ? 1 : 0;
 }
 
-// Half-wave (32 lanes) per row, 2 rows per wave: four independent
-// index/value/gather loads in flight per lane; fixed summation order (per
-// lane a0..a3, then a fixed 32-lane tree).  Products accumulate in T.
-template <typename T, typename VT>
-__device__ __forceinline__ typename Vec2<T>::type row_spmv2_h(int beg, int cnt, const int *__restrict__ jlist,
-                                                              const VT *__restrict__ val,
-                                                              const typename Vec2<T>::type *__restrict__ xv) {
+// Four consecutive stored values of an aligned row as T (two 16-byte loads
+// of double pairs, or one of four floats).
+template <typename T, typename VT> struct Quad;
+template <> struct Quad<double, double> {
+  double v[4];
+  __device__ __forceinline__ void load(const double *p) {
+    const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+  }
+};
+template <> struct Quad<float, float> {
+  float v[4];
+  __device__ __forceinline__ void load(const float *p) {
+    const float4 a = *reinterpret_cast<const float4 *>(p);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  }
+};
+
+// G lanes per row over a row that starts 16-byte aligned and is padded to a
+// multiple of 4 entries (value 0): each lane takes groups of 4 consecutive
+// entries (one 16-byte index load, 16-byte value loads), U groups in flight,
+// group g of the row on lane g % G.  Fixed summation order (per lane two
+// accumulators per right-hand side, then a G-lane butterfly), so a row's sum
+// depends only on the row and (G, U).  Returns the sums on every lane of the
+// row's group.
+template <typename T, typename VT, int G, int U>
+__device__ __forceinline__ typename Vec2<T>::type row_spmv4(int beg, int cnt, const int *__restrict__ jlist,
+                                                            const VT *__restrict__ val,
+                                                            const typename Vec2<T>::type *__restrict__ xv) {
   typedef typename Vec2<T>::type T2;
-  const int sl = lane_id() & 31;
-  T a0 = 0, a1 = 0, a2 = 0, a3 = 0, b0 = 0, b1 = 0, b2 = 0, b3 = 0;
-  const int end = beg + cnt;
-  int k = beg + sl;
-  for (; k + 96 < end; k += 128) {
-    const int j0 = jlist[k], j1 = jlist[k + 32], j2 = jlist[k + 64], j3 = jlist[k + 96];
-    const T v0 = (T)val[k], v1 = (T)val[k + 32], v2 = (T)val[k + 64], v3 = (T)val[k + 96];
-    const T2 x0 = xv[j0], x1 = xv[j1], x2 = xv[j2], x3 = xv[j3];
-    a0 += v0 * x0.x; b0 += v0 * x0.y;
-    a1 += v1 * x1.x; b1 += v1 * x1.y;
-    a2 += v2 * x2.x; b2 += v2 * x2.y;
-    a3 += v3 * x3.x; b3 += v3 * x3.y;
-  }
-  for (; k < end; k += 32) {
-    const T2 xx = xv[jlist[k]];
-    const T v = (T)val[k];
-    a0 += v * xx.x;
-    b0 += v * xx.y;
-  }
-  T a = (a0 + a1) + (a2 + a3), b = (b0 + b1) + (b2 + b3);
+  const int sl = lane_id() % G;
+  const int n4 = (cnt + 3) >> 2;
+  const int4 *J = reinterpret_cast<const int4 *>(jlist + beg);
+  const VT *V = val + beg;
+  T a = 0, b = 0, c = 0, d = 0;
+  int g = sl;
+  for (; g + (U - 1) * G < n4; g += U * G) {
+    int4 jj[U];
+    Quad<T, VT> vv[U];
 #pragma unroll
-  for (int o = 16; o > 0; o >>= 1) {
-    a += __shfl_xor(a, o, SWR_WAVE);
-    b += __shfl_xor(b, o, SWR_WAVE);
+    for (int u = 0; u < U; u++) jj[u] = J[g + u * G];
+#pragma unroll
+    for (int u = 0; u < U; u++) vv[u].load(V + 4 * (g + u * G));
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const T2 x0 = xv[jj[u].x], x1 = xv[jj[u].y], x2 = xv[jj[u].z], x3 = xv[jj[u].w];
+      a += vv[u].v[0] * x0.x; b += vv[u].v[0] * x0.y;
+      c += vv[u].v[1] * x1.x; d += vv[u].v[1] * x1.y;
+      a += vv[u].v[2] * x2.x; b += vv[u].v[2] * x2.y;
+      c += vv[u].v[3] * x3.x; d += vv[u].v[3] * x3.y;
+    }
+  }
+  for (; g < n4; g += G) {
+    const int4 jj = J[g];
+    Quad<T, VT> vv;
+    vv.load(V + 4 * g);
+    const T2 x0 = xv[jj.x], x1 = xv[jj.y], x2 = xv[jj.z], x3 = xv[jj.w];
+    a += vv.v[0] * x0.x; b += vv.v[0] * x0.y;
+    c += vv.v[1] * x1.x; d += vv.v[1] * x1.y;
+    a += vv.v[2] * x2.x; b += vv.v[2] * x2.y;
+    c += vv.v[3] * x3.x; d += vv.v[3] * x3.y;
+  }
+  T s = a + c, t = b + d;
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) {
+    s += __shfl_xor(s, o, SWR_WAVE);
+    t += __shfl_xor(t, o, SWR_WAVE);
   }
   T2 r;
-  r.x = a;
-  r.y = b;
+  r.x = s;
+  r.y = t;
   return r;
+}
+
+// SpMV shapes the autotuner chooses from: G lanes per row, U groups of 4
+// entries in flight per lane; every shape runs 8 rows per workgroup (8 G
+// threads), so the row-block layout of a batch is the same for all.
+enum { SPMV_G8U4 = 0, SPMV_G16U2, SPMV_G16U4, SPMV_G32U2, SPMV_NSHAPES };
+constexpr int SPMV_DEFAULT = SPMV_G16U4;
+constexpr const char *kSpmvShapeNames[SPMV_NSHAPES] = {"G8U4", "G16U2", "G16U4", "G32U2"};
+template <int SH> struct SpmvShape;
+template <> struct SpmvShape<SPMV_G8U4> { static constexpr int G = 8, U = 4; };
+template <> struct SpmvShape<SPMV_G16U2> { static constexpr int G = 16, U = 2; };
+template <> struct SpmvShape<SPMV_G16U4> { static constexpr int G = 16, U = 4; };
+template <> struct SpmvShape<SPMV_G32U2> { static constexpr int G = 32, U = 2; };
+
+// Calls f(std::integral_constant<int, SH>) for the runtime shape sh.
+template <class F>
+__host__ inline void spmv_dispatch(int sh, F &&f) {
+  switch (sh) {
+    case SPMV_G8U4: f(std::integral_constant<int, SPMV_G8U4>()); break;
+    case SPMV_G16U2: f(std::integral_constant<int, SPMV_G16U2>()); break;
+    case SPMV_G32U2: f(std::integral_constant<int, SPMV_G32U2>()); break;
+    default: f(std::integral_constant<int, SPMV_G16U4>()); break;
+  }
 }
 
 template <typename T2, typename T>
@@ -185,15 +246,16 @@ __device__ __forceinline__ bool cg_any_active(int PH, const CgScalars *__restric
 
 // q = eta d + H d (sparse_matvec_C_para, fix_qeq_reax_sw64.c:424-466) of the
 // vector phase PH multiplies: x (PH_S1), d0 (PH_S2), d1 (PH_S3) or the
-// current d (PH_K).  Half a wave per row, two rows per wave.
-template <int PH, bool BATCH, typename T, typename VT>
-__global__ void __launch_bounds__(QT)
+// current d (PH_K).  G lanes per row (shape SH), 8 rows per workgroup.
+template <int PH, bool BATCH, typename T, typename VT, int SH>
+__global__ void __launch_bounds__(8 * SpmvShape<SH>::G)
 k_cg_spmv(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
           const int *__restrict__ numnbrs, const int *__restrict__ jlist, const VT *__restrict__ val, CgVecsT<T> V,
           int par_in, const CgScalars *__restrict__ cgs, const int *__restrict__ row_sys) {
   typedef typename Vec2<T>::type T2;
+  constexpr int G = SpmvShape<SH>::G, U = SpmvShape<SH>::U;
   const int w = wave_in_block(), lane = lane_id();
-  const int i = blockIdx.x * RPB + w * 2 + (lane >> 5);
+  const int i = blockIdx.x * RPB + w * (SWR_WAVE / G) + lane / G;
   // The row's extent is requested together with the solve state, so the
   // two latencies overlap (the state decides whether the row runs).
   const int beg0 = i < n ? firstnbr[i] : 0, cnt0 = i < n ? numnbrs[i] : 0;
@@ -205,8 +267,8 @@ k_cg_spmv(int n, const DevParams *__restrict__ P, const int *__restrict__ type, 
   const bool live = i < n && (!BATCH || cg_any_active(PH, cgs + 2 * rs));
   const T2 *src = PH == PH_S1 ? V.x : PH == PH_S2 ? V.d[0] : PH == PH_S3 ? V.d[1] : (par_in ? V.d[1] : V.d[0]);
   const int beg = live ? beg0 : 0, cnt = live ? cnt0 : 0;
-  T2 q = row_spmv2_h<T, VT>(beg, cnt, jlist, val, src);
-  if ((lane & 31) == 0 && live) {
+  T2 q = row_spmv4<T, VT, G, U>(beg, cnt, jlist, val, src);
+  if (lane % G == 0 && live) {
     const T eta = (T)P->eta[type[i]];
     const T2 xi = src[i];
     q.x = eta * xi.x + q.x;
@@ -374,41 +436,47 @@ struct CgRun {
   double tol;         // > 0: set at setup; <= 0: preset in cgs[].tol
   int imax;           // iteration limit (the reference's imax = 200 for the FP64 solve)
   int kt_iter, kt_update, kt_setup, kt_scalars;
+  double *partials;   // per-workgroup partial dots, [8][pstride]
+  int pstride;
 };
 
 template <typename T, typename VT>
 int launch_cg(swr_system *S, const CgRun<T, VT> &R, int ph, int par_in, int xzero) {
   const int n = S->n;
   const int nsp = div_up(n, RPB);
-  const int ps = S->cg_pstride;
+  const int ps = R.pstride;
   const int kt = ph == PH_K ? R.kt_iter : R.kt_setup;
   const int ktu = ph == PH_K ? R.kt_update : R.kt_setup;
   const int kts = ph == PH_K ? R.kt_scalars : R.kt_setup;
 #define SP_ARGS n, S->dp, S->type, S->H_firstnbr, S->H_numnbrs, S->H_jlist, R.val, R.V, par_in, R.cgs, R.L.row_sys
-#define UP_ARGS R.V, par_in, R.cgs, R.L, S->partials, ps, xzero
-#define SC_ARGS S->partials, ps, R.cgs, R.L, R.imax, R.tol, S->status, R.iters_out
+#define UP_ARGS R.V, par_in, R.cgs, R.L, R.partials, ps, xzero
+#define SC_ARGS R.partials, ps, R.cgs, R.L, R.imax, R.tol, S->status, R.iters_out
   swr_kt_begin(S, kt);
-  if (R.L.nsys > 1) {
-    switch (ph) {
-      case PH_S1:
-        if (!xzero) hipLaunchKernelGGL((k_cg_spmv<PH_S1, true, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS);
-        break;
-      case PH_S2: hipLaunchKernelGGL((k_cg_spmv<PH_S2, true, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
-      case PH_S3: hipLaunchKernelGGL((k_cg_spmv<PH_S3, true, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
-      case PH_A1: break;
-      default: hipLaunchKernelGGL((k_cg_spmv<PH_K, true, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
+  spmv_dispatch(S->spmv_shape[0], [&](auto shc) {
+    constexpr int SH = decltype(shc)::value;
+    const dim3 gb(nsp), tb(8 * SpmvShape<SH>::G);
+    if (R.L.nsys > 1) {
+      switch (ph) {
+        case PH_S1:
+          if (!xzero) hipLaunchKernelGGL((k_cg_spmv<PH_S1, true, T, VT, SH>), gb, tb, 0, S->stream, SP_ARGS);
+          break;
+        case PH_S2: hipLaunchKernelGGL((k_cg_spmv<PH_S2, true, T, VT, SH>), gb, tb, 0, S->stream, SP_ARGS); break;
+        case PH_S3: hipLaunchKernelGGL((k_cg_spmv<PH_S3, true, T, VT, SH>), gb, tb, 0, S->stream, SP_ARGS); break;
+        case PH_A1: break;
+        default: hipLaunchKernelGGL((k_cg_spmv<PH_K, true, T, VT, SH>), gb, tb, 0, S->stream, SP_ARGS); break;
+      }
+    } else {
+      switch (ph) {
+        case PH_S1:
+          if (!xzero) hipLaunchKernelGGL((k_cg_spmv<PH_S1, false, T, VT, SH>), gb, tb, 0, S->stream, SP_ARGS);
+          break;
+        case PH_S2: hipLaunchKernelGGL((k_cg_spmv<PH_S2, false, T, VT, SH>), gb, tb, 0, S->stream, SP_ARGS); break;
+        case PH_S3: hipLaunchKernelGGL((k_cg_spmv<PH_S3, false, T, VT, SH>), gb, tb, 0, S->stream, SP_ARGS); break;
+        case PH_A1: break;
+        default: hipLaunchKernelGGL((k_cg_spmv<PH_K, false, T, VT, SH>), gb, tb, 0, S->stream, SP_ARGS); break;
+      }
     }
-  } else {
-    switch (ph) {
-      case PH_S1:
-        if (!xzero) hipLaunchKernelGGL((k_cg_spmv<PH_S1, false, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS);
-        break;
-      case PH_S2: hipLaunchKernelGGL((k_cg_spmv<PH_S2, false, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
-      case PH_S3: hipLaunchKernelGGL((k_cg_spmv<PH_S3, false, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
-      case PH_A1: break;
-      default: hipLaunchKernelGGL((k_cg_spmv<PH_K, false, T, VT>), dim3(nsp), dim3(QT), 0, S->stream, SP_ARGS); break;
-    }
-  }
+  });
   swr_kt_end(S, kt);
   swr_kt_begin(S, ktu);
   const dim3 gt(R.L.ntiles);

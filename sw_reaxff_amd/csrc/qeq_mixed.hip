@@ -101,8 +101,8 @@ __device__ __forceinline__ bool pcg_live(const PcgState *__restrict__ c) { retur
 // right-hand side of the inner solve.  Half-wave per row as in k_cg.  Rows
 // of systems whose refinement has finished are skipped (not on the first
 // pass, whose state is the previous solve's).
-template <bool BATCH>
-__global__ void __launch_bounds__(QT)
+template <bool BATCH, int SH>
+__global__ void __launch_bounds__(8 * SpmvShape<SH>::G)
 k_resid(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
         const int *__restrict__ numnbrs, const int *__restrict__ jlist, const double *__restrict__ val,
         const double2 *__restrict__ x, const double2 *__restrict__ b, const double *__restrict__ Hd,
@@ -111,12 +111,13 @@ k_resid(int n, const DevParams *__restrict__ P, const int *__restrict__ type, co
   int r0, r1;
   sblk_rows<BATCH>(n, sblk_beg, r0, r1);
   if (BATCH && !first && r0 < r1 && ms[row_sys[r0]].done) return;
+  constexpr int G = SpmvShape<SH>::G, U = SpmvShape<SH>::U;
   const int w = wave_in_block(), lane = lane_id();
-  const int i = r0 + w * 2 + (lane >> 5);
+  const int i = r0 + w * (SWR_WAVE / G) + lane / G;
   const bool in = i < r1;
-  const bool owner = (lane & 31) == 0 && in;
+  const bool owner = lane % G == 0 && in;
   const int beg = in ? firstnbr[i] : 0, cnt = in ? numnbrs[i] : 0;
-  double2 q = row_spmv2_h<double, double>(beg, cnt, jlist, val, x);
+  double2 q = row_spmv4<double, double, G, U>(beg, cnt, jlist, val, x);
   double d[6] = {0, 0, 0, 0, 0, 0};
   if (owner) {
     const double eta = P->eta[type[i]];
@@ -128,7 +129,7 @@ k_resid(int n, const DevParams *__restrict__ P, const int *__restrict__ type, co
     d[4] = bi.x * bi.x; d[5] = bi.y * bi.y;
     r32[i] = make_float2((float)rs, (float)rt);
   }
-  block_partials<6, QT>(d, partials, pstride, blockIdx.x);
+  block_partials<6, 8 * G>(d, partials, pstride, blockIdx.x);
 }
 
 // Outer test and the inner tolerances, one workgroup per system (over its
@@ -260,25 +261,27 @@ k_pcg_init_scalars(const double *__restrict__ partials, int pstride, CgLayout L,
   if (!pcg_live(c)) atomicAdd(&ctl->inner_ndone, 1);
 }
 
-template <bool BATCH>
-__global__ void __launch_bounds__(QT)
+template <bool BATCH, int SH>
+__global__ void __launch_bounds__(8 * SpmvShape<SH>::G)
 k_pcg_spmv(int n, int nsys, const DevParams *__restrict__ P, const int *__restrict__ type,
            const int *__restrict__ firstnbr, const int *__restrict__ numnbrs, const int *__restrict__ jlist,
            const float *__restrict__ val, const float2 *__restrict__ p, float2 *__restrict__ q,
            const MixCtl *__restrict__ ctl, const PcgState *__restrict__ pst, double *__restrict__ partials,
-           int pstride, const int *__restrict__ sblk_beg, const int *__restrict__ row_sys) {
+           int pstride, const int *__restrict__ sblk_beg, const int *__restrict__ row_sys, int gate = 1) {
   int r0, r1;
   sblk_rows<BATCH>(n, sblk_beg, r0, r1);
+  constexpr int G = SpmvShape<SH>::G, U = SpmvShape<SH>::U;
   const int w = wave_in_block(), lane = lane_id();
-  const int i = r0 + w * 2 + (lane >> 5);
+  const int i = r0 + w * (SWR_WAVE / G) + lane / G;
   const bool in = i < r1;
   // the row's extent is requested together with the solve state
   const int beg0 = in ? firstnbr[i] : 0, cnt0 = in ? numnbrs[i] : 0;
-  if (ctl->inner_ndone >= nsys && cnt0 >= 0) return;
-  if (BATCH && r0 < r1 && !pcg_live(pst + 2 * row_sys[r0])) return;
-  const float2 sum = row_spmv2_h<float, float>(beg0, cnt0, jlist, val, p);
+  // (gate 0: the autotuner's timing launches, which run every row)
+  if (gate && ctl->inner_ndone >= nsys && cnt0 >= 0) return;
+  if (gate && BATCH && r0 < r1 && !pcg_live(pst + 2 * row_sys[r0])) return;
+  const float2 sum = row_spmv4<float, float, G, U>(beg0, cnt0, jlist, val, p);
   double dd[2] = {0, 0};
-  if ((lane & 31) == 0 && in) {
+  if (lane % G == 0 && in) {
     const float eta = (float)P->eta[type[i]];
     const float2 pi = p[i];
     const float2 qq = make_float2(eta * pi.x + sum.x, eta * pi.y + sum.y);
@@ -286,7 +289,7 @@ k_pcg_spmv(int n, int nsys, const DevParams *__restrict__ P, const int *__restri
     dd[0] = (double)pi.x * qq.x;
     dd[1] = (double)pi.y * qq.y;
   }
-  block_partials<2, QT>(dd, partials, pstride, blockIdx.x);
+  block_partials<2, 8 * G>(dd, partials, pstride, blockIdx.x);
 }
 
 template <int NT>
@@ -449,14 +452,18 @@ int swr_qeq_mixed_solve(swr_system *S) {
   int outer = 0;
   for (;; outer++) {
     swr_kt_begin(S, KT_MIXED_OUTER);
-    if (batch)
-      hipLaunchKernelGGL(k_resid<true>, dim3(nblk), dim3(QT), 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
-                         S->H_numnbrs, S->H_jlist, S->H_val, (const double2 *)x, b, S->Hdia_inv, rhs, S->partials,
-                         ps, S->sblk_beg, S->row_sys, ms, outer == 0 ? 1 : 0);
-    else
-      hipLaunchKernelGGL(k_resid<false>, dim3(nblk), dim3(QT), 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
-                         S->H_numnbrs, S->H_jlist, S->H_val, (const double2 *)x, b, S->Hdia_inv, rhs, S->partials,
-                         ps, S->sblk_beg, S->row_sys, ms, outer == 0 ? 1 : 0);
+    spmv_dispatch(S->spmv_shape[2], [&](auto shc) {
+      constexpr int SH = decltype(shc)::value;
+      const dim3 tb(8 * SpmvShape<SH>::G);
+      if (batch)
+        hipLaunchKernelGGL((k_resid<true, SH>), dim3(nblk), tb, 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
+                           S->H_numnbrs, S->H_jlist, S->H_val, (const double2 *)x, b, S->Hdia_inv, rhs, S->partials,
+                           ps, S->sblk_beg, S->row_sys, ms, outer == 0 ? 1 : 0);
+      else
+        hipLaunchKernelGGL((k_resid<false, SH>), dim3(nblk), tb, 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
+                           S->H_numnbrs, S->H_jlist, S->H_val, (const double2 *)x, b, S->Hdia_inv, rhs, S->partials,
+                           ps, S->sblk_beg, S->row_sys, ms, outer == 0 ? 1 : 0);
+    });
     if (batch)
       hipLaunchKernelGGL(k_resid_scalars<BT>, dim3(nsys), dim3(BT), 0, S->stream, S->partials, ps, S->sys_sblk, ctl,
                          ms, inner, S->ctl.tol * OUTER_TOL_FACTOR, outer == 0 ? 1 : 0, S->status, S->sys_iters);
@@ -485,14 +492,18 @@ int swr_qeq_mixed_solve(swr_system *S) {
     int nb = std::max(2, S->mixed_last_inner + 1);
     while (it < INNER_IMAX) {
       for (int k = 0; k < nb && it < INNER_IMAX; k++, it++) {
-        if (batch)
-          hipLaunchKernelGGL(k_pcg_spmv<true>, dim3(nblk), dim3(QT), 0, S->stream, n, nsys, S->dp, S->type,
-                             S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
-                             S->partials, ps, S->sblk_beg, S->row_sys);
-        else
-          hipLaunchKernelGGL(k_pcg_spmv<false>, dim3(nblk), dim3(QT), 0, S->stream, n, nsys, S->dp, S->type,
-                             S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
-                             S->partials, ps, S->sblk_beg, S->row_sys);
+        spmv_dispatch(S->spmv_shape[1], [&](auto shc) {
+          constexpr int SH = decltype(shc)::value;
+          const dim3 tb(8 * SpmvShape<SH>::G);
+          if (batch)
+            hipLaunchKernelGGL((k_pcg_spmv<true, SH>), dim3(nblk), tb, 0, S->stream, n, nsys, S->dp, S->type,
+                               S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
+                               S->partials, ps, S->sblk_beg, S->row_sys);
+          else
+            hipLaunchKernelGGL((k_pcg_spmv<false, SH>), dim3(nblk), tb, 0, S->stream, n, nsys, S->dp, S->type,
+                               S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
+                               S->partials, ps, S->sblk_beg, S->row_sys);
+        });
         if (batch)
           hipLaunchKernelGGL(k_pcg_alpha<BT>, dim3(nsys), dim3(BT), 0, S->stream, S->partials, ps, S->sys_sblk, pst);
         else
@@ -523,5 +534,63 @@ int swr_qeq_mixed_solve(swr_system *S) {
     HIP_CHECK(hipGetLastError());
   }
   S->mixed_outer = outer + 1;
+  return SWR_OK;
+}
+
+// Autotuner hook: the average time of `reps` launches of the mixed solve's
+// FP64 residual (which = 0) or FP32 inner SpMV (which = 1) in shape sh on the
+// current H, after at least one mixed-precision solve (its workspace and
+// vectors are scratch between solves).
+int swr_qeq_time_mixed(swr_system *S, int which, int sh, int reps, float *ms) {
+  if (!S->mixed || !S->mixw || !S->H_val32 || S->n <= 0) return SWR_ERR_STATE;
+  const int n = S->n, nsys = S->nsys;
+  const bool batch = nsys > 1;
+  const int nblk = batch ? S->nsblk : div_up(n, RPB);
+  const int ps = S->cg_pstride;
+  char *w = (char *)S->mixw;
+  MixCtl *ctl = (MixCtl *)w;
+  MixState *mst = (MixState *)(w + MS_OFF);
+  PcgState *pst = (PcgState *)(w + pcg_off(nsys));
+  float2 *base = (float2 *)(w + vec_off(nsys));
+  const size_t c = S->ncap;
+  float2 *pv = base + MV_P * c, *qv = base + MV_Q * c, *rhs = base + MV_B * c;
+  const double2 *x = (const double2 *)S->st, *b = (const double2 *)S->b_s;
+  hipEvent_t e[2];
+  HIP_CHECK(hipEventCreate(&e[0]));
+  HIP_CHECK(hipEventCreate(&e[1]));
+  spmv_dispatch(sh, [&](auto shc) {
+    constexpr int SH = decltype(shc)::value;
+    const dim3 tb(8 * SpmvShape<SH>::G);
+    for (int r = -1; r < reps; r++) {
+      if (r == 0) (void)hipEventRecord(e[0], S->stream);
+      if (which == 0) {
+        if (batch)
+          hipLaunchKernelGGL((k_resid<true, SH>), dim3(nblk), tb, 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
+                             S->H_numnbrs, S->H_jlist, S->H_val, x, b, S->Hdia_inv, rhs, S->partials, ps, S->sblk_beg,
+                             S->row_sys, mst, 1);
+        else
+          hipLaunchKernelGGL((k_resid<false, SH>), dim3(nblk), tb, 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
+                             S->H_numnbrs, S->H_jlist, S->H_val, x, b, S->Hdia_inv, rhs, S->partials, ps, S->sblk_beg,
+                             S->row_sys, mst, 1);
+      } else {
+        if (batch)
+          hipLaunchKernelGGL((k_pcg_spmv<true, SH>), dim3(nblk), tb, 0, S->stream, n, nsys, S->dp, S->type,
+                             S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
+                             S->partials, ps, S->sblk_beg, S->row_sys, 0);
+        else
+          hipLaunchKernelGGL((k_pcg_spmv<false, SH>), dim3(nblk), tb, 0, S->stream, n, nsys, S->dp, S->type,
+                             S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
+                             S->partials, ps, S->sblk_beg, S->row_sys, 0);
+      }
+    }
+    (void)hipEventRecord(e[1], S->stream);
+  });
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventSynchronize(e[1]));
+  float t = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&t, e[0], e[1]));
+  *ms = t / std::max(reps, 1);
+  (void)hipEventDestroy(e[0]);
+  (void)hipEventDestroy(e[1]);
   return SWR_OK;
 }

@@ -32,6 +32,9 @@
 #define SWR_F_FIXED_RANGE (1u << 9)  // a bonded term beyond the fixed-point accumulators' range
 #define SWR_F_CG_MAXIT   (1u << 8)   // a QEq solve stopped at its iteration limit (a warning, as in the reference)
 
+// far-pass gatherer/writer wave splits (lists.hip) the autotuner chooses from
+enum { SWR_FAR_G4W2 = 0, SWR_FAR_G4W4, SWR_FAR_G6W2, SWR_FAR_G8W4, SWR_FAR_NSHAPES };
+
 #define SWR_MAX_ATOM_BONDS 32        // bonds one atom may hold while building its list (the reference: MAX_LEN 20, MAXBOND 35)
 
 #define HIP_CHECK(expr)                                                    \
@@ -198,7 +201,11 @@ struct swr_system {
   double binlo[3] = {0, 0, 0}, binsize[3] = {0, 0, 0}, bininv[3] = {0, 0, 0};
   int nbins = 0, binscap = 0;
   size_t cap_stencil = 0, cap_bin_of = 0, cap_bp_id = 0, cap_bp_pos = 0, cap_vcount = 0, cap_voff = 0;
-  int *bin_of = nullptr, *bin_count = nullptr, *bin_start = nullptr, *bin_fill = nullptr;
+  int *bin_of = nullptr, *bin_count = nullptr, *bin_start = nullptr;
+  // counting sort by bin (radix passes staged in LDS): key/value ping-pong
+  // buffers and the per-(digit, block) counts and starts
+  int *rs_keys = nullptr, *rs_vals = nullptr, *rs_hist = nullptr;
+  size_t cap_rs_keys = 0, cap_rs_vals = 0, cap_rs_hist = 0;
   double4 *bp_pos = nullptr;     // atoms sorted by bin: x, y, z, unused
   int *bp_id = nullptr;
   int nrows = 0;
@@ -218,6 +225,7 @@ struct swr_system {
   int *bond_index = nullptr, *bond_end = nullptr;
   int bond_cap = 0;
   swr_bond_t *bonds = nullptr;
+  int *bond_nbr = nullptr;         // [bond_cap] bonds[].nbr as a compact array (sym_index searches, marks)
   swr_bond_order_t *bo_data = nullptr;
   double *BO = nullptr, *BOpi = nullptr, *Cdbo = nullptr, *Cdbopi = nullptr, *Cdbopi2 = nullptr;
   int *hbond_index = nullptr, *hbond_end = nullptr;   // [ncap] by Hindex
@@ -321,6 +329,12 @@ struct swr_system {
   int mixed_last_inner = 0;        // iterations of the last inner FP32 solve (sizes the next launch batch)
   double mixed_rho0[2] = {0, 0};   // its first outer residual sqrt(u.r)/|b| (s, t)
   int imax = 200;
+  // autotuned launch shapes (csrc/tune.cpp): SpMV shape of the FP64 CG [0],
+  // of the FP32 inner solve [1] and of the FP64 residual [2], gatherer/
+  // writer split of the far pass (-1: the default)
+  int spmv_shape[3] = {-1, -1, -1};
+  int far_shape = -1;
+  std::string tune_report;         // JSON text of the last swr_autotune
 
   // ---- status
   DevStatus *status = nullptr;     // device
@@ -396,6 +410,10 @@ int swr_qeq_cg_run(swr_system *S, int first_batch);
 int swr_qeq_calc_q(swr_system *S);
 int swr_qeq_layout(swr_system *S);
 int swr_qeq_mixed_solve(swr_system *S);
+// autotuner timing hooks (qeq.hip, qeq_mixed.hip, lists.hip)
+int swr_qeq_time_spmv(swr_system *S, int sh, int reps, float *ms);
+int swr_qeq_time_mixed(swr_system *S, int which, int sh, int reps, float *ms);
+int swr_lists_time_far(swr_system *S, int sh, int reps, float *ms);
 
 template <class T>
 inline int swr_alloc(T **p, size_t count) {
