@@ -368,9 +368,9 @@ __global__ void __launch_bounds__(BT) k_dbond_nbr(int N, DbArgs A) {
 // over the atom's hydrogen-bond entries: the energy, CEhb1 and j's own
 // terms are summed by a fixed butterfly; the terms of i (summed over the
 // entries, one set per bond) and of k (summed over the bonds, one set per
-// entry) go into fixed-point accumulators (fix_add: 2^-40 resolution below
-// 2^20, sums up to 2^42) by integer atomics, whose sums do not depend on the
-// order they arrive in; k_hb_fold adds them to fCdDelta and clears them.
+// entry) go into fixed-point accumulators (fix_add: 2^-40 resolution for
+// every term, sums up to 2^43) by integer atomics, whose sums do not depend
+// on the order they arrive in; k_hb_fold adds them to fCdDelta and clears them.
 // The results are the same on every run.
 constexpr double HB_THRESHOLD = 1e-2;   // reaxc_defs_sunway.h:91
 constexpr double HB_FIX = 1099511627776.0;   // 2^40
@@ -427,16 +427,19 @@ __device__ __forceinline__ void hb_triple(const DevHb &hbp, double BOij, const d
 }
 
 // One fixed-point accumulator is a pair of 64-bit words: a fine word in
-// units of 2^-40 and a coarse word in units of 2^-20, coff elements after
-// it.  A term below 2^20 in magnitude goes to the fine word whole; a larger
-// one is split exactly into a multiple of 2^-20 (coarse word) and a
-// remainder below 2^-21 (fine word), so the pair holds sums up to 2^42 and
-// small terms keep the 2^-40 resolution.  Both words are integers, so the
-// sums do not depend on the order the adds arrive in.  A non-finite term,
-// a term beyond 2^42, or an add that wraps a word (detected from the value
-// atomicAdd returns) raises SWR_F_FIXED_RANGE, which the next swr_sync
+// units of 2^-40 (range 2^23) and a coarse word in units of 2^-20 (range
+// 2^43), coff elements after it.  A term below 2^10 in magnitude goes to the
+// fine word whole; a larger one is split exactly into a multiple of 2^-20
+// (coarse word) and a remainder of at most 2^-21 (fine word).  Every term
+// thus keeps the 2^-40 resolution (at most 2^-41 rounding per add), the fine
+// word holds at most 2^10 per add (8192 adds before it could wrap), and the
+// pair holds sums up to 2^43.  Both words are integers, so the sums do not
+// depend on the order the adds arrive in.  A non-finite term, a term beyond
+// ~3.8e12 (2^62 coarse units), or an add that wraps a word (detected from the
+// value atomicAdd returns) raises SWR_F_FIXED_RANGE, which the next swr_sync
 // reports as SWR_ERR_NUMERIC.
-constexpr double FIX_COARSE = 1048576.0;   // 2^20: coarse units and the split threshold
+constexpr double FIX_COARSE = 1048576.0;   // 2^20: coarse units
+constexpr double FIX_SPLIT = 1024.0;       // terms at least this large are split
 
 __device__ __forceinline__ void fix_word_add(unsigned int *flags, unsigned long long *a, long long d) {
   const long long old = (long long)atomicAdd(a, (unsigned long long)d);
@@ -445,9 +448,9 @@ __device__ __forceinline__ void fix_word_add(unsigned int *flags, unsigned long 
 }
 
 __device__ __forceinline__ void fix_add(unsigned int *flags, unsigned long long *a, size_t coff, double v) {
-  if (!(fabs(v) < FIX_COARSE)) {
+  if (!(fabs(v) < FIX_SPLIT)) {
     const double hi = rint(v * FIX_COARSE);
-    if (!(fabs(hi) < 4.0e18)) {   // not finite, or beyond 2^42
+    if (!(fabs(hi) < 4.0e18)) {   // not finite, or beyond ~3.8e12
       atomicOr(flags, SWR_F_FIXED_RANGE);
       return;
     }
