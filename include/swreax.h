@@ -164,6 +164,19 @@ int swr_sparse_matvec_h(swr_system *sys, const double *x_host, double *b_host);
  * called at :576-577).  Synchronises; writes the iteration counts. */
 int swr_qeq_cg(swr_system *sys, int *iters_s, int *iters_t);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the FP64 solve of swr_qeq_cg on sys's H, allocating
+ * nothing.  b (right-hand sides) and x (warm start in, solution out) are
+ * device arrays of n (s, t) pairs, interleaved; every other buffer the
+ * kernels use (CG vectors, partial dots, scalars, counts, layout) lives in a
+ * per-GPU static device array of about 5.8 MB, held until the process ends.
+ * At most 32768 local atoms, one system, FP64 QEq (SWR_ERR_ARG otherwise);
+ * needs swr_qeq_init_matvec first.  Synchronises; writes the iteration
+ * counts.  Every call on one GPU shares that array: calls that overlap on
+ * one GPU race. */
+int swr_qeq_cg_noalloc(swr_system *sys, const double *b, double *x, int *iters_s, int *iters_t);
+
 /* a16: q = s - (sum s / sum t) t, history shift, ghost q refresh
  * (FixQEqReaxSunway::calculate_Q, fix_qeq_reax_sunway.cpp:1697-1755). */
 int swr_calculate_q(swr_system *sys);

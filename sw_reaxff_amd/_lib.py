@@ -33,6 +33,7 @@ EXPORTS = [
     "swr_hist_free", "swr_atom_energy", "swr_add_dbond_forces", "swr_hydrogen_bonds",
     "swr_compute_forces", "swr_set_masses", "swr_valence_angles", "swr_torsion_angles", "swr_fix_nve_initial_integrate", "swr_fix_nve_final_integrate",
     "swr_autotune", "swr_autotune_load", "swr_set_shape", "swr_get_shape", "swr_tune_report",
+    "swr_qeq_cg_noalloc",
 ]
 
 
@@ -84,6 +85,7 @@ def lib():
         L.swr_sparse_matvec.argtypes = [vp, vp, vp]
         L.swr_sparse_matvec_h.argtypes = [vp, dp, dp]
         L.swr_qeq_cg.argtypes = [vp, ip, ip]
+        L.swr_qeq_cg_noalloc.argtypes = [vp, vp, vp, ip, ip]
         L.swr_qeq_pre_force.argtypes = [vp, ip, ip]
         L.swr_get.restype = i64
         L.swr_get.argtypes = [vp, C.c_char_p, vp]

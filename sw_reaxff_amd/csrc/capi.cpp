@@ -233,6 +233,8 @@ int status_to_rc(unsigned int f) {
 
 }  // namespace
 
+int swr_status_to_rc(unsigned int f) { return status_to_rc(f); }
+
 // Grow every per-atom array to hold Ncap atoms (contents preserved).
 int swr_nbr_ensure_atoms(swr_system *S, int Ncap) {
   if (Ncap <= S->Ncap) return SWR_OK;
@@ -344,6 +346,12 @@ int swr_create(const char *ffield_path, int ntypes, const char *const *elements,
   }
   int rc;
   if ((rc = swr_alloc(&S->dp, 1)) || (rc = swr_alloc(&S->status, 1)) || (rc = swr_alloc(&S->counters, 4))) {
+    swr_destroy(S);
+    return rc;
+  }
+  // the device address of the no-allocation entry point's scratch, once
+  // per device (with this device current)
+  if ((rc = swr_noalloc_init(S->device))) {
     swr_destroy(S);
     return rc;
   }
@@ -640,6 +648,11 @@ int swr_qeq_cg(swr_system *S, int *iters_s, int *iters_t) {
   if (iters_s) *iters_s = ms;
   if (iters_t) *iters_t = mt;
   return status_to_rc(S->status_host->flags);
+}
+
+int swr_qeq_cg_noalloc(swr_system *S, const double *b, double *x, int *iters_s, int *iters_t) {
+  if (!S) return SWR_ERR_ARG;
+  return swr_qeq_cg_noalloc_run(S, b, x, iters_s, iters_t);
 }
 
 int swr_set_batch(swr_system *S, int nsys, const int *row_offsets) {

@@ -20,6 +20,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 namespace {
 
@@ -449,4 +451,145 @@ extern "C" int swr_hist_free(swr_hist_store *h) {
   if (h->dev) (void)hipFree(h->dev);
   delete h;
   return SWR_OK;
+}
+
+// ---------------------------------------------------------------- no-allocation CG
+// swr_qeq_cg_noalloc runs the FP64 solve above for callers that cannot
+// allocate device memory (code under graph capture, tight serving loops):
+// the caller passes the right-hand sides and the solutions, and everything
+// else the kernels need beyond sys's H -- the eleven CG vectors, the
+// per-workgroup partial dots, the CG scalars, the iteration counts and the
+// one-system tile layout -- lives in `scratch`, a file-scope device array of
+// NA_BYTES per GPU, held until the process ends.  Its device address is
+// taken once per device at set-up (swr_noalloc_init, called by swr_create)
+// with hipGetSymbolAddress; host code uses only that address.  Every call
+// on one GPU shares its array: calls that overlap on one GPU race.
+namespace {
+
+constexpr int NA_MAX_ROWS = 32768;                       // rows (local atoms) at most
+constexpr int NA_TILES = NA_MAX_ROWS / UT + 1;           // vector-kernel tiles at most
+constexpr int NA_PSTRIDE = NA_MAX_ROWS / RPB + NA_TILES; // >= max(SpMV blocks, tiles)
+constexpr size_t NA_VEC = sizeof(double2) * NA_MAX_ROWS;
+constexpr size_t NA_OFF_PART = (size_t)CGV_COUNT * NA_VEC;
+constexpr size_t NA_OFF_CGS = NA_OFF_PART + sizeof(double) * 8 * NA_PSTRIDE;
+constexpr size_t NA_OFF_ROWSYS = (NA_OFF_CGS + sizeof(CgScalars) * 4 + 255) & ~(size_t)255;
+constexpr size_t NA_OFF_TBEG = NA_OFF_ROWSYS + sizeof(int) * NA_MAX_ROWS;
+constexpr size_t NA_OFF_TSYS = NA_OFF_TBEG + sizeof(int) * (NA_TILES + 1);
+constexpr size_t NA_OFF_STILE = NA_OFF_TSYS + sizeof(int) * NA_TILES;
+constexpr size_t NA_OFF_ITERS = NA_OFF_STILE + sizeof(int) * 2;
+constexpr size_t NA_OFF_DONE = NA_OFF_ITERS + sizeof(int) * 2;
+constexpr size_t NA_BYTES = (NA_OFF_DONE + sizeof(int) + 255) & ~(size_t)255;
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mu;
+unsigned char *na_ptr[NA_MAX_DEVICES] = {};
+
+// One-system layout of the vector kernels in the scratch: tiles of UT rows,
+// all rows of system 0; the CG scalars start at zero.
+__global__ void k_na_layout(int n, int ntiles, int *__restrict__ row_sys, int *__restrict__ tile_beg,
+                            int *__restrict__ tile_sys, int *__restrict__ sys_tile, CgScalars *__restrict__ cgs) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) row_sys[k] = 0;
+  if (k <= ntiles) tile_beg[k] = k < ntiles ? k * UT : n;
+  if (k < ntiles) tile_sys[k] = 0;
+  if (k == 0) { sys_tile[0] = 0; sys_tile[1] = ntiles; }
+  if (k < 4) cgs[k] = CgScalars{};
+}
+
+}  // namespace
+
+// The scratch array itself; host code never names it after swr_noalloc_init.
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+int swr_noalloc_init(int device) {
+  if (device < 0 || device >= NA_MAX_DEVICES) {
+    swr_set_error_msg("device index beyond the no-allocation entry point's table");
+    return SWR_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> g(na_mu);
+  if (na_ptr[device]) return SWR_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    swr_set_error_msg(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return SWR_ERR_HIP;
+  }
+  na_ptr[device] = (unsigned char *)p;
+  return SWR_OK;
+}
+
+int swr_qeq_cg_noalloc_run(swr_system *S, const double *b, double *x, int *iters_s, int *iters_t) {
+  const int n = S->n;
+  if (n <= 0 || !S->H_val || !S->Hdia_inv) {
+    swr_set_error_msg("swr_qeq_cg_noalloc: no H (call swr_qeq_init_matvec first)");
+    return SWR_ERR_STATE;
+  }
+  if (n > NA_MAX_ROWS) {
+    swr_set_error_msg("swr_qeq_cg_noalloc: " + std::to_string(n) + " rows, at most " +
+                      std::to_string(NA_MAX_ROWS));
+    return SWR_ERR_ARG;
+  }
+  if (S->nsys > 1 || S->mixed) {
+    swr_set_error_msg("swr_qeq_cg_noalloc: one system, FP64 QEq only");
+    return SWR_ERR_ARG;
+  }
+  if (!b || !x) return SWR_ERR_ARG;
+  unsigned char *base;
+  {
+    std::lock_guard<std::mutex> g(na_mu);
+    base = (S->device >= 0 && S->device < NA_MAX_DEVICES) ? na_ptr[S->device] : nullptr;
+  }
+  if (!base) {
+    swr_set_error_msg("swr_qeq_cg_noalloc: scratch address of this device not set up");
+    return SWR_ERR_STATE;
+  }
+  const int ntiles = div_up(n, UT);
+  double2 *vb = (double2 *)base;
+  CgVecsT<double> V;
+  V.r = vb + CGV_R * (size_t)NA_MAX_ROWS; V.u = vb + CGV_U * (size_t)NA_MAX_ROWS;
+  V.w = vb + CGV_W * (size_t)NA_MAX_ROWS; V.z = vb + CGV_Z * (size_t)NA_MAX_ROWS;
+  V.ss = vb + CGV_SS * (size_t)NA_MAX_ROWS; V.v = vb + CGV_V * (size_t)NA_MAX_ROWS;
+  V.p = vb + CGV_P * (size_t)NA_MAX_ROWS; V.m = vb + CGV_M * (size_t)NA_MAX_ROWS;
+  V.d[0] = vb + CGV_D0 * (size_t)NA_MAX_ROWS; V.d[1] = vb + CGV_D1 * (size_t)NA_MAX_ROWS;
+  V.q = vb + CGV_Q * (size_t)NA_MAX_ROWS;
+  V.x = (double2 *)x;
+  V.b = (const double2 *)b;
+  V.Hd = S->Hdia_inv;
+  CgLayout L;
+  L.row_sys = (int *)(base + NA_OFF_ROWSYS);
+  L.tile_beg = (int *)(base + NA_OFF_TBEG);
+  L.tile_sys = (int *)(base + NA_OFF_TSYS);
+  L.sys_tile = (int *)(base + NA_OFF_STILE);
+  L.nsys = 1;
+  L.ntiles = ntiles;
+  L.done_count = (int *)(base + NA_OFF_DONE);
+  CgScalars *cgs = (CgScalars *)(base + NA_OFF_CGS);
+  hipLaunchKernelGGL(k_na_layout, dim3(div_up(std::max(n, ntiles + 1), 256)), dim3(256), 0, S->stream, n, ntiles,
+                     (int *)L.row_sys, (int *)L.tile_beg, (int *)L.tile_sys, (int *)L.sys_tile, cgs);
+  CgRun<double, double> R;
+  R.V = V;
+  R.val = S->H_val;
+  R.L = L;
+  R.cgs = cgs;
+  R.iters_out = (int *)(base + NA_OFF_ITERS);
+  int its[2] = {0, 0};
+  R.iters_host = its;
+  R.tol = S->ctl.tol;
+  R.imax = S->imax;
+  R.kt_iter = KT_CG_ITER;
+  R.kt_update = KT_CG_UPDATE;
+  R.kt_setup = KT_CG_SETUP;
+  R.kt_scalars = KT_CG_SCALARS;
+  R.partials = (double *)(base + NA_OFF_PART);
+  R.pstride = NA_PSTRIDE;
+  int its0 = 0;
+  for (size_t k = 0; k < S->h_sys_iters.size(); k += 2) its0 = std::max(its0, S->h_sys_iters[k]);
+  const bool ktime = S->ktime;
+  S->ktime = false;
+  const int rc = cg_solve(S, R, its0 + 2, 0);
+  S->ktime = ktime;
+  if (rc) return rc;
+  if (iters_s) *iters_s = its[0];
+  if (iters_t) *iters_t = its[1];
+  return swr_status_to_rc(S->status_host->flags);
 }

@@ -413,6 +413,10 @@ int swr_qeq_mixed_solve(swr_system *S);
 // autotuner timing hooks (qeq.hip, qeq_mixed.hip, lists.hip)
 int swr_qeq_time_spmv(swr_system *S, int sh, int reps, float *ms);
 int swr_qeq_time_mixed(swr_system *S, int which, int sh, int reps, float *ms);
+// the no-allocation FP64 CG (qeq.hip): per-device scratch address, solve
+int swr_noalloc_init(int device);
+int swr_qeq_cg_noalloc_run(swr_system *S, const double *b, double *x, int *iters_s, int *iters_t);
+int swr_status_to_rc(unsigned int flags);
 int swr_lists_time_far(swr_system *S, int sh, int reps, float *ms);
 
 template <class T>
