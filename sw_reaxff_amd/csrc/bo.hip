@@ -29,7 +29,7 @@ k_bo(int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__restrict__
      double *__restrict__ Delta_val, double *__restrict__ vlpex, double *__restrict__ nlp,
      double *__restrict__ nlp_temp, double *__restrict__ Clp, double *__restrict__ Delta_lp,
      double *__restrict__ dDelta_lp, double *__restrict__ Delta_lp_temp, double *__restrict__ dDelta_lp_temp) {
-  const int gid = blockIdx.x * BO_T + threadIdx.x;
+  const int gid = xcd_block(gridDim.x) * BO_T + threadIdx.x;
   const int i = gid / BO_LPA, sl = gid % BO_LPA;
   const int lead = lane_id() - sl;        // the group's first lane
   if (i >= N) return;                     // whole groups leave together
@@ -47,14 +47,15 @@ k_bo(int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__restrict__
     const int ej = pj < be ? atoms[j].type : -1;
     if (ej >= 0) {
       // sym_index: slot of i in j's nbr-sorted list
-      int lo = bond_index[j], hi = bond_end[j] - 1, sym = bonds[pj].sym_index;
+      // (stored only when found, which it always is for pairs of atoms of
+      // the system; the record is not read)
+      int lo = bond_index[j], hi = bond_end[j] - 1;
       while (lo <= hi) {
         const int mid = (lo + hi) >> 1;
         const int v = bond_nbr[mid];
-        if (v == i) { sym = mid; break; }
+        if (v == i) { bonds[pj].sym_index = mid; break; }
         if (v < i) lo = mid + 1; else hi = mid - 1;
       }
-      bonds[pj].sym_index = sym;
       const DevTbp tw = P->tbp[ei * SWR_MAX_ELEM + ej];
       swr_bond_order_t &bo = bo_data[pj];
       double bo_ij = BO[pj];

@@ -364,41 +364,49 @@ k_far_ws(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *_
   }
   // Software pipeline without register copies (a copy of a loaded value
   // waits for the load): while chunk p is evaluated, chunk p+2's atoms and
-  // chunk p+3's ids are requested, into three neighbour-record sets and
-  // three id registers that rotate with period 3.  Every phase of an active
-  // gatherer issues the same loads (indices clamped to the row), and the
-  // gatherer, idle-gatherer and writer loops are separate, so the
-  // compiler's wait counts let two phases of loads stay in flight.
+  // chunk p+6's ids are requested, into three neighbour-record sets that
+  // rotate with period 3 and six id registers that rotate with period 6.
+  // An id load thus has four phases to arrive before its atoms are
+  // requested (Verlet ids stream from HBM, whose read latency under the
+  // far list's write stream is a few microseconds), an atom gather two.
+  // Every phase of an active gatherer issues the same loads (indices clamped
+  // to the row), and the gatherer, idle-gatherer and writer loops are
+  // separate, so the compiler's wait counts let the loads stay in flight.
   const long long clast = cend - 1;
   FarNbr X = {}, Y = {}, Z = {};
-  int R0 = 0, R1 = 0, R2 = 0;
+  int I0 = 0, I1 = 0, I2 = 0, I3 = 0, I4 = 0, I5 = 0;
   if (nch > 0) {
-    const int j0 = vlist[min(cbeg + lane, clast)];
-    const int j1 = vlist[min(cbeg + SWR_WAVE + lane, clast)];
-    R0 = vlist[min(cbeg + 2 * SWR_WAVE + lane, clast)];
-    X.a = atoms[j0];
-    X.j = j0;
-    Y.a = atoms[j1];
-    Y.j = j1;
+    I0 = vlist[min(cbeg + lane, clast)];
+    I1 = vlist[min(cbeg + SWR_WAVE + lane, clast)];
+    I2 = vlist[min(cbeg + 2 * SWR_WAVE + lane, clast)];
+    I3 = vlist[min(cbeg + 3 * SWR_WAVE + lane, clast)];
+    I4 = vlist[min(cbeg + 4 * SWR_WAVE + lane, clast)];
+    I5 = vlist[min(cbeg + 5 * SWR_WAVE + lane, clast)];
+    X.a = atoms[I0];
+    X.j = I0;
+    Y.a = atoms[I1];
+    Y.j = I1;
   }
   lds_barrier();
   int nph = 0;
   for (int k = 0; k < FG; k++) nph = max(nph, S.nphase[k]);
-  // phases 0..nph (the writers drain phase p-1 in phase p), rounded up to
-  // whole rotations; identical in every wave of the block
-  const int nphr = __builtin_amdgcn_readfirstlane((nph + 1 + 2) / 3 * 3);
+  // phases 0..nph (the writers drain phase p-1 in phase p); identical in
+  // every wave of the block
+  const int nphr = __builtin_amdgcn_readfirstlane(nph + 1);
 
   const long long far0 = cbeg;
   int nfar = 0, ncand = 0;
 
-  // gatherer phase: evaluate chunk p (cur), request chunk p+3's ids into
-  // jfree and chunk p+2's atoms (ids jaddr) into far2
+  // gatherer phase: evaluate chunk p (cur), request chunk p+2's atoms (ids
+  // jaddr) into far2 and then chunk p+6's ids into jfree (in this order:
+  // waiting for a chunk's atoms then never waits for the ids requested
+  // after them)
   auto gather = [&](int p, const FarNbr &cur, FarNbr &far2, const int &jaddr, int &jfree) {
     const int buf = p & 1;
     const long long k = cbeg + (long long)p * SWR_WAVE + lane;
-    jfree = vlist[min(k + 3 * SWR_WAVE, clast)];
     far2.a = atoms[jaddr];
     far2.j = jaddr;
+    jfree = vlist[min(k + 6 * SWR_WAVE, clast)];
     if (p < nch) {
       const bool valid = k < cend;
       double dx = 0, dy = 0, dz = 0, d = 0;
@@ -451,11 +459,25 @@ k_far_ws(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *_
   };
 
   if (gatherer && nch > 0) {
-    for (int p = 0; p < nphr; p += 3) {
-      gather(p, X, Z, R0, R1);
-      gather(p + 1, Y, X, R1, R2);
-      gather(p + 2, Z, Y, R2, R0);
+    // six phases per trip (the period of the id ring): a loop with one exit,
+    // then the last nphr % 6 phases in straight-line code continuing the
+    // rotation (early exits from the loop make the compiler's wait counts
+    // at its header conservative); nphr is wave-uniform, so every wave of
+    // the block passes nphr barriers
+    int p = 0;
+    for (; p + 6 <= nphr; p += 6) {
+      gather(p, X, Z, I2, I0);
+      gather(p + 1, Y, X, I3, I1);
+      gather(p + 2, Z, Y, I4, I2);
+      gather(p + 3, X, Z, I5, I3);
+      gather(p + 4, Y, X, I0, I4);
+      gather(p + 5, Z, Y, I1, I5);
     }
+    if (p < nphr) gather(p, X, Z, I2, I0);
+    if (p + 1 < nphr) gather(p + 1, Y, X, I3, I1);
+    if (p + 2 < nphr) gather(p + 2, Z, Y, I4, I2);
+    if (p + 3 < nphr) gather(p + 3, X, Z, I5, I3);
+    if (p + 4 < nphr) gather(p + 4, Y, X, I0, I4);
   } else if (gatherer) {
     for (int p = 0; p < nphr; p++) lds_barrier();
   } else {

@@ -38,7 +38,8 @@ k_vdw_coulomb(int n, const DevParams *__restrict__ P, const swr_atom_pack_t *__r
               double *__restrict__ partials) {
   __shared__ double sacc[NBW][8];
   const int w = wave_in_block(), lane = lane_id();
-  const int i = blockIdx.x * NBW + w;
+  const int blk = blockIdx.x;   // rows and partial slot
+  const int i = blk * NBW + w;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // eng_vdwl, eng_coul, virial[6]
   int ei = -1;
   swr_atom_pack_t ai = {};
@@ -143,7 +144,7 @@ k_vdw_coulomb(int n, const DevParams *__restrict__ P, const swr_atom_pack_t *__r
   if (threadIdx.x < 8) {
     double t = 0.0;
     for (int ww = 0; ww < NBW; ww++) t += sacc[ww][threadIdx.x];
-    partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = t;  // component-major
+    partials[(size_t)threadIdx.x * gridDim.x + blk] = t;  // component-major
   }
 }
 

@@ -38,38 +38,83 @@ __device__ __host__ __forceinline__ long long h_row_start(long long voff_i, long
   return (voff_i + 4 * i + 3) & ~3LL;
 }
 
+// A Verlet candidate of the H build: its id, position, type and owner.
+struct HCand {
+  int j, t, o;
+  double x, y, z;
+};
+
+__device__ __forceinline__ void h_fetch(HCand &c, int j, const double *__restrict__ x, const int *__restrict__ type,
+                                        const int *__restrict__ owner) {
+  c.j = j;
+  c.x = x[3 * j];
+  c.y = x[3 * j + 1];
+  c.z = x[3 * j + 2];
+  c.t = type[j];
+  c.o = owner[j];
+}
+
+// One wave per local row over its Verlet candidates, 64 at a time.  The
+// loop is software-pipelined as the far pass is: while chunk c is
+// evaluated, chunk c+2's positions, types and owners and chunk c+6's ids
+// are requested, into three candidate sets rotating with period 3 and six
+// id registers rotating with period 6, so an id load has four chunks and a
+// gather two to arrive (a row has ~10 chunks at 96k water atoms).
 __global__ void __launch_bounds__(QT)
 k_build_H(int n, const DevParams *__restrict__ P, const double *__restrict__ x, const int *__restrict__ type,
           const int *__restrict__ owner, const long long *__restrict__ voff, const int *__restrict__ vlist,
           int *__restrict__ firstnbr, int *__restrict__ numnbrs, int *__restrict__ jlist, double *__restrict__ val,
           float *__restrict__ val32, double *__restrict__ Hdia_inv, double2 *__restrict__ b2,
           double2 *__restrict__ st, const double *__restrict__ s_hist, const double *__restrict__ t_hist) {
+  // the shielding table in LDS: a global load in the loop would make each
+  // chunk wait for every prefetched load before it (the wait counter
+  // retires loads in issue order)
+  constexpr int nt1 = SWR_MAX_TYPES + 1;
+  __shared__ double shl[nt1 * nt1];
+  for (int k = threadIdx.x; k < nt1 * nt1; k += QT) shl[k] = P->shld[k];
+  __syncthreads();
   const int i = xcd_block(gridDim.x) * QW + wave_in_block();
   if (i >= n) return;
   const int lane = lane_id();
   const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
   const int ti = type[i];
-  const int nt1 = SWR_MAX_TYPES + 1;
   const double swbsq = P->swbsq;
   double T[8];
 #pragma unroll
   for (int k = 0; k < 8; k++) T[k] = P->Tap[k];
   const long long beg = voff[i], end = voff[i + 1];
   const long long dst = h_row_start(beg, i);
+  const int nch = (int)((end - beg + SWR_WAVE - 1) / SWR_WAVE);
+  const long long clast = end - 1;
   int cnt = 0;
-  for (long long base = beg; base < end; base += SWR_WAVE) {
-    const long long k = base + lane;
+  HCand A = {}, B = {}, C = {};
+  int I0 = 0, I1 = 0, I2 = 0, I3 = 0, I4 = 0, I5 = 0;
+  if (nch > 0) {
+    I0 = vlist[min(beg + lane, clast)];
+    I1 = vlist[min(beg + SWR_WAVE + lane, clast)];
+    I2 = vlist[min(beg + 2 * SWR_WAVE + lane, clast)];
+    I3 = vlist[min(beg + 3 * SWR_WAVE + lane, clast)];
+    I4 = vlist[min(beg + 4 * SWR_WAVE + lane, clast)];
+    I5 = vlist[min(beg + 5 * SWR_WAVE + lane, clast)];
+    h_fetch(A, I0, x, type, owner);
+    h_fetch(B, I1, x, type, owner);
+  }
+  // chunk c: evaluate cur, request chunk c+2's candidates (ids jaddr) into
+  // far2 and then chunk c+6's ids into jfree (in this order: waiting for a
+  // chunk's candidates then never waits for the ids requested after them)
+  auto chunk = [&](int c, const HCand &cur, HCand &far2, const int &jaddr, int &jfree) {
+    const long long k = beg + (long long)c * SWR_WAVE + lane;
+    h_fetch(far2, jaddr, x, type, owner);
+    jfree = vlist[min(k + 6 * SWR_WAVE, clast)];
     bool hit = false;
-    int jo = 0;
     double hv = 0.0;
     if (k < end) {
-      const int j = vlist[k];
-      const double dx = x[3 * j] - xi, dy = x[3 * j + 1] - yi, dz = x[3 * j + 2] - zi;
+      const double dx = cur.x - xi, dy = cur.y - yi, dz = cur.z - zi;
       const double r_sqr = dx * dx + dy * dy + dz * dz;
       if (r_sqr <= swbsq) {
         hit = true;
         const double r = sqrt(r_sqr);
-        const double gamma = P->shld[ti * nt1 + type[j]];
+        const double gamma = shl[ti * nt1 + cur.t];
         double Taper = T[7] * r + T[6];
         Taper = Taper * r + T[5];
         Taper = Taper * r + T[4];
@@ -85,18 +130,35 @@ k_build_H(int n, const DevParams *__restrict__ P, const double *__restrict__ x, 
         const double denom = r * r * r + gamma;
         const double root = cbrt(denom) * (1.0 + (0.3333333333333 - 1.0 / 3.0) * (double)__logf((float)denom));
         hv = Taper * EV_TO_KCAL_PER_MOL / root;
-        jo = owner[j];
       }
     }
     const unsigned long long m = __ballot(hit);
     if (hit) {
       const long long slot = dst + cnt + ballot_rank(m);
-      jlist[slot] = jo;
+      jlist[slot] = cur.o;
       if (val) val[slot] = hv;
       if (val32) val32[slot] = (float)hv;
     }
     cnt += __popcll(m);
+  };
+  // six chunks per trip (the period of the id ring); a loop with one exit,
+  // then the last nch % 6 chunks in straight-line code continuing the
+  // rotation (early exits from the loop make the compiler's wait counts at
+  // its header conservative)
+  int c = 0;
+  for (; c + 6 <= nch; c += 6) {
+    chunk(c, A, C, I2, I0);
+    chunk(c + 1, B, A, I3, I1);
+    chunk(c + 2, C, B, I4, I2);
+    chunk(c + 3, A, C, I5, I3);
+    chunk(c + 4, B, A, I0, I4);
+    chunk(c + 5, C, B, I1, I5);
   }
+  if (c < nch) chunk(c, A, C, I2, I0);
+  if (c + 1 < nch) chunk(c + 1, B, A, I3, I1);
+  if (c + 2 < nch) chunk(c + 2, C, B, I4, I2);
+  if (c + 3 < nch) chunk(c + 3, A, C, I5, I3);
+  if (c + 4 < nch) chunk(c + 4, B, A, I0, I4);
   // padding to a multiple of 4 entries: column i with value 0
   if (lane < ((cnt + 3) & ~3) - cnt) {
     const long long slot = dst + cnt + lane;

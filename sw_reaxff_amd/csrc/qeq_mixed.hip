@@ -84,12 +84,12 @@ inline size_t vec_off(int nsys) { return (pcg_off(nsys) + sizeof(PcgState) * 2 *
 // Rows of a half-wave-per-row block: a batch uses the per-system block
 // layout (sblk_beg), one system the plain 8-row blocks.
 template <bool BATCH>
-__device__ __forceinline__ void sblk_rows(int n, const int *__restrict__ sblk_beg, int &r0, int &r1) {
+__device__ __forceinline__ void sblk_rows(int n, const int *__restrict__ sblk_beg, int blk, int &r0, int &r1) {
   if (BATCH) {
-    r0 = sblk_beg[blockIdx.x];
-    r1 = sblk_beg[blockIdx.x + 1];
+    r0 = sblk_beg[blk];
+    r1 = sblk_beg[blk + 1];
   } else {
-    r0 = blockIdx.x * RPB;
+    r0 = blk * RPB;
     r1 = min(n, r0 + RPB);
   }
 }
@@ -108,8 +108,9 @@ k_resid(int n, const DevParams *__restrict__ P, const int *__restrict__ type, co
         const double2 *__restrict__ x, const double2 *__restrict__ b, const double *__restrict__ Hd,
         float2 *__restrict__ r32, double *__restrict__ partials, int pstride, const int *__restrict__ sblk_beg,
         const int *__restrict__ row_sys, const MixState *__restrict__ ms, int first) {
+  const int blk = blockIdx.x;   // rows and partial slot
   int r0, r1;
-  sblk_rows<BATCH>(n, sblk_beg, r0, r1);
+  sblk_rows<BATCH>(n, sblk_beg, blk, r0, r1);
   if (BATCH && !first && r0 < r1 && ms[row_sys[r0]].done) return;
   constexpr int G = SpmvShape<SH>::G, U = SpmvShape<SH>::U;
   const int w = wave_in_block(), lane = lane_id();
@@ -129,7 +130,7 @@ k_resid(int n, const DevParams *__restrict__ P, const int *__restrict__ type, co
     d[4] = bi.x * bi.x; d[5] = bi.y * bi.y;
     r32[i] = make_float2((float)rs, (float)rt);
   }
-  block_partials<6, 8 * G>(d, partials, pstride, blockIdx.x);
+  block_partials<6, 8 * G>(d, partials, pstride, blk);
 }
 
 // Outer test and the inner tolerances, one workgroup per system (over its
@@ -268,8 +269,9 @@ k_pcg_spmv(int n, int nsys, const DevParams *__restrict__ P, const int *__restri
            const float *__restrict__ val, const float2 *__restrict__ p, float2 *__restrict__ q,
            const MixCtl *__restrict__ ctl, const PcgState *__restrict__ pst, double *__restrict__ partials,
            int pstride, const int *__restrict__ sblk_beg, const int *__restrict__ row_sys, int gate = 1) {
+  const int blk = blockIdx.x;   // rows and partial slot
   int r0, r1;
-  sblk_rows<BATCH>(n, sblk_beg, r0, r1);
+  sblk_rows<BATCH>(n, sblk_beg, blk, r0, r1);
   constexpr int G = SpmvShape<SH>::G, U = SpmvShape<SH>::U;
   const int w = wave_in_block(), lane = lane_id();
   const int i = r0 + w * (SWR_WAVE / G) + lane / G;
@@ -289,7 +291,7 @@ k_pcg_spmv(int n, int nsys, const DevParams *__restrict__ P, const int *__restri
     dd[0] = (double)pi.x * qq.x;
     dd[1] = (double)pi.y * qq.y;
   }
-  block_partials<2, 8 * G>(dd, partials, pstride, blockIdx.x);
+  block_partials<2, 8 * G>(dd, partials, pstride, blk);
 }
 
 template <int NT>
