@@ -63,7 +63,7 @@ def counts(sysh):
                 nnz=int(sysh.get("H_numnbrs").sum()))
 
 
-SLOTS = ("verlet", "bins", "far", "bonds", "reset", "validate", "mark", "bo", "build_H", "cg_setup", "cg_iter",
+SLOTS = ("verlet", "bins", "far", "bonds", "reset", "validate", "mark", "bo", "build_H", "panel", "cg_setup", "cg_iter",
          "cg_update", "cg_scalars", "qeq_q", "mixed_inner", "mixed_outer")
 
 
@@ -91,6 +91,11 @@ def algorithmic_bytes(slot, c, mixed=False):
         return 64 * c["bonds"] + 304 * c["bonds"] + N * (40 + 8 + 16 + 24 + 4 + 16 + 8)
     if slot == "build_H":
         return 4 * (c["verlet"] * n // max(N, 1)) + 24 * N + 4 * N + 12 * c["nnz"] + n * (8 + 16 + 16 + 80 + 8)
+    if slot == "panel":
+        # the SpMV panels of H: each entry's column read and its 16-bit list
+        # position written, per row its extent and panel-order id, per panel
+        # its list (at most one column per entry; counted once per row)
+        return 4 * c["nnz"] + 2 * c["nnz"] + n * (4 + 4 + 4) + 4 * n
     if slot == "bo":
         return 8 * N * 20 + c["bonds"] * (56 + 200 + 8 + 16)
     if slot == "verlet":

@@ -25,13 +25,16 @@ def rel_err(a, b, floor=1e-300):
     return np.max(np.abs(a - b) / den) if a.size else 0.0
 
 
-def rel_err_scaled(a, b, frac=1e-5):
-    """rel_err with a floor at frac times the array's largest magnitude:
-    entries that are differences of much larger terms (Delta' = sum BO -
-    valency near zero, a coefficient that cancels) are compared absolutely
-    at rtol * frac of that magnitude, a few units in its last place."""
+def rel_err_scaled(a, b, frac=1e-5, unit=0.0):
+    """rel_err with a floor at frac times the larger of the array's largest
+    magnitude and unit: entries that are differences of much larger terms
+    (Delta' = sum BO - valency near zero, a coefficient that cancels) are
+    compared absolutely at rtol * frac of that magnitude, a few units in its
+    last place.  unit = 1 for per-atom sums of bond orders and the valences
+    and lone-pair counts they are compared with, which are of order one
+    whatever the size of their difference."""
     b = np.asarray(b, dtype=np.float64)
-    scale = float(np.abs(b).max()) if b.size else 0.0
+    scale = max(float(np.abs(b).max()) if b.size else 0.0, unit)
     return rel_err(a, b, max(frac * scale, 1e-300))
 
 
@@ -125,15 +128,16 @@ def compare_state(h, o, q_atol, bo_rtol=BO_RTOL, far=True):
     assert np.array_equal(gB["dbond_index"] - gb[own], oB["dbond_index"] - ob[own])
     assert np.array_equal(gB["sym_index"] - gb[gB["nbr"]], oB["sym_index"] - ob[oB["nbr"]])
     # bond orders strictly relative (floor 1e-12: BO'-bo_cut of a weak bond);
-    # their coefficients and the per-atom sums relative down to 1e-5 of
-    # each array's largest magnitude, absolute below it
+    # their coefficients relative down to 1e-5 of each array's largest
+    # magnitude and the per-atom sums down to 1e-5 of the larger of that and
+    # one, absolute below it
     worst = {"BO": rel_err(h.get("BO")[gs], o.get("BO")[os_], 1e-12),
              "BOpi": rel_err(h.get("BOpi").reshape(-1, 2)[gs], o.get("BOpi").reshape(-1, 2)[os_], 1e-12)}
     gbo, obo = h.get("bo_data")[gs], o.get("bo_data")[os_]
     for f in BO_FIELDS:
         worst[f] = rel_err_scaled(gbo[f], obo[f])
     for nm in WS_ARRAYS:
-        worst[nm] = rel_err_scaled(h.get(nm), o.get(nm))
+        worst[nm] = rel_err_scaled(h.get(nm), o.get(nm), unit=1.0)
     st["bo_rel"] = float(max(worst.values()))
     if st["bo_rel"] > bo_rtol:
         arrays = {"BO": (h.get("BO")[gs], o.get("BO")[os_])}

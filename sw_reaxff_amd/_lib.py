@@ -151,7 +151,10 @@ DTYPES = {
     "hbond_index": np.int32, "hbond_end": np.int32, "hbonds": HBOND_DT, "hbond_far": np.int64,
     "H_val32": np.float32, "sys_iters": np.int32,
     "bond_mark": np.int32, "H_firstnbr": np.int32, "H_numnbrs": np.int32, "H_jlist": np.int32,
+    "panel_rows": np.int32, "panel_begin": np.int32, "panel_count": np.int32, "panel_columns": np.int32,
+    "H_jloc": np.uint16,
 }
+PANEL_MAXC = 3072   # columns a panel's list holds (SWR_PANEL_MAXC)
 
 
 class System:

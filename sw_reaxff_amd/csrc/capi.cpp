@@ -380,7 +380,7 @@ int swr_destroy(swr_system *S) {
   void *ptrs[] = {S->dp, S->x, S->v, S->q, S->type, S->tag, S->owner, S->img, S->mask, S->packed,
                   S->bin_of, S->bin_count, S->bin_start, S->rs_keys, S->rs_vals, S->rs_hist, S->bp_pos, S->bp_id, S->stencil_rows,
                   S->voff, S->vcount, S->vlist, S->far, S->far_index, S->far_end, S->num_bonds, S->num_hbonds,
-                  S->Hindex, S->bond_index, S->bond_end, S->bonds, S->bond_nbr, S->bo_data, S->BO, S->BOpi, S->Cdbo,
+                  S->Hindex, S->bond_index, S->bond_end, S->bonds, S->bond_nbr, S->bo_data, S->prow, S->pbeg, S->pcnt, S->pcol, S->pkey, S->jloc, S->sys_panel, S->BO, S->BOpi, S->Cdbo,
                   S->Cdbopi,
                   S->Cdbopi2, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,
                   S->fcd, S->nb_partials, S->nb_result, S->nb_eatom, S->be_part, S->be_ce, S->be_bufc, S->be_result, S->hb_part,
@@ -855,6 +855,11 @@ bool lookup(swr_system *S, const std::string &k, View &v) {
   if (k == "H_val") return set(S->H_val, 8 * S->Hcap);
   if (k == "H_val32") return set(S->H_val32, S->H_val32 ? 4 * S->Hcap : 0);
   if (k == "Hdia_inv") return set(S->Hdia_inv, 8 * n);
+  if (k == "panel_rows") return set(S->prow, S->prow ? 4 * (int64_t)S->prow_n : 0);
+  if (k == "panel_begin") return set(S->pbeg, S->pbeg ? 4 * ((int64_t)S->npanel + 1) : 0);
+  if (k == "panel_count") return set(S->pcnt, S->pcnt ? 4 * (int64_t)S->npanel : 0);
+  if (k == "panel_columns") return set(S->pcol, S->pcol ? 4 * (int64_t)S->npanel * SWR_PANEL_MAXC : 0);
+  if (k == "H_jloc") return set(S->jloc, S->jloc ? 2 * S->Hcap : 0);
   if (k == "b_s") return set(S->b_s, 8 * n, 1);
   if (k == "b_t") return set(S->b_s, 8 * n, 2);
   if (k == "s") return set(S->st, 8 * n, 1);

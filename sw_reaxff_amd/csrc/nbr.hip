@@ -454,6 +454,36 @@ int swr_nbr_pbc(swr_system *S, const double lo[3], const double hi[3]) {
   return SWR_OK;
 }
 
+// Stable sort of n (key, value) pairs by the low nbits of the key with the
+// LDS radix passes above: vals_in == nullptr sorts the indices 0..n-1;
+// the sorted values go to vals_out (which may alias neither input).  Uses
+// the binning's scratch (rs_keys, rs_vals, rs_hist), sized for N >= n.
+int swr_radix_sort_i32(swr_system *S, const int *keys, const int *vals_in, int n, int nbits, int *vals_out) {
+  if (n <= 0) return SWR_OK;
+  const int rs_nblk = div_up(n, RS_CHUNK);
+  int rc;
+  if ((rc = swr_reserve(&S->rs_keys, S->cap_rs_keys, 2 * (size_t)n)) ||
+      (rc = swr_reserve(&S->rs_vals, S->cap_rs_vals, (size_t)n)) ||
+      (rc = swr_reserve(&S->rs_hist, S->cap_rs_hist, 2 * (size_t)RS_RADIX * rs_nblk + 1)))
+    return rc;
+  const int passes = std::max(1, (nbits + 7) / 8);
+  int *hist = S->rs_hist, *off = S->rs_hist + (size_t)RS_RADIX * rs_nblk;
+  const int *kin = keys, *vin = vals_in;
+  int *kbuf[2] = {S->rs_keys, S->rs_keys + n};
+  for (int p = 0; p < passes; p++) {
+    int *vout = (p == passes - 1) ? vals_out : ((passes - 1 - p) % 2 ? S->rs_vals : vals_out);
+    int *kout = kbuf[p & 1];
+    hipLaunchKernelGGL(k_rs_hist, dim3(rs_nblk), dim3(RS_T), 0, S->stream, kin, n, 8 * p, rs_nblk, hist);
+    if ((rc = swr_scan_i32(S, hist, off, RS_RADIX * rs_nblk, nullptr))) return rc;
+    hipLaunchKernelGGL(k_rs_scatter, dim3(rs_nblk), dim3(RS_T), 0, S->stream, kin, vin, n, 8 * p, rs_nblk,
+                       (const int *)off, kout, vout);
+    kin = kout;
+    vin = vout;
+  }
+  HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
 // Bin all N atoms and build the full Verlet list (CSR: voff[N+1], vlist).
 // Full neighbour list of all N atoms (nrow = N: the ghost list) or of the
 // local atoms only (nrow = n: the atom-only list, ghost rows empty).
@@ -528,5 +558,6 @@ int swr_nbr_build(swr_system *S, int local_only) {
                      S->vcount, (const long long *)S->voff, S->vlist, nrow);
   swr_kt_end(S, KT_VERLET);
   HIP_CHECK(hipGetLastError());
+  S->nbr_gen++;
   return SWR_OK;
 }

@@ -272,7 +272,7 @@ int swr_qeq_build_H(swr_system *S) {
                      S->s_hist, S->t_hist);
   swr_kt_end(S, KT_BUILD_H);
   HIP_CHECK(hipGetLastError());
-  return SWR_OK;
+  return swr_panel_build(S);
 }
 
 int swr_qeq_spmv(swr_system *S, const double *xv, double *b) {
@@ -355,6 +355,7 @@ int swr_qeq_layout(swr_system *S) {
   S->nsblk = nsblk;
   S->h_sys_iters.assign(2 * nsys, 0);
   S->layout_dirty = false;
+  S->layout_gen++;
   return SWR_OK;
 }
 
@@ -378,7 +379,7 @@ int swr_qeq_time_spmv(swr_system *S, int sh, int reps, float *ms) {
   if (rc) return rc;
   if (!S->H_val || S->n <= 0) return SWR_ERR_STATE;
   CgVecsT<double> V = make_vecs(S);
-  const int n = S->n, nsp = div_up(n, RPB);
+  const int n = S->n;
   const CgScalars *cgs = S->cgs;
   const int *row_sys = S->row_sys;
   hipEvent_t e[2];
@@ -386,15 +387,15 @@ int swr_qeq_time_spmv(swr_system *S, int sh, int reps, float *ms) {
   HIP_CHECK(hipEventCreate(&e[1]));
   spmv_dispatch(sh, [&](auto shc) {
     constexpr int SH = decltype(shc)::value;
-    const dim3 gb(nsp), tb(8 * SpmvShape<SH>::G);
+    const dim3 gb(S->npanel), tb(SpmvShape<SH>::NT);
     for (int r = -1; r < reps; r++) {   // one untimed launch first
       if (r == 0) (void)hipEventRecord(e[0], S->stream);
       if (S->nsys > 1)
         hipLaunchKernelGGL((k_cg_spmv<PH_S2, true, double, double, SH>), gb, tb, 0, S->stream, n, S->dp, S->type,
-                           S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, V, 0, cgs, row_sys);
+                           S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, V, 0, cgs, row_sys, panel_args(S));
       else
         hipLaunchKernelGGL((k_cg_spmv<PH_S2, false, double, double, SH>), gb, tb, 0, S->stream, n, S->dp, S->type,
-                           S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, V, 0, cgs, row_sys);
+                           S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, V, 0, cgs, row_sys, panel_args(S));
     }
     (void)hipEventRecord(e[1], S->stream);
   });

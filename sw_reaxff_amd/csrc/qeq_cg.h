@@ -150,24 +150,24 @@ template <> struct Quad<float, float> {
 
 // G lanes per row over a row that starts 16-byte aligned and is padded to a
 // multiple of 4 entries (value 0): each lane takes groups of 4 consecutive
-// entries (one 16-byte index load, 16-byte value loads), U groups in flight,
-// group g of the row on lane g % G.  Fixed summation order (per lane two
-// accumulators per right-hand side, then a G-lane butterfly), so a row's sum
-// depends only on the row and (G, U).  Returns the sums on every lane of the
-// row's group.
-template <typename T, typename VT, int G, int U>
-__device__ __forceinline__ typename Vec2<T>::type row_spmv4(int beg, int cnt, const int *__restrict__ jlist,
-                                                            const VT *__restrict__ val,
-                                                            const typename Vec2<T>::type *__restrict__ xv) {
+// entries (one index load -- four 32-bit columns of x, or four 16-bit
+// positions in a panel's LDS list --, 16-byte value loads), U groups in
+// flight, group g of the row on lane g % G.  J and V point at the row's
+// first entry, xv at x or at the panel's list.  Fixed summation order (per
+// lane two accumulators per right-hand side, then a G-lane butterfly), so
+// a row's sum depends only on the row and (G, U).  Returns the sums on every
+// lane of the row's group.
+template <typename T, typename VT, int G, int U, typename I4>
+__device__ __forceinline__ typename Vec2<T>::type row_sum4(int cnt, const I4 *__restrict__ J,
+                                                           const VT *__restrict__ V,
+                                                           const typename Vec2<T>::type *__restrict__ xv) {
   typedef typename Vec2<T>::type T2;
   const int sl = lane_id() % G;
   const int n4 = (cnt + 3) >> 2;
-  const int4 *J = reinterpret_cast<const int4 *>(jlist + beg);
-  const VT *V = val + beg;
   T a = 0, b = 0, c = 0, d = 0;
   int g = sl;
   for (; g + (U - 1) * G < n4; g += U * G) {
-    int4 jj[U];
+    I4 jj[U];
     Quad<T, VT> vv[U];
 #pragma unroll
     for (int u = 0; u < U; u++) jj[u] = J[g + u * G];
@@ -183,7 +183,7 @@ __device__ __forceinline__ typename Vec2<T>::type row_spmv4(int beg, int cnt, co
     }
   }
   for (; g < n4; g += G) {
-    const int4 jj = J[g];
+    const I4 jj = J[g];
     Quad<T, VT> vv;
     vv.load(V + 4 * g);
     const T2 x0 = xv[jj.x], x1 = xv[jj.y], x2 = xv[jj.z], x3 = xv[jj.w];
@@ -204,26 +204,66 @@ __device__ __forceinline__ typename Vec2<T>::type row_spmv4(int beg, int cnt, co
   return r;
 }
 
-// SpMV shapes the autotuner chooses from: G lanes per row, U groups of 4
-// entries in flight per lane; every shape runs 8 rows per workgroup (8 G
-// threads), so the row-block layout of a batch is the same for all.
-enum { SPMV_G8U4 = 0, SPMV_G16U2, SPMV_G16U4, SPMV_G32U2, SPMV_NSHAPES };
-constexpr int SPMV_DEFAULT = SPMV_G16U4;
-constexpr const char *kSpmvShapeNames[SPMV_NSHAPES] = {"G8U4", "G16U2", "G16U4", "G32U2"};
+// The SpMV panels of swr_system (panel.hip): rows in panel order, panel
+// p's rows [pbeg[p], pbeg[p+1]) of prow, its column list pcol[p][0, pcnt[p])
+// (pcnt < 0: none) and each entry's position in it (jloc).
+struct PanelArgs {
+  const int *prow, *pbeg, *pcnt, *pcol;
+  const unsigned short *jloc;
+};
+inline PanelArgs panel_args(const swr_system *S) { return PanelArgs{S->prow, S->pbeg, S->pcnt, S->pcol, S->jloc}; }
+
+// One workgroup of NT threads per panel: the panel's entries of x (src) are
+// copied into the LDS list xl once (when the panel has a list), then G lanes
+// per row sum each row of the panel, NT/G rows at a time, and call
+// row(i, live, q) on every lane of the row's group (live: a row of the
+// panel; q: the row's sums).  Panels without a list gather from src.
+template <typename T, typename VT, int NT, int G, int U, class Row>
+__device__ __forceinline__ void panel_rows(int p, const PanelArgs &A, const int *__restrict__ firstnbr,
+                                           const int *__restrict__ numnbrs, const int *__restrict__ jlist,
+                                           const VT *__restrict__ val, const typename Vec2<T>::type *__restrict__ src,
+                                           typename Vec2<T>::type *xl, Row row) {
+  typedef typename Vec2<T>::type T2;
+  const int r0 = A.pbeg[p], r1 = A.pbeg[p + 1];
+  const int m = A.pcnt[p];
+  if (m >= 0) {
+    const int *pc = A.pcol + (size_t)p * SWR_PANEL_MAXC;
+    for (int k = threadIdx.x; k < m; k += NT) xl[k] = src[pc[k]];
+    __syncthreads();
+  }
+  const int grp = threadIdx.x / G;
+  for (int rb = r0; rb < r1; rb += NT / G) {
+    const int rr = rb + grp;
+    const bool live = rr < r1;
+    const int i = live ? A.prow[rr] : 0;
+    const int beg = live ? firstnbr[i] : 0, cnt = live ? numnbrs[i] : 0;
+    T2 q;
+    if (m >= 0)
+      q = row_sum4<T, VT, G, U>(cnt, reinterpret_cast<const ushort4 *>(A.jloc + beg), val + beg, xl);
+    else   // (rare: one group in flight keeps the kernel's registers those of the list path)
+      q = row_sum4<T, VT, G, 1>(cnt, reinterpret_cast<const int4 *>(jlist + beg), val + beg, src);
+    row(i, live, q);
+  }
+}
+
+// SpMV shapes the autotuner chooses from: NT threads per panel, G lanes
+// per row, U groups of 4 entries in flight per lane.
+enum { SPMV_T512G8U4 = 0, SPMV_T512G8U2, SPMV_T256G8U4, SPMV_T256G4U4, SPMV_NSHAPES };
+constexpr int SPMV_DEFAULT = SPMV_T512G8U4;
 template <int SH> struct SpmvShape;
-template <> struct SpmvShape<SPMV_G8U4> { static constexpr int G = 8, U = 4; };
-template <> struct SpmvShape<SPMV_G16U2> { static constexpr int G = 16, U = 2; };
-template <> struct SpmvShape<SPMV_G16U4> { static constexpr int G = 16, U = 4; };
-template <> struct SpmvShape<SPMV_G32U2> { static constexpr int G = 32, U = 2; };
+template <> struct SpmvShape<SPMV_T512G8U4> { static constexpr int NT = 512, G = 8, U = 4; };
+template <> struct SpmvShape<SPMV_T512G8U2> { static constexpr int NT = 512, G = 8, U = 2; };
+template <> struct SpmvShape<SPMV_T256G8U4> { static constexpr int NT = 256, G = 8, U = 4; };
+template <> struct SpmvShape<SPMV_T256G4U4> { static constexpr int NT = 256, G = 4, U = 4; };
 
 // Calls f(std::integral_constant<int, SH>) for the runtime shape sh.
 template <class F>
 __host__ inline void spmv_dispatch(int sh, F &&f) {
   switch (sh) {
-    case SPMV_G8U4: f(std::integral_constant<int, SPMV_G8U4>()); break;
-    case SPMV_G16U2: f(std::integral_constant<int, SPMV_G16U2>()); break;
-    case SPMV_G32U2: f(std::integral_constant<int, SPMV_G32U2>()); break;
-    default: f(std::integral_constant<int, SPMV_G16U4>()); break;
+    case SPMV_T512G8U2: f(std::integral_constant<int, SPMV_T512G8U2>()); break;
+    case SPMV_T256G8U4: f(std::integral_constant<int, SPMV_T256G8U4>()); break;
+    case SPMV_T256G4U4: f(std::integral_constant<int, SPMV_T256G4U4>()); break;
+    default: f(std::integral_constant<int, SPMV_T512G8U4>()); break;
   }
 }
 
@@ -246,35 +286,33 @@ __device__ __forceinline__ bool cg_any_active(int PH, const CgScalars *__restric
 
 // q = eta d + H d (sparse_matvec_C_para, fix_qeq_reax_sw64.c:424-466) of the
 // vector phase PH multiplies: x (PH_S1), d0 (PH_S2), d1 (PH_S3) or the
-// current d (PH_K).  G lanes per row (shape SH), 8 rows per workgroup.
+// current d (PH_K).  One workgroup per SpMV panel (shape SH).
 template <int PH, bool BATCH, typename T, typename VT, int SH>
-__global__ void __launch_bounds__(8 * SpmvShape<SH>::G)
+__global__ void __launch_bounds__(SpmvShape<SH>::NT)
 k_cg_spmv(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
           const int *__restrict__ numnbrs, const int *__restrict__ jlist, const VT *__restrict__ val, CgVecsT<T> V,
-          int par_in, const CgScalars *__restrict__ cgs, const int *__restrict__ row_sys) {
+          int par_in, const CgScalars *__restrict__ cgs, const int *__restrict__ row_sys, PanelArgs A) {
   typedef typename Vec2<T>::type T2;
-  constexpr int G = SpmvShape<SH>::G, U = SpmvShape<SH>::U;
-  const int w = wave_in_block(), lane = lane_id();
-  const int i = blockIdx.x * RPB + w * (SWR_WAVE / G) + lane / G;
-  // The row's extent is requested together with the solve state, so the
-  // two latencies overlap (the state decides whether the row runs).
-  const int beg0 = i < n ? firstnbr[i] : 0, cnt0 = i < n ? numnbrs[i] : 0;
-  const int rs = BATCH && i < n ? row_sys[i] : 0;
-  // one system: the whole launch is skipped once both solves have finished;
-  // a batch: rows of systems whose solves have finished are skipped (a
-  // half-wave decides for its own row)
-  if (!BATCH && !cg_any_active(PH, cgs) && cnt0 >= 0) return;
-  const bool live = i < n && (!BATCH || cg_any_active(PH, cgs + 2 * rs));
-  const T2 *src = PH == PH_S1 ? V.x : PH == PH_S2 ? V.d[0] : PH == PH_S3 ? V.d[1] : (par_in ? V.d[1] : V.d[0]);
-  const int beg = live ? beg0 : 0, cnt = live ? cnt0 : 0;
-  T2 q = row_spmv4<T, VT, G, U>(beg, cnt, jlist, val, src);
-  if (lane % G == 0 && live) {
-    const T eta = (T)P->eta[type[i]];
-    const T2 xi = src[i];
-    q.x = eta * xi.x + q.x;
-    q.y = eta * xi.y + q.y;
-    V.q[i] = q;
+  constexpr int NT = SpmvShape<SH>::NT, G = SpmvShape<SH>::G, U = SpmvShape<SH>::U;
+  __shared__ T2 xl[SWR_PANEL_MAXC];
+  const int p = blockIdx.x;
+  // skipped once both solves of the panel's system have finished (a panel
+  // holds rows of one system)
+  if (!BATCH && !cg_any_active(PH, cgs)) return;
+  if (BATCH) {
+    const int r0 = A.pbeg[p];
+    if (r0 < A.pbeg[p + 1] && !cg_any_active(PH, cgs + 2 * row_sys[A.prow[r0]])) return;
   }
+  const T2 *src = PH == PH_S1 ? V.x : PH == PH_S2 ? V.d[0] : PH == PH_S3 ? V.d[1] : (par_in ? V.d[1] : V.d[0]);
+  panel_rows<T, VT, NT, G, U>(p, A, firstnbr, numnbrs, jlist, val, src, xl, [&](int i, bool live, T2 q) {
+    if (lane_id() % G == 0 && live) {
+      const T eta = (T)P->eta[type[i]];
+      const T2 xi = src[i];
+      q.x = eta * xi.x + q.x;
+      q.y = eta * xi.y + q.y;
+      V.q[i] = q;
+    }
+  });
 }
 
 // The vector part of phase PH, one thread per row.  PH_S1..S3 are the setup
@@ -443,18 +481,18 @@ struct CgRun {
 template <typename T, typename VT>
 int launch_cg(swr_system *S, const CgRun<T, VT> &R, int ph, int par_in, int xzero) {
   const int n = S->n;
-  const int nsp = div_up(n, RPB);
   const int ps = R.pstride;
   const int kt = ph == PH_K ? R.kt_iter : R.kt_setup;
   const int ktu = ph == PH_K ? R.kt_update : R.kt_setup;
   const int kts = ph == PH_K ? R.kt_scalars : R.kt_setup;
-#define SP_ARGS n, S->dp, S->type, S->H_firstnbr, S->H_numnbrs, S->H_jlist, R.val, R.V, par_in, R.cgs, R.L.row_sys
+#define SP_ARGS n, S->dp, S->type, S->H_firstnbr, S->H_numnbrs, S->H_jlist, R.val, R.V, par_in, R.cgs, R.L.row_sys, \
+                panel_args(S)
 #define UP_ARGS R.V, par_in, R.cgs, R.L, R.partials, ps, xzero
 #define SC_ARGS R.partials, ps, R.cgs, R.L, R.imax, R.tol, S->status, R.iters_out
   swr_kt_begin(S, kt);
   spmv_dispatch(S->spmv_shape[0], [&](auto shc) {
     constexpr int SH = decltype(shc)::value;
-    const dim3 gb(nsp), tb(8 * SpmvShape<SH>::G);
+    const dim3 gb(S->npanel), tb(SpmvShape<SH>::NT);
     if (R.L.nsys > 1) {
       switch (ph) {
         case PH_S1:

@@ -81,56 +81,41 @@ constexpr size_t MS_OFF = 64;
 inline size_t pcg_off(int nsys) { return (MS_OFF + sizeof(MixState) * (size_t)nsys + 255) & ~(size_t)255; }
 inline size_t vec_off(int nsys) { return (pcg_off(nsys) + sizeof(PcgState) * 2 * (size_t)nsys + 255) & ~(size_t)255; }
 
-// Rows of a half-wave-per-row block: a batch uses the per-system block
-// layout (sblk_beg), one system the plain 8-row blocks.
-template <bool BATCH>
-__device__ __forceinline__ void sblk_rows(int n, const int *__restrict__ sblk_beg, int blk, int &r0, int &r1) {
-  if (BATCH) {
-    r0 = sblk_beg[blk];
-    r1 = sblk_beg[blk + 1];
-  } else {
-    r0 = blk * RPB;
-    r1 = min(n, r0 + RPB);
-  }
-}
-
 __device__ __forceinline__ bool pcg_live(const PcgState *__restrict__ c) { return (c[0].active | c[1].active) != 0; }
 
 // r = b - (eta x + H x) in FP64 (FP64 values); u = M^-1 r; the dots
 // u.r (s, t) and r.r (s, t), b.b (s, t); r is also written as the FP32
-// right-hand side of the inner solve.  Half-wave per row as in k_cg.  Rows
-// of systems whose refinement has finished are skipped (not on the first
-// pass, whose state is the previous solve's).
+// right-hand side of the inner solve.  One workgroup per SpMV panel (its
+// partial dots in slot p).  Panels of systems whose refinement has finished
+// are skipped (not on the first pass, whose state is the previous solve's).
 template <bool BATCH, int SH>
-__global__ void __launch_bounds__(8 * SpmvShape<SH>::G)
+__global__ void __launch_bounds__(SpmvShape<SH>::NT)
 k_resid(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
         const int *__restrict__ numnbrs, const int *__restrict__ jlist, const double *__restrict__ val,
         const double2 *__restrict__ x, const double2 *__restrict__ b, const double *__restrict__ Hd,
-        float2 *__restrict__ r32, double *__restrict__ partials, int pstride, const int *__restrict__ sblk_beg,
+        float2 *__restrict__ r32, double *__restrict__ partials, int pstride, PanelArgs A,
         const int *__restrict__ row_sys, const MixState *__restrict__ ms, int first) {
-  const int blk = blockIdx.x;   // rows and partial slot
-  int r0, r1;
-  sblk_rows<BATCH>(n, sblk_beg, blk, r0, r1);
-  if (BATCH && !first && r0 < r1 && ms[row_sys[r0]].done) return;
-  constexpr int G = SpmvShape<SH>::G, U = SpmvShape<SH>::U;
-  const int w = wave_in_block(), lane = lane_id();
-  const int i = r0 + w * (SWR_WAVE / G) + lane / G;
-  const bool in = i < r1;
-  const bool owner = lane % G == 0 && in;
-  const int beg = in ? firstnbr[i] : 0, cnt = in ? numnbrs[i] : 0;
-  double2 q = row_spmv4<double, double, G, U>(beg, cnt, jlist, val, x);
-  double d[6] = {0, 0, 0, 0, 0, 0};
-  if (owner) {
-    const double eta = P->eta[type[i]];
-    const double2 xi = x[i], bi = b[i];
-    const double rs = bi.x - (eta * xi.x + q.x), rt = bi.y - (eta * xi.y + q.y);
-    const double h = Hd[i];
-    d[0] = h * rs * rs; d[1] = h * rt * rt;
-    d[2] = rs * rs; d[3] = rt * rt;
-    d[4] = bi.x * bi.x; d[5] = bi.y * bi.y;
-    r32[i] = make_float2((float)rs, (float)rt);
+  constexpr int NT = SpmvShape<SH>::NT, G = SpmvShape<SH>::G, U = SpmvShape<SH>::U;
+  __shared__ double2 xl[SWR_PANEL_MAXC];
+  const int p = blockIdx.x;
+  if (BATCH && !first) {
+    const int r0 = A.pbeg[p];
+    if (r0 < A.pbeg[p + 1] && ms[row_sys[A.prow[r0]]].done) return;
   }
-  block_partials<6, 8 * G>(d, partials, pstride, blk);
+  double d[6] = {0, 0, 0, 0, 0, 0};
+  panel_rows<double, double, NT, G, U>(p, A, firstnbr, numnbrs, jlist, val, x, xl, [&](int i, bool live, double2 q) {
+    if (lane_id() % G == 0 && live) {
+      const double eta = P->eta[type[i]];
+      const double2 xi = x[i], bi = b[i];
+      const double rs = bi.x - (eta * xi.x + q.x), rt = bi.y - (eta * xi.y + q.y);
+      const double h = Hd[i];
+      d[0] += h * rs * rs; d[1] += h * rt * rt;
+      d[2] += rs * rs; d[3] += rt * rt;
+      d[4] += bi.x * bi.x; d[5] += bi.y * bi.y;
+      r32[i] = make_float2((float)rs, (float)rt);
+    }
+  });
+  block_partials<6, NT>(d, partials, pstride, p);
 }
 
 // Outer test and the inner tolerances, one workgroup per system (over its
@@ -139,14 +124,14 @@ k_resid(int n, const DevParams *__restrict__ P, const int *__restrict__ type, co
 // and its d stays 0.
 template <int NT>
 __global__ void __launch_bounds__(NT)
-k_resid_scalars(const double *__restrict__ partials, int pstride, const int *__restrict__ sys_sblk,
+k_resid_scalars(const double *__restrict__ partials, int pstride, const int *__restrict__ sys_panel,
                 MixCtl *__restrict__ ctl, MixState *__restrict__ msa, CgScalars *__restrict__ inner_all, double tol,
                 int first, DevStatus *st, int *__restrict__ iters_all) {
   const int sys = blockIdx.x;
   MixState m = msa[sys];
   if (!first && m.done) return;
   double tot[6];
-  reduce_partials<6, NT>(partials, pstride, sys_sblk[sys], sys_sblk[sys + 1], tot);
+  reduce_partials<6, NT>(partials, pstride, sys_panel[sys], sys_panel[sys + 1], tot);
   if (threadIdx.x != 0) return;
   CgScalars *inner = inner_all + 2 * sys;
   if (first) {
@@ -263,46 +248,44 @@ k_pcg_init_scalars(const double *__restrict__ partials, int pstride, CgLayout L,
 }
 
 template <bool BATCH, int SH>
-__global__ void __launch_bounds__(8 * SpmvShape<SH>::G)
+__global__ void __launch_bounds__(SpmvShape<SH>::NT)
 k_pcg_spmv(int n, int nsys, const DevParams *__restrict__ P, const int *__restrict__ type,
            const int *__restrict__ firstnbr, const int *__restrict__ numnbrs, const int *__restrict__ jlist,
            const float *__restrict__ val, const float2 *__restrict__ p, float2 *__restrict__ q,
            const MixCtl *__restrict__ ctl, const PcgState *__restrict__ pst, double *__restrict__ partials,
-           int pstride, const int *__restrict__ sblk_beg, const int *__restrict__ row_sys, int gate = 1) {
-  const int blk = blockIdx.x;   // rows and partial slot
-  int r0, r1;
-  sblk_rows<BATCH>(n, sblk_beg, blk, r0, r1);
-  constexpr int G = SpmvShape<SH>::G, U = SpmvShape<SH>::U;
-  const int w = wave_in_block(), lane = lane_id();
-  const int i = r0 + w * (SWR_WAVE / G) + lane / G;
-  const bool in = i < r1;
-  // the row's extent is requested together with the solve state
-  const int beg0 = in ? firstnbr[i] : 0, cnt0 = in ? numnbrs[i] : 0;
-  // (gate 0: the autotuner's timing launches, which run every row)
-  if (gate && ctl->inner_ndone >= nsys && cnt0 >= 0) return;
-  if (gate && BATCH && r0 < r1 && !pcg_live(pst + 2 * row_sys[r0])) return;
-  const float2 sum = row_spmv4<float, float, G, U>(beg0, cnt0, jlist, val, p);
-  double dd[2] = {0, 0};
-  if (lane % G == 0 && in) {
-    const float eta = (float)P->eta[type[i]];
-    const float2 pi = p[i];
-    const float2 qq = make_float2(eta * pi.x + sum.x, eta * pi.y + sum.y);
-    q[i] = qq;
-    dd[0] = (double)pi.x * qq.x;
-    dd[1] = (double)pi.y * qq.y;
+           int pstride, PanelArgs A, const int *__restrict__ row_sys, int gate = 1) {
+  constexpr int NT = SpmvShape<SH>::NT, G = SpmvShape<SH>::G, U = SpmvShape<SH>::U;
+  __shared__ float2 xl[SWR_PANEL_MAXC];
+  const int pn = blockIdx.x;   // panel and partial slot
+  // (gate 0: the autotuner's timing launches, which run every panel)
+  if (gate && ctl->inner_ndone >= nsys) return;
+  if (gate && BATCH) {
+    const int r0 = A.pbeg[pn];
+    if (r0 < A.pbeg[pn + 1] && !pcg_live(pst + 2 * row_sys[A.prow[r0]])) return;
   }
-  block_partials<2, 8 * G>(dd, partials, pstride, blk);
+  double dd[2] = {0, 0};
+  panel_rows<float, float, NT, G, U>(pn, A, firstnbr, numnbrs, jlist, val, p, xl, [&](int i, bool live, float2 sum) {
+    if (lane_id() % G == 0 && live) {
+      const float eta = (float)P->eta[type[i]];
+      const float2 pi = p[i];
+      const float2 qq = make_float2(eta * pi.x + sum.x, eta * pi.y + sum.y);
+      q[i] = qq;
+      dd[0] += (double)pi.x * qq.x;
+      dd[1] += (double)pi.y * qq.y;
+    }
+  });
+  block_partials<2, NT>(dd, partials, pstride, pn);
 }
 
 template <int NT>
 __global__ void __launch_bounds__(NT)
-k_pcg_alpha(const double *__restrict__ partials, int pstride, const int *__restrict__ sys_sblk,
+k_pcg_alpha(const double *__restrict__ partials, int pstride, const int *__restrict__ sys_panel,
             PcgState *__restrict__ pst) {
   const int sys = blockIdx.x;
   PcgState *c = pst + 2 * sys;
   if (!pcg_live(c)) return;
   double tot[2];
-  reduce_partials<2, NT>(partials, pstride, sys_sblk[sys], sys_sblk[sys + 1], tot);
+  reduce_partials<2, NT>(partials, pstride, sys_panel[sys], sys_panel[sys + 1], tot);
   if (threadIdx.x != 0) return;
   for (int s = 0; s < 2; s++) c[s].alpha = c[s].active ? c[s].rz / tot[s] : 0.0;
 }
@@ -433,7 +416,7 @@ int swr_qeq_mixed_solve(swr_system *S) {
   const int nsys = S->nsys;
   const bool batch = nsys > 1;
   const CgLayout L = swr_qeq_cg_layout(S);
-  const int nblk = batch ? S->nsblk : div_up(n, RPB);   // half-wave-per-row kernels
+  const int nblk = S->npanel;   // SpMV kernels: one workgroup per panel
   const int ps = S->cg_pstride;
   char *w = (char *)S->mixw;
   MixCtl *ctl = (MixCtl *)w;
@@ -456,21 +439,21 @@ int swr_qeq_mixed_solve(swr_system *S) {
     swr_kt_begin(S, KT_MIXED_OUTER);
     spmv_dispatch(S->spmv_shape[2], [&](auto shc) {
       constexpr int SH = decltype(shc)::value;
-      const dim3 tb(8 * SpmvShape<SH>::G);
+      const dim3 tb(SpmvShape<SH>::NT);
       if (batch)
         hipLaunchKernelGGL((k_resid<true, SH>), dim3(nblk), tb, 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
                            S->H_numnbrs, S->H_jlist, S->H_val, (const double2 *)x, b, S->Hdia_inv, rhs, S->partials,
-                           ps, S->sblk_beg, S->row_sys, ms, outer == 0 ? 1 : 0);
+                           ps, panel_args(S), S->row_sys, ms, outer == 0 ? 1 : 0);
       else
         hipLaunchKernelGGL((k_resid<false, SH>), dim3(nblk), tb, 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
                            S->H_numnbrs, S->H_jlist, S->H_val, (const double2 *)x, b, S->Hdia_inv, rhs, S->partials,
-                           ps, S->sblk_beg, S->row_sys, ms, outer == 0 ? 1 : 0);
+                           ps, panel_args(S), S->row_sys, ms, outer == 0 ? 1 : 0);
     });
     if (batch)
-      hipLaunchKernelGGL(k_resid_scalars<BT>, dim3(nsys), dim3(BT), 0, S->stream, S->partials, ps, S->sys_sblk, ctl,
+      hipLaunchKernelGGL(k_resid_scalars<BT>, dim3(nsys), dim3(BT), 0, S->stream, S->partials, ps, S->sys_panel, ctl,
                          ms, inner, S->ctl.tol * OUTER_TOL_FACTOR, outer == 0 ? 1 : 0, S->status, S->sys_iters);
     else
-      hipLaunchKernelGGL(k_resid_scalars<RT>, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, S->sys_sblk, ctl,
+      hipLaunchKernelGGL(k_resid_scalars<RT>, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, S->sys_panel, ctl,
                          ms, inner, S->ctl.tol * OUTER_TOL_FACTOR, outer == 0 ? 1 : 0, S->status, S->sys_iters);
     swr_kt_end(S, KT_MIXED_OUTER);
     HIP_CHECK(hipGetLastError());
@@ -496,20 +479,20 @@ int swr_qeq_mixed_solve(swr_system *S) {
       for (int k = 0; k < nb && it < INNER_IMAX; k++, it++) {
         spmv_dispatch(S->spmv_shape[1], [&](auto shc) {
           constexpr int SH = decltype(shc)::value;
-          const dim3 tb(8 * SpmvShape<SH>::G);
+          const dim3 tb(SpmvShape<SH>::NT);
           if (batch)
             hipLaunchKernelGGL((k_pcg_spmv<true, SH>), dim3(nblk), tb, 0, S->stream, n, nsys, S->dp, S->type,
                                S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
-                               S->partials, ps, S->sblk_beg, S->row_sys);
+                               S->partials, ps, panel_args(S), S->row_sys);
           else
             hipLaunchKernelGGL((k_pcg_spmv<false, SH>), dim3(nblk), tb, 0, S->stream, n, nsys, S->dp, S->type,
                                S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
-                               S->partials, ps, S->sblk_beg, S->row_sys);
+                               S->partials, ps, panel_args(S), S->row_sys);
         });
         if (batch)
-          hipLaunchKernelGGL(k_pcg_alpha<BT>, dim3(nsys), dim3(BT), 0, S->stream, S->partials, ps, S->sys_sblk, pst);
+          hipLaunchKernelGGL(k_pcg_alpha<BT>, dim3(nsys), dim3(BT), 0, S->stream, S->partials, ps, S->sys_panel, pst);
         else
-          hipLaunchKernelGGL(k_pcg_alpha<RT>, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, S->sys_sblk, pst);
+          hipLaunchKernelGGL(k_pcg_alpha<RT>, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, S->sys_panel, pst);
         hipLaunchKernelGGL(k_pcg_update, dim3(L.ntiles), dim3(UT), 0, S->stream, L, S->Hdia_inv, dv, rv, zv,
                            (const float2 *)pv, (const float2 *)qv, pst, S->partials, ps);
         if (batch)
@@ -547,7 +530,7 @@ int swr_qeq_time_mixed(swr_system *S, int which, int sh, int reps, float *ms) {
   if (!S->mixed || !S->mixw || !S->H_val32 || S->n <= 0) return SWR_ERR_STATE;
   const int n = S->n, nsys = S->nsys;
   const bool batch = nsys > 1;
-  const int nblk = batch ? S->nsblk : div_up(n, RPB);
+  const int nblk = S->npanel;
   const int ps = S->cg_pstride;
   char *w = (char *)S->mixw;
   MixCtl *ctl = (MixCtl *)w;
@@ -562,27 +545,27 @@ int swr_qeq_time_mixed(swr_system *S, int which, int sh, int reps, float *ms) {
   HIP_CHECK(hipEventCreate(&e[1]));
   spmv_dispatch(sh, [&](auto shc) {
     constexpr int SH = decltype(shc)::value;
-    const dim3 tb(8 * SpmvShape<SH>::G);
+    const dim3 tb(SpmvShape<SH>::NT);
     for (int r = -1; r < reps; r++) {
       if (r == 0) (void)hipEventRecord(e[0], S->stream);
       if (which == 0) {
         if (batch)
           hipLaunchKernelGGL((k_resid<true, SH>), dim3(nblk), tb, 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
-                             S->H_numnbrs, S->H_jlist, S->H_val, x, b, S->Hdia_inv, rhs, S->partials, ps, S->sblk_beg,
+                             S->H_numnbrs, S->H_jlist, S->H_val, x, b, S->Hdia_inv, rhs, S->partials, ps, panel_args(S),
                              S->row_sys, mst, 1);
         else
           hipLaunchKernelGGL((k_resid<false, SH>), dim3(nblk), tb, 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
-                             S->H_numnbrs, S->H_jlist, S->H_val, x, b, S->Hdia_inv, rhs, S->partials, ps, S->sblk_beg,
+                             S->H_numnbrs, S->H_jlist, S->H_val, x, b, S->Hdia_inv, rhs, S->partials, ps, panel_args(S),
                              S->row_sys, mst, 1);
       } else {
         if (batch)
           hipLaunchKernelGGL((k_pcg_spmv<true, SH>), dim3(nblk), tb, 0, S->stream, n, nsys, S->dp, S->type,
                              S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
-                             S->partials, ps, S->sblk_beg, S->row_sys, 0);
+                             S->partials, ps, panel_args(S), S->row_sys, 0);
         else
           hipLaunchKernelGGL((k_pcg_spmv<false, SH>), dim3(nblk), tb, 0, S->stream, n, nsys, S->dp, S->type,
                              S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
-                             S->partials, ps, S->sblk_beg, S->row_sys, 0);
+                             S->partials, ps, panel_args(S), S->row_sys, 0);
       }
     }
     (void)hipEventRecord(e[1], S->stream);

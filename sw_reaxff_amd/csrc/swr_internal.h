@@ -18,6 +18,8 @@
 
 #define SWR_MAX_ELEM 8    // force-field elements usable per system
 #define SWR_MAX_TYPES 8   // LAMMPS atom types per system
+#define SWR_PANEL_ROWS 64      // H rows per SpMV panel
+#define SWR_PANEL_MAXC 3072    // distinct columns a panel's LDS list holds
 #define SWR_WAVE 64
 
 // ---- error flags raised by kernels (bit mask in DevStatus::flags)
@@ -175,7 +177,7 @@ struct CgScalars {
 // Kernel timer slots.
 enum { KT_VERLET = 0, KT_FAR, KT_BUILD_H, KT_CG_ITER, KT_CG_SETUP, KT_BO, KT_VALIDATE, KT_SPMV, KT_RESET,
        KT_QEQ_Q, KT_BINS, KT_MIXED_INNER, KT_MIXED_OUTER, KT_BONDS, KT_CG_SCALARS, KT_MARK, KT_CG_UPDATE, KT_NONBONDED, KT_ATOM_ENERGY,
-       KT_DBOND, KT_HBONDS, KT_VALENCE, KT_TORSION, KT_COUNT };
+       KT_DBOND, KT_HBONDS, KT_VALENCE, KT_TORSION, KT_PANEL, KT_COUNT };
 
 struct swr_system {
   // ---- host-side configuration
@@ -287,6 +289,19 @@ struct swr_system {
   size_t cand_cap = 0;
   double *ws = nullptr;            // 13 arrays of N doubles, see WS_* below
   // ---- QEq
+  // SpMV panels (panel.hip): the local rows in Morton order of their bins,
+  // grouped by system, cut into panels of at most SWR_PANEL_ROWS rows of
+  // one system; a panel's distinct H columns (at most SWR_PANEL_MAXC) are
+  // listed in pcol, and each H entry holds its column's index in that list
+  // in jloc, so the SpMV gathers x from LDS.  pcnt[p] < 0: panel p gathers
+  // from global memory (more columns than the list holds, or panels off).
+  int *prow = nullptr, *pbeg = nullptr, *pcnt = nullptr, *pcol = nullptr, *pkey = nullptr;
+  int *sys_panel = nullptr;   // [nsys + 1] system s owns panels [sys_panel[s], sys_panel[s+1])
+  unsigned short *jloc = nullptr;
+  size_t cap_prow = 0, cap_pbeg = 0, cap_pcnt = 0, cap_pcol = 0, cap_pkey = 0, cap_jloc = 0, cap_sys_panel = 0;
+  int npanel = 0, prow_n = -1;
+  long long nbr_gen = 0, layout_gen = 0, prow_nbr_gen = -1, prow_layout_gen = -1;
+  int panel_shape = -1;   // tuning family "panels": 0 (default) lists on, 1 off
   int *H_firstnbr = nullptr, *H_numnbrs = nullptr, *H_jlist = nullptr;
   double *H_val = nullptr;
   float *H_val32 = nullptr;
@@ -418,6 +433,11 @@ int swr_noalloc_init(int device);
 int swr_qeq_cg_noalloc_run(swr_system *S, const double *b, double *x, int *iters_s, int *iters_t);
 int swr_status_to_rc(unsigned int flags);
 int swr_lists_time_far(swr_system *S, int sh, int reps, float *ms);
+// stable LDS radix sort of (key, value) pairs (nbr.hip)
+int swr_radix_sort_i32(swr_system *S, const int *keys, const int *vals_in, int n, int nbits, int *vals_out);
+// SpMV panels (panel.hip): row order + panel ranges when stale, then the
+// column lists and local indices of the current H
+int swr_panel_build(swr_system *S);
 
 template <class T>
 inline int swr_alloc(T **p, size_t count) {

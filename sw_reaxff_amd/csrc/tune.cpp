@@ -4,13 +4,16 @@
 // depends on the workload (row lengths, system sizes, how many systems a
 // batch holds) more than on anything known at compile time:
 //
-//   spmv_fp64   the FP64 CG's SpMV (k_cg_spmv): G lanes per H row and U
-//               groups of 4 entries in flight per lane (G8U4, G16U2, G16U4,
-//               G32U2);
+//   spmv_fp64   the FP64 CG's SpMV (k_cg_spmv): T threads per panel of H
+//               rows, G lanes per row and U groups of 4 entries in flight
+//               per lane (T512G8U4, T512G8U2, T256G8U4, T256G4U4);
 //   spmv_fp32   the mixed-precision inner solve's FP32 SpMV (k_pcg_spmv);
 //   spmv_resid  the mixed-precision outer FP64 residual (k_resid);
 //   far         the far pass (k_far_ws): gatherer and writer waves per
-//               workgroup (G4W2, G4W4, G6W2, G8W4).
+//               workgroup (G4W2, G4W4, G6W2, G8W4);
+//   panels      whether the SpMV panels keep their LDS column lists (on)
+//               or gather every entry from global memory (off); not timed,
+//               set with swr_set_shape (e.g. to compare the two paths).
 //
 // swr_autotune times every shape of every family on the system's current
 // state with HIP events and keeps the fastest; later launches use it.  A
@@ -34,30 +37,35 @@
 
 namespace {
 
-const char *const kSpmvNames[] = {"G8U4", "G16U2", "G16U4", "G32U2"};
+const char *const kSpmvNames[] = {"T512G8U4", "T512G8U2", "T256G8U4", "T256G4U4"};
 constexpr int kSpmvN = 4;
-constexpr int kSpmvDefault = 2;   // G16U4
+constexpr int kSpmvDefault = 0;   // T512G8U4
 const char *const kFarNames[] = {"G4W2", "G4W4", "G6W2", "G8W4"};
 constexpr int kFarN = SWR_FAR_NSHAPES;
+
+const char *const kPanelNames[] = {"on", "off"};
 
 struct Family {
   const char *name;
   const char *const *shapes;
   int nshapes;
   int dflt;
+  bool timed;   // swr_autotune times it (otherwise set with swr_set_shape only)
 };
-const Family kFamilies[] = {{"spmv_fp64", kSpmvNames, kSpmvN, kSpmvDefault},
-                            {"spmv_fp32", kSpmvNames, kSpmvN, kSpmvDefault},
-                            {"spmv_resid", kSpmvNames, kSpmvN, kSpmvDefault},
-                            {"far", kFarNames, kFarN, SWR_FAR_G4W2}};
-constexpr int kNFam = 4;
+const Family kFamilies[] = {{"spmv_fp64", kSpmvNames, kSpmvN, kSpmvDefault, true},
+                            {"spmv_fp32", kSpmvNames, kSpmvN, kSpmvDefault, true},
+                            {"spmv_resid", kSpmvNames, kSpmvN, kSpmvDefault, true},
+                            {"far", kFarNames, kFarN, SWR_FAR_G4W2, true},
+                            {"panels", kPanelNames, 2, 0, false}};
+constexpr int kNFam = 5;
 
 int *slot(swr_system *S, int fam) {
   switch (fam) {
     case 0: return &S->spmv_shape[0];
     case 1: return &S->spmv_shape[1];
     case 2: return &S->spmv_shape[2];
-    default: return &S->far_shape;
+    case 3: return &S->far_shape;
+    default: return &S->panel_shape;
   }
 }
 
@@ -129,6 +137,7 @@ int swr_autotune(swr_system *S, int reps) {
   rep << "{\"device\": \"" << device_name(S) << "\", \"atoms\": " << S->n << ", \"reps\": " << reps;
   bool far_done = false;
   for (int fam = 0; fam < kNFam; fam++) {
+    if (!kFamilies[fam].timed) continue;
     const bool ok = fam == 0 ? S->H_val != nullptr
                   : fam <= 2 ? (S->mixed && S->mixw != nullptr && S->H_val32 != nullptr)
                              : (S->far != nullptr && S->bonds != nullptr);
@@ -194,7 +203,7 @@ int swr_set_shape(swr_system *S, const char *family, const char *shape) {
     return SWR_ERR_ARG;
   }
   if (int rc = swr_sync(S)) return rc;
-  *slot(S, fi) = k;
+  *slot(S, fi) = k;   // (panels: from the next H build, swr_qeq_init_matvec)
   return SWR_OK;
 }
 

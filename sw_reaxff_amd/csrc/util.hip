@@ -140,7 +140,7 @@ const char *const swr_kt_names[KT_COUNT] = {"verlet", "far", "build_H", "cg_iter
                                             "validate", "spmv", "reset", "qeq_q", "bins", "mixed_inner",
                                             "mixed_outer", "bonds", "cg_scalars", "mark", "cg_update", "nonbonded",
                                             "atom_energy", "dbond", "hbonds", "valence",
-                                            "torsion"};
+                                            "torsion", "panel"};
 
 // Records a start event for kernel slot k on the system stream.
 void swr_kt_begin(swr_system *S, int k) {

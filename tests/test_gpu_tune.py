@@ -10,7 +10,7 @@ from .parity import harness, water_c0
 
 pytestmark = pytest.mark.gpu
 
-SPMV = ["G8U4", "G16U2", "G16U4", "G32U2"]
+SPMV = ["T512G8U4", "T512G8U2", "T256G8U4", "T256G4U4"]
 FAR = ["G4W2", "G4W4", "G6W2", "G8W4"]
 
 
@@ -74,7 +74,7 @@ def test_spmv_shapes_give_the_same_charges(gpu_lib, ffield, mixed):
             dq = float(np.max(np.abs(q - q0)))
             print(f"[tune] {fam} {shape}: iterations {its} (default {its0}) max|dq| {dq:.2e}", flush=True)
             assert dq <= atol, (fam, shape, dq)
-        h.set_shape(fam, "G16U4")
+        h.set_shape(fam, "T512G8U4")
     if mixed:
         rep = h.autotune(reps=3)
         assert rep["spmv_fp32"]["chosen"] in SPMV and rep["spmv_resid"]["chosen"] in SPMV
