@@ -153,14 +153,14 @@ template <> struct Quad<float, float> {
 // entries (one index load -- four 32-bit columns of x, or four 16-bit
 // positions in a panel's LDS list --, 16-byte value loads), U groups in
 // flight, group g of the row on lane g % G.  J and V point at the row's
-// first entry, xv at x or at the panel's list.  Fixed summation order (per
-// lane two accumulators per right-hand side, then a G-lane butterfly), so
-// a row's sum depends only on the row and (G, U).  Returns the sums on every
-// lane of the row's group.
-template <typename T, typename VT, int G, int U, typename I4>
+// first entry; xv(k) returns the vector entry of index k (x[k], or entry k
+// of the panel's list).  Fixed summation order (per lane two accumulators
+// per right-hand side, then a G-lane butterfly), so a row's sum depends
+// only on the row and (G, U).  Returns the sums on every lane of the row's
+// group.
+template <typename T, typename VT, int G, int U, typename I4, class XV>
 __device__ __forceinline__ typename Vec2<T>::type row_sum4(int cnt, const I4 *__restrict__ J,
-                                                           const VT *__restrict__ V,
-                                                           const typename Vec2<T>::type *__restrict__ xv) {
+                                                           const VT *__restrict__ V, XV xv) {
   typedef typename Vec2<T>::type T2;
   const int sl = lane_id() % G;
   const int n4 = (cnt + 3) >> 2;
@@ -175,7 +175,7 @@ __device__ __forceinline__ typename Vec2<T>::type row_sum4(int cnt, const I4 *__
     for (int u = 0; u < U; u++) vv[u].load(V + 4 * (g + u * G));
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const T2 x0 = xv[jj[u].x], x1 = xv[jj[u].y], x2 = xv[jj[u].z], x3 = xv[jj[u].w];
+      const T2 x0 = xv(jj[u].x), x1 = xv(jj[u].y), x2 = xv(jj[u].z), x3 = xv(jj[u].w);
       a += vv[u].v[0] * x0.x; b += vv[u].v[0] * x0.y;
       c += vv[u].v[1] * x1.x; d += vv[u].v[1] * x1.y;
       a += vv[u].v[2] * x2.x; b += vv[u].v[2] * x2.y;
@@ -186,7 +186,7 @@ __device__ __forceinline__ typename Vec2<T>::type row_sum4(int cnt, const I4 *__
     const I4 jj = J[g];
     Quad<T, VT> vv;
     vv.load(V + 4 * g);
-    const T2 x0 = xv[jj.x], x1 = xv[jj.y], x2 = xv[jj.z], x3 = xv[jj.w];
+    const T2 x0 = xv(jj.x), x1 = xv(jj.y), x2 = xv(jj.z), x3 = xv(jj.w);
     a += vv.v[0] * x0.x; b += vv.v[0] * x0.y;
     c += vv.v[1] * x1.x; d += vv.v[1] * x1.y;
     a += vv.v[2] * x2.x; b += vv.v[2] * x2.y;
@@ -213,22 +213,23 @@ struct PanelArgs {
 };
 inline PanelArgs panel_args(const swr_system *S) { return PanelArgs{S->prow, S->pbeg, S->pcnt, S->pcol, S->jloc}; }
 
-// One workgroup of NT threads per panel: the panel's entries of x (src) are
-// copied into the LDS list xl once (when the panel has a list), then G lanes
-// per row sum each row of the panel, NT/G rows at a time, and call
-// row(i, live, q) on every lane of the row's group (live: a row of the
-// panel; q: the row's sums).  Panels without a list gather from src.
-template <typename T, typename VT, int NT, int G, int U, class Row>
+// One workgroup of NT threads per panel: the panel's entries of the vector
+// (src(j) returns its entry j) are copied into the LDS list xl once (when
+// the panel has a list), then G lanes per row sum each row of the panel,
+// NT/G rows at a time, and call row(i, live, q) on every lane of the row's
+// group (live: a row of the panel; q: the row's sums).  Panels without a
+// list gather through src.
+template <typename T, typename VT, int NT, int G, int U, class Src, class Row>
 __device__ __forceinline__ void panel_rows(int p, const PanelArgs &A, const int *__restrict__ firstnbr,
                                            const int *__restrict__ numnbrs, const int *__restrict__ jlist,
-                                           const VT *__restrict__ val, const typename Vec2<T>::type *__restrict__ src,
-                                           typename Vec2<T>::type *xl, Row row) {
+                                           const VT *__restrict__ val, Src src, typename Vec2<T>::type *xl,
+                                           Row row) {
   typedef typename Vec2<T>::type T2;
   const int r0 = A.pbeg[p], r1 = A.pbeg[p + 1];
   const int m = A.pcnt[p];
   if (m >= 0) {
     const int *pc = A.pcol + (size_t)p * SWR_PANEL_MAXC;
-    for (int k = threadIdx.x; k < m; k += NT) xl[k] = src[pc[k]];
+    for (int k = threadIdx.x; k < m; k += NT) xl[k] = src(pc[k]);
     __syncthreads();
   }
   const int grp = threadIdx.x / G;
@@ -239,7 +240,8 @@ __device__ __forceinline__ void panel_rows(int p, const PanelArgs &A, const int 
     const int beg = live ? firstnbr[i] : 0, cnt = live ? numnbrs[i] : 0;
     T2 q;
     if (m >= 0)
-      q = row_sum4<T, VT, G, U>(cnt, reinterpret_cast<const ushort4 *>(A.jloc + beg), val + beg, xl);
+      q = row_sum4<T, VT, G, U>(cnt, reinterpret_cast<const ushort4 *>(A.jloc + beg), val + beg,
+                                [&](int k) { return xl[k]; });
     else   // (rare: one group in flight keeps the kernel's registers those of the list path)
       q = row_sum4<T, VT, G, 1>(cnt, reinterpret_cast<const int4 *>(jlist + beg), val + beg, src);
     row(i, live, q);
@@ -304,7 +306,8 @@ k_cg_spmv(int n, const DevParams *__restrict__ P, const int *__restrict__ type, 
     if (r0 < A.pbeg[p + 1] && !cg_any_active(PH, cgs + 2 * row_sys[A.prow[r0]])) return;
   }
   const T2 *src = PH == PH_S1 ? V.x : PH == PH_S2 ? V.d[0] : PH == PH_S3 ? V.d[1] : (par_in ? V.d[1] : V.d[0]);
-  panel_rows<T, VT, NT, G, U>(p, A, firstnbr, numnbrs, jlist, val, src, xl, [&](int i, bool live, T2 q) {
+  panel_rows<T, VT, NT, G, U>(p, A, firstnbr, numnbrs, jlist, val, [&](int j) { return src[j]; }, xl,
+                              [&](int i, bool live, T2 q) {
     if (lane_id() % G == 0 && live) {
       const T eta = (T)P->eta[type[i]];
       const T2 xi = src[i];

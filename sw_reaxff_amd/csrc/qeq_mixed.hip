@@ -75,11 +75,12 @@ struct MixCtl {
   int pad;
 };
 
-// Workspace layout: MixCtl at 0, MixState[nsys] from byte 64, PcgState[2 nsys]
-// and the float2 vectors each from the next 256-byte boundary.
+// Workspace layout: MixCtl at 0, MixState[nsys] from byte 64, two buffers of
+// PcgState[2 nsys] (the inner state alternates between them) and the float2
+// vectors each from the next 256-byte boundary.
 constexpr size_t MS_OFF = 64;
 inline size_t pcg_off(int nsys) { return (MS_OFF + sizeof(MixState) * (size_t)nsys + 255) & ~(size_t)255; }
-inline size_t vec_off(int nsys) { return (pcg_off(nsys) + sizeof(PcgState) * 2 * (size_t)nsys + 255) & ~(size_t)255; }
+inline size_t vec_off(int nsys) { return (pcg_off(nsys) + sizeof(PcgState) * 4 * (size_t)nsys + 255) & ~(size_t)255; }
 
 __device__ __forceinline__ bool pcg_live(const PcgState *__restrict__ c) { return (c[0].active | c[1].active) != 0; }
 
@@ -103,7 +104,8 @@ k_resid(int n, const DevParams *__restrict__ P, const int *__restrict__ type, co
     if (r0 < A.pbeg[p + 1] && ms[row_sys[A.prow[r0]]].done) return;
   }
   double d[6] = {0, 0, 0, 0, 0, 0};
-  panel_rows<double, double, NT, G, U>(p, A, firstnbr, numnbrs, jlist, val, x, xl, [&](int i, bool live, double2 q) {
+  panel_rows<double, double, NT, G, U>(p, A, firstnbr, numnbrs, jlist, val, [&](int j) { return x[j]; }, xl,
+                                       [&](int i, bool live, double2 q) {
     if (lane_id() % G == 0 && live) {
       const double eta = P->eta[type[i]];
       const double2 xi = x[i], bi = b[i];
@@ -196,15 +198,19 @@ __global__ void k_mixed_update(int n, double2 *__restrict__ x, const float2 *__r
 // standard form, FP32 vectors (s and t side by side as float2), FP64 dots
 // and scalars.  Unlike the pipelined recurrence it needs no setup products
 // and keeps converging at the FP64 rate over the one to two orders of
-// magnitude an inner solve asks for.  One iteration: q = eta p + H p with
-// the p.q partials (k_pcg_spmv), alpha (k_pcg_alpha), d += alpha p,
-// r -= alpha q, z = M^-1 r with r.z (k_pcg_update), beta and the stopping
-// test sqrt(r.z) <= tol |b| on the outer test's norm (k_pcg_beta),
-// p = z + beta p (k_pcg_p).  Each system has its own scalars (one
-// workgroup per system sums its own blocks' partials in a fixed order, so a
-// system takes the same iterations alone and in a batch); a right-hand side
-// that stops is frozen, rows of a system whose solves both stopped are
-// skipped, and launches after every system stopped exit at once.
+// magnitude an inner solve asks for.  Iteration k is three launches:
+// k_pcg_spmv computes q = eta p + H p with the p.q partials; k_pcg_update
+// sums those partials of its system in every workgroup (the same fixed order
+// everywhere, so all get the same alpha), then d += alpha p, r -= alpha q,
+// z = M^-1 r with the r.z partials; k_pcg_p sums those the same way, forms
+// beta and the stopping test sqrt(r.z) <= tol |b| on the outer test's norm,
+// and sets p = z + beta p.  (Forming p inside the next SpMV instead, from
+// two alternating buffers, made the SpMV 10 us slower than k_pcg_p takes.)
+// Each system has its own scalars (its workgroups sum its own partials in a
+// fixed order, so a system takes the same iterations alone and in a batch);
+// a right-hand side that stops is frozen, panels and tiles of a system whose
+// solves both stopped are skipped, and launches after every system stopped
+// exit at once.
 
 __global__ void __launch_bounds__(UT)
 k_pcg_init(CgLayout L, const float2 *__restrict__ rhs, const double *__restrict__ Hd, float2 *__restrict__ d,
@@ -243,6 +249,7 @@ k_pcg_init_scalars(const double *__restrict__ partials, int pstride, CgLayout L,
     c[s].iter = 0;
     c[s].alpha = c[s].beta = 0.0;
     c[s].active = (c[s].tol > 0.0 && c[s].bnorm > 0.0 && sqrt(fmax(c[s].rz, 0.0)) > c[s].tol * c[s].bnorm) ? 1 : 0;
+    c[2 * gridDim.x + s] = c[s];   // both state buffers (a system without tiles is never carried)
   }
   if (!pcg_live(c)) atomicAdd(&ctl->inner_ndone, 1);
 }
@@ -252,7 +259,7 @@ __global__ void __launch_bounds__(SpmvShape<SH>::NT)
 k_pcg_spmv(int n, int nsys, const DevParams *__restrict__ P, const int *__restrict__ type,
            const int *__restrict__ firstnbr, const int *__restrict__ numnbrs, const int *__restrict__ jlist,
            const float *__restrict__ val, const float2 *__restrict__ p, float2 *__restrict__ q,
-           const MixCtl *__restrict__ ctl, const PcgState *__restrict__ pst, double *__restrict__ partials,
+           const MixCtl *__restrict__ ctl, const PcgState *__restrict__ pst, double *__restrict__ pq_partials,
            int pstride, PanelArgs A, const int *__restrict__ row_sys, int gate = 1) {
   constexpr int NT = SpmvShape<SH>::NT, G = SpmvShape<SH>::G, U = SpmvShape<SH>::U;
   __shared__ float2 xl[SWR_PANEL_MAXC];
@@ -264,7 +271,8 @@ k_pcg_spmv(int n, int nsys, const DevParams *__restrict__ P, const int *__restri
     if (r0 < A.pbeg[pn + 1] && !pcg_live(pst + 2 * row_sys[A.prow[r0]])) return;
   }
   double dd[2] = {0, 0};
-  panel_rows<float, float, NT, G, U>(pn, A, firstnbr, numnbrs, jlist, val, p, xl, [&](int i, bool live, float2 sum) {
+  panel_rows<float, float, NT, G, U>(pn, A, firstnbr, numnbrs, jlist, val, [&](int j) { return p[j]; }, xl,
+                                     [&](int i, bool live, float2 sum) {
     if (lane_id() % G == 0 && live) {
       const float eta = (float)P->eta[type[i]];
       const float2 pi = p[i];
@@ -274,31 +282,28 @@ k_pcg_spmv(int n, int nsys, const DevParams *__restrict__ P, const int *__restri
       dd[1] += (double)pi.y * qq.y;
     }
   });
-  block_partials<2, NT>(dd, partials, pstride, pn);
-}
-
-template <int NT>
-__global__ void __launch_bounds__(NT)
-k_pcg_alpha(const double *__restrict__ partials, int pstride, const int *__restrict__ sys_panel,
-            PcgState *__restrict__ pst) {
-  const int sys = blockIdx.x;
-  PcgState *c = pst + 2 * sys;
-  if (!pcg_live(c)) return;
-  double tot[2];
-  reduce_partials<2, NT>(partials, pstride, sys_panel[sys], sys_panel[sys + 1], tot);
-  if (threadIdx.x != 0) return;
-  for (int s = 0; s < 2; s++) c[s].alpha = c[s].active ? c[s].rz / tot[s] : 0.0;
+  block_partials<2, NT>(dd, pq_partials, pstride, pn);
 }
 
 __global__ void __launch_bounds__(UT)
 k_pcg_update(CgLayout L, const double *__restrict__ Hd, float2 *__restrict__ d, float2 *__restrict__ r,
              float2 *__restrict__ z, const float2 *__restrict__ p, const float2 *__restrict__ q,
-             const PcgState *__restrict__ pst, double *__restrict__ partials, int pstride) {
+             const PcgState *__restrict__ pst, const double *__restrict__ pq_partials,
+             const int *__restrict__ sys_panel, double *__restrict__ partials, int pstride) {
+  __shared__ double alpha[2];
   const int tile = blockIdx.x;
-  const PcgState *c = pst + 2 * L.tile_sys[tile];
+  const int sys = L.tile_sys[tile];
+  const PcgState *c = pst + 2 * sys;
   if (!pcg_live(c)) return;
+  {  // alpha = r.z / p.q of the system (its panels' p.q partials)
+    double tot[2];
+    reduce_partials<2, UT>(pq_partials, pstride, sys_panel[sys], sys_panel[sys + 1], tot);
+    if (threadIdx.x == 0)
+      for (int s = 0; s < 2; s++) alpha[s] = c[s].active ? c[s].rz / tot[s] : 0.0;
+    __syncthreads();
+  }
   const int i = L.tile_beg[tile] + threadIdx.x;
-  const float as = (float)c[0].alpha, at = (float)c[1].alpha;
+  const float as = (float)alpha[0], at = (float)alpha[1];
   double dd[2] = {0, 0};
   if (i < L.tile_beg[tile + 1]) {
     const float2 pp = p[i], qq = q[i];
@@ -316,42 +321,59 @@ k_pcg_update(CgLayout L, const double *__restrict__ Hd, float2 *__restrict__ d, 
   block_partials<2, UT>(dd, partials, pstride, tile);
 }
 
-template <int NT>
-__global__ void __launch_bounds__(NT)
-k_pcg_beta(const double *__restrict__ partials, int pstride, CgLayout L, PcgState *__restrict__ pst,
-           MixCtl *__restrict__ ctl, int imax) {
-  const int sys = blockIdx.x;
-  PcgState *c = pst + 2 * sys;
-  if (!pcg_live(c)) return;
-  double tot[2];
-  reduce_partials<2, NT>(partials, pstride, L.sys_tile[sys], L.sys_tile[sys + 1], tot);
-  if (threadIdx.x != 0) return;
-  for (int s = 0; s < 2; s++) {
-    if (!c[s].active) { c[s].beta = 0.0; continue; }
-    c[s].iter += 1;
-    c[s].beta = tot[s] / c[s].rz;
-    c[s].rz = tot[s];
-    c[s].active = (c[s].iter < imax && sqrt(fmax(c[s].rz, 0.0)) > c[s].tol * c[s].bnorm) ? 1 : 0;
-  }
-  if (!pcg_live(c)) {
-    atomicAdd(&ctl->inner_ndone, 1);
-    atomicMax(&ctl->inner_itmax, max(c[0].iter, c[1].iter));
-  }
-}
-
+// beta, the stopping test and p = z + beta p in one launch: every workgroup
+// of a system sums the system's r.z partials in the same fixed order (so
+// all reach the same beta and the same stopping decision), and the system's
+// first tile writes the new state into `next` while the others still read
+// `cur` (the state alternates between two buffers, iteration by iteration;
+// a stopped system's state is carried over unchanged).
 __global__ void __launch_bounds__(UT)
-k_pcg_p(CgLayout L, const float2 *__restrict__ z, float2 *__restrict__ p, const PcgState *__restrict__ pst) {
+k_pcg_p(CgLayout L, const float2 *__restrict__ z, float2 *__restrict__ p, const PcgState *__restrict__ cur,
+        PcgState *__restrict__ next, const double *__restrict__ partials, int pstride, MixCtl *__restrict__ ctl,
+        int imax) {
+  __shared__ float beta[2];
+  __shared__ int act[2];
   const int tile = blockIdx.x;
-  const PcgState *c = pst + 2 * L.tile_sys[tile];
-  if (!pcg_live(c)) return;
+  const int sys = L.tile_sys[tile];
+  const bool first = tile == L.sys_tile[sys];
+  const PcgState *c = cur + 2 * sys;
+  if (!pcg_live(c)) {
+    if (first && threadIdx.x < 2) next[2 * sys + threadIdx.x] = c[threadIdx.x];
+    return;
+  }
+  {
+    double tot[2];
+    reduce_partials<2, UT>(partials, pstride, L.sys_tile[sys], L.sys_tile[sys + 1], tot);
+    if (threadIdx.x == 0) {
+      PcgState ns[2] = {c[0], c[1]};
+      for (int s = 0; s < 2; s++) {
+        if (!ns[s].active) { ns[s].beta = 0.0; continue; }
+        ns[s].iter += 1;
+        ns[s].beta = tot[s] / ns[s].rz;
+        ns[s].rz = tot[s];
+        ns[s].active = (ns[s].iter < imax && sqrt(fmax(ns[s].rz, 0.0)) > ns[s].tol * ns[s].bnorm) ? 1 : 0;
+      }
+      for (int s = 0; s < 2; s++) {
+        beta[s] = (float)ns[s].beta;
+        act[s] = ns[s].active;
+      }
+      if (first) {
+        next[2 * sys] = ns[0];
+        next[2 * sys + 1] = ns[1];
+        if (!pcg_live(ns)) {
+          atomicAdd(&ctl->inner_ndone, 1);
+          atomicMax(&ctl->inner_itmax, max(ns[0].iter, ns[1].iter));
+        }
+      }
+    }
+    __syncthreads();
+  }
   const int i = L.tile_beg[tile] + threadIdx.x;
   if (i >= L.tile_beg[tile + 1]) return;
-  const float bs = (float)c[0].beta, bt = (float)c[1].beta;
-  const bool as = c[0].active != 0, at = c[1].active != 0;
   const float2 zz = z[i];
   float2 pp = p[i];
-  if (as) pp.x = zz.x + bs * pp.x;
-  if (at) pp.y = zz.y + bt * pp.y;
+  if (act[0]) pp.x = zz.x + beta[0] * pp.x;
+  if (act[1]) pp.y = zz.y + beta[1] * pp.y;
   p[i] = pp;
 }
 
@@ -427,6 +449,7 @@ int swr_qeq_mixed_solve(swr_system *S) {
   float2 *dv = base + MV_D * c, *rv = base + MV_R * c, *zv = base + MV_Z * c, *pv = base + MV_P * c,
          *qv = base + MV_Q * c;
   float2 *rhs = base + MV_B * c;      // the inner solve's b: the FP32 residual
+  double *pq = S->partials + 2 * (size_t)ps;  // p.q partials (rows 2-3; r.z uses rows 0-1)
   CgScalars *inner = S->cgs + 2 * nsys;  // after the FP64 solve's s, t: the inner tolerances
   double2 *x = (double2 *)S->st;
   const double2 *b = (const double2 *)S->b_s;
@@ -477,31 +500,24 @@ int swr_qeq_mixed_solve(swr_system *S) {
     int nb = std::max(2, S->mixed_last_inner + 1);
     while (it < INNER_IMAX) {
       for (int k = 0; k < nb && it < INNER_IMAX; k++, it++) {
+        PcgState *cur = pst + (it & 1) * 2 * nsys, *nxt = pst + ((it + 1) & 1) * 2 * nsys;
         spmv_dispatch(S->spmv_shape[1], [&](auto shc) {
           constexpr int SH = decltype(shc)::value;
           const dim3 tb(SpmvShape<SH>::NT);
           if (batch)
             hipLaunchKernelGGL((k_pcg_spmv<true, SH>), dim3(nblk), tb, 0, S->stream, n, nsys, S->dp, S->type,
-                               S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
-                               S->partials, ps, panel_args(S), S->row_sys);
+                               S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, cur,
+                               pq, ps, panel_args(S), S->row_sys);
           else
             hipLaunchKernelGGL((k_pcg_spmv<false, SH>), dim3(nblk), tb, 0, S->stream, n, nsys, S->dp, S->type,
-                               S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
-                               S->partials, ps, panel_args(S), S->row_sys);
+                               S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, cur,
+                               pq, ps, panel_args(S), S->row_sys);
         });
-        if (batch)
-          hipLaunchKernelGGL(k_pcg_alpha<BT>, dim3(nsys), dim3(BT), 0, S->stream, S->partials, ps, S->sys_panel, pst);
-        else
-          hipLaunchKernelGGL(k_pcg_alpha<RT>, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, S->sys_panel, pst);
         hipLaunchKernelGGL(k_pcg_update, dim3(L.ntiles), dim3(UT), 0, S->stream, L, S->Hdia_inv, dv, rv, zv,
-                           (const float2 *)pv, (const float2 *)qv, pst, S->partials, ps);
-        if (batch)
-          hipLaunchKernelGGL(k_pcg_beta<BT>, dim3(nsys), dim3(BT), 0, S->stream, S->partials, ps, L, pst, ctl,
-                             INNER_IMAX);
-        else
-          hipLaunchKernelGGL(k_pcg_beta<RT>, dim3(nsys), dim3(RT), 0, S->stream, S->partials, ps, L, pst, ctl,
-                             INNER_IMAX);
-        hipLaunchKernelGGL(k_pcg_p, dim3(L.ntiles), dim3(UT), 0, S->stream, L, (const float2 *)zv, pv, pst);
+                           (const float2 *)pv, (const float2 *)qv, cur, (const double *)pq,
+                           (const int *)S->sys_panel, S->partials, ps);
+        hipLaunchKernelGGL(k_pcg_p, dim3(L.ntiles), dim3(UT), 0, S->stream, L, (const float2 *)zv, pv, cur, nxt,
+                           (const double *)S->partials, ps, ctl, INNER_IMAX);
       }
       HIP_CHECK(hipGetLastError());
       MixCtl hc;
@@ -513,7 +529,9 @@ int swr_qeq_mixed_solve(swr_system *S) {
       }
       nb = 4;
     }
-    hipLaunchKernelGGL(k_pcg_finish, dim3(1), dim3(256), 0, S->stream, nsys, pst, inner, S->status->inner_iters);
+    // (the state after `it` iterations; stopped systems were carried into it)
+    hipLaunchKernelGGL(k_pcg_finish, dim3(1), dim3(256), 0, S->stream, nsys, pst + (it & 1) * 2 * nsys, inner,
+                       S->status->inner_iters);
     swr_kt_end(S, KT_MIXED_INNER);
     hipLaunchKernelGGL(k_mixed_update, dim3(div_up(n, 256)), dim3(256), 0, S->stream, n, x, (const float2 *)dv);
     HIP_CHECK(hipGetLastError());
@@ -539,6 +557,7 @@ int swr_qeq_time_mixed(swr_system *S, int which, int sh, int reps, float *ms) {
   float2 *base = (float2 *)(w + vec_off(nsys));
   const size_t c = S->ncap;
   float2 *pv = base + MV_P * c, *qv = base + MV_Q * c, *rhs = base + MV_B * c;
+  double *pq = S->partials + 2 * (size_t)ps;
   const double2 *x = (const double2 *)S->st, *b = (const double2 *)S->b_s;
   hipEvent_t e[2];
   HIP_CHECK(hipEventCreate(&e[0]));
@@ -561,11 +580,11 @@ int swr_qeq_time_mixed(swr_system *S, int which, int sh, int reps, float *ms) {
         if (batch)
           hipLaunchKernelGGL((k_pcg_spmv<true, SH>), dim3(nblk), tb, 0, S->stream, n, nsys, S->dp, S->type,
                              S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
-                             S->partials, ps, panel_args(S), S->row_sys, 0);
+                             pq, ps, panel_args(S), S->row_sys, 0);
         else
           hipLaunchKernelGGL((k_pcg_spmv<false, SH>), dim3(nblk), tb, 0, S->stream, n, nsys, S->dp, S->type,
                              S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val32, (const float2 *)pv, qv, ctl, pst,
-                             S->partials, ps, panel_args(S), S->row_sys, 0);
+                             pq, ps, panel_args(S), S->row_sys, 0);
       }
     }
     (void)hipEventRecord(e[1], S->stream);
