@@ -461,6 +461,35 @@ __device__ __forceinline__ void fix_add(unsigned int *flags, unsigned long long 
   if (f != 0) fix_word_add(flags, a, f);
 }
 
+// K fix_adds whose fine-word results are checked for a wrap only after all
+// K were issued, so a lane waits for the returned words once per group
+// instead of once per add.  Terms of FIX_SPLIT and more (rare) take fix_add.
+template <int K>
+__device__ __forceinline__ void fix_add_group(unsigned int *flags, unsigned long long *const (&a)[K],
+                                              const size_t (&coff)[K], const double (&v)[K]) {
+  unsigned long long old[K];
+  long long d[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    old[k] = 0ull;
+    d[k] = 0;
+    if (!(fabs(v[k]) < FIX_SPLIT)) {
+      fix_add(flags, a[k], coff[k], v[k]);
+    } else {
+      d[k] = llrint(v[k] * HB_FIX);
+      if (d[k] != 0) old[k] = atomicAdd(a[k], (unsigned long long)d[k]);
+    }
+  }
+  bool wrap = false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const long long o = (long long)old[k];
+    const long long sum = (long long)(old[k] + (unsigned long long)d[k]);
+    wrap |= ((o ^ sum) & (d[k] ^ sum)) < 0;
+  }
+  if (wrap) atomicOr(flags, SWR_F_FIXED_RANGE);
+}
+
 // The value of the accumulator pair at a (coarse word coff elements on).
 __device__ __forceinline__ double fix_value(const unsigned long long *a, size_t coff) {
   return (double)(long long)a[0] * (1.0 / HB_FIX) + (double)(long long)a[coff] * (1.0 / FIX_COARSE);
@@ -509,9 +538,13 @@ __global__ void __launch_bounds__(BT) k_hb(int n, HbArgs A) {
       g1 += t.ce2 * t.di[1];
       g2 += t.ce2 * t.di[2];
       unsigned long long *ak3 = A.acc + 3 * (size_t)k;
-      fix_add(A.flags, ak3, A.coff, t.ce2 * t.dk[0] + t.ce3 / r_jk * dvec_jk[0]);
-      fix_add(A.flags, ak3 + 1, A.coff, t.ce2 * t.dk[1] + t.ce3 / r_jk * dvec_jk[1]);
-      fix_add(A.flags, ak3 + 2, A.coff, t.ce2 * t.dk[2] + t.ce3 / r_jk * dvec_jk[2]);
+      {
+        unsigned long long *const ga[3] = {ak3, ak3 + 1, ak3 + 2};
+        const size_t gc[3] = {A.coff, A.coff, A.coff};
+        const double gv[3] = {t.ce2 * t.dk[0] + t.ce3 / r_jk * dvec_jk[0], t.ce2 * t.dk[1] + t.ce3 / r_jk * dvec_jk[1],
+                              t.ce2 * t.dk[2] + t.ce3 / r_jk * dvec_jk[2]};
+        fix_add_group<3>(A.flags, ga, gc, gv);
+      }
     }
     ce1 = wave_sum(ce1);
     g0 = wave_sum(g0);
@@ -520,9 +553,10 @@ __global__ void __launch_bounds__(BT) k_hb(int n, HbArgs A) {
     if (lane == 0) {
       A.Cdbo[pi] += ce1;
       unsigned long long *ai3 = A.acc + 3 * (size_t)i;
-      fix_add(A.flags, ai3, A.coff, g0);
-      fix_add(A.flags, ai3 + 1, A.coff, g1);
-      fix_add(A.flags, ai3 + 2, A.coff, g2);
+      unsigned long long *const ga[3] = {ai3, ai3 + 1, ai3 + 2};
+      const size_t gc[3] = {A.coff, A.coff, A.coff};
+      const double gv[3] = {g0, g1, g2};
+      fix_add_group<3>(A.flags, ga, gc, gv);
     }
   }
   e = wave_sum(e);
@@ -565,8 +599,44 @@ __global__ void __launch_bounds__(BT) k_hb_fold(int N, unsigned long long *__res
 // h, and the angle derivative into fCdDelta[.][0..2] of k, j and h.  One
 // thread per j writes j's own bond slots and row; the terms of k and h go
 // into 64-bit fixed-point accumulators by integer atomics (order-independent
-// sums), folded into fCdDelta by k_va_fold.
+// sums), folded into fCdDelta by k_va_fold.  The centres with angles are
+// listed by angle count (k_va_count, k_wl_scan, k_va_scatter: a centre with
+// ten bonds has 45 pairs, one with two bonds one) and the threads take them
+// largest first.
 constexpr double DEG2RAD_PI = 3.14159265;   // constPI, reaxc_defs_sunway.h:61 (DEG2RAD :55)
+
+// ---------------------------------------------------------------- work lists
+// The valence and torsion passes give each lane one item (a centre atom, a
+// central bond) and most items have little or no work while a few have
+// dozens of angles: items are counted first, bucketed by count and handed
+// to lanes largest first, so the lanes of a wave do about the same work.
+// Counters: the list length, a histogram of the counts and each bucket's
+// next free position.  Which lane takes which item changes no result: every
+// item writes only its own entries and adds into the others through
+// fixed-point integer atomics.
+constexpr int WL_NB = 128;    // count buckets; counts of WL_NB - 1 and more share the last
+enum { WL_NTASK = 0, WL_HIST = 1, WL_CUR = WL_HIST + WL_NB, WL_CTL_INTS = WL_CUR + WL_NB };
+__device__ __forceinline__ int wl_bucket(int c) { return c < WL_NB - 1 ? c : WL_NB - 1; }
+
+// Bucket starts, largest count first; the histogram is left at zero.
+__global__ void __launch_bounds__(WL_NB) k_wl_scan(int *__restrict__ ctl) {
+  __shared__ int h[WL_NB];
+  const int b = threadIdx.x;
+  h[b] = ctl[WL_HIST + b];
+  ctl[WL_HIST + b] = 0;
+  __syncthreads();
+  int start = 0;
+  for (int q = b + 1; q < WL_NB; ++q) start += h[q];
+  ctl[WL_CUR + b] = start;
+  if (b == 0) ctl[WL_NTASK] = start + h[0];
+}
+
+// Workgroup histogram of counts c (> 0 adds) into the global one.
+__device__ __forceinline__ void wl_hist_flush(int *hist, int *__restrict__ ctl) {
+  __syncthreads();
+  for (int b = threadIdx.x; b < WL_NB; b += BT)
+    if (hist[b]) atomicAdd(&ctl[WL_HIST + b], hist[b]);
+}
 
 struct VaArgs {
   unsigned int *flags;   // DevStatus::flags
@@ -580,11 +650,79 @@ struct VaArgs {
   size_t stride;
   unsigned long long *acc;   // [2][N][4] (fine, then coarse words)
   size_t coff;               // 4N
+  int *cnt;                  // [n] angle count, then rank | bucket
+  int *order;                // [n] centres with angles, largest count first
+  int *ctl;                  // WL_CTL_INTS counters
 };
 
-__global__ void __launch_bounds__(BT) k_valence(int n, VaArgs A) {
+// Angle evaluations of local centre j: the pairs of its bonds that pass the
+// body's filters, times their parameter sets.
+__device__ __forceinline__ int va_count(const VaArgs &A, int j) {
+  const DevParams *__restrict__ P = A.P;
+  const int tj = A.atoms[j].type;
+  const int b0 = A.bidx[j], b1 = A.bend[j];
+  if (tj < 0 || b1 <= b0) return 0;
+  const double thb_cut = P->thb_cut, thb_cutsq = P->thb_cutsq;
+  int c = 0;
+  for (int pk = b0; pk < b1; ++pk) {
+    const double BO_jk = A.BO[pk];
+    if (!(BO_jk > thb_cut)) continue;
+    const int k = A.bonds[pk].nbr;
+    if (A.bend[k] <= A.bidx[k]) continue;
+    const int tk = A.atoms[k].type;
+    for (int ph = pk + 1; ph < b1; ++ph) {
+      const double BO_hj = A.BO[ph];
+      if (!(BO_hj > thb_cut) || !(BO_jk * BO_hj > thb_cutsq)) continue;
+      const int th = A.atoms[A.bonds[ph].nbr].type;
+      if (tk < 0 || th < 0) continue;
+      c += P->thb_n[(tk * SWR_MAX_ELEM + tj) * SWR_MAX_ELEM + th];
+    }
+  }
+  return c;
+}
+
+// One thread per local atom: its count and the histogram; a centre without
+// angles gets its zero energy rows here (the valence kernel skips it).
+__global__ void __launch_bounds__(BT) k_va_count(int n, VaArgs A) {
+  __shared__ int hist[WL_NB];
+  for (int b = threadIdx.x; b < WL_NB; b += BT) hist[b] = 0;
+  __syncthreads();
   const int j = blockIdx.x * BT + threadIdx.x;
-  if (j >= n) return;
+  if (j < n) {
+    const int c = va_count(A, j);
+    A.cnt[j] = c;
+    if (c > 0) {
+      atomicAdd(&hist[wl_bucket(c)], 1);
+    } else {
+      A.part[j] = 0.0;
+      A.part[A.stride + j] = 0.0;
+      A.part[2 * A.stride + j] = 0.0;
+    }
+  }
+  wl_hist_flush(hist, A.ctl);
+}
+
+__global__ void __launch_bounds__(BT) k_va_scatter(int n, VaArgs A) {
+  __shared__ int cnt[WL_NB], base[WL_NB];
+  for (int b = threadIdx.x; b < WL_NB; b += BT) cnt[b] = 0;
+  __syncthreads();
+  const int j = blockIdx.x * BT + threadIdx.x;
+  int b = -1, r = 0;
+  if (j < n && A.cnt[j] > 0) {
+    b = wl_bucket(A.cnt[j]);
+    r = atomicAdd(&cnt[b], 1);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < WL_NB; q += BT) base[q] = cnt[q] ? atomicAdd(&A.ctl[WL_CUR + q], cnt[q]) : 0;
+  __syncthreads();
+  if (b >= 0) A.order[base[b] + r] = j;
+}
+
+// One thread per listed centre j (largest angle count first).
+__global__ void __launch_bounds__(BT) k_valence(VaArgs A) {
+  const int t = blockIdx.x * BT + threadIdx.x;
+  if (t >= A.ctl[WL_NTASK]) return;
+  const int j = A.order[t];
   const DevParams *__restrict__ P = A.P;
   const int tj = A.atoms[j].type;
   const int b0 = A.bidx[j], b1 = A.bend[j];
@@ -740,10 +878,11 @@ __global__ void __launch_bounds__(BT) k_valence(int n, VaArgs A) {
         A.Cdbo[pk] += cdbo_jk;
         A.Cdbo[ph] += cdbo_hj;
         unsigned long long *ak = A.acc + 4 * (size_t)k, *ah = A.acc + 4 * (size_t)h;
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-          fix_add(A.flags, ak + d, A.coff, ck[d]);
-          fix_add(A.flags, ah + d, A.coff, chh[d]);
+        {
+          unsigned long long *const ga[8] = {ak, ak + 1, ak + 2, ak + 3, ah, ah + 1, ah + 2, ah + 3};
+          const size_t gc[8] = {A.coff, A.coff, A.coff, A.coff, A.coff, A.coff, A.coff, A.coff};
+          const double gv[8] = {ck[0], ck[1], ck[2], ck[3], chh[0], chh[1], chh[2], chh[3]};
+          fix_add_group<8>(A.flags, ga, gc, gv);
         }
       }
     }
@@ -785,15 +924,28 @@ __global__ void __launch_bounds__(BT) k_va_fold(int N, unsigned long long *__res
 
 // ---------------------------------------------------------------- torsions
 // The torsion part of Merge_Torsion_Valence_Angles (host form
-// reaxc_torsion_angles_sw64.c:812-1408; Calculate_Omega :21-104).  Four
-// lanes per local atom j, each over every fourth bond j-k of j (taken once
-// per physical bond: orig_id order, then position for images of one atom),
-// the other bonds j-i and the bonds k-l of k: torsion and conjugation
-// energies; the slot's own Cdbopi and Cdbo written directly; j's row and
-// energies summed over the lanes by a fixed shuffle tree; Cdbo of j-i and
-// k-l and the CdDelta and derivative terms of i, k, l into 64-bit
-// fixed-point accumulators by integer atomics, folded by k_va_fold /
-// k_to_slot_fold.
+// reaxc_torsion_angles_sw64.c:812-1408; Calculate_Omega :21-104).  The work
+// is one task per central bond j-k of a local atom j (taken once per
+// physical bond: orig_id order, then position for images of one atom): over
+// the other bonds j-i of j and the bonds k-l of k, the torsion and
+// conjugation energies.  Most bonds make no torsion and a few make dozens,
+// so the tasks are listed with their exact quadruple counts and handed to
+// lanes in decreasing count (a wave's lanes then do about the same work):
+//
+//   k_to_count    four lanes per local atom: per bond slot the number of
+//                 quadruples the torsion body evaluates (the same filters),
+//                 a histogram of the counts;
+//   k_wl_scan     start of each count bucket in the list (largest first);
+//   k_to_scatter  one thread per local atom: its central bonds with work
+//                 into the list (the order inside a bucket is arbitrary and
+//                 changes no result);
+//   k_torsion     one lane per listed central bond: the slot's Cdbopi and
+//                 Cdbo written directly, j's row terms and energies into
+//                 the slot's row entry, Cdbo of j-i and k-l and the terms of
+//                 i, k, l into 64-bit fixed-point accumulators by integer
+//                 atomics (folded by k_va_fold / k_to_slot_fold);
+//   k_to_row      one thread per local atom: its slots' row entries summed
+//                 in slot order into its row and energies.
 struct ToArgs {
   unsigned int *flags;   // DevStatus::flags
   const DevParams *P;
@@ -806,7 +958,13 @@ struct ToArgs {
   unsigned long long *acc;    // [2][N][4] (fine, then coarse words)
   unsigned long long *sacc;   // [2][bond slots]
   size_t coff, scoff;         // 4N, bond slots
+  const int *bnbr;            // bonds[].nbr, compact
+  int *tc;                    // [bond slots] quadruple count, then rank | bucket
+  int2 *task;                 // [bond slots] (j, slot) of the listed central bonds
+  int *ctl;                   // WL_CTL_INTS counters
+  double *row;                // [bond slots][6] j's row terms and energies per central bond
 };
+
 
 // Calculate_Theta + Calculate_dCos_Theta of bonds a (first) and b from one
 // centre: di belongs to a's far atom, dj to the centre, dk to b's.
@@ -835,24 +993,111 @@ __device__ __forceinline__ double tan_inv(double s, double c) {
   return c / s;
 }
 
-constexpr int TO_LANES = 4;   // lanes per centre atom j, each over every 4th bond slot of j
+// Quadruples the torsion body evaluates for the bond in slot pk of local
+// atom j (type tj >= 0, slots [b0, b1)): the body's filters in its order;
+// 0 when the bond is no central bond (its BO, the orig_id order, a k without
+// bonds or type) or makes no torsion.
+__device__ __forceinline__ int to_count(const ToArgs &A, const swr_atom_pack_t &aj, int tj, int b0, int b1,
+                                        int pk, double thb_cut) {
+  const double BO_jk = A.BO[pk];
+  if (!(BO_jk - thb_cut > 0.0)) return 0;
+  const int k = A.bnbr[pk];
+  const int k0 = A.bidx[k], k1 = A.bend[k];
+  if (k1 <= k0) return 0;
+  const swr_atom_pack_t ak = A.atoms[k];
+  if (!owns_bond(aj, ak)) return 0;
+  const int tk = ak.type;
+  if (tk < 0) return 0;
+  const int pj = A.bonds[pk].sym_index;
+  int c = 0;
+  for (int ph = b0; ph < b1; ++ph) {
+    if (ph == pk) continue;
+    const double BO_ij = A.BO[ph];
+    if (!(BO_ij > thb_cut)) continue;
+    const int i = A.bnbr[ph];
+    const int ti = A.atoms[i].type;
+    if (ti < 0) continue;
+    for (int pw = k0; pw < k1; ++pw) {
+      if (pw == pj) continue;
+      const double BO_kl = A.BO[pw];
+      if (!(BO_kl > thb_cut) || !(BO_ij * BO_jk * BO_kl > thb_cut)) continue;
+      const int l = A.bnbr[pw];
+      if (i == l) continue;
+      const int tl = A.atoms[l].type;
+      if (tl < 0) continue;
+      if (!A.P->tor[((ti * SWR_MAX_ELEM + tj) * SWR_MAX_ELEM + tk) * SWR_MAX_ELEM + tl].cnt) continue;
+      ++c;
+    }
+  }
+  return c;
+}
 
-__global__ void __launch_bounds__(BT) k_torsion(int n, ToArgs A) {
+constexpr int TO_CNT_LANES = 4;   // lanes per atom in k_to_count, each over every 4th slot
+
+__global__ void __launch_bounds__(BT) k_to_count(int n, ToArgs A) {
+  __shared__ int hist[WL_NB];
+  for (int b = threadIdx.x; b < WL_NB; b += BT) hist[b] = 0;
+  __syncthreads();
   const int tid = blockIdx.x * BT + threadIdx.x;
-  const int j = tid / TO_LANES, t = tid % TO_LANES;
+  const int j = tid / TO_CNT_LANES;
+  if (j < n) {
+    const swr_atom_pack_t aj = A.atoms[j];
+    const int b0 = A.bidx[j], b1 = A.bend[j];
+    const double thb_cut = A.P->thb_cut;
+    for (int pk = b0 + tid % TO_CNT_LANES; pk < b1; pk += TO_CNT_LANES) {
+      const int c = aj.type >= 0 ? to_count(A, aj, aj.type, b0, b1, pk, thb_cut) : 0;
+      A.tc[pk] = c;
+      if (c > 0) {
+        atomicAdd(&hist[wl_bucket(c)], 1);
+      } else {
+        double *rw = A.row + 6 * (size_t)pk;
+#pragma unroll
+        for (int q = 0; q < 6; q++) rw[q] = 0.0;
+      }
+    }
+  }
+  wl_hist_flush(hist, A.ctl);
+}
+
+__global__ void __launch_bounds__(BT) k_to_scatter(int n, ToArgs A) {
+  __shared__ int cnt[WL_NB], base[WL_NB];
+  for (int b = threadIdx.x; b < WL_NB; b += BT) cnt[b] = 0;
+  __syncthreads();
+  const int j = blockIdx.x * BT + threadIdx.x;
+  const int b0 = j < n ? A.bidx[j] : 0, b1 = j < n ? A.bend[j] : 0;
+  for (int pk = b0; pk < b1; ++pk) {   // rank in the workgroup's share of the bucket
+    const int c = A.tc[pk];
+    if (c > 0) {
+      const int b = wl_bucket(c);
+      A.tc[pk] = (atomicAdd(&cnt[b], 1) << 8) | b;
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < WL_NB; b += BT) base[b] = cnt[b] ? atomicAdd(&A.ctl[WL_CUR + b], cnt[b]) : 0;
+  __syncthreads();
+  for (int pk = b0; pk < b1; ++pk) {
+    const int v = A.tc[pk];
+    if (v > 0) A.task[base[v & 255] + (v >> 8)] = make_int2(j, pk);
+  }
+}
+
+__global__ void __launch_bounds__(BT) k_torsion(ToArgs A) {
+  const int task = blockIdx.x * BT + threadIdx.x;
+  if (task >= A.ctl[WL_NTASK]) return;
+  const int2 jp = A.task[task];
+  const int j = jp.x, pk = jp.y;
   const DevParams *__restrict__ P = A.P;
   double e_tor = 0.0, e_con = 0.0;
   double fj[4] = {0.0, 0.0, 0.0, 0.0};
-  // no early return: every lane reaches the group reduction below
-  const swr_atom_pack_t aj = A.atoms[j < n ? j : 0];
-  const int tj = j < n ? aj.type : -1;
-  const int b0 = j < n ? A.bidx[j] : 0, b1 = j < n ? A.bend[j] : 0;
-  if (tj >= 0) {
+  const swr_atom_pack_t aj = A.atoms[j];
+  const int tj = aj.type;
+  const int b0 = A.bidx[j], b1 = A.bend[j];
+  {
     constexpr double MIN_SINE = 1e-10;
     const double thb_cut = P->thb_cut;
     const double p_tor2 = P->p_tor2, p_tor3 = P->p_tor3, p_tor4 = P->p_tor4, p_cot2 = P->p_cot2;
     const double Delta_j = A.bo_dboc[2 * (size_t)j + 1];
-    for (int pk = b0 + t; pk < b1; pk += TO_LANES) do {
+    do {
       const double BO_jk = A.BO[pk];
       if (!(BO_jk - thb_cut > 0.0)) break;
       const swr_bond_t bjk = A.bonds[pk];
@@ -1000,7 +1245,7 @@ __global__ void __launch_bounds__(BT) k_torsion(int n, ToArgs A) {
           cd_k += CEtors3;
           cdbo_ij += CEtors4 + Cconj * (BOA_ij - 1.5e0);
           cdbo_jk += CEtors5 + Cconj * (BOA_jk - 1.5e0);
-          fix_add(A.flags, A.sacc + pw, A.scoff, CEtors6 + Cconj * (BOA_kl - 1.5e0));
+          const double cdbo_kl = CEtors6 + Cconj * (BOA_kl - 1.5e0);
           const double c7 = CEtors7 + CEconj4, c8 = CEtors8 + CEconj5, c9 = CEtors9 + CEconj6;
           double fl[3];
 #pragma unroll
@@ -1015,46 +1260,54 @@ __global__ void __launch_bounds__(BT) k_torsion(int n, ToArgs A) {
             fl[q] = c8 * jkl_dk[q] + c9 * w_l;
           }
           unsigned long long *al4 = A.acc + 4 * (size_t)l;
-          fix_add(A.flags, al4, A.coff, fl[0]);
-          fix_add(A.flags, al4 + 1, A.coff, fl[1]);
-          fix_add(A.flags, al4 + 2, A.coff, fl[2]);
+          unsigned long long *const ga[4] = {A.sacc + pw, al4, al4 + 1, al4 + 2};
+          const size_t gc[4] = {A.scoff, A.coff, A.coff, A.coff};
+          const double gv[4] = {cdbo_kl, fl[0], fl[1], fl[2]};
+          fix_add_group<4>(A.flags, ga, gc, gv);
         }
-        if (cdbo_ij != 0.0) fix_add(A.flags, A.sacc + ph, A.scoff, cdbo_ij);
-        if (fi[0] != 0.0 || fi[1] != 0.0 || fi[2] != 0.0) {
+        if (cdbo_ij != 0.0 || fi[0] != 0.0 || fi[1] != 0.0 || fi[2] != 0.0) {
           unsigned long long *ai4 = A.acc + 4 * (size_t)i;
-          fix_add(A.flags, ai4, A.coff, fi[0]);
-          fix_add(A.flags, ai4 + 1, A.coff, fi[1]);
-          fix_add(A.flags, ai4 + 2, A.coff, fi[2]);
+          unsigned long long *const ga[4] = {A.sacc + ph, ai4, ai4 + 1, ai4 + 2};
+          const size_t gc[4] = {A.scoff, A.coff, A.coff, A.coff};
+          const double gv[4] = {cdbo_ij, fi[0], fi[1], fi[2]};
+          fix_add_group<4>(A.flags, ga, gc, gv);
         }
       }
       if (cdbopi_jk != 0.0) A.Cdbopi[pk] += cdbopi_jk;
       if (cdbo_jk != 0.0) A.Cdbo[pk] += cdbo_jk;
       if (cd_k != 0.0 || fk[0] != 0.0 || fk[1] != 0.0 || fk[2] != 0.0) {
         unsigned long long *ak4 = A.acc + 4 * (size_t)k;
-        fix_add(A.flags, ak4, A.coff, fk[0]);
-        fix_add(A.flags, ak4 + 1, A.coff, fk[1]);
-        fix_add(A.flags, ak4 + 2, A.coff, fk[2]);
-        fix_add(A.flags, ak4 + 3, A.coff, cd_k);
+        unsigned long long *const ga[4] = {ak4, ak4 + 1, ak4 + 2, ak4 + 3};
+        const size_t gc[4] = {A.coff, A.coff, A.coff, A.coff};
+        const double gv[4] = {fk[0], fk[1], fk[2], cd_k};
+        fix_add_group<4>(A.flags, ga, gc, gv);
       }
     } while (false);
   }
-  // the group's lanes combined by a fixed xor tree (the same order on every
-  // run); lane 0 writes j's row and energies
+  double *rw = A.row + 6 * (size_t)pk;
 #pragma unroll
-  for (int m = 1; m < TO_LANES; m <<= 1) {
-    e_tor += __shfl_xor(e_tor, m, TO_LANES);
-    e_con += __shfl_xor(e_con, m, TO_LANES);
+  for (int q = 0; q < 4; q++) rw[q] = fj[q];
+  rw[4] = e_tor;
+  rw[5] = e_con;
+}
+
+// j's row and torsion/conjugation energies: its slots' entries in slot order.
+__global__ void __launch_bounds__(BT) k_to_row(int n, ToArgs A) {
+  const int j = blockIdx.x * BT + threadIdx.x;
+  if (j >= n) return;
+  const int b0 = A.bidx[j], b1 = A.bend[j];
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int pk = b0; pk < b1; ++pk) {
+    const double *rw = A.row + 6 * (size_t)pk;
 #pragma unroll
-    for (int q = 0; q < 4; q++) fj[q] += __shfl_xor(fj[q], m, TO_LANES);
+    for (int q = 0; q < 6; q++) s[q] += rw[q];
   }
-  if (j < n && t == 0) {
-    if (tj >= 0) {
+  if (A.atoms[j].type >= 0) {
 #pragma unroll
-      for (int q = 0; q < 4; q++) A.fcd[4 * (size_t)j + q] += fj[q];
-    }
-    A.part[j] = e_tor;
-    A.part[A.stride + j] = e_con;
+    for (int q = 0; q < 4; q++) A.fcd[4 * (size_t)j + q] += s[q];
   }
+  A.part[j] = s[4];
+  A.part[A.stride + j] = s[5];
 }
 
 // Cdbo of the bond slots += their fixed-point terms; accumulators left at zero.
@@ -1196,12 +1449,18 @@ int swr_valence_angles_run(swr_system *S) {
   int rc;
   const size_t stride = (size_t)std::max(n, 1);
   const size_t old_acc = S->cap_va_acc;
+  const size_t old_ctl = S->cap_va_ctl;
   if ((rc = swr_reserve(&S->va_part, S->cap_va_part, 3 * stride)) ||
       (rc = swr_reserve(&S->va_result, S->cap_va_result, 4)) ||
-      (rc = swr_reserve(&S->va_acc, S->cap_va_acc, 8 * (size_t)N)))
+      (rc = swr_reserve(&S->va_acc, S->cap_va_acc, 8 * (size_t)N)) ||
+      (rc = swr_reserve(&S->va_cnt, S->cap_va_cnt, stride)) ||
+      (rc = swr_reserve(&S->va_order, S->cap_va_order, stride)) ||
+      (rc = swr_reserve(&S->va_ctl, S->cap_va_ctl, (size_t)WL_CTL_INTS)))
     return rc;
   if (S->cap_va_acc != old_acc)
     HIP_CHECK(hipMemsetAsync(S->va_acc, 0, sizeof(unsigned long long) * S->cap_va_acc, S->stream));
+  // the histogram starts at zero; k_wl_scan leaves it so
+  if (S->cap_va_ctl != old_ctl) HIP_CHECK(hipMemsetAsync(S->va_ctl, 0, sizeof(int) * S->cap_va_ctl, S->stream));
   S->va_stride = stride;
   VaArgs A;
   A.P = S->dp;
@@ -1227,8 +1486,16 @@ int swr_valence_angles_run(swr_system *S) {
   A.acc = S->va_acc;
   A.coff = 4 * (size_t)N;
   A.flags = &S->status->flags;
+  A.cnt = S->va_cnt;
+  A.order = S->va_order;
+  A.ctl = S->va_ctl;
   swr_kt_begin(S, KT_VALENCE);
-  if (n > 0) hipLaunchKernelGGL(k_valence, dim3(div_up(n, BT)), dim3(BT), 0, S->stream, n, A);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_va_count, dim3(div_up(n, BT)), dim3(BT), 0, S->stream, n, A);
+    hipLaunchKernelGGL(k_wl_scan, dim3(1), dim3(WL_NB), 0, S->stream, S->va_ctl);
+    hipLaunchKernelGGL(k_va_scatter, dim3(div_up(n, BT)), dim3(BT), 0, S->stream, n, A);
+    hipLaunchKernelGGL(k_valence, dim3(div_up(n, BT)), dim3(BT), 0, S->stream, A);
+  }
   hipLaunchKernelGGL(k_va_fold, dim3(div_up(N, BT)), dim3(BT), 0, S->stream, N, S->va_acc, S->fcd);
   hipLaunchKernelGGL(k_sum_rows, dim3(3), dim3(1024), 0, S->stream, n, (const double *)S->va_part, stride,
                      S->va_result);
@@ -1246,16 +1513,23 @@ int swr_torsion_angles_run(swr_system *S) {
   int rc;
   const size_t stride = (size_t)std::max(n, 1);
   const int nslots = S->bond_cap;
-  const size_t old_acc = S->cap_to_acc, old_sacc = S->cap_to_sacc;
+  const size_t old_acc = S->cap_to_acc, old_sacc = S->cap_to_sacc, old_ctl = S->cap_to_ctl;
+  const size_t slots = (size_t)std::max(nslots, 1);
   if ((rc = swr_reserve(&S->to_part, S->cap_to_part, 2 * stride)) ||
       (rc = swr_reserve(&S->to_result, S->cap_to_result, 4)) ||
       (rc = swr_reserve(&S->to_acc, S->cap_to_acc, 8 * (size_t)N)) ||
-      (rc = swr_reserve(&S->to_sacc, S->cap_to_sacc, 2 * (size_t)std::max(nslots, 1))))
+      (rc = swr_reserve(&S->to_sacc, S->cap_to_sacc, 2 * slots)) ||
+      (rc = swr_reserve(&S->to_tc, S->cap_to_tc, slots)) ||
+      (rc = swr_reserve(&S->to_task, S->cap_to_task, 2 * slots)) ||
+      (rc = swr_reserve(&S->to_row, S->cap_to_row, 6 * slots)) ||
+      (rc = swr_reserve(&S->to_ctl, S->cap_to_ctl, (size_t)WL_CTL_INTS)))
     return rc;
   if (S->cap_to_acc != old_acc)
     HIP_CHECK(hipMemsetAsync(S->to_acc, 0, sizeof(unsigned long long) * S->cap_to_acc, S->stream));
   if (S->cap_to_sacc != old_sacc)
     HIP_CHECK(hipMemsetAsync(S->to_sacc, 0, sizeof(unsigned long long) * S->cap_to_sacc, S->stream));
+  // the histogram starts at zero; k_wl_scan leaves it so
+  if (S->cap_to_ctl != old_ctl) HIP_CHECK(hipMemsetAsync(S->to_ctl, 0, sizeof(int) * S->cap_to_ctl, S->stream));
   S->to_stride = stride;
   ToArgs A;
   A.P = S->dp;
@@ -1275,10 +1549,21 @@ int swr_torsion_angles_run(swr_system *S) {
   A.coff = 4 * (size_t)N;
   A.flags = &S->status->flags;
   A.sacc = S->to_sacc;
-  A.scoff = (size_t)std::max(nslots, 1);
+  A.scoff = slots;
+  A.bnbr = S->bond_nbr;
+  A.tc = S->to_tc;
+  A.task = reinterpret_cast<int2 *>(S->to_task);
+  A.ctl = S->to_ctl;
+  A.row = S->to_row;
   swr_kt_begin(S, KT_TORSION);
-  if (n > 0)
-    hipLaunchKernelGGL(k_torsion, dim3(div_up(n * TO_LANES, BT)), dim3(BT), 0, S->stream, n, A);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_to_count, dim3(div_up(n * TO_CNT_LANES, BT)), dim3(BT), 0, S->stream, n, A);
+    hipLaunchKernelGGL(k_wl_scan, dim3(1), dim3(WL_NB), 0, S->stream, S->to_ctl);
+    hipLaunchKernelGGL(k_to_scatter, dim3(div_up(n, BT)), dim3(BT), 0, S->stream, n, A);
+    // (at most one task per bond slot; lanes past the list's length exit)
+    hipLaunchKernelGGL(k_torsion, dim3(div_up(nslots, BT)), dim3(BT), 0, S->stream, A);
+    hipLaunchKernelGGL(k_to_row, dim3(div_up(n, BT)), dim3(BT), 0, S->stream, n, A);
+  }
   hipLaunchKernelGGL(k_va_fold, dim3(div_up(N, BT)), dim3(BT), 0, S->stream, N, S->to_acc, S->fcd);
   if (nslots > 0)
     hipLaunchKernelGGL(k_to_slot_fold, dim3(div_up(nslots, BT)), dim3(BT), 0, S->stream, nslots, S->to_sacc, S->Cdbo);

@@ -384,8 +384,8 @@ int swr_destroy(swr_system *S) {
                   S->Cdbopi,
                   S->Cdbopi2, S->hbond_index, S->hbond_end, S->hbonds, S->bo_dboc, S->dDeltap_self, S->bond_mark,
                   S->fcd, S->nb_partials, S->nb_result, S->nb_eatom, S->be_part, S->be_ce, S->be_bufc, S->be_result, S->hb_part,
-                  S->hb_result, S->hb_acc, S->va_part, S->va_result, S->va_acc, S->to_part, S->to_result, S->to_acc,
-                  S->to_sacc, S->f, S->gcsr_cnt, S->gcsr_start, S->gcsr_list,
+                  S->hb_result, S->hb_acc, S->va_part, S->va_result, S->va_acc, S->va_cnt, S->va_order, S->va_ctl, S->to_part, S->to_result, S->to_acc,
+                  S->to_sacc, S->to_tc, S->to_task, S->to_ctl, S->to_row, S->f, S->gcsr_cnt, S->gcsr_start, S->gcsr_list,
                   S->bond_mark2, S->counts, S->ws, S->H_firstnbr, S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32,
                   S->Hdia_inv, S->b_s, S->st, S->s_hist, S->t_hist, S->cgv, S->cgs, S->partials, S->counters,
                   S->status, S->scan_tmp, S->tmp_i, S->mixw, S->cand, S->cand_count, S->row_sys, S->tile_beg,

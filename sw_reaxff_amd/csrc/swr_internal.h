@@ -264,11 +264,21 @@ struct swr_system {
   double *va_part = nullptr, *va_result = nullptr;
   unsigned long long *va_acc = nullptr;
   size_t cap_va_part = 0, cap_va_result = 0, cap_va_acc = 0, va_stride = 0;
+  // the valence work list: per local atom its angle count (then its rank in
+  // its count bucket), the centres ordered by count, and the counters
+  int *va_cnt = nullptr, *va_order = nullptr, *va_ctl = nullptr;
+  size_t cap_va_cnt = 0, cap_va_order = 0, cap_va_ctl = 0;
   // torsions: per-atom energies [2][to_stride], totals, fixed-point terms of
   // atoms [N][4] and of bond slots [bond_cap] (zero between calls)
   double *to_part = nullptr, *to_result = nullptr;
   unsigned long long *to_acc = nullptr, *to_sacc = nullptr;
   size_t cap_to_part = 0, cap_to_result = 0, cap_to_acc = 0, cap_to_sacc = 0, to_stride = 0;
+  // the torsion work list: per bond slot its quadruple count (then its rank
+  // in its count bucket), the central bonds (j, slot) ordered by count, per
+  // slot the six row terms of its central bond, and the counters
+  int *to_tc = nullptr, *to_task = nullptr, *to_ctl = nullptr;
+  double *to_row = nullptr;
+  size_t cap_to_tc = 0, cap_to_task = 0, cap_to_ctl = 0, cap_to_row = 0;
   // forces on local atoms (ghost rows of fCdDelta summed onto owners) and the
   // ghost-per-owner CSR that sums them in a fixed order
   double *f = nullptr;                     // [n][3]
