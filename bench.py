@@ -494,7 +494,7 @@ def main():
     ap.add_argument("--config", default="water96k", choices=sorted(CONFIGS) + ["clusters512", "serve"])
     ap.add_argument("--requests", type=int, default=4000, help="serve: requests in the stream (per rate)")
     ap.add_argument("--pool", type=int, default=400, help="serve: distinct systems the requests draw from")
-    ap.add_argument("--rates", type=float, nargs="+", default=[1000, 1500, 1800, 2000, 2200, 2400],
+    ap.add_argument("--rates", type=float, nargs="+", default=[1000, 1500, 2000, 2400, 2600, 2800],
                     help="serve: Poisson arrival rates (1/s) to sweep")
     ap.add_argument("--lanes", type=int, default=4, help="serve: concurrent batch workers")
     ap.add_argument("--reps", type=int, default=2, help="serve: runs of the stream per rate (the slowest counts)")
