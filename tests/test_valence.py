@@ -16,8 +16,10 @@ GPU parity: energies, Cdbo/Cdbopi/Cdbopi2 per bond, CdDelta and the
 gradient after the chain rule at 1e-10 relative (water box, clusters).
 
 Torsions (the torsion part of the same function): finite differences of
-e_tor + e_con (the same four systems), zero net force and torque, and the
-same GPU parity.
+e_tor + e_con (the same four systems), zero net force and torque, the
+dihedral dependence of e_tor and e_con on an isolated chain against the
+published functional form (cosine series, zero at trans, V1 : V3 from the
+raw ffield text), and the same GPU parity.
 """
 import math
 
@@ -275,6 +277,72 @@ def test_torsion_gradient_has_no_net_force_or_torque(oracle_lib, ffield):
     assert np.abs(g.sum(axis=0)).max() <= 1e-11 * scale * s.n
     torque = np.cross(s.x - s.x.mean(axis=0), g).sum(axis=0)
     assert np.abs(torque).max() <= 1e-10 * scale * s.n
+
+
+def _chain(els4, r, theta, omega):
+    """Atoms i-j-k-l, bonds r, both bond angles theta, dihedral omega
+    (0 = cis: i and l on the same side of the j-k bond)."""
+    j = np.zeros(3)
+    k = np.array([r[1], 0.0, 0.0])
+    i = j + r[0] * np.array([-math.cos(math.pi - theta), math.sin(math.pi - theta), 0.0])
+    l = k + r[2] * np.array([math.cos(math.pi - theta), math.sin(math.pi - theta) * math.cos(omega),
+                             math.sin(math.pi - theta) * math.sin(omega)])
+    x = np.array([i, j, k, l]) + 10.0
+    order = ["C", "H", "O", "N"]
+    t = np.array([order.index(e) + 1 for e in els4])
+    return systems.System(x=x, v=np.zeros_like(x), type=t, tag=np.arange(1, 5), elements=order,
+                          lo=np.zeros(3), hi=np.full(3, 30.0), periodic=False)
+
+
+@pytest.mark.parametrize("els4", [("N", "C", "N", "N"), ("O", "C", "O", "O")])
+def test_torsion_energy_dihedral_series(oracle_lib, ffield, els4):
+    """The torsion and conjugation energies' dependence on the dihedral
+    angle, against the published functional form.  Rotating l about the j-k
+    bond of an isolated chain keeps every bond length and bond angle (so
+    every bond order, f10, f11, f12 and sin(theta)); then
+      E_tor(w) = f10 s [V1 (1 + cos w) + V2 e (1 - cos 2w) + V3 (1 + cos 3w)] / 2,
+      E_con(w) = p_cot1 f12 [1 + (cos^2 w - 1) s],   s = sin th_ijk sin th_jkl,
+    with 0 < e = exp(p_tor1 (2 - BOpi_jk - f11)^2) <= 1 (p_tor1 < 0).
+    Fitted over twelve dihedrals: E_tor is a cosine series of degree three
+    with no sine terms, vanishes at trans, its cos w and cos 3w amplitudes
+    stand as V1 : V3 from the raw ffield text, its constant is the sum of
+    those two less the cos 2w one, and its cos 2w amplitude carries -V2 e;
+    E_con is c0 + c2 cos 2w with c0 / c2 = 2 / s - 1.  A dropped term, a
+    wrong multiple of w or a sign slip in any term fails one of these."""
+    ff = FField(ffield)
+    prm = ff.torsion(*els4)
+    assert prm is not None and prm["V1"] != 0 and prm["V2"] != 0 and prm["V3"] != 0 and prm["p_cot1"] != 0
+    # long bonds and wide angles put the 1-3 pairs at 3.55 A, beyond bo_cut
+    theta = math.radians(160.0)
+    r = (1.8, 1.8, 1.8)
+    ws = np.radians(np.arange(0.0, 360.0, 30.0))
+    et, ec = [], []
+    for w in ws:
+        s = _chain(els4, r, theta, w)
+        o = _oracle(oracle_lib, ffield, s)
+        bi, be, nbr = o.get("bond_index"), o.get("bond_end"), o.get("bonds")["nbr"]
+        pairs = {tuple(sorted((a, int(nbr[p])))) for a in range(4) for p in range(bi[a], be[a])}
+        assert pairs == {(0, 1), (1, 2), (2, 3)}, (np.degrees(w), pairs)   # the chain's bonds only
+        e = o.torsion_angles()
+        et.append(e[0])
+        ec.append(e[1])
+    et, ec = np.array(et), np.array(ec)
+    A = np.stack([np.ones_like(ws), np.cos(ws), np.cos(2 * ws), np.cos(3 * ws), np.sin(ws), np.sin(2 * ws),
+                  np.sin(3 * ws)], axis=1)
+    a, *_ = np.linalg.lstsq(A, et, rcond=None)
+    c, *_ = np.linalg.lstsq(A, ec, rcond=None)
+    st, sc = np.abs(et).max(), np.abs(ec).max()
+    print(f"[torsion series] {els4}: E_tor fit {a} E_con fit {c}", flush=True)
+    assert st > 0 and sc > 0
+    assert np.abs(A @ a - et).max() <= 1e-10 * st and np.abs(a[4:]).max() <= 1e-10 * st
+    assert abs(et[6]) <= 1e-12 * st                                  # trans (180 degrees)
+    assert a[1] / a[3] == pytest.approx(prm["V1"] / prm["V3"], rel=1e-9)
+    assert a[0] == pytest.approx(a[1] - a[2] + a[3], rel=1e-9, abs=1e-12 * st)
+    ev = -a[2] / a[1] * prm["V1"] / prm["V2"]                        # e of the V2 term
+    assert 0.0 < ev <= 1.0
+    sth = math.sin(theta) ** 2
+    assert np.abs(A @ c - ec).max() <= 1e-10 * sc and np.abs(c[[1, 3, 4, 5, 6]]).max() <= 1e-10 * sc
+    assert c[0] / c[2] == pytest.approx(2.0 / sth - 1.0, rel=1e-9)
 
 
 def _torsion_fix_floor(bi, be, nbr, N):
