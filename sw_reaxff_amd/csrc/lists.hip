@@ -24,7 +24,6 @@ namespace {
 constexpr int LT = 256;                 // threads per block
 constexpr int LW = LT / SWR_WAVE;       // waves per block
 constexpr int MB = SWR_MAX_ATOM_BONDS;  // LDS bond slots per atom
-constexpr int HW = SWR_WAVE / 2;         // lanes per atom in k_bonds (two atoms per wave)
 
 // Orders this wave's LDS writes before its later LDS reads (each wave only
 // touches its own LDS slice, so no workgroup barrier is needed).
@@ -79,11 +78,6 @@ __global__ void k_reset_apply(int n, int N, const int *__restrict__ hflag, const
 // Per atom, the far-list slots of its bond candidates (d <= bond_cut and
 // inside the pair's BO' radius); the bond pass evaluates them.
 constexpr int MCAND = 64;
-
-struct BondRec {
-  double d, dx, dy, dz, BO;
-  int nbr, ej;
-};
 
 // Streaming 16-byte store: the far list is written once per step and read
 // back only sparsely, so it should not displace the gathered atoms in L2.
@@ -537,12 +531,17 @@ k_far_ws(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *_
   }
 }
 
-// Pass 2: half a wave (32 lanes) per atom, two atoms per wave, evaluates the
-// atom's bond candidates (one per lane), keeps BO' >= bo_cut in candidate
-// (= far-list) order, sorts the bonds by nbr (my_bubble_sort,
-// reaxc_forces_sw64.c:31-75) and writes the bond records, the per-atom sums
-// and BO section 1.  An atom has a handful of candidates, so a full wave per
-// atom left most lanes idle.
+// Pass 2: BL lanes per atom, SWR_WAVE / BL atoms per wave, evaluate the
+// atom's bond candidates (one per lane, BL at a time), keep BO' >= bo_cut in
+// candidate (= far-list) order, sort the bonds by nbr (my_bubble_sort,
+// reaxc_forces_sw64.c:31-75) and write the bond records, the per-atom sums
+// and BO section 1.  An atom has a handful of candidates (about 3 in water,
+// 6 in water/methanol clusters), so wider groups left most lanes idle; a
+// bond's LDS slot keeps only its far slot and neighbour, and the write pass
+// reads the far entry again.  The per-atom sums run in slot order (one
+// lane of the group adds the group's values in order through shuffles).
+constexpr int BL = 8;   // lanes per atom in k_bonds
+
 template <bool COUNT_ONLY>
 __global__ void __launch_bounds__(LT)
 k_bonds(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__restrict__ atoms,
@@ -554,39 +553,47 @@ k_bonds(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__
         double *__restrict__ Cdbopi2, double *__restrict__ bo_dboc, double *__restrict__ dDeltap_self,
         int *__restrict__ bond_mark, double *__restrict__ Deltap, double *__restrict__ Deltap_boc,
         int *__restrict__ bond_count, int bond_cap, DevStatus *st) {
-  __shared__ BondRec srec[2 * LW][MB];
-  __shared__ double sdbop[2 * LW][MB][3];
+  constexpr int APW = SWR_WAVE / BL;   // atoms per wave
+  __shared__ int sfar[APW * LW][MB];   // far slot of the atom's m-th bond (candidate order)
+  __shared__ int snbr[APW * LW][MB];
   const int w = wave_in_block();
   const int lane = lane_id();
-  const int half = lane / HW, sl = lane % HW;
-  const int slot_a = 2 * w + half;  // this atom's LDS row
-  const int i0 = (xcd_block(gridDim.x) * LW + w) * 2;
-  if (i0 >= N) return;              // both atoms of the wave past the end
-  const int i = i0 + half;
+  const int grp = lane / BL, sl = lane % BL;
+  const int slot_a = APW * w + grp;  // this atom's LDS row
+  const int i0 = (xcd_block(gridDim.x) * LW + w) * APW;
+  if (i0 >= N) return;               // every atom of the wave past the end
+  const int i = i0 + grp;
   const bool live = i < N;
   const int ei = live ? atoms[i].type : -1;
   const double bo_cut = P->bo_cut;
   const int ncand = live ? min(cand_count[i], MCAND) : 0;
-  const int ncmax = max(ncand, __shfl_xor(ncand, HW, SWR_WAVE));  // uniform loop bound over the two atoms
+  int ncmax = ncand;                 // uniform loop bound over the wave's atoms
+#pragma unroll
+  for (int o = BL; o < SWR_WAVE; o <<= 1) ncmax = max(ncmax, __shfl_xor(ncmax, o, SWR_WAVE));
   const int *mycand = cand + (size_t)(live ? i : 0) * MCAND;
+  const unsigned long long gmask = (1ull << BL) - 1ull;
   const unsigned long long below = (1ull << sl) - 1ull;
   int nb = 0;
-  for (int c0 = 0; c0 < ncmax; c0 += HW) {
+  for (int c0 = 0; c0 < ncmax; c0 += BL) {
     const int c = c0 + sl;
     bool isb = false;
-    BondRec r;
+    int fs = 0, jn = 0;
     if (c < ncand && ei >= 0) {
-      const swr_far_neighbor_t e = far[mycand[c]];
+      fs = mycand[c];
+      const swr_far_neighbor_t e = far[fs];
       const BOpTerms t = bop_terms(P, ei, e.type, e.d);
       if (t.BO >= bo_cut) {
         isb = true;
-        r.d = e.d; r.dx = e.dvec[0]; r.dy = e.dvec[1]; r.dz = e.dvec[2]; r.BO = t.BO; r.nbr = e.nbr; r.ej = e.type;
+        jn = e.nbr;
       }
     }
-    const unsigned long long bm = (__ballot(isb) >> (HW * half)) & 0xffffffffull;
+    const unsigned long long bm = (__ballot(isb) >> (BL * grp)) & gmask;
     if (!COUNT_ONLY && isb) {
       const int slot = nb + __popcll(bm & below);
-      if (slot < MB) srec[slot_a][slot] = r;
+      if (slot < MB) {
+        sfar[slot_a][slot] = fs;
+        snbr[slot_a][slot] = jn;
+      }
     }
     nb += __popcll(bm);
   }
@@ -601,54 +608,68 @@ k_bonds(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__
     bond_end[i] = start;
   }
   const bool write = live && !over;
+  const int nbw = write ? nb : 0;
+  int nbmax = nbw;
+#pragma unroll
+  for (int o = BL; o < SWR_WAVE; o <<= 1) nbmax = max(nbmax, __shfl_xor(nbmax, o, SWR_WAVE));
   wave_lds_sync();
-  if (write && sl < nb) {
-    const BondRec rc = srec[slot_a][sl];
-    const BOpTerms t = bop_terms(P, ei, rc.ej, rc.d);
-    const DevTbp &tw = P->tbp[ei * SWR_MAX_ELEM + rc.ej];
-    const double r2 = rc.d * rc.d;
-    const double Cs = tw.p_bo2 * t.C12 / r2, Cp = tw.p_bo4 * t.C34 / r2, Cpp = tw.p_bo6 * t.C56 / r2;
-    const double dv[3] = {rc.dx, rc.dy, rc.dz};
-    const double cdb = -(t.BO_s * Cs + t.pi * Cp + t.pi2 * Cpp);
-    swr_bond_order_t o;
-    o.BO_s = t.BO_s - bo_cut;
-    for (int c = 0; c < 3; c++) {
-      o.dln_BOp_s[c] = -t.BO_s * Cs * dv[c];
-      o.dln_BOp_pi[c] = -t.pi * Cp * dv[c];
-      o.dln_BOp_pi2[c] = -t.pi2 * Cpp * dv[c];
-      o.dBOp[c] = cdb * dv[c];
-      sdbop[slot_a][sl][c] = o.dBOp[c];
+  // bo_dboc[i][0] and dDeltap_self[i] accumulate in candidate order
+  // (reaxc_forces_sw64.c:1253,1262), starting from Reset's zeros.
+  double tot = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int m0 = 0; m0 < nbmax; m0 += BL) {
+    const int m = m0 + sl;
+    double bm_ = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    if (m < nbw) {
+      const swr_far_neighbor_t e = far[sfar[slot_a][m]];
+      const int jn = snbr[slot_a][m];
+      const BOpTerms t = bop_terms(P, ei, e.type, e.d);
+      const DevTbp &tw = P->tbp[ei * SWR_MAX_ELEM + e.type];
+      const double r2 = e.d * e.d;
+      const double Cs = tw.p_bo2 * t.C12 / r2, Cp = tw.p_bo4 * t.C34 / r2, Cpp = tw.p_bo6 * t.C56 / r2;
+      const double dv[3] = {e.dvec[0], e.dvec[1], e.dvec[2]};
+      const double cdb = -(t.BO_s * Cs + t.pi * Cp + t.pi2 * Cpp);
+      swr_bond_order_t o;
+      o.BO_s = t.BO_s - bo_cut;
+      for (int c = 0; c < 3; c++) {
+        o.dln_BOp_s[c] = -t.BO_s * Cs * dv[c];
+        o.dln_BOp_pi[c] = -t.pi * Cp * dv[c];
+        o.dln_BOp_pi2[c] = -t.pi2 * Cpp * dv[c];
+        o.dBOp[c] = cdb * dv[c];
+      }
+      o.C1dbo = o.C2dbo = o.C3dbo = 0.0;
+      o.C1dbopi = o.C2dbopi = o.C3dbopi = o.C4dbopi = 0.0;
+      o.C1dbopi2 = o.C2dbopi2 = o.C3dbopi2 = o.C4dbopi2 = 0.0;
+      o.CdboReduction = nullptr;
+      int rank = 0;
+      for (int q = 0; q < nbw; q++) rank += (snbr[slot_a][q] < jn) ? 1 : 0;
+      const int p = start + rank;
+      swr_bond_t b;
+      b.d = e.d; b.dvec[0] = dv[0]; b.dvec[1] = dv[1]; b.dvec[2] = dv[2];
+      b.nbr = jn;
+      b.dbond_index = start + m;
+      b.rel_box[0] = b.rel_box[1] = b.rel_box[2] = 0;
+      b.sym_index = 0;
+      bonds[p] = b;
+      bond_nbr[p] = jn;
+      bo_data[p] = o;
+      BO[p] = t.BO - bo_cut;
+      BOpi[2 * (size_t)p] = t.pi;
+      BOpi[2 * (size_t)p + 1] = t.pi2;
+      Cdbo[p] = 0.0; Cdbopi[p] = 0.0; Cdbopi2[p] = 0.0;
+      bm_ = t.BO - bo_cut;
+      g0 = o.dBOp[0]; g1 = o.dBOp[1]; g2 = o.dBOp[2];
     }
-    o.C1dbo = o.C2dbo = o.C3dbo = 0.0;
-    o.C1dbopi = o.C2dbopi = o.C3dbopi = o.C4dbopi = 0.0;
-    o.C1dbopi2 = o.C2dbopi2 = o.C3dbopi2 = o.C4dbopi2 = 0.0;
-    o.CdboReduction = nullptr;
-    int rank = 0;
-    for (int m = 0; m < nb; m++) rank += (srec[slot_a][m].nbr < rc.nbr) ? 1 : 0;
-    const int p = start + rank;
-    swr_bond_t b;
-    b.d = rc.d; b.dvec[0] = dv[0]; b.dvec[1] = dv[1]; b.dvec[2] = dv[2];
-    b.nbr = rc.nbr;
-    b.dbond_index = start + sl;
-    b.rel_box[0] = b.rel_box[1] = b.rel_box[2] = 0;
-    b.sym_index = 0;
-    bonds[p] = b;
-    bond_nbr[p] = rc.nbr;
-    bo_data[p] = o;
-    BO[p] = t.BO - bo_cut;
-    BOpi[2 * (size_t)p] = t.pi;
-    BOpi[2 * (size_t)p + 1] = t.pi2;
-    Cdbo[p] = 0.0; Cdbopi[p] = 0.0; Cdbopi2[p] = 0.0;
+    // the group's slots m0 .. m0 + BL - 1 in order, onto its first lane
+    for (int k = 0; k < BL; k++) {
+      const int src = grp * BL + k;
+      const double vb = __shfl(bm_, src, SWR_WAVE), v0 = __shfl(g0, src, SWR_WAVE);
+      const double v1 = __shfl(g1, src, SWR_WAVE), v2 = __shfl(g2, src, SWR_WAVE);
+      if (m0 + k < nbw) {
+        tot += vb; s0 += v0; s1 += v1; s2 += v2;
+      }
+    }
   }
-  wave_lds_sync();
   if (write && sl == 0) {
-    // bo_dboc[i][0] and dDeltap_self[i] accumulate in candidate order
-    // (reaxc_forces_sw64.c:1253,1262), starting from Reset's zeros.
-    double tot = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int m = 0; m < nb; m++) {
-      tot += srec[slot_a][m].BO - bo_cut;
-      s0 += sdbop[slot_a][m][0]; s1 += sdbop[slot_a][m][1]; s2 += sdbop[slot_a][m][2];
-    }
     bond_end[i] = start + nb;
     bo_dboc[2 * (size_t)i] = tot;
     bo_dboc[2 * (size_t)i + 1] = 0.0;
@@ -823,7 +844,8 @@ int swr_lists_run(swr_system *S, int mode) {
                      S->far_index, S->far_end, S->cand, S->cand_count, S->Hindex, S->num_hbonds, S->hbond_index,  \
                      S->hbond_end, S->hbonds, hb_count, S->hbond_cap, S->status)
 #define BOND_ARGS(k)                                                                                              \
-  hipLaunchKernelGGL(k, dim3(div_up(N, 2 * LW)), dim3(LT), 0, S->stream, S->n, N, S->dp, S->packed, S->far, S->cand,          \
+  hipLaunchKernelGGL(k, dim3(div_up(N, (SWR_WAVE / BL) * LW)), dim3(LT), 0, S->stream, S->n, N, S->dp, S->packed,     \
+                     S->far, S->cand,                                                                              \
                      S->cand_count, S->num_bonds, S->bond_index, S->bond_end, S->bonds, S->bond_nbr, S->bo_data,    \
                      S->BO, S->BOpi,                                                                               \
                      S->Cdbo, S->Cdbopi, S->Cdbopi2, S->bo_dboc, S->dDeltap_self, S->bond_mark,                    \
