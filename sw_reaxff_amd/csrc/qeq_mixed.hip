@@ -46,6 +46,11 @@ constexpr int MAX_OUTER = 20;
 constexpr double OUTER_TOL_FACTOR = 0.001;  // outer target = tol * this
 constexpr double INNER_TOL_MIN = 3e-4;
 constexpr int INNER_IMAX = 50;
+// A pass after one whose residual norms were all at most this (relative to
+// |b|) forms its residual incrementally, r -= eta d + H32 d, from the FP32
+// values and the last inner correction d: its rounding, ~1e-6 |H d| with
+// |H d| ~ |r| <= INC_RHO |b|, stays two orders below the outer target.
+constexpr double INC_RHO = 1e-4;
 // Threads of the per-system scalar kernels in a batch: most systems of a
 // batch are small, and a 256-thread workgroup finds room on a busy CU far
 // sooner than a 1024-thread one (one system alone uses RT).
@@ -58,6 +63,7 @@ struct MixState {
   int converged[2];
   int matvecs[2];      // FP64 residual SpMVs + FP32 inner iterations, per rhs
   double rho[2];       // sqrt(u.r)/|b| of the latest residual
+  int inc;             // the next residual may be formed incrementally
 };
 
 // Inner FP32 PCG state of one right-hand side of one system.
@@ -85,38 +91,61 @@ inline size_t vec_off(int nsys) { return (pcg_off(nsys) + sizeof(PcgState) * 4 *
 __device__ __forceinline__ bool pcg_live(const PcgState *__restrict__ c) { return (c[0].active | c[1].active) != 0; }
 
 // r = b - (eta x + H x) in FP64 (FP64 values); u = M^-1 r; the dots
-// u.r (s, t) and r.r (s, t), b.b (s, t); r is also written as the FP32
-// right-hand side of the inner solve.  One workgroup per SpMV panel (its
-// partial dots in slot p).  Panels of systems whose refinement has finished
-// are skipped (not on the first pass, whose state is the previous solve's).
+// u.r (s, t) and r.r (s, t), b.b (s, t); r is kept in FP64 (r64) and also
+// written as the FP32 right-hand side of the inner solve.  For a system the
+// previous pass flagged (ms.inc: its residual was already small), r is
+// formed incrementally instead, r64 -= eta d + H d on the FP32 values and
+// the last inner correction d (whose x += d has run), FP64 from there on.
+// One workgroup per SpMV panel (its partial dots in slot p).  Panels of
+// systems whose refinement has finished are skipped (not on the first pass,
+// whose state is the previous solve's).
 template <bool BATCH, int SH>
 __global__ void __launch_bounds__(SpmvShape<SH>::NT)
 k_resid(int n, const DevParams *__restrict__ P, const int *__restrict__ type, const int *__restrict__ firstnbr,
         const int *__restrict__ numnbrs, const int *__restrict__ jlist, const double *__restrict__ val,
-        const double2 *__restrict__ x, const double2 *__restrict__ b, const double *__restrict__ Hd,
+        const float *__restrict__ val32, const double2 *__restrict__ x, const float2 *__restrict__ dinc,
+        const double2 *__restrict__ b, const double *__restrict__ Hd, double2 *__restrict__ r64,
         float2 *__restrict__ r32, double *__restrict__ partials, int pstride, PanelArgs A,
         const int *__restrict__ row_sys, const MixState *__restrict__ ms, int first) {
   constexpr int NT = SpmvShape<SH>::NT, G = SpmvShape<SH>::G, U = SpmvShape<SH>::U;
   __shared__ double2 xl[SWR_PANEL_MAXC];
   const int p = blockIdx.x;
-  if (BATCH && !first) {
+  int sys = 0;
+  if (BATCH) {
     const int r0 = A.pbeg[p];
-    if (r0 < A.pbeg[p + 1] && ms[row_sys[A.prow[r0]]].done) return;
+    sys = r0 < A.pbeg[p + 1] ? row_sys[A.prow[r0]] : 0;
+    if (!first && ms[sys].done) return;
   }
   double d[6] = {0, 0, 0, 0, 0, 0};
-  panel_rows<double, double, NT, G, U>(p, A, firstnbr, numnbrs, jlist, val, [&](int j) { return x[j]; }, xl,
-                                       [&](int i, bool live, double2 q) {
-    if (lane_id() % G == 0 && live) {
-      const double eta = P->eta[type[i]];
-      const double2 xi = x[i], bi = b[i];
-      const double rs = bi.x - (eta * xi.x + q.x), rt = bi.y - (eta * xi.y + q.y);
-      const double h = Hd[i];
-      d[0] += h * rs * rs; d[1] += h * rt * rt;
-      d[2] += rs * rs; d[3] += rt * rt;
-      d[4] += bi.x * bi.x; d[5] += bi.y * bi.y;
-      r32[i] = make_float2((float)rs, (float)rt);
-    }
-  });
+  auto dots = [&](int i, double rs, double rt) {
+    const double2 bi = b[i];
+    const double h = Hd[i];
+    d[0] += h * rs * rs; d[1] += h * rt * rt;
+    d[2] += rs * rs; d[3] += rt * rt;
+    d[4] += bi.x * bi.x; d[5] += bi.y * bi.y;
+    r64[i] = make_double2(rs, rt);
+    r32[i] = make_float2((float)rs, (float)rt);
+  };
+  if (!first && ms[sys].inc) {
+    panel_rows<float, float, NT, G, U>(p, A, firstnbr, numnbrs, jlist, val32, [&](int j) { return dinc[j]; },
+                                       reinterpret_cast<float2 *>(xl), [&](int i, bool live, float2 q) {
+      if (lane_id() % G == 0 && live) {
+        const float eta = (float)P->eta[type[i]];
+        const float2 di = dinc[i];
+        const double2 r = r64[i];
+        dots(i, r.x - (double)(eta * di.x + q.x), r.y - (double)(eta * di.y + q.y));
+      }
+    });
+  } else {
+    panel_rows<double, double, NT, G, U>(p, A, firstnbr, numnbrs, jlist, val, [&](int j) { return x[j]; }, xl,
+                                         [&](int i, bool live, double2 q) {
+      if (lane_id() % G == 0 && live) {
+        const double eta = P->eta[type[i]];
+        const double2 xi = x[i], bi = b[i];
+        dots(i, bi.x - (eta * xi.x + q.x), bi.y - (eta * xi.y + q.y));
+      }
+    });
+  }
   block_partials<6, NT>(d, partials, pstride, p);
 }
 
@@ -167,6 +196,9 @@ k_resid_scalars(const double *__restrict__ partials, int pstride, const int *__r
     const double tin = 0.9 * tol * bn / fmax(rn, 1e-300);
     inner[s].tol = fmin(0.5, fmax(INNER_TOL_MIN, tin));
   }
+  m.inc = 1;
+  for (int s = 0; s < 2; s++)
+    if (!m.converged[s] && !(m.rho[s] <= INC_RHO)) m.inc = 0;
   m.done = (m.converged[0] && m.converged[1]) ? 1 : 0;
   if (!m.done && m.outer >= MAX_OUTER) {
     m.done = 1;
@@ -410,8 +442,13 @@ k_pcg_finish(int nsys, const PcgState *__restrict__ pst, CgScalars *__restrict__
 // Inner vectors (float2), carved from the workspace after the states.
 enum { MV_D = 0, MV_R, MV_Z, MV_P, MV_Q, MV_B, MV_COUNT };
 
+// The FP64 residual (double2 per row) after the float2 vectors.
+inline size_t r64_off(int nsys, size_t ncap) {
+  return (vec_off(nsys) + sizeof(float2) * MV_COUNT * ncap + 255) & ~(size_t)255;
+}
+
 int ensure_mixw(swr_system *S) {
-  const size_t need = vec_off(S->nsys) + sizeof(float2) * MV_COUNT * (size_t)S->ncap;
+  const size_t need = r64_off(S->nsys, (size_t)S->ncap) + sizeof(double2) * (size_t)S->ncap;
   if (S->mixw && S->mixw_bytes >= need) return SWR_OK;
   if (S->mixw) (void)hipFree(S->mixw);
   S->mixw = nullptr;
@@ -448,6 +485,7 @@ int swr_qeq_mixed_solve(swr_system *S) {
   const size_t c = S->ncap;
   float2 *dv = base + MV_D * c, *rv = base + MV_R * c, *zv = base + MV_Z * c, *pv = base + MV_P * c,
          *qv = base + MV_Q * c;
+  double2 *r64 = (double2 *)(w + r64_off(nsys, c));
   float2 *rhs = base + MV_B * c;      // the inner solve's b: the FP32 residual
   double *pq = S->partials + 2 * (size_t)ps;  // p.q partials (rows 2-3; r.z uses rows 0-1)
   CgScalars *inner = S->cgs + 2 * nsys;  // after the FP64 solve's s, t: the inner tolerances
@@ -465,11 +503,13 @@ int swr_qeq_mixed_solve(swr_system *S) {
       const dim3 tb(SpmvShape<SH>::NT);
       if (batch)
         hipLaunchKernelGGL((k_resid<true, SH>), dim3(nblk), tb, 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
-                           S->H_numnbrs, S->H_jlist, S->H_val, (const double2 *)x, b, S->Hdia_inv, rhs, S->partials,
+                           S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32, (const double2 *)x, (const float2 *)dv, b,
+                           S->Hdia_inv, r64, rhs, S->partials,
                            ps, panel_args(S), S->row_sys, ms, outer == 0 ? 1 : 0);
       else
         hipLaunchKernelGGL((k_resid<false, SH>), dim3(nblk), tb, 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
-                           S->H_numnbrs, S->H_jlist, S->H_val, (const double2 *)x, b, S->Hdia_inv, rhs, S->partials,
+                           S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32, (const double2 *)x, (const float2 *)dv, b,
+                           S->Hdia_inv, r64, rhs, S->partials,
                            ps, panel_args(S), S->row_sys, ms, outer == 0 ? 1 : 0);
     });
     if (batch)
@@ -556,7 +596,8 @@ int swr_qeq_time_mixed(swr_system *S, int which, int sh, int reps, float *ms) {
   PcgState *pst = (PcgState *)(w + pcg_off(nsys));
   float2 *base = (float2 *)(w + vec_off(nsys));
   const size_t c = S->ncap;
-  float2 *pv = base + MV_P * c, *qv = base + MV_Q * c, *rhs = base + MV_B * c;
+  float2 *pv = base + MV_P * c, *qv = base + MV_Q * c, *rhs = base + MV_B * c, *dv = base + MV_D * c;
+  double2 *r64 = (double2 *)(w + r64_off(nsys, c));
   double *pq = S->partials + 2 * (size_t)ps;
   const double2 *x = (const double2 *)S->st, *b = (const double2 *)S->b_s;
   hipEvent_t e[2];
@@ -570,11 +611,13 @@ int swr_qeq_time_mixed(swr_system *S, int which, int sh, int reps, float *ms) {
       if (which == 0) {
         if (batch)
           hipLaunchKernelGGL((k_resid<true, SH>), dim3(nblk), tb, 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
-                             S->H_numnbrs, S->H_jlist, S->H_val, x, b, S->Hdia_inv, rhs, S->partials, ps, panel_args(S),
+                             S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32, x, (const float2 *)dv, b, S->Hdia_inv, r64,
+                             rhs, S->partials, ps, panel_args(S),
                              S->row_sys, mst, 1);
         else
           hipLaunchKernelGGL((k_resid<false, SH>), dim3(nblk), tb, 0, S->stream, n, S->dp, S->type, S->H_firstnbr,
-                             S->H_numnbrs, S->H_jlist, S->H_val, x, b, S->Hdia_inv, rhs, S->partials, ps, panel_args(S),
+                             S->H_numnbrs, S->H_jlist, S->H_val, S->H_val32, x, (const float2 *)dv, b, S->Hdia_inv, r64,
+                             rhs, S->partials, ps, panel_args(S),
                              S->row_sys, mst, 1);
       } else {
         if (batch)
