@@ -355,10 +355,12 @@ int swr_batch_pack(int nsys, const int *nat, const double *x, const int *type, c
 /* ---- autotuner of launch shapes (no reference counterpart: the Sunway
  * kernels have fixed ISTEP/SNSTEP tiles).  Families: "spmv_fp64" (the FP64
  * CG's SpMV), "spmv_fp32" (the mixed-precision inner SpMV), "spmv_resid"
- * (its FP64 residual) with shapes G8U4 G16U2 G16U4 G32U2 (lanes per H row,
- * groups of 4 entries in flight per lane), and "far" (the far pass) with
- * G4W2 G4W4 G6W2 G8W4 (gatherer and writer waves per workgroup).
- * swr_autotune times every shape the system's current state allows
+ * (its FP64 residual) with shapes T512G8U4 T512G8U2 T256G8U4 T256G4U4
+ * (threads per workgroup, lanes per H row, entries in flight per lane), and
+ * "far" (the far pass) with G4W2 G4W4 G6W2 G8W4 (gatherer and writer waves
+ * per workgroup).
+ * "panels" (on, off: the SpMV's LDS column lists) is set with swr_set_shape
+ * only.  swr_autotune times every shape the system's current state allows
  * (reps launches each, HIP events) and keeps the fastest; run it between
  * steps.  A shape changes the order of sums inside an H row, never a list.
  * With the environment variable SWR_TUNE_CACHE naming a file, the choices

@@ -256,15 +256,6 @@ k_far(int n, int N, const DevParams *__restrict__ P, const swr_atom_pack_t *__re
   }
 }
 
-// Orders LDS accesses around a workgroup barrier without waiting for the
-// wave's outstanding global stores (a full workgroup fence would add
-// s_waitcnt vmcnt(0), and on this architecture vmcnt counts stores too).
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // Pass 1 for a Verlet-list source, split between two kinds of waves.
 //
 // A wave's vector-memory counter (vmcnt) counts loads and stores together

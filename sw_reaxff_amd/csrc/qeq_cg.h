@@ -158,29 +158,59 @@ template <> struct Quad<float, float> {
 // per right-hand side, then a G-lane butterfly), so a row's sum depends
 // only on the row and (G, U).  Returns the sums on every lane of the row's
 // group.
-template <typename T, typename VT, int G, int U, typename I4, class XV>
+// A lane's first U groups of a row (see row_sum4), requested ahead.
+template <typename T, typename VT, int U, typename I4>
+struct RowBatch {
+  I4 jj[U];
+  Quad<T, VT> vv[U];
+};
+
+// Requests the first U groups of this lane's share of a row when the row
+// has that many (the first trip of row_sum4's main loop).
+template <typename T, typename VT, int G, int U, typename I4>
+__device__ __forceinline__ void row_batch0(int cnt, const I4 *__restrict__ J, const VT *__restrict__ V,
+                                           RowBatch<T, VT, U, I4> &b) {
+  const int sl = lane_id() % G;
+  if (sl + (U - 1) * G < ((cnt + 3) >> 2)) {
+#pragma unroll
+    for (int u = 0; u < U; u++) b.jj[u] = J[sl + u * G];
+#pragma unroll
+    for (int u = 0; u < U; u++) b.vv[u].load(V + 4 * (sl + u * G));
+  }
+}
+
+// PRE: the main loop's first trip uses the groups row_batch0 requested into
+// pre (the same entries, summed in the same order).
+template <typename T, typename VT, int G, int U, bool PRE = false, typename I4, class XV>
 __device__ __forceinline__ typename Vec2<T>::type row_sum4(int cnt, const I4 *__restrict__ J,
-                                                           const VT *__restrict__ V, XV xv) {
+                                                           const VT *__restrict__ V, XV xv,
+                                                           const RowBatch<T, VT, U, I4> *pre = nullptr) {
   typedef typename Vec2<T>::type T2;
   const int sl = lane_id() % G;
   const int n4 = (cnt + 3) >> 2;
   T a = 0, b = 0, c = 0, d = 0;
-  int g = sl;
-  for (; g + (U - 1) * G < n4; g += U * G) {
-    I4 jj[U];
-    Quad<T, VT> vv[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) jj[u] = J[g + u * G];
-#pragma unroll
-    for (int u = 0; u < U; u++) vv[u].load(V + 4 * (g + u * G));
+  auto batch = [&](const RowBatch<T, VT, U, I4> &r) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const T2 x0 = xv(jj[u].x), x1 = xv(jj[u].y), x2 = xv(jj[u].z), x3 = xv(jj[u].w);
-      a += vv[u].v[0] * x0.x; b += vv[u].v[0] * x0.y;
-      c += vv[u].v[1] * x1.x; d += vv[u].v[1] * x1.y;
-      a += vv[u].v[2] * x2.x; b += vv[u].v[2] * x2.y;
-      c += vv[u].v[3] * x3.x; d += vv[u].v[3] * x3.y;
+      const T2 x0 = xv(r.jj[u].x), x1 = xv(r.jj[u].y), x2 = xv(r.jj[u].z), x3 = xv(r.jj[u].w);
+      a += r.vv[u].v[0] * x0.x; b += r.vv[u].v[0] * x0.y;
+      c += r.vv[u].v[1] * x1.x; d += r.vv[u].v[1] * x1.y;
+      a += r.vv[u].v[2] * x2.x; b += r.vv[u].v[2] * x2.y;
+      c += r.vv[u].v[3] * x3.x; d += r.vv[u].v[3] * x3.y;
     }
+  };
+  int g = sl;
+  if (PRE && g + (U - 1) * G < n4) {
+    batch(*pre);
+    g += U * G;
+  }
+  for (; g + (U - 1) * G < n4; g += U * G) {
+    RowBatch<T, VT, U, I4> r;
+#pragma unroll
+    for (int u = 0; u < U; u++) r.jj[u] = J[g + u * G];
+#pragma unroll
+    for (int u = 0; u < U; u++) r.vv[u].load(V + 4 * (g + u * G));
+    batch(r);
   }
   for (; g < n4; g += G) {
     const I4 jj = J[g];
@@ -219,6 +249,14 @@ inline PanelArgs panel_args(const swr_system *S) { return PanelArgs{S->prow, S->
 // NT/G rows at a time, and call row(i, live, q) on every lane of the row's
 // group (live: a row of the panel; q: the row's sums).  Panels without a
 // list gather through src.
+//
+// With a list, the fill's first loads (FU list entries per thread at once),
+// the first row pass's set-up and its first U groups of H values are all
+// requested before the list is written, in that order (vmcnt retires in
+// issue order, so writing those entries waits only for the fill's loads),
+// and the barrier orders LDS only: the values stream from HBM while the
+// list fills.  FU = 2 keeps the default shape within 128 registers (four
+// waves per SIMD).
 template <typename T, typename VT, int NT, int G, int U, class Src, class Row>
 __device__ __forceinline__ void panel_rows(int p, const PanelArgs &A, const int *__restrict__ firstnbr,
                                            const int *__restrict__ numnbrs, const int *__restrict__ jlist,
@@ -227,13 +265,38 @@ __device__ __forceinline__ void panel_rows(int p, const PanelArgs &A, const int 
   typedef typename Vec2<T>::type T2;
   const int r0 = A.pbeg[p], r1 = A.pbeg[p + 1];
   const int m = A.pcnt[p];
-  if (m >= 0) {
-    const int *pc = A.pcol + (size_t)p * SWR_PANEL_MAXC;
-    for (int k = threadIdx.x; k < m; k += NT) xl[k] = src(pc[k]);
-    __syncthreads();
-  }
   const int grp = threadIdx.x / G;
-  for (int rb = r0; rb < r1; rb += NT / G) {
+  int rb = r0;
+  if (m >= 0) {
+    constexpr int FU = 2;
+    const int *pc = A.pcol + (size_t)p * SWR_PANEL_MAXC;
+    int kc[FU];
+#pragma unroll
+    for (int u = 0; u < FU; u++) {
+      const int k = threadIdx.x + u * NT;
+      kc[u] = k < m ? pc[k] : 0;
+    }
+    const int rr = rb + grp;
+    const bool live = rr < r1;
+    const int i = live ? A.prow[rr] : 0;
+    T2 xk[FU];
+#pragma unroll
+    for (int u = 0; u < FU; u++)
+      if (threadIdx.x + u * NT < m) xk[u] = src(kc[u]);
+    const int beg = live ? firstnbr[i] : 0, cnt = live ? numnbrs[i] : 0;
+    const ushort4 *J = reinterpret_cast<const ushort4 *>(A.jloc + beg);
+    RowBatch<T, VT, U, ushort4> b0;
+    row_batch0<T, VT, G, U>(cnt, J, val + beg, b0);
+#pragma unroll
+    for (int u = 0; u < FU; u++)
+      if (threadIdx.x + u * NT < m) xl[threadIdx.x + u * NT] = xk[u];
+    for (int k = threadIdx.x + FU * NT; k < m; k += NT) xl[k] = src(pc[k]);
+    lds_barrier();
+    const T2 q = row_sum4<T, VT, G, U, true>(cnt, J, val + beg, [&](int k) { return xl[k]; }, &b0);
+    row(i, live, q);
+    rb += NT / G;
+  }
+  for (; rb < r1; rb += NT / G) {
     const int rr = rb + grp;
     const bool live = rr < r1;
     const int i = live ? A.prow[rr] : 0;

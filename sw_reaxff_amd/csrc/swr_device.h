@@ -9,6 +9,15 @@
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (SWR_WAVE - 1); }
 __device__ __forceinline__ int wave_in_block() { return threadIdx.x / SWR_WAVE; }
 
+// Workgroup barrier that orders LDS accesses only: it does not wait for the
+// wave's outstanding global loads and stores (a full workgroup fence, as in
+// __syncthreads, adds s_waitcnt vmcnt(0), and vmcnt counts both).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Number of set bits of a wave ballot below this lane.
 __device__ __forceinline__ int ballot_rank(unsigned long long m) {
   const int l = lane_id();
