@@ -11,7 +11,10 @@ hbonds, list validation and the corrected bond orders.
 Prints ONE JSON line: ms/step (lower is better), atom-steps/s, the roofline
 of the dominant kernel (HIP events on the library's stream) and the CPU
 oracle timed on this machine's host cores on a bounded sample of the same
-workload.
+workload.  The default water96k run adds short legs to the same line: the
+mixed-precision QEq on the same box (its solve time against the FP64 solve
+and its charge and force errors against FP64 at 1e-10), the batched
+clusters512 evaluation and a short serving run.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--config water96k|water1944|water768k]
                     [--mixed] [--cpu-budget-s 20] [--no-cpu]
@@ -332,6 +335,8 @@ def qeq_leg(s, args, mixed, tol, steps):
     lm.setup()
     lm.run(args.warmup)
     h = lm.system
+    if args.autotune:
+        h.autotune(reps=3)
     h.ktimer_enable(True)
     for slot in SLOTS:
         h.ktimer_read(slot, reset=True)
@@ -348,7 +353,7 @@ def qeq_leg(s, args, mixed, tol, steps):
             "qeq_ms_per_step": round(qeq, 4), "iters_per_step": [round(its[0], 2), round(its[1], 2)]}
 
 
-def mixed_errors(s, args):
+def mixed_errors(s, args, steps=None):
     """The mixed-precision run against an FP64 run at a tight tolerance
     (1e-10), stepped side by side from the same atoms: the zero-force NVE
     moves both identically, so every step compares the same positions.  Per
@@ -370,7 +375,8 @@ def mixed_errors(s, args):
         runs.append(lm)
     n = s.n
     dq, df, dq64, df64, dqm, dfm = [], [], [], [], [], []
-    for step in range(args.error_steps):
+    steps = args.error_steps if steps is None else steps
+    for step in range(steps):
         out = []
         for lm in runs:
             lm.run(1)
@@ -393,13 +399,32 @@ def mixed_errors(s, args):
     if dqm:
         matched = {"tol": args.tol * MIXED_OUTER_FACTOR, "max_abs_dq_e": max(dqm),
                    "median_abs_dq_e": float(np.median(dqm)), "max_rel_dF": max(dfm)}
-    return {"steps": args.error_steps, "reference": "FP64 QEq at tol 1e-10, same trajectory",
+    return {"steps": steps, "reference": "FP64 QEq at tol 1e-10, same trajectory",
             "max_abs_dq_e": max(dq), "median_abs_dq_e": float(np.median(dq)), "max_rel_dF": max(df),
             "fp64_at_same_tol": {"max_abs_dq_e": max(dq64), "median_abs_dq_e": float(np.median(dq64)),
                                  "max_rel_dF": max(df64)},
             "fp64_at_mixed_outer_target": matched,
             "per_step_max_abs_dq_e": [float(f"{v:.3e}") for v in dq],
             "per_step_max_rel_dF": [float(f"{v:.3e}") for v in df]}
+
+
+def mixed_compare(s, args, qeq_mixed, ksteps, error_steps):
+    """The mixed-precision QEq against the FP64 one on box s: its charge and
+    force errors against FP64 at 1e-10 over error_steps side-by-side steps,
+    and its solve time (qeq_mixed ms/step) against the FP64 solve at the
+    same tolerance and at the mixed solve's own outer target (the FP64
+    tolerance that reaches the same accuracy), each timed in a run of its
+    own."""
+    errors = mixed_errors(s, args, error_steps) if error_steps > 0 else None
+    if errors:
+        log(f"mixed vs FP64: {errors}")
+    legs = [qeq_leg(s, args, False, args.tol, ksteps),
+            qeq_leg(s, args, False, args.tol * MIXED_OUTER_FACTOR, ksteps)]
+    cmp = {"mixed_qeq_ms_per_step": round(qeq_mixed, 4), "fp64": legs,
+           "speedup_vs_fp64_same_tol": round(legs[0]["qeq_ms_per_step"] / qeq_mixed, 3),
+           "speedup_vs_fp64_at_outer_target": round(legs[1]["qeq_ms_per_step"] / qeq_mixed, 3)}
+    log(f"QEq mixed vs FP64: {cmp}")
+    return errors, cmp
 
 
 def bench_serve(args, rates=None, requests=None, reps=None, pool=None):
@@ -509,6 +534,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6, help="QEq tolerance (the reference's: 1e-6)")
     ap.add_argument("--error-steps", type=int, default=200,
                     help="mixed: steps of the side-by-side comparison with an FP64 run (0: skip)")
+    ap.add_argument("--leg-error-steps", type=int, default=20,
+                    help="water96k default run: side-by-side steps of its mixed-precision leg's error check")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="launch with the default shapes instead of the autotuner's")
     ap.add_argument("--no-legs", dest="legs", action="store_false",
@@ -686,23 +713,18 @@ def main():
     if args.mixed:
         qeq_mixed = sum(kt[k][0] for k in QEQ_SLOTS[True] if k in kt) / ksteps
         lmp.close()   # the comparisons hold several systems of this size at once
-        if args.error_steps > 0:
-            errors = mixed_errors(s, args)
-            log(f"mixed vs FP64: {errors}")
-        # the QEq solve against FP64 at the reference tolerance and at the
-        # mixed solve's own outer target (the FP64 tolerance that reaches
-        # the same accuracy), each timed in a run of its own
-        legs = [qeq_leg(s, args, False, args.tol, ksteps),
-                qeq_leg(s, args, False, args.tol * MIXED_OUTER_FACTOR, ksteps)]
-        qeq_cmp = {"mixed_qeq_ms_per_step": round(qeq_mixed, 4), "fp64": legs,
-                   "speedup_vs_fp64_same_tol": round(legs[0]["qeq_ms_per_step"] / qeq_mixed, 3),
-                   "speedup_vs_fp64_at_outer_target": round(legs[1]["qeq_ms_per_step"] / qeq_mixed, 3)}
-        log(f"QEq mixed vs FP64: {qeq_cmp}")
+        errors, qeq_cmp = mixed_compare(s, args, qeq_mixed, ksteps, args.error_steps)
     # the metric's second half (batched systems/s, serving p50/p99) as extra
-    # keys of the same line: short runs of the clusters512 and serve workloads
+    # keys of the same line: short runs of the clusters512 and serve workloads;
+    # and the mixed-precision QEq on the same box (its own run, then the FP64
+    # comparison runs and a short side-by-side error check)
     legs = {}
     if args.legs and not args.mixed:
         lmp.close()
+        log(f"mixed-precision QEq on the same box ({args.leg_error_steps} side-by-side error steps)")
+        mq = qeq_leg(s, args, True, args.tol, ksteps)
+        errors, qeq_cmp = mixed_compare(s, args, mq["qeq_ms_per_step"], ksteps, args.leg_error_steps)
+        qeq_cmp["mixed"] = mq
         cl = bench_clusters(args, nsteps=3, plain_sample=64)
         legs["clusters512"] = {k: cl[k] for k in ("value", "unit", "ms_per_step", "batch_latency_ms",
                                                   "plain_loop_systems_per_s", "speedup_vs_plain_loop",
