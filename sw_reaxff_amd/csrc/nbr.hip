@@ -231,50 +231,86 @@ __global__ void k_bin_pos(int N, const int *__restrict__ bp_id, const double *__
 }
 
 // ---------------------------------------------------------------- Verlet
+// The 64-bit value v of lane l (l uniform across the wave).
+__device__ __forceinline__ long long readlane_ll(long long v, int l) {
+  const int lo = __builtin_amdgcn_readlane((int)v, l), hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+  return ((long long)hi << 32) | (unsigned)lo;
+}
+
+// The same rows as k_verlet's per-atom form (stencil rows in order, each
+// row's candidates in bin order, so every row is the same list in the same
+// order), built one wave per bin: the atoms of a bin share the bin's
+// stencil, so each candidate is loaded once per bin, one per lane, and
+// tested against every atom of the bin in turn.  The atom's position and id
+// are wave-uniform (scalar loads, one atom ahead), its row start and
+// running count live in lane a % 64 of oa and ca; lanes past a row's last
+// candidate test a point at infinity (no branch).  Per atom the arithmetic is the per-atom form's;
+// the candidates' cache traffic drops by the bin occupancy (~20 atoms in
+// water at half-cutoff bins).
 template <bool WRITE>
 __global__ void __launch_bounds__(NB_THREADS)
-k_verlet(const double *__restrict__ x, int N, const int *__restrict__ bin_of, const int *__restrict__ bin_start,
-         const double4 *__restrict__ bp_pos, const int *__restrict__ bp_id, const int4 *__restrict__ rows, int nrows,
-         int nx, int ny, int nz, double cutsq, int *__restrict__ vcount, const long long *__restrict__ voff,
-         int *__restrict__ vlist, int nrow) {
+k_verlet_bins(int nbins, const int *__restrict__ bin_start, const double4 *__restrict__ bp_pos,
+              const int *__restrict__ bp_id, const int4 *__restrict__ rows, int nrows, int nx, int ny, int nz,
+              double cutsq, int *__restrict__ vcount, const long long *__restrict__ voff, int *__restrict__ vlist,
+              int nrow) {
   const int nblk = gridDim.x;
-  const int blk = xcd_block(nblk);
-  const int i = blk * WAVES_PER_BLOCK + wave_in_block();
-  if (i >= N) return;
+  const int b = xcd_block(nblk) * WAVES_PER_BLOCK + wave_in_block();
+  if (b >= nbins) return;
   const int lane = lane_id();
-  if (i >= nrow) {  // rows past nrow stay empty (the local-atom-only list)
-    if (!WRITE && lane == 0) vcount[i] = 0;
-    return;
-  }
-  const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
-  const int b = bin_of[i];
+  const int a0 = __builtin_amdgcn_readfirstlane(bin_start[b]), a1 = __builtin_amdgcn_readfirstlane(bin_start[b + 1]);
   const int bx = b % nx, by = (b / nx) % ny, bz = b / (nx * ny);
-  long long out = WRITE ? voff[i] : 0;
-  int cnt = 0;
-  for (int r = 0; r < nrows; r++) {
-    const int4 rw = rows[r];
-    const int cy = by + rw.x, cz = bz + rw.y;
-    if (cy < 0 || cy >= ny || cz < 0 || cz >= nz) continue;
-    const int cx0 = max(bx - rw.z, 0), cx1 = min(bx + rw.z, nx - 1);
-    const int rowbase = (cz * ny + cy) * nx;
-    const int s = bin_start[rowbase + cx0], e = bin_start[rowbase + cx1 + 1];
-    for (int base = s; base < e; base += SWR_WAVE) {
-      const int k = base + lane;
-      bool hit = false;
-      int j = -1;
-      if (k < e) {
-        j = bp_id[k];
-        const double4 p = bp_pos[k];
-        const double dx = xi - p.x, dy = yi - p.y, dz = zi - p.z;
-        const double rsq = dx * dx + dy * dy + dz * dz;
-        hit = (j != i) && (rsq <= cutsq);
+  for (int g = a0; g < a1; g += SWR_WAVE) {
+    const int ma = min(SWR_WAVE, a1 - g);
+    int ca = 0;   // lane a: the running count of atom g + a
+    long long oa = 0;   // lane a: its row start
+    if (WRITE && lane < ma) {
+      const int ia = bp_id[g + lane];
+      if (ia < nrow) oa = voff[ia];
+    }
+    for (int r = 0; r < nrows; r++) {
+      const int4 rw = rows[r];
+      const int cy = by + rw.x, cz = bz + rw.y;
+      if (cy < 0 || cy >= ny || cz < 0 || cz >= nz) continue;
+      const int cx0 = max(bx - rw.z, 0), cx1 = min(bx + rw.z, nx - 1);
+      const int rowbase = (cz * ny + cy) * nx;
+      const int s = bin_start[rowbase + cx0], e = bin_start[rowbase + cx1 + 1];
+      for (int base = s; base < e; base += SWR_WAVE) {
+        const int k = base + lane;
+        int j = -1;
+        double px = HUGE_VAL, py = HUGE_VAL, pz = HUGE_VAL;
+        if (k < e) {
+          j = bp_id[k];
+          const double4 p = bp_pos[k];
+          px = p.x;
+          py = p.y;
+          pz = p.z;
+        }
+        // atom a + 1's id and position are requested while atom a is tested
+        // (uniform: scalar loads)
+        int inx = bp_id[g];
+        double4 pnx = bp_pos[g];
+        for (int a = 0; a < ma; a++) {
+          const int i = inx;
+          const double4 pa = pnx;
+          if (a + 1 < ma) {
+            inx = bp_id[g + a + 1];
+            pnx = bp_pos[g + a + 1];
+          }
+          if (i >= nrow) continue;   // rows past nrow stay empty (the local-atom-only list)
+          const double dx = pa.x - px, dy = pa.y - py, dz = pa.z - pz;
+          const double rsq = dx * dx + dy * dy + dz * dz;
+          const bool hit = (j != i) && (rsq <= cutsq);
+          const unsigned long long m = __ballot(hit);
+          if (WRITE && hit) vlist[readlane_ll(oa, a) + __builtin_amdgcn_readlane(ca, a) + ballot_rank(m)] = j;
+          if (lane == a) ca += __popcll(m);
+        }
       }
-      const unsigned long long m = __ballot(hit);
-      if (WRITE && hit) vlist[out + cnt + ballot_rank(m)] = j;
-      cnt += __popcll(m);
+    }
+    if (!WRITE && lane < ma) {
+      const int ia = bp_id[g + lane];
+      vcount[ia] = ia < nrow ? ca : 0;
     }
   }
-  if (!WRITE && lane == 0) vcount[i] = cnt;
 }
 
 }  // namespace
@@ -534,9 +570,9 @@ int swr_nbr_build(swr_system *S, int local_only) {
   if ((rc = swr_reserve(&S->vcount, S->cap_vcount, N)) || (rc = swr_reserve(&S->voff, S->cap_voff, (size_t)N + 1)))
     return rc;
   const double cutsq = S->cutneigh * S->cutneigh;
-  const int nblk = div_up(N, WAVES_PER_BLOCK);
+  const int nblk = div_up(S->nbins, WAVES_PER_BLOCK);
   swr_kt_begin(S, KT_VERLET);
-  hipLaunchKernelGGL(k_verlet<false>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->x, N, S->bin_of, S->bin_start,
+  hipLaunchKernelGGL(k_verlet_bins<false>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->nbins, S->bin_start,
                      S->bp_pos, S->bp_id, S->stencil_rows, S->nrows, S->nbin[0], S->nbin[1], S->nbin[2], cutsq,
                      S->vcount, (const long long *)nullptr, (int *)nullptr, nrow);
   swr_kt_end(S, KT_VERLET);
@@ -553,7 +589,7 @@ int swr_nbr_build(swr_system *S, int local_only) {
   HIP_CHECK(hipMemcpyAsync(&S->vtotal_local, S->voff + S->n, sizeof(long long), hipMemcpyDeviceToHost, S->stream));
   HIP_CHECK(hipStreamSynchronize(S->stream));
   swr_kt_begin(S, KT_VERLET);
-  hipLaunchKernelGGL(k_verlet<true>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->x, N, S->bin_of, S->bin_start,
+  hipLaunchKernelGGL(k_verlet_bins<true>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->nbins, S->bin_start,
                      S->bp_pos, S->bp_id, S->stencil_rows, S->nrows, S->nbin[0], S->nbin[1], S->nbin[2], cutsq,
                      S->vcount, (const long long *)S->voff, S->vlist, nrow);
   swr_kt_end(S, KT_VERLET);
