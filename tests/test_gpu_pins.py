@@ -165,3 +165,31 @@ def test_gpu_atomonly_list_equals_oracle_local_rows(gpu_lib, oracle_lib, ffield)
     w = np.lexsort((olst[:ooff[n]], oi))
     assert np.array_equal(alst[:aoff[n]][g], olst[:ooff[n]][w])
     assert not np.any(alst[:aoff[n]] == gi)
+
+
+def test_gpu_verlet_bins_beyond_one_wave(gpu_lib, oracle_lib, ffield):
+    """The Verlet build takes a bin's atoms 64 at a time (one per lane) and
+    streams the bin's stencil past them: a dense random cloud of 1,500 atoms
+    in a 20-A cube at a 16-A cutoff (two 10-A bins a side, ~190 atoms each,
+    so three groups per bin) against the oracle's brute-force enumeration
+    (npair_full_bin_ghost_sw5.c:183-199): every row equal as a set, j != i."""
+    rng = np.random.default_rng(7)
+    n = 1500
+    x = rng.uniform(0.0, 20.0, size=(n, 3))
+    types = np.ones(n, np.int32)   # LAMMPS types, 1-based
+    h = _system(gpu_lib, ffield, ["O"], x, types)
+    h.build_neighbors(16.0)
+    h.sync()
+    off, lst = verlet_of(h)
+    o = oracle_lib.Oracle(ffield, ["O"], threads=8)
+    o.set_atoms(n, x, types, np.arange(1, n + 1, dtype=np.int32), np.arange(n, dtype=np.int32))
+    o.build_verlet(16.0, brute=True)
+    ooff, olst = o.get("verlet_offsets"), o.get("verlet_list")
+    cnt = np.diff(off)
+    print(f"[verlet] {n} atoms, {off[-1]} pairs, rows {cnt.min()}..{cnt.max()}", flush=True)
+    assert np.array_equal(cnt, np.diff(ooff))
+    rows = np.repeat(np.arange(n), cnt)
+    assert not np.any(lst[:off[-1]] == rows)
+    g = np.lexsort((lst[:off[-1]], rows))
+    w = np.lexsort((olst[:ooff[-1]], rows))
+    assert np.array_equal(lst[:off[-1]][g], olst[:ooff[-1]][w])
