@@ -237,16 +237,17 @@ __device__ __forceinline__ long long readlane_ll(long long v, int l) {
   return ((long long)hi << 32) | (unsigned)lo;
 }
 
-// The same rows as k_verlet's per-atom form (stencil rows in order, each
-// row's candidates in bin order, so every row is the same list in the same
-// order), built one wave per bin: the atoms of a bin share the bin's
-// stencil, so each candidate is loaded once per bin, one per lane, and
-// tested against every atom of the bin in turn.  The atom's position and id
-// are wave-uniform (scalar loads, one atom ahead), its row start and
-// running count live in lane a % 64 of oa and ca; lanes past a row's last
-// candidate test a point at infinity (no branch).  Per atom the arithmetic is the per-atom form's;
-// the candidates' cache traffic drops by the bin occupancy (~20 atoms in
-// water at half-cutoff bins).
+// The full Verlet list, row i = every j != i within cutneigh, in stencil
+// order (stencil rows in order, each row's candidates in bin order: the
+// list one wave per atom walking its stencil would write), built one wave
+// per bin: the atoms of a bin share the bin's stencil, so each candidate is
+// loaded once per bin, one per lane, and tested against every atom of the
+// bin in turn (64 atoms at a time).  The atom's position and id are
+// wave-uniform (scalar loads, one atom ahead), its row start and running
+// count live in lane a % 64 of oa and ca; lanes past a row's last
+// candidate test a point at infinity (no branch).  The candidates' cache
+// traffic is that of one wave per atom divided by the bin occupancy (~20
+// atoms in water at half-cutoff bins).
 template <bool WRITE>
 __global__ void __launch_bounds__(NB_THREADS)
 k_verlet_bins(int nbins, const int *__restrict__ bin_start, const double4 *__restrict__ bp_pos,
