@@ -378,7 +378,7 @@ int swr_destroy(swr_system *S) {
   if (!S) return SWR_OK;
   if (S->stream) (void)hipStreamSynchronize(S->stream);
   void *ptrs[] = {S->dp, S->x, S->v, S->q, S->type, S->tag, S->owner, S->img, S->mask, S->packed,
-                  S->bin_of, S->bin_count, S->bin_start, S->rs_keys, S->rs_vals, S->rs_hist, S->bp_pos, S->bp_id, S->stencil_rows,
+                  S->bin_of, S->bin_count, S->bin_start, S->bin_occ, S->rs_keys, S->rs_vals, S->rs_hist, S->bp_pos, S->bp_id, S->stencil_rows,
                   S->voff, S->vcount, S->vlist, S->far, S->far_index, S->far_end, S->num_bonds, S->num_hbonds,
                   S->Hindex, S->bond_index, S->bond_end, S->bonds, S->bond_nbr, S->bo_data, S->prow, S->pbeg, S->pcnt, S->pcol, S->pkey, S->jloc, S->sys_panel, S->BO, S->BOpi, S->Cdbo,
                   S->Cdbopi,

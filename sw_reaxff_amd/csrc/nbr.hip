@@ -210,8 +210,11 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const int *__restrict__ key
 }
 
 // Atoms per bin from the sorted keys: each run of equal keys writes its
-// length (bins without atoms keep the zero they were cleared to).
-__global__ void k_bin_runs(const int *__restrict__ skeys, int N, int *__restrict__ bin_count) {
+// length (bins without atoms keep the zero they were cleared to) and
+// appends its bin to the occupied-bin list occ (occ[0] counts them; the
+// Verlet build takes one wave per entry, and the rows it writes do not
+// depend on the entries' order).
+__global__ void k_bin_runs(const int *__restrict__ skeys, int N, int *__restrict__ bin_count, int *__restrict__ occ) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= N) return;
   const int key = skeys[k];
@@ -219,6 +222,7 @@ __global__ void k_bin_runs(const int *__restrict__ skeys, int N, int *__restrict
   int e = k + 1;
   while (e < N && skeys[e] == key) e++;
   bin_count[key] = e - k;
+  occ[1 + atomicAdd(&occ[0], 1)] = key;   // the occupied bins, in no particular order
 }
 
 // Positions of the sorted atoms (x, y, z, unused).
@@ -240,7 +244,7 @@ __device__ __forceinline__ long long readlane_ll(long long v, int l) {
 // The full Verlet list, row i = every j != i within cutneigh, in stencil
 // order (stencil rows in order, each row's candidates in bin order: the
 // list one wave per atom walking its stencil would write), built one wave
-// per bin: the atoms of a bin share the bin's stencil, so each candidate is
+// per occupied bin: the atoms of a bin share the bin's stencil, so each candidate is
 // loaded once per bin, one per lane, and tested against every atom of the
 // bin in turn (64 atoms at a time).  The atom's position and id are
 // wave-uniform (scalar loads, one atom ahead), its row start and running
@@ -250,13 +254,19 @@ __device__ __forceinline__ long long readlane_ll(long long v, int l) {
 // atoms in water at half-cutoff bins).
 template <bool WRITE>
 __global__ void __launch_bounds__(NB_THREADS)
-k_verlet_bins(int nbins, const int *__restrict__ bin_start, const double4 *__restrict__ bp_pos,
+k_verlet_bins(const int *__restrict__ occ, const int *__restrict__ bin_start, const double4 *__restrict__ bp_pos,
               const int *__restrict__ bp_id, const int4 *__restrict__ rows, int nrows, int nx, int ny, int nz,
               double cutsq, int *__restrict__ vcount, const long long *__restrict__ voff, int *__restrict__ vlist,
               int nrow) {
-  const int nblk = gridDim.x;
-  const int b = xcd_block(nblk) * WAVES_PER_BLOCK + wave_in_block();
-  if (b >= nbins) return;
+  // the grid is sized for the worst case (every atom in its own bin); the
+  // blocks that hold occupied bins are spread over the XCDs by their own
+  // count, so each XCD takes a contiguous share of them
+  const int nocc = occ[0];
+  const int nact = (nocc + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+  if ((int)blockIdx.x >= nact) return;
+  const int t = xcd_block(nact) * WAVES_PER_BLOCK + wave_in_block();
+  if (t >= nocc) return;
+  const int b = occ[1 + t];   // an occupied bin (a mostly empty grid, as a batch's, costs nothing)
   const int lane = lane_id();
   const int a0 = __builtin_amdgcn_readfirstlane(bin_start[b]), a1 = __builtin_amdgcn_readfirstlane(bin_start[b + 1]);
   const int bx = b % nx, by = (b / nx) % ny, bz = b / (nx * ny);
@@ -529,7 +539,7 @@ int swr_nbr_build(swr_system *S, int local_only) {
   const int nrow = local_only ? S->n : N;
   int rc;
   if ((rc = swr_reserve(&S->bin_of, S->cap_bin_of, N)) || (rc = swr_reserve(&S->bp_id, S->cap_bp_id, N)) ||
-      (rc = swr_reserve(&S->bp_pos, S->cap_bp_pos, N)))
+      (rc = swr_reserve(&S->bp_pos, S->cap_bp_pos, N)) || (rc = swr_reserve(&S->bin_occ, S->cap_bin_occ, (size_t)N + 1)))
     return rc;
   // counting sort by bin, staged in LDS (k_rs_*): passes of 8 bits up to the
   // largest bin id; ping-pong between the key/value buffers so that the
@@ -562,7 +572,8 @@ int swr_nbr_build(swr_system *S, int local_only) {
       vin = vout;
     }
     HIP_CHECK(hipMemsetAsync(S->bin_count, 0, sizeof(int) * (S->nbins + 1), S->stream));
-    hipLaunchKernelGGL(k_bin_runs, dim3(div_up(N, 256)), dim3(256), 0, S->stream, kin, N, S->bin_count);
+    HIP_CHECK(hipMemsetAsync(S->bin_occ, 0, sizeof(int), S->stream));
+    hipLaunchKernelGGL(k_bin_runs, dim3(div_up(N, 256)), dim3(256), 0, S->stream, kin, N, S->bin_count, S->bin_occ);
   }
   if ((rc = swr_scan_i32(S, S->bin_count, S->bin_start, S->nbins + 1, nullptr))) return rc;
   hipLaunchKernelGGL(k_bin_pos, dim3(div_up(N, 256)), dim3(256), 0, S->stream, N, (const int *)S->bp_id, S->x,
@@ -571,9 +582,10 @@ int swr_nbr_build(swr_system *S, int local_only) {
   if ((rc = swr_reserve(&S->vcount, S->cap_vcount, N)) || (rc = swr_reserve(&S->voff, S->cap_voff, (size_t)N + 1)))
     return rc;
   const double cutsq = S->cutneigh * S->cutneigh;
-  const int nblk = div_up(S->nbins, WAVES_PER_BLOCK);
+  // one wave per occupied bin (at most min(nbins, N) of them)
+  const int nblk = div_up(std::min(S->nbins, N), WAVES_PER_BLOCK);
   swr_kt_begin(S, KT_VERLET);
-  hipLaunchKernelGGL(k_verlet_bins<false>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->nbins, S->bin_start,
+  hipLaunchKernelGGL(k_verlet_bins<false>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->bin_occ, S->bin_start,
                      S->bp_pos, S->bp_id, S->stencil_rows, S->nrows, S->nbin[0], S->nbin[1], S->nbin[2], cutsq,
                      S->vcount, (const long long *)nullptr, (int *)nullptr, nrow);
   swr_kt_end(S, KT_VERLET);
@@ -590,7 +602,7 @@ int swr_nbr_build(swr_system *S, int local_only) {
   HIP_CHECK(hipMemcpyAsync(&S->vtotal_local, S->voff + S->n, sizeof(long long), hipMemcpyDeviceToHost, S->stream));
   HIP_CHECK(hipStreamSynchronize(S->stream));
   swr_kt_begin(S, KT_VERLET);
-  hipLaunchKernelGGL(k_verlet_bins<true>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->nbins, S->bin_start,
+  hipLaunchKernelGGL(k_verlet_bins<true>, dim3(nblk), dim3(NB_THREADS), 0, S->stream, S->bin_occ, S->bin_start,
                      S->bp_pos, S->bp_id, S->stencil_rows, S->nrows, S->nbin[0], S->nbin[1], S->nbin[2], cutsq,
                      S->vcount, (const long long *)S->voff, S->vlist, nrow);
   swr_kt_end(S, KT_VERLET);

@@ -204,6 +204,8 @@ struct swr_system {
   int nbins = 0, binscap = 0;
   size_t cap_stencil = 0, cap_bin_of = 0, cap_bp_id = 0, cap_bp_pos = 0, cap_vcount = 0, cap_voff = 0;
   int *bin_of = nullptr, *bin_count = nullptr, *bin_start = nullptr;
+  int *bin_occ = nullptr;        // [0]: number of occupied bins, [1..]: their ids
+  size_t cap_bin_occ = 0;
   // counting sort by bin (radix passes staged in LDS): key/value ping-pong
   // buffers and the per-(digit, block) counts and starts
   int *rs_keys = nullptr, *rs_vals = nullptr, *rs_hist = nullptr;
