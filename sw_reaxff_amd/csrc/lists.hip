@@ -805,9 +805,12 @@ int swr_lists_reset(swr_system *S) {
   HIP_CHECK(hipMemsetAsync(S->fcd, 0, sizeof(double) * 4 * (size_t)N, S->stream));
   hipLaunchKernelGGL(k_reset_flags, dim3(div_up(N, 256)), dim3(256), 0, S->stream, S->packed, S->dp, n, N,
                      S->ctl.hbond_cut, S->num_hbonds, hflag, hcap);
-  if ((rc = swr_scan_i32(S, hflag, hpos, N, &S->status->numH))) return rc;
-  if ((rc = swr_scan_i32(S, hcap, hstart, N, &S->status->total_hbonds))) return rc;
-  if ((rc = swr_scan_i32(S, S->num_bonds, S->bond_index, N, &S->status->total_bonds))) return rc;
+  {  // the three prefix sums of Reset in one set of launches
+    const int *in[3] = {hflag, hcap, S->num_bonds};
+    int *out[3] = {hpos, hstart, S->bond_index};
+    int *tot[3] = {&S->status->numH, &S->status->total_hbonds, &S->status->total_bonds};
+    if ((rc = swr_scan3_i32(S, in, out, N, tot))) return rc;
+  }
   hipLaunchKernelGGL(k_reset_apply, dim3(div_up(N, 256)), dim3(256), 0, S->stream, n, N, hflag, hpos, hstart,
                      S->Hindex, S->bond_index, S->bond_end, S->hbond_index, S->hbond_end, S->status, S->bond_cap,
                      S->hbond_cap);

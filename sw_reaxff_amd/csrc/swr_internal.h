@@ -400,6 +400,7 @@ enum { CGV_R = 0, CGV_U, CGV_W, CGV_Z, CGV_SS, CGV_V, CGV_P, CGV_D0, CGV_D1, CGV
 // ---- helpers implemented in the .hip files
 int swr_scan_i32_to_i64(swr_system *S, const int *in, long long *out, int n, long long *total_host);
 int swr_scan_i32(swr_system *S, const int *in, int *out, int n, int *total_dev);
+int swr_scan3_i32(swr_system *S, const int *const in[3], int *const out[3], int n, int *const total[3]);
 int swr_ensure_tmp(swr_system *S, size_t bytes);
 
 // Stage implementations (nbr.hip, lists.hip, bo.hip, qeq.hip, qeq_mixed.hip).

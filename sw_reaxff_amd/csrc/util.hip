@@ -80,6 +80,70 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const TI *in, int n
   }
 }
 
+// Three exclusive int scans of the same length in one set of three launches
+// (blockIdx.y picks the array): the same tiles, sums and order as three
+// separate scans.
+struct Scan3 {
+  const int *in[3];
+  int *out[3];
+  int *total[3];
+};
+template <class T>
+__device__ __forceinline__ T pick3(int a, T x, T y, T z) { return a == 0 ? x : a == 1 ? y : z; }
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_tile_sums3(Scan3 A, int n, int ntiles, int *tiles) {
+  __shared__ int lds[SCAN_THREADS / SWR_WAVE];
+  const int a = blockIdx.y;
+  const int *in = pick3(a, A.in[0], A.in[1], A.in[2]);
+  const long long base = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++)
+    if (base + k < n) s += in[base + k];
+  int tot;
+  block_excl_scan<int>(s, lds, &tot);
+  if (threadIdx.x == 0) tiles[(size_t)a * ntiles + blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles3(Scan3 A, int ntiles, int *tiles) {
+  __shared__ int lds[SCAN_THREADS / SWR_WAVE];
+  const int a = blockIdx.y;
+  int *ts = tiles + (size_t)a * ntiles;
+  int carry = 0;
+  for (int c = 0; c < ntiles; c += SCAN_THREADS) {
+    int i = c + threadIdx.x;
+    int v = i < ntiles ? ts[i] : 0;
+    int tot;
+    int ex = block_excl_scan<int>(v, lds, &tot);
+    if (i < ntiles) ts[i] = carry + ex;
+    carry += tot;
+  }
+  int *total = pick3(a, A.total[0], A.total[1], A.total[2]);
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply3(Scan3 A, int n, int ntiles, const int *tiles) {
+  __shared__ int lds[SCAN_THREADS / SWR_WAVE];
+  const int a = blockIdx.y;
+  const int *in = pick3(a, A.in[0], A.in[1], A.in[2]);
+  int *out = pick3(a, A.out[0], A.out[1], A.out[2]);
+  const long long base = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int vals[SCAN_ITEMS];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    vals[k] = (base + k < n) ? in[base + k] : 0;
+    s += vals[k];
+  }
+  int tot;
+  int ex = block_excl_scan<int>(s, lds, &tot) + tiles[(size_t)a * ntiles + blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    if (base + k < n) out[base + k] = ex;
+    ex += vals[k];
+  }
+}
+
 template <class TI, class TO>
 int scan_impl(swr_system *S, const TI *in, TO *out, int n, TO *total_dev) {
   int ntiles = div_up(n > 0 ? n : 1, SCAN_TILE);
@@ -128,6 +192,38 @@ int swr_scan_i32_to_i64(swr_system *S, const int *in, long long *out, int n, lon
     HIP_CHECK(hipMemcpyAsync(total_host, out + n, sizeof(long long), hipMemcpyDeviceToHost, S->stream));
     HIP_CHECK(hipStreamSynchronize(S->stream));
   }
+  return SWR_OK;
+}
+
+// Three exclusive scans of n ints each (in[k] -> out[k], totals to the
+// device ints total[k]) in three launches.
+int swr_scan3_i32(swr_system *S, const int *const in[3], int *const out[3], int n, int *const total[3]) {
+  if (n <= 0) {
+    for (int k = 0; k < 3; k++)
+      if (total[k]) HIP_CHECK(hipMemsetAsync(total[k], 0, sizeof(int), S->stream));
+    return SWR_OK;
+  }
+  const int ntiles = div_up(n, SCAN_TILE);
+  const size_t need = sizeof(int) * 3 * (size_t)ntiles;
+  if (need > S->scan_tmp_bytes) {
+    if (S->scan_tmp) (void)hipFree(S->scan_tmp);
+    S->scan_tmp = nullptr;
+    S->scan_tmp_bytes = 0;
+    HIP_CHECK(hipMalloc(&S->scan_tmp, need * 2));
+    S->scan_tmp_bytes = need * 2;
+  }
+  int *tiles = (int *)S->scan_tmp;
+  Scan3 A;
+  for (int k = 0; k < 3; k++) {
+    A.in[k] = in[k];
+    A.out[k] = out[k];
+    A.total[k] = total[k];
+  }
+  hipLaunchKernelGGL(k_tile_sums3, dim3(ntiles, 3), dim3(SCAN_THREADS), 0, S->stream, A, n, ntiles, tiles);
+  hipLaunchKernelGGL(k_scan_tiles3, dim3(1, 3), dim3(SCAN_THREADS), 0, S->stream, A, ntiles, tiles);
+  hipLaunchKernelGGL(k_scan_apply3, dim3(ntiles, 3), dim3(SCAN_THREADS), 0, S->stream, A, n, ntiles,
+                     (const int *)tiles);
+  HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }
 
