@@ -36,8 +36,8 @@ __global__ void k_border_flag(const double *__restrict__ x, int nlast, int d, do
   flag[i] = side == 0 ? (v <= thr) : (v >= thr);
 }
 
-__global__ void k_border_scatter(double *__restrict__ x, int *__restrict__ type, int *__restrict__ tag,
-                                 int *__restrict__ owner, signed char *__restrict__ img,
+__global__ void k_border_scatter(double *__restrict__ x, double *__restrict__ q, int *__restrict__ type,
+                                 int *__restrict__ tag, int *__restrict__ owner, signed char *__restrict__ img,
                                  int *__restrict__ num_bonds, int *__restrict__ num_hbonds,
                                  const int *__restrict__ flag, const int *__restrict__ pos, int nlast, int ncur,
                                  int d, double shift, int sgn) {
@@ -46,6 +46,7 @@ __global__ void k_border_scatter(double *__restrict__ x, int *__restrict__ type,
   int g = ncur + pos[i];
   for (int k = 0; k < 3; k++) x[3 * g + k] = x[3 * i + k];
   x[3 * g + d] = x[3 * i + d] + shift;
+  q[g] = q[i];   // the charge travels with the border data, as atom style charge's does
   type[g] = type[i];
   tag[g] = tag[i];
   owner[g] = owner[i];
@@ -461,8 +462,8 @@ int swr_nbr_borders(swr_system *S, const double lo[3], const double hi[3], doubl
         pos = S->tmp_i + nlast + 1;
       }
       double L = hi[d] - lo[d];
-      hipLaunchKernelGGL(k_border_scatter, dim3(div_up(nlast, 256)), dim3(256), 0, S->stream, S->x, S->type,
-                         S->tag, S->owner, S->img, S->num_bonds, S->num_hbonds, flag, pos, nlast, ncur, d,
+      hipLaunchKernelGGL(k_border_scatter, dim3(div_up(nlast, 256)), dim3(256), 0, S->stream, S->x, S->q,
+                         S->type, S->tag, S->owner, S->img, S->num_bonds, S->num_hbonds, flag, pos, nlast, ncur, d,
                          side == 0 ? L : -L, side == 0 ? 1 : -1);
       ncur += add;
     }
