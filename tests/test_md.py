@@ -263,6 +263,33 @@ def test_oracle_nve_energy_error_is_second_order(oracle_lib, ffield):
 
 
 @pytest.mark.gpu
+def test_gpu_borders_carry_charges_without_qeq(ffield):
+    """With QEq off nothing else writes the ghosts' charges, so the border
+    pass must copy each ghost's charge from the atom it images (as LAMMPS's
+    atom style charge sends q with the border data).  Local charges are set
+    by hand, the ghosts marked with a sentinel, and the harness's next
+    rebuild must leave every ghost with its owner's charge."""
+    s = systems.water_box(64, seed=6)
+    lmp = _harness(s, ffield, 1e-8, dt=0.1, qeq=False)
+    h = lmp.system
+    h.sync()
+    n, N = h.scalar("n"), h.scalar("N")
+    q = h.get("q").copy()
+    rng = np.random.default_rng(6)
+    q[:n] = rng.uniform(-0.9, 0.9, n)
+    q[n:N] = 7.0
+    h.put("q", q)
+    lmp.run(10)   # crosses the rebuild: ghosts are made again from the locals
+    h.sync()
+    n, N = h.scalar("n"), h.scalar("N")
+    q2, owner = h.get("q")[:N], h.get("owner")[:N]
+    print(f"[borders] n={n} N={N}: ghost charges from owners", flush=True)
+    assert N > n
+    assert np.array_equal(q2[:n], q[:n])
+    assert np.array_equal(q2[n:], q2[owner[n:]])
+
+
+@pytest.mark.gpu
 def test_gpu_md_trajectory_matches_oracle(gpu_lib, oracle_lib, ffield):
     """20 steps of 0.1 fs with `forces yes` (zero charges) through the harness
     (device borders, rebuild every 10 steps, forward communication between)
