@@ -361,6 +361,16 @@ int swr_create(const char *ffield_path, int ntypes, const char *const *elements,
     return SWR_ERR_NOMEM;
   }
   std::memset(S->status_host, 0, sizeof(DevStatus));
+  if (hipHostMalloc((void **)&S->pin_done, 64, 0) != hipSuccess ||
+      hipHostMalloc(&S->pin_mix, SWR_PIN_MIX, 0) != hipSuccess) {
+    swr_set_error_msg("hipHostMalloc failed");
+    swr_destroy(S);
+    return SWR_ERR_NOMEM;
+  }
+  if ((rc = swr_pin_iters_reserve(S, 64))) {
+    swr_destroy(S);
+    return rc;
+  }
   // the system's stream does not wait for the null stream: everything goes on it
   if (hipMemcpyAsync(S->dp, &S->hp, sizeof(DevParams), hipMemcpyHostToDevice, S->stream) != hipSuccess ||
       hipMemsetAsync(S->status, 0, sizeof(DevStatus), S->stream) != hipSuccess ||
@@ -394,6 +404,9 @@ int swr_destroy(swr_system *S) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (S->status_host) (void)hipHostFree(S->status_host);
+  if (S->pin_done) (void)hipHostFree(S->pin_done);
+  if (S->pin_mix) (void)hipHostFree(S->pin_mix);
+  if (S->pin_iters) (void)hipHostFree(S->pin_iters);
   if (S->mark_chg_host) (void)hipHostFree(S->mark_chg_host);
   if (S->stream) (void)hipStreamDestroy(S->stream);
   delete S;
@@ -636,9 +649,10 @@ int swr_qeq_cg(swr_system *S, int *iters_s, int *iters_t) {
   if (rc) return rc;
   if (S->mixed) {  // the FP64 solve's last done check already brought status and counts back
     S->h_sys_iters.resize(2 * (size_t)S->nsys);
-    HIP_CHECK(hipMemcpyAsync(S->h_sys_iters.data(), S->sys_iters, sizeof(int) * 2 * S->nsys,
-                             hipMemcpyDeviceToHost, S->stream));
-    if ((rc = refresh_status(S))) return rc;
+    HIP_CHECK(hipMemcpyAsync(S->pin_iters, S->sys_iters, sizeof(int) * 2 * S->nsys, hipMemcpyDeviceToHost,
+                             S->stream));
+    if ((rc = refresh_status(S))) return rc;   // (synchronises)
+    std::memcpy(S->h_sys_iters.data(), S->pin_iters, sizeof(int) * 2 * S->nsys);
   }
   int ms = 0, mt = 0;
   for (int k = 0; k < S->nsys; k++) {

@@ -291,11 +291,30 @@ int swr_qeq_spmv(swr_system *S, const double *xv, double *b) {
 
 // Row/tile layout of the (batch of) systems and the buffers sized by it.
 // Rebuilt when the atoms or the batch partition changed.
+int swr_pin_iters_reserve(swr_system *S, int nsys) {
+  const int need = 2 * std::max(nsys, 1);
+  if (need <= S->pin_iters_cap) return SWR_OK;
+  const int cap = std::max(need, 2 * S->pin_iters_cap);
+  if (S->pin_iters) {
+    HIP_CHECK(hipStreamSynchronize(S->stream));
+    HIP_CHECK(hipHostFree(S->pin_iters));
+    S->pin_iters = nullptr;
+    S->pin_iters_cap = 0;
+  }
+  if (hipHostMalloc((void **)&S->pin_iters, sizeof(int) * (size_t)cap, 0) != hipSuccess) {
+    swr_set_error_msg("hipHostMalloc failed");
+    return SWR_ERR_NOMEM;
+  }
+  S->pin_iters_cap = cap;
+  return SWR_OK;
+}
+
 int swr_qeq_layout(swr_system *S) {
   if (!S->layout_dirty) return SWR_OK;
   const int n = S->n;
   if (S->sys_row.size() < 2 || S->sys_row.back() != n) S->sys_row = {0, n};
   const int nsys = (int)S->sys_row.size() - 1;
+  if (int rc = swr_pin_iters_reserve(S, nsys)) return rc;
   std::vector<int> row_sys(std::max(n, 1)), tile_beg, tile_sys, sys_tile(nsys + 1), sblk_beg, sys_sblk(nsys + 1);
   for (int s = 0; s < nsys; s++) {
     sys_tile[s] = (int)tile_sys.size();
@@ -414,7 +433,7 @@ int swr_qeq_time_spmv(swr_system *S, int sh, int reps, float *ms) {
 int swr_qeq_cg_run(swr_system *S, int first_batch) {
   int rc = swr_qeq_layout(S);
   if (rc) return rc;
-  CgRun<double, double> R;
+  CgRun<double, double> R{};
   R.V = make_vecs(S);
   R.val = S->H_val;
   R.L = swr_qeq_cg_layout(S);
@@ -422,6 +441,8 @@ int swr_qeq_cg_run(swr_system *S, int first_batch) {
   R.iters_out = S->sys_iters;
   S->h_sys_iters.resize(2 * (size_t)S->nsys);
   R.iters_host = S->h_sys_iters.data();
+  R.pin_done = S->pin_done;
+  R.pin_iters = S->pin_iters;
   R.tol = S->ctl.tol;
   R.imax = S->imax;
   R.kt_iter = KT_CG_ITER;
@@ -629,7 +650,7 @@ int swr_qeq_cg_noalloc_run(swr_system *S, const double *b, double *x, int *iters
   CgScalars *cgs = (CgScalars *)(base + NA_OFF_CGS);
   hipLaunchKernelGGL(k_na_layout, dim3(div_up(std::max(n, ntiles + 1), 256)), dim3(256), 0, S->stream, n, ntiles,
                      (int *)L.row_sys, (int *)L.tile_beg, (int *)L.tile_sys, (int *)L.sys_tile, cgs);
-  CgRun<double, double> R;
+  CgRun<double, double> R{};
   R.V = V;
   R.val = S->H_val;
   R.L = L;

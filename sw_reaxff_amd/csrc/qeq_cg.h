@@ -18,6 +18,7 @@
 // sqrt(u.r)/|b| <= tol (or i = imax) and is frozen while the other runs.
 #pragma once
 
+#include <cstring>
 #include <type_traits>
 
 #include "swr_internal.h"
@@ -537,6 +538,8 @@ struct CgRun {
   CgScalars *cgs;     // 2 per system (s, t)
   int *iters_out;     // 2 ints per system on the device
   int *iters_host;    // when set: the host copy, refreshed at every done check
+  int *pin_done;      // when set: pinned host int and iteration counts the
+  int *pin_iters;     // done checks copy into (then iters_host from pin_iters)
   double tol;         // > 0: set at setup; <= 0: preset in cgs[].tol
   int imax;           // iteration limit (the reference's imax = 200 for the FP64 solve)
   int kt_iter, kt_update, kt_setup, kt_scalars;
@@ -626,14 +629,18 @@ int cg_solve(swr_system *S, const CgRun<T, VT> &R, int first_batch, int xzero) {
   while (k < R.imax + 1) {
     for (int b = 0; b < batch && k < R.imax + 1; b++, k++)
       if ((rc = launch_cg(S, R, PH_K, (k - 1) & 1, 0))) return rc;
-    // one round trip: done count, status block and iteration counts
+    // one round trip: done count, status block and iteration counts (into
+    // pinned memory when the run has it)
     int done = 0;
-    HIP_CHECK(hipMemcpyAsync(&done, R.L.done_count, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    int *dh = R.pin_done ? R.pin_done : &done;
+    int *ih = R.pin_iters ? R.pin_iters : R.iters_host;
+    HIP_CHECK(hipMemcpyAsync(dh, R.L.done_count, sizeof(int), hipMemcpyDeviceToHost, S->stream));
     HIP_CHECK(hipMemcpyAsync(S->status_host, S->status, sizeof(DevStatus), hipMemcpyDeviceToHost, S->stream));
-    if (R.iters_host)
-      HIP_CHECK(hipMemcpyAsync(R.iters_host, R.iters_out, sizeof(int) * 2 * R.L.nsys, hipMemcpyDeviceToHost,
-                               S->stream));
+    if (ih)
+      HIP_CHECK(hipMemcpyAsync(ih, R.iters_out, sizeof(int) * 2 * R.L.nsys, hipMemcpyDeviceToHost, S->stream));
     HIP_CHECK(hipStreamSynchronize(S->stream));
+    done = *dh;
+    if (R.iters_host && ih != R.iters_host) std::memcpy(R.iters_host, ih, sizeof(int) * 2 * R.L.nsys);
     swr_after_sync(S);
     if (done >= R.L.nsys) break;
     batch = 4;

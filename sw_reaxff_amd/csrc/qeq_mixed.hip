@@ -493,8 +493,11 @@ int swr_qeq_mixed_solve(swr_system *S) {
   const double2 *b = (const double2 *)S->b_s;
   if (nblk < 1 || L.ntiles < 1) return SWR_OK;
   HIP_CHECK(hipMemsetAsync(ctl, 0, sizeof(MixCtl), S->stream));
-  struct { MixCtl c; char pad[MS_OFF - sizeof(MixCtl)]; MixState m0; } h;
-  static_assert(offsetof(decltype(h), m0) == MS_OFF, "host copy mirrors the workspace");
+  struct Head { MixCtl c; char pad[MS_OFF - sizeof(MixCtl)]; MixState m0; };
+  static_assert(offsetof(Head, m0) == MS_OFF, "host copy mirrors the workspace");
+  static_assert(sizeof(Head) + sizeof(MixCtl) <= SWR_PIN_MIX, "the pinned copy holds both reads");
+  Head &h = *(Head *)S->pin_mix;                          // pinned: the copies are plain DMA
+  MixCtl &hc = *(MixCtl *)((char *)S->pin_mix + sizeof(Head));
   int outer = 0;
   for (;; outer++) {
     swr_kt_begin(S, KT_MIXED_OUTER);
@@ -560,7 +563,6 @@ int swr_qeq_mixed_solve(swr_system *S) {
                            (const double *)S->partials, ps, ctl, INNER_IMAX);
       }
       HIP_CHECK(hipGetLastError());
-      MixCtl hc;
       HIP_CHECK(hipMemcpyAsync(&hc, ctl, sizeof(hc), hipMemcpyDeviceToHost, S->stream));
       HIP_CHECK(hipStreamSynchronize(S->stream));
       if (hc.inner_ndone >= nsys) {

@@ -205,6 +205,12 @@ struct swr_system {
   size_t cap_stencil = 0, cap_bin_of = 0, cap_bp_id = 0, cap_bp_pos = 0, cap_vcount = 0, cap_voff = 0;
   int *bin_of = nullptr, *bin_count = nullptr, *bin_start = nullptr;
   int *bin_occ = nullptr;        // [0]: number of occupied bins, [1..]: their ids
+  // pinned host buffers the done checks copy into (device-to-host copies
+  // into pageable memory are staged through a bounce buffer)
+  int *pin_done = nullptr;        // the CG's count of finished systems
+  void *pin_mix = nullptr;        // the mixed solve's counters and state (SWR_PIN_MIX bytes)
+  int *pin_iters = nullptr;       // iteration counts, 2 per system
+  int pin_iters_cap = 0;
   size_t cap_bin_occ = 0;
   // counting sort by bin (radix passes staged in LDS): key/value ping-pong
   // buffers and the per-(digit, block) counts and starts
@@ -435,6 +441,10 @@ int swr_bo_run(swr_system *S);
 int swr_qeq_build_H(swr_system *S);
 int swr_qeq_spmv(swr_system *S, const double *xv, double *b);
 int swr_qeq_cg_run(swr_system *S, int first_batch);
+#define SWR_PIN_MIX 4096
+// Grows the pinned iteration-count buffer to nsys systems (synchronises the
+// stream before replacing it).
+int swr_pin_iters_reserve(swr_system *S, int nsys);
 int swr_qeq_calc_q(swr_system *S);
 int swr_qeq_layout(swr_system *S);
 int swr_qeq_mixed_solve(swr_system *S);
